@@ -1,0 +1,326 @@
+"""ctypes front end of the FP64 CPU oracle (``oracle/sphref.c``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py``.  The product path never calls
+it.
+
+Besides the per-operator restatements, :class:`RefSimulation` restates the
+reference time loop for the operation lists this path covers:
+
+* ``Solver.solve`` (solver/solver.py:523-590) with a fixed ``TimeStep``
+  (solver/time_step_functions.py:3-6);
+* ``Integrator.eval/step`` (solver/integrator.py:287-367), Euler (:420-445),
+  RK2 midpoint (:450-498) and PredictorCorrector (:614-683);
+* ``SPHCalc.sph`` (sph/sph_calc.pyx:307-358): non-integrating outputs reset
+  to zero first, ``to_reset`` props (``dt_fac``) zeroed per calc, one
+  function per source array in array order, outputs accumulated ``+=``;
+* neighbour lists rebuilt lazily after every position change
+  (base/particles.py:184-227, base/nnps.pyx:561-621).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(HERE, "libsphref.so")
+_lib = None
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I64 = ctypes.POINTER(ctypes.c_int64)
+_I32 = ctypes.POINTER(ctypes.c_int32)
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        from . import build as _b
+        _b.build()
+    lib = ctypes.CDLL(_LIB_PATH)
+    i64, i32, dbl, vp = ctypes.c_int64, ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+    lib.sphref_real_to_int.argtypes = [dbl, dbl]
+    lib.sphref_real_to_int.restype = ctypes.c_int
+    lib.sphref_compute_cell_size.argtypes = [dbl, dbl, dbl, dbl]
+    lib.sphref_compute_cell_size.restype = dbl
+    lib.sphref_nnps.argtypes = [vp, vp, vp, vp, i64, vp, vp, vp, i64, dbl, dbl, i32, vp, vp, i32]
+    lib.sphref_nnps.restype = i64
+    lib.sphref_nnps_brute.argtypes = [vp, vp, vp, vp, i64, vp, vp, vp, i64, dbl, vp, vp, i32]
+    lib.sphref_nnps_brute.restype = i64
+    lib.sphref_cubic_spline.argtypes = [i32, i64] + [vp] * 11
+    lib.sphref_cubic_spline.restype = None
+    lib.sphref_summation_density.argtypes = [i32, i64] + [vp] * 12 + [i32]
+    lib.sphref_summation_density.restype = None
+    lib.sphref_tait.argtypes = [i64, vp, dbl, dbl, dbl, vp, vp]
+    lib.sphref_tait.restype = None
+    lib.sphref_momentum.argtypes = [i32, i64] + [vp] * 23 + [dbl, dbl, dbl] + [vp] * 4 + [i32]
+    lib.sphref_momentum.restype = None
+    lib.sphref_gravity.argtypes = [i64, dbl, dbl, dbl, vp, vp, vp]
+    lib.sphref_gravity.restype = None
+    lib.sphref_step.argtypes = [i64, vp, vp, dbl, vp]
+    lib.sphref_step.restype = None
+    lib.sphref_pc_final.argtypes = [i64, vp, vp]
+    lib.sphref_pc_final.restype = None
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "oracle arrays must be contiguous"
+    return a.ctypes.data
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def real_to_int(v, step):
+    """base/cell.pyx:44-61"""
+    return _load().sphref_real_to_int(float(v), float(step))
+
+
+def compute_cell_size(max_h, radius_scale, min_size=-1.0, max_size=0.0):
+    """base/cell.pyx:911-941"""
+    return _load().sphref_compute_cell_size(float(max_h), float(radius_scale),
+                                            float(min_size), float(max_size))
+
+
+def nnps(dst, src, radius_scale, cell_size, same_array, nthreads=1, nd_real=None):
+    """FixedDestNbrParticleLocator lists (base/nnps.pyx:458-621).
+
+    ``dst``/``src`` are dicts with x, y, z (and h for dst).  Returns
+    ``(offsets int64[nd+1], indices int32[nnz])`` in reference order."""
+    lib = _load()
+    dx, dy, dz, dh = (_f64(dst[k]) for k in ("x", "y", "z", "h"))
+    sx, sy, sz = (_f64(src[k]) for k in ("x", "y", "z"))
+    nd = len(dx) if nd_real is None else int(nd_real)
+    offsets = np.zeros(nd + 1, dtype=np.int64)
+    nnz = lib.sphref_nnps(_p(dx), _p(dy), _p(dz), _p(dh), nd, _p(sx), _p(sy), _p(sz), len(sx),
+                          float(radius_scale), float(cell_size), int(bool(same_array)),
+                          _p(offsets), None, nthreads)
+    indices = np.zeros(max(nnz, 1), dtype=np.int32)
+    lib.sphref_nnps(_p(dx), _p(dy), _p(dz), _p(dh), nd, _p(sx), _p(sy), _p(sz), len(sx),
+                    float(radius_scale), float(cell_size), int(bool(same_array)),
+                    _p(offsets), _p(indices), nthreads)
+    return offsets, indices[:nnz]
+
+
+def nnps_brute(dst, src, radius_scale, nthreads=1, nd_real=None):
+    """Brute-force O(N*M) sets, ascending (base/nnps.pyx:78-94 pattern)."""
+    lib = _load()
+    dx, dy, dz, dh = (_f64(dst[k]) for k in ("x", "y", "z", "h"))
+    sx, sy, sz = (_f64(src[k]) for k in ("x", "y", "z"))
+    nd = len(dx) if nd_real is None else int(nd_real)
+    offsets = np.zeros(nd + 1, dtype=np.int64)
+    nnz = lib.sphref_nnps_brute(_p(dx), _p(dy), _p(dz), _p(dh), nd, _p(sx), _p(sy), _p(sz),
+                                len(sx), float(radius_scale), _p(offsets), None, nthreads)
+    indices = np.zeros(max(nnz, 1), dtype=np.int32)
+    lib.sphref_nnps_brute(_p(dx), _p(dy), _p(dz), _p(dh), nd, _p(sx), _p(sy), _p(sz),
+                          len(sx), float(radius_scale), _p(offsets), _p(indices), nthreads)
+    return offsets, indices[:nnz]
+
+
+def cubic_spline(dim, a, b, h):
+    """W(a, b, h) and gradW for arrays of point pairs (base/kernels.pyx:355-417)."""
+    lib = _load()
+    a = [_f64(c) for c in a]
+    b = [_f64(c) for c in b]
+    h = _f64(h)
+    n = len(h)
+    w = np.empty(n)
+    g = [np.empty(n) for _ in range(3)]
+    lib.sphref_cubic_spline(int(dim), n, *[_p(c) for c in a], *[_p(c) for c in b], _p(h),
+                            _p(w), *[_p(c) for c in g])
+    return w, np.stack(g)
+
+
+def summation_density(dim, dst, src, offsets, indices, rho_out, nthreads=1):
+    """rho_out[i] += sum_j m_j W (sph/funcs/density_funcs.pyx:37-73)."""
+    lib = _load()
+    nd = len(offsets) - 1
+    d = [_f64(dst[k]) for k in ("x", "y", "z", "h")]
+    s = [_f64(src[k]) for k in ("x", "y", "z", "h", "m")]
+    lib.sphref_summation_density(int(dim), nd, *[_p(c) for c in d], *[_p(c) for c in s],
+                                 _p(offsets), _p(indices), _p(rho_out), nthreads)
+
+
+def tait(rho, ro, co, gamma, p_out, cs_out, n_real=None):
+    """sph/funcs/eos_funcs.pyx:118-129 (accumulating into p_out, cs_out)."""
+    n = len(rho) if n_real is None else int(n_real)
+    _load().sphref_tait(n, _p(_f64(rho)), float(ro), float(co), float(gamma),
+                        _p(p_out), _p(cs_out))
+
+
+MOM_PROPS = ("x", "y", "z", "h", "rho", "p", "u", "v", "w", "cs")
+MOM_SRC_PROPS = ("x", "y", "z", "h", "m", "rho", "p", "u", "v", "w", "cs")
+
+
+def momentum(dim, dst, src, offsets, indices, alpha, beta, eta, ax, ay, az, dt_fac, nthreads=1):
+    """sph/funcs/pressure_funcs.pyx:246-391: a += ..., dt_fac = max(...)."""
+    lib = _load()
+    nd = len(offsets) - 1
+    d = [_f64(dst[k]) for k in MOM_PROPS]
+    s = [_f64(src[k]) for k in MOM_SRC_PROPS]
+    lib.sphref_momentum(int(dim), nd, *[_p(c) for c in d], *[_p(c) for c in s],
+                        _p(offsets), _p(indices), float(alpha), float(beta), float(eta),
+                        _p(ax), _p(ay), _p(az), _p(dt_fac), nthreads)
+
+
+def gravity(n, g, ax, ay, az):
+    _load().sphref_gravity(int(n), float(g[0]), float(g[1]), float(g[2]), _p(ax), _p(ay), _p(az))
+
+
+def step(x0, a, dt, x):
+    """solver/integrator.py:356-360"""
+    _load().sphref_step(len(x), _p(x0), _p(a), float(dt), _p(x))
+
+
+def pc_final(x0, x):
+    """solver/integrator.py:632-643"""
+    _load().sphref_pc_final(len(x), _p(x0), _p(x))
+
+
+# ---------------------------------------------------------------------------
+# whole-run restatement
+# ---------------------------------------------------------------------------
+
+class RefArray:
+    """Minimal SoA particle store for the oracle (dict of float64 arrays)."""
+
+    def __init__(self, name, ptype, props):
+        self.name = name
+        self.ptype = ptype
+        self.props = {k: np.array(v, dtype=np.float64) for k, v in props.items()}
+        n = len(self.props["x"])
+        for k, dv in (("x", 0.0), ("y", 0.0), ("z", 0.0), ("u", 0.0), ("v", 0.0), ("w", 0.0),
+                      ("m", 1.0), ("h", 1.0), ("p", 0.0), ("e", 0.0), ("rho", 1.0),
+                      ("cs", 0.0), ("dt_fac", 0.0)):
+            if k not in self.props:
+                self.props[k] = np.full(n, dv)
+        self.n = n
+
+    def __getitem__(self, k):
+        return self.props[k]
+
+
+class RefSimulation:
+    """The WCSPH operation list of this path, driven like Solver.solve.
+
+    Operations, in solver order (sph_equation.py:101-228, solver.py:222-282):
+      1. SPHRho            on fluids, from ``density_from`` types, updates rho
+      2. TaitEquation      on fluids, updates p, cs
+      3. MomentumEquation  on fluids, from ``momentum_from`` types, integrates u,v(,w)
+      4. GravityForce      on fluids (if ``gravity`` given), integrates u,v(,w)
+      5. PositionStepping  on fluids, integrates x,y(,z)
+    """
+
+    def __init__(self, arrays, dim, integrator="rk2", co=1.0, ro=1000.0, gamma=7.0,
+                 alpha=1.0, beta=1.0, eta=0.1, gravity=None, density_from=(0,),
+                 momentum_from=(0,), cell_radius_scale=2.0, nthreads=1):
+        self.arrays = arrays
+        self.dim = dim
+        self.integrator = integrator
+        self.co, self.ro, self.gamma = co, ro, gamma
+        self.alpha, self.beta, self.eta = alpha, beta, eta
+        self.gravity = gravity
+        self.density_from = density_from
+        self.momentum_from = momentum_from
+        self.cell_radius_scale = cell_radius_scale
+        self.nthreads = nthreads
+        self.fluids = [a for a in arrays if a.ptype == 0]
+        self.vel = ["u", "v", "w"][:dim]
+        self.pos = ["x", "y", "z"][:dim]
+        self.stepped = self.vel + self.pos  # momentum/gravity updates, then position step
+        self.lists = None
+        self.dirty = True
+        self.pairs = 0
+
+    # -- NNPS -----------------------------------------------------------
+    def cell_size(self):
+        max_h = max(float(np.max(a["h"])) for a in self.arrays if a.n > 0)
+        return compute_cell_size(max_h, self.cell_radius_scale)
+
+    def rebuild(self):
+        cs = self.cell_size()
+        self.lists = {}
+        for d in self.fluids:
+            for s in self.arrays:
+                if s.ptype in set(self.density_from) | set(self.momentum_from):
+                    self.lists[(d.name, s.name)] = nnps(d, s, 2.0, cs, d is s, self.nthreads)
+        self.dirty = False
+
+    def _lists(self):
+        if self.dirty or self.lists is None:
+            self.rebuild()
+        return self.lists
+
+    # -- one RHS evaluation (Integrator.eval) ------------------------------
+    def eval(self, acc):
+        dim = self.dim
+        lists = self._lists()
+        for d in self.fluids:
+            d["rho"][:] = 0.0
+            for s in self.arrays:
+                if s.ptype in self.density_from:
+                    off, idx = lists[(d.name, s.name)]
+                    summation_density(dim, d, s, off, idx, d["rho"], self.nthreads)
+                    self.pairs += len(idx)
+        for d in self.fluids:
+            d["p"][:] = 0.0
+            d["cs"][:] = 0.0
+            tait(d["rho"], self.ro, self.co, self.gamma, d["p"], d["cs"])
+        for d in self.fluids:
+            a = acc[d.name]
+            for k in self.stepped:
+                a[k][:] = 0.0
+            d["dt_fac"][:] = 0.0
+            az = a["w"] if dim == 3 else np.zeros(d.n)
+            for s in self.arrays:
+                if s.ptype in self.momentum_from:
+                    off, idx = lists[(d.name, s.name)]
+                    momentum(dim, d, s, off, idx, self.alpha, self.beta, self.eta,
+                             a["u"], a["v"], az, d["dt_fac"], self.nthreads)
+                    self.pairs += len(idx)
+            if self.gravity is not None:
+                gravity(d.n, self.gravity, a["u"], a["v"], az)
+            for vk, xk in zip(self.vel, self.pos):
+                a[xk][:] = d[vk]
+
+    def _step(self, acc, init, dt):
+        for d in self.fluids:
+            for k in self.stepped:
+                step(init[d.name][k], acc[d.name][k], dt, d[k])
+        self.dirty = True
+
+    def integrate(self, dt):
+        acc = {d.name: {k: np.zeros(d.n) for k in self.stepped} for d in self.fluids}
+        init = {d.name: {k: d[k].copy() for k in self.stepped} for d in self.fluids}
+        if self.integrator == "euler":
+            self.eval(acc)
+            self._step(acc, init, dt)
+        elif self.integrator == "rk2":
+            self.eval(acc)
+            self._step(acc, init, dt / 2)
+            self.eval(acc)
+            self._step(acc, init, dt)
+        elif self.integrator == "pc":
+            self.eval(acc)
+            self._step(acc, init, dt / 2)
+            self.eval(acc)
+            self._step(acc, init, dt / 2)
+            for d in self.fluids:
+                for k in self.stepped:
+                    pc_final(init[d.name][k], d[k])
+            self.dirty = True
+        else:
+            raise ValueError(self.integrator)
+        return acc
+
+    def run(self, dt, nsteps):
+        acc = None
+        for _ in range(nsteps):
+            acc = self.integrate(dt)
+        return acc

@@ -1,0 +1,444 @@
+/*
+ * sphref.c -- FP64 CPU oracle for the PySPH particle hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the correctness checker and the
+ * CPU baseline: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it.  The product path (pysph_amd) never calls it.
+ *
+ * It restates, in plain C with the same expression order, the Cython
+ * arithmetic of the reference (sabago/pysph, PySPH 0.9beta).  Each function
+ * cites the reference file:line it follows.  Build flags: -O2
+ * -ffp-contract=off (no FMA contraction, like the reference's baseline
+ * x86-64/SSE2 build), optional OpenMP for the multi-core baseline.
+ *
+ * Semantics fixed here (reference citations):
+ *   cell id        (int)floor(x / cell_size)          base/cell.pyx:44-84
+ *   stencil        (2d+1)^3, d=(int)ceil(r/cell), x outer, y, z inner
+ *                                                      base/cell.pyx:95-118, :996-1033
+ *   cell contents  ascending particle index           base/cell.pyx:1229-1306
+ *   predicate      distance2(src,dst) < (h_dst*rs)^2  base/nnps.pyx:271-356, :525-559
+ *                  distance2 order                    base/point.pxd:35-37
+ *   self slot      own index swapped to the last slot base/nnps.pyx:596-621
+ *   cubic spline   W, gradW                           base/kernels.pyx:51-57, :349-453
+ *   SPHRho         sum m_b W(r_ab, (h_a+h_b)/2)       sph/funcs/density_funcs.pyx:37-73
+ *   Tait           p, cs                              sph/funcs/eos_funcs.pyx:80-129
+ *   Momentum + AV  pressure + Monaghan Pi_ab, dt_fac  sph/funcs/pressure_funcs.pyx:185-391
+ *   Gravity        += (gx,gy,gz)                      sph/funcs/external_force.pyx:38-44
+ *   PositionStep   a_x = u                            sph/funcs/position_funcs.pyx:29-43
+ *   stage          x = x0 + a*dt ; PC final 2x - x0   solver/integrator.py:340-367, :632-683
+ *   eval loop      out += per-dest result (LocalReal) sph/sph_func.pyx:208-227, :427-441, :513-531
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define REF_PI 3.141592653589793 /* numpy.pi, base/kernels.pyx:23 */
+
+/* ------------------------------------------------------------------ */
+/* cells                                                               */
+/* ------------------------------------------------------------------ */
+
+/* base/cell.pyx:44-61 */
+int sphref_real_to_int(double real_val, double step)
+{
+    return (int)floor(real_val / step);
+}
+
+/* base/cell.pyx:911-941 (radius_scale * max_h, clamped; 0 -> 1.0) */
+double sphref_compute_cell_size(double max_h, double radius_scale,
+                                double min_size, double max_size)
+{
+    double cell_size = radius_scale * max_h;
+    if (cell_size < min_size)
+        cell_size = min_size;
+    else if (cell_size > max_size && max_size > 0)
+        cell_size = max_size;
+    if (cell_size == 0.0)
+        cell_size = 1.0;
+    return cell_size;
+}
+
+typedef struct {
+    int cx, cy, cz;
+    int32_t idx;
+} ref_cell_entry;
+
+static int cmp_entry(const void *a, const void *b)
+{
+    const ref_cell_entry *p = (const ref_cell_entry *)a;
+    const ref_cell_entry *q = (const ref_cell_entry *)b;
+    if (p->cx != q->cx) return p->cx < q->cx ? -1 : 1;
+    if (p->cy != q->cy) return p->cy < q->cy ? -1 : 1;
+    if (p->cz != q->cz) return p->cz < q->cz ? -1 : 1;
+    if (p->idx != q->idx) return p->idx < q->idx ? -1 : 1;
+    return 0;
+}
+
+static int cmp_cell(const ref_cell_entry *p, int cx, int cy, int cz)
+{
+    if (p->cx != cx) return p->cx < cx ? -1 : 1;
+    if (p->cy != cy) return p->cy < cy ? -1 : 1;
+    if (p->cz != cz) return p->cz < cz ? -1 : 1;
+    return 0;
+}
+
+/* first entry with cell >= (cx,cy,cz) */
+static int64_t lower_bound(const ref_cell_entry *e, int64_t n, int cx, int cy, int cz)
+{
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = lo + (hi - lo) / 2;
+        if (cmp_cell(&e[mid], cx, cy, cz) < 0) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+/* Bin the sources: the dict of cells with ascending indices per cell
+ * (base/cell.pyx:1229-1306) restated as entries sorted by (cell, index). */
+static ref_cell_entry *bin_sources(const double *sx, const double *sy, const double *sz,
+                                   int64_t ns, double cell_size)
+{
+    ref_cell_entry *e = (ref_cell_entry *)malloc(sizeof(ref_cell_entry) * (size_t)(ns > 0 ? ns : 1));
+    for (int64_t j = 0; j < ns; ++j) {
+        e[j].cx = sphref_real_to_int(sx[j], cell_size);
+        e[j].cy = sphref_real_to_int(sy[j], cell_size);
+        e[j].cz = sphref_real_to_int(sz[j], cell_size);
+        e[j].idx = (int32_t)j;
+    }
+    qsort(e, (size_t)ns, sizeof(ref_cell_entry), cmp_entry);
+    return e;
+}
+
+/* One destination query in reference order: stencil cells x->y->z, each
+ * cell's indices ascending, predicate distance2 < radius^2 (strict).
+ * base/nnps.pyx:271-356 with radius from :552 and stencil cell.pyx:996-1033.
+ * If out != NULL the indices are written; the count is returned. */
+static int64_t query_one(const ref_cell_entry *e, int64_t ns,
+                         const double *sx, const double *sy, const double *sz,
+                         double px, double py, double pz, double radius,
+                         double cell_size, int32_t *out)
+{
+    double radius2 = radius * radius;
+    int distance = (int)ceil(radius / cell_size);
+    int cx = sphref_real_to_int(px, cell_size);
+    int cy = sphref_real_to_int(py, cell_size);
+    int cz = sphref_real_to_int(pz, cell_size);
+    int64_t cnt = 0;
+    for (int i = -distance; i <= distance; ++i)
+        for (int j = -distance; j <= distance; ++j)
+            for (int k = -distance; k <= distance; ++k) {
+                int64_t p = lower_bound(e, ns, cx + i, cy + j, cz + k);
+                for (; p < ns && cmp_cell(&e[p], cx + i, cy + j, cz + k) == 0; ++p) {
+                    int32_t idx = e[p].idx;
+                    /* cPoint_distance2(src, dst), base/point.pxd:35-37 */
+                    double d2 = ((sx[idx] - px) * (sx[idx] - px) + (sy[idx] - py) * (sy[idx] - py) +
+                                 (sz[idx] - pz) * (sz[idx] - pz));
+                    if (d2 < radius2) {
+                        if (out) out[cnt] = idx;
+                        ++cnt;
+                    }
+                }
+            }
+    return cnt;
+}
+
+/* Per-destination neighbour lists of FixedDestNbrParticleLocator
+ * (base/nnps.pyx:458-621): rows for dests [0, nd_real), indices in the
+ * reference order, own index moved to the last slot when same_array.
+ * Pass indices == NULL to get the row offsets only (offsets[nd_real+1]);
+ * returns nnz.  */
+int64_t sphref_nnps(const double *dx, const double *dy, const double *dz, const double *dh,
+                    int64_t nd_real,
+                    const double *sx, const double *sy, const double *sz, int64_t ns,
+                    double radius_scale, double cell_size, int same_array,
+                    int64_t *offsets, int32_t *indices, int nthreads)
+{
+    ref_cell_entry *e = bin_sources(sx, sy, sz, ns, cell_size);
+    (void)nthreads;
+    if (!indices) {
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads > 0 ? nthreads : 1)
+        for (int64_t i = 0; i < nd_real; ++i) {
+            double radius = dh[i] * radius_scale;
+            offsets[i + 1] = query_one(e, ns, sx, sy, sz, dx[i], dy[i], dz[i], radius, cell_size, NULL);
+        }
+        offsets[0] = 0;
+        for (int64_t i = 0; i < nd_real; ++i) offsets[i + 1] += offsets[i];
+    } else {
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads > 0 ? nthreads : 1)
+        for (int64_t i = 0; i < nd_real; ++i) {
+            double radius = dh[i] * radius_scale;
+            int32_t *row = indices + offsets[i];
+            int64_t len = query_one(e, ns, sx, sy, sz, dx[i], dy[i], dz[i], radius, cell_size, row);
+            /* keep the particle itself at the last index, nnps.pyx:612-619 */
+            if (same_array) {
+                for (int64_t j = 0; j < len - 1; ++j) {
+                    if (row[j] == (int32_t)i) {
+                        row[j] = row[len - 1];
+                        row[len - 1] = (int32_t)i;
+                        break;
+                    }
+                }
+            }
+        }
+    }
+    free(e);
+    return offsets[nd_real];
+}
+
+/* Brute-force O(N*M) neighbour sets in ascending index order, the same
+ * predicate.  The reference checks its locators against brute force
+ * (base/tests/test_locator.py:187-238, base/nnps.pyx:78-94). */
+int64_t sphref_nnps_brute(const double *dx, const double *dy, const double *dz, const double *dh,
+                          int64_t nd_real,
+                          const double *sx, const double *sy, const double *sz, int64_t ns,
+                          double radius_scale, int64_t *offsets, int32_t *indices, int nthreads)
+{
+    (void)nthreads;
+    if (!indices) {
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : 1)
+        for (int64_t i = 0; i < nd_real; ++i) {
+            double radius = dh[i] * radius_scale, r2 = radius * radius;
+            int64_t c = 0;
+            for (int64_t j = 0; j < ns; ++j) {
+                double d2 = ((sx[j] - dx[i]) * (sx[j] - dx[i]) + (sy[j] - dy[i]) * (sy[j] - dy[i]) +
+                             (sz[j] - dz[i]) * (sz[j] - dz[i]));
+                if (d2 < r2) ++c;
+            }
+            offsets[i + 1] = c;
+        }
+        offsets[0] = 0;
+        for (int64_t i = 0; i < nd_real; ++i) offsets[i + 1] += offsets[i];
+    } else {
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nthreads > 0 ? nthreads : 1)
+        for (int64_t i = 0; i < nd_real; ++i) {
+            double radius = dh[i] * radius_scale, r2 = radius * radius;
+            int64_t c = offsets[i];
+            for (int64_t j = 0; j < ns; ++j) {
+                double d2 = ((sx[j] - dx[i]) * (sx[j] - dx[i]) + (sy[j] - dy[i]) * (sy[j] - dy[i]) +
+                             (sz[j] - dz[i]) * (sz[j] - dz[i]));
+                if (d2 < r2) indices[c++] = (int32_t)j;
+            }
+        }
+    }
+    return offsets[nd_real];
+}
+
+/* ------------------------------------------------------------------ */
+/* cubic spline, base/kernels.pyx:51-57, :349-453                      */
+/* ------------------------------------------------------------------ */
+
+static double h_dim(double h, int dim)
+{
+    if (dim == 1) return 1 / h;
+    else if (dim == 2) return 1 / (h * h);
+    else return 1 / (h * h * h);
+}
+
+/* self.fac = _fac(1.0), base/kernels.pyx:75, :441-447 */
+static double cubic_fac1(int dim)
+{
+    if (dim == 1) return (2. / 3.) / 1.0;
+    else if (dim == 2) return 10 / (7 * REF_PI) / (1.0 * 1.0);
+    else return 1. / REF_PI / (1.0 * 1.0 * 1.0);
+}
+
+/* CubicSplineKernel.function(pa, pb, h), base/kernels.pyx:355-378 */
+static double cubic_w(int dim, double pax, double pay, double paz,
+                      double pbx, double pby, double pbz, double h)
+{
+    double rab = sqrt((pax - pbx) * (pax - pbx) + (pay - pby) * (pay - pby) +
+                      (paz - pbz) * (paz - pbz));
+    double q = rab / h;
+    double val;
+    double fac = h_dim(h, dim) * cubic_fac1(dim);
+    if (q > 2.0)
+        val = 0.0;
+    else if (q > 1.0)
+        val = 0.25 * (2 - q) * (2 - q) * (2 - q);
+    else
+        val = 1 - 1.5 * (q * q) * (1 - 0.5 * q);
+    return val * fac;
+}
+
+/* CubicSplineKernel.gradient(pa, pb, h), base/kernels.pyx:380-417 */
+static void cubic_gradw(int dim, double pax, double pay, double paz,
+                        double pbx, double pby, double pbz, double h, double *g)
+{
+    double rab = sqrt((pax - pbx) * (pax - pbx) + (pay - pby) * (pay - pby) +
+                      (paz - pbz) * (paz - pbz));
+    double q = rab / h;
+    double val = 0.0;
+    double fac = h_dim(h, dim) * cubic_fac1(dim);
+    double rx = pax - pbx, ry = pay - pby, rz = paz - pbz;
+    if (q > 2.0) {
+    } else if (q >= 1.0) {
+        val = -0.75 * (2 - q) * (2 - q) / (h * rab);
+    } else if (q > 1e-14) {
+        val = 3.0 * (0.75 * q - 1) / (h * h);
+    }
+    g[0] = rx * (val * fac);
+    g[1] = ry * (val * fac);
+    g[2] = rz * (val * fac);
+}
+
+/* vectorised kernel evaluation for tests: W and gradW for n pairs */
+void sphref_cubic_spline(int dim, int64_t n,
+                         const double *ax, const double *ay, const double *az,
+                         const double *bx, const double *by, const double *bz,
+                         const double *h, double *w, double *gx, double *gy, double *gz)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        double g[3];
+        if (w) w[i] = cubic_w(dim, ax[i], ay[i], az[i], bx[i], by[i], bz[i], h[i]);
+        cubic_gradw(dim, ax[i], ay[i], az[i], bx[i], by[i], bz[i], h[i], g);
+        if (gx) gx[i] = g[0];
+        if (gy) gy[i] = g[1];
+        if (gz) gz[i] = g[2];
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* operators                                                            */
+/* ------------------------------------------------------------------ */
+
+/* SPHRho: CSPHFunctionParticle.eval_single + SPHRho.eval_nbr_csph
+ * (sph/sph_func.pyx:513-531, sph/funcs/density_funcs.pyx:37-73) under
+ * SPHFunction.eval (sph/sph_func.pyx:208-227): out[i] += result. */
+void sphref_summation_density(int dim, int64_t nd_real,
+                              const double *dx, const double *dy, const double *dz, const double *dh,
+                              const double *sx, const double *sy, const double *sz,
+                              const double *sh, const double *sm,
+                              const int64_t *offsets, const int32_t *indices,
+                              double *rho_out, int nthreads)
+{
+    (void)nthreads;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int64_t i = 0; i < nd_real; ++i) {
+        double nr = 0.0, dnr = 0.0;
+        for (int64_t k = offsets[i]; k < offsets[i + 1]; ++k) {
+            int32_t j = indices[k];
+            double ha = dh[i];
+            double hb = sh[j];
+            double hab = 0.5 * (ha + hb);
+            double w = cubic_w(dim, dx[i], dy[i], dz[i], sx[j], sy[j], sz[j], hab);
+            nr += w * sm[j];
+        }
+        if (dnr != 0.0) nr /= dnr;
+        rho_out[i] += nr;
+    }
+}
+
+/* TaitEquation.eval_single, sph/funcs/eos_funcs.pyx:118-129 with
+ * B = co*co*ro/gamma (:90); out += result (sph_func.pyx:222-225). */
+void sphref_tait(int64_t n_real, const double *rho, double ro, double co, double gamma,
+                 double *p_out, double *cs_out)
+{
+    double B = co * co * ro / gamma;
+    for (int64_t i = 0; i < n_real; ++i) {
+        double rhoa = rho[i];
+        double ratio = rhoa / ro;
+        double gamma2 = 0.5 * (gamma - 1.0);
+        double tmp = pow(ratio, gamma);
+        p_out[i] += (tmp - 1.0) * B;
+        cs_out[i] += pow(ratio, gamma2) * co;
+    }
+}
+
+/* MomentumEquation.eval_nbr, sph/funcs/pressure_funcs.pyx:246-391
+ * (hks=False, deltap=None) summed by SPHFunctionParticle.eval_single
+ * (sph/sph_func.pyx:427-441); out += result; dt_fac max-reduced. */
+void sphref_momentum(int dim, int64_t nd_real,
+                     const double *dx, const double *dy, const double *dz, const double *dh,
+                     const double *drho, const double *dp, const double *du, const double *dv,
+                     const double *dw, const double *dcs,
+                     const double *sx, const double *sy, const double *sz, const double *sh,
+                     const double *sm, const double *srho, const double *sp, const double *su,
+                     const double *sv, const double *sw, const double *scs,
+                     const int64_t *offsets, const int32_t *indices,
+                     double alpha, double beta, double eta,
+                     double *ax, double *ay, double *az, double *dt_fac, int nthreads)
+{
+    (void)nthreads;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int64_t i = 0; i < nd_real; ++i) {
+        double nr0 = 0.0, nr1 = 0.0, nr2 = 0.0;
+        double dtf = dt_fac[i];
+        for (int64_t k = offsets[i]; k < offsets[i + 1]; ++k) {
+            int32_t j = indices[k];
+            double ha = dh[i];
+            double hb = sh[j];
+            double hab = 0.5 * (ha + hb);
+            double ca = dcs[i];
+            double cb = scs[j];
+            double rabx = dx[i] - sx[j];
+            double raby = dy[i] - sy[j];
+            double rabz = dz[i] - sz[j];
+            double vabx = du[i] - su[j];
+            double vaby = dv[i] - sv[j];
+            double vabz = dw[i] - sw[j];
+            double dot = vabx * rabx + vaby * raby + vabz * rabz; /* cPoint_dot */
+            double rab2 = rabx * rabx + raby * raby + rabz * rabz; /* cPoint_norm */
+            double f = fabs(hab * dot / (rab2 + eta * eta * hab * hab));
+            if (f > dtf) dtf = f; /* max(d_dt_fac, dt_fac) */
+            double Pa = dp[i], rhoa = drho[i];
+            double Pb = sp[j], rhob = srho[j], mb = sm[j];
+            double rhoa2 = 1.0 / (rhoa * rhoa);
+            double rhob2 = 1.0 / (rhob * rhob);
+            double tmp = Pa * rhoa2 + Pb * rhob2;
+            double piab = 0;
+            if (dot < 0) {
+                double cab = 0.5 * (ca + cb);
+                double rhoab = 0.5 * (rhoa + rhob);
+                double mu = hab * dot;
+                mu /= (rab2 + eta * eta * hab * hab);
+                piab = -alpha * cab * mu + beta * mu * mu;
+                piab /= rhoab;
+            }
+            tmp += piab;
+            tmp *= -mb;
+            double g[3];
+            cubic_gradw(dim, dx[i], dy[i], dz[i], sx[j], sy[j], sz[j], hab, g);
+            nr0 += tmp * g[0];
+            nr1 += tmp * g[1];
+            nr2 += tmp * g[2];
+        }
+        dt_fac[i] = dtf;
+        ax[i] += nr0;
+        ay[i] += nr1;
+        if (az) az[i] += nr2;
+    }
+}
+
+/* GravityForce.eval_single, sph/funcs/external_force.pyx:38-44 */
+void sphref_gravity(int64_t n_real, double gx, double gy, double gz,
+                    double *ax, double *ay, double *az)
+{
+    for (int64_t i = 0; i < n_real; ++i) {
+        ax[i] += gx;
+        ay[i] += gy;
+        if (az) az[i] += gz;
+    }
+}
+
+/* Integrator.step: updated = initial + step * dt, solver/integrator.py:356-360
+ * (numpy: the product is rounded, then the sum). */
+void sphref_step(int64_t n, const double *x0, const double *a, double dt, double *x)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        double t = a[i] * dt;
+        x[i] = x0[i] + t;
+    }
+}
+
+/* PredictorCorrectorIntegrator.final_step, solver/integrator.py:632-643 */
+void sphref_pc_final(int64_t n, const double *x0, double *x)
+{
+    for (int64_t i = 0; i < n; ++i) {
+        double t = 2 * x[i];
+        x[i] = t - x0[i];
+    }
+}

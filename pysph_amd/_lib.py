@@ -1,0 +1,184 @@
+"""ctypes binding of libsphhip.so (the C ABI declared in include/sphhip.h).
+
+The library is the only compute path of pysph_amd: there is no CPU
+fallback.  Loading fails loudly when the shared object is missing, and the
+context fails loudly when no HIP device is visible.
+"""
+import ctypes
+import os
+import threading
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsphhip.so")
+
+SPH_OK = 0
+SPH_EINVAL = -1
+SPH_ENEEDCAP = -2
+SPH_EHIP = -3
+SPH_EDOMAIN = -4
+SPH_ENOMEM = -5
+
+ORDER_MORTON = 0
+ORDER_LINEAR = 1
+
+MAX_STEP_PROPS = 16
+
+_vp = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_i32 = ctypes.c_int
+_dbl = ctypes.c_double
+
+
+class SphGrid(ctypes.Structure):
+    _fields_ = [("cell_size", _dbl), ("max_h", _dbl), ("min_h", _dbl),
+                ("cmin", ctypes.c_int32 * 3), ("dims", ctypes.c_int32 * 3),
+                ("bits", ctypes.c_int32 * 3), ("order", ctypes.c_int32),
+                ("ncells", _i64)]
+
+
+class MomDst(ctypes.Structure):
+    _fields_ = [(k, _vp) for k in ("x", "y", "z", "h", "rho", "p", "u", "v", "w", "cs")]
+
+
+class MomSrc(ctypes.Structure):
+    _fields_ = [(k, _vp) for k in ("x", "y", "z", "h", "m", "rho", "p", "u", "v", "w", "cs")]
+
+
+class StepProps(ctypes.Structure):
+    _fields_ = [("nprops", ctypes.c_int32),
+                ("x", _vp * MAX_STEP_PROPS),
+                ("x0", _vp * MAX_STEP_PROPS),
+                ("a", _vp * MAX_STEP_PROPS)]
+
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "sph_version": (ctypes.c_char_p, []),
+    "sph_ctx_create": (_i32, [ctypes.POINTER(_vp), _i32, _vp]),
+    "sph_ctx_destroy": (_i32, [_vp]),
+    "sph_ctx_set_stream": (_i32, [_vp, _vp]),
+    "sph_ctx_synchronize": (_i32, [_vp]),
+    "sph_last_error": (ctypes.c_char_p, [_vp]),
+    "sph_grid_build": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _dbl, _i32, _i64,
+                              ctypes.POINTER(SphGrid)]),
+    "sph_bin_array": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _i64, _vp, _vp, _vp,
+                             _vp, _vp, _vp]),
+    "sph_nnps_build": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _vp, _i64, _vp, _vp,
+                              _vp, _vp, _vp, _i64, _dbl, _i32, _vp, _vp, _i64,
+                              ctypes.POINTER(_i64)]),
+    "sph_summation_density": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp] + [_vp] * 9 + [_i32, _vp]),
+    "sph_tait": (_i32, [_vp, _i64, _i64, _vp, _dbl, _dbl, _dbl, _i32, _vp, _vp]),
+    "sph_momentum_av": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, ctypes.POINTER(MomDst),
+                               ctypes.POINTER(MomSrc), _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp,
+                               _vp]),
+    "sph_gravity": (_i32, [_vp, _i64, _i64, _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp]),
+    "sph_position_stepping": (_i32, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sph_stage_step": (_i32, [_vp, _i64, ctypes.POINTER(StepProps), _dbl]),
+    "sph_pc_final": (_i32, [_vp, _i64, ctypes.POINTER(StepProps)]),
+    "sph_copy": (_i32, [_vp, _i64, _vp, _vp]),
+    "sph_fill": (_i32, [_vp, _i64, _dbl, _vp]),
+}
+
+EXPORTED = tuple(_SIGNATURES)
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+class SphError(RuntimeError):
+    """A libsphhip call returned a negative status."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"libsphhip error {code}: {msg}")
+        self.code = code
+
+
+def load_library(path=LIB_PATH):
+    """Load libsphhip.so and declare its signatures (no GPU needed)."""
+    global _lib
+    with _lib_lock:
+        if _lib is not None and path == LIB_PATH:
+            return _lib
+        if not os.path.exists(path):
+            raise ImportError(
+                f"{path} is missing: build it with "
+                f"`python -c 'import __graft_entry__ as g; g.build()'` "
+                f"(pysph_amd has no CPU fallback)")
+        lib = ctypes.CDLL(path)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        if path == LIB_PATH:
+            _lib = lib
+        return lib
+
+
+def ptr(t):
+    """Device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+class Context:
+    """One sph_ctx per device, bound to the caller's current torch stream."""
+
+    def __init__(self, device=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("pysph_amd needs a HIP device (torch.cuda.is_available() is "
+                               "False); it has no CPU fallback")
+        self.lib = load_library()
+        self.device = torch.device(device) if device is not None else torch.device("cuda", 0)
+        idx = self.device.index if self.device.index is not None else 0
+        self.device_index = idx
+        self.stream = torch.cuda.current_stream(self.device)
+        h = _vp()
+        rc = self.lib.sph_ctx_create(ctypes.byref(h), idx, _vp(self.stream.cuda_stream))
+        if rc != SPH_OK:
+            raise SphError(rc, f"sph_ctx_create(device={idx}) failed")
+        self.handle = h
+
+    def sync_stream(self):
+        """Follow torch's current stream (torch ops and our kernels stay ordered)."""
+        s = torch.cuda.current_stream(self.device)
+        if s.cuda_stream != self.stream.cuda_stream:
+            self.stream = s
+            self.check(self.lib.sph_ctx_set_stream(self.handle, _vp(s.cuda_stream)))
+
+    def check(self, rc, allow=()):
+        if rc == SPH_OK or rc in allow:
+            return rc
+        msg = self.lib.sph_last_error(self.handle)
+        raise SphError(rc, msg.decode() if msg else "")
+
+    def synchronize(self):
+        self.check(self.lib.sph_ctx_synchronize(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self.lib.sph_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_contexts = {}
+
+
+def context(device=None):
+    """The shared Context for `device` (default cuda:0)."""
+    dev = torch.device(device) if device is not None else torch.device("cuda", 0)
+    key = dev.index if dev.index is not None else 0
+    ctx = _contexts.get(key)
+    if ctx is None:
+        ctx = Context(torch.device("cuda", key))
+        _contexts[key] = ctx
+    ctx.sync_stream()
+    return ctx

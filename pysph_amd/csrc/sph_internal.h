@@ -1,0 +1,121 @@
+// Internal definitions shared by the HIP translation units of libsphhip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#include "sphhip.h"
+
+#define SPH_VERSION_STRING "pysph_amd libsphhip 0.1 (gfx950)"
+
+struct sph_ctx {
+    int device;
+    hipStream_t stream;
+    char err[512];
+    // growable device scratch, carved by the calls that need it
+    void *scratch;
+    size_t scratch_bytes;
+    // pinned host staging for small readbacks
+    void *pinned;
+    size_t pinned_bytes;
+    // per-axis cell-key tables of the current grid (device, int32)
+    int32_t *key_tab;
+    int64_t key_tab_len;
+    int32_t key_dims[3], key_bits[3], key_order;
+};
+
+int sph_set_error(sph_ctx *ctx, int code, const char *fmt, ...);
+int sph_ensure_scratch(sph_ctx *ctx, size_t bytes);
+int sph_ensure_pinned(sph_ctx *ctx, size_t bytes);
+int sph_upload_key_tables(sph_ctx *ctx, const sph_grid *g);
+
+#define SPH_HIP_CHECK(ctx, expr)                                                              \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return sph_set_error((ctx), SPH_EHIP, "%s failed: %s", #expr,                    \
+                                 hipGetErrorString(e_));                                      \
+    } while (0)
+
+#define SPH_LAUNCH_CHECK(ctx, what)                                                           \
+    do {                                                                                      \
+        hipError_t e_ = hipGetLastError();                                                    \
+        if (e_ != hipSuccess)                                                                 \
+            return sph_set_error((ctx), SPH_EHIP, "launch of %s failed: %s", (what),         \
+                                 hipGetErrorString(e_));                                      \
+    } while (0)
+
+#define SPH_REQUIRE(ctx, cond, ...)                                                           \
+    do {                                                                                      \
+        if (!(cond)) return sph_set_error((ctx), SPH_EINVAL, __VA_ARGS__);                    \
+    } while (0)
+
+static inline unsigned sph_blocks(int64_t n, int threads)
+{
+    int64_t b = (n + threads - 1) / threads;
+    return (unsigned)(b > 0 ? b : 1);
+}
+
+// Exclusive prefix sums (device-wide), stream-ordered.  `data` has n
+// elements; the scanned values are written in place and the total is
+// written to data[n] (so data must hold n+1 elements).
+int sph_scan_exclusive_i32(sph_ctx *ctx, int32_t *data, int64_t n);
+int sph_scan_exclusive_i64(sph_ctx *ctx, int64_t *data, int64_t n);
+
+// ------------------------------------------------------------------------
+// Cubic spline kernel in the reference's expression order.
+// base/kernels.pyx:51-57 (h_dim), :355-378 (function), :380-417 (gradient),
+// :441-447 (_fac, taken at h = 1.0 in KernelBase.__init__, :75).
+// Compiled with -ffp-contract=off: no FMA contraction, IEEE sqrt and
+// division, so each value is the same double the reference computes.
+// ------------------------------------------------------------------------
+#define SPH_PI 3.141592653589793
+
+__device__ __forceinline__ double sph_h_dim(double h, int dim)
+{
+    if (dim == 1) return 1 / h;
+    else if (dim == 2) return 1 / (h * h);
+    else return 1 / (h * h * h);
+}
+
+__host__ __device__ __forceinline__ double sph_cubic_fac1(int dim)
+{
+    if (dim == 1) return (2. / 3.) / 1.0;
+    else if (dim == 2) return 10 / (7 * SPH_PI) / (1.0 * 1.0);
+    else return 1. / SPH_PI / (1.0 * 1.0 * 1.0);
+}
+
+// W(pa, pb, h) given the three coordinate differences pa - pb
+__device__ __forceinline__ double sph_cubic_w(int dim, double rx, double ry, double rz, double h)
+{
+    double rab = sqrt(rx * rx + ry * ry + rz * rz);
+    double q = rab / h;
+    double val;
+    double fac = sph_h_dim(h, dim) * sph_cubic_fac1(dim);
+    if (q > 2.0)
+        val = 0.0;
+    else if (q > 1.0)
+        val = 0.25 * (2 - q) * (2 - q) * (2 - q);
+    else
+        val = 1 - 1.5 * (q * q) * (1 - 0.5 * q);
+    return val * fac;
+}
+
+// gradient factor val*fac so that gradW = r * (val*fac)
+__device__ __forceinline__ double sph_cubic_gfac(int dim, double rx, double ry, double rz,
+                                                 double h)
+{
+    double rab = sqrt(rx * rx + ry * ry + rz * rz);
+    double q = rab / h;
+    double val = 0.0;
+    double fac = sph_h_dim(h, dim) * sph_cubic_fac1(dim);
+    if (q > 2.0) {
+    } else if (q >= 1.0) {
+        val = -0.75 * (2 - q) * (2 - q) / (h * rab);
+    } else if (q > 1e-14) {
+        val = 3.0 * (0.75 * q - 1) / (h * h);
+    }
+    return val * fac;
+}

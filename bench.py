@@ -1,0 +1,249 @@
+"""Benchmark of the PySPH particle hot path on one MI355X.
+
+Default workload: the 3D WCSPH dam break -- 1,000,000 fluid particles
+(1.0 x 0.5 x 2.0 column at dx = 0.01) in a 4.0 x 0.5 x 2.5 open tank of
+three wall layers (752,208 particles), h = 1.3 dx, rho0 = 1000,
+c0 = 10 sqrt(2 g 2), gamma = 7, alpha = 0.1, beta = 0, gravity -9.81 z,
+PredictorCorrector, dt = 0.25 h / c0, FP64.  One step = one full
+PredictorCorrector step through the public solver API: two evaluations,
+each rebuilding the fluid<-fluid and fluid<-wall neighbour lists, then
+density, Tait, momentum + AV, gravity, position stepping and the stage
+arithmetic.
+
+metric: pair-interactions/s = sum of neighbour-list lengths over every
+neighbour-operator evaluation (density and momentum, both sources, both
+stages) per second of wall time.
+
+``--workload lattice`` runs the NNPS + SummationDensity micro-workload
+(100^3 jittered lattice, h = 1.2 dx) instead.
+
+Also reported: the roofline fraction of the dominant kernel (momentum)
+measured with HIP events on its stream, and the FP64 CPU oracle timed on
+the host cores on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def _args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=("dambreak", "lattice"), default="dambreak")
+    ap.add_argument("--dx", type=float, default=0.01)
+    ap.add_argument("--n", type=int, default=100, help="lattice edge for --workload lattice")
+    ap.add_argument("--cell-order", choices=("morton", "linear"), default="morton")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def momentum_bytes(nd, nnz, ns_other):
+    """Algorithmic HBM bytes of one momentum launch: per destination the
+    int64 row offset, 10 destination properties + dt_fac read, ax ay az
+    read-modify-write and dt_fac written (8+11*8+3*16+8 = 152 B), 4 B per
+    list entry, and -- when the source is another array -- its 11
+    properties once (88 B per source particle)."""
+    return nd * 152 + 4 * nnz + 88 * ns_other
+
+
+def run_dambreak(args):
+    import torch
+    from pysph_amd import profiling
+    from pysph_amd.solver.configs import dambreak_solver
+    from pysph_amd.solver.generators import dam_break_3d
+
+    data = dam_break_3d(dx=args.dx)
+    fl, wl, prm = data
+    s, particles, _ = dambreak_solver(data=data, cell_order=args.cell_order)
+    fluid, wall = particles.arrays
+    integ = s.integrator
+    dt = prm["dt"]
+    dens_calc = integ.calcs[0]
+    mom_calc = [c for c in integ.calcs if c.id.startswith("mom")][0]
+
+    for _ in range(args.warmup):
+        integ.integrate(dt)
+    torch.cuda.synchronize()
+    profiling.reset()
+    profiling.enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        integ.integrate(dt)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    profiling.enable(False)
+    pairs = profiling.pairs()
+    prof = profiling.summary()
+    elapsed = t1 - t0
+    nd = fluid.get_number_of_particles()
+    nw = wall.get_number_of_particles()
+    nnz_ff = mom_calc.funcs[0].nbr_locator.nnz
+    nnz_fw = mom_calc.funcs[1].nbr_locator.nnz
+    # dominant kernel: momentum (two launches per evaluation: fluid, wall source)
+    mom = prof.get("momentum_av", {"mean_ms": float("nan"), "count": 0})
+    b_ff = momentum_bytes(nd, nnz_ff, 0)
+    b_fw = momentum_bytes(nd, nnz_fw, nw)
+    bytes_per_launch = (b_ff + b_fw) / 2.0
+    achieved = bytes_per_launch / (mom["mean_ms"] * 1e-3) / 1e9
+    result = {
+        "metric": "pair-interactions/s",
+        "value": pairs / elapsed,
+        "unit": "pair-interactions/s",
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "dtype": "f64",
+        "config": {
+            "workload": "3D WCSPH dam break, PredictorCorrector step, FP64",
+            "fluid_particles": nd, "wall_particles": nw, "dx": args.dx,
+            "h": prm["h"], "c0": prm["c0"], "dt": dt, "cell_order": args.cell_order,
+            "steps_per_s": args.steps / elapsed,
+            "pairs_per_step": pairs / args.steps,
+            "nnz_fluid_fluid": nnz_ff, "nnz_fluid_wall": nnz_fw,
+            "mean_neighbours": (nnz_ff + nnz_fw) / nd,
+            "kernel_ms": {k: round(v["mean_ms"], 4) for k, v in prof.items()},
+            "kernel_launches": {k: v["count"] for k, v in prof.items()},
+        },
+        "roofline": {
+            "kernel": "k_momentum",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "mean_launch_ms": mom["mean_ms"],
+        },
+    }
+    return result, data
+
+
+def run_lattice(args):
+    import torch
+    from pysph_amd import profiling
+    from pysph_amd.base import api as base
+    from pysph_amd.sph import api as sph
+    from pysph_amd.solver.generators import jittered_lattice_3d
+
+    d = jittered_lattice_3d(n=args.n)
+    pa = base.get_particle_array(name="fluid", **d)
+    particles = base.Particles([pa], cell_order=args.cell_order)
+    kernel = base.CubicSplineKernel(dim=3)
+    func = sph.SPHRho.withargs().get_func(pa, pa)
+    calc = sph.SPHCalc(particles, [pa], pa, kernel, [func], ['rho'])
+
+    def step():
+        pa.set_dirty(True)  # force the rebuild: one step = binning + lists + density
+        particles.update()
+        calc.sph('rho')
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    profiling.reset()
+    profiling.enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    profiling.enable(False)
+    prof = profiling.summary()
+    n = pa.get_number_of_particles()
+    nnz = func.nbr_locator.nnz
+    elapsed = t1 - t0
+    dens = prof["summation_density"]
+    b = n * (8 + 4 * 8 + 8) + 4 * nnz  # offsets, x y z h m... see DESIGN
+    achieved = b / (dens["mean_ms"] * 1e-3) / 1e9
+    return {
+        "metric": "pair-interactions/s",
+        "value": nnz * args.steps / elapsed,
+        "unit": "pair-interactions/s",
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "dtype": "f64",
+        "config": {"workload": f"3D NNPS + SummationDensity, {args.n}^3 jittered lattice, FP64",
+                   "particles": n, "nnz": nnz, "steps_per_s": args.steps / elapsed,
+                   "kernel_ms": {k: round(v["mean_ms"], 4) for k, v in prof.items()}},
+        "roofline": {"kernel": "k_summation_density", "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes_per_launch": b,
+                     "mean_launch_ms": dens["mean_ms"]},
+    }, d
+
+
+def cpu_baseline_dambreak(args, data):
+    """The FP64 oracle on the host cores: one PredictorCorrector step of the
+    same 1M-fluid dam break (about 10-30 s of CPU work)."""
+    from oracle import cpu_ref
+    fl, wl, prm = data
+    threads = args.cpu_threads or len(os.sched_getaffinity(0))
+    ref = cpu_ref.RefSimulation([cpu_ref.RefArray("fluid", 0, fl), cpu_ref.RefArray("wall", 1, wl)],
+                                dim=3, integrator="pc", co=prm["c0"], ro=prm["rho0"], gamma=7.0,
+                                alpha=0.1, beta=0.0, gravity=(0.0, 0.0, prm["g"]),
+                                density_from=(0, 1), momentum_from=(0, 1), nthreads=threads)
+    t0 = time.perf_counter()
+    ref.integrate(prm["dt"])
+    t1 = time.perf_counter()
+    return {"value": ref.pairs / (t1 - t0), "unit": "pair-interactions/s", "cores": threads,
+            "kind": "port",
+            "sample": f"1 PredictorCorrector step of the same dam break ({ref.pairs} pair "
+                      f"interactions, {t1 - t0:.2f} s, {threads} OpenMP threads, gcc -O2 "
+                      f"-ffp-contract=off)",
+            "steps_per_s": 1.0 / (t1 - t0)}
+
+
+def cpu_baseline_lattice(args, d):
+    from oracle import cpu_ref
+    import numpy as np
+    threads = args.cpu_threads or len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    cs = cpu_ref.compute_cell_size(d["h"].max(), 2.0)
+    off, idx = cpu_ref.nnps(d, d, 2.0, cs, True, nthreads=threads)
+    rho = np.zeros(len(d["x"]))
+    cpu_ref.summation_density(3, d, d, off, idx, rho, nthreads=threads)
+    t1 = time.perf_counter()
+    return {"value": len(idx) / (t1 - t0), "unit": "pair-interactions/s", "cores": threads,
+            "kind": "port", "sample": f"1 NNPS + SummationDensity pass of the same lattice "
+                                      f"({t1 - t0:.2f} s, {threads} OpenMP threads)"}
+
+
+def main():
+    args = _args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs the hot path on a single MI355X "
+              f"and has no multi-GPU path (asked for --gpus {args.gpus}).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(1)
+    if args.workload == "dambreak":
+        result, data = run_dambreak(args)
+    else:
+        result, data = run_lattice(args)
+    out = {"metric": result["metric"], "value": result["value"], "unit": result["unit"],
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": result["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": result["dtype"], "data": "synthetic",
+           "config": result["config"], "roofline": result["roofline"]}
+    if not args.no_cpu_baseline:
+        if args.workload == "dambreak":
+            out["cpu_baseline"] = cpu_baseline_dambreak(args, data)
+        else:
+            out["cpu_baseline"] = cpu_baseline_lattice(args, data)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -7,7 +7,16 @@
  * Conventions
  *  - Every pointer argument is a DEVICE pointer (e.g. torch.Tensor.data_ptr())
  *    unless its comment says "host".  Property arrays are contiguous SoA,
- *    one array per property, in the particle order of the ParticleArray.
+ *    one array per property.
+ *  - Two particle orders exist.  ORIGINAL order is the order of the caller's
+ *    ParticleArray.  SORTED order is the order of sph_bin_array: particles
+ *    grouped by cell (cells in Morton or linear order), ascending original
+ *    index inside a cell; perm[t] is the original index at sorted position
+ *    t and rank[i] the sorted position of original index i.  "Shadow"
+ *    arrays are properties copied into sorted order (sph_gather_sorted);
+ *    the neighbour-list and operator kernels read shadows (coalesced,
+ *    cell-coherent loads) and write their results in original order (and
+ *    optionally into a shadow too).
  *  - Calls are asynchronous and ordered on the context's stream.  Only the
  *    calls that return a size to the host (sph_grid_build, sph_nnps_build)
  *    synchronise the stream.
@@ -15,11 +24,12 @@
  *    of the last failure is sph_last_error(ctx).  The library never prints
  *    and never aborts.
  *  - Buffers are owned by the caller; the context owns only scratch (scan
- *    partials, reduction buffers, counters) and frees it in sph_ctx_destroy.
+ *    partials, reduction buffers, key tables) and frees it in sph_ctx_destroy.
  *  - Accumulating outputs follow SPHFunction.eval (sph/sph_func.pyx:208-227):
  *    with accumulate=0 the output is (0.0 + result), with accumulate=1 it is
- *    (out + result).  Destinations at index >= n_real (non-LocalReal, which
- *    ParticleArray.align_particles keeps at the end) are set to 0.
+ *    (out + result).  Destinations whose original index is >= n_real
+ *    (non-LocalReal, which ParticleArray.align_particles keeps at the end)
+ *    get 0.
  */
 #ifndef SPHHIP_H
 #define SPHHIP_H
@@ -55,11 +65,13 @@ typedef struct sph_grid {
 
 enum { SPH_ORDER_MORTON = 0, SPH_ORDER_LINEAR = 1 };
 
+#define SPH_MAX_PROPS 16
+
 /* ---- context ---------------------------------------------------------- */
 
 /* Create a context on HIP device `device`; `stream` is a hipStream_t (NULL
- * = the null stream).  Replaces nothing in the reference: the reference's
- * device state lives in pyopencl objects (solver/cl_utils.py:1-110). */
+ * = the null stream).  The reference keeps its device state in pyopencl
+ * objects (solver/cl_utils.py:1-110, particle_array.pyx:1269-1396). */
 int sph_ctx_create(sph_ctx **out, int device, void *stream);
 int sph_ctx_destroy(sph_ctx *ctx);
 int sph_ctx_set_stream(sph_ctx *ctx, void *stream);
@@ -73,93 +85,148 @@ const char *sph_version(void);
 /* CellManager.compute_cell_size + bounding box over `narrays` arrays.
  * xs/ys/zs/hs are HOST arrays of DEVICE pointers, ns HOST counts.
  * radius_scale, min_cell_size, max_cell_size as in cell.pyx:802-941.
- * `order` selects the cell order of the counting sort.  Synchronises.
- * Replaces CellManager.compute_cell_size/_compute_minmax_h
- * (base/cell.pyx:911-941, :1140-1158). */
+ * `order` selects the cell order of the counting sort; max_cells bounds the
+ * dense cell table.  Synchronises.  Replaces
+ * CellManager.compute_cell_size/_compute_minmax_h (cell.pyx:911-941, :1140-1158). */
 int sph_grid_build(sph_ctx *ctx, int narrays, const double *const *xs, const double *const *ys,
                    const double *const *zs, const double *const *hs, const int64_t *ns,
                    double radius_scale, double min_cell_size, double max_cell_size, int order,
                    int64_t max_cells, sph_grid *grid /* host out */);
 
 /* Counting sort of one array into the grid's cells (stable: ascending
- * particle index inside a cell).  Outputs:
+ * original index inside a cell).  Outputs:
  *   cell_start[grid->ncells + 1]  first sorted position of every cell
  *   perm[n]                       original index at each sorted position
- *   sx, sy, sz [n]                positions in sorted order
+ *   rank[n]                       sorted position of each original index
  *   cell_of[n]                    cell key of each particle (original order)
- * Replaces CellManager._rebin_particles/_insert_particles
- * (base/cell.pyx:1179-1306). */
+ *   cells[min(n, ncells)]         keys of the non-empty cells, ascending
+ *   *ncells_out (host)            number of non-empty cells
+ * Synchronises (reads the cell count).  Replaces
+ * CellManager._rebin_particles/_insert_particles (base/cell.pyx:1179-1306). */
 int sph_bin_array(sph_ctx *ctx, const sph_grid *grid, const double *x, const double *y,
-                  const double *z, int64_t n, int32_t *cell_start, int32_t *perm,
-                  double *sx, double *sy, double *sz, int32_t *cell_of);
+                  const double *z, int64_t n, int32_t *cell_start, int32_t *perm, int32_t *rank,
+                  int32_t *cell_of, int32_t *cells, int64_t *ncells_out);
 
-/* Neighbour lists of FixedDestNbrParticleLocator (base/nnps.pyx:458-621) for
- * destinations [0, nd_real): source j is a neighbour of destination i iff
- * distance2(src_j, dst_i) < (h_i*radius_scale)^2, stencil (2d+1)^3 with
- * d = ceil(h_i*radius_scale/cell_size).  Row i holds source indices (original
- * order) in the reference order: stencil cells x->y->z, ascending index in a
- * cell, and with same_array != 0 the destination's own index moved to the
- * last slot (nnps.pyx:612-619).
- *   offsets[nd_real + 1] (int64), indices[cap] (int32).
- * If cap < nnz, returns SPH_ENEEDCAP with *nnz_out = nnz and offsets filled.
- * Synchronises (reads nnz). */
-int sph_nnps_build(sph_ctx *ctx, const sph_grid *grid, const double *dx, const double *dy,
-                   const double *dz, const double *dh, int64_t nd_real,
-                   const int32_t *src_cell_start, const int32_t *src_perm, const double *src_sx,
-                   const double *src_sy, const double *src_sz, int64_t ns, double radius_scale,
-                   int same_array, int64_t *offsets, int32_t *indices, int64_t cap,
+/* Sorted particle records: SPH_REC doubles per particle in sorted order,
+ * the layout every neighbour kernel reads (one 96-byte record per
+ * neighbour instead of one load per property). */
+#define SPH_REC 12
+enum sph_rec_field {
+    SPH_R_X = 0, SPH_R_Y, SPH_R_Z, SPH_R_H, SPH_R_M, SPH_R_RHO, SPH_R_CS, SPH_R_P,
+    SPH_R_U, SPH_R_V, SPH_R_W, SPH_R_PRHO2 /* p * (1/(rho*rho)), filled by sph_momentum_av */
+};
+
+/* Shadows: dst[k][t * dst_stride] = src[k][perm[t]] for k < nprops (sorted
+ * copies of ParticleArray properties, e.g. record fields with stride
+ * SPH_REC).  No reference counterpart: the reference reads its carrays in
+ * place (sph_func.pyx:180-206). */
+typedef struct sph_prop_list {
+    int32_t nprops;
+    int64_t dst_stride;
+    const double *src[SPH_MAX_PROPS];
+    double *dst[SPH_MAX_PROPS];
+} sph_prop_list;
+int sph_gather_sorted(sph_ctx *ctx, int64_t n, const int32_t *perm, const sph_prop_list *props);
+
+/* Neighbour lists of FixedDestNbrParticleLocator (base/nnps.pyx:458-621)
+ * in SORTED space, stored as sliced ELLPACK with 64-row slices (SELL-64):
+ * destinations are grouped into slices of 64 consecutive sorted positions;
+ * entry k of the row of sorted destination t is
+ *     indices[slice_off[t / 64] + 64 * k + t % 64],   k < counts[t],
+ * so the 64 lanes of a wavefront read one coalesced 256-byte line per k.
+ * Rows are empty when d_perm[t] >= nd_real; entries are SORTED source
+ * positions.  Source j is a neighbour of destination i iff
+ * distance2(src_j, dst_i) < (h_i*radius_scale)^2 (stencil (2d+1)^3,
+ * d = ceil(h_i*radius_scale/cell_size)).  Entries follow the reference
+ * order: stencil cells x -> y -> z, ascending index inside a cell, and with
+ * same_array != 0 the destination itself moved to the last slot
+ * (nnps.pyx:612-619).  One wavefront handles one destination cell; its
+ * lanes test 64 candidates at a time and compact the hits with ballots.
+ *   d_rec [nd_total*SPH_REC], d_perm [nd_total], d_cell_start and d_cells /
+ *   nd_cells: the destination's binning; s_cell_start, s_rec [ns*SPH_REC]:
+ *   the source's.
+ *   counts[nd_total] (int32), slice_off[ceil(nd_total/64) + 1] (int64),
+ *   indices[cap] (int32).
+ * Returns *slots_out = slice_off[last] (the storage the entries need) and
+ * *nnz_out = sum of counts; if cap < slots returns SPH_ENEEDCAP with both
+ * filled.  Synchronises (reads the sizes). */
+int sph_nnps_build(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
+                   const int32_t *d_perm, const int32_t *d_cell_start, const int32_t *d_cells,
+                   int64_t nd_cells, int64_t nd_total, int64_t nd_real,
+                   const int32_t *s_cell_start, const double *s_rec, int64_t ns,
+                   double radius_scale, int same_array, int32_t *counts, int64_t *slice_off,
+                   int32_t *indices, int64_t cap, int64_t *slots_out /* host */,
                    int64_t *nnz_out /* host */);
 
-/* ---- operators --------------------------------------------------------- */
+/* The same lists in ORIGINAL index space as CSR, the layout of the
+ * reference's per-destination LongArrays (nnps.pyx:596-621): row i
+ * (i < nd_real) holds original source indices in the same order.
+ * out_offsets[nd_real + 1]; out_indices[nnz]. */
+int sph_nnps_export(sph_ctx *ctx, int64_t nd_real, const int32_t *d_rank,
+                    const int32_t *s_perm, const int32_t *counts, const int64_t *slice_off,
+                    const int32_t *indices, int64_t *out_offsets, int32_t *out_indices);
+
+/* ---- operators (sorted records in, original order out) ------------------ */
 
 /* SPHRho summation density (sph/funcs/density_funcs.pyx:37-73, driven by
  * CSPHFunctionParticle.eval_single sph/sph_func.pyx:513-531):
- *   rho[i] (+)= sum_{j in row i} m_j W(x_i - x_j, (h_i+h_j)/2),  dim in {1,2,3}.
- * Destinations in [nd_real, nd_total) get 0. */
-int sph_summation_density(sph_ctx *ctx, int dim, int64_t nd_real, int64_t nd_total,
-                          const int64_t *offsets, const int32_t *indices, const double *dx,
-                          const double *dy, const double *dz, const double *dh,
-                          const double *sx, const double *sy, const double *sz,
-                          const double *sh, const double *sm, int accumulate, double *rho);
+ *   rho_i (+)= sum_{j in row} m_j W(x_i - x_j, (h_i+h_j)/2),  dim in {1,2,3}.
+ * (counts, slice_off, indices) are the SELL-64 lists of sph_nnps_build.
+ * rho[nd_total] is written in original order (rho[d_perm[t]]); out_s (may
+ * be NULL) receives the same values at out_s[t * out_s_stride]. */
+int sph_summation_density(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                          const int32_t *d_perm, const int32_t *counts,
+                          const int64_t *slice_off, const int32_t *indices,
+                          const double *d_rec, const double *s_rec, int accumulate, double *rho,
+                          double *out_s, int64_t out_s_stride);
 
-/* TaitEquation (sph/funcs/eos_funcs.pyx:80-129):
- *   p (+)= (pow(rho/ro, gamma) - 1) * co*co*ro/gamma,  cs (+)= pow(rho/ro, (gamma-1)/2) * co */
-int sph_tait(sph_ctx *ctx, int64_t n_real, int64_t n_total, const double *rho, double ro,
-             double co, double gamma, int accumulate, double *p, double *cs);
+/* TaitEquation (sph/funcs/eos_funcs.pyx:80-129) over sorted positions:
+ *   p (+)= (pow(rho/ro, gamma) - 1) * co*co*ro/gamma,
+ *   cs (+)= pow(rho/ro, (gamma-1)/2) * co
+ * rho_s[t * s_stride] is the sorted density; p, cs are written in original
+ * order and p_s, cs_s (may be NULL) at [t * s_stride]. */
+int sph_tait(sph_ctx *ctx, int64_t n_total, int64_t n_real, const int32_t *perm,
+             const double *rho_s, int64_t s_stride, double ro, double co, double gamma,
+             int accumulate, double *p, double *cs, double *p_s, double *cs_s);
 
 /* MomentumEquation with Monaghan artificial viscosity, hks=False, no
  * artificial pressure (sph/funcs/pressure_funcs.pyx:185-391):
  *   a_i (+)= sum_j -m_j (P_i/rho_i^2 + P_j/rho_j^2 + Pi_ij) gradW_ij(h_ij)
  *   dt_fac_i = max(dt_fac_i, |h_ij v_ij.r_ij / (r_ij^2 + eta^2 h_ij^2)|)
- * az may be NULL (no z output). dst = {x,y,z,h,rho,p,u,v,w,cs},
- * src = {x,y,z,h,m,rho,p,u,v,w,cs}. */
-typedef struct sph_mom_dst {
-    const double *x, *y, *z, *h, *rho, *p, *u, *v, *w, *cs;
-} sph_mom_dst;
-typedef struct sph_mom_src {
-    const double *x, *y, *z, *h, *m, *rho, *p, *u, *v, *w, *cs;
-} sph_mom_src;
-int sph_momentum_av(sph_ctx *ctx, int dim, int64_t nd_real, int64_t nd_total,
-                    const int64_t *offsets, const int32_t *indices, const sph_mom_dst *dst,
-                    const sph_mom_src *src, double alpha, double beta, double eta,
-                    int accumulate, double *ax, double *ay, double *az, double *dt_fac);
+ * d_rec/s_rec are sorted records (s_rec has ns records; its SPH_R_PRHO2
+ * field is recomputed from its p and rho fields first); ax, ay, az (az may
+ * be NULL) and dt_fac are in original order. */
+int sph_momentum_av(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                    const int32_t *d_perm, const int32_t *counts, const int64_t *slice_off,
+                    const int32_t *indices,
+                    const double *d_rec, double *s_rec, int64_t ns, double alpha, double beta,
+                    double eta, int accumulate, double *ax, double *ay, double *az,
+                    double *dt_fac);
 
-/* GravityForce (sph/funcs/external_force.pyx:38-44): a (+)= g; az may be NULL. */
+/* GravityForce (sph/funcs/external_force.pyx:38-44): a (+)= g; az may be
+ * NULL.  Original order. */
 int sph_gravity(sph_ctx *ctx, int64_t n_real, int64_t n_total, double gx, double gy, double gz,
                 int accumulate, double *ax, double *ay, double *az);
 
-/* PositionStepping (sph/funcs/position_funcs.pyx:29-43): a_x = u (assignment). */
+/* PositionStepping (sph/funcs/position_funcs.pyx:29-43): a_x = u
+ * (assignment).  Original order. */
 int sph_position_stepping(sph_ctx *ctx, int64_t n_real, int64_t n_total, const double *u,
                           const double *v, const double *w, double *ax, double *ay, double *az);
 
+/* CubicSplineKernel.function / .gradient (base/kernels.pyx:355-417) for n
+ * point pairs: w[i] = W(a_i - b_i, h_i), g*[i] = gradW components (any
+ * output may be NULL). */
+int sph_cubic_spline(sph_ctx *ctx, int dim, int64_t n, const double *ax, const double *ay,
+                     const double *az, const double *bx, const double *by, const double *bz,
+                     const double *h, double *w, double *gx, double *gy, double *gz);
+
 /* ---- integrator stage arithmetic (solver/integrator.py) ----------------- */
 
-#define SPH_MAX_STEP_PROPS 16
 typedef struct sph_step_props {
     int32_t nprops;
-    double *x[SPH_MAX_STEP_PROPS];         /* stepped property (out) */
-    const double *x0[SPH_MAX_STEP_PROPS];  /* initial value */
-    const double *a[SPH_MAX_STEP_PROPS];   /* acceleration */
+    double *x[SPH_MAX_PROPS];         /* stepped property (out) */
+    const double *x0[SPH_MAX_PROPS];  /* initial value */
+    const double *a[SPH_MAX_PROPS];   /* acceleration */
 } sph_step_props;
 
 /* Integrator.step (integrator.py:340-367): x = x0 + a*dt for every prop. */

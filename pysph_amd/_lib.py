@@ -23,7 +23,8 @@ SPH_ENOMEM = -5
 ORDER_MORTON = 0
 ORDER_LINEAR = 1
 
-MAX_STEP_PROPS = 16
+MAX_PROPS = 16
+MAX_STEP_PROPS = MAX_PROPS
 
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -38,19 +39,18 @@ class SphGrid(ctypes.Structure):
                 ("ncells", _i64)]
 
 
-class MomDst(ctypes.Structure):
-    _fields_ = [(k, _vp) for k in ("x", "y", "z", "h", "rho", "p", "u", "v", "w", "cs")]
-
-
-class MomSrc(ctypes.Structure):
-    _fields_ = [(k, _vp) for k in ("x", "y", "z", "h", "m", "rho", "p", "u", "v", "w", "cs")]
-
-
 class StepProps(ctypes.Structure):
     _fields_ = [("nprops", ctypes.c_int32),
-                ("x", _vp * MAX_STEP_PROPS),
-                ("x0", _vp * MAX_STEP_PROPS),
-                ("a", _vp * MAX_STEP_PROPS)]
+                ("x", _vp * MAX_PROPS),
+                ("x0", _vp * MAX_PROPS),
+                ("a", _vp * MAX_PROPS)]
+
+
+class PropList(ctypes.Structure):
+    _fields_ = [("nprops", ctypes.c_int32),
+                ("dst_stride", _i64),
+                ("src", _vp * MAX_PROPS),
+                ("dst", _vp * MAX_PROPS)]
 
 
 # name -> (restype, argtypes)
@@ -64,15 +64,19 @@ _SIGNATURES = {
     "sph_grid_build": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _dbl, _i32, _i64,
                               ctypes.POINTER(SphGrid)]),
     "sph_bin_array": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _i64, _vp, _vp, _vp,
-                             _vp, _vp, _vp]),
-    "sph_nnps_build": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _vp, _i64, _vp, _vp,
-                              _vp, _vp, _vp, _i64, _dbl, _i32, _vp, _vp, _i64,
-                              ctypes.POINTER(_i64)]),
-    "sph_summation_density": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp] + [_vp] * 9 + [_i32, _vp]),
-    "sph_tait": (_i32, [_vp, _i64, _i64, _vp, _dbl, _dbl, _dbl, _i32, _vp, _vp]),
-    "sph_momentum_av": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, ctypes.POINTER(MomDst),
-                               ctypes.POINTER(MomSrc), _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp,
-                               _vp]),
+                             _vp, _vp, ctypes.POINTER(_i64)]),
+    "sph_gather_sorted": (_i32, [_vp, _i64, _vp, ctypes.POINTER(PropList)]),
+    "sph_nnps_build": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _vp, _i64, _i64, _i64,
+                              _vp, _vp, _i64, _dbl, _i32, _vp, _vp, _vp, _i64,
+                              ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "sph_nnps_export": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "sph_summation_density": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32,
+                                     _vp, _vp, _i64]),
+    "sph_tait": (_i32, [_vp, _i64, _i64, _vp, _vp, _i64, _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp,
+                        _vp]),
+    "sph_momentum_av": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _dbl,
+                               _dbl, _dbl, _i32, _vp, _vp, _vp, _vp]),
+    "sph_cubic_spline": (_i32, [_vp, _i32, _i64] + [_vp] * 11),
     "sph_gravity": (_i32, [_vp, _i64, _i64, _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp]),
     "sph_position_stepping": (_i32, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sph_stage_step": (_i32, [_vp, _i64, ctypes.POINTER(StepProps), _dbl]),

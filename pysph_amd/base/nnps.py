@@ -4,13 +4,20 @@
 arrays, cell size ``radius_scale * max(h)`` clamped by min/max cell size
 (:911-941).  ``update`` re-bins lazily when any array is dirty (:943-976)
 with a counting sort into Morton-ordered cells (``sph_grid_build`` +
-``sph_bin_array``).
+``sph_bin_array``).  The manager also keeps *sorted shadows*: copies of the
+particle properties in cell order, which the operator kernels read with
+coalesced, cell-coherent loads.  A shadow is refreshed when the binning
+changed or the array's property was written since (``ParticleArray.touch``
+versions); kernels that produce a property write both the original-order
+array and its shadow.
 
 ``FixedDestNbrParticleLocator`` (base/nnps.pyx:385-686): one neighbour list
-per destination particle, cached as a CSR (int64 offsets, int32 indices in
-the source's particle order) and rebuilt lazily when the cell manager has
-re-binned (``sph_nnps_build``).  Rows hold the reference's order: stencil
-cells x -> y -> z, ascending index inside a cell, own index last.
+per destination particle, rebuilt lazily after the cell manager re-binned
+(``sph_nnps_build``).  Internally the lists live in sorted space (rows per
+sorted destination, entries are sorted source positions); ``get_csr`` /
+``get_nearest_particles`` export them to original indices in the
+reference's order: stencil cells x -> y -> z, ascending index inside a
+cell, own index last.
 
 ``NNPSManager`` (base/nnps.pyx:791-932): one locator per
 (source, destination, radius_scale).
@@ -20,7 +27,13 @@ import ctypes
 import numpy as np
 import torch
 
-from .. import _lib
+from .. import _lib, profiling
+
+#: sorted record layout (SPH_R_* in include/sphhip.h): property -> field
+REC = 12
+REC_FIELDS = {"x": 0, "y": 1, "z": 2, "h": 3, "m": 4, "rho": 5, "cs": 6, "p": 7, "u": 8,
+              "v": 9, "w": 10}
+SHADOW_PROPS = tuple(REC_FIELDS)
 
 
 class NeighborLocatorType:
@@ -30,18 +43,26 @@ class NeighborLocatorType:
 
 
 class _Bins:
-    """Per-array result of the counting sort."""
-
-    __slots__ = ("cell_start", "perm", "sx", "sy", "sz", "cell_of", "n")
+    """Per-array result of the counting sort and its sorted records (one
+    row of REC doubles per particle, fields REC_FIELDS)."""
 
     def __init__(self, n, ncells, device):
         self.n = n
+        self.device = device
         self.cell_start = torch.empty(ncells + 1, dtype=torch.int32, device=device)
         self.perm = torch.empty(max(n, 1), dtype=torch.int32, device=device)
-        self.sx = torch.empty(max(n, 1), dtype=torch.float64, device=device)
-        self.sy = torch.empty(max(n, 1), dtype=torch.float64, device=device)
-        self.sz = torch.empty(max(n, 1), dtype=torch.float64, device=device)
+        self.rank = torch.empty(max(n, 1), dtype=torch.int32, device=device)
         self.cell_of = torch.empty(max(n, 1), dtype=torch.int32, device=device)
+        self.cells = torch.empty(max(min(n, ncells), 1), dtype=torch.int32, device=device)
+        self.ncells_nonempty = 0
+        self.rec = torch.zeros((max(n, 1), REC), dtype=torch.float64, device=device)
+        self.shadow_ver = {}
+
+    def field_ptr(self, prop):
+        return self.rec.data_ptr() + 8 * REC_FIELDS[prop]
+
+    def invalidate(self):
+        self.shadow_ver = {}
 
 
 class CellManager:
@@ -95,6 +116,15 @@ class CellManager:
         if not arrays:
             self.is_dirty = False
             return 0
+        with profiling.region("bin"):
+            self._rebin(arrays)
+        for a in arrays:
+            a.set_dirty(False)
+        self.version += 1
+        self.is_dirty = False
+        return 0
+
+    def _rebin(self, arrays):
         ctx = self._ctx()
         lib = ctx.lib
         n = len(arrays)
@@ -116,38 +146,77 @@ class CellManager:
             if b is None or b.n != np_ or b.cell_start.numel() != grid.ncells + 1:
                 b = _Bins(np_, grid.ncells, a.device)
                 self.bins[a.name] = b
+            nne = ctypes.c_int64(0)
             ctx.check(lib.sph_bin_array(ctx.handle, ctypes.byref(grid),
                                         _lib.ptr(a.get_carray("x")), _lib.ptr(a.get_carray("y")),
                                         _lib.ptr(a.get_carray("z")), np_,
                                         _lib.ptr(b.cell_start), _lib.ptr(b.perm),
-                                        _lib.ptr(b.sx), _lib.ptr(b.sy), _lib.ptr(b.sz),
-                                        _lib.ptr(b.cell_of)))
-        for a in arrays:
-            a.set_dirty(False)
-        self.version += 1
-        self.is_dirty = False
-        return 0
+                                        _lib.ptr(b.rank), _lib.ptr(b.cell_of), _lib.ptr(b.cells),
+                                        ctypes.byref(nne)))
+            b.ncells_nonempty = int(nne.value)
+            b.invalidate()
+
+    # -- sorted records --------------------------------------------------
+    def shadows(self, array, props):
+        """The sorted record tensor of `array` with the fields of `props`
+        refreshed where stale (binning changed or the property was written)."""
+        self.update()
+        b = self.bins[array.name]
+        stale = [p for p in props if b.shadow_ver.get(p) != array.version(p)]
+        if stale:
+            ctx = self._ctx()
+            pl = _lib.PropList()
+            pl.nprops = len(stale)
+            pl.dst_stride = REC
+            for k, p in enumerate(stale):
+                src = array.get_carray(p)
+                if src is None:
+                    raise AttributeError(f"property {p} not present in {array.name}")
+                if src.dtype != torch.float64:
+                    raise TypeError(f"property {p} of {array.name} must be float64")
+                pl.src[k] = _lib.ptr(src)
+                pl.dst[k] = b.field_ptr(p)
+            with profiling.region("gather_sorted"):
+                ctx.check(ctx.lib.sph_gather_sorted(ctx.handle, b.n, _lib.ptr(b.perm),
+                                                    ctypes.byref(pl)))
+            for p in stale:
+                b.shadow_ver[p] = array.version(p)
+        return b.rec
+
+    def shadow_out(self, array, prop):
+        """Device pointer of the record field a kernel may write together
+        with `prop` (None when `prop` is not a record field)."""
+        if prop not in REC_FIELDS:
+            return None
+        return self.bins[array.name].field_ptr(prop)
+
+    def written(self, array, prop, shadow_written):
+        """Record a kernel write of `prop` (and of its record field if it wrote it)."""
+        array.touch(prop)
+        b = self.bins.get(array.name)
+        if b is not None and shadow_written and prop in REC_FIELDS:
+            b.shadow_ver[prop] = array.version(prop)
+
+    def sorted_field(self, array, prop):
+        """Host copy of one record field in sorted order (diagnostics)."""
+        rec = self.shadows(array, (prop,))
+        return rec[:self.bins[array.name].n, REC_FIELDS[prop]].cpu().numpy()
 
     # reference-style accessors
     def get_number_of_cells(self):
-        """Number of non-empty cells (len(cells_dict) in the reference)."""
+        """Number of non-empty cells over all binned arrays (len(cells_dict))."""
         self.update()
-        total = 0
+        total = None
         for b in self.bins.values():
-            counts = (b.cell_start[1:] - b.cell_start[:-1])
-            total = counts if isinstance(total, int) else total + counts
-        if isinstance(total, int):
-            return 0
-        return int((total > 0).sum().item())
+            counts = b.cell_start[1:] - b.cell_start[:-1]
+            total = counts if total is None else total + counts
+        return 0 if total is None else int((total > 0).sum().item())
 
     def cell_counts(self):
-        """Host dict cell-key -> particles per array (tests/diagnostics)."""
+        """Host dict array name -> particles per cell (diagnostics)."""
         self.update()
-        out = {}
-        for name, b in self.bins.items():
-            c = (b.cell_start[1:] - b.cell_start[:-1]).cpu().numpy()
-            out[name] = c
-        return out
+        return {name: (b.cell_start[1:] - b.cell_start[:-1]).cpu().numpy()
+                for name, b in self.bins.items()}
 
 
 class FixedDestNbrParticleLocator:
@@ -156,6 +225,8 @@ class FixedDestNbrParticleLocator:
     def __init__(self, source, dest, radius_scale, cell_manager=None, h="h"):
         if cell_manager is None:
             raise ValueError("FixedDestNbrParticleLocator needs a CellManager")
+        if h != "h":
+            raise NotImplementedError("the smoothing length property must be 'h'")
         self.source = source
         self.dest = dest
         self.radius_scale = float(radius_scale)
@@ -167,12 +238,17 @@ class FixedDestNbrParticleLocator:
             raise ValueError(f"Destination {dest.name} does not exist")
         if self.source_index is None:
             raise ValueError(f"Source {source.name} does not exist")
-        self.offsets = None
+        # SELL-64 lists in sorted space: counts per sorted destination,
+        # slice offsets, entries (see sph_nnps_build)
+        self.counts = None
+        self.slice_off = None
         self.indices = None
         self.nnz = 0
+        self.slots = 0
         self.built_version = -1
         self.is_dirty = True
         self.nbuilds = 0
+        self._export = None
         self._host = None
 
     def update_status(self):
@@ -185,9 +261,11 @@ class FixedDestNbrParticleLocator:
         cm.update()
         if self.built_version == cm.version and not self.is_dirty:
             return 0
-        self._build()
+        with profiling.region("nnps_build"):
+            self._build()
         self.built_version = cm.version
         self.is_dirty = False
+        self._export = None
         self._host = None
         return 0
 
@@ -196,32 +274,62 @@ class FixedDestNbrParticleLocator:
         ctx = cm._ctx()
         lib = ctx.lib
         dst, src = self.dest, self.source
-        nd = dst.num_real_particles
-        b = cm.bins[src.name]
+        nd_total = dst.get_number_of_particles()
+        nd_real = dst.num_real_particles
+        bd, bs = cm.bins[dst.name], cm.bins[src.name]
+        drec = cm.shadows(dst, ("x", "y", "z", "h"))
+        srec = cm.shadows(src, ("x", "y", "z"))
         dev = dst.device
-        if self.offsets is None or self.offsets.numel() != nd + 1:
-            self.offsets = torch.empty(nd + 1, dtype=torch.int64, device=dev)
+        nslices = (nd_total + 63) // 64
+        if self.counts is None or self.counts.numel() != max(nd_total, 1):
+            self.counts = torch.empty(max(nd_total, 1), dtype=torch.int32, device=dev)
+            self.slice_off = torch.empty(nslices + 1, dtype=torch.int64, device=dev)
         if self.indices is None:
-            self.indices = torch.empty(max(1, 64 * nd), dtype=torch.int32, device=dev)
+            self.indices = torch.empty(max(1, 96 * nd_total), dtype=torch.int32, device=dev)
+        slots = ctypes.c_int64(0)
         nnz = ctypes.c_int64(0)
-        args = lambda: (ctx.handle, ctypes.byref(cm.grid), _lib.ptr(dst.get_carray("x")),
-                        _lib.ptr(dst.get_carray("y")), _lib.ptr(dst.get_carray("z")),
-                        _lib.ptr(dst.get_carray(self.h)), nd, _lib.ptr(b.cell_start),
-                        _lib.ptr(b.perm), _lib.ptr(b.sx), _lib.ptr(b.sy), _lib.ptr(b.sz),
-                        b.n, self.radius_scale, int(src is dst), _lib.ptr(self.offsets),
-                        _lib.ptr(self.indices), self.indices.numel(), ctypes.byref(nnz))
-        rc = ctx.check(lib.sph_nnps_build(*args()), allow=(_lib.SPH_ENEEDCAP,))
+        p = _lib.ptr
+
+        def call():
+            return lib.sph_nnps_build(ctx.handle, ctypes.byref(cm.grid), p(drec), p(bd.perm),
+                                      p(bd.cell_start), p(bd.cells), bd.ncells_nonempty,
+                                      nd_total, nd_real, p(bs.cell_start), p(srec), bs.n,
+                                      self.radius_scale, int(src is dst), p(self.counts),
+                                      p(self.slice_off), p(self.indices), self.indices.numel(),
+                                      ctypes.byref(slots), ctypes.byref(nnz))
+
+        rc = ctx.check(call(), allow=(_lib.SPH_ENEEDCAP,))
         if rc == _lib.SPH_ENEEDCAP:
-            self.indices = torch.empty(int(nnz.value * 1.15) + 1024, dtype=torch.int32,
+            self.indices = torch.empty(int(slots.value * 1.15) + 4096, dtype=torch.int32,
                                        device=dev)
-            ctx.check(lib.sph_nnps_build(*args()))
+            ctx.check(call())
         self.nnz = int(nnz.value)
+        self.slots = int(slots.value)
         self.nbuilds += 1
 
-    def get_csr(self):
-        """Device CSR (offsets int64[nd+1], indices int32[nnz]) after update()."""
+    def get_sorted_lists(self):
+        """SELL-64 lists in sorted space: (counts int32[nd], slice_off
+        int64[nd/64+1], indices int32[slots])."""
         self.update()
-        return self.offsets, self.indices[:self.nnz]
+        return self.counts, self.slice_off, self.indices
+
+    def get_csr(self):
+        """Lists in original index space: (offsets int64[nd_real+1],
+        indices int32[nnz]); row i = neighbours of destination i."""
+        self.update()
+        if self._export is None:
+            cm = self.cell_manager
+            ctx = cm._ctx()
+            nd_real = self.dest.num_real_particles
+            dev = self.dest.device
+            off = torch.empty(nd_real + 1, dtype=torch.int64, device=dev)
+            idx = torch.empty(max(1, self.nnz), dtype=torch.int32, device=dev)
+            p = _lib.ptr
+            ctx.check(ctx.lib.sph_nnps_export(ctx.handle, nd_real, p(cm.bins[self.dest.name].rank),
+                                              p(cm.bins[self.source.name].perm), p(self.counts),
+                                              p(self.slice_off), p(self.indices), p(off), p(idx)))
+            self._export = (off, idx[:self.nnz])
+        return self._export
 
     def _host_csr(self):
         if self._host is None:

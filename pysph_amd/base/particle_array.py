@@ -55,6 +55,8 @@ class ParticleArray:
         self.particle_type = particle_type
         self.cl_precision = cl_precision
         self.num_real_particles = 0
+        # per-property write counters; device shadows compare against them
+        self._version = {}
         object.__setattr__(self, "_initialising", False)
         if props:
             self.initialize(**props)
@@ -88,6 +90,17 @@ class ParticleArray:
     def set_dirty(self, value):
         self.is_dirty = bool(value)
 
+    def touch(self, *props):
+        """Record that the named properties were written (all when none are
+        given).  Code that writes a tensor from ``get_carray`` directly must
+        call this so sorted device copies are refreshed."""
+        names = props if props else tuple(self.properties) + tuple(self.temporary_arrays)
+        for p in names:
+            self._version[p] = self._version.get(p, 0) + 1
+
+    def version(self, prop):
+        return self._version.get(prop, 0)
+
     def set_indices_invalid(self, value):
         self.indices_invalid = bool(value)
 
@@ -102,6 +115,7 @@ class ParticleArray:
                              ("pid", "int")):
             self.properties[pname] = torch.zeros(0, dtype=_DTYPES[ptype], device=self.device)
         self.temporary_arrays = {}
+        self.touch()
         self.is_dirty = True
         self.indices_invalid = True
 
@@ -181,6 +195,7 @@ class ParticleArray:
                     raise ValueError(
                         f"size mismatch setting {prop}: {src.numel()} != {target.numel()}")
                 target.copy_(src)
+            self.touch(prop)
             if prop == "tag":
                 self.align_particles()
             if prop in ("x", "y", "z"):
@@ -229,6 +244,7 @@ class ParticleArray:
             if data is not None:
                 arr = self.properties[prop_name]
                 arr.copy_(torch.as_tensor(data, dtype=arr.dtype).to(self.device))
+                self.touch(prop_name)
             return
         dt = _DTYPES.get(data_type, torch.float64)
         if data is not None:
@@ -236,12 +252,14 @@ class ParticleArray:
         else:
             t = torch.full((n,), default, dtype=dt, device=self.device)
         self.properties[prop_name] = t.contiguous()
+        self.touch()
 
     def add_temporary_array(self, arr_name):
         if arr_name in self.temporary_arrays or arr_name in self.properties:
             return
         self.temporary_arrays[arr_name] = torch.zeros(self.get_number_of_particles(),
                                                       dtype=torch.float64, device=self.device)
+        self.touch(arr_name)
 
     def get_property_arrays(self, all=False):
         ret = {}
@@ -272,6 +290,7 @@ class ParticleArray:
         for d in (self.properties, self.temporary_arrays):
             for k in list(d):
                 d[k] = d[k][order].contiguous()
+        self.touch()
         self.is_dirty = True
         self.indices_invalid = True
         return 0
@@ -296,6 +315,7 @@ class ParticleArray:
         for d in (self.properties, self.temporary_arrays):
             for k in list(d):
                 d[k] = d[k][keep].contiguous()
+        self.touch()
         self.align_particles()
         self.is_dirty = True
         self.indices_invalid = True
@@ -304,11 +324,13 @@ class ParticleArray:
         """dst <- src for every (src: dst) pair (particle_array.pyx:1180-1209)."""
         for prop, dst in props.items():
             self.get_carray(dst).copy_(self.get_carray(prop))
+            self.touch(dst)
 
     def set_to_zero(self, props):
         """particle_array.pyx:1211-1223"""
         for prop in props:
             self.get_carray(prop).zero_()
+            self.touch(prop)
 
     def get_type_name(self, prop):
         return _TYPE_NAMES[self.properties[prop].dtype]

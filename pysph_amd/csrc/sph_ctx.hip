@@ -194,32 +194,51 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_tile_scan(T *__restrict__ data
         *total_out = run;
 }
 
+static int scan_levels(int64_t n, size_t elem, int64_t *levels_n, size_t *levels_off,
+                       size_t *bytes_out)
+{
+    int nl = 0;
+    size_t bytes = 0;
+    int64_t m = n;
+    while (m > SCAN_TILE) {
+        if (nl >= 7) return -1;
+        int64_t nb = (m + SCAN_TILE - 1) / SCAN_TILE;
+        levels_n[nl] = nb;
+        levels_off[nl] = bytes;
+        bytes += (size_t)(nb + 1) * elem;
+        bytes = (bytes + 255) & ~(size_t)255;
+        ++nl;
+        m = nb;
+    }
+    *bytes_out = bytes + 256;
+    return nl;
+}
+
+size_t sph_scan_scratch_bytes(int64_t n, size_t elem)
+{
+    int64_t ln[8];
+    size_t lo[8], bytes = 256;
+    if (n > 0) scan_levels(n, elem, ln, lo, &bytes);
+    return bytes;
+}
+
 template <typename T>
-static int scan_exclusive(sph_ctx *ctx, T *data, int64_t n)
+static int scan_exclusive(sph_ctx *ctx, T *data, int64_t n, size_t scratch_off)
 {
     if (n <= 0) {
         SPH_HIP_CHECK(ctx, hipMemsetAsync(data, 0, sizeof(T), ctx->stream));
         return SPH_OK;
     }
-    // tile-sum levels live in scratch: level sizes n/TILE, n/TILE^2, ...
+    // tile-sum levels live in scratch (from scratch_off): n/TILE, n/TILE^2, ...
     int64_t levels_n[8];
     size_t levels_off[8];
-    int nl = 0;
     size_t bytes = 0;
-    int64_t m = n;
-    while (m > SCAN_TILE) {
-        if (nl >= 7) return sph_set_error(ctx, SPH_EINVAL, "scan too large");
-        int64_t nb = (m + SCAN_TILE - 1) / SCAN_TILE;
-        levels_n[nl] = nb;
-        levels_off[nl] = bytes;
-        bytes += (size_t)(nb + 1) * sizeof(T);
-        bytes = (bytes + 255) & ~(size_t)255;
-        ++nl;
-        m = nb;
-    }
-    int rc = sph_ensure_scratch(ctx, bytes + 256);
+    const int nl = scan_levels(n, sizeof(T), levels_n, levels_off, &bytes);
+    if (nl < 0) return sph_set_error(ctx, SPH_EINVAL, "scan too large");
+    scratch_off = (scratch_off + 255) & ~(size_t)255;
+    int rc = sph_ensure_scratch(ctx, scratch_off + bytes);
     if (rc) return rc;
-    char *base = (char *)ctx->scratch;
+    char *base = (char *)ctx->scratch + scratch_off;
     // up-sweep
     T *cur = data;
     int64_t cur_n = n;
@@ -247,12 +266,22 @@ static int scan_exclusive(sph_ctx *ctx, T *data, int64_t n)
     return SPH_OK;
 }
 
+int sph_scan_exclusive_i32_at(sph_ctx *ctx, int32_t *data, int64_t n, size_t scratch_off)
+{
+    return scan_exclusive<int32_t>(ctx, data, n, scratch_off);
+}
+
+int sph_scan_exclusive_i64_at(sph_ctx *ctx, int64_t *data, int64_t n, size_t scratch_off)
+{
+    return scan_exclusive<int64_t>(ctx, data, n, scratch_off);
+}
+
 int sph_scan_exclusive_i32(sph_ctx *ctx, int32_t *data, int64_t n)
 {
-    return scan_exclusive<int32_t>(ctx, data, n);
+    return scan_exclusive<int32_t>(ctx, data, n, 0);
 }
 
 int sph_scan_exclusive_i64(sph_ctx *ctx, int64_t *data, int64_t n)
 {
-    return scan_exclusive<int64_t>(ctx, data, n);
+    return scan_exclusive<int64_t>(ctx, data, n, 0);
 }

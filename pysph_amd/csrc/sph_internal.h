@@ -58,11 +58,28 @@ static inline unsigned sph_blocks(int64_t n, int threads)
     return (unsigned)(b > 0 ? b : 1);
 }
 
+// Blocks are dispatched round-robin over the 8 XCDs of an MI355X, each XCD
+// with its own L2.  Remap block b so that every XCD walks a contiguous range
+// of tiles (of the cell-sorted order): neighbouring particles then share
+// one L2.  Bijective on [0, nb).
+#define SPH_NXCD 8
+__device__ __forceinline__ unsigned sph_xcd_tile(unsigned b, unsigned nb)
+{
+    const unsigned q = nb / SPH_NXCD, r = nb % SPH_NXCD;
+    const unsigned x = b % SPH_NXCD, k = b / SPH_NXCD;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // Exclusive prefix sums (device-wide), stream-ordered.  `data` has n
 // elements; the scanned values are written in place and the total is
 // written to data[n] (so data must hold n+1 elements).
 int sph_scan_exclusive_i32(sph_ctx *ctx, int32_t *data, int64_t n);
 int sph_scan_exclusive_i64(sph_ctx *ctx, int64_t *data, int64_t n);
+// the same with the scan's partial sums placed in the context scratch from
+// byte offset scratch_off on (data itself may live below that offset)
+int sph_scan_exclusive_i32_at(sph_ctx *ctx, int32_t *data, int64_t n, size_t scratch_off);
+int sph_scan_exclusive_i64_at(sph_ctx *ctx, int64_t *data, int64_t n, size_t scratch_off);
+size_t sph_scan_scratch_bytes(int64_t n, size_t elem);
 
 // ------------------------------------------------------------------------
 // Cubic spline kernel in the reference's expression order.
@@ -100,6 +117,40 @@ __device__ __forceinline__ double sph_cubic_w(int dim, double rx, double ry, dou
         val = 0.25 * (2 - q) * (2 - q) * (2 - q);
     else
         val = 1 - 1.5 * (q * q) * (1 - 0.5 * q);
+    return val * fac;
+}
+
+// W with the factor h_dim(h)*sigma supplied by the caller (the same value
+// sph_cubic_w computes)
+__device__ __forceinline__ double sph_cubic_w_fac(double rx, double ry, double rz, double h,
+                                                  double fac)
+{
+    double rab = sqrt(rx * rx + ry * ry + rz * rz);
+    double q = rab / h;
+    double val;
+    if (q > 2.0)
+        val = 0.0;
+    else if (q > 1.0)
+        val = 0.25 * (2 - q) * (2 - q) * (2 - q);
+    else
+        val = 1 - 1.5 * (q * q) * (1 - 0.5 * q);
+    return val * fac;
+}
+
+// gradient factor with h_dim(h)*sigma supplied by the caller.  The two
+// branches of kernels.pyx:408-411 divide different numerators by different
+// denominators; selecting both first and dividing once gives the same
+// quotient with one division per pair instead of two divergent ones.
+__device__ __forceinline__ double sph_cubic_gfac_fac(double rx, double ry, double rz, double h,
+                                                     double fac)
+{
+    const double rab = sqrt(rx * rx + ry * ry + rz * rz);
+    const double q = rab / h;
+    const bool outer = q >= 1.0;
+    const double num = outer ? -0.75 * (2 - q) * (2 - q) : 3.0 * (0.75 * q - 1);
+    const double den = outer ? h * rab : h * h;
+    const double v = num / den;
+    const double val = (q > 2.0 || !(outer || q > 1e-14)) ? 0.0 : v;
     return val * fac;
 }
 

@@ -346,14 +346,17 @@ __global__ void __launch_bounds__(NN_THREADS)
 }
 
 // Restore ascending particle index inside every cell (the atomics above
-// hand out ranks in arbitrary order).  One thread per cell; cells hold a
-// handful of particles, insertion sort is the right tool.
+// hand out ranks in arbitrary order), then record the inverse permutation
+// and flag non-empty cells.  One thread per cell; cells hold a handful of
+// particles.
 __global__ void __launch_bounds__(NN_THREADS)
-    k_cell_sort(const int32_t *__restrict__ start, int64_t ncells, int32_t *__restrict__ perm)
+    k_cell_sort(const int32_t *__restrict__ start, int64_t ncells, int32_t *__restrict__ perm,
+                int32_t *__restrict__ rank, int32_t *__restrict__ nonempty)
 {
     int64_t c = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
     if (c >= ncells) return;
     int32_t s = start[c], e = start[c + 1];
+    nonempty[c] = e > s ? 1 : 0;
     for (int32_t a = s + 1; a < e; ++a) {
         int32_t v = perm[a];
         int32_t b = a - 1;
@@ -363,44 +366,52 @@ __global__ void __launch_bounds__(NN_THREADS)
         }
         perm[b + 1] = v;
     }
+    for (int32_t a = s; a < e; ++a) rank[perm[a]] = a;
 }
 
 __global__ void __launch_bounds__(NN_THREADS)
-    k_gather3(const int32_t *__restrict__ perm, int64_t n, const double *__restrict__ x,
-              const double *__restrict__ y, const double *__restrict__ z,
-              double *__restrict__ sx, double *__restrict__ sy, double *__restrict__ sz)
+    k_compact_cells(const int32_t *__restrict__ start, const int32_t *__restrict__ slot,
+                    int64_t ncells, int32_t *__restrict__ cells)
 {
-    int64_t p = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
-    if (p >= n) return;
-    int32_t i = perm[p];
-    sx[p] = x[i];
-    sy[p] = y[i];
-    sz[p] = z[i];
+    int64_t c = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
+    if (c >= ncells) return;
+    if (start[c + 1] > start[c]) cells[slot[c]] = (int32_t)c;
+}
+
+static int read_i64(sph_ctx *ctx, const void *dev, int64_t *out, bool is32)
+{
+    int rc = sph_ensure_pinned(ctx, sizeof(int64_t));
+    if (rc) return rc;
+    SPH_HIP_CHECK(ctx, hipMemcpyAsync(ctx->pinned, dev, is32 ? sizeof(int32_t) : sizeof(int64_t),
+                                      hipMemcpyDeviceToHost, ctx->stream));
+    SPH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    *out = is32 ? (int64_t) * (int32_t *)ctx->pinned : *(int64_t *)ctx->pinned;
+    return SPH_OK;
 }
 
 extern "C" int sph_bin_array(sph_ctx *ctx, const sph_grid *grid, const double *x,
                              const double *y, const double *z, int64_t n, int32_t *cell_start,
-                             int32_t *perm, double *sx, double *sy, double *sz,
-                             int32_t *cell_of)
+                             int32_t *perm, int32_t *rank, int32_t *cell_of, int32_t *cells,
+                             int64_t *ncells_out)
 {
     if (!ctx) return SPH_EINVAL;
-    SPH_REQUIRE(ctx, grid && grid->ncells > 0 && cell_start, "sph_bin_array: bad grid");
+    SPH_REQUIRE(ctx, grid && grid->ncells > 0 && cell_start && ncells_out,
+                "sph_bin_array: bad grid / null output");
     SPH_REQUIRE(ctx, n >= 0 && n < ((int64_t)1 << 31), "sph_bin_array: bad n %lld",
                 (long long)n);
-    SPH_REQUIRE(ctx, n == 0 || (x && y && z && perm && sx && sy && sz && cell_of),
+    SPH_REQUIRE(ctx, n == 0 || (x && y && z && perm && rank && cell_of && cells),
                 "sph_bin_array: null array");
     int rc = sph_upload_key_tables(ctx, grid);
     if (rc) return rc;
     GridDev g = make_griddev(ctx, grid);
     const int64_t nc = grid->ncells;
+    *ncells_out = 0;
     hipLaunchKernelGGL(k_fill_i32, dim3(sph_blocks(nc + 1, NN_THREADS)), dim3(NN_THREADS), 0,
                        ctx->stream, cell_start, nc + 1, 0);
     SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
     if (n == 0) return SPH_OK;
-    // the scan below uses the context scratch, so the per-particle rank is
-    // parked in sx (n doubles >= n int32) until the scatter has consumed it;
-    // k_gather3 then overwrites sx with the sorted x coordinates.
-    int32_t *rank = (int32_t *)sx;
+    // rank first holds the slot inside the cell handed out by the atomics;
+    // k_cell_sort overwrites it with the sorted position
     hipLaunchKernelGGL(k_bin_keys, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
                        ctx->stream, g, x, y, z, n, cell_start, cell_of, rank);
     SPH_LAUNCH_CHECK(ctx, "k_bin_keys");
@@ -409,157 +420,406 @@ extern "C" int sph_bin_array(sph_ctx *ctx, const sph_grid *grid, const double *x
     hipLaunchKernelGGL(k_bin_scatter, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
                        ctx->stream, cell_of, rank, n, cell_start, perm);
     SPH_LAUNCH_CHECK(ctx, "k_bin_scatter");
+    // non-empty flags -> slots (scan) -> compact list of cell keys
+    const size_t slot_bytes = (sizeof(int32_t) * (size_t)(nc + 1) + 511) & ~(size_t)255;
+    rc = sph_ensure_scratch(ctx, slot_bytes + sph_scan_scratch_bytes(nc, sizeof(int32_t)));
+    if (rc) return rc;
+    int32_t *slot = (int32_t *)ctx->scratch;
     hipLaunchKernelGGL(k_cell_sort, dim3(sph_blocks(nc, NN_THREADS)), dim3(NN_THREADS), 0,
-                       ctx->stream, cell_start, nc, perm);
+                       ctx->stream, cell_start, nc, perm, rank, slot);
     SPH_LAUNCH_CHECK(ctx, "k_cell_sort");
-    hipLaunchKernelGGL(k_gather3, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
-                       ctx->stream, perm, n, x, y, z, sx, sy, sz);
-    SPH_LAUNCH_CHECK(ctx, "k_gather3");
+    // the scan's own partial sums live past the slot array in the scratch
+    rc = sph_scan_exclusive_i32_at(ctx, slot, nc, slot_bytes);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_compact_cells, dim3(sph_blocks(nc, NN_THREADS)), dim3(NN_THREADS), 0,
+                       ctx->stream, cell_start, slot, nc, cells);
+    SPH_LAUNCH_CHECK(ctx, "k_compact_cells");
+    return read_i64(ctx, slot + nc, ncells_out, true);
+}
+
+__global__ void __launch_bounds__(NN_THREADS)
+    k_gather_sorted(int64_t n, const int32_t *__restrict__ perm, sph_prop_list pl)
+{
+    int64_t t = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
+    if (t >= n) return;
+    const int32_t i = perm[t];
+    const int64_t o = t * pl.dst_stride;
+    for (int k = 0; k < pl.nprops; ++k) pl.dst[k][o] = pl.src[k][i];
+}
+
+extern "C" int sph_gather_sorted(sph_ctx *ctx, int64_t n, const int32_t *perm,
+                                 const sph_prop_list *props)
+{
+    if (!ctx) return SPH_EINVAL;
+    SPH_REQUIRE(ctx, props && props->nprops >= 0 && props->nprops <= SPH_MAX_PROPS &&
+                         props->dst_stride >= 1,
+                "sph_gather_sorted: bad props");
+    if (n <= 0 || props->nprops == 0) return SPH_OK;
+    SPH_REQUIRE(ctx, perm, "sph_gather_sorted: null perm");
+    for (int k = 0; k < props->nprops; ++k)
+        SPH_REQUIRE(ctx, props->src[k] && props->dst[k], "sph_gather_sorted: null prop %d", k);
+    hipLaunchKernelGGL(k_gather_sorted, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
+                       ctx->stream, n, perm, *props);
+    SPH_LAUNCH_CHECK(ctx, "k_gather_sorted");
     return SPH_OK;
 }
 
 // ---------------------------------------------------------------------------
-// neighbour lists
+// neighbour lists (sorted space): one 64-lane wavefront per destination cell
 // ---------------------------------------------------------------------------
-struct NnpsArgs {
+#define CW 64          // lanes per wavefront = threads per block of the cell kernel
+#define MAXSEG 64      // stencil cells handled per batch (27 for d = 1)
+
+struct CellArgs {
     GridDev g;
-    const double *dx, *dy, *dz, *dh;
-    int64_t nd_real;
-    const int32_t *dperm;  // destination visiting order (may be null)
-    int64_t nd_visit;      // length of dperm (or nd_real)
-    const int32_t *start;
-    const int32_t *sperm;
-    const double *sx, *sy, *sz;
+    const double *drec;
+    const int32_t *dperm, *dstart, *dcells;
+    int64_t ncell_list, nd_real;
+    const int32_t *sstart;
+    const double *srec;
     double rs;
     int same;
 };
 
-// Visits the candidates of destination i in the reference order and calls
-// f(sorted_pos) for each neighbour.
-template <typename F>
-__device__ __forceinline__ void for_each_neighbour(const NnpsArgs &a, int64_t i, F f)
+__device__ __forceinline__ int wave_max_i(int v)
 {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, 64));
+    return v;
+}
+
+__device__ __forceinline__ int wave_incl_scan_i(int v)
+{
+    const int lane = __lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// broadcast lane `l` (wave-uniform) of a 64-bit value to the wavefront
+__device__ __forceinline__ double wave_bcast(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ int64_t wave_bcast_i64(int64_t v, int l)
+{
+    const int lo = __builtin_amdgcn_readlane((int)(v & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(v >> 32), l);
+    return ((int64_t)hi << 32) | (uint32_t)lo;
+}
+
+template <bool FILL>
+__global__ void __launch_bounds__(CW)
+    k_nnps_cells(CellArgs a, int32_t *__restrict__ counts, const int64_t *__restrict__ slice_off,
+                 int32_t *__restrict__ indices)
+{
+    __shared__ int32_t seg_start[MAXSEG];
+    __shared__ int32_t seg_end[MAXSEG];  // inclusive prefix of the segment sizes
+    const int lane = threadIdx.x;
+    const int64_t w = sph_xcd_tile(blockIdx.x, gridDim.x);
+    if (w >= a.ncell_list) return;  // uniform: one wavefront per block
     const GridDev &g = a.g;
-    const double px = a.dx[i], py = a.dy[i], pz = a.dz[i];
-    const double radius = a.dh[i] * a.rs;
-    const double radius2 = radius * radius;
-    const int distance = (int)ceil(radius / g.cs);
-    const int cx = cell_coord(px, g.cs) - g.cmin0;
-    const int cy = cell_coord(py, g.cs) - g.cmin1;
-    const int cz = cell_coord(pz, g.cs) - g.cmin2;
-    const int x0 = max(cx - distance, 0), x1 = min(cx + distance, g.d0 - 1);
-    const int y0 = max(cy - distance, 0), y1 = min(cy + distance, g.d1 - 1);
-    const int z0 = max(cz - distance, 0), z1 = min(cz + distance, g.d2 - 1);
+    const int32_t key = a.dcells[w];
+    const int32_t ds = a.dstart[key], de = a.dstart[key + 1];
+    // the cell's coordinates, recomputed exactly as the binning did
+    const double *r0 = a.drec + (int64_t)ds * SPH_REC;
+    const int cx = min(max(cell_coord(r0[SPH_R_X], g.cs) - g.cmin0, 0), g.d0 - 1);
+    const int cy = min(max(cell_coord(r0[SPH_R_Y], g.cs) - g.cmin1, 0), g.d1 - 1);
+    const int cz = min(max(cell_coord(r0[SPH_R_Z], g.cs) - g.cmin2, 0), g.d2 - 1);
+    // stencil reach: the largest (int)ceil(h*rs/cell) over the cell; a
+    // destination with a smaller reach only sees extra candidates that fail
+    // its distance test, and its own stencil is a sub-box visited in order
+    int dl = 0;
+    for (int32_t t = ds + lane; t < de; t += CW)
+        dl = max(dl, (int)ceil(a.drec[(int64_t)t * SPH_REC + SPH_R_H] * a.rs / g.cs));
+    const int dist = wave_max_i(dl);
+    const int x0 = max(cx - dist, 0), x1 = min(cx + dist, g.d0 - 1);
+    const int y0 = max(cy - dist, 0), y1 = min(cy + dist, g.d1 - 1);
+    const int z0 = max(cz - dist, 0), z1 = min(cz + dist, g.d2 - 1);
+    const int nx = x1 - x0 + 1, ny = y1 - y0 + 1, nz = z1 - z0 + 1;
+    const int nst = nx * ny * nz;
     const int32_t *tx = g.tab, *ty = g.tab + g.d0, *tz = g.tab + g.d0 + g.d1;
-    for (int X = x0; X <= x1; ++X) {
-        const int32_t kx = tx[X];
-        for (int Y = y0; Y <= y1; ++Y) {
-            const int32_t kxy = kx + ty[Y];
-            for (int Z = z0; Z <= z1; ++Z) {
-                const int32_t key = kxy + tz[Z];
-                const int32_t s = a.start[key], e = a.start[key + 1];
-                for (int32_t p = s; p < e; ++p) {
-                    const double ex = a.sx[p] - px, ey = a.sy[p] - py, ez = a.sz[p] - pz;
-                    const double d2 = ex * ex + ey * ey + ez * ez;
-                    if (d2 < radius2) f(p);
+
+    for (int32_t g0 = ds; g0 < de; g0 += CW) {  // destination groups of <= 64
+        const int gn = min(CW, de - g0);
+        const int32_t my_t = g0 + lane;  // destination owned by this lane
+        // this lane's destination, broadcast to the wavefront in the loop below
+        double mpx = 0.0, mpy = 0.0, mpz = 0.0, mr2 = -1.0;
+        bool mreal = false;
+        if (lane < gn) {
+            const double *dr = a.drec + (int64_t)my_t * SPH_REC;
+            mpx = dr[SPH_R_X];
+            mpy = dr[SPH_R_Y];
+            mpz = dr[SPH_R_Z];
+            const double radius = dr[SPH_R_H] * a.rs;
+            mr2 = radius * radius;
+            mreal = a.dperm[my_t] < a.nd_real;
+        }
+        const uint64_t realmask = __ballot(mreal);
+        // SELL-64: entry k of row t lives at slice_off[t/64] + 64*k + t%64
+        int64_t myaddr = 0;    // FILL: address of entry 0 of my row
+        int32_t mycount = 0;   // entries found so far (COUNT and FILL)
+        int32_t myself = -1;   // FILL: entry holding my own index
+        int32_t mylast = -1;   // FILL: last neighbour written to my row
+        if (FILL && lane < gn) myaddr = slice_off[my_t >> 6] + (my_t & 63);
+        for (int b0 = 0; b0 < nst; b0 += MAXSEG) {  // stencil batches
+            const int sIdx = b0 + lane;
+            int32_t st = 0, cnt = 0;
+            if (lane < MAXSEG && sIdx < nst) {
+                const int iz = sIdx % nz, iy = (sIdx / nz) % ny, ix = sIdx / (nz * ny);
+                const int32_t k = tx[x0 + ix] + ty[y0 + iy] + tz[z0 + iz];
+                st = a.sstart[k];
+                cnt = a.sstart[k + 1] - st;
+            }
+            const int incl = wave_incl_scan_i(cnt);
+            const int total = __shfl(incl, CW - 1, 64);
+            __syncthreads();
+            if (lane < MAXSEG) {
+                seg_start[lane] = st;
+                seg_end[lane] = incl;
+            }
+            __syncthreads();
+            for (int c0 = 0; c0 < total; c0 += CW) {
+                const int q = c0 + lane;
+                const bool valid = q < total;
+                int32_t p = 0;
+                double sx = 0.0, sy = 0.0, sz = 0.0;
+                if (valid) {
+                    int lo = 0, hi = MAXSEG - 1;  // first segment with end > q
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (seg_end[mid] > q) hi = mid; else lo = mid + 1;
+                    }
+                    const int prev = lo > 0 ? seg_end[lo - 1] : 0;
+                    p = seg_start[lo] + (q - prev);
+                    const double *sr = a.srec + (int64_t)p * SPH_REC;
+                    sx = sr[SPH_R_X];
+                    sy = sr[SPH_R_Y];
+                    sz = sr[SPH_R_Z];
                 }
+                for (int d = 0; d < gn; ++d) {
+                    if (!((realmask >> d) & 1)) continue;  // non-LocalReal: empty row
+                    const int32_t t = g0 + d;
+                    const double px = wave_bcast(mpx, d), py = wave_bcast(mpy, d),
+                                 pz = wave_bcast(mpz, d), radius2 = wave_bcast(mr2, d);
+                    // cPoint_distance2(src, dst), base/point.pxd:35-37
+                    const double ex = sx - px, ey = sy - py, ez = sz - pz;
+                    const double d2 = ex * ex + ey * ey + ez * ez;
+                    const bool hit = valid && d2 < radius2;
+                    const uint64_t mask = __ballot(hit);
+                    if (mask == 0) continue;
+                    const int nh = __popcll(mask);
+                    if (!FILL) {
+                        if (lane == d) mycount += nh;
+                    } else {
+                        const int64_t addr0 = wave_bcast_i64(myaddr, d);
+                        const int32_t k0 = __builtin_amdgcn_readlane(mycount, d);
+                        const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+                        if (hit)  // the slice's lines fill up in L2 from several cells
+                            indices[addr0 + ((int64_t)(k0 + __popcll(mask & below)) << 6)] = p;
+                        const int lastlane = 63 - __clzll(mask);
+                        const int32_t lastp = __builtin_amdgcn_readlane(p, lastlane);
+                        const uint64_t selfm = __ballot(a.same && hit && p == t);
+                        if (lane == d) {
+                            if (selfm) {
+                                const int sl = __ffsll((long long)selfm) - 1;
+                                const uint64_t bsl = sl == 0 ? 0ull : (~0ull >> (64 - sl));
+                                myself = k0 + __popcll(mask & bsl);
+                            }
+                            mycount = k0 + nh;
+                            mylast = lastp;
+                        }
+                    }
+                }
+            }
+        }
+        if (lane < gn) {
+            if (!FILL) {
+                counts[my_t] = mycount;
+            } else if (a.same && myself >= 0 && myself < mycount - 1) {
+                // keep the particle itself at the last index (nnps.pyx:612-619)
+                indices[myaddr + ((int64_t)myself << 6)] = mylast;
+                indices[myaddr + ((int64_t)(mycount - 1) << 6)] = my_t;
             }
         }
     }
 }
 
-__global__ void __launch_bounds__(NN_THREADS) k_nnps_count(NnpsArgs a, int64_t *__restrict__ counts)
+// slice s of 64 destinations: storage 64 * max(count) and its entry total
+__global__ void __launch_bounds__(NN_THREADS)
+    k_slices(const int32_t *__restrict__ counts, int64_t nd_total, int64_t nslices,
+             int64_t *__restrict__ slice_w, int64_t *__restrict__ slice_sum)
 {
-    int64_t t = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
-    if (t >= a.nd_visit) return;
-    int64_t i = a.dperm ? (int64_t)a.dperm[t] : t;
-    if (i >= a.nd_real) return;
-    int64_t c = 0;
-    for_each_neighbour(a, i, [&](int32_t) { ++c; });
-    counts[i] = c;
+    const int64_t t = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
+    const int c = t < nd_total ? counts[t] : 0;
+    int m = c, sum = c;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        m = max(m, __shfl_xor(m, d, 64));
+        sum += __shfl_xor(sum, d, 64);
+    }
+    const int64_t s = t >> 6;
+    if ((threadIdx.x & 63) == 0 && s < nslices) {
+        slice_w[s] = 64 * (int64_t)m;
+        slice_sum[s] = sum;
+    }
+}
+
+// total of n int64 values by one block (deterministic tree order)
+__global__ void __launch_bounds__(1024)
+    k_sum_i64(const int64_t *__restrict__ v, int64_t n, int64_t *__restrict__ out)
+{
+    __shared__ int64_t part[1024 / 64];
+    int64_t s = 0;
+    for (int64_t i = threadIdx.x; i < n; i += 1024) s += v[i];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t tot = 0;
+        for (int w = 0; w < 1024 / 64; ++w) tot += part[w];
+        *out = tot;
+    }
+}
+
+extern "C" int sph_nnps_build(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
+                              const int32_t *d_perm, const int32_t *d_cell_start,
+                              const int32_t *d_cells, int64_t nd_cells, int64_t nd_total,
+                              int64_t nd_real, const int32_t *s_cell_start, const double *s_rec,
+                              int64_t ns, double radius_scale, int same_array, int32_t *counts,
+                              int64_t *slice_off, int32_t *indices, int64_t cap,
+                              int64_t *slots_out, int64_t *nnz_out)
+{
+    if (!ctx) return SPH_EINVAL;
+    SPH_REQUIRE(ctx, grid && counts && slice_off && slots_out && nnz_out,
+                "sph_nnps_build: null argument");
+    SPH_REQUIRE(ctx, nd_total >= 0 && nd_total < ((int64_t)1 << 31) && nd_real >= 0 &&
+                         nd_real <= nd_total && nd_cells >= 0 && nd_cells <= nd_total,
+                "sph_nnps_build: bad destination counts");
+    SPH_REQUIRE(ctx, ns == 0 || (s_cell_start && s_rec), "sph_nnps_build: null source arrays");
+    SPH_REQUIRE(ctx, nd_total == 0 || (d_rec && d_perm && d_cell_start && d_cells),
+                "sph_nnps_build: null destination arrays");
+    int rc = sph_upload_key_tables(ctx, grid);
+    if (rc) return rc;
+    *slots_out = 0;
+    *nnz_out = 0;
+    const int64_t nslices = (nd_total + 63) / 64;
+    // every row starts empty: non-real rows and cells without candidates stay 0
+    hipLaunchKernelGGL(k_fill_i32, dim3(sph_blocks(nd_total + 1, NN_THREADS)), dim3(NN_THREADS),
+                       0, ctx->stream, counts, nd_total, 0);
+    SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
+    hipLaunchKernelGGL(k_fill_i32, dim3(sph_blocks(2 * (nslices + 1), NN_THREADS)),
+                       dim3(NN_THREADS), 0, ctx->stream, (int32_t *)slice_off,
+                       2 * (nslices + 1), 0);
+    SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
+    if (nd_total == 0 || ns == 0 || nd_real == 0 || nd_cells == 0) return SPH_OK;
+    CellArgs a;
+    a.g = make_griddev(ctx, grid);
+    a.drec = d_rec;
+    a.dperm = d_perm;
+    a.dstart = d_cell_start;
+    a.dcells = d_cells;
+    a.ncell_list = nd_cells;
+    a.nd_real = nd_real;
+    a.sstart = s_cell_start;
+    a.srec = s_rec;
+    a.rs = radius_scale;
+    a.same = same_array;
+    hipLaunchKernelGGL(k_nnps_cells<false>, dim3((unsigned)nd_cells), dim3(CW), 0, ctx->stream, a,
+                       counts, (const int64_t *)nullptr, (int32_t *)nullptr);
+    SPH_LAUNCH_CHECK(ctx, "k_nnps_cells<count>");
+    // slice widths -> offsets (scan); slice totals -> entry total
+    const size_t sums_bytes = ((size_t)(nslices + 1) * sizeof(int64_t) + 511) & ~(size_t)255;
+    rc = sph_ensure_scratch(ctx, sums_bytes + sph_scan_scratch_bytes(nslices, sizeof(int64_t)));
+    if (rc) return rc;
+    int64_t *slice_sum = (int64_t *)ctx->scratch;
+    int64_t *nnz_dev = slice_sum + nslices;
+    hipLaunchKernelGGL(k_slices, dim3(sph_blocks(nslices * 64, NN_THREADS)), dim3(NN_THREADS), 0,
+                       ctx->stream, counts, nd_total, nslices, slice_off, slice_sum);
+    SPH_LAUNCH_CHECK(ctx, "k_slices");
+    hipLaunchKernelGGL(k_sum_i64, dim3(1), dim3(1024), 0, ctx->stream, slice_sum, nslices, nnz_dev);
+    SPH_LAUNCH_CHECK(ctx, "k_sum_i64");
+    rc = sph_scan_exclusive_i64_at(ctx, slice_off, nslices, sums_bytes);
+    if (rc) return rc;
+    rc = sph_ensure_pinned(ctx, 2 * sizeof(int64_t));
+    if (rc) return rc;
+    SPH_HIP_CHECK(ctx, hipMemcpyAsync(ctx->pinned, slice_off + nslices, sizeof(int64_t),
+                                      hipMemcpyDeviceToHost, ctx->stream));
+    SPH_HIP_CHECK(ctx, hipMemcpyAsync((char *)ctx->pinned + sizeof(int64_t), nnz_dev,
+                                      sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    SPH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const int64_t slots = ((int64_t *)ctx->pinned)[0];
+    *slots_out = slots;
+    *nnz_out = ((int64_t *)ctx->pinned)[1];
+    if (slots > cap || !indices) {
+        sph_set_error(ctx, SPH_ENEEDCAP, "neighbour capacity %lld < %lld", (long long)cap,
+                      (long long)slots);
+        return SPH_ENEEDCAP;
+    }
+    hipLaunchKernelGGL(k_nnps_cells<true>, dim3((unsigned)nd_cells), dim3(CW), 0, ctx->stream, a,
+                       counts, (const int64_t *)slice_off, indices);
+    SPH_LAUNCH_CHECK(ctx, "k_nnps_cells<fill>");
+    return SPH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// export to original index space
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(NN_THREADS)
+    k_export_counts(int64_t nd_real, const int32_t *__restrict__ rank,
+                    const int32_t *__restrict__ counts, int64_t *__restrict__ out_counts)
+{
+    int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
+    if (i >= nd_real) return;
+    out_counts[i] = counts[rank[i]];
 }
 
 __global__ void __launch_bounds__(NN_THREADS)
-    k_nnps_fill(NnpsArgs a, const int64_t *__restrict__ offsets, int32_t *__restrict__ indices)
+    k_export_rows(int64_t nd_real, const int32_t *__restrict__ rank,
+                  const int32_t *__restrict__ s_perm, const int32_t *__restrict__ counts,
+                  const int64_t *__restrict__ slice_off, const int32_t *__restrict__ indices,
+                  const int64_t *__restrict__ out_offsets, int32_t *__restrict__ out_indices)
 {
-    int64_t t = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
-    if (t >= a.nd_visit) return;
-    int64_t i = a.dperm ? (int64_t)a.dperm[t] : t;
-    if (i >= a.nd_real) return;
-    int32_t *row = indices + offsets[i];
-    int64_t c = 0, selfpos = -1;
-    const int32_t me = (int32_t)i;
-    for_each_neighbour(a, i, [&](int32_t p) {
-        const int32_t j = a.sperm[p];
-        if (selfpos < 0 && j == me) selfpos = c;
-        row[c++] = j;
-    });
-    // keep the particle itself at the last index (nnps.pyx:612-619)
-    if (a.same && selfpos >= 0 && selfpos < c - 1) {
-        row[selfpos] = row[c - 1];
-        row[c - 1] = me;
-    }
+    int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
+    if (i >= nd_real) return;
+    const int32_t t = rank[i];
+    const int32_t c = counts[t];
+    const int32_t *src = indices + slice_off[t >> 6] + (t & 63);
+    int32_t *dst = out_indices + out_offsets[i];
+    for (int32_t k = 0; k < c; ++k) dst[k] = s_perm[src[(int64_t)k << 6]];
 }
 
-extern "C" int sph_nnps_build(sph_ctx *ctx, const sph_grid *grid, const double *dx,
-                              const double *dy, const double *dz, const double *dh,
-                              int64_t nd_real, const int32_t *src_cell_start,
-                              const int32_t *src_perm, const double *src_sx,
-                              const double *src_sy, const double *src_sz, int64_t ns,
-                              double radius_scale, int same_array, int64_t *offsets,
-                              int32_t *indices, int64_t cap, int64_t *nnz_out)
+extern "C" int sph_nnps_export(sph_ctx *ctx, int64_t nd_real, const int32_t *d_rank,
+                               const int32_t *s_perm, const int32_t *counts,
+                               const int64_t *slice_off, const int32_t *indices,
+                               int64_t *out_offsets, int32_t *out_indices)
 {
     if (!ctx) return SPH_EINVAL;
-    SPH_REQUIRE(ctx, grid && offsets && nnz_out, "sph_nnps_build: null argument");
-    SPH_REQUIRE(ctx, nd_real >= 0 && nd_real < ((int64_t)1 << 31), "sph_nnps_build: bad nd");
-    SPH_REQUIRE(ctx, ns == 0 || (src_cell_start && src_perm && src_sx && src_sy && src_sz),
-                "sph_nnps_build: null source arrays");
-    SPH_REQUIRE(ctx, nd_real == 0 || (dx && dy && dz && dh), "sph_nnps_build: null dest");
-    int rc = sph_upload_key_tables(ctx, grid);
-    if (rc) return rc;
-    *nnz_out = 0;
-    if (nd_real == 0 || ns == 0) {
-        hipLaunchKernelGGL(k_fill_i32, dim3(sph_blocks(2 * (nd_real + 1), NN_THREADS)),
-                           dim3(NN_THREADS), 0, ctx->stream, (int32_t *)offsets,
-                           2 * (nd_real + 1), 0);
+    SPH_REQUIRE(ctx, nd_real >= 0 && out_offsets, "sph_nnps_export: bad arguments");
+    if (nd_real == 0) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(NN_THREADS), 0, ctx->stream,
+                           (int32_t *)out_offsets, 2, 0);
         SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
         return SPH_OK;
     }
-    NnpsArgs a;
-    a.g = make_griddev(ctx, grid);
-    a.dx = dx;
-    a.dy = dy;
-    a.dz = dz;
-    a.dh = dh;
-    a.nd_real = nd_real;
-    a.dperm = nullptr;
-    a.nd_visit = nd_real;
-    a.start = src_cell_start;
-    a.sperm = src_perm;
-    a.sx = src_sx;
-    a.sy = src_sy;
-    a.sz = src_sz;
-    a.rs = radius_scale;
-    a.same = same_array;
-    hipLaunchKernelGGL(k_nnps_count, dim3(sph_blocks(a.nd_visit, NN_THREADS)), dim3(NN_THREADS), 0,
-                       ctx->stream, a, offsets);
-    SPH_LAUNCH_CHECK(ctx, "k_nnps_count");
-    rc = sph_scan_exclusive_i64(ctx, offsets, nd_real);
+    SPH_REQUIRE(ctx, d_rank && s_perm && counts && slice_off && indices && out_indices,
+                "sph_nnps_export: null array");
+    hipLaunchKernelGGL(k_export_counts, dim3(sph_blocks(nd_real, NN_THREADS)), dim3(NN_THREADS),
+                       0, ctx->stream, nd_real, d_rank, counts, out_offsets);
+    SPH_LAUNCH_CHECK(ctx, "k_export_counts");
+    int rc = sph_scan_exclusive_i64(ctx, out_offsets, nd_real);
     if (rc) return rc;
-    rc = sph_ensure_pinned(ctx, sizeof(int64_t));
-    if (rc) return rc;
-    SPH_HIP_CHECK(ctx, hipMemcpyAsync(ctx->pinned, offsets + nd_real, sizeof(int64_t),
-                                      hipMemcpyDeviceToHost, ctx->stream));
-    SPH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    int64_t nnz = *(int64_t *)ctx->pinned;
-    *nnz_out = nnz;
-    if (nnz > cap || !indices) {
-        sph_set_error(ctx, SPH_ENEEDCAP, "neighbour capacity %lld < %lld", (long long)cap,
-                      (long long)nnz);
-        return SPH_ENEEDCAP;
-    }
-    hipLaunchKernelGGL(k_nnps_fill, dim3(sph_blocks(a.nd_visit, NN_THREADS)), dim3(NN_THREADS), 0,
-                       ctx->stream, a, (const int64_t *)offsets, indices);
-    SPH_LAUNCH_CHECK(ctx, "k_nnps_fill");
+    hipLaunchKernelGGL(k_export_rows, dim3(sph_blocks(nd_real, NN_THREADS)), dim3(NN_THREADS), 0,
+                       ctx->stream, nd_real, d_rank, s_perm, counts, slice_off, indices,
+                       out_offsets, out_indices);
+    SPH_LAUNCH_CHECK(ctx, "k_export_rows");
     return SPH_OK;
 }

@@ -9,53 +9,105 @@
 
 #define OP_THREADS 256
 
+// Sorted records: fields SPH_R_* of SPH_REC doubles, 16-byte aligned; the
+// neighbour loops fetch them as double2 pairs.
+__device__ __forceinline__ const double2 *rec2(const double *rec, int64_t j)
+{
+    return reinterpret_cast<const double2 *>(rec + j * SPH_REC);
+}
+
 // ---------------------------------------------------------------------------
 // SPHRho
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(OP_THREADS)
-    k_summation_density(int dim, int64_t nd_real, int64_t nd_total,
-                        const int64_t *__restrict__ offsets, const int32_t *__restrict__ indices,
-                        const double *__restrict__ dx, const double *__restrict__ dy,
-                        const double *__restrict__ dz, const double *__restrict__ dh,
-                        const double *__restrict__ sx, const double *__restrict__ sy,
-                        const double *__restrict__ sz, const double *__restrict__ sh,
-                        const double *__restrict__ sm, int accumulate, double *__restrict__ rho)
+// one neighbour term of SPHRho (density_funcs.pyx:41-73)
+__device__ __forceinline__ double rho_term(int dim, double xa, double ya, double za, double ha,
+                                           double facA, double2 sxy, double2 szh, double2 smr)
 {
-    int64_t i = (int64_t)blockIdx.x * OP_THREADS + threadIdx.x;
-    if (i >= nd_total) return;
-    if (i >= nd_real) {  // non-LocalReal destinations (sph_func.pyx:226-227)
-        rho[i] = 0;
-        return;
+    const double hb = szh.y;
+    // with h_b == h_a, 0.5*(h_a+h_b) == h_a exactly and h_dim(h)*sigma is the
+    // destination's value: reuse it instead of dividing again
+    double hab, fac;
+    if (hb == ha) {
+        hab = ha;
+        fac = facA;
+    } else {
+        hab = 0.5 * (ha + hb);
+        fac = sph_h_dim(hab, dim) * sph_cubic_fac1(dim);
     }
-    const double xa = dx[i], ya = dy[i], za = dz[i], ha = dh[i];
-    double nr = 0.0;
-    const int64_t e = offsets[i + 1];
-    for (int64_t k = offsets[i]; k < e; ++k) {
-        const int32_t j = indices[k];
-        const double hb = sh[j];
-        const double hab = 0.5 * (ha + hb);
-        const double w = sph_cubic_w(dim, xa - sx[j], ya - sy[j], za - sz[j], hab);
-        nr += w * sm[j];
-    }
-    rho[i] = (accumulate ? rho[i] : 0.0) + nr;
+    return sph_cubic_w_fac(xa - sxy.x, ya - sxy.y, za - szh.x, hab, fac) * smr.x;
 }
 
-extern "C" int sph_summation_density(sph_ctx *ctx, int dim, int64_t nd_real, int64_t nd_total,
-                                     const int64_t *offsets, const int32_t *indices,
-                                     const double *dx, const double *dy, const double *dz,
-                                     const double *dh, const double *sx, const double *sy,
-                                     const double *sz, const double *sh, const double *sm,
-                                     int accumulate, double *rho)
+__global__ void __launch_bounds__(OP_THREADS)
+    k_summation_density(int dim, int64_t nd_total, int64_t nd_real,
+                        const int32_t *__restrict__ dperm, const int32_t *__restrict__ counts,
+                        const int64_t *__restrict__ slice_off,
+                        const int32_t *__restrict__ indices, const double *__restrict__ drec,
+                        const double *__restrict__ srec, int accumulate,
+                        double *__restrict__ rho, double *__restrict__ out_s, int64_t out_stride)
+{
+    const int64_t t =
+        (int64_t)sph_xcd_tile(blockIdx.x, gridDim.x) * OP_THREADS + threadIdx.x;
+    if (t >= nd_total) return;
+    const int32_t i = dperm[t];
+    if (i >= nd_real) {  // non-LocalReal destinations (sph_func.pyx:226-227)
+        rho[i] = 0;
+        if (out_s) out_s[t * out_stride] = 0;
+        return;
+    }
+    const double2 *dr = rec2(drec, t);
+    const double2 dxy = dr[0], dzh = dr[1];
+    const double xa = dxy.x, ya = dxy.y, za = dzh.x, ha = dzh.y;
+    const double facA = sph_h_dim(ha, dim) * sph_cubic_fac1(dim);
+    double nr = 0.0;
+    // SELL-64 row: entry k at row[64*k]; each load is one coalesced line for
+    // the wavefront and is read once, so it bypasses the caches
+    const int32_t *row = indices + slice_off[t >> 6] + (t & 63);
+    const int32_t e = counts[t];
+    int32_t k = 0;
+    // four neighbours' loads in flight, then their terms added in list order
+    for (; k + 4 <= e; k += 4) {
+        const int32_t j0 = __builtin_nontemporal_load(row + ((int64_t)k << 6)),
+                      j1 = __builtin_nontemporal_load(row + ((int64_t)(k + 1) << 6)),
+                      j2 = __builtin_nontemporal_load(row + ((int64_t)(k + 2) << 6)),
+                      j3 = __builtin_nontemporal_load(row + ((int64_t)(k + 3) << 6));
+        const double2 *s0 = rec2(srec, j0), *s1 = rec2(srec, j1), *s2 = rec2(srec, j2),
+                      *s3 = rec2(srec, j3);
+        const double2 a0 = s0[0], b0 = s0[1], c0 = s0[2];
+        const double2 a1 = s1[0], b1 = s1[1], c1 = s1[2];
+        const double2 a2 = s2[0], b2 = s2[1], c2 = s2[2];
+        const double2 a3 = s3[0], b3 = s3[1], c3 = s3[2];
+        nr += rho_term(dim, xa, ya, za, ha, facA, a0, b0, c0);
+        nr += rho_term(dim, xa, ya, za, ha, facA, a1, b1, c1);
+        nr += rho_term(dim, xa, ya, za, ha, facA, a2, b2, c2);
+        nr += rho_term(dim, xa, ya, za, ha, facA, a3, b3, c3);
+    }
+    for (; k < e; ++k) {
+        const double2 *sr = rec2(srec, __builtin_nontemporal_load(row + ((int64_t)k << 6)));
+        nr += rho_term(dim, xa, ya, za, ha, facA, sr[0], sr[1], sr[2]);
+    }
+    const double base = accumulate ? (out_s ? out_s[t * out_stride] : rho[i]) : 0.0;
+    const double val = base + nr;
+    rho[i] = val;
+    if (out_s) out_s[t * out_stride] = val;
+}
+
+extern "C" int sph_summation_density(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                                     const int32_t *d_perm, const int32_t *counts,
+                                     const int64_t *slice_off, const int32_t *indices,
+                                     const double *d_rec,
+                                     const double *s_rec, int accumulate, double *rho,
+                                     double *out_s, int64_t out_s_stride)
 {
     if (!ctx) return SPH_EINVAL;
     SPH_REQUIRE(ctx, dim >= 1 && dim <= 3, "sph_summation_density: bad dim %d", dim);
     SPH_REQUIRE(ctx, nd_real >= 0 && nd_total >= nd_real, "sph_summation_density: bad n");
     if (nd_total == 0) return SPH_OK;
-    SPH_REQUIRE(ctx, rho && (nd_real == 0 || (offsets && dx && dy && dz && dh)),
+    SPH_REQUIRE(ctx, rho && d_perm && counts && slice_off && d_rec &&
+                         (!out_s || out_s_stride >= 1),
                 "sph_summation_density: null argument");
     hipLaunchKernelGGL(k_summation_density, dim3(sph_blocks(nd_total, OP_THREADS)),
-                       dim3(OP_THREADS), 0, ctx->stream, dim, nd_real, nd_total, offsets,
-                       indices, dx, dy, dz, dh, sx, sy, sz, sh, sm, accumulate, rho);
+                       dim3(OP_THREADS), 0, ctx->stream, dim, nd_total, nd_real, d_perm, counts,
+                       slice_off, indices, d_rec, s_rec, accumulate, rho, out_s, out_s_stride);
     SPH_LAUNCH_CHECK(ctx, "k_summation_density");
     return SPH_OK;
 }
@@ -64,38 +116,49 @@ extern "C" int sph_summation_density(sph_ctx *ctx, int dim, int64_t nd_real, int
 // Tait equation of state
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(OP_THREADS)
-    k_tait(int64_t n_real, int64_t n_total, const double *__restrict__ rho, double ro,
-           double co, double gamma, double B, int accumulate, double *__restrict__ p,
-           double *__restrict__ cs)
+    k_tait(int64_t n_total, int64_t n_real, const int32_t *__restrict__ perm,
+           const double *__restrict__ rho_s, int64_t st, double ro, double co, double gamma,
+           double B, int accumulate, double *__restrict__ p, double *__restrict__ cs,
+           double *__restrict__ p_s, double *__restrict__ cs_s)
 {
-    int64_t i = (int64_t)blockIdx.x * OP_THREADS + threadIdx.x;
-    if (i >= n_total) return;
-    if (i >= n_real) {
-        p[i] = 0;
-        cs[i] = 0;
-        return;
+    int64_t t = (int64_t)blockIdx.x * OP_THREADS + threadIdx.x;
+    if (t >= n_total) return;
+    const int32_t i = perm[t];
+    double r0 = 0.0, r1 = 0.0;
+    if (i < n_real) {
+        const double rhoa = rho_s[t * st];
+        const double ratio = rhoa / ro;
+        const double gamma2 = 0.5 * (gamma - 1.0);
+        const double tmp = pow(ratio, gamma);
+        r0 = (tmp - 1.0) * B;
+        r1 = pow(ratio, gamma2) * co;
+        if (accumulate) {
+            r0 = (p_s ? p_s[t * st] : p[i]) + r0;
+            r1 = (cs_s ? cs_s[t * st] : cs[i]) + r1;
+        } else {
+            r0 = 0.0 + r0;
+            r1 = 0.0 + r1;
+        }
     }
-    const double rhoa = rho[i];
-    const double ratio = rhoa / ro;
-    const double gamma2 = 0.5 * (gamma - 1.0);
-    const double tmp = pow(ratio, gamma);
-    const double r0 = (tmp - 1.0) * B;
-    const double r1 = pow(ratio, gamma2) * co;
-    p[i] = (accumulate ? p[i] : 0.0) + r0;
-    cs[i] = (accumulate ? cs[i] : 0.0) + r1;
+    p[i] = r0;
+    cs[i] = r1;
+    if (p_s) p_s[t * st] = r0;
+    if (cs_s) cs_s[t * st] = r1;
 }
 
-extern "C" int sph_tait(sph_ctx *ctx, int64_t n_real, int64_t n_total, const double *rho,
-                        double ro, double co, double gamma, int accumulate, double *p,
-                        double *cs)
+extern "C" int sph_tait(sph_ctx *ctx, int64_t n_total, int64_t n_real, const int32_t *perm,
+                        const double *rho_s, int64_t s_stride, double ro, double co,
+                        double gamma, int accumulate, double *p, double *cs, double *p_s,
+                        double *cs_s)
 {
     if (!ctx) return SPH_EINVAL;
     SPH_REQUIRE(ctx, n_real >= 0 && n_total >= n_real, "sph_tait: bad n");
     if (n_total == 0) return SPH_OK;
-    SPH_REQUIRE(ctx, rho && p && cs, "sph_tait: null argument");
+    SPH_REQUIRE(ctx, perm && rho_s && p && cs && s_stride >= 1, "sph_tait: null argument");
     const double B = co * co * ro / gamma;  // eos_funcs.pyx:90
     hipLaunchKernelGGL(k_tait, dim3(sph_blocks(n_total, OP_THREADS)), dim3(OP_THREADS), 0,
-                       ctx->stream, n_real, n_total, rho, ro, co, gamma, B, accumulate, p, cs);
+                       ctx->stream, n_total, n_real, perm, rho_s, s_stride, ro, co, gamma, B,
+                       accumulate, p, cs, p_s, cs_s);
     SPH_LAUNCH_CHECK(ctx, "k_tait");
     return SPH_OK;
 }
@@ -103,61 +166,108 @@ extern "C" int sph_tait(sph_ctx *ctx, int64_t n_real, int64_t n_total, const dou
 // ---------------------------------------------------------------------------
 // MomentumEquation with Monaghan artificial viscosity
 // ---------------------------------------------------------------------------
+struct MomDest {
+    double xa, ya, za, ha, facA, rhoa, ca, Pa, ua, va, wa, rhoa2;
+};
+
+// one neighbour term of MomentumEquation (pressure_funcs.pyx:252-391)
+__device__ __forceinline__ void mom_term(int dim, const MomDest &D, double alpha, double beta,
+                                         double eta, const double2 s0, const double2 s1,
+                                         const double2 s2, const double2 s3, const double2 s4,
+                                         const double2 s5, double &dtf, double &nr0, double &nr1,
+                                         double &nr2)
+{
+    const double hb = s1.y;
+    double hab, fac;
+    if (hb == D.ha) {  // 0.5*(h_a+h_a) == h_a: reuse the destination's h_dim(h)*sigma
+        hab = D.ha;
+        fac = D.facA;
+    } else {
+        hab = 0.5 * (D.ha + hb);
+        fac = sph_h_dim(hab, dim) * sph_cubic_fac1(dim);
+    }
+    const double cb = s3.x;
+    const double rabx = D.xa - s0.x;
+    const double raby = D.ya - s0.y;
+    const double rabz = D.za - s1.x;
+    const double vabx = D.ua - s4.x;
+    const double vaby = D.va - s4.y;
+    const double vabz = D.wa - s5.x;
+    const double dot = vabx * rabx + vaby * raby + vabz * rabz;
+    const double rab2 = rabx * rabx + raby * raby + rabz * rabz;
+    const double den = rab2 + eta * eta * hab * hab;
+    const double q = hab * dot / den;  // the same quotient as mu below
+    const double f = fabs(q);
+    if (f > dtf) dtf = f;
+    const double rhob = s2.y, mb = s2.x;
+    // Pb * (1.0/(rhob*rhob)), computed once per source particle (k_prho2)
+    double tmp = D.Pa * D.rhoa2 + s5.y;
+    double piab = 0;
+    if (dot < 0) {
+        const double cab = 0.5 * (D.ca + cb);
+        const double rhoab = 0.5 * (D.rhoa + rhob);
+        const double mu = q;  // mu = hab*dot; mu /= (rab2 + eta*eta*hab*hab)
+        piab = -alpha * cab * mu + beta * mu * mu;
+        piab /= rhoab;
+    }
+    tmp += piab;
+    tmp *= -mb;
+    const double g = sph_cubic_gfac_fac(rabx, raby, rabz, hab, fac);
+    nr0 += tmp * (rabx * g);
+    nr1 += tmp * (raby * g);
+    nr2 += tmp * (rabz * g);
+}
+
 __global__ void __launch_bounds__(OP_THREADS)
-    k_momentum(int dim, int64_t nd_real, int64_t nd_total, const int64_t *__restrict__ offsets,
-               const int32_t *__restrict__ indices, sph_mom_dst d, sph_mom_src s, double alpha,
+    k_momentum(int dim, int64_t nd_total, int64_t nd_real, const int32_t *__restrict__ dperm,
+               const int32_t *__restrict__ counts, const int64_t *__restrict__ slice_off,
+               const int32_t *__restrict__ indices,
+               const double *__restrict__ drec, const double *__restrict__ srec, double alpha,
                double beta, double eta, int accumulate, double *__restrict__ ax,
                double *__restrict__ ay, double *__restrict__ az, double *__restrict__ dt_fac)
 {
-    int64_t i = (int64_t)blockIdx.x * OP_THREADS + threadIdx.x;
-    if (i >= nd_total) return;
+    const int64_t t =
+        (int64_t)sph_xcd_tile(blockIdx.x, gridDim.x) * OP_THREADS + threadIdx.x;
+    if (t >= nd_total) return;
+    const int32_t i = dperm[t];
     if (i >= nd_real) {
         ax[i] = 0;
         ay[i] = 0;
         if (az) az[i] = 0;
         return;
     }
-    const double xa = d.x[i], ya = d.y[i], za = d.z[i], ha = d.h[i];
-    const double ua = d.u[i], va = d.v[i], wa = d.w[i];
-    const double ca = d.cs[i], Pa = d.p[i], rhoa = d.rho[i];
-    const double rhoa2 = 1.0 / (rhoa * rhoa);
+    const double2 *dr = rec2(drec, t);
+    const double2 d0 = dr[0], d1 = dr[1], d2 = dr[2], d3 = dr[3], d4 = dr[4], d5 = dr[5];
+    MomDest D;
+    D.xa = d0.x;
+    D.ya = d0.y;
+    D.za = d1.x;
+    D.ha = d1.y;
+    D.facA = sph_h_dim(D.ha, dim) * sph_cubic_fac1(dim);
+    D.rhoa = d2.y;
+    D.ca = d3.x;
+    D.Pa = d3.y;
+    D.ua = d4.x;
+    D.va = d4.y;
+    D.wa = d5.x;
+    D.rhoa2 = 1.0 / (D.rhoa * D.rhoa);
     double dtf = dt_fac[i];
     double nr0 = 0.0, nr1 = 0.0, nr2 = 0.0;
-    const int64_t e = offsets[i + 1];
-    for (int64_t k = offsets[i]; k < e; ++k) {
-        const int32_t j = indices[k];
-        const double hb = s.h[j];
-        const double hab = 0.5 * (ha + hb);
-        const double cb = s.cs[j];
-        const double rabx = xa - s.x[j];
-        const double raby = ya - s.y[j];
-        const double rabz = za - s.z[j];
-        const double vabx = ua - s.u[j];
-        const double vaby = va - s.v[j];
-        const double vabz = wa - s.w[j];
-        const double dot = vabx * rabx + vaby * raby + vabz * rabz;
-        const double rab2 = rabx * rabx + raby * raby + rabz * rabz;
-        const double den = rab2 + eta * eta * hab * hab;
-        const double q = hab * dot / den;  // the same quotient as mu below
-        const double f = fabs(q);
-        if (f > dtf) dtf = f;
-        const double Pb = s.p[j], rhob = s.rho[j], mb = s.m[j];
-        const double rhob2 = 1.0 / (rhob * rhob);
-        double tmp = Pa * rhoa2 + Pb * rhob2;
-        double piab = 0;
-        if (dot < 0) {
-            const double cab = 0.5 * (ca + cb);
-            const double rhoab = 0.5 * (rhoa + rhob);
-            const double mu = q;  // mu = hab*dot; mu /= (rab2 + eta*eta*hab*hab)
-            piab = -alpha * cab * mu + beta * mu * mu;
-            piab /= rhoab;
-        }
-        tmp += piab;
-        tmp *= -mb;
-        const double g = sph_cubic_gfac(dim, rabx, raby, rabz, hab);
-        nr0 += tmp * (rabx * g);
-        nr1 += tmp * (raby * g);
-        nr2 += tmp * (rabz * g);
+    const int32_t *row = indices + slice_off[t >> 6] + (t & 63);  // SELL-64 row
+    const int32_t e = counts[t];
+    int32_t k = 0;
+    for (; k + 2 <= e; k += 2) {  // two neighbours' loads in flight
+        const double2 *p = rec2(srec, __builtin_nontemporal_load(row + ((int64_t)k << 6))),
+                      *q = rec2(srec, __builtin_nontemporal_load(row + ((int64_t)(k + 1) << 6)));
+        const double2 p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3], p4 = p[4], p5 = p[5];
+        const double2 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5];
+        mom_term(dim, D, alpha, beta, eta, p0, p1, p2, p3, p4, p5, dtf, nr0, nr1, nr2);
+        mom_term(dim, D, alpha, beta, eta, q0, q1, q2, q3, q4, q5, dtf, nr0, nr1, nr2);
+    }
+    if (k < e) {
+        const double2 *p = rec2(srec, __builtin_nontemporal_load(row + ((int64_t)k << 6)));
+        mom_term(dim, D, alpha, beta, eta, p[0], p[1], p[2], p[3], p[4], p[5], dtf, nr0, nr1,
+                 nr2);
     }
     dt_fac[i] = dtf;
     ax[i] = (accumulate ? ax[i] : 0.0) + nr0;
@@ -165,21 +275,78 @@ __global__ void __launch_bounds__(OP_THREADS)
     if (az) az[i] = (accumulate ? az[i] : 0.0) + nr2;
 }
 
-extern "C" int sph_momentum_av(sph_ctx *ctx, int dim, int64_t nd_real, int64_t nd_total,
-                               const int64_t *offsets, const int32_t *indices,
-                               const sph_mom_dst *dst, const sph_mom_src *src, double alpha,
-                               double beta, double eta, int accumulate, double *ax, double *ay,
-                               double *az, double *dt_fac)
+// P/rho^2 of every source record, the per-pair term of pressure_funcs.pyx:302-305
+__global__ void __launch_bounds__(OP_THREADS) k_prho2(int64_t n, double *__restrict__ rec)
+{
+    const int64_t j = (int64_t)blockIdx.x * OP_THREADS + threadIdx.x;
+    if (j >= n) return;
+    double *r = rec + j * SPH_REC;
+    const double rhob = r[SPH_R_RHO];
+    const double rhob2 = 1.0 / (rhob * rhob);
+    r[SPH_R_PRHO2] = r[SPH_R_P] * rhob2;
+}
+
+extern "C" int sph_momentum_av(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                               const int32_t *d_perm, const int32_t *counts,
+                               const int64_t *slice_off, const int32_t *indices,
+                               const double *d_rec, double *s_rec,
+                               int64_t ns, double alpha, double beta, double eta, int accumulate,
+                               double *ax, double *ay, double *az, double *dt_fac)
 {
     if (!ctx) return SPH_EINVAL;
     SPH_REQUIRE(ctx, dim >= 1 && dim <= 3, "sph_momentum_av: bad dim %d", dim);
     SPH_REQUIRE(ctx, nd_real >= 0 && nd_total >= nd_real, "sph_momentum_av: bad n");
     if (nd_total == 0) return SPH_OK;
-    SPH_REQUIRE(ctx, dst && src && ax && ay && dt_fac, "sph_momentum_av: null argument");
+    SPH_REQUIRE(ctx, d_rec && ax && ay && dt_fac && d_perm && counts && slice_off && ns >= 0 &&
+                         (ns == 0 || s_rec),
+                "sph_momentum_av: null argument");
+    if (ns > 0) {
+        hipLaunchKernelGGL(k_prho2, dim3(sph_blocks(ns, OP_THREADS)), dim3(OP_THREADS), 0,
+                           ctx->stream, ns, s_rec);
+        SPH_LAUNCH_CHECK(ctx, "k_prho2");
+    }
     hipLaunchKernelGGL(k_momentum, dim3(sph_blocks(nd_total, OP_THREADS)), dim3(OP_THREADS), 0,
-                       ctx->stream, dim, nd_real, nd_total, offsets, indices, *dst, *src, alpha,
-                       beta, eta, accumulate, ax, ay, az, dt_fac);
+                       ctx->stream, dim, nd_total, nd_real, d_perm, counts, slice_off, indices,
+                       d_rec, s_rec, alpha, beta, eta, accumulate, ax, ay, az, dt_fac);
     SPH_LAUNCH_CHECK(ctx, "k_momentum");
+    return SPH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// the cubic spline itself
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(OP_THREADS)
+    k_cubic_spline(int dim, int64_t n, const double *__restrict__ ax,
+                   const double *__restrict__ ay, const double *__restrict__ az,
+                   const double *__restrict__ bx, const double *__restrict__ by,
+                   const double *__restrict__ bz, const double *__restrict__ h,
+                   double *__restrict__ w, double *__restrict__ gx, double *__restrict__ gy,
+                   double *__restrict__ gz)
+{
+    int64_t i = (int64_t)blockIdx.x * OP_THREADS + threadIdx.x;
+    if (i >= n) return;
+    const double rx = ax[i] - bx[i], ry = ay[i] - by[i], rz = az[i] - bz[i];
+    if (w) w[i] = sph_cubic_w(dim, rx, ry, rz, h[i]);
+    if (gx || gy || gz) {
+        const double g = sph_cubic_gfac(dim, rx, ry, rz, h[i]);
+        if (gx) gx[i] = rx * g;
+        if (gy) gy[i] = ry * g;
+        if (gz) gz[i] = rz * g;
+    }
+}
+
+extern "C" int sph_cubic_spline(sph_ctx *ctx, int dim, int64_t n, const double *ax,
+                                const double *ay, const double *az, const double *bx,
+                                const double *by, const double *bz, const double *h, double *w,
+                                double *gx, double *gy, double *gz)
+{
+    if (!ctx) return SPH_EINVAL;
+    SPH_REQUIRE(ctx, dim >= 1 && dim <= 3, "sph_cubic_spline: bad dim %d", dim);
+    if (n <= 0) return SPH_OK;
+    SPH_REQUIRE(ctx, ax && ay && az && bx && by && bz && h, "sph_cubic_spline: null input");
+    hipLaunchKernelGGL(k_cubic_spline, dim3(sph_blocks(n, OP_THREADS)), dim3(OP_THREADS), 0,
+                       ctx->stream, dim, n, ax, ay, az, bx, by, bz, h, w, gx, gy, gz);
+    SPH_LAUNCH_CHECK(ctx, "k_cubic_spline");
     return SPH_OK;
 }
 
@@ -270,7 +437,7 @@ __global__ void __launch_bounds__(OP_THREADS) k_pc_final(int64_t n, sph_step_pro
 extern "C" int sph_stage_step(sph_ctx *ctx, int64_t n, const sph_step_props *props, double dt)
 {
     if (!ctx) return SPH_EINVAL;
-    SPH_REQUIRE(ctx, props && props->nprops >= 0 && props->nprops <= SPH_MAX_STEP_PROPS,
+    SPH_REQUIRE(ctx, props && props->nprops >= 0 && props->nprops <= SPH_MAX_PROPS,
                 "sph_stage_step: bad props");
     if (n <= 0 || props->nprops == 0) return SPH_OK;
     hipLaunchKernelGGL(k_stage_step, dim3(sph_blocks(n, OP_THREADS)), dim3(OP_THREADS), 0,
@@ -282,7 +449,7 @@ extern "C" int sph_stage_step(sph_ctx *ctx, int64_t n, const sph_step_props *pro
 extern "C" int sph_pc_final(sph_ctx *ctx, int64_t n, const sph_step_props *props)
 {
     if (!ctx) return SPH_EINVAL;
-    SPH_REQUIRE(ctx, props && props->nprops >= 0 && props->nprops <= SPH_MAX_STEP_PROPS,
+    SPH_REQUIRE(ctx, props && props->nprops >= 0 && props->nprops <= SPH_MAX_PROPS,
                 "sph_pc_final: bad props");
     if (n <= 0 || props->nprops == 0) return SPH_OK;
     hipLaunchKernelGGL(k_pc_final, dim3(sph_blocks(n, OP_THREADS)), dim3(OP_THREADS), 0,

@@ -60,8 +60,9 @@ class Integrator:
         for array in self.arrays:
             n = array.get_number_of_particles()
             for prop in self.step_props[array.name][step]:
-                acc = array.get_carray(self.step_props[array.name][step][prop][1])
-                ctx.check(ctx.lib.sph_fill(ctx.handle, n, 0.0, _lib.ptr(acc)))
+                name = self.step_props[array.name][step][prop][1]
+                ctx.check(ctx.lib.sph_fill(ctx.handle, n, 0.0, _lib.ptr(array.get_carray(name))))
+                array.touch(name)
 
     def save_initial_arrays(self):
         ctx = self._ctx()
@@ -70,6 +71,7 @@ class Integrator:
             for src, dst in self.initial_properties[array.name].items():
                 ctx.check(ctx.lib.sph_copy(ctx.handle, n, _lib.ptr(array.get_carray(src)),
                                            _lib.ptr(array.get_carray(dst))))
+                array.touch(dst)
 
     def eval(self):
         k_num = self.cstep
@@ -102,10 +104,11 @@ class Integrator:
             if not to_step:
                 continue
             triples = [(prop, v[0], v[1]) for prop, v in to_step.items()]
-            for c in range(0, len(triples), _lib.MAX_STEP_PROPS):
-                sp = self._props_struct(array, triples[c:c + _lib.MAX_STEP_PROPS])
+            for c in range(0, len(triples), _lib.MAX_PROPS):
+                sp = self._props_struct(array, triples[c:c + _lib.MAX_PROPS])
                 ctx.check(ctx.lib.sph_stage_step(ctx.handle, array.get_number_of_particles(),
                                                  ctypes.byref(sp), float(dt)))
+            array.touch(*to_step)
             for prop, v in to_step.items():
                 if prop in ('u', 'v', 'w'):
                     self.velocity_accelerations[array.name][k_num][v[1]] = array.get_carray(v[1])
@@ -184,10 +187,11 @@ class PredictorCorrectorIntegrator(Integrator):
             if not to_step:
                 continue
             triples = [(prop, v[0], None) for prop, v in to_step.items()]
-            for c in range(0, len(triples), _lib.MAX_STEP_PROPS):
-                sp = self._props_struct(array, triples[c:c + _lib.MAX_STEP_PROPS])
+            for c in range(0, len(triples), _lib.MAX_PROPS):
+                sp = self._props_struct(array, triples[c:c + _lib.MAX_PROPS])
                 ctx.check(ctx.lib.sph_pc_final(ctx.handle, array.get_number_of_particles(),
                                                ctypes.byref(sp)))
+            array.touch(*to_step)
             if any(p in ('x', 'y', 'z') for p in to_step):
                 array.set_dirty(True)
 

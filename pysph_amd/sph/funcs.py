@@ -5,11 +5,28 @@ TaitEquation      sph/funcs/eos_funcs.pyx:62-129
 MomentumEquation  sph/funcs/pressure_funcs.pyx:174-391 (hks=False, deltap=None)
 GravityForce      sph/funcs/external_force.pyx:13-44
 PositionStepping  sph/funcs/position_funcs.pyx:10-43
-"""
-import ctypes
 
-from .. import _lib
+The neighbour functions read the sorted shadows of their arrays (see
+base/nnps.py) and the sorted-space lists of their locator; results land in
+the caller's particle order and, for shadowed properties, in the shadow.
+"""
+import torch
+
+from .. import _lib, profiling
+from ..base.nnps import REC, REC_FIELDS
 from .sph_func import CSPHFunctionParticle, SPHFunction, SPHFunctionParticle
+
+_identity = {}
+
+
+def _identity_perm(n, device):
+    """Sorted order == original order (functions used without a CellManager)."""
+    key = (n, str(device))
+    t = _identity.get(key)
+    if t is None:
+        t = torch.arange(max(n, 1), dtype=torch.int32, device=device)
+        _identity[key] = t
+    return t
 
 
 class SPHRho(CSPHFunctionParticle):
@@ -27,16 +44,26 @@ class SPHRho(CSPHFunctionParticle):
         self.src_reads = ['x', 'y', 'z', 'h', 'm']
         self.dst_reads = ['x', 'y', 'z', 'h', 'tag']
 
-    def eval(self, kernel, output1, output2, output3, accumulate=True):
+    def eval(self, kernel, output1, output2, output3, accumulate=True, names=None):
         ctx = self._ctx()
-        off, idx = self.csr()
-        nd_real = self.dest.num_real_particles
-        nd = self.dest.get_number_of_particles()
+        loc = self.nbr_locator
+        cm = loc.cell_manager
+        cnt, soff, idx = loc.get_sorted_lists()
+        name = names[0] if names else None
+        read = ('x', 'y', 'z', 'h') + ((name,) if name in REC_FIELDS and accumulate else ())
+        drec = cm.shadows(self.dest, read)
+        srec = cm.shadows(self.source, ('x', 'y', 'z', 'h', 'm'))
+        out_s = cm.shadow_out(self.dest, name) if name else None
         p = _lib.ptr
-        ctx.check(ctx.lib.sph_summation_density(
-            ctx.handle, kernel.dim, nd_real, nd, p(off), p(idx), p(self.d('x')), p(self.d('y')),
-            p(self.d('z')), p(self.d('h')), p(self.s('x')), p(self.s('y')), p(self.s('z')),
-            p(self.s('h')), p(self.s('m')), int(bool(accumulate)), p(output1)))
+        profiling.count_pairs(loc.nnz)
+        with profiling.region("summation_density"):
+            ctx.check(ctx.lib.sph_summation_density(
+                ctx.handle, kernel.dim, self.dest.get_number_of_particles(),
+                self.dest.num_real_particles, p(cm.bins[self.dest.name].perm), p(cnt), p(soff),
+                p(idx),
+                p(drec), p(srec), int(bool(accumulate)), p(output1), out_s, REC))
+        if name:
+            cm.written(self.dest, name, out_s is not None)
 
 
 class TaitEquation(SPHFunction):
@@ -58,13 +85,36 @@ class TaitEquation(SPHFunction):
         self.src_reads = []
         self.dst_reads = ['rho']
 
-    def eval(self, kernel, output1, output2, output3, accumulate=True):
+    def eval(self, kernel, output1, output2, output3, accumulate=True, names=None):
         ctx = self._ctx()
         p = _lib.ptr
-        ctx.check(ctx.lib.sph_tait(ctx.handle, self.dest.num_real_particles,
-                                   self.dest.get_number_of_particles(), p(self.d('rho')),
-                                   self.ro, self.co, self.gamma, int(bool(accumulate)),
-                                   p(output1), p(output2)))
+        n = self.dest.get_number_of_particles()
+        cm = getattr(self, "cell_manager", None)
+        n0 = names[0] if names else None
+        n1 = names[1] if names else None
+        p_s = cs_s = None
+        if cm is not None and self.dest.name in cm.array_indices:
+            read = ('rho',) + tuple(k for k in (n0, n1) if accumulate and k in REC_FIELDS)
+            rec = cm.shadows(self.dest, read)
+            rho_s, stride = rec.data_ptr() + 8 * REC_FIELDS['rho'], REC
+            perm = cm.bins[self.dest.name].perm
+            p_s = cm.shadow_out(self.dest, n0) if n0 else None
+            cs_s = cm.shadow_out(self.dest, n1) if n1 else None
+        else:
+            cm = None
+            rho_s, stride = p(self.d('rho')), 1
+            perm = _identity_perm(n, self.dest.device)
+        with profiling.region("tait"):
+            ctx.check(ctx.lib.sph_tait(ctx.handle, n, self.dest.num_real_particles, p(perm),
+                                       rho_s, stride, self.ro, self.co, self.gamma,
+                                       int(bool(accumulate)), p(output1), p(output2), p_s, cs_s))
+        if cm is not None:
+            if n0:
+                cm.written(self.dest, n0, p_s is not None)
+            if n1:
+                cm.written(self.dest, n1, cs_s is not None)
+        else:
+            self.dest.touch()
 
 
 class MomentumEquation(SPHFunctionParticle):
@@ -72,6 +122,8 @@ class MomentumEquation(SPHFunctionParticle):
     viscous time-step factor dt_fac (pressure_funcs.pyx:246-391)."""
 
     writes = (True, True, True)
+    DST = ('x', 'y', 'z', 'h', 'rho', 'p', 'u', 'v', 'w', 'cs')
+    SRC = ('x', 'y', 'z', 'h', 'm', 'rho', 'p', 'u', 'v', 'w', 'cs')
 
     def __init__(self, source, dest, setup_arrays=True, alpha=1, beta=1, gamma=1.4, eta=0.1,
                  deltap=None, n=1, eps=0.2, **kwargs):
@@ -91,8 +143,8 @@ class MomentumEquation(SPHFunctionParticle):
         self.to_reset = ['dt_fac']
 
     def set_src_dst_reads(self):
-        self.src_reads = ['x', 'y', 'z', 'h', 'm', 'rho', 'p', 'u', 'v', 'w', 'cs']
-        self.dst_reads = ['x', 'y', 'z', 'h', 'rho', 'p', 'u', 'v', 'w', 'cs', 'tag']
+        self.src_reads = list(self.SRC)
+        self.dst_reads = list(self.DST) + ['tag']
 
     def setup_arrays(self):
         if self.dest.get_carray('dt_fac') is None:
@@ -100,19 +152,26 @@ class MomentumEquation(SPHFunctionParticle):
         if 'dt_fac' not in self.dst_reads:
             self.dst_reads.append('dt_fac')
 
-    def eval(self, kernel, output1, output2, output3, accumulate=True):
+    def eval(self, kernel, output1, output2, output3, accumulate=True, names=None):
         ctx = self._ctx()
-        off, idx = self.csr()
+        loc = self.nbr_locator
+        cm = loc.cell_manager
+        cnt, soff, idx = loc.get_sorted_lists()
+        drec = cm.shadows(self.dest, self.DST)
+        srec = cm.shadows(self.source, self.SRC)
         p = _lib.ptr
-        dst = _lib.MomDst(*[p(self.d(k)) for k in ('x', 'y', 'z', 'h', 'rho', 'p', 'u', 'v', 'w',
-                                                  'cs')])
-        src = _lib.MomSrc(*[p(self.s(k)) for k in ('x', 'y', 'z', 'h', 'm', 'rho', 'p', 'u', 'v',
-                                                  'w', 'cs')])
-        ctx.check(ctx.lib.sph_momentum_av(
-            ctx.handle, kernel.dim, self.dest.num_real_particles,
-            self.dest.get_number_of_particles(), p(off), p(idx), ctypes.byref(dst),
-            ctypes.byref(src), self.alpha, self.beta, self.eta, int(bool(accumulate)),
-            p(output1), p(output2), p(output3), p(self.d('dt_fac'))))
+        profiling.count_pairs(loc.nnz)
+        with profiling.region("momentum_av"):
+            ctx.check(ctx.lib.sph_momentum_av(
+                ctx.handle, kernel.dim, self.dest.get_number_of_particles(),
+                self.dest.num_real_particles, p(cm.bins[self.dest.name].perm), p(cnt), p(soff),
+                p(idx),
+                p(drec), p(srec), cm.bins[self.source.name].n, self.alpha, self.beta, self.eta,
+                int(bool(accumulate)), p(output1), p(output2), p(output3), p(self.d('dt_fac'))))
+        for name in (names or ()):
+            if name:
+                cm.written(self.dest, name, False)
+        cm.written(self.dest, 'dt_fac', False)
 
 
 class GravityForce(SPHFunction):
@@ -128,13 +187,14 @@ class GravityForce(SPHFunction):
         self.gy = float(gy)
         self.gz = float(gz)
 
-    def eval(self, kernel, output1, output2, output3, accumulate=True):
+    def eval(self, kernel, output1, output2, output3, accumulate=True, names=None):
         ctx = self._ctx()
         p = _lib.ptr
         ctx.check(ctx.lib.sph_gravity(ctx.handle, self.dest.num_real_particles,
                                       self.dest.get_number_of_particles(), self.gx, self.gy,
                                       self.gz, int(bool(accumulate)), p(output1), p(output2),
                                       p(output3)))
+        self.dest.touch(*[n for n in (names or ()) if n])
 
 
 class PositionStepping(SPHFunction):
@@ -151,9 +211,10 @@ class PositionStepping(SPHFunction):
         self.dst_reads = ['u', 'v', 'w', 'tag']
         self.src_reads = []
 
-    def eval(self, kernel, output1, output2, output3, accumulate=True):
+    def eval(self, kernel, output1, output2, output3, accumulate=True, names=None):
         ctx = self._ctx()
         p = _lib.ptr
         ctx.check(ctx.lib.sph_position_stepping(
             ctx.handle, self.dest.num_real_particles, self.dest.get_number_of_particles(),
             p(self.d('u')), p(self.d('v')), p(self.d('w')), p(output1), p(output2), p(output3)))
+        self.dest.touch(*[n for n in (names or ()) if n])

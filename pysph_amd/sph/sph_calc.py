@@ -46,6 +46,8 @@ class SPHCalc:
         self.src_reads = func.src_reads
         self.dst_reads = func.dst_reads
         self.nnps_manager = self.particles.nnps_manager
+        for f in self.funcs:
+            f.cell_manager = self.particles.cell_manager
         self.nbr_locators = []
         if self.nbr_info:
             for src, f in zip(self.sources, self.funcs):
@@ -60,20 +62,21 @@ class SPHCalc:
 
     def _outputs(self, names):
         defaults = ('_tmpx', '_tmpy', '_tmpz')
-        outs = []
+        outs, resolved = [], []
         for k in range(3):
             name = names[k] if k < len(names) and names[k] is not None else defaults[k]
             arr = self.dest.get_carray(name)
             if arr is None:
                 raise AttributeError(f"property {name} not present in {self.dest.name}")
             outs.append(arr)
-        return outs
+            resolved.append(name)
+        return outs, tuple(resolved)
 
     def sph(self, output_array1=None, output_array2=None, output_array3=None,
             exclude_self=False):
         if exclude_self:
             raise NotImplementedError("exclude_self is not part of this path")
-        outs = self._outputs((output_array1, output_array2, output_array3))
+        outs, names = self._outputs((output_array1, output_array2, output_array3))
         ctx = _lib.context(self.dest.device)
         n = self.dest.get_number_of_particles()
         accumulate = True
@@ -84,18 +87,22 @@ class SPHCalc:
             for k in range(3):
                 if not first.writes[k]:
                     ctx.check(ctx.lib.sph_fill(ctx.handle, n, 0.0, _lib.ptr(outs[k])))
+                    self.dest.touch(names[k])
             accumulate = False
         for prop in self.to_reset:
             ctx.check(ctx.lib.sph_fill(ctx.handle, n, 0.0, _lib.ptr(self.dest.get_carray(prop))))
+            self.dest.touch(prop)
         for f in self.funcs:
-            f.eval(self.kernel, outs[0], outs[1], outs[2], accumulate=accumulate)
+            f.eval(self.kernel, outs[0], outs[1], outs[2], accumulate=accumulate, names=names)
             accumulate = True
         if self.dest.is_dirty:
             self.particles.update()
 
     def sph_array(self, output1, output2, output3, exclude_self=False):
+        """Accumulate every function into the given output tensors."""
         for f in self.funcs:
             f.eval(self.kernel, output1, output2, output3, accumulate=True)
+        self.dest.touch()
 
 
 HIPCalc = SPHCalc

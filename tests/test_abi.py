@@ -1,0 +1,74 @@
+"""The C-ABI library loads and exports every entry point include/sphhip.h
+declares (CPU only: no compute call)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?\w+\s*\*?\s*(sph_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    for must in ("sph_ctx_create", "sph_nnps_build", "sph_summation_density", "sph_tait",
+                 "sph_momentum_av", "sph_stage_step", "sph_last_error"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from pysph_amd import _lib
+    lib_path = _lib.LIB_PATH
+    if not os.path.exists(lib_path):
+        pytest.skip("libsphhip.so not built")
+    lib = ctypes.CDLL(lib_path)
+    missing = [n for n in sorted(_declared()) if not hasattr(lib, n)]
+    assert not missing, missing
+    # every exported sph_ symbol is declared, and the ctypes table covers all
+    out = subprocess.run(["nm", "-D", "--defined-only", lib_path], capture_output=True,
+                         text=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T sph_" in l}
+    assert exported == _declared()
+    assert set(_lib.EXPORTED) == _declared()
+
+
+def test_binding_loads_without_device():
+    from pysph_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libsphhip.so not built")
+    lib = _lib.load_library()
+    assert b"gfx950" in lib.sph_version()
+    # error paths that need no device
+    assert lib.sph_ctx_create(None, 0, None) == _lib.SPH_EINVAL
+    assert lib.sph_last_error(None) == b"null sph_ctx"
+
+
+def test_gfx950_code_object():
+    from pysph_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libsphhip.so not built")
+    data = open(_lib.LIB_PATH, "rb").read()
+    assert b"gfx950" in data
+
+
+def test_product_path_has_no_oracle_dependency():
+    """pysph_amd must not import, load or call the oracle."""
+    pkg = os.path.join(ROOT, "pysph_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for fn in files:
+            if fn.endswith((".py", ".hip", ".h", ".cpp")):
+                text = open(os.path.join(dirpath, fn)).read()
+                assert "oracle" not in text and "sphref" not in text, fn

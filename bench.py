@@ -27,6 +27,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
@@ -39,13 +41,36 @@ def _args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("dambreak", "lattice"), default="dambreak")
+    ap.add_argument("--workload", choices=("dambreak", "lattice", "batch"), default="dambreak")
+    ap.add_argument("--nsims", type=int, default=256, help="drops of --workload batch")
+    ap.add_argument("--graph-steps", type=int, default=10,
+                    help="RK2 steps per captured graph (--workload batch)")
+    ap.add_argument("--seq-sims", type=int, default=0,
+                    help="drops timed one by one on the plain path for the sequential "
+                         "comparison of --workload batch (0: all)")
     ap.add_argument("--dx", type=float, default=0.01)
     ap.add_argument("--n", type=int, default=100, help="lattice edge for --workload lattice")
     ap.add_argument("--cell-order", choices=("morton", "linear"), default="morton")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="launch one k_copy of this many MiB before the timed steps (a kernel "
+                         "of known traffic that scales hardware byte counters)")
     return ap.parse_args()
+
+
+def copy_calibration(mib):
+    """One sph_copy of `mib` MiB (reads and writes mib MiB each)."""
+    import torch
+    from pysph_amd import _lib
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    ctx = _lib.context(a.device)
+    torch.cuda.synchronize()
+    ctx.check(ctx.lib.sph_copy(ctx.handle, n, _lib.ptr(a), _lib.ptr(b)))
+    torch.cuda.synchronize()
+    del a, b
 
 
 def momentum_bytes(nd, nnz, ns_other):
@@ -74,6 +99,8 @@ def run_dambreak(args):
 
     for _ in range(args.warmup):
         integ.integrate(dt)
+    if args.copy_calibration_mb:
+        copy_calibration(args.copy_calibration_mb)
     torch.cuda.synchronize()
     profiling.reset()
     profiling.enable(True)
@@ -164,7 +191,8 @@ def run_lattice(args):
     nnz = func.nbr_locator.nnz
     elapsed = t1 - t0
     dens = prof["summation_density"]
-    b = n * (8 + 4 * 8 + 8) + 4 * nnz  # offsets, x y z h m... see DESIGN
+    # per destination: row offset, x y z h m read once, rho written; 4 B per entry
+    b = n * (8 + 5 * 8 + 8) + 4 * nnz
     achieved = b / (dens["mean_ms"] * 1e-3) / 1e9
     return {
         "metric": "pair-interactions/s",
@@ -180,6 +208,84 @@ def run_lattice(args):
                      "traffic": None, "algorithmic_bytes_per_launch": b,
                      "mean_launch_ms": dens["mean_ms"]},
     }, d
+
+
+def run_batch(args):
+    """256 independent 2D elliptical drops (radius U[0.4, 2], dx from
+    {0.01, 0.02, 0.04}, velocity scale U[50, 150], seeded), RK2, dt = 1e-5,
+    advanced together by captured HIP graphs of --graph-steps steps; compared
+    with running the same drops one after another through the plain path."""
+    import torch
+    from pysph_amd.batch import BatchedSimulation, batched_drops
+    from pysph_amd.solver.configs import drop_solver
+    t_start = time.perf_counter()
+
+    def progress(msg):
+        print(f"[batch {time.perf_counter() - t_start:7.2f}s] {msg}", file=sys.stderr, flush=True)
+
+    dt = 1e-5
+    sims, prm = batched_drops(nsims=args.nsims)
+    progress(f"{len(sims)} drops generated, {sum(len(d['x']) for d in sims)} particles")
+    b = BatchedSimulation(sims)
+    progress(f"packed: key_cap {b.key_cap} max_cells {b.max_cells} list_cap {b.list_cap} "
+             f"workspace {b.work.numel() / 2**20:.0f} MiB")
+    b.step(dt)
+    b.check()
+    progress("eager step done")
+    gs = max(1, args.graph_steps)
+    b.capture(dt, nsteps=gs)
+    progress(f"graph of {gs} steps captured")
+    nrep_w = max(1, args.warmup // gs)
+    for _ in range(nrep_w):
+        b.replay()
+    b.check()
+    progress("warm-up replays done")
+    pairs0 = b.pair_interactions()
+    nrep = max(1, args.steps // gs)
+    t0 = time.perf_counter()
+    for _ in range(nrep):
+        b.replay()
+    b.synchronize()
+    t1 = time.perf_counter()
+    b.check()
+    pairs = b.pair_interactions() - pairs0
+    steps = nrep * gs
+    elapsed = t1 - t0
+    batch_sps = steps / elapsed
+    progress(f"timed {steps} steps: {1e3 * elapsed / steps:.3f} ms/step")
+    # sequential plain-path runs of the same drops (steps/s of the whole set)
+    order = np.argsort([len(d["x"]) for d in sims])
+    pick = order if not args.seq_sims else order[np.linspace(0, len(order) - 1, args.seq_sims).astype(int)]
+    seq_time = 0.0
+    seq_steps = 2
+    for k in pick:
+        solver, particles, _ = drop_solver(data=dict(sims[k]), dx=float(prm["dx"][k]))
+        solver.integrator.integrate(dt)          # first build, allocations
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        for _ in range(seq_steps):
+            solver.integrator.integrate(dt)
+        torch.cuda.synchronize()
+        seq_time += (time.perf_counter() - ta) / seq_steps
+        progress(f"sequential drop {k}: {len(sims[k]['x'])} particles")
+    seq_time *= len(sims) / len(pick)
+    seq_sps = 1.0 / seq_time
+    return {
+        "metric": "steps/s",
+        "value": batch_sps,
+        "unit": "steps/s (all drops advanced one RK2 step)",
+        "ms_per_step": 1e3 * elapsed / steps,
+        "dtype": "f64",
+        "config": {"workload": f"{len(sims)} independent 2D elliptical drops, RK2, dt 1e-5, FP64, "
+                               f"captured HIP graphs of {gs} steps",
+                   "particles": b.n, "drops": len(sims),
+                   "smallest_drop": int(min(b.counts)), "largest_drop": int(max(b.counts)),
+                   "pair_interactions_per_s": pairs / elapsed,
+                   "sequential_plain_gpu_steps_per_s": seq_sps,
+                   "sequential_drops_timed": int(len(pick)),
+                   "speedup_vs_sequential": batch_sps / seq_sps},
+        "roofline": None,
+    }, None
 
 
 def cpu_baseline_dambreak(args, data):
@@ -230,14 +336,16 @@ def main():
         sys.exit(1)
     if args.workload == "dambreak":
         result, data = run_dambreak(args)
-    else:
+    elif args.workload == "lattice":
         result, data = run_lattice(args)
+    else:
+        result, data = run_batch(args)
     out = {"metric": result["metric"], "value": result["value"], "unit": result["unit"],
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": result["ms_per_step"], "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": result["dtype"], "data": "synthetic",
            "config": result["config"], "roofline": result["roofline"]}
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and args.workload != "batch":
         if args.workload == "dambreak":
             out["cpu_baseline"] = cpu_baseline_dambreak(args, data)
         else:

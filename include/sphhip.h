@@ -109,11 +109,14 @@ int sph_bin_array(sph_ctx *ctx, const sph_grid *grid, const double *x, const dou
 
 /* Sorted particle records: SPH_REC doubles per particle in sorted order,
  * the layout every neighbour kernel reads (one 96-byte record per
- * neighbour instead of one load per property). */
+ * neighbour instead of one load per property).  Fields are paired in
+ * 16-byte slots by use: (x y)(z m) is all SPHRho reads when h is uniform,
+ * (u v)(w P/rho^2)(rho cs) complete MomentumEquation, (h p) last. */
 #define SPH_REC 12
 enum sph_rec_field {
-    SPH_R_X = 0, SPH_R_Y, SPH_R_Z, SPH_R_H, SPH_R_M, SPH_R_RHO, SPH_R_CS, SPH_R_P,
-    SPH_R_U, SPH_R_V, SPH_R_W, SPH_R_PRHO2 /* p * (1/(rho*rho)), filled by sph_momentum_av */
+    SPH_R_X = 0, SPH_R_Y, SPH_R_Z, SPH_R_M, SPH_R_U, SPH_R_V, SPH_R_W,
+    SPH_R_PRHO2, /* p * (1/(rho*rho)), filled by sph_momentum_av */
+    SPH_R_RHO, SPH_R_CS, SPH_R_H, SPH_R_P
 };
 
 /* Shadows: dst[k][t * dst_stride] = src[k][perm[t]] for k < nprops (sorted
@@ -173,12 +176,14 @@ int sph_nnps_export(sph_ctx *ctx, int64_t nd_real, const int32_t *d_rank,
  *   rho_i (+)= sum_{j in row} m_j W(x_i - x_j, (h_i+h_j)/2),  dim in {1,2,3}.
  * (counts, slice_off, indices) are the SELL-64 lists of sph_nnps_build.
  * rho[nd_total] is written in original order (rho[d_perm[t]]); out_s (may
- * be NULL) receives the same values at out_s[t * out_s_stride]. */
+ * be NULL) receives the same values at out_s[t * out_s_stride].
+ * uniform_h = 1 asserts that every destination and source particle has the
+ * same h (then (h_i+h_j)/2 == h_i exactly and h_j is not read). */
 int sph_summation_density(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
                           const int32_t *d_perm, const int32_t *counts,
                           const int64_t *slice_off, const int32_t *indices,
-                          const double *d_rec, const double *s_rec, int accumulate, double *rho,
-                          double *out_s, int64_t out_s_stride);
+                          const double *d_rec, const double *s_rec, int uniform_h,
+                          int accumulate, double *rho, double *out_s, int64_t out_s_stride);
 
 /* TaitEquation (sph/funcs/eos_funcs.pyx:80-129) over sorted positions:
  *   p (+)= (pow(rho/ro, gamma) - 1) * co*co*ro/gamma,
@@ -195,12 +200,13 @@ int sph_tait(sph_ctx *ctx, int64_t n_total, int64_t n_real, const int32_t *perm,
  *   dt_fac_i = max(dt_fac_i, |h_ij v_ij.r_ij / (r_ij^2 + eta^2 h_ij^2)|)
  * d_rec/s_rec are sorted records (s_rec has ns records; its SPH_R_PRHO2
  * field is recomputed from its p and rho fields first); ax, ay, az (az may
- * be NULL) and dt_fac are in original order. */
+ * be NULL) and dt_fac are in original order.  uniform_h as for
+ * sph_summation_density. */
 int sph_momentum_av(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
                     const int32_t *d_perm, const int32_t *counts, const int64_t *slice_off,
                     const int32_t *indices,
-                    const double *d_rec, double *s_rec, int64_t ns, double alpha, double beta,
-                    double eta, int accumulate, double *ax, double *ay, double *az,
+                    const double *d_rec, double *s_rec, int64_t ns, int uniform_h, double alpha,
+                    double beta, double eta, int accumulate, double *ax, double *ay, double *az,
                     double *dt_fac);
 
 /* GravityForce (sph/funcs/external_force.pyx:38-44): a (+)= g; az may be
@@ -237,6 +243,56 @@ int sph_pc_final(sph_ctx *ctx, int64_t n, const sph_step_props *props);
 int sph_copy(sph_ctx *ctx, int64_t n, const double *src, double *dst);
 /* fill n doubles with value (reset_accelerations / set_to_zero) */
 int sph_fill(sph_ctx *ctx, int64_t n, double value, double *dst);
+
+/* ---- batched simulations ------------------------------------------------ */
+
+/* Many independent simulations packed into one set of arrays (simulation s
+ * owns original indices [sim_off[s], sim_off[s+1]); sim_of[i] = s), each
+ * with its own cell grid (cell size radius_scale*max h of its particles,
+ * its own bounding box), so each one's neighbour lists and results are those
+ * of the single-simulation path.  No reference counterpart: the reference
+ * runs one simulation per Solver (solver/solver.py:523-590); a step here is
+ * RK2Integrator.integrate (solver/integrator.py:467-498) over SPHRho,
+ * TaitEquation, MomentumEquation (+ artificial viscosity) and
+ * PositionStepping, for every simulation at once.
+ * All arrays are device arrays of n doubles in the packed original order;
+ * z, w are read in 2D too (zeros), z0, w0, az, aw only in 3D.
+ * The step never synchronises with the host: capacities bound every size and
+ * violations set bits of *status (1: neighbour slots > list_cap, 2: cell keys
+ * > key_cap, 4: non-finite coordinate or h, 8: grid too large, 16: non-empty
+ * cells > max_cells); a step with a nonzero status is void.  *pairs
+ * accumulates the list entries of every build (each build's lists serve two
+ * operators). */
+typedef struct sph_batch_desc {
+    int32_t nsims, dim;
+    int64_t n;
+    const int64_t *sim_off; /* nsims + 1 */
+    const int32_t *sim_of;  /* n */
+    double *x, *y, *z, *u, *v, *w, *h, *m, *rho, *p, *cs;
+    double *x0, *y0, *z0, *u0, *v0, *w0; /* stage initial values */
+    double *ax, *ay, *az, *au, *av, *aw; /* accelerations of x, y, z, u, v, w */
+    double *dt_fac;
+    double co, ro, gamma, alpha, beta, eta, radius_scale;
+    int64_t key_cap;   /* cells of all grids together */
+    int64_t max_cells; /* non-empty cells */
+    int64_t list_cap;  /* neighbour-list slots */
+    int32_t maxch;     /* candidate chunks of 64 replayed by the list fill, 1..16 */
+    void *work;        /* device workspace of sph_batch_workspace_size bytes */
+    int64_t work_bytes;
+    int32_t *status;   /* device */
+    int64_t *pairs;    /* device */
+} sph_batch_desc;
+
+int sph_batch_workspace_size(sph_ctx *ctx, const sph_batch_desc *d, int64_t *bytes);
+/* one RK2 step of every simulation, stream-ordered */
+int sph_batch_rk2_step(sph_ctx *ctx, const sph_batch_desc *d, double dt);
+/* capture nsteps RK2 steps in a HIP graph (the context needs its own
+ * stream; run one eager step first so the context scratch has its final
+ * size -- the context then refuses to grow it) */
+int sph_batch_graph_create(sph_ctx *ctx, const sph_batch_desc *d, double dt, int nsteps,
+                           void **graph);
+int sph_batch_graph_launch(sph_ctx *ctx, void *graph);
+int sph_batch_graph_destroy(sph_ctx *ctx, void *graph);
 
 #ifdef __cplusplus
 }

@@ -53,6 +53,20 @@ class PropList(ctypes.Structure):
                 ("dst", _vp * MAX_PROPS)]
 
 
+class BatchDesc(ctypes.Structure):
+    """sph_batch_desc (include/sphhip.h)."""
+    _fields_ = ([("nsims", ctypes.c_int32), ("dim", ctypes.c_int32), ("n", _i64),
+                 ("sim_off", _vp), ("sim_of", _vp)]
+                + [(k, _vp) for k in ("x", "y", "z", "u", "v", "w", "h", "m", "rho", "p", "cs",
+                                      "x0", "y0", "z0", "u0", "v0", "w0",
+                                      "ax", "ay", "az", "au", "av", "aw", "dt_fac")]
+                + [(k, _dbl) for k in ("co", "ro", "gamma", "alpha", "beta", "eta",
+                                       "radius_scale")]
+                + [("key_cap", _i64), ("max_cells", _i64), ("list_cap", _i64),
+                   ("maxch", ctypes.c_int32), ("work", _vp), ("work_bytes", _i64),
+                   ("status", _vp), ("pairs", _vp)])
+
+
 # name -> (restype, argtypes)
 _SIGNATURES = {
     "sph_version": (ctypes.c_char_p, []),
@@ -71,17 +85,22 @@ _SIGNATURES = {
                               ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "sph_nnps_export": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sph_summation_density": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32,
-                                     _vp, _vp, _i64]),
+                                     _i32, _vp, _vp, _i64]),
     "sph_tait": (_i32, [_vp, _i64, _i64, _vp, _vp, _i64, _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp,
                         _vp]),
-    "sph_momentum_av": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _dbl,
-                               _dbl, _dbl, _i32, _vp, _vp, _vp, _vp]),
+    "sph_momentum_av": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32,
+                               _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp, _vp]),
     "sph_cubic_spline": (_i32, [_vp, _i32, _i64] + [_vp] * 11),
     "sph_gravity": (_i32, [_vp, _i64, _i64, _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp]),
     "sph_position_stepping": (_i32, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sph_stage_step": (_i32, [_vp, _i64, ctypes.POINTER(StepProps), _dbl]),
     "sph_pc_final": (_i32, [_vp, _i64, ctypes.POINTER(StepProps)]),
     "sph_copy": (_i32, [_vp, _i64, _vp, _vp]),
+    "sph_batch_workspace_size": (_i32, [_vp, _vp, ctypes.POINTER(_i64)]),
+    "sph_batch_rk2_step": (_i32, [_vp, _vp, _dbl]),
+    "sph_batch_graph_create": (_i32, [_vp, _vp, _dbl, _i32, ctypes.POINTER(_vp)]),
+    "sph_batch_graph_launch": (_i32, [_vp, _vp]),
+    "sph_batch_graph_destroy": (_i32, [_vp, _vp]),
     "sph_fill": (_i32, [_vp, _i64, _dbl, _vp]),
 }
 
@@ -128,9 +147,11 @@ def ptr(t):
 
 
 class Context:
-    """One sph_ctx per device, bound to the caller's current torch stream."""
+    """One sph_ctx per device, bound to the caller's current torch stream, or
+    -- when `stream` is given -- to that torch stream for good (a context of
+    its own, e.g. for graph capture)."""
 
-    def __init__(self, device=None):
+    def __init__(self, device=None, stream=None):
         if not torch.cuda.is_available():
             raise RuntimeError("pysph_amd needs a HIP device (torch.cuda.is_available() is "
                                "False); it has no CPU fallback")
@@ -138,7 +159,8 @@ class Context:
         self.device = torch.device(device) if device is not None else torch.device("cuda", 0)
         idx = self.device.index if self.device.index is not None else 0
         self.device_index = idx
-        self.stream = torch.cuda.current_stream(self.device)
+        self.fixed_stream = stream is not None
+        self.stream = stream if stream is not None else torch.cuda.current_stream(self.device)
         h = _vp()
         rc = self.lib.sph_ctx_create(ctypes.byref(h), idx, _vp(self.stream.cuda_stream))
         if rc != SPH_OK:
@@ -147,6 +169,8 @@ class Context:
 
     def sync_stream(self):
         """Follow torch's current stream (torch ops and our kernels stay ordered)."""
+        if self.fixed_stream:
+            return
         s = torch.cuda.current_stream(self.device)
         if s.cuda_stream != self.stream.cuda_stream:
             self.stream = s

@@ -31,8 +31,8 @@ from .. import _lib, profiling
 
 #: sorted record layout (SPH_R_* in include/sphhip.h): property -> field
 REC = 12
-REC_FIELDS = {"x": 0, "y": 1, "z": 2, "h": 3, "m": 4, "rho": 5, "cs": 6, "p": 7, "u": 8,
-              "v": 9, "w": 10}
+REC_FIELDS = {"x": 0, "y": 1, "z": 2, "m": 3, "u": 4, "v": 5, "w": 6, "rho": 8, "cs": 9,
+              "h": 10, "p": 11}
 SHADOW_PROPS = tuple(REC_FIELDS)
 
 
@@ -65,6 +65,10 @@ class _Bins:
         self.shadow_ver = {}
 
 
+def _grid_signature(g):
+    return (g.cell_size, tuple(g.cmin), tuple(g.dims), tuple(g.bits), g.order, g.ncells)
+
+
 class CellManager:
     """Device cell grid over a list of particle arrays (base/cell.pyx:742-1643)."""
 
@@ -83,6 +87,9 @@ class CellManager:
         self.cell_size = 0.0
         self.grid = None
         self.bins = {}
+        self._wanted = {}
+        self._h_uniform = False
+        self._h_versions = {}
         self.version = 0
         self.is_dirty = True
         self.initialized = False
@@ -138,11 +145,17 @@ class CellManager:
                                      self.min_cell_size, self.max_cell_size,
                                      self.ORDERS[self.order], self.max_cells,
                                      ctypes.byref(grid)))
+        sig = _grid_signature(grid)
+        same_grid = self.grid is not None and _grid_signature(self.grid) == sig
         self.grid = grid
+        self._h_uniform = grid.min_h == grid.max_h
+        self._h_versions = {a.name: a.version("h") for a in arrays}
         self.cell_size = grid.cell_size
         for a in arrays:
             np_ = a.get_number_of_particles()
             b = self.bins.get(a.name)
+            if same_grid and b is not None and b.n == np_ and not a.is_dirty:
+                continue  # unmoved array on an unchanged grid: its bins still hold
             if b is None or b.n != np_ or b.cell_start.numel() != grid.ncells + 1:
                 b = _Bins(np_, grid.ncells, a.device)
                 self.bins[a.name] = b
@@ -156,13 +169,30 @@ class CellManager:
             b.ncells_nonempty = int(nne.value)
             b.invalidate()
 
+    def uniform_h(self):
+        """True when every binned particle has the same h (as of the last
+        binning, with no h written since): the operator kernels then take
+        0.5*(h_a+h_b) == h_a without reading h_b."""
+        self.update()
+        return bool(self._h_uniform) and all(
+            a.version("h") == self._h_versions.get(a.name) for a in self.arrays_to_bin)
+
     # -- sorted records --------------------------------------------------
     def shadows(self, array, props):
         """The sorted record tensor of `array` with the fields of `props`
         refreshed where stale (binning changed or the property was written)."""
         self.update()
         b = self.bins[array.name]
-        stale = [p for p in props if b.shadow_ver.get(p) != array.version(p)]
+        # every field asked for so far is refreshed with this one launch, so
+        # that one evaluation needs about one gather per array
+        wanted = self._wanted.setdefault(array.name, [])
+        for p in props:
+            if p not in wanted:
+                wanted.append(p)
+        if any(b.shadow_ver.get(p) != array.version(p) for p in props):
+            stale = [p for p in wanted if b.shadow_ver.get(p) != array.version(p)]
+        else:
+            stale = []
         if stale:
             ctx = self._ctx()
             pl = _lib.PropList()

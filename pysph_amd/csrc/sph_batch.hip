@@ -1,0 +1,413 @@
+// Many independent simulations advanced together.
+//
+// The simulations are packed into one set of particle arrays (simulation s
+// owns original indices [sim_off[s], sim_off[s+1])).  Each simulation keeps
+// its own cell grid -- cell size radius_scale*max(h) over its particles and
+// its own bounding box, exactly what CellManager would build for it alone
+// (base/cell.pyx:911-941, :1140-1158) -- so its neighbour lists, and with
+// them every sum, are the ones the single-simulation path produces.  The
+// grids are laid side by side in one key space (key_base[s] + linear cell
+// index), so one binning pass, one list build and one launch per operator
+// serve every simulation.
+//
+// A step runs without any host synchronisation: sizes are bounded by the
+// workspace capacities and checked on the device (status bits), so a whole
+// step, or several, can be captured in a HIP graph and replayed.
+//
+// The step is RK2Integrator.integrate (solver/integrator.py:467-498) over
+// the operation list of the 2D/3D drop: SPHRho (non-integrating), Tait
+// (non-integrating), MomentumEquation with artificial viscosity and
+// PositionStepping (integrating).
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "sph_internal.h"
+
+#define BT_THREADS 256
+
+// ---------------------------------------------------------------------------
+// per-simulation grids
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double blk_min(double v, double *red)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = fmin(v, __shfl_xor(v, d, 64));
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double r = red[0];
+    for (int w = 1; w < BT_THREADS / 64; ++w) r = fmin(r, red[w]);
+    return r;
+}
+
+// One block per simulation: bounding box and h range of its particles, then
+// its grid (CellManager.compute_cell_size with the Particles defaults: no
+// clamps, 0 -> 1.0).  ncell[s] = cells of the grid.
+__global__ void __launch_bounds__(BT_THREADS)
+    k_batch_grid(const int64_t *__restrict__ sim_off, const double *__restrict__ x,
+                 const double *__restrict__ y, const double *__restrict__ z,
+                 const double *__restrict__ h, double radius_scale, GridDev *__restrict__ grids,
+                 int64_t *__restrict__ ncell, int32_t *__restrict__ status)
+{
+    __shared__ double red[BT_THREADS / 64];
+    const int s = blockIdx.x;
+    const int64_t b = sim_off[s], e = sim_off[s + 1];
+    double v[8];
+    for (int k = 0; k < 8; ++k) v[k] = INFINITY;
+    double bad = 0.0;
+    for (int64_t i = b + threadIdx.x; i < e; i += BT_THREADS) {
+        const double a = x[i], bb = y[i], c = z[i], hh = h[i];
+        if (!isfinite(a) || !isfinite(bb) || !isfinite(c) || !isfinite(hh)) {
+            bad = 1.0;
+            continue;
+        }
+        v[0] = fmin(v[0], a);
+        v[1] = fmin(v[1], bb);
+        v[2] = fmin(v[2], c);
+        v[3] = fmin(v[3], -a);
+        v[4] = fmin(v[4], -bb);
+        v[5] = fmin(v[5], -c);
+        v[6] = fmin(v[6], -hh);
+        v[7] = fmin(v[7], hh);
+    }
+    for (int k = 0; k < 8; ++k) v[k] = blk_min(v[k], red);
+    bad = -blk_min(-bad, red);
+    if (threadIdx.x != 0) return;
+    GridDev g;
+    g.tab = nullptr;
+    g.key_base = 0;
+    int64_t cells = 1;
+    if (e <= b || bad > 0.0) {
+        if (bad > 0.0) atomicOr(status, 4);
+        g.cs = 1.0;
+        g.cmin0 = g.cmin1 = g.cmin2 = 0;
+        g.d0 = g.d1 = g.d2 = 1;
+    } else {
+        double cs = radius_scale * (-v[6]);
+        if (cs == 0.0) cs = 1.0;
+        g.cs = cs;
+        int cm[3], dm[3];
+        for (int a = 0; a < 3; ++a) {
+            const double fl = floor(v[a] / cs), fh = floor(-v[a + 3] / cs);
+            if (fabs(fl) > 1.0e9 || fabs(fh) > 1.0e9 || fh - fl + 1.0 > 1.0e6) {
+                atomicOr(status, 8);
+                cm[a] = 0;
+                dm[a] = 1;
+            } else {
+                cm[a] = (int)fl;
+                dm[a] = (int)(fh - fl) + 1;
+            }
+            cells *= dm[a];
+        }
+        g.cmin0 = cm[0];
+        g.cmin1 = cm[1];
+        g.cmin2 = cm[2];
+        g.d0 = dm[0];
+        g.d1 = dm[1];
+        g.d2 = dm[2];
+    }
+    grids[s] = g;
+    ncell[s] = cells;
+}
+
+// key_base of every simulation (exclusive sum of the cell counts); the key
+// space must fit key_cap (status |= 2 otherwise)
+__global__ void k_batch_key_base(int nsims, const int64_t *__restrict__ ncell, int64_t key_cap,
+                                 GridDev *__restrict__ grids, int32_t *__restrict__ status)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int64_t base = 0;
+    for (int s = 0; s < nsims; ++s) {
+        if (base + ncell[s] <= key_cap) {
+            grids[s].key_base = (int32_t)base;
+        } else {
+            // no room: a one-cell grid at key 0 keeps every later access in
+            // bounds; the step is void (status |= 2)
+            GridDev g = grids[s];
+            g.key_base = 0;
+            g.d0 = g.d1 = g.d2 = 1;
+            grids[s] = g;
+        }
+        base += ncell[s];
+    }
+    if (base > key_cap) *status |= 2;
+}
+
+__global__ void __launch_bounds__(BT_THREADS)
+    k_batch_keys(const GridDev *__restrict__ grids, const int32_t *__restrict__ sim_of,
+                 const double *__restrict__ x, const double *__restrict__ y,
+                 const double *__restrict__ z, int64_t n, int64_t key_cap,
+                 int32_t *__restrict__ count, int32_t *__restrict__ cell_of,
+                 int32_t *__restrict__ slot, int32_t *__restrict__ status)
+{
+    const int64_t i = (int64_t)blockIdx.x * BT_THREADS + threadIdx.x;
+    if (i >= n) return;
+    const GridDev g = grids[sim_of[i]];
+    const int cx = min(max(cell_coord(x[i], g.cs) - g.cmin0, 0), g.d0 - 1);
+    const int cy = min(max(cell_coord(y[i], g.cs) - g.cmin1, 0), g.d1 - 1);
+    const int cz = min(max(cell_coord(z[i], g.cs) - g.cmin2, 0), g.d2 - 1);
+    int64_t key = (int64_t)g.key_base + ((int64_t)cx * g.d1 + cy) * g.d2 + cz;
+    if (key >= key_cap) {  // the key space overflowed (status 2 is set too)
+        atomicOr(status, 2);
+        key = key_cap - 1;
+    }
+    cell_of[i] = (int32_t)key;
+    slot[i] = atomicAdd(&count[key], 1);
+}
+
+// more non-empty cells than the list kernels are launched for: status |= 16
+__global__ void k_batch_check_cells(const int32_t *__restrict__ ncell_dev, int64_t max_cells,
+                                    int32_t *__restrict__ status)
+{
+    if (threadIdx.x == 0 && *ncell_dev > max_cells) *status |= 16;
+}
+
+// ---------------------------------------------------------------------------
+// workspace
+// ---------------------------------------------------------------------------
+namespace {
+
+struct BatchWork {
+    GridDev *grids;
+    int64_t *ncell;
+    int32_t *cell_start, *cell_of, *rank, *perm, *cells, *slot, *tmp;
+    double *rec;
+    int32_t *counts;
+    int64_t *slice_off;
+    int32_t *indices;
+    size_t bytes;
+};
+
+size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// carve the caller's workspace (base may be NULL to only compute the size)
+BatchWork carve(const sph_batch_desc *d, char *base)
+{
+    BatchWork w;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        char *p = base ? base + o : nullptr;
+        o += al256(bytes);
+        return p;
+    };
+    const int64_t n = d->n, kc = d->key_cap, nsl = (n + 63) / 64;
+    w.grids = (GridDev *)take(sizeof(GridDev) * (size_t)d->nsims);
+    w.ncell = (int64_t *)take(sizeof(int64_t) * (size_t)d->nsims);
+    w.cell_start = (int32_t *)take(sizeof(int32_t) * (size_t)(kc + 1));
+    w.cell_of = (int32_t *)take(sizeof(int32_t) * (size_t)n);
+    w.rank = (int32_t *)take(sizeof(int32_t) * (size_t)n);
+    w.perm = (int32_t *)take(sizeof(int32_t) * (size_t)n);
+    w.cells = (int32_t *)take(sizeof(int32_t) * (size_t)kc);
+    w.slot = (int32_t *)take(sizeof(int32_t) * (size_t)(kc + 1));
+    w.tmp = (int32_t *)take(sizeof(int32_t) * (size_t)n);
+    w.rec = (double *)take(sizeof(double) * SPH_REC * (size_t)n);
+    w.counts = (int32_t *)take(sizeof(int32_t) * (size_t)(n + 1));
+    w.slice_off = (int64_t *)take(sizeof(int64_t) * 2 * (size_t)(nsl + 1));
+    w.indices = (int32_t *)take(sizeof(int32_t) * (size_t)d->list_cap);
+    w.bytes = o;
+    return w;
+}
+
+int check_desc(sph_ctx *ctx, const sph_batch_desc *d)
+{
+    SPH_REQUIRE(ctx, d, "batch: null descriptor");
+    SPH_REQUIRE(ctx, d->nsims >= 1 && (d->dim == 2 || d->dim == 3), "batch: bad nsims/dim");
+    SPH_REQUIRE(ctx, d->n >= 1 && d->n < ((int64_t)1 << 31), "batch: bad n");
+    SPH_REQUIRE(ctx, d->key_cap >= 1 && d->key_cap < ((int64_t)1 << 31) - 1 && d->list_cap >= 1 &&
+                         d->max_cells >= 1 && d->max_cells <= d->key_cap,
+                "batch: bad capacities");
+    SPH_REQUIRE(ctx, d->maxch >= 1 && d->maxch <= 16, "batch: bad maxch");
+    SPH_REQUIRE(ctx, d->sim_off && d->sim_of && d->x && d->y && d->z && d->u && d->v && d->w &&
+                         d->h && d->m && d->rho && d->p && d->cs && d->x0 && d->y0 && d->u0 &&
+                         d->v0 && d->ax && d->ay && d->au && d->av && d->dt_fac && d->status &&
+                         d->pairs,
+                "batch: null array");
+    SPH_REQUIRE(ctx, d->dim == 2 || (d->z0 && d->w0 && d->az && d->aw),
+                "batch: 3D needs z0, w0, az, aw");
+    SPH_REQUIRE(ctx, d->work, "batch: null workspace");
+    return SPH_OK;
+}
+
+// one evaluation of the operation list at the current positions
+int batch_eval(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w)
+{
+    const int64_t n = d->n;
+    int rc;
+    // -- binning into the per-simulation grids
+    hipLaunchKernelGGL(k_batch_grid, dim3(d->nsims), dim3(BT_THREADS), 0, ctx->stream,
+                       d->sim_off, d->x, d->y, d->z, d->h, d->radius_scale, w.grids, w.ncell,
+                       d->status);
+    SPH_LAUNCH_CHECK(ctx, "k_batch_grid");
+    hipLaunchKernelGGL(k_batch_key_base, dim3(1), dim3(64), 0, ctx->stream, d->nsims, w.ncell,
+                       d->key_cap, w.grids, d->status);
+    SPH_LAUNCH_CHECK(ctx, "k_batch_key_base");
+    hipLaunchKernelGGL(k_fill_i32, dim3(sph_blocks(d->key_cap + 1, BT_THREADS)),
+                       dim3(BT_THREADS), 0, ctx->stream, w.cell_start, d->key_cap + 1, 0);
+    SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
+    hipLaunchKernelGGL(k_batch_keys, dim3(sph_blocks(n, BT_THREADS)), dim3(BT_THREADS), 0,
+                       ctx->stream, w.grids, d->sim_of, d->x, d->y, d->z, n, d->key_cap,
+                       w.cell_start, w.cell_of, w.rank, d->status);
+    SPH_LAUNCH_CHECK(ctx, "k_batch_keys");
+    rc = sph_bin_sort(ctx, n, d->key_cap, w.cell_of, w.rank, w.cell_start, w.perm, w.cells,
+                      w.slot, w.tmp, 0);
+    if (rc) return rc;
+    const int32_t *ncell_dev = w.slot + d->key_cap;
+    hipLaunchKernelGGL(k_batch_check_cells, dim3(1), dim3(64), 0, ctx->stream, ncell_dev,
+                       d->max_cells, d->status);
+    SPH_LAUNCH_CHECK(ctx, "k_batch_check_cells");
+    // -- sorted records of the fields the operators read
+    sph_prop_list pl;
+    memset(&pl, 0, sizeof(pl));
+    pl.dst_stride = SPH_REC;
+    const double *src[8] = {d->x, d->y, d->z, d->m, d->u, d->v, d->w, d->h};
+    const int fld[8] = {SPH_R_X, SPH_R_Y, SPH_R_Z, SPH_R_M, SPH_R_U, SPH_R_V, SPH_R_W, SPH_R_H};
+    pl.nprops = 8;
+    for (int k = 0; k < 8; ++k) {
+        pl.src[k] = src[k];
+        pl.dst[k] = w.rec + fld[k];
+    }
+    rc = sph_gather_sorted(ctx, n, w.perm, &pl);
+    if (rc) return rc;
+    // -- neighbour lists
+    rc = sph_nnps_build_batch(ctx, w.grids, d->sim_of, w.rec, w.perm, w.cell_start, w.cells,
+                              ncell_dev, d->max_cells, n, d->radius_scale, d->maxch, w.counts,
+                              w.slice_off, w.indices, d->list_cap, d->status, d->pairs);
+    if (rc) return rc;
+    // -- SPHRho, Tait (non-integrating: outputs start from 0)
+    rc = sph_summation_density(ctx, d->dim, n, n, w.perm, w.counts, w.slice_off, w.indices,
+                               w.rec, w.rec, 0, 0, d->rho, w.rec + SPH_R_RHO, SPH_REC);
+    if (rc) return rc;
+    rc = sph_tait(ctx, n, n, w.perm, w.rec + SPH_R_RHO, SPH_REC, d->ro, d->co, d->gamma, 0,
+                  d->p, d->cs, w.rec + SPH_R_P, w.rec + SPH_R_CS);
+    if (rc) return rc;
+    // -- MomentumEquation (dt_fac reset per calc), PositionStepping
+    rc = sph_fill(ctx, n, 0.0, d->dt_fac);
+    if (rc) return rc;
+    rc = sph_momentum_av(ctx, d->dim, n, n, w.perm, w.counts, w.slice_off, w.indices, w.rec,
+                         w.rec, n, 0, d->alpha, d->beta, d->eta, 0, d->au, d->av,
+                         d->dim == 3 ? d->aw : nullptr, d->dt_fac);
+    if (rc) return rc;
+    return sph_position_stepping(ctx, n, n, d->u, d->v, d->dim == 3 ? d->w : nullptr, d->ax,
+                                 d->ay, d->dim == 3 ? d->az : nullptr);
+}
+
+int batch_stage(sph_ctx *ctx, const sph_batch_desc *d, double dt)
+{
+    sph_step_props sp;
+    memset(&sp, 0, sizeof(sp));
+    double *x[6] = {d->u, d->v, d->x, d->y, d->w, d->z};
+    const double *x0[6] = {d->u0, d->v0, d->x0, d->y0, d->w0, d->z0};
+    const double *a[6] = {d->au, d->av, d->ax, d->ay, d->aw, d->az};
+    sp.nprops = d->dim == 3 ? 6 : 4;
+    for (int k = 0; k < sp.nprops; ++k) {
+        sp.x[k] = x[k];
+        sp.x0[k] = x0[k];
+        sp.a[k] = a[k];
+    }
+    return sph_stage_step(ctx, d->n, &sp, dt);
+}
+
+int batch_rk2(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w, double dt)
+{
+    // save_initial_arrays
+    double *dst[6] = {d->u0, d->v0, d->x0, d->y0, d->w0, d->z0};
+    const double *src[6] = {d->u, d->v, d->x, d->y, d->w, d->z};
+    const int np = d->dim == 3 ? 6 : 4;
+    for (int k = 0; k < np; ++k) {
+        int rc = sph_copy(ctx, d->n, src[k], dst[k]);
+        if (rc) return rc;
+    }
+    int rc = batch_eval(ctx, d, w);
+    if (rc) return rc;
+    rc = batch_stage(ctx, d, dt / 2);
+    if (rc) return rc;
+    rc = batch_eval(ctx, d, w);
+    if (rc) return rc;
+    return batch_stage(ctx, d, dt);
+}
+
+struct BatchGraph {
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+};
+
+}  // namespace
+
+extern "C" int sph_batch_workspace_size(sph_ctx *ctx, const sph_batch_desc *d, int64_t *bytes)
+{
+    if (!ctx) return SPH_EINVAL;
+    SPH_REQUIRE(ctx, d && bytes, "sph_batch_workspace_size: null argument");
+    *bytes = (int64_t)carve(d, nullptr).bytes;
+    return SPH_OK;
+}
+
+extern "C" int sph_batch_rk2_step(sph_ctx *ctx, const sph_batch_desc *d, double dt)
+{
+    if (!ctx) return SPH_EINVAL;
+    int rc = check_desc(ctx, d);
+    if (rc) return rc;
+    BatchWork w = carve(d, (char *)d->work);
+    SPH_REQUIRE(ctx, d->work_bytes >= (int64_t)w.bytes, "batch: workspace of %lld bytes < %zu",
+                (long long)d->work_bytes, w.bytes);
+    return batch_rk2(ctx, d, w, dt);
+}
+
+extern "C" int sph_batch_graph_create(sph_ctx *ctx, const sph_batch_desc *d, double dt,
+                                      int nsteps, void **graph_out)
+{
+    if (!ctx) return SPH_EINVAL;
+    SPH_REQUIRE(ctx, graph_out && nsteps >= 1, "sph_batch_graph_create: bad argument");
+    *graph_out = nullptr;
+    int rc = check_desc(ctx, d);
+    if (rc) return rc;
+    SPH_REQUIRE(ctx, ctx->stream != nullptr, "sph_batch_graph_create: the context needs a stream");
+    BatchWork w = carve(d, (char *)d->work);
+    SPH_REQUIRE(ctx, d->work_bytes >= (int64_t)w.bytes, "batch: workspace of %lld bytes < %zu",
+                (long long)d->work_bytes, w.bytes);
+    // the scratch must not move while a graph refers to it: the caller runs
+    // one eager step first (which sizes it) and the context stays frozen
+    ctx->frozen = 1;
+    BatchGraph *g = (BatchGraph *)calloc(1, sizeof(BatchGraph));
+    if (!g) return sph_set_error(ctx, SPH_ENOMEM, "graph handle");
+    SPH_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < nsteps && rc == SPH_OK; ++s) rc = batch_rk2(ctx, d, w, dt);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc != SPH_OK || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        free(g);
+        if (rc != SPH_OK) return rc;
+        return sph_set_error(ctx, SPH_EHIP, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        free(g);
+        return sph_set_error(ctx, SPH_EHIP, "hipGraphInstantiate failed: %s", hipGetErrorString(e));
+    }
+    g->graph = graph;
+    *graph_out = g;
+    return SPH_OK;
+}
+
+extern "C" int sph_batch_graph_launch(sph_ctx *ctx, void *graph)
+{
+    if (!ctx) return SPH_EINVAL;
+    SPH_REQUIRE(ctx, graph, "sph_batch_graph_launch: null graph");
+    SPH_HIP_CHECK(ctx, hipGraphLaunch(((BatchGraph *)graph)->exec, ctx->stream));
+    return SPH_OK;
+}
+
+extern "C" int sph_batch_graph_destroy(sph_ctx *ctx, void *graph)
+{
+    if (!ctx) return SPH_EINVAL;
+    if (!graph) return SPH_OK;
+    BatchGraph *g = (BatchGraph *)graph;
+    SPH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    free(g);
+    return SPH_OK;
+}

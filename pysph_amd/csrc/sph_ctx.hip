@@ -77,6 +77,10 @@ int sph_set_error(sph_ctx *ctx, int code, const char *fmt, ...)
 int sph_ensure_scratch(sph_ctx *ctx, size_t bytes)
 {
     if (bytes <= ctx->scratch_bytes) return SPH_OK;
+    if (ctx->frozen)
+        return sph_set_error(ctx, SPH_EINVAL,
+                             "scratch of %zu bytes needed but a captured graph holds the "
+                             "current one (%zu bytes)", bytes, ctx->scratch_bytes);
     size_t nb = bytes + bytes / 4 + 4096;
     if (ctx->scratch) {
         SPH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -222,11 +226,19 @@ size_t sph_scan_scratch_bytes(int64_t n, size_t elem)
     return bytes;
 }
 
+// data[0] = 0 by a kernel, not hipMemsetAsync: the scans may be captured in
+// a HIP graph
+template <typename T> __global__ void k_zero_first(T *data)
+{
+    if (threadIdx.x == 0) data[0] = 0;
+}
+
 template <typename T>
 static int scan_exclusive(sph_ctx *ctx, T *data, int64_t n, size_t scratch_off)
 {
     if (n <= 0) {
-        SPH_HIP_CHECK(ctx, hipMemsetAsync(data, 0, sizeof(T), ctx->stream));
+        hipLaunchKernelGGL(k_zero_first<T>, dim3(1), dim3(64), 0, ctx->stream, data);
+        SPH_LAUNCH_CHECK(ctx, "k_zero_first");
         return SPH_OK;
     }
     // tile-sum levels live in scratch (from scratch_off): n/TILE, n/TILE^2, ...

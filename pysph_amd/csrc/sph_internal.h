@@ -24,6 +24,8 @@ struct sph_ctx {
     int32_t *key_tab;
     int64_t key_tab_len;
     int32_t key_dims[3], key_bits[3], key_order;
+    // set once a graph refers to the scratch: it may no longer be reallocated
+    int frozen;
 };
 
 int sph_set_error(sph_ctx *ctx, int code, const char *fmt, ...);
@@ -69,6 +71,50 @@ __device__ __forceinline__ unsigned sph_xcd_tile(unsigned b, unsigned nb)
     const unsigned x = b % SPH_NXCD, k = b / SPH_NXCD;
     return x * q + (x < r ? x : r) + k;
 }
+
+// A cell grid as the kernels see it.  Cell keys come from the per-axis
+// tables tab = tx | ty | tz (Morton or linear order) or, with tab == NULL,
+// are key_base + the linear index (the per-simulation grids of a batch).
+struct GridDev {
+    double cs;
+    int cmin0, cmin1, cmin2;
+    int d0, d1, d2;
+    const int32_t *tab;
+    int32_t key_base;
+};
+
+__device__ __forceinline__ int cell_coord(double v, double cs) { return (int)floor(v / cs); }
+
+__device__ __forceinline__ int32_t cell_key(const GridDev &g, int cx, int cy, int cz)
+{
+    if (g.tab) return g.tab[cx] + g.tab[g.d0 + cy] + g.tab[g.d0 + g.d1 + cz];
+    return g.key_base + (cx * g.d1 + cy) * g.d2 + cz;
+}
+
+__global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v);
+
+// Counting-sort tail shared by the single-grid and batched binning: given
+// cell_of[n] and, per particle, its slot handed out by the atomic count
+// (in rank), and the per-cell counts in cell_start[0..nc), produce
+// cell_start (exclusive scan), perm/rank (ascending index inside every
+// cell), slot[0..nc] (exclusive scan of the non-empty flags; slot[nc] = the
+// number of non-empty cells) and cells[] (their keys).  tmp holds n ints.
+// Scan partials go to the context scratch from scratch_off on.
+int sph_bin_sort(sph_ctx *ctx, int64_t n, int64_t nc, const int32_t *cell_of, int32_t *rank,
+                 int32_t *cell_start, int32_t *perm, int32_t *cells, int32_t *slot, int32_t *tmp,
+                 size_t scratch_off);
+
+// Neighbour lists of a batch of simulations packed in one array, without
+// host synchronisation (capturable in a graph): grids[sim_of[i]] is the grid
+// of original particle i, *ncell_dev the number of non-empty cells (at most
+// max_cells).  Lists needing more than cap slots set bit 0 of *status and
+// are not written; nnz is added to *nnz_acc.
+int sph_nnps_build_batch(sph_ctx *ctx, const GridDev *grids, const int32_t *sim_of,
+                         const double *rec, const int32_t *perm, const int32_t *cell_start,
+                         const int32_t *cells, const int32_t *ncell_dev, int64_t max_cells,
+                         int64_t n, double radius_scale, int maxch, int32_t *counts,
+                         int64_t *slice_off, int32_t *indices, int64_t cap, int32_t *status,
+                         int64_t *nnz_acc);
 
 // Exclusive prefix sums (device-wide), stream-ordered.  `data` has n
 // elements; the scanned values are written in place and the total is

@@ -282,13 +282,6 @@ extern "C" int sph_grid_build(sph_ctx *ctx, int narrays, const double *const *xs
 // ---------------------------------------------------------------------------
 // binning
 // ---------------------------------------------------------------------------
-struct GridDev {
-    double cs;
-    int cmin0, cmin1, cmin2;
-    int d0, d1, d2;
-    const int32_t *tab;  // tx | ty | tz
-};
-
 static GridDev make_griddev(sph_ctx *ctx, const sph_grid *g)
 {
     GridDev d;
@@ -300,14 +293,8 @@ static GridDev make_griddev(sph_ctx *ctx, const sph_grid *g)
     d.d1 = g->dims[1];
     d.d2 = g->dims[2];
     d.tab = ctx->key_tab;
+    d.key_base = 0;
     return d;
-}
-
-__device__ __forceinline__ int cell_coord(double v, double cs) { return (int)floor(v / cs); }
-
-__device__ __forceinline__ int32_t cell_key(const GridDev &g, int cx, int cy, int cz)
-{
-    return g.tab[cx] + g.tab[g.d0 + cy] + g.tab[g.d0 + g.d1 + cz];
 }
 
 __global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v)
@@ -337,36 +324,38 @@ __global__ void __launch_bounds__(NN_THREADS)
 }
 
 __global__ void __launch_bounds__(NN_THREADS)
-    k_bin_scatter(const int32_t *__restrict__ cell_of, const int32_t *__restrict__ rank,
-                  int64_t n, const int32_t *__restrict__ start, int32_t *__restrict__ perm)
+    k_bin_scatter(const int32_t *__restrict__ cell_of, const int32_t *__restrict__ slot,
+                  int64_t n, const int32_t *__restrict__ start, int32_t *__restrict__ tmp)
 {
     int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
     if (i >= n) return;
-    perm[start[cell_of[i]] + rank[i]] = (int32_t)i;
+    tmp[start[cell_of[i]] + slot[i]] = (int32_t)i;
 }
 
-// Restore ascending particle index inside every cell (the atomics above
-// hand out ranks in arbitrary order), then record the inverse permutation
-// and flag non-empty cells.  One thread per cell; cells hold a handful of
-// particles.
+// Ascending particle index inside every cell (the atomics above hand out
+// slots in arbitrary order): particle i's position is the number of members
+// of its cell with a smaller index.  Writes perm and its inverse rank.
 __global__ void __launch_bounds__(NN_THREADS)
-    k_cell_sort(const int32_t *__restrict__ start, int64_t ncells, int32_t *__restrict__ perm,
-                int32_t *__restrict__ rank, int32_t *__restrict__ nonempty)
+    k_bin_rank(const int32_t *__restrict__ cell_of, int64_t n, const int32_t *__restrict__ start,
+               const int32_t *__restrict__ tmp, int32_t *__restrict__ perm,
+               int32_t *__restrict__ rank)
+{
+    int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
+    if (i >= n) return;
+    const int32_t key = cell_of[i];
+    const int32_t s = start[key], e = start[key + 1];
+    int32_t r = 0;
+    for (int32_t a = s; a < e; ++a) r += tmp[a] < (int32_t)i;
+    perm[s + r] = (int32_t)i;
+    rank[i] = s + r;
+}
+
+__global__ void __launch_bounds__(NN_THREADS)
+    k_cell_flags(const int32_t *__restrict__ start, int64_t ncells, int32_t *__restrict__ nonempty)
 {
     int64_t c = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
     if (c >= ncells) return;
-    int32_t s = start[c], e = start[c + 1];
-    nonempty[c] = e > s ? 1 : 0;
-    for (int32_t a = s + 1; a < e; ++a) {
-        int32_t v = perm[a];
-        int32_t b = a - 1;
-        while (b >= s && perm[b] > v) {
-            perm[b + 1] = perm[b];
-            --b;
-        }
-        perm[b + 1] = v;
-    }
-    for (int32_t a = s; a < e; ++a) rank[perm[a]] = a;
+    nonempty[c] = start[c + 1] > start[c] ? 1 : 0;
 }
 
 __global__ void __launch_bounds__(NN_THREADS)
@@ -386,6 +375,30 @@ static int read_i64(sph_ctx *ctx, const void *dev, int64_t *out, bool is32)
                                       hipMemcpyDeviceToHost, ctx->stream));
     SPH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     *out = is32 ? (int64_t) * (int32_t *)ctx->pinned : *(int64_t *)ctx->pinned;
+    return SPH_OK;
+}
+
+int sph_bin_sort(sph_ctx *ctx, int64_t n, int64_t nc, const int32_t *cell_of, int32_t *rank,
+                 int32_t *cell_start, int32_t *perm, int32_t *cells, int32_t *slot, int32_t *tmp,
+                 size_t scratch_off)
+{
+    int rc = sph_scan_exclusive_i32_at(ctx, cell_start, nc, scratch_off);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_bin_scatter, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
+                       ctx->stream, cell_of, rank, n, cell_start, tmp);
+    SPH_LAUNCH_CHECK(ctx, "k_bin_scatter");
+    hipLaunchKernelGGL(k_bin_rank, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
+                       ctx->stream, cell_of, n, cell_start, tmp, perm, rank);
+    SPH_LAUNCH_CHECK(ctx, "k_bin_rank");
+    // non-empty flags -> slots (scan) -> compact list of cell keys
+    hipLaunchKernelGGL(k_cell_flags, dim3(sph_blocks(nc, NN_THREADS)), dim3(NN_THREADS), 0,
+                       ctx->stream, cell_start, nc, slot);
+    SPH_LAUNCH_CHECK(ctx, "k_cell_flags");
+    rc = sph_scan_exclusive_i32_at(ctx, slot, nc, scratch_off);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_compact_cells, dim3(sph_blocks(nc, NN_THREADS)), dim3(NN_THREADS), 0,
+                       ctx->stream, cell_start, slot, nc, cells);
+    SPH_LAUNCH_CHECK(ctx, "k_compact_cells");
     return SPH_OK;
 }
 
@@ -410,30 +423,23 @@ extern "C" int sph_bin_array(sph_ctx *ctx, const sph_grid *grid, const double *x
                        ctx->stream, cell_start, nc + 1, 0);
     SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
     if (n == 0) return SPH_OK;
+    // scratch: [non-empty flags / slots (nc+1) | unsorted cell members (n) | scan partials];
+    // the scan of cell_start runs before either region is written
+    const size_t slot_bytes = (sizeof(int32_t) * (size_t)(nc + 1) + 511) & ~(size_t)255;
+    const size_t tmp_bytes = (sizeof(int32_t) * (size_t)n + 255) & ~(size_t)255;
+    const size_t scan_bytes = sph_scan_scratch_bytes(nc, sizeof(int32_t));
+    rc = sph_ensure_scratch(ctx, slot_bytes + tmp_bytes + scan_bytes);
+    if (rc) return rc;
+    int32_t *slot = (int32_t *)ctx->scratch;
+    int32_t *tmp = (int32_t *)((char *)ctx->scratch + slot_bytes);
     // rank first holds the slot inside the cell handed out by the atomics;
-    // k_cell_sort overwrites it with the sorted position
+    // sph_bin_sort overwrites it with the sorted position
     hipLaunchKernelGGL(k_bin_keys, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
                        ctx->stream, g, x, y, z, n, cell_start, cell_of, rank);
     SPH_LAUNCH_CHECK(ctx, "k_bin_keys");
-    rc = sph_scan_exclusive_i32(ctx, cell_start, nc);
+    rc = sph_bin_sort(ctx, n, nc, cell_of, rank, cell_start, perm, cells, slot, tmp,
+                      slot_bytes + tmp_bytes);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_bin_scatter, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
-                       ctx->stream, cell_of, rank, n, cell_start, perm);
-    SPH_LAUNCH_CHECK(ctx, "k_bin_scatter");
-    // non-empty flags -> slots (scan) -> compact list of cell keys
-    const size_t slot_bytes = (sizeof(int32_t) * (size_t)(nc + 1) + 511) & ~(size_t)255;
-    rc = sph_ensure_scratch(ctx, slot_bytes + sph_scan_scratch_bytes(nc, sizeof(int32_t)));
-    if (rc) return rc;
-    int32_t *slot = (int32_t *)ctx->scratch;
-    hipLaunchKernelGGL(k_cell_sort, dim3(sph_blocks(nc, NN_THREADS)), dim3(NN_THREADS), 0,
-                       ctx->stream, cell_start, nc, perm, rank, slot);
-    SPH_LAUNCH_CHECK(ctx, "k_cell_sort");
-    // the scan's own partial sums live past the slot array in the scratch
-    rc = sph_scan_exclusive_i32_at(ctx, slot, nc, slot_bytes);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_compact_cells, dim3(sph_blocks(nc, NN_THREADS)), dim3(NN_THREADS), 0,
-                       ctx->stream, cell_start, slot, nc, cells);
-    SPH_LAUNCH_CHECK(ctx, "k_compact_cells");
     return read_i64(ctx, slot + nc, ncells_out, true);
 }
 
@@ -470,6 +476,8 @@ extern "C" int sph_gather_sorted(sph_ctx *ctx, int64_t n, const int32_t *perm,
 #define CW 64          // lanes per wavefront = threads per block of the cell kernel
 #define MAXSEG 64      // stencil cells handled per batch (27 for d = 1)
 
+#define NN_MAXCH 16    // candidate chunks (of 64) whose hit masks the count pass keeps
+
 struct CellArgs {
     GridDev g;
     const double *drec;
@@ -479,6 +487,27 @@ struct CellArgs {
     const double *srec;
     double rs;
     int same;
+    // hit masks of the count pass: chunk c of sorted destination t at
+    // masks[c * mstride + t]; cell_ovf[w] = 1 when cell w has more than
+    // NN_MAXCH chunks (its fill recomputes the distance tests)
+    uint64_t *masks;
+    int64_t mstride;
+    int32_t *cell_ovf;
+    // per cell: the sorted source of each stencil candidate q (q <
+    // NN_MAXCH*64, cells without the overflow flag); per destination: its cell
+    int32_t *cand;
+    int32_t *dcell_of;
+    int maxch;  // chunks kept per destination (<= NN_MAXCH); cand holds 64*maxch per cell
+    // batched simulations: per-simulation grids (grids[sim_of[original
+    // index]] replaces g), the number of destination cells read on the
+    // device, and the capacity the fill checks on the device (lists that do
+    // not fit set bit 0 of *status and are not written)
+    const GridDev *grids;
+    const int32_t *sim_of;
+    const int32_t *ncell_dev;
+    int64_t nslices;
+    int64_t cap;
+    int32_t *status;
 };
 
 __device__ __forceinline__ int wave_max_i(int v)
@@ -515,6 +544,20 @@ __device__ __forceinline__ int64_t wave_bcast_i64(int64_t v, int l)
     return ((int64_t)hi << 32) | (uint32_t)lo;
 }
 
+__device__ __forceinline__ uint64_t wave_bcast_u64(uint64_t v, int l)
+{
+    const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, l);
+    const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), l);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// FILL = false: count pass.  Runs the distance tests, writes counts[t], the
+//   64-bit hit mask of every (destination, candidate chunk) pair to a.masks,
+//   the cell of every destination and the source of each candidate.  Cells
+//   whose stencil needs more than one batch or more than NN_MAXCH chunks are
+//   flagged in a.cell_ovf.
+// FILL = true: writes the SELL-64 rows of the flagged cells only (the other
+//   cells are written by k_nnps_fill_slices from the masks).
 template <bool FILL>
 __global__ void __launch_bounds__(CW)
     k_nnps_cells(CellArgs a, int32_t *__restrict__ counts, const int64_t *__restrict__ slice_off,
@@ -522,12 +565,16 @@ __global__ void __launch_bounds__(CW)
 {
     __shared__ int32_t seg_start[MAXSEG];
     __shared__ int32_t seg_end[MAXSEG];  // inclusive prefix of the segment sizes
+    __shared__ double4 dsts[CW];         // (x, y, z, radius^2) of the group's destinations
     const int lane = threadIdx.x;
     const int64_t w = sph_xcd_tile(blockIdx.x, gridDim.x);
-    if (w >= a.ncell_list) return;  // uniform: one wavefront per block
-    const GridDev &g = a.g;
+    const int64_t ncl = a.ncell_dev ? (int64_t)*a.ncell_dev : a.ncell_list;
+    if (w >= ncl) return;  // uniform: one wavefront per block
+    if (FILL && (a.cell_ovf[w] == 0 || (a.status && slice_off[a.nslices] > a.cap)))
+        return;  // written from the masks, or no room
     const int32_t key = a.dcells[w];
     const int32_t ds = a.dstart[key], de = a.dstart[key + 1];
+    const GridDev g = a.grids ? a.grids[a.sim_of[a.dperm[ds]]] : a.g;
     // the cell's coordinates, recomputed exactly as the binning did
     const double *r0 = a.drec + (int64_t)ds * SPH_REC;
     const int cx = min(max(cell_coord(r0[SPH_R_X], g.cs) - g.cmin0, 0), g.d0 - 1);
@@ -537,29 +584,29 @@ __global__ void __launch_bounds__(CW)
     // destination with a smaller reach only sees extra candidates that fail
     // its distance test, and its own stencil is a sub-box visited in order
     int dl = 0;
-    for (int32_t t = ds + lane; t < de; t += CW)
-        dl = max(dl, (int)ceil(a.drec[(int64_t)t * SPH_REC + SPH_R_H] * a.rs / g.cs));
+    for (int32_t t = ds + lane; t < de; t += CW) {
+        const double r = a.drec[(int64_t)t * SPH_REC + SPH_R_H] * a.rs / g.cs;
+        // bounded so that a non-finite h (rejected on the host, flagged in a
+        // batch) cannot overflow the stencil arithmetic; a reach above 256
+        // cells needs a cell size clamped below radius/256
+        dl = max(dl, r > 0.0 ? (r < 256.0 ? (int)ceil(r) : 256) : 0);
+    }
     const int dist = wave_max_i(dl);
     const int x0 = max(cx - dist, 0), x1 = min(cx + dist, g.d0 - 1);
     const int y0 = max(cy - dist, 0), y1 = min(cy + dist, g.d1 - 1);
     const int z0 = max(cz - dist, 0), z1 = min(cz + dist, g.d2 - 1);
     const int nx = x1 - x0 + 1, ny = y1 - y0 + 1, nz = z1 - z0 + 1;
     const int nst = nx * ny * nz;
-    const int32_t *tx = g.tab, *ty = g.tab + g.d0, *tz = g.tab + g.d0 + g.d1;
 
+    int nchunks = 0;  // chunks per destination over all stencil batches
     for (int32_t g0 = ds; g0 < de; g0 += CW) {  // destination groups of <= 64
         const int gn = min(CW, de - g0);
         const int32_t my_t = g0 + lane;  // destination owned by this lane
-        // this lane's destination, broadcast to the wavefront in the loop below
-        double mpx = 0.0, mpy = 0.0, mpz = 0.0, mr2 = -1.0;
         bool mreal = false;
         if (lane < gn) {
             const double *dr = a.drec + (int64_t)my_t * SPH_REC;
-            mpx = dr[SPH_R_X];
-            mpy = dr[SPH_R_Y];
-            mpz = dr[SPH_R_Z];
             const double radius = dr[SPH_R_H] * a.rs;
-            mr2 = radius * radius;
+            dsts[lane] = make_double4(dr[SPH_R_X], dr[SPH_R_Y], dr[SPH_R_Z], radius * radius);
             mreal = a.dperm[my_t] < a.nd_real;
         }
         const uint64_t realmask = __ballot(mreal);
@@ -569,12 +616,13 @@ __global__ void __launch_bounds__(CW)
         int32_t myself = -1;   // FILL: entry holding my own index
         int32_t mylast = -1;   // FILL: last neighbour written to my row
         if (FILL && lane < gn) myaddr = slice_off[my_t >> 6] + (my_t & 63);
+        int cidx = 0;          // chunk index over the stencil batches
         for (int b0 = 0; b0 < nst; b0 += MAXSEG) {  // stencil batches
             const int sIdx = b0 + lane;
             int32_t st = 0, cnt = 0;
             if (lane < MAXSEG && sIdx < nst) {
                 const int iz = sIdx % nz, iy = (sIdx / nz) % ny, ix = sIdx / (nz * ny);
-                const int32_t k = tx[x0 + ix] + ty[y0 + iy] + tz[z0 + iz];
+                const int32_t k = cell_key(g, x0 + ix, y0 + iy, z0 + iz);
                 st = a.sstart[k];
                 cnt = a.sstart[k + 1] - st;
             }
@@ -586,7 +634,7 @@ __global__ void __launch_bounds__(CW)
                 seg_end[lane] = incl;
             }
             __syncthreads();
-            for (int c0 = 0; c0 < total; c0 += CW) {
+            for (int c0 = 0; c0 < total; c0 += CW, ++cidx) {
                 const int q = c0 + lane;
                 const bool valid = q < total;
                 int32_t p = 0;
@@ -599,26 +647,31 @@ __global__ void __launch_bounds__(CW)
                     }
                     const int prev = lo > 0 ? seg_end[lo - 1] : 0;
                     p = seg_start[lo] + (q - prev);
+                    if (!FILL && g0 == ds && b0 == 0 && q < a.maxch * CW)
+                        a.cand[w * (a.maxch * CW) + q] = p;
                     const double *sr = a.srec + (int64_t)p * SPH_REC;
                     sx = sr[SPH_R_X];
                     sy = sr[SPH_R_Y];
                     sz = sr[SPH_R_Z];
                 }
+                uint32_t mlo = 0, mhi = 0;  // count: my destination's hit mask of this chunk
                 for (int d = 0; d < gn; ++d) {
                     if (!((realmask >> d) & 1)) continue;  // non-LocalReal: empty row
                     const int32_t t = g0 + d;
-                    const double px = wave_bcast(mpx, d), py = wave_bcast(mpy, d),
-                                 pz = wave_bcast(mpz, d), radius2 = wave_bcast(mr2, d);
+                    const double4 D = dsts[d];  // same address in every lane
                     // cPoint_distance2(src, dst), base/point.pxd:35-37
-                    const double ex = sx - px, ey = sy - py, ez = sz - pz;
+                    const double ex = sx - D.x, ey = sy - D.y, ez = sz - D.z;
                     const double d2 = ex * ex + ey * ey + ez * ez;
-                    const bool hit = valid && d2 < radius2;
+                    const bool hit = valid && d2 < D.w;
                     const uint64_t mask = __ballot(hit);
                     if (mask == 0) continue;
-                    const int nh = __popcll(mask);
                     if (!FILL) {
-                        if (lane == d) mycount += nh;
+                        if (lane == d) {
+                            mlo = (uint32_t)mask;
+                            mhi = (uint32_t)(mask >> 32);
+                        }
                     } else {
+                        const int nh = __popcll(mask);
                         const int64_t addr0 = wave_bcast_i64(myaddr, d);
                         const int32_t k0 = __builtin_amdgcn_readlane(mycount, d);
                         const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -638,17 +691,82 @@ __global__ void __launch_bounds__(CW)
                         }
                     }
                 }
+                if (!FILL) {
+                    mycount += __popc(mlo) + __popc(mhi);
+                    if (cidx < a.maxch && lane < gn)
+                        a.masks[(int64_t)cidx * a.mstride + my_t] = ((uint64_t)mhi << 32) | mlo;
+                }
             }
         }
+        nchunks = cidx;
         if (lane < gn) {
             if (!FILL) {
                 counts[my_t] = mycount;
+                a.dcell_of[my_t] = (int32_t)w;
             } else if (a.same && myself >= 0 && myself < mycount - 1) {
                 // keep the particle itself at the last index (nnps.pyx:612-619)
                 indices[myaddr + ((int64_t)myself << 6)] = mylast;
                 indices[myaddr + ((int64_t)(mycount - 1) << 6)] = my_t;
             }
         }
+    }
+    if (!FILL && lane == 0) a.cell_ovf[w] = nst > MAXSEG || nchunks > a.maxch;
+}
+
+// SELL-64 rows from the count pass's hit masks: one lane per destination,
+// one wavefront per 64-row slice, so that entry k of the 64 rows is one
+// contiguous 256-byte store.  Candidate q = 64*chunk + bit of the cell's
+// stencil is sorted source cand[cell][q].
+__global__ void __launch_bounds__(NN_THREADS)
+    k_nnps_fill_slices(CellArgs a, int64_t nd_total, int32_t *__restrict__ counts,
+                       const int64_t *__restrict__ slice_off, int32_t *__restrict__ indices)
+{
+    const int64_t t = (int64_t)sph_xcd_tile(blockIdx.x, gridDim.x) * NN_THREADS + threadIdx.x;
+    if (t >= nd_total) return;
+    if (a.status && slice_off[a.nslices] > a.cap) {
+        counts[t] = 0;  // no room (status is set): empty rows, so no reader touches the slots
+        return;
+    }
+    const int32_t n = counts[t];
+    if (n == 0) return;
+    const int32_t w = a.dcell_of[t];
+    if (a.cell_ovf[w]) return;  // written by the per-cell pass
+    int32_t *row = indices + slice_off[t >> 6] + (t & 63);
+    const int32_t *cand = a.cand + (int64_t)w * (a.maxch * CW);
+    const uint64_t *mk = a.masks + t;
+    int c = 0;
+    uint64_t m = mk[0];
+    int32_t self_k = -1, last_p = -1;
+    // FILL_UNROLL candidates per round: their table loads are in flight together
+    constexpr int FILL_UNROLL = 8;
+    for (int32_t k0 = 0; k0 < n; k0 += FILL_UNROLL) {
+        int32_t q[FILL_UNROLL];
+#pragma unroll
+        for (int j = 0; j < FILL_UNROLL; ++j) {
+            q[j] = -1;
+            if (k0 + j < n) {
+                while (m == 0 && c + 1 < a.maxch) m = mk[(int64_t)(++c) * a.mstride];
+                if (m != 0) {  // always: the masks hold exactly n bits
+                    q[j] = (c << 6) + __ffsll((unsigned long long)m) - 1;
+                    m &= m - 1;
+                }
+            }
+        }
+        int32_t p[FILL_UNROLL];
+#pragma unroll
+        for (int j = 0; j < FILL_UNROLL; ++j) p[j] = q[j] >= 0 ? cand[q[j]] : -1;
+#pragma unroll
+        for (int j = 0; j < FILL_UNROLL; ++j) {
+            if (q[j] < 0) continue;
+            row[(int64_t)(k0 + j) << 6] = p[j];
+            if (p[j] == t) self_k = k0 + j;
+            last_p = p[j];
+        }
+    }
+    if (a.same && self_k >= 0 && self_k < n - 1) {
+        // keep the particle itself at the last index (nnps.pyx:612-619)
+        row[(int64_t)self_k << 6] = last_p;
+        row[(int64_t)(n - 1) << 6] = (int32_t)t;
     }
 }
 
@@ -690,6 +808,90 @@ __global__ void __launch_bounds__(1024)
     }
 }
 
+// status / totals of a device-side build: bit 0 of *status when the lists
+// need more than cap slots; *nnz_acc += nnz (pair count kept on the device)
+__global__ void k_nnps_dev_status(const int64_t *__restrict__ slots, const int64_t *__restrict__ nnz,
+                                  int64_t cap, int32_t *__restrict__ status,
+                                  int64_t *__restrict__ nnz_acc)
+{
+    if (threadIdx.x != 0) return;
+    if (*slots > cap) *status |= 1;
+    if (nnz_acc) *nnz_acc += *nnz;
+}
+
+// Count pass, slice offsets and totals, then (after the caller's check of
+// the capacity) the fill.  Scratch: [hit masks | cell flags | candidate
+// sources | cell of destination | slice sums | scan partials].
+static int nnps_count(sph_ctx *ctx, CellArgs &a, int64_t ncell_launch, int64_t nd_total,
+                      int32_t *counts, int64_t *slice_off, int64_t **nnz_dev)
+{
+    const int64_t nslices = (nd_total + 63) / 64;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t masks_bytes = al((size_t)a.maxch * nd_total * sizeof(uint64_t));
+    const size_t ovf_bytes = al((size_t)ncell_launch * sizeof(int32_t));
+    const size_t cand_bytes = al((size_t)ncell_launch * a.maxch * CW * sizeof(int32_t));
+    const size_t dcell_bytes = al((size_t)nd_total * sizeof(int32_t));
+    const size_t sums_bytes = al((size_t)(nslices + 1) * sizeof(int64_t) + 256);
+    const size_t sums_off = masks_bytes + ovf_bytes + cand_bytes + dcell_bytes;
+    int rc = sph_ensure_scratch(ctx, sums_off + sums_bytes +
+                                         sph_scan_scratch_bytes(nslices, sizeof(int64_t)));
+    if (rc) return rc;
+    char *sc = (char *)ctx->scratch;
+    a.masks = (uint64_t *)sc;
+    a.mstride = nd_total;
+    a.cell_ovf = (int32_t *)(sc + masks_bytes);
+    a.cand = (int32_t *)(sc + masks_bytes + ovf_bytes);
+    a.dcell_of = (int32_t *)(sc + masks_bytes + ovf_bytes + cand_bytes);
+    a.nslices = nslices;
+    hipLaunchKernelGGL(k_nnps_cells<false>, dim3((unsigned)ncell_launch), dim3(CW), 0,
+                       ctx->stream, a, counts, (const int64_t *)nullptr, (int32_t *)nullptr);
+    SPH_LAUNCH_CHECK(ctx, "k_nnps_cells<count>");
+    // slice widths -> offsets (scan); slice totals -> entry total
+    int64_t *slice_sum = (int64_t *)(sc + sums_off);
+    *nnz_dev = slice_sum + nslices;
+    hipLaunchKernelGGL(k_slices, dim3(sph_blocks(nslices * 64, NN_THREADS)), dim3(NN_THREADS), 0,
+                       ctx->stream, counts, nd_total, nslices, slice_off, slice_sum);
+    SPH_LAUNCH_CHECK(ctx, "k_slices");
+    hipLaunchKernelGGL(k_sum_i64, dim3(1), dim3(1024), 0, ctx->stream, slice_sum, nslices,
+                       *nnz_dev);
+    SPH_LAUNCH_CHECK(ctx, "k_sum_i64");
+    return sph_scan_exclusive_i64_at(ctx, slice_off, nslices, sums_off + sums_bytes);
+}
+
+static int nnps_fill(sph_ctx *ctx, const CellArgs &a, int64_t ncell_launch, int64_t nd_total,
+                     int32_t *counts, const int64_t *slice_off, int32_t *indices)
+{
+    hipLaunchKernelGGL(k_nnps_fill_slices, dim3(sph_blocks(nd_total, NN_THREADS)),
+                       dim3(NN_THREADS), 0, ctx->stream, a, nd_total, counts, slice_off, indices);
+    SPH_LAUNCH_CHECK(ctx, "k_nnps_fill_slices");
+    hipLaunchKernelGGL(k_nnps_cells<true>, dim3((unsigned)ncell_launch), dim3(CW), 0, ctx->stream,
+                       a, (int32_t *)counts, slice_off, indices);
+    SPH_LAUNCH_CHECK(ctx, "k_nnps_cells<fill>");
+    return SPH_OK;
+}
+
+static int nnps_clear(sph_ctx *ctx, int64_t nd_total, int32_t *counts, int64_t *slice_off)
+{
+    const int64_t nslices = (nd_total + 63) / 64;
+    // every row starts empty: non-real rows and cells without candidates stay 0
+    hipLaunchKernelGGL(k_fill_i32, dim3(sph_blocks(nd_total + 1, NN_THREADS)), dim3(NN_THREADS),
+                       0, ctx->stream, counts, nd_total, 0);
+    SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
+    hipLaunchKernelGGL(k_fill_i32, dim3(sph_blocks(2 * (nslices + 1), NN_THREADS)),
+                       dim3(NN_THREADS), 0, ctx->stream, (int32_t *)slice_off,
+                       2 * (nslices + 1), 0);
+    SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
+    return SPH_OK;
+}
+
+static CellArgs cell_args_base()
+{
+    CellArgs a;
+    memset(&a, 0, sizeof(a));
+    a.maxch = NN_MAXCH;
+    return a;
+}
+
 extern "C" int sph_nnps_build(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
                               const int32_t *d_perm, const int32_t *d_cell_start,
                               const int32_t *d_cells, int64_t nd_cells, int64_t nd_total,
@@ -712,16 +914,10 @@ extern "C" int sph_nnps_build(sph_ctx *ctx, const sph_grid *grid, const double *
     *slots_out = 0;
     *nnz_out = 0;
     const int64_t nslices = (nd_total + 63) / 64;
-    // every row starts empty: non-real rows and cells without candidates stay 0
-    hipLaunchKernelGGL(k_fill_i32, dim3(sph_blocks(nd_total + 1, NN_THREADS)), dim3(NN_THREADS),
-                       0, ctx->stream, counts, nd_total, 0);
-    SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
-    hipLaunchKernelGGL(k_fill_i32, dim3(sph_blocks(2 * (nslices + 1), NN_THREADS)),
-                       dim3(NN_THREADS), 0, ctx->stream, (int32_t *)slice_off,
-                       2 * (nslices + 1), 0);
-    SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
+    rc = nnps_clear(ctx, nd_total, counts, slice_off);
+    if (rc) return rc;
     if (nd_total == 0 || ns == 0 || nd_real == 0 || nd_cells == 0) return SPH_OK;
-    CellArgs a;
+    CellArgs a = cell_args_base();
     a.g = make_griddev(ctx, grid);
     a.drec = d_rec;
     a.dperm = d_perm;
@@ -733,21 +929,8 @@ extern "C" int sph_nnps_build(sph_ctx *ctx, const sph_grid *grid, const double *
     a.srec = s_rec;
     a.rs = radius_scale;
     a.same = same_array;
-    hipLaunchKernelGGL(k_nnps_cells<false>, dim3((unsigned)nd_cells), dim3(CW), 0, ctx->stream, a,
-                       counts, (const int64_t *)nullptr, (int32_t *)nullptr);
-    SPH_LAUNCH_CHECK(ctx, "k_nnps_cells<count>");
-    // slice widths -> offsets (scan); slice totals -> entry total
-    const size_t sums_bytes = ((size_t)(nslices + 1) * sizeof(int64_t) + 511) & ~(size_t)255;
-    rc = sph_ensure_scratch(ctx, sums_bytes + sph_scan_scratch_bytes(nslices, sizeof(int64_t)));
-    if (rc) return rc;
-    int64_t *slice_sum = (int64_t *)ctx->scratch;
-    int64_t *nnz_dev = slice_sum + nslices;
-    hipLaunchKernelGGL(k_slices, dim3(sph_blocks(nslices * 64, NN_THREADS)), dim3(NN_THREADS), 0,
-                       ctx->stream, counts, nd_total, nslices, slice_off, slice_sum);
-    SPH_LAUNCH_CHECK(ctx, "k_slices");
-    hipLaunchKernelGGL(k_sum_i64, dim3(1), dim3(1024), 0, ctx->stream, slice_sum, nslices, nnz_dev);
-    SPH_LAUNCH_CHECK(ctx, "k_sum_i64");
-    rc = sph_scan_exclusive_i64_at(ctx, slice_off, nslices, sums_bytes);
+    int64_t *nnz_dev = nullptr;
+    rc = nnps_count(ctx, a, nd_cells, nd_total, counts, slice_off, &nnz_dev);
     if (rc) return rc;
     rc = sph_ensure_pinned(ctx, 2 * sizeof(int64_t));
     if (rc) return rc;
@@ -764,10 +947,45 @@ extern "C" int sph_nnps_build(sph_ctx *ctx, const sph_grid *grid, const double *
                       (long long)slots);
         return SPH_ENEEDCAP;
     }
-    hipLaunchKernelGGL(k_nnps_cells<true>, dim3((unsigned)nd_cells), dim3(CW), 0, ctx->stream, a,
-                       counts, (const int64_t *)slice_off, indices);
-    SPH_LAUNCH_CHECK(ctx, "k_nnps_cells<fill>");
-    return SPH_OK;
+    return nnps_fill(ctx, a, nd_cells, nd_total, counts, slice_off, indices);
+}
+
+int sph_nnps_build_batch(sph_ctx *ctx, const GridDev *grids, const int32_t *sim_of,
+                         const double *rec, const int32_t *perm, const int32_t *cell_start,
+                         const int32_t *cells, const int32_t *ncell_dev, int64_t max_cells,
+                         int64_t n, double radius_scale, int maxch, int32_t *counts,
+                         int64_t *slice_off, int32_t *indices, int64_t cap, int32_t *status,
+                         int64_t *nnz_acc)
+{
+    SPH_REQUIRE(ctx, maxch >= 1 && maxch <= NN_MAXCH, "sph_nnps_build_batch: bad maxch");
+    int rc = nnps_clear(ctx, n, counts, slice_off);
+    if (rc) return rc;
+    if (n == 0 || max_cells == 0) return SPH_OK;
+    CellArgs a = cell_args_base();
+    a.grids = grids;
+    a.sim_of = sim_of;
+    a.ncell_dev = ncell_dev;
+    a.drec = rec;
+    a.dperm = perm;
+    a.dstart = cell_start;
+    a.dcells = cells;
+    a.ncell_list = max_cells;
+    a.nd_real = n;
+    a.sstart = cell_start;
+    a.srec = rec;
+    a.rs = radius_scale;
+    a.same = 1;
+    a.maxch = maxch;
+    a.cap = cap;
+    a.status = status;
+    int64_t *nnz_dev = nullptr;
+    rc = nnps_count(ctx, a, max_cells, n, counts, slice_off, &nnz_dev);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_nnps_dev_status, dim3(1), dim3(64), 0, ctx->stream,
+                       (const int64_t *)(slice_off + a.nslices), (const int64_t *)nnz_dev, cap,
+                       status, nnz_acc);
+    SPH_LAUNCH_CHECK(ctx, "k_nnps_dev_status");
+    return nnps_fill(ctx, a, max_cells, n, counts, slice_off, indices);
 }
 
 // ---------------------------------------------------------------------------

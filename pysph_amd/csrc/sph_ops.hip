@@ -19,24 +19,26 @@ __device__ __forceinline__ const double2 *rec2(const double *rec, int64_t j)
 // ---------------------------------------------------------------------------
 // SPHRho
 // ---------------------------------------------------------------------------
-// one neighbour term of SPHRho (density_funcs.pyx:41-73)
+// one neighbour term of SPHRho (density_funcs.pyx:41-73); sxy = (x, y) and
+// szm = (z, m) of the source record, hb its h (not read when UH)
+template <bool UH>
 __device__ __forceinline__ double rho_term(int dim, double xa, double ya, double za, double ha,
-                                           double facA, double2 sxy, double2 szh, double2 smr)
+                                           double facA, double2 sxy, double2 szm, double hb)
 {
-    const double hb = szh.y;
     // with h_b == h_a, 0.5*(h_a+h_b) == h_a exactly and h_dim(h)*sigma is the
     // destination's value: reuse it instead of dividing again
     double hab, fac;
-    if (hb == ha) {
+    if (UH || hb == ha) {
         hab = ha;
         fac = facA;
     } else {
         hab = 0.5 * (ha + hb);
         fac = sph_h_dim(hab, dim) * sph_cubic_fac1(dim);
     }
-    return sph_cubic_w_fac(xa - sxy.x, ya - sxy.y, za - szh.x, hab, fac) * smr.x;
+    return sph_cubic_w_fac(xa - sxy.x, ya - sxy.y, za - szm.x, hab, fac) * szm.y;
 }
 
+template <bool UH>
 __global__ void __launch_bounds__(OP_THREADS)
     k_summation_density(int dim, int64_t nd_total, int64_t nd_real,
                         const int32_t *__restrict__ dperm, const int32_t *__restrict__ counts,
@@ -55,8 +57,8 @@ __global__ void __launch_bounds__(OP_THREADS)
         return;
     }
     const double2 *dr = rec2(drec, t);
-    const double2 dxy = dr[0], dzh = dr[1];
-    const double xa = dxy.x, ya = dxy.y, za = dzh.x, ha = dzh.y;
+    const double2 dxy = dr[0], dzm = dr[1];
+    const double xa = dxy.x, ya = dxy.y, za = dzm.x, ha = drec[t * SPH_REC + SPH_R_H];
     const double facA = sph_h_dim(ha, dim) * sph_cubic_fac1(dim);
     double nr = 0.0;
     // SELL-64 row: entry k at row[64*k]; each load is one coalesced line for
@@ -72,18 +74,27 @@ __global__ void __launch_bounds__(OP_THREADS)
                       j3 = __builtin_nontemporal_load(row + ((int64_t)(k + 3) << 6));
         const double2 *s0 = rec2(srec, j0), *s1 = rec2(srec, j1), *s2 = rec2(srec, j2),
                       *s3 = rec2(srec, j3);
-        const double2 a0 = s0[0], b0 = s0[1], c0 = s0[2];
-        const double2 a1 = s1[0], b1 = s1[1], c1 = s1[2];
-        const double2 a2 = s2[0], b2 = s2[1], c2 = s2[2];
-        const double2 a3 = s3[0], b3 = s3[1], c3 = s3[2];
-        nr += rho_term(dim, xa, ya, za, ha, facA, a0, b0, c0);
-        nr += rho_term(dim, xa, ya, za, ha, facA, a1, b1, c1);
-        nr += rho_term(dim, xa, ya, za, ha, facA, a2, b2, c2);
-        nr += rho_term(dim, xa, ya, za, ha, facA, a3, b3, c3);
+        const double2 a0 = s0[0], b0 = s0[1];
+        const double2 a1 = s1[0], b1 = s1[1];
+        const double2 a2 = s2[0], b2 = s2[1];
+        const double2 a3 = s3[0], b3 = s3[1];
+        double h0 = ha, h1 = ha, h2 = ha, h3 = ha;
+        if (!UH) {
+            h0 = srec[(int64_t)j0 * SPH_REC + SPH_R_H];
+            h1 = srec[(int64_t)j1 * SPH_REC + SPH_R_H];
+            h2 = srec[(int64_t)j2 * SPH_REC + SPH_R_H];
+            h3 = srec[(int64_t)j3 * SPH_REC + SPH_R_H];
+        }
+        nr += rho_term<UH>(dim, xa, ya, za, ha, facA, a0, b0, h0);
+        nr += rho_term<UH>(dim, xa, ya, za, ha, facA, a1, b1, h1);
+        nr += rho_term<UH>(dim, xa, ya, za, ha, facA, a2, b2, h2);
+        nr += rho_term<UH>(dim, xa, ya, za, ha, facA, a3, b3, h3);
     }
     for (; k < e; ++k) {
-        const double2 *sr = rec2(srec, __builtin_nontemporal_load(row + ((int64_t)k << 6)));
-        nr += rho_term(dim, xa, ya, za, ha, facA, sr[0], sr[1], sr[2]);
+        const int32_t j = __builtin_nontemporal_load(row + ((int64_t)k << 6));
+        const double2 *sr = rec2(srec, j);
+        const double hb = UH ? ha : srec[(int64_t)j * SPH_REC + SPH_R_H];
+        nr += rho_term<UH>(dim, xa, ya, za, ha, facA, sr[0], sr[1], hb);
     }
     const double base = accumulate ? (out_s ? out_s[t * out_stride] : rho[i]) : 0.0;
     const double val = base + nr;
@@ -94,9 +105,9 @@ __global__ void __launch_bounds__(OP_THREADS)
 extern "C" int sph_summation_density(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
                                      const int32_t *d_perm, const int32_t *counts,
                                      const int64_t *slice_off, const int32_t *indices,
-                                     const double *d_rec,
-                                     const double *s_rec, int accumulate, double *rho,
-                                     double *out_s, int64_t out_s_stride)
+                                     const double *d_rec, const double *s_rec, int uniform_h,
+                                     int accumulate, double *rho, double *out_s,
+                                     int64_t out_s_stride)
 {
     if (!ctx) return SPH_EINVAL;
     SPH_REQUIRE(ctx, dim >= 1 && dim <= 3, "sph_summation_density: bad dim %d", dim);
@@ -105,7 +116,8 @@ extern "C" int sph_summation_density(sph_ctx *ctx, int dim, int64_t nd_total, in
     SPH_REQUIRE(ctx, rho && d_perm && counts && slice_off && d_rec &&
                          (!out_s || out_s_stride >= 1),
                 "sph_summation_density: null argument");
-    hipLaunchKernelGGL(k_summation_density, dim3(sph_blocks(nd_total, OP_THREADS)),
+    auto kern = uniform_h ? k_summation_density<true> : k_summation_density<false>;
+    hipLaunchKernelGGL(kern, dim3(sph_blocks(nd_total, OP_THREADS)),
                        dim3(OP_THREADS), 0, ctx->stream, dim, nd_total, nd_real, d_perm, counts,
                        slice_off, indices, d_rec, s_rec, accumulate, rho, out_s, out_s_stride);
     SPH_LAUNCH_CHECK(ctx, "k_summation_density");
@@ -170,46 +182,47 @@ struct MomDest {
     double xa, ya, za, ha, facA, rhoa, ca, Pa, ua, va, wa, rhoa2;
 };
 
-// one neighbour term of MomentumEquation (pressure_funcs.pyx:252-391)
+// one neighbour term of MomentumEquation (pressure_funcs.pyx:252-391).
+// Source record slots: s0 = (x, y), s1 = (z, m), s2 = (u, v), s3 = (w, P/rho^2),
+// s4 = (rho, cs); hb its h (not read when UH).
+template <bool UH>
 __device__ __forceinline__ void mom_term(int dim, const MomDest &D, double alpha, double beta,
                                          double eta, const double2 s0, const double2 s1,
                                          const double2 s2, const double2 s3, const double2 s4,
-                                         const double2 s5, double &dtf, double &nr0, double &nr1,
+                                         double hb, double &dtf, double &nr0, double &nr1,
                                          double &nr2)
 {
-    const double hb = s1.y;
     double hab, fac;
-    if (hb == D.ha) {  // 0.5*(h_a+h_a) == h_a: reuse the destination's h_dim(h)*sigma
+    if (UH || hb == D.ha) {  // 0.5*(h_a+h_a) == h_a: reuse the destination's h_dim(h)*sigma
         hab = D.ha;
         fac = D.facA;
     } else {
         hab = 0.5 * (D.ha + hb);
         fac = sph_h_dim(hab, dim) * sph_cubic_fac1(dim);
     }
-    const double cb = s3.x;
     const double rabx = D.xa - s0.x;
     const double raby = D.ya - s0.y;
     const double rabz = D.za - s1.x;
-    const double vabx = D.ua - s4.x;
-    const double vaby = D.va - s4.y;
-    const double vabz = D.wa - s5.x;
+    const double vabx = D.ua - s2.x;
+    const double vaby = D.va - s2.y;
+    const double vabz = D.wa - s3.x;
     const double dot = vabx * rabx + vaby * raby + vabz * rabz;
     const double rab2 = rabx * rabx + raby * raby + rabz * rabz;
     const double den = rab2 + eta * eta * hab * hab;
     const double q = hab * dot / den;  // the same quotient as mu below
     const double f = fabs(q);
-    if (f > dtf) dtf = f;
-    const double rhob = s2.y, mb = s2.x;
+    dtf = f > dtf ? f : dtf;
+    const double mb = s1.y, rhob = s4.x, cb = s4.y;
     // Pb * (1.0/(rhob*rhob)), computed once per source particle (k_prho2)
-    double tmp = D.Pa * D.rhoa2 + s5.y;
-    double piab = 0;
-    if (dot < 0) {
-        const double cab = 0.5 * (D.ca + cb);
-        const double rhoab = 0.5 * (D.rhoa + rhob);
-        const double mu = q;  // mu = hab*dot; mu /= (rab2 + eta*eta*hab*hab)
-        piab = -alpha * cab * mu + beta * mu * mu;
-        piab /= rhoab;
-    }
+    double tmp = D.Pa * D.rhoa2 + s3.y;
+    // Monaghan viscosity, applied when dot < 0; evaluated for every pair and
+    // selected, which costs less than a divergent branch in a wavefront
+    const double cab = 0.5 * (D.ca + cb);
+    const double rhoab = 0.5 * (D.rhoa + rhob);
+    const double mu = q;  // mu = hab*dot; mu /= (rab2 + eta*eta*hab*hab)
+    double pi_v = -alpha * cab * mu + beta * mu * mu;
+    pi_v /= rhoab;
+    const double piab = dot < 0 ? pi_v : 0.0;
     tmp += piab;
     tmp *= -mb;
     const double g = sph_cubic_gfac_fac(rabx, raby, rabz, hab, fac);
@@ -218,6 +231,7 @@ __device__ __forceinline__ void mom_term(int dim, const MomDest &D, double alpha
     nr2 += tmp * (rabz * g);
 }
 
+template <bool UH>
 __global__ void __launch_bounds__(OP_THREADS)
     k_momentum(int dim, int64_t nd_total, int64_t nd_real, const int32_t *__restrict__ dperm,
                const int32_t *__restrict__ counts, const int64_t *__restrict__ slice_off,
@@ -242,14 +256,14 @@ __global__ void __launch_bounds__(OP_THREADS)
     D.xa = d0.x;
     D.ya = d0.y;
     D.za = d1.x;
-    D.ha = d1.y;
+    D.ua = d2.x;
+    D.va = d2.y;
+    D.wa = d3.x;
+    D.rhoa = d4.x;
+    D.ca = d4.y;
+    D.ha = d5.x;
+    D.Pa = d5.y;
     D.facA = sph_h_dim(D.ha, dim) * sph_cubic_fac1(dim);
-    D.rhoa = d2.y;
-    D.ca = d3.x;
-    D.Pa = d3.y;
-    D.ua = d4.x;
-    D.va = d4.y;
-    D.wa = d5.x;
     D.rhoa2 = 1.0 / (D.rhoa * D.rhoa);
     double dtf = dt_fac[i];
     double nr0 = 0.0, nr1 = 0.0, nr2 = 0.0;
@@ -257,17 +271,22 @@ __global__ void __launch_bounds__(OP_THREADS)
     const int32_t e = counts[t];
     int32_t k = 0;
     for (; k + 2 <= e; k += 2) {  // two neighbours' loads in flight
-        const double2 *p = rec2(srec, __builtin_nontemporal_load(row + ((int64_t)k << 6))),
-                      *q = rec2(srec, __builtin_nontemporal_load(row + ((int64_t)(k + 1) << 6)));
-        const double2 p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3], p4 = p[4], p5 = p[5];
-        const double2 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5];
-        mom_term(dim, D, alpha, beta, eta, p0, p1, p2, p3, p4, p5, dtf, nr0, nr1, nr2);
-        mom_term(dim, D, alpha, beta, eta, q0, q1, q2, q3, q4, q5, dtf, nr0, nr1, nr2);
+        const int32_t jp = __builtin_nontemporal_load(row + ((int64_t)k << 6)),
+                      jq = __builtin_nontemporal_load(row + ((int64_t)(k + 1) << 6));
+        const double2 *p = rec2(srec, jp), *q = rec2(srec, jq);
+        const double2 p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3], p4 = p[4];
+        const double2 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4];
+        const double hp = UH ? D.ha : srec[(int64_t)jp * SPH_REC + SPH_R_H];
+        const double hq = UH ? D.ha : srec[(int64_t)jq * SPH_REC + SPH_R_H];
+        mom_term<UH>(dim, D, alpha, beta, eta, p0, p1, p2, p3, p4, hp, dtf, nr0, nr1, nr2);
+        mom_term<UH>(dim, D, alpha, beta, eta, q0, q1, q2, q3, q4, hq, dtf, nr0, nr1, nr2);
     }
     if (k < e) {
-        const double2 *p = rec2(srec, __builtin_nontemporal_load(row + ((int64_t)k << 6)));
-        mom_term(dim, D, alpha, beta, eta, p[0], p[1], p[2], p[3], p[4], p[5], dtf, nr0, nr1,
-                 nr2);
+        const int32_t jp = __builtin_nontemporal_load(row + ((int64_t)k << 6));
+        const double2 *p = rec2(srec, jp);
+        const double hp = UH ? D.ha : srec[(int64_t)jp * SPH_REC + SPH_R_H];
+        mom_term<UH>(dim, D, alpha, beta, eta, p[0], p[1], p[2], p[3], p[4], hp, dtf, nr0, nr1,
+                     nr2);
     }
     dt_fac[i] = dtf;
     ax[i] = (accumulate ? ax[i] : 0.0) + nr0;
@@ -290,7 +309,8 @@ extern "C" int sph_momentum_av(sph_ctx *ctx, int dim, int64_t nd_total, int64_t 
                                const int32_t *d_perm, const int32_t *counts,
                                const int64_t *slice_off, const int32_t *indices,
                                const double *d_rec, double *s_rec,
-                               int64_t ns, double alpha, double beta, double eta, int accumulate,
+                               int64_t ns, int uniform_h, double alpha, double beta, double eta,
+                               int accumulate,
                                double *ax, double *ay, double *az, double *dt_fac)
 {
     if (!ctx) return SPH_EINVAL;
@@ -305,7 +325,8 @@ extern "C" int sph_momentum_av(sph_ctx *ctx, int dim, int64_t nd_total, int64_t 
                            ctx->stream, ns, s_rec);
         SPH_LAUNCH_CHECK(ctx, "k_prho2");
     }
-    hipLaunchKernelGGL(k_momentum, dim3(sph_blocks(nd_total, OP_THREADS)), dim3(OP_THREADS), 0,
+    auto kern = uniform_h ? k_momentum<true> : k_momentum<false>;
+    hipLaunchKernelGGL(kern, dim3(sph_blocks(nd_total, OP_THREADS)), dim3(OP_THREADS), 0,
                        ctx->stream, dim, nd_total, nd_real, d_perm, counts, slice_off, indices,
                        d_rec, s_rec, alpha, beta, eta, accumulate, ax, ay, az, dt_fac);
     SPH_LAUNCH_CHECK(ctx, "k_momentum");

@@ -61,7 +61,8 @@ class SPHRho(CSPHFunctionParticle):
                 ctx.handle, kernel.dim, self.dest.get_number_of_particles(),
                 self.dest.num_real_particles, p(cm.bins[self.dest.name].perm), p(cnt), p(soff),
                 p(idx),
-                p(drec), p(srec), int(bool(accumulate)), p(output1), out_s, REC))
+                p(drec), p(srec), int(cm.uniform_h()), int(bool(accumulate)), p(output1), out_s,
+                REC))
         if name:
             cm.written(self.dest, name, out_s is not None)
 
@@ -166,7 +167,8 @@ class MomentumEquation(SPHFunctionParticle):
                 ctx.handle, kernel.dim, self.dest.get_number_of_particles(),
                 self.dest.num_real_particles, p(cm.bins[self.dest.name].perm), p(cnt), p(soff),
                 p(idx),
-                p(drec), p(srec), cm.bins[self.source.name].n, self.alpha, self.beta, self.eta,
+                p(drec), p(srec), cm.bins[self.source.name].n, int(cm.uniform_h()), self.alpha,
+                self.beta, self.eta,
                 int(bool(accumulate)), p(output1), p(output2), p(output3), p(self.d('dt_fac'))))
         for name in (names or ()):
             if name:

@@ -234,6 +234,41 @@ def test_momentum_lattice_vs_oracle(gpu):
     assert max(errs) <= TOL
 
 
+def test_density_momentum_variable_h_vs_oracle(gpu):
+    """Per-particle h (the kernels read h_b and form 0.5*(h_a+h_b))."""
+    base, sph, _ = _api()
+    from pysph_amd.solver.generators import jittered_lattice_3d
+    rng = np.random.default_rng(13)
+    d = _random_state(jittered_lattice_3d(n=20, seed=5, hdx=1.3), rng)
+    d["h"] = d["h"] * rng.uniform(0.8, 1.25, len(d["x"]))
+    pa = base.get_particle_array(name="fluid", **d)
+    particles = base.Particles([pa])
+    assert not particles.cell_manager.uniform_h()
+    kernel = base.CubicSplineKernel(dim=3)
+    rho_f = sph.SPHRho.withargs().get_func(pa, pa)
+    sph.SPHCalc(particles, [pa], pa, kernel, [rho_f], ['_tmpx']).sph('_tmpx')
+    mom = sph.MomentumEquation.withargs(alpha=0.1, beta=0.2).get_func(pa, pa)
+    calc = sph.SPHCalc(particles, [pa], pa, kernel, [mom], ['_tmpx', '_tmpy', '_tmpz'],
+                       integrates=True)
+    rho_gpu = pa.get('_tmpx')
+    for k in ('_tmpx', '_tmpy', '_tmpz'):
+        pa.get_carray(k).zero_()
+    calc.sph('_tmpx', '_tmpy', '_tmpz')
+    full = {k: pa.get(k) for k in ("x", "y", "z", "h", "m", "rho", "p", "u", "v", "w", "cs")}
+    off, idx = cpu_ref.nnps(full, full, 2.0, particles.cell_manager.cell_size, True, nthreads=8)
+    rho = np.zeros(len(d["x"]))
+    cpu_ref.summation_density(3, full, full, off, idx, rho, nthreads=8)
+    ax, ay, az, dtf = (np.zeros(len(d["x"])) for _ in range(4))
+    cpu_ref.momentum(3, full, full, off, idx, 0.1, 0.2, 0.1, ax, ay, az, dtf, nthreads=8)
+    errs = [rel_err(rho_gpu, rho)] + [
+        rel_err(pa.get(k), r) for k, r in (('_tmpx', ax), ('_tmpy', ay), ('_tmpz', az),
+                                           ('dt_fac', dtf))]
+    print("variable h rel err rho ax ay az dt_fac:", ["%.3g" % e for e in errs],
+          "bit-equal rho %.4f ax %.4f" % (bit_equal_frac(rho_gpu, rho),
+                                          bit_equal_frac(pa.get('_tmpx'), ax)))
+    assert max(errs) <= TOL
+
+
 def test_momentum_conservation(gpu):
     """Pairwise antisymmetry: sum_a m_a a_a = 0 for constant h, no body force."""
     base, sph, _ = _api()

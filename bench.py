@@ -212,9 +212,11 @@ def run_lattice(args):
 
 def run_batch(args):
     """256 independent 2D elliptical drops (radius U[0.4, 2], dx from
-    {0.01, 0.02, 0.04}, velocity scale U[50, 150], seeded), RK2, dt = 1e-5,
-    advanced together by captured HIP graphs of --graph-steps steps; compared
-    with running the same drops one after another through the plain path."""
+    {0.01, 0.02, 0.04}, velocity scale U[50, 150], seeded), RK2 with each
+    drop's time step 1e-5*min(1, dx/0.025) (the CFL number of the dx = 0.025
+    drop), advanced together by captured HIP graphs of --graph-steps steps;
+    compared with running the same drops one after another through the plain
+    path."""
     import torch
     from pysph_amd.batch import BatchedSimulation, batched_drops
     from pysph_amd.solver.configs import drop_solver
@@ -223,10 +225,10 @@ def run_batch(args):
     def progress(msg):
         print(f"[batch {time.perf_counter() - t_start:7.2f}s] {msg}", file=sys.stderr, flush=True)
 
-    dt = 1e-5
     sims, prm = batched_drops(nsims=args.nsims)
     progress(f"{len(sims)} drops generated, {sum(len(d['x']) for d in sims)} particles")
-    b = BatchedSimulation(sims)
+    b = BatchedSimulation(sims, dt_per_sim=prm["dt"])
+    dt = 1.0  # multiplies every drop's own time step
     progress(f"packed: key_cap {b.key_cap} max_cells {b.max_cells} list_cap {b.list_cap} "
              f"workspace {b.work.numel() / 2**20:.0f} MiB")
     b.step(dt)
@@ -260,11 +262,12 @@ def run_batch(args):
     seq_steps = 2
     for k in pick:
         solver, particles, _ = drop_solver(data=dict(sims[k]), dx=float(prm["dx"][k]))
-        solver.integrator.integrate(dt)          # first build, allocations
+        dtk = float(prm["dt"][k])
+        solver.integrator.integrate(dtk)          # first build, allocations
         torch.cuda.synchronize()
         ta = time.perf_counter()
         for _ in range(seq_steps):
-            solver.integrator.integrate(dt)
+            solver.integrator.integrate(dtk)
         torch.cuda.synchronize()
         seq_time += (time.perf_counter() - ta) / seq_steps
         progress(f"sequential drop {k}: {len(sims[k]['x'])} particles")
@@ -276,8 +279,9 @@ def run_batch(args):
         "unit": "steps/s (all drops advanced one RK2 step)",
         "ms_per_step": 1e3 * elapsed / steps,
         "dtype": "f64",
-        "config": {"workload": f"{len(sims)} independent 2D elliptical drops, RK2, dt 1e-5, FP64, "
-                               f"captured HIP graphs of {gs} steps",
+        "config": {"workload": f"{len(sims)} independent 2D elliptical drops, RK2, FP64, "
+                               f"dt 1e-5*min(1, dx/0.025) per drop, captured HIP graphs of "
+                               f"{gs} steps",
                    "particles": b.n, "drops": len(sims),
                    "smallest_drop": int(min(b.counts)), "largest_drop": int(max(b.counts)),
                    "pair_interactions_per_s": pairs / elapsed,

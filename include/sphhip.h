@@ -260,7 +260,8 @@ int sph_fill(sph_ctx *ctx, int64_t n, double value, double *dst);
  * The step never synchronises with the host: capacities bound every size and
  * violations set bits of *status (1: neighbour slots > list_cap, 2: cell keys
  * > key_cap, 4: non-finite coordinate or h, 8: grid too large, 16: non-empty
- * cells > max_cells); a step with a nonzero status is void.  *pairs
+ * cells > max_cells, 32: reverse mode over particles of unequal h); a step
+ * with a nonzero status is void.  *pairs
  * accumulates the list entries of every build (each build's lists serve two
  * operators). */
 typedef struct sph_batch_desc {
@@ -272,6 +273,8 @@ typedef struct sph_batch_desc {
     double *x0, *y0, *z0, *u0, *v0, *w0; /* stage initial values */
     double *ax, *ay, *az, *au, *av, *aw; /* accelerations of x, y, z, u, v, w */
     double *dt_fac;
+    const double *dt_sim; /* nsims time steps, or NULL; when set, the dt of a
+                           * step or stage call multiplies dt_sim[s] */
     double co, ro, gamma, alpha, beta, eta, radius_scale;
     int64_t key_cap;   /* cells of all grids together */
     int64_t max_cells; /* non-empty cells */
@@ -293,6 +296,30 @@ int sph_batch_graph_create(sph_ctx *ctx, const sph_batch_desc *d, double dt, int
                            void **graph);
 int sph_batch_graph_launch(sph_ctx *ctx, void *graph);
 int sph_batch_graph_destroy(sph_ctx *ctx, void *graph);
+
+/* The pieces of a step, for callers that drive the stages themselves (the
+ * reverse mode recomputes them): x0 <- x etc. (Integrator.save_initial_arrays,
+ * solver/integrator.py:262-285); one evaluation at the current positions
+ * (binning, lists, SPHRho, Tait, MomentumEquation, PositionStepping), which
+ * leaves the lists and sorted records in the workspace; x <- x0 + a*dt for
+ * x y (z) u v (w) (Integrator.step, :340-367). */
+int sph_batch_save_initial(sph_ctx *ctx, const sph_batch_desc *d);
+int sph_batch_eval(sph_ctx *ctx, const sph_batch_desc *d);
+int sph_batch_stage(sph_ctx *ctx, const sph_batch_desc *d, double dt);
+
+/* Reverse mode of the evaluation last run by sph_batch_eval on the same
+ * workspace, over its frozen neighbour lists.  With a = the MomentumEquation
+ * accelerations (densities from SPHRho, pressures and sound speeds from
+ * Tait) and abar their cotangent, writes xbar = (da/dx)^T abar,
+ * ubar = (da/du)^T abar (the dx/dt = u part is the caller's) and
+ * *alpha_bar = (da/dalpha)^T abar (device scalar).  Every simulation must
+ * have one h for all its particles (then the lists are symmetric and each
+ * particle gathers the adjoint of every pair term from its own row, with no
+ * atomics); status |= 32 otherwise.  No reference counterpart: the reference
+ * has no derivatives. */
+int sph_batch_vjp(sph_ctx *ctx, const sph_batch_desc *d, const double *abar_u,
+                  const double *abar_v, const double *abar_w, double *xbar, double *ybar,
+                  double *zbar, double *ubar, double *vbar, double *wbar, double *alpha_bar);
 
 #ifdef __cplusplus
 }

@@ -59,7 +59,8 @@ class BatchDesc(ctypes.Structure):
                  ("sim_off", _vp), ("sim_of", _vp)]
                 + [(k, _vp) for k in ("x", "y", "z", "u", "v", "w", "h", "m", "rho", "p", "cs",
                                       "x0", "y0", "z0", "u0", "v0", "w0",
-                                      "ax", "ay", "az", "au", "av", "aw", "dt_fac")]
+                                      "ax", "ay", "az", "au", "av", "aw", "dt_fac",
+                                      "dt_sim")]
                 + [(k, _dbl) for k in ("co", "ro", "gamma", "alpha", "beta", "eta",
                                        "radius_scale")]
                 + [("key_cap", _i64), ("max_cells", _i64), ("list_cap", _i64),
@@ -101,6 +102,10 @@ _SIGNATURES = {
     "sph_batch_graph_create": (_i32, [_vp, _vp, _dbl, _i32, ctypes.POINTER(_vp)]),
     "sph_batch_graph_launch": (_i32, [_vp, _vp]),
     "sph_batch_graph_destroy": (_i32, [_vp, _vp]),
+    "sph_batch_save_initial": (_i32, [_vp, _vp]),
+    "sph_batch_eval": (_i32, [_vp, _vp]),
+    "sph_batch_stage": (_i32, [_vp, _vp, _dbl]),
+    "sph_batch_vjp": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sph_fill": (_i32, [_vp, _i64, _dbl, _vp]),
 }
 

@@ -28,7 +28,8 @@ _STATE = ("x", "y", "z", "u", "v", "w", "h", "m", "rho", "p", "cs",
 
 STATUS_BITS = {1: "neighbour-list slots exceed list_cap", 2: "cell keys exceed key_cap",
                4: "non-finite coordinate or h", 8: "grid too large",
-               16: "non-empty cells exceed max_cells"}
+               16: "non-empty cells exceed max_cells",
+               32: "reverse mode needs one h per simulation"}
 
 
 class BatchError(RuntimeError):
@@ -40,11 +41,15 @@ def _initial_cells(sims, radius_scale):
     total = nonempty = 0
     for d in sims:
         h = np.asarray(d["h"], dtype=np.float64)
-        cs = radius_scale * h.max()
+        pts = [np.asarray(d.get(k, np.zeros(len(d["x"]))), dtype=np.float64)
+               for k in ("x", "y", "z")]
+        ok = np.isfinite(h) & np.isfinite(pts[0]) & np.isfinite(pts[1]) & np.isfinite(pts[2])
+        if not ok.any():
+            continue
+        cs = radius_scale * h[ok].max()
         if cs == 0.0:
             cs = 1.0
-        c = [np.floor(np.asarray(d[k], dtype=np.float64) / cs).astype(np.int64)
-             for k in ("x", "y", "z")]
+        c = [np.floor(a[ok] / cs).astype(np.int64) for a in pts]
         dims = [int(a.max() - a.min() + 1) for a in c]
         total += dims[0] * dims[1] * dims[2]
         keys = ((c[0] - c[0].min()) * dims[1] + (c[1] - c[1].min())) * dims[2] + (c[2] - c[2].min())
@@ -62,7 +67,7 @@ class BatchedSimulation:
 
     def __init__(self, sims, dim=2, co=1400.0, ro=1.0, gamma=7.0, alpha=0.1, beta=0.0, eta=0.1,
                  radius_scale=2.0, device=None, key_cap=None, max_cells=None, list_cap=None,
-                 maxch=None, cell_margin=4.0, list_slots_per_particle=None):
+                 maxch=None, cell_margin=4.0, list_slots_per_particle=None, dt_per_sim=None):
         if not torch.cuda.is_available():
             raise RuntimeError("pysph_amd needs a HIP device; it has no CPU fallback")
         if dim not in (2, 3):
@@ -96,6 +101,12 @@ class BatchedSimulation:
             self.sim_off = torch.from_numpy(self.sim_off_host).to(self.device)
             self.sim_of = torch.from_numpy(
                 np.repeat(np.arange(self.nsims, dtype=np.int32), counts)).to(self.device)
+            self.dt_sim = None
+            if dt_per_sim is not None:
+                dts = np.asarray(dt_per_sim, dtype=np.float64)
+                if dts.shape != (self.nsims,):
+                    raise ValueError("dt_per_sim needs one time step per simulation")
+                self.dt_sim = torch.from_numpy(dts.copy()).to(self.device)
             self.status = torch.zeros(1, dtype=torch.int32, device=self.device)
             self.pairs = torch.zeros(1, dtype=torch.int64, device=self.device)
         total, nonempty = _initial_cells(sims, radius_scale)
@@ -116,6 +127,7 @@ class BatchedSimulation:
         d.key_cap, d.max_cells, d.list_cap, d.maxch = (self.key_cap, self.max_cells,
                                                        self.list_cap, self.maxch)
         d.status, d.pairs = self.status.data_ptr(), self.pairs.data_ptr()
+        d.dt_sim = self.dt_sim.data_ptr() if self.dt_sim is not None else None
         nbytes = ctypes.c_int64(0)
         d.work = None
         self.ctx.check(lib.sph_batch_workspace_size(self.ctx.handle, ctypes.byref(d),
@@ -129,8 +141,9 @@ class BatchedSimulation:
         self.steps_done = 0
 
     # ------------------------------------------------------------------
-    def step(self, dt):
-        """One RK2 step of every simulation (queued on the batch's stream)."""
+    def step(self, dt=1.0):
+        """One RK2 step of every simulation (queued on the batch's stream).
+        With per-simulation time steps, dt multiplies them (1.0: one step)."""
         self.ctx.check(self.ctx.lib.sph_batch_rk2_step(self.ctx.handle, ctypes.byref(self.desc),
                                                        float(dt)))
         self.steps_done += 1
@@ -193,11 +206,19 @@ class BatchedSimulation:
             pass
 
 
+def drop_time_step(dx, dt_ref=1e-5, dx_ref=0.025):
+    """The elliptical drop's dt = 1e-5 at dx = 0.025, scaled with dx so that
+    every resolution keeps that run's CFL number c0*dt/h (at dx = 0.01 the
+    unscaled 1e-5 gives c0*dt/h = 1.08 and the drop blows up)."""
+    return dt_ref * min(1.0, float(dx) / dx_ref)
+
+
 def batched_drops(nsims=256, seed=20261202, radius_range=(0.4, 2.0), dxs=(0.01, 0.02, 0.04),
                   vel_range=(50.0, 150.0), rho0=1.0, c0=1400.0, hdx=1.3):
     """The batched elliptical drops: radius ~ U(radius_range), dx drawn from
     dxs, velocity scale ~ U(vel_range) (u = -s x, v = s y), seeded PCG64.
-    Returns (list of drop dicts, per-drop parameters)."""
+    Returns (list of drop dicts, per-drop parameters incl. the time step
+    drop_time_step(dx))."""
     from .solver.generators import elliptical_drop
     rng = np.random.default_rng(seed)
     radii = rng.uniform(radius_range[0], radius_range[1], nsims)
@@ -205,5 +226,6 @@ def batched_drops(nsims=256, seed=20261202, radius_range=(0.4, 2.0), dxs=(0.01, 
     vel = rng.uniform(vel_range[0], vel_range[1], nsims)
     sims = [elliptical_drop(dx=float(dx[k]), radius=float(radii[k]), rho0=rho0, c0=c0,
                             vel=float(vel[k]), hdx=hdx) for k in range(nsims)]
-    return sims, dict(radius=radii, dx=dx, vel=vel)
+    return sims, dict(radius=radii, dx=dx, vel=vel,
+                      dt=np.array([drop_time_step(v) for v in dx]))
 

@@ -83,7 +83,8 @@ __global__ void __launch_bounds__(BT_THREADS)
         if (bad > 0.0) atomicOr(status, 4);
         g.cs = 1.0;
         g.cmin0 = g.cmin1 = g.cmin2 = 0;
-        g.d0 = g.d1 = g.d2 = 1;
+        g.d0 = bad > 0.0 ? 0 : 1;  // d0 = 0: void, its particles go to the trash cell
+        g.d1 = g.d2 = 1;
     } else {
         double cs = radius_scale * (-v[6]);
         if (cs == 0.0) cs = 1.0;
@@ -94,7 +95,7 @@ __global__ void __launch_bounds__(BT_THREADS)
             if (fabs(fl) > 1.0e9 || fabs(fh) > 1.0e9 || fh - fl + 1.0 > 1.0e6) {
                 atomicOr(status, 8);
                 cm[a] = 0;
-                dm[a] = 1;
+                dm[a] = 0;  // void (see above)
             } else {
                 cm[a] = (int)fl;
                 dm[a] = (int)(fh - fl) + 1;
@@ -104,35 +105,37 @@ __global__ void __launch_bounds__(BT_THREADS)
         g.cmin0 = cm[0];
         g.cmin1 = cm[1];
         g.cmin2 = cm[2];
-        g.d0 = dm[0];
-        g.d1 = dm[1];
-        g.d2 = dm[2];
+        g.d0 = (dm[0] && dm[1] && dm[2]) ? dm[0] : 0;
+        g.d1 = dm[1] ? dm[1] : 1;
+        g.d2 = dm[2] ? dm[2] : 1;
     }
     grids[s] = g;
-    ncell[s] = cells;
+    ncell[s] = g.d0 ? cells : 0;
 }
 
-// key_base of every simulation (exclusive sum of the cell counts); the key
-// space must fit key_cap (status |= 2 otherwise)
+// key_base of every simulation (exclusive sum of the cell counts).  The
+// last key, key_cap - 1, is the trash cell: particles of a void simulation
+// (non-finite values, a grid too large, or no room left in the key space --
+// status |= 2) go there and get no neighbour lists, so a simulation that
+// blew up costs no more than one that did not.
 __global__ void k_batch_key_base(int nsims, const int64_t *__restrict__ ncell, int64_t key_cap,
                                  GridDev *__restrict__ grids, int32_t *__restrict__ status)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     int64_t base = 0;
     for (int s = 0; s < nsims; ++s) {
-        if (base + ncell[s] <= key_cap) {
+        if (grids[s].d0 == 0) continue;
+        if (base + ncell[s] <= key_cap - 1) {
             grids[s].key_base = (int32_t)base;
+            base += ncell[s];
         } else {
-            // no room: a one-cell grid at key 0 keeps every later access in
-            // bounds; the step is void (status |= 2)
             GridDev g = grids[s];
             g.key_base = 0;
-            g.d0 = g.d1 = g.d2 = 1;
+            g.d0 = 0;
             grids[s] = g;
+            *status |= 2;
         }
-        base += ncell[s];
     }
-    if (base > key_cap) *status |= 2;
 }
 
 __global__ void __launch_bounds__(BT_THREADS)
@@ -145,13 +148,16 @@ __global__ void __launch_bounds__(BT_THREADS)
     const int64_t i = (int64_t)blockIdx.x * BT_THREADS + threadIdx.x;
     if (i >= n) return;
     const GridDev g = grids[sim_of[i]];
-    const int cx = min(max(cell_coord(x[i], g.cs) - g.cmin0, 0), g.d0 - 1);
-    const int cy = min(max(cell_coord(y[i], g.cs) - g.cmin1, 0), g.d1 - 1);
-    const int cz = min(max(cell_coord(z[i], g.cs) - g.cmin2, 0), g.d2 - 1);
-    int64_t key = (int64_t)g.key_base + ((int64_t)cx * g.d1 + cy) * g.d2 + cz;
-    if (key >= key_cap) {  // the key space overflowed (status 2 is set too)
-        atomicOr(status, 2);
-        key = key_cap - 1;
+    int64_t key = key_cap - 1;  // trash cell of void simulations
+    if (g.d0 > 0) {
+        const int cx = min(max(cell_coord(x[i], g.cs) - g.cmin0, 0), g.d0 - 1);
+        const int cy = min(max(cell_coord(y[i], g.cs) - g.cmin1, 0), g.d1 - 1);
+        const int cz = min(max(cell_coord(z[i], g.cs) - g.cmin2, 0), g.d2 - 1);
+        key = (int64_t)g.key_base + ((int64_t)cx * g.d1 + cy) * g.d2 + cz;
+        if (key >= key_cap - 1) {  // cannot happen after k_batch_key_base; kept in bounds
+            atomicOr(status, 2);
+            key = key_cap - 1;
+        }
     }
     cell_of[i] = (int32_t)key;
     slot[i] = atomicAdd(&count[key], 1);
@@ -162,6 +168,218 @@ __global__ void k_batch_check_cells(const int32_t *__restrict__ ncell_dev, int64
                                     int32_t *__restrict__ status)
 {
     if (threadIdx.x == 0 && *ncell_dev > max_cells) *status |= 16;
+}
+
+// x = x0 + a * (scale * dt_sim[s]) for the stepped properties (the
+// per-simulation time step of Integrator.step, solver/integrator.py:340-367)
+__global__ void __launch_bounds__(BT_THREADS)
+    k_batch_stage(int64_t n, const int32_t *__restrict__ sim_of, const double *__restrict__ dt_sim,
+                  double scale, sph_step_props pr)
+{
+    const int64_t i = (int64_t)blockIdx.x * BT_THREADS + threadIdx.x;
+    if (i >= n) return;
+    const double dt = scale * dt_sim[sim_of[i]];
+    for (int k = 0; k < pr.nprops; ++k) {
+        const double t = pr.a[k][i] * dt;
+        pr.x[k][i] = pr.x0[k][i] + t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// reverse mode of one evaluation over its (frozen) neighbour lists
+// ---------------------------------------------------------------------------
+// F(X) = (u, a(x, u)) with a = MomentumEquation + AV over the densities of
+// SPHRho and the Tait pressures.  Given the cotangent abar of a, the
+// vector-Jacobian product gives xbar = (da/dx)^T abar, ubar = (da/du)^T abar
+// and alphabar = (da/dalpha)^T abar.  With one h per simulation the lists
+// are symmetric (b in L(a) <=> a in L(b)), so every particle gathers the
+// adjoint of each pair term from its own row -- the term of its own
+// acceleration (cotangent abar_a) and the term of its neighbour's
+// (cotangent abar_b) -- and no atomics are needed.
+
+// gradient factor G (gradW_ab = r_ab G) of the cubic spline and dG/d|r|
+__device__ __forceinline__ void cubic_g_dg(double rab, double h, double fac, double &G,
+                                           double &dG)
+{
+    const double q = rab / h;
+    if (q > 2.0 || !(q >= 1.0 || q > 1e-14)) {
+        G = 0.0;
+        dG = 0.0;
+    } else if (q >= 1.0) {
+        const double t = 2.0 - q;
+        G = fac * (-0.75 * t * t / (h * rab));
+        dG = fac * 0.75 * t * (2.0 / (h * h * rab) + t / (h * rab * rab));
+    } else {
+        G = fac * 3.0 * (0.75 * q - 1.0) / (h * h);
+        dG = fac * 2.25 / (h * h * h);
+    }
+}
+
+__global__ void __launch_bounds__(BT_THREADS)
+    k_gather3(int64_t n, const int32_t *__restrict__ perm, const double *__restrict__ a0,
+              const double *__restrict__ a1, const double *__restrict__ a2,
+              double *__restrict__ out)
+{
+    const int64_t t = (int64_t)blockIdx.x * BT_THREADS + threadIdx.x;
+    if (t >= n) return;
+    const int32_t i = perm[t];
+    out[3 * t] = a0[i];
+    out[3 * t + 1] = a1[i];
+    out[3 * t + 2] = a2 ? a2[i] : 0.0;
+}
+
+// momentum part: per sorted particle a, xbar/ubar from the pair terms, the
+// density adjoint rhobar_a (through P = p/rho^2, the Tait p(rho) and
+// cs(rho) and the viscosity's rho and cs) and the alpha terms of its own row
+__global__ void __launch_bounds__(BT_THREADS)
+    k_vjp_momentum(int64_t n, const int32_t *__restrict__ counts,
+                   const int64_t *__restrict__ slice_off, const int32_t *__restrict__ indices,
+                   const double *__restrict__ rec, const double *__restrict__ abar, int dim,
+                   double alpha, double beta, double eta, double ro, double co, double gamma,
+                   double *__restrict__ xbar, double *__restrict__ ubar,
+                   double *__restrict__ rhobar, double *__restrict__ alphabar,
+                   int32_t *__restrict__ status)
+{
+    const int64_t t = (int64_t)sph_xcd_tile(blockIdx.x, gridDim.x) * BT_THREADS + threadIdx.x;
+    if (t >= n) return;
+    const double *ra = rec + t * SPH_REC;
+    const double xa = ra[SPH_R_X], ya = ra[SPH_R_Y], za = ra[SPH_R_Z], ma = ra[SPH_R_M];
+    const double ua = ra[SPH_R_U], va = ra[SPH_R_V], wa = ra[SPH_R_W];
+    const double rhoa = ra[SPH_R_RHO], ca = ra[SPH_R_CS], ha = ra[SPH_R_H], pa = ra[SPH_R_P];
+    const double Pa = ra[SPH_R_PRHO2];
+    const double fac = sph_h_dim(ha, dim) * sph_cubic_fac1(dim);
+    const double Ax = abar[3 * t], Ay = abar[3 * t + 1], Az = abar[3 * t + 2];
+    double xb0 = 0, xb1 = 0, xb2 = 0, ub0 = 0, ub1 = 0, ub2 = 0;
+    double Pbar = 0, cbar_a = 0, rhobar_pi = 0, albar = 0;
+    const int32_t *row = indices + slice_off[t >> 6] + (t & 63);
+    const int32_t e = counts[t];
+    bool bad_h = false;
+    for (int32_t k = 0; k < e; ++k) {
+        const int32_t b = row[(int64_t)k << 6];
+        if (b == (int32_t)t) continue;  // r = 0: no gradient, no viscosity
+        const double *rb = rec + (int64_t)b * SPH_REC;
+        if (rb[SPH_R_H] != ha) bad_h = true;
+        const double rx = xa - rb[SPH_R_X], ry = ya - rb[SPH_R_Y], rz = za - rb[SPH_R_Z];
+        const double vx = ua - rb[SPH_R_U], vy = va - rb[SPH_R_V], vz = wa - rb[SPH_R_W];
+        const double mb = rb[SPH_R_M], rhob = rb[SPH_R_RHO], cb = rb[SPH_R_CS];
+        const double Pb = rb[SPH_R_PRHO2];
+        const double Bx = abar[3 * (int64_t)b], By = abar[3 * (int64_t)b + 1],
+                     Bz = abar[3 * (int64_t)b + 2];
+        const double r2 = rx * rx + ry * ry + rz * rz;
+        const double rab = sqrt(r2);
+        double G, dG;
+        cubic_g_dg(rab, ha, fac, G, dG);
+        const double dot = vx * rx + vy * ry + vz * rz;
+        const double den = r2 + eta * eta * ha * ha;
+        const double mu = ha * dot / den;
+        const double cm = 0.5 * (ca + cb), rm = 0.5 * (rhoa + rhob);
+        const bool visc = dot < 0;
+        const double Pi = visc ? (-alpha * cm * mu + beta * mu * mu) / rm : 0.0;
+        const double s_ab = -mb * (Pa + Pb + Pi), s_ba = -ma * (Pb + Pa + Pi);
+        // own term T_ab = s_ab r G (cotangent A); the neighbour's T_ba = -s_ba r G (cotangent B)
+        const double gA = (Ax * rx + Ay * ry + Az * rz) * G;
+        const double gB = -(Bx * rx + By * ry + Bz * rz) * G;
+        Pbar += -mb * gA - ma * gB;
+        const double pibar_own = -mb * gA, pibar_tot = pibar_own - ma * gB;
+        // gradient: d(r G)/dr = G I + (dG/|r|) r r^T applied to s_ab A (own
+        // term) and, with r_ba = -r, to -s_ba B (the neighbour's term)
+        const double qx = s_ab * Ax - s_ba * Bx, qy = s_ab * Ay - s_ba * By,
+                     qz = s_ab * Az - s_ba * Bz;
+        const double rq = (rab > 0.0 ? dG / rab : 0.0) * (rx * qx + ry * qy + rz * qz);
+        xb0 += G * qx + rq * rx;
+        xb1 += G * qy + rq * ry;
+        xb2 += G * qz + rq * rz;
+        if (visc) {
+            const double mubar = pibar_tot * (-alpha * cm + 2.0 * beta * mu) / rm;
+            cbar_a += 0.5 * pibar_tot * (-alpha * mu / rm);
+            rhobar_pi += 0.5 * pibar_tot * (-Pi / rm);
+            albar += pibar_own * (-cm * mu / rm);
+            const double dotbar = mubar * ha / den;
+            const double r2bar = -mubar * ha * dot / (den * den);
+            xb0 += dotbar * vx + 2.0 * r2bar * rx;
+            xb1 += dotbar * vy + 2.0 * r2bar * ry;
+            xb2 += dotbar * vz + 2.0 * r2bar * rz;
+            ub0 += dotbar * rx;
+            ub1 += dotbar * ry;
+            ub2 += dotbar * rz;
+        }
+    }
+    if (bad_h) atomicOr(status, 32);
+    // P = p * (1/rho^2); p = (pow(rho/ro, gamma) - 1) B; cs = pow(rho/ro, (gamma-1)/2) co
+    const double Bc = co * co * ro / gamma;
+    const double ratio = rhoa / ro;
+    const double dp = Bc * gamma * pow(ratio, gamma - 1.0) / ro;
+    const double g2 = 0.5 * (gamma - 1.0);
+    const double dc = co * g2 * pow(ratio, g2 - 1.0) / ro;
+    const double inv2 = 1.0 / (rhoa * rhoa);
+    rhobar[t] = rhobar_pi + Pbar * (-2.0 * pa * inv2 / rhoa) + Pbar * inv2 * dp + cbar_a * dc;
+    xbar[3 * t] = xb0;
+    xbar[3 * t + 1] = xb1;
+    xbar[3 * t + 2] = xb2;
+    ubar[3 * t] = ub0;
+    ubar[3 * t + 1] = ub1;
+    ubar[3 * t + 2] = ub2;
+    alphabar[t] = albar;
+}
+
+// density part: xbar_a += sum_b (rhobar_a m_b + rhobar_b m_a) gradW_ab, then
+// everything back to original order
+__global__ void __launch_bounds__(BT_THREADS)
+    k_vjp_density(int64_t n, const int32_t *__restrict__ perm, const int32_t *__restrict__ counts,
+                  const int64_t *__restrict__ slice_off, const int32_t *__restrict__ indices,
+                  const double *__restrict__ rec, const double *__restrict__ rhobar, int dim,
+                  const double *__restrict__ xbar_s, const double *__restrict__ ubar_s,
+                  double *__restrict__ xbar, double *__restrict__ ybar, double *__restrict__ zbar,
+                  double *__restrict__ ubar, double *__restrict__ vbar, double *__restrict__ wbar)
+{
+    const int64_t t = (int64_t)sph_xcd_tile(blockIdx.x, gridDim.x) * BT_THREADS + threadIdx.x;
+    if (t >= n) return;
+    const double *ra = rec + t * SPH_REC;
+    const double xa = ra[SPH_R_X], ya = ra[SPH_R_Y], za = ra[SPH_R_Z], ma = ra[SPH_R_M];
+    const double ha = ra[SPH_R_H];
+    const double fac = sph_h_dim(ha, dim) * sph_cubic_fac1(dim);
+    const double rba = rhobar[t];
+    double xb0 = xbar_s[3 * t], xb1 = xbar_s[3 * t + 1], xb2 = xbar_s[3 * t + 2];
+    const int32_t *row = indices + slice_off[t >> 6] + (t & 63);
+    const int32_t e = counts[t];
+    for (int32_t k = 0; k < e; ++k) {
+        const int32_t b = row[(int64_t)k << 6];
+        if (b == (int32_t)t) continue;
+        const double *rb = rec + (int64_t)b * SPH_REC;
+        const double rx = xa - rb[SPH_R_X], ry = ya - rb[SPH_R_Y], rz = za - rb[SPH_R_Z];
+        const double rab = sqrt(rx * rx + ry * ry + rz * rz);
+        double G, dG;
+        cubic_g_dg(rab, ha, fac, G, dG);
+        const double c = (rba * rb[SPH_R_M] + rhobar[b] * ma) * G;
+        xb0 += c * rx;
+        xb1 += c * ry;
+        xb2 += c * rz;
+    }
+    const int32_t i = perm[t];
+    xbar[i] = xb0;
+    ybar[i] = xb1;
+    if (zbar) zbar[i] = xb2;
+    ubar[i] = ubar_s[3 * t];
+    vbar[i] = ubar_s[3 * t + 1];
+    if (wbar) wbar[i] = ubar_s[3 * t + 2];
+}
+
+// deterministic sum of n doubles by one block
+__global__ void __launch_bounds__(1024)
+    k_sum_f64(const double *__restrict__ v, int64_t n, double *__restrict__ out)
+{
+    __shared__ double part[1024 / 64];
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 1024) s += v[i];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot = 0.0;
+        for (int w = 0; w < 1024 / 64; ++w) tot += part[w];
+        *out = tot;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -177,6 +395,9 @@ struct BatchWork {
     int32_t *counts;
     int64_t *slice_off;
     int32_t *indices;
+    // reverse mode, sorted order: cotangent of the accelerations, adjoints
+    // of x and u (3 each), of rho, and the per-particle alpha terms
+    double *abar, *xbar, *ubar, *rhobar, *alphabar;
     size_t bytes;
 };
 
@@ -206,6 +427,11 @@ BatchWork carve(const sph_batch_desc *d, char *base)
     w.counts = (int32_t *)take(sizeof(int32_t) * (size_t)(n + 1));
     w.slice_off = (int64_t *)take(sizeof(int64_t) * 2 * (size_t)(nsl + 1));
     w.indices = (int32_t *)take(sizeof(int32_t) * (size_t)d->list_cap);
+    w.abar = (double *)take(sizeof(double) * 3 * (size_t)n);
+    w.xbar = (double *)take(sizeof(double) * 3 * (size_t)n);
+    w.ubar = (double *)take(sizeof(double) * 3 * (size_t)n);
+    w.rhobar = (double *)take(sizeof(double) * (size_t)n);
+    w.alphabar = (double *)take(sizeof(double) * (size_t)n);
     w.bytes = o;
     return w;
 }
@@ -215,7 +441,7 @@ int check_desc(sph_ctx *ctx, const sph_batch_desc *d)
     SPH_REQUIRE(ctx, d, "batch: null descriptor");
     SPH_REQUIRE(ctx, d->nsims >= 1 && (d->dim == 2 || d->dim == 3), "batch: bad nsims/dim");
     SPH_REQUIRE(ctx, d->n >= 1 && d->n < ((int64_t)1 << 31), "batch: bad n");
-    SPH_REQUIRE(ctx, d->key_cap >= 1 && d->key_cap < ((int64_t)1 << 31) - 1 && d->list_cap >= 1 &&
+    SPH_REQUIRE(ctx, d->key_cap >= 2 && d->key_cap < ((int64_t)1 << 31) - 1 && d->list_cap >= 1 &&
                          d->max_cells >= 1 && d->max_cells <= d->key_cap,
                 "batch: bad capacities");
     SPH_REQUIRE(ctx, d->maxch >= 1 && d->maxch <= 16, "batch: bad maxch");
@@ -250,8 +476,9 @@ int batch_eval(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w)
                        ctx->stream, w.grids, d->sim_of, d->x, d->y, d->z, n, d->key_cap,
                        w.cell_start, w.cell_of, w.rank, d->status);
     SPH_LAUNCH_CHECK(ctx, "k_batch_keys");
+    const int32_t trash = (int32_t)(d->key_cap - 1);
     rc = sph_bin_sort(ctx, n, d->key_cap, w.cell_of, w.rank, w.cell_start, w.perm, w.cells,
-                      w.slot, w.tmp, 0);
+                      w.slot, w.tmp, trash, 0);
     if (rc) return rc;
     const int32_t *ncell_dev = w.slot + d->key_cap;
     hipLaunchKernelGGL(k_batch_check_cells, dim3(1), dim3(64), 0, ctx->stream, ncell_dev,
@@ -273,7 +500,8 @@ int batch_eval(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w)
     // -- neighbour lists
     rc = sph_nnps_build_batch(ctx, w.grids, d->sim_of, w.rec, w.perm, w.cell_start, w.cells,
                               ncell_dev, d->max_cells, n, d->radius_scale, d->maxch, w.counts,
-                              w.slice_off, w.indices, d->list_cap, d->status, d->pairs);
+                              w.slice_off, w.indices, d->list_cap, d->status, d->pairs,
+                              trash);
     if (rc) return rc;
     // -- SPHRho, Tait (non-integrating: outputs start from 0)
     rc = sph_summation_density(ctx, d->dim, n, n, w.perm, w.counts, w.slice_off, w.indices,
@@ -306,7 +534,11 @@ int batch_stage(sph_ctx *ctx, const sph_batch_desc *d, double dt)
         sp.x0[k] = x0[k];
         sp.a[k] = a[k];
     }
-    return sph_stage_step(ctx, d->n, &sp, dt);
+    if (!d->dt_sim) return sph_stage_step(ctx, d->n, &sp, dt);
+    hipLaunchKernelGGL(k_batch_stage, dim3(sph_blocks(d->n, BT_THREADS)), dim3(BT_THREADS), 0,
+                       ctx->stream, d->n, d->sim_of, d->dt_sim, dt, sp);
+    SPH_LAUNCH_CHECK(ctx, "k_batch_stage");
+    return SPH_OK;
 }
 
 int batch_rk2(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w, double dt)
@@ -352,6 +584,82 @@ extern "C" int sph_batch_rk2_step(sph_ctx *ctx, const sph_batch_desc *d, double 
     SPH_REQUIRE(ctx, d->work_bytes >= (int64_t)w.bytes, "batch: workspace of %lld bytes < %zu",
                 (long long)d->work_bytes, w.bytes);
     return batch_rk2(ctx, d, w, dt);
+}
+
+static int batch_prep(sph_ctx *ctx, const sph_batch_desc *d, BatchWork *w)
+{
+    if (!ctx) return SPH_EINVAL;
+    int rc = check_desc(ctx, d);
+    if (rc) return rc;
+    *w = carve(d, (char *)d->work);
+    SPH_REQUIRE(ctx, d->work_bytes >= (int64_t)w->bytes, "batch: workspace of %lld bytes < %zu",
+                (long long)d->work_bytes, w->bytes);
+    return SPH_OK;
+}
+
+extern "C" int sph_batch_save_initial(sph_ctx *ctx, const sph_batch_desc *d)
+{
+    BatchWork w;
+    int rc = batch_prep(ctx, d, &w);
+    if (rc) return rc;
+    double *dst[6] = {d->u0, d->v0, d->x0, d->y0, d->w0, d->z0};
+    const double *src[6] = {d->u, d->v, d->x, d->y, d->w, d->z};
+    for (int k = 0; k < (d->dim == 3 ? 6 : 4); ++k) {
+        rc = sph_copy(ctx, d->n, src[k], dst[k]);
+        if (rc) return rc;
+    }
+    return SPH_OK;
+}
+
+extern "C" int sph_batch_eval(sph_ctx *ctx, const sph_batch_desc *d)
+{
+    BatchWork w;
+    int rc = batch_prep(ctx, d, &w);
+    if (rc) return rc;
+    return batch_eval(ctx, d, w);
+}
+
+extern "C" int sph_batch_stage(sph_ctx *ctx, const sph_batch_desc *d, double dt)
+{
+    BatchWork w;
+    int rc = batch_prep(ctx, d, &w);
+    if (rc) return rc;
+    return batch_stage(ctx, d, dt);
+}
+
+extern "C" int sph_batch_vjp(sph_ctx *ctx, const sph_batch_desc *d, const double *abar_u,
+                             const double *abar_v, const double *abar_w, double *xbar,
+                             double *ybar, double *zbar, double *ubar, double *vbar,
+                             double *wbar, double *alpha_bar)
+{
+    BatchWork w;
+    int rc = batch_prep(ctx, d, &w);
+    if (rc) return rc;
+    SPH_REQUIRE(ctx, abar_u && abar_v && xbar && ybar && ubar && vbar && alpha_bar,
+                "sph_batch_vjp: null argument");
+    SPH_REQUIRE(ctx, d->dim == 2 || (abar_w && zbar && wbar), "sph_batch_vjp: 3D needs w/z");
+    const int64_t n = d->n;
+    hipLaunchKernelGGL(k_gather3, dim3(sph_blocks(n, BT_THREADS)), dim3(BT_THREADS), 0,
+                       ctx->stream, n, (const int32_t *)w.perm, abar_u, abar_v,
+                       d->dim == 3 ? abar_w : nullptr, w.abar);
+    SPH_LAUNCH_CHECK(ctx, "k_gather3");
+    hipLaunchKernelGGL(k_vjp_momentum, dim3(sph_blocks(n, BT_THREADS)), dim3(BT_THREADS), 0,
+                       ctx->stream, n, (const int32_t *)w.counts, (const int64_t *)w.slice_off,
+                       (const int32_t *)w.indices, (const double *)w.rec,
+                       (const double *)w.abar, d->dim, d->alpha, d->beta, d->eta, d->ro, d->co,
+                       d->gamma, w.xbar, w.ubar, w.rhobar, w.alphabar, d->status);
+    SPH_LAUNCH_CHECK(ctx, "k_vjp_momentum");
+    hipLaunchKernelGGL(k_vjp_density, dim3(sph_blocks(n, BT_THREADS)), dim3(BT_THREADS), 0,
+                       ctx->stream, n, (const int32_t *)w.perm, (const int32_t *)w.counts,
+                       (const int64_t *)w.slice_off, (const int32_t *)w.indices,
+                       (const double *)w.rec, (const double *)w.rhobar, d->dim,
+                       (const double *)w.xbar, (const double *)w.ubar, xbar, ybar,
+                       d->dim == 3 ? zbar : nullptr, ubar, vbar, d->dim == 3 ? wbar : nullptr);
+    SPH_LAUNCH_CHECK(ctx, "k_vjp_density");
+    hipLaunchKernelGGL(k_sum_f64, dim3(1), dim3(1024), 0, ctx->stream, (const double *)w.alphabar,
+                       n, alpha_bar);
+    SPH_LAUNCH_CHECK(ctx, "k_sum_f64");
+    return SPH_OK;
 }
 
 extern "C" int sph_batch_graph_create(sph_ctx *ctx, const sph_batch_desc *d, double dt,

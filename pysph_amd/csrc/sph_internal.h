@@ -99,22 +99,24 @@ __global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v);
 // cell_start (exclusive scan), perm/rank (ascending index inside every
 // cell), slot[0..nc] (exclusive scan of the non-empty flags; slot[nc] = the
 // number of non-empty cells) and cells[] (their keys).  tmp holds n ints.
-// Scan partials go to the context scratch from scratch_off on.
+// The members of cell `trash` (-1: none) are left unordered.  Scan partials
+// go to the context scratch from scratch_off on.
 int sph_bin_sort(sph_ctx *ctx, int64_t n, int64_t nc, const int32_t *cell_of, int32_t *rank,
                  int32_t *cell_start, int32_t *perm, int32_t *cells, int32_t *slot, int32_t *tmp,
-                 size_t scratch_off);
+                 int32_t trash, size_t scratch_off);
 
 // Neighbour lists of a batch of simulations packed in one array, without
 // host synchronisation (capturable in a graph): grids[sim_of[i]] is the grid
 // of original particle i, *ncell_dev the number of non-empty cells (at most
 // max_cells).  Lists needing more than cap slots set bit 0 of *status and
-// are not written; nnz is added to *nnz_acc.
+// are not written; nnz is added to *nnz_acc.  Particles in cell `trash` get
+// empty rows.
 int sph_nnps_build_batch(sph_ctx *ctx, const GridDev *grids, const int32_t *sim_of,
                          const double *rec, const int32_t *perm, const int32_t *cell_start,
                          const int32_t *cells, const int32_t *ncell_dev, int64_t max_cells,
                          int64_t n, double radius_scale, int maxch, int32_t *counts,
                          int64_t *slice_off, int32_t *indices, int64_t cap, int32_t *status,
-                         int64_t *nnz_acc);
+                         int64_t *nnz_acc, int32_t trash);
 
 // Exclusive prefix sums (device-wide), stream-ordered.  `data` has n
 // elements; the scanned values are written in place and the total is

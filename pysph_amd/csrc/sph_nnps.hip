@@ -337,7 +337,7 @@ __global__ void __launch_bounds__(NN_THREADS)
 // of its cell with a smaller index.  Writes perm and its inverse rank.
 __global__ void __launch_bounds__(NN_THREADS)
     k_bin_rank(const int32_t *__restrict__ cell_of, int64_t n, const int32_t *__restrict__ start,
-               const int32_t *__restrict__ tmp, int32_t *__restrict__ perm,
+               const int32_t *__restrict__ tmp, int32_t trash, int32_t *__restrict__ perm,
                int32_t *__restrict__ rank)
 {
     int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
@@ -345,7 +345,10 @@ __global__ void __launch_bounds__(NN_THREADS)
     const int32_t key = cell_of[i];
     const int32_t s = start[key], e = start[key + 1];
     int32_t r = 0;
-    for (int32_t a = s; a < e; ++a) r += tmp[a] < (int32_t)i;
+    if (key == trash)
+        r = rank[i];  // unordered: the atomic slot (no lists are built for this cell)
+    else
+        for (int32_t a = s; a < e; ++a) r += tmp[a] < (int32_t)i;
     perm[s + r] = (int32_t)i;
     rank[i] = s + r;
 }
@@ -380,7 +383,7 @@ static int read_i64(sph_ctx *ctx, const void *dev, int64_t *out, bool is32)
 
 int sph_bin_sort(sph_ctx *ctx, int64_t n, int64_t nc, const int32_t *cell_of, int32_t *rank,
                  int32_t *cell_start, int32_t *perm, int32_t *cells, int32_t *slot, int32_t *tmp,
-                 size_t scratch_off)
+                 int32_t trash, size_t scratch_off)
 {
     int rc = sph_scan_exclusive_i32_at(ctx, cell_start, nc, scratch_off);
     if (rc) return rc;
@@ -388,7 +391,7 @@ int sph_bin_sort(sph_ctx *ctx, int64_t n, int64_t nc, const int32_t *cell_of, in
                        ctx->stream, cell_of, rank, n, cell_start, tmp);
     SPH_LAUNCH_CHECK(ctx, "k_bin_scatter");
     hipLaunchKernelGGL(k_bin_rank, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
-                       ctx->stream, cell_of, n, cell_start, tmp, perm, rank);
+                       ctx->stream, cell_of, n, cell_start, tmp, trash, perm, rank);
     SPH_LAUNCH_CHECK(ctx, "k_bin_rank");
     // non-empty flags -> slots (scan) -> compact list of cell keys
     hipLaunchKernelGGL(k_cell_flags, dim3(sph_blocks(nc, NN_THREADS)), dim3(NN_THREADS), 0,
@@ -437,7 +440,7 @@ extern "C" int sph_bin_array(sph_ctx *ctx, const sph_grid *grid, const double *x
     hipLaunchKernelGGL(k_bin_keys, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
                        ctx->stream, g, x, y, z, n, cell_start, cell_of, rank);
     SPH_LAUNCH_CHECK(ctx, "k_bin_keys");
-    rc = sph_bin_sort(ctx, n, nc, cell_of, rank, cell_start, perm, cells, slot, tmp,
+    rc = sph_bin_sort(ctx, n, nc, cell_of, rank, cell_start, perm, cells, slot, tmp, -1,
                       slot_bytes + tmp_bytes);
     if (rc) return rc;
     return read_i64(ctx, slot + nc, ncells_out, true);
@@ -508,6 +511,7 @@ struct CellArgs {
     int64_t nslices;
     int64_t cap;
     int32_t *status;
+    int32_t trash;  // cell key that gets no lists (particles of void simulations), or -1
 };
 
 __device__ __forceinline__ int wave_max_i(int v)
@@ -573,6 +577,7 @@ __global__ void __launch_bounds__(CW)
     if (FILL && (a.cell_ovf[w] == 0 || (a.status && slice_off[a.nslices] > a.cap)))
         return;  // written from the masks, or no room
     const int32_t key = a.dcells[w];
+    if (key == a.trash) return;  // rows stay empty
     const int32_t ds = a.dstart[key], de = a.dstart[key + 1];
     const GridDev g = a.grids ? a.grids[a.sim_of[a.dperm[ds]]] : a.g;
     // the cell's coordinates, recomputed exactly as the binning did
@@ -889,6 +894,7 @@ static CellArgs cell_args_base()
     CellArgs a;
     memset(&a, 0, sizeof(a));
     a.maxch = NN_MAXCH;
+    a.trash = -1;
     return a;
 }
 
@@ -955,7 +961,7 @@ int sph_nnps_build_batch(sph_ctx *ctx, const GridDev *grids, const int32_t *sim_
                          const int32_t *cells, const int32_t *ncell_dev, int64_t max_cells,
                          int64_t n, double radius_scale, int maxch, int32_t *counts,
                          int64_t *slice_off, int32_t *indices, int64_t cap, int32_t *status,
-                         int64_t *nnz_acc)
+                         int64_t *nnz_acc, int32_t trash)
 {
     SPH_REQUIRE(ctx, maxch >= 1 && maxch <= NN_MAXCH, "sph_nnps_build_batch: bad maxch");
     int rc = nnps_clear(ctx, n, counts, slice_off);
@@ -978,6 +984,7 @@ int sph_nnps_build_batch(sph_ctx *ctx, const GridDev *grids, const int32_t *sim_
     a.maxch = maxch;
     a.cap = cap;
     a.status = status;
+    a.trash = trash;
     int64_t *nnz_dev = nullptr;
     rc = nnps_count(ctx, a, max_cells, n, counts, slice_off, &nnz_dev);
     if (rc) return rc;

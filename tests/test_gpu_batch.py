@@ -22,16 +22,17 @@ def test_batch_matches_single_simulation_runs(gpu):
     from pysph_amd.batch import BatchedSimulation
     from pysph_amd.solver.configs import drop_solver
     sims, prm = _small_drops()
-    steps, dt = 4, 1e-5
-    b = BatchedSimulation(sims)
+    steps = 4
+    dts = np.array([1e-5, 7e-6, 1e-5, 5e-6, 9e-6])[:len(sims)]
+    b = BatchedSimulation(sims, dt_per_sim=dts)
     for _ in range(steps):
-        b.step(dt)
+        b.step()
     b.check()
     worst = 0.0
     for s, d in enumerate(sims):
         solver, particles, _ = drop_solver(data=dict(d), dx=float(prm["dx"][s]))
         for _ in range(steps):
-            solver.integrator.integrate(dt)
+            solver.integrator.integrate(float(dts[s]))
         fl = particles.arrays[0]
         for k in KEYS:
             e = rel_err(b.get(k, s), fl.get(k))
@@ -70,3 +71,27 @@ def test_batch_capacity_overflow_is_reported(gpu):
     b.step(1e-5)
     with pytest.raises(BatchError, match="list_cap"):
         b.check()
+
+
+def test_batch_void_simulation_is_reported_and_cheap(gpu):
+    """A simulation with a non-finite coordinate is flagged and its particles
+    get no lists; the others still match their single-simulation runs."""
+    import time
+    from pysph_amd.batch import BatchedSimulation, BatchError
+    from pysph_amd.solver.configs import drop_solver
+    sims, prm = _small_drops(n=3, seed=11)
+    bad = dict(sims[1])
+    bad["x"] = bad["x"].copy()
+    bad["x"][5] = np.nan
+    b = BatchedSimulation([sims[0], bad, sims[2]])
+    t0 = time.perf_counter()
+    b.step(1e-5)
+    b.synchronize()
+    assert time.perf_counter() - t0 < 30.0
+    with pytest.raises(BatchError, match="non-finite"):
+        b.check()
+    for s in (0, 2):
+        solver, particles, _ = drop_solver(data=dict(sims[s]), dx=float(prm["dx"][s]))
+        solver.integrator.integrate(1e-5)
+        for k in ("x", "rho"):
+            assert np.array_equal(b.get(k, s), particles.arrays[0].get(k)), (s, k)

@@ -41,7 +41,12 @@ def _args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("dambreak", "lattice", "batch"), default="dambreak")
+    ap.add_argument("--workload", choices=("dambreak", "lattice", "batch", "adjoint"),
+                    default="dambreak")
+    ap.add_argument("--fd-particles", type=int, default=64,
+                    help="particles whose u0 and v0 gradients --workload adjoint checks by "
+                         "central differences")
+    ap.add_argument("--lbfgs-iters", type=int, default=20)
     ap.add_argument("--nsims", type=int, default=256, help="drops of --workload batch")
     ap.add_argument("--graph-steps", type=int, default=10,
                     help="RK2 steps per captured graph (--workload batch)")
@@ -71,6 +76,19 @@ def copy_calibration(mib):
     ctx.check(ctx.lib.sph_copy(ctx.handle, n, _lib.ptr(a), _lib.ptr(b)))
     torch.cuda.synchronize()
     del a, b
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_momentum_dambreak_1M_traffic.json")
+
+
+def measured_traffic():
+    """HBM bytes per k_momentum launch from the committed rocprofv3 byte
+    counters (FETCH_SIZE, WRITE_SIZE, scaled by a 1 GiB copy), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            return float(json.load(f)["hbm_bytes_per_launch"])
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def momentum_bytes(nd, nnz, ns_other):
@@ -148,7 +166,7 @@ def run_dambreak(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic() if args.dx == 0.01 else None,
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "mean_launch_ms": mom["mean_ms"],
         },
@@ -292,6 +310,166 @@ def run_batch(args):
     }, None
 
 
+def run_adjoint(args):
+    """Gradient of the final-position MSE of the 2D drop (dx = 0.02, 7,860
+    particles, RK2, dt = 1e-5, 200 steps, checkpoint every 20) against a
+    target run from u = -120 x, v = 120 y, with respect to the per-particle
+    initial velocities (around u = -100 x, v = 100 y plus seeded N(0, 5)) and
+    alpha; checked against central differences (eps = 1e-6) on 64 random
+    particles, then fitted with 20 L-BFGS iterations."""
+    import torch
+    from pysph_amd.adjoint import Rollout, rollout
+    from pysph_amd.batch import BatchedSimulation
+    from pysph_amd.solver.generators import elliptical_drop
+    t_start = time.perf_counter()
+
+    def progress(msg):
+        print(f"[adjoint {time.perf_counter() - t_start:7.2f}s] {msg}", file=sys.stderr,
+              flush=True)
+
+    dx, steps, ckpt, dt, eps = 0.02, 200, 20, 1e-5, 1e-6
+    d = elliptical_drop(dx=dx, radius=1.0, vel=100.0)
+    n = len(d["x"])
+    h = float(d["h"][0])
+    pos0 = np.stack([d["x"], d["y"]], 1)
+    roll = Rollout(pos0, d["m"], h, nsteps=steps, dt=dt, checkpoint=ckpt)
+    dev = roll.sim.device
+    tx, ty = roll.forward(torch.tensor(-120.0 * d["x"], device=dev),
+                          torch.tensor(120.0 * d["y"], device=dev), 0.1)
+    tx, ty = tx.clone(), ty.clone()
+    rng = np.random.default_rng(20261202)
+    u_init = -100.0 * d["x"] + rng.normal(0.0, 5.0, n)
+    v_init = 100.0 * d["y"] + rng.normal(0.0, 5.0, n)
+    progress(f"{n} particles, target run done")
+
+    def loss_fn(x, y):
+        return ((x - tx) ** 2 + (y - ty) ** 2).mean()
+
+    u0 = torch.tensor(u_init, device=dev, requires_grad=True)
+    v0 = torch.tensor(v_init, device=dev, requires_grad=True)
+    alpha = torch.tensor(0.1, dtype=torch.float64, device=dev, requires_grad=True)
+    # warm-up, then timed forward and backward
+    x, y = rollout(roll, alpha, u0, v0)
+    torch.autograd.grad(loss_fn(x, y), (u0, v0, alpha))
+    reps = 3
+    t_fwd = t_bwd = 0.0
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        x, y = rollout(roll, alpha, u0, v0)
+        loss = loss_fn(x, y)
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        gu, gv, ga = torch.autograd.grad(loss, (u0, v0, alpha))
+        torch.cuda.synchronize()
+        tc = time.perf_counter()
+        t_fwd += (tb - ta) / reps
+        t_bwd += (tc - tb) / reps
+    progress(f"forward {1e3 * t_fwd:.1f} ms, backward {1e3 * t_bwd:.1f} ms")
+    # central differences: every perturbed rollout is one simulation of a batch
+    picks = rng.choice(n, args.fd_particles, replace=False)
+    which = [(int(i), comp, sgn) for i in picks for comp in ("u", "v") for sgn in (1.0, -1.0)]
+
+    def batch_losses(sim_list, alpha_val):
+        b = BatchedSimulation(sim_list, alpha=alpha_val)
+        for _ in range(steps):
+            b.step(dt)
+        b.check()
+        xs, ys = b.get("x"), b.get("y")
+        txn, tyn = tx.cpu().numpy(), ty.cpu().numpy()
+        out = []
+        for k in range(len(sim_list)):
+            sl = slice(b.sim_off_host[k], b.sim_off_host[k + 1])
+            out.append(float((((xs[sl] - txn) ** 2 + (ys[sl] - tyn) ** 2)).mean()))
+        return out
+
+    gmap = {"u": gu.cpu().numpy(), "v": gv.cpu().numpy()}
+
+    def fd_errors(e):
+        sims_e = []
+        for i, comp, sgn in which:
+            dd = dict(d)
+            dd["u"], dd["v"] = u_init.copy(), v_init.copy()
+            dd[comp][i] += sgn * e
+            sims_e.append(dd)
+        L = batch_losses(sims_e, 0.1)
+        g, fd = [], []
+        for k in range(0, len(sims_e), 2):
+            i, comp, _ = which[k]
+            g.append(gmap[comp][i])
+            fd.append((L[k] - L[k + 1]) / (2 * e))
+        g, fd = np.array(g), np.array(fd)
+        rel = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-300)
+        return rel, np.abs(g - fd).max() / np.abs(fd).max()
+
+    rel6, norm6 = fd_errors(eps)
+    rel8, norm8 = fd_errors(1e-8)
+    d0 = dict(d)
+    d0["u"], d0["v"] = u_init.copy(), v_init.copy()
+    alpha_sweep = {}
+    for e in (1e-5, 1e-6, 1e-7, 1e-8):
+        la = batch_losses([d0], 0.1 + e)[0]
+        lb = batch_losses([d0], 0.1 - e)[0]
+        fda = (la - lb) / (2 * e)
+        alpha_sweep[f"{e:g}"] = abs(float(ga) - fda) / abs(fda)
+    progress(f"FD eps 1e-6: {len(rel6)} velocity components, max rel err {rel6.max():.3g}, "
+             f"median {np.median(rel6):.3g}, max err / max |FD| {norm6:.3g}; eps 1e-8: max "
+             f"{rel8.max():.3g}, median {np.median(rel8):.3g}; alpha rel err by eps {alpha_sweep}")
+    # fit in scaled variables: velocities in units of 100 about the initial
+    # guess, loss relative to its initial value
+    s_u = 100.0
+    du = torch.zeros(n, dtype=torch.float64, device=dev, requires_grad=True)
+    dv = torch.zeros(n, dtype=torch.float64, device=dev, requires_grad=True)
+    a_fit = torch.tensor(0.1, dtype=torch.float64, device=dev, requires_grad=True)
+    u_base = torch.tensor(u_init, device=dev)
+    v_base = torch.tensor(v_init, device=dev)
+    opt = torch.optim.LBFGS([du, dv, a_fit], lr=1.0, max_iter=args.lbfgs_iters,
+                            history_size=20, line_search_fn="strong_wolfe",
+                            tolerance_grad=1e-14, tolerance_change=1e-18)
+    evals = [0]
+    losses = []
+    l0 = float(loss.item())
+
+    def closure():
+        opt.zero_grad()
+        xx, yy = rollout(roll, a_fit, u_base + s_u * du, v_base + s_u * dv)
+        lo = loss_fn(xx, yy) / l0
+        lo.backward()
+        evals[0] += 1
+        losses.append(float(lo.item()) * l0)
+        return lo
+
+    t0 = time.perf_counter()
+    opt.step(closure)
+    t_fit = time.perf_counter() - t0
+    progress(f"L-BFGS: {evals[0]} evaluations, loss {losses[0]:.4g} -> {losses[-1]:.4g}")
+    return {
+        "metric": "adjoint/forward cost ratio",
+        "value": t_bwd / t_fwd,
+        "unit": "backward time / forward time",
+        "ms_per_step": 1e3 * t_fwd / steps,
+        "higher_is_better": False,
+        "dtype": "f64",
+        "config": {"workload": f"2D elliptical drop dx {dx} ({n} particles), RK2 dt {dt}, "
+                               f"{steps} steps, checkpoint every {ckpt}, FP64",
+                   "forward_ms": 1e3 * t_fwd, "backward_ms": 1e3 * t_bwd,
+                   "fd_eps": eps, "fd_components": int(len(rel6)),
+                   "fd_max_rel_err": float(rel6.max()),
+                   "fd_median_rel_err": float(np.median(rel6)),
+                   "fd_max_err_over_max_fd": float(norm6),
+                   "fd_eps_1e-8_max_rel_err": float(rel8.max()),
+                   "fd_eps_1e-8_median_rel_err": float(np.median(rel8)),
+                   "fd_alpha_rel_err_by_eps": alpha_sweep,
+                   "grad_alpha": float(ga),
+                   "lbfgs_iterations": args.lbfgs_iters, "lbfgs_evaluations": evals[0],
+                   "loss_initial": losses[0], "loss_final": losses[-1],
+                   "alpha_fitted": float(a_fit.item()), "fit_s": t_fit,
+                   "fit_max_velocity_change": float(s_u * max(du.abs().max().item(),
+                                                              dv.abs().max().item()))},
+        "roofline": None,
+    }, None
+
+
 def cpu_baseline_dambreak(args, data):
     """The FP64 oracle on the host cores: one PredictorCorrector step of the
     same 1M-fluid dam break (about 10-30 s of CPU work)."""
@@ -342,14 +520,17 @@ def main():
         result, data = run_dambreak(args)
     elif args.workload == "lattice":
         result, data = run_lattice(args)
-    else:
+    elif args.workload == "batch":
         result, data = run_batch(args)
+    else:
+        result, data = run_adjoint(args)
     out = {"metric": result["metric"], "value": result["value"], "unit": result["unit"],
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": result["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "ms_per_step": result["ms_per_step"],
+           "higher_is_better": result.get("higher_is_better", True), "scaling": "weak",
            "vs_baseline": None, "dtype": result["dtype"], "data": "synthetic",
            "config": result["config"], "roofline": result["roofline"]}
-    if not args.no_cpu_baseline and args.workload != "batch":
+    if not args.no_cpu_baseline and args.workload in ("dambreak", "lattice"):
         if args.workload == "dambreak":
             out["cpu_baseline"] = cpu_baseline_dambreak(args, data)
         else:

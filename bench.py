@@ -47,6 +47,9 @@ def _args():
                     help="particles whose u0 and v0 gradients --workload adjoint checks by "
                          "central differences")
     ap.add_argument("--lbfgs-iters", type=int, default=20)
+    ap.add_argument("--precision", choices=("double", "single"), default="double",
+                    help="arithmetic of the lattice and batch workloads (single: float records, "
+                         "lists and operators)")
     ap.add_argument("--nsims", type=int, default=256, help="drops of --workload batch")
     ap.add_argument("--graph-steps", type=int, default=10,
                     help="RK2 steps per captured graph (--workload batch)")
@@ -182,7 +185,7 @@ def run_lattice(args):
     from pysph_amd.solver.generators import jittered_lattice_3d
 
     d = jittered_lattice_3d(n=args.n)
-    pa = base.get_particle_array(name="fluid", **d)
+    pa = base.get_particle_array(name="fluid", cl_precision=args.precision, **d)
     particles = base.Particles([pa], cell_order=args.cell_order)
     kernel = base.CubicSplineKernel(dim=3)
     func = sph.SPHRho.withargs().get_func(pa, pa)
@@ -217,8 +220,9 @@ def run_lattice(args):
         "value": nnz * args.steps / elapsed,
         "unit": "pair-interactions/s",
         "ms_per_step": 1e3 * elapsed / args.steps,
-        "dtype": "f64",
-        "config": {"workload": f"3D NNPS + SummationDensity, {args.n}^3 jittered lattice, FP64",
+        "dtype": "f32" if args.precision == "single" else "f64",
+        "config": {"workload": f"3D NNPS + SummationDensity, {args.n}^3 jittered lattice, "
+                               f"{'FP32' if args.precision == 'single' else 'FP64'}",
                    "particles": n, "nnz": nnz, "steps_per_s": args.steps / elapsed,
                    "kernel_ms": {k: round(v["mean_ms"], 4) for k, v in prof.items()}},
         "roofline": {"kernel": "k_summation_density", "bound": "hbm", "achieved": achieved,
@@ -245,7 +249,7 @@ def run_batch(args):
 
     sims, prm = batched_drops(nsims=args.nsims)
     progress(f"{len(sims)} drops generated, {sum(len(d['x']) for d in sims)} particles")
-    b = BatchedSimulation(sims, dt_per_sim=prm["dt"])
+    b = BatchedSimulation(sims, dt_per_sim=prm["dt"], precision=args.precision)
     dt = 1.0  # multiplies every drop's own time step
     progress(f"packed: key_cap {b.key_cap} max_cells {b.max_cells} list_cap {b.list_cap} "
              f"workspace {b.work.numel() / 2**20:.0f} MiB")
@@ -296,8 +300,9 @@ def run_batch(args):
         "value": batch_sps,
         "unit": "steps/s (all drops advanced one RK2 step)",
         "ms_per_step": 1e3 * elapsed / steps,
-        "dtype": "f64",
-        "config": {"workload": f"{len(sims)} independent 2D elliptical drops, RK2, FP64, "
+        "dtype": "f32" if args.precision == "single" else "f64",
+        "config": {"workload": f"{len(sims)} independent 2D elliptical drops, RK2, "
+                               f"{'FP32 operators' if args.precision == 'single' else 'FP64'}, "
                                f"dt 1e-5*min(1, dx/0.025) per drop, captured HIP graphs of "
                                f"{gs} steps",
                    "particles": b.n, "drops": len(sims),

@@ -131,6 +131,22 @@ typedef struct sph_prop_list {
 } sph_prop_list;
 int sph_gather_sorted(sph_ctx *ctx, int64_t n, const int32_t *perm, const sph_prop_list *props);
 
+/* ---- single precision (FP32) variant ----------------------------------
+ * The reference runs its OpenCL path in single precision when the arrays
+ * are created with cl_precision="single" (base/particle_array.pyx:1400-1488,
+ * sph/sph_calc.pyx:444-714).  Here the particle arrays stay FP64; the
+ * sorted records are float and the list predicate and operator arithmetic
+ * run in float.  Pairs at d ~ 2h may differ from the FP64 lists. */
+typedef struct sph_prop_list_f32 {
+    int32_t nprops;
+    int64_t dst_stride;
+    const double *src[SPH_MAX_PROPS];
+    float *dst[SPH_MAX_PROPS];
+} sph_prop_list_f32;
+/* dst[k][t * dst_stride] = (float)src[k][perm[t]] */
+int sph_gather_sorted_f32(sph_ctx *ctx, int64_t n, const int32_t *perm,
+                          const sph_prop_list_f32 *props);
+
 /* Neighbour lists of FixedDestNbrParticleLocator (base/nnps.pyx:458-621)
  * in SORTED space, stored as sliced ELLPACK with 64-row slices (SELL-64):
  * destinations are grouped into slices of 64 consecutive sorted positions;
@@ -160,6 +176,20 @@ int sph_nnps_build(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
                    double radius_scale, int same_array, int32_t *counts, int64_t *slice_off,
                    int32_t *indices, int64_t cap, int64_t *slots_out /* host */,
                    int64_t *nnz_out /* host */);
+
+/* sph_nnps_build over float records (d_rec, s_rec with SPH_REC floats per
+ * particle): the distance test d2 < (h*radius_scale)^2 is evaluated in
+ * float.  d_x, d_y, d_z are the destination's double coordinates in
+ * original order (the ones the binning used), from which each destination
+ * cell's position is taken. */
+int sph_nnps_build_f32(sph_ctx *ctx, const sph_grid *grid, const float *d_rec,
+                       const double *d_x, const double *d_y, const double *d_z,
+                       const int32_t *d_perm, const int32_t *d_cell_start,
+                       const int32_t *d_cells, int64_t nd_cells, int64_t nd_total,
+                       int64_t nd_real, const int32_t *s_cell_start, const float *s_rec,
+                       int64_t ns, double radius_scale, int same_array, int32_t *counts,
+                       int64_t *slice_off, int32_t *indices, int64_t cap, int64_t *slots_out,
+                       int64_t *nnz_out);
 
 /* The same lists in ORIGINAL index space as CSR, the layout of the
  * reference's per-destination LongArrays (nnps.pyx:596-621): row i
@@ -208,6 +238,23 @@ int sph_momentum_av(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
                     const double *d_rec, double *s_rec, int64_t ns, int uniform_h, double alpha,
                     double beta, double eta, int accumulate, double *ax, double *ay, double *az,
                     double *dt_fac);
+
+/* Single-precision twins of the three operators: float records, float
+ * arithmetic, results widened to the double original-order arrays (out_s,
+ * p_s, cs_s are float record fields). */
+int sph_summation_density_f32(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                              const int32_t *d_perm, const int32_t *counts,
+                              const int64_t *slice_off, const int32_t *indices,
+                              const float *d_rec, const float *s_rec, int uniform_h,
+                              int accumulate, double *rho, float *out_s, int64_t out_s_stride);
+int sph_tait_f32(sph_ctx *ctx, int64_t n_total, int64_t n_real, const int32_t *perm,
+                 const float *rho_s, int64_t s_stride, double ro, double co, double gamma,
+                 int accumulate, double *p, double *cs, float *p_s, float *cs_s);
+int sph_momentum_av_f32(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                        const int32_t *d_perm, const int32_t *counts, const int64_t *slice_off,
+                        const int32_t *indices, const float *d_rec, float *s_rec, int64_t ns,
+                        int uniform_h, double alpha, double beta, double eta, int accumulate,
+                        double *ax, double *ay, double *az, double *dt_fac);
 
 /* GravityForce (sph/funcs/external_force.pyx:38-44): a (+)= g; az may be
  * NULL.  Original order. */
@@ -280,6 +327,7 @@ typedef struct sph_batch_desc {
     int64_t max_cells; /* non-empty cells */
     int64_t list_cap;  /* neighbour-list slots */
     int32_t maxch;     /* candidate chunks of 64 replayed by the list fill, 1..16 */
+    int32_t f32;       /* 1: float records, lists and operators (state stays FP64) */
     void *work;        /* device workspace of sph_batch_workspace_size bytes */
     int64_t work_bytes;
     int32_t *status;   /* device */

@@ -39,6 +39,13 @@ class SphGrid(ctypes.Structure):
                 ("ncells", _i64)]
 
 
+class PropListF32(ctypes.Structure):
+    _fields_ = [("nprops", ctypes.c_int32),
+                ("dst_stride", _i64),
+                ("src", _vp * MAX_PROPS),
+                ("dst", _vp * MAX_PROPS)]
+
+
 class StepProps(ctypes.Structure):
     _fields_ = [("nprops", ctypes.c_int32),
                 ("x", _vp * MAX_PROPS),
@@ -64,7 +71,8 @@ class BatchDesc(ctypes.Structure):
                 + [(k, _dbl) for k in ("co", "ro", "gamma", "alpha", "beta", "eta",
                                        "radius_scale")]
                 + [("key_cap", _i64), ("max_cells", _i64), ("list_cap", _i64),
-                   ("maxch", ctypes.c_int32), ("work", _vp), ("work_bytes", _i64),
+                   ("maxch", ctypes.c_int32), ("f32", ctypes.c_int32), ("work", _vp),
+                   ("work_bytes", _i64),
                    ("status", _vp), ("pairs", _vp)])
 
 
@@ -97,6 +105,16 @@ _SIGNATURES = {
     "sph_stage_step": (_i32, [_vp, _i64, ctypes.POINTER(StepProps), _dbl]),
     "sph_pc_final": (_i32, [_vp, _i64, ctypes.POINTER(StepProps)]),
     "sph_copy": (_i32, [_vp, _i64, _vp, _vp]),
+    "sph_gather_sorted_f32": (_i32, [_vp, _i64, _vp, ctypes.POINTER(PropListF32)]),
+    "sph_nnps_build_f32": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _vp, _vp, _vp,
+                                  _vp, _i64, _i64, _i64, _vp, _vp, _i64, _dbl, _i32, _vp, _vp,
+                                  _vp, _i64, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "sph_summation_density_f32": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp,
+                                         _i32, _i32, _vp, _vp, _i64]),
+    "sph_tait_f32": (_i32, [_vp, _i64, _i64, _vp, _vp, _i64, _dbl, _dbl, _dbl, _i32, _vp, _vp,
+                            _vp, _vp]),
+    "sph_momentum_av_f32": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
+                                   _i32, _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp, _vp]),
     "sph_batch_workspace_size": (_i32, [_vp, _vp, ctypes.POINTER(_i64)]),
     "sph_batch_rk2_step": (_i32, [_vp, _vp, _dbl]),
     "sph_batch_graph_create": (_i32, [_vp, _vp, _dbl, _i32, ctypes.POINTER(_vp)]),

@@ -46,7 +46,7 @@ class _Bins:
     """Per-array result of the counting sort and its sorted records (one
     row of REC doubles per particle, fields REC_FIELDS)."""
 
-    def __init__(self, n, ncells, device):
+    def __init__(self, n, ncells, device, dtype=torch.float64):
         self.n = n
         self.device = device
         self.cell_start = torch.empty(ncells + 1, dtype=torch.int32, device=device)
@@ -55,11 +55,11 @@ class _Bins:
         self.cell_of = torch.empty(max(n, 1), dtype=torch.int32, device=device)
         self.cells = torch.empty(max(min(n, ncells), 1), dtype=torch.int32, device=device)
         self.ncells_nonempty = 0
-        self.rec = torch.zeros((max(n, 1), REC), dtype=torch.float64, device=device)
+        self.rec = torch.zeros((max(n, 1), REC), dtype=dtype, device=device)
         self.shadow_ver = {}
 
     def field_ptr(self, prop):
-        return self.rec.data_ptr() + 8 * REC_FIELDS[prop]
+        return self.rec.data_ptr() + self.rec.element_size() * REC_FIELDS[prop]
 
     def invalidate(self):
         self.shadow_ver = {}
@@ -75,7 +75,8 @@ class CellManager:
     ORDERS = {"morton": _lib.ORDER_MORTON, "linear": _lib.ORDER_LINEAR}
 
     def __init__(self, arrays_to_bin=(), min_cell_size=-1.0, max_cell_size=0.0, initialize=True,
-                 radius_scale=2.0, domain=None, order="morton", max_cells=1 << 28):
+                 radius_scale=2.0, domain=None, order="morton", max_cells=1 << 28,
+                 precision="double"):
         self.arrays_to_bin = list(arrays_to_bin)
         self.min_cell_size = float(min_cell_size)
         self.max_cell_size = float(max_cell_size)
@@ -83,6 +84,12 @@ class CellManager:
         self.domain = domain
         self.order = order
         self.max_cells = int(max_cells)
+        if precision not in ("double", "single"):
+            raise ValueError("precision must be 'double' or 'single'")
+        # "single": float sorted records, lists and operator arithmetic in
+        # float (the reference's cl_precision="single")
+        self.precision = precision
+        self.rec_dtype = torch.float32 if precision == "single" else torch.float64
         self.array_indices = {a.name: i for i, a in enumerate(self.arrays_to_bin)}
         self.cell_size = 0.0
         self.grid = None
@@ -157,7 +164,7 @@ class CellManager:
             if same_grid and b is not None and b.n == np_ and not a.is_dirty:
                 continue  # unmoved array on an unchanged grid: its bins still hold
             if b is None or b.n != np_ or b.cell_start.numel() != grid.ncells + 1:
-                b = _Bins(np_, grid.ncells, a.device)
+                b = _Bins(np_, grid.ncells, a.device, self.rec_dtype)
                 self.bins[a.name] = b
             nne = ctypes.c_int64(0)
             ctx.check(lib.sph_bin_array(ctx.handle, ctypes.byref(grid),
@@ -195,7 +202,8 @@ class CellManager:
             stale = []
         if stale:
             ctx = self._ctx()
-            pl = _lib.PropList()
+            single = self.precision == "single"
+            pl = _lib.PropListF32() if single else _lib.PropList()
             pl.nprops = len(stale)
             pl.dst_stride = REC
             for k, p in enumerate(stale):
@@ -206,9 +214,9 @@ class CellManager:
                     raise TypeError(f"property {p} of {array.name} must be float64")
                 pl.src[k] = _lib.ptr(src)
                 pl.dst[k] = b.field_ptr(p)
+            gather = ctx.lib.sph_gather_sorted_f32 if single else ctx.lib.sph_gather_sorted
             with profiling.region("gather_sorted"):
-                ctx.check(ctx.lib.sph_gather_sorted(ctx.handle, b.n, _lib.ptr(b.perm),
-                                                    ctypes.byref(pl)))
+                ctx.check(gather(ctx.handle, b.n, _lib.ptr(b.perm), ctypes.byref(pl)))
             for p in stale:
                 b.shadow_ver[p] = array.version(p)
         return b.rec
@@ -321,6 +329,14 @@ class FixedDestNbrParticleLocator:
         p = _lib.ptr
 
         def call():
+            if cm.precision == "single":
+                return lib.sph_nnps_build_f32(
+                    ctx.handle, ctypes.byref(cm.grid), p(drec), p(dst.get_carray("x")),
+                    p(dst.get_carray("y")), p(dst.get_carray("z")), p(bd.perm),
+                    p(bd.cell_start), p(bd.cells), bd.ncells_nonempty, nd_total, nd_real,
+                    p(bs.cell_start), p(srec), bs.n, self.radius_scale, int(src is dst),
+                    p(self.counts), p(self.slice_off), p(self.indices), self.indices.numel(),
+                    ctypes.byref(slots), ctypes.byref(nnz))
             return lib.sph_nnps_build(ctx.handle, ctypes.byref(cm.grid), p(drec), p(bd.perm),
                                       p(bd.cell_start), p(bd.cells), bd.ncells_nonempty,
                                       nd_total, nd_real, p(bs.cell_start), p(srec), bs.n,

@@ -14,7 +14,7 @@ class Particles:
     def __init__(self, arrays=(), in_parallel=False, variable_h=False, load_balancing=True,
                  locator_type=NeighborLocatorType.SPHNeighborLocator, periodic_domain=None,
                  min_cell_size=-1, max_cell_size=0, max_radius_scale=2, update_particles=True,
-                 cell_order="morton"):
+                 cell_order="morton", precision=None):
         assert len(arrays) > 0, "Particles must be given some arrays!"
         if in_parallel:
             raise NotImplementedError("the MPI path is not part of this build")
@@ -23,6 +23,10 @@ class Particles:
         prec = arrays[0].cl_precision
         for arr in arrays[1:]:
             assert arr.cl_precision == prec, "All arrays must have the same cl_precision"
+        # the arrays' cl_precision selects the device arithmetic, as it does
+        # for the reference's OpenCL path: "single" -> float records, lists
+        # and operators
+        self.precision = precision or ("single" if prec == "single" else "double")
         self.variable_h = variable_h
         self.in_parallel = False
         self.load_balancing = load_balancing
@@ -44,7 +48,7 @@ class Particles:
                                         min_cell_size=self.min_cell_size,
                                         max_cell_size=self.max_cell_size,
                                         radius_scale=self.max_radius_scale,
-                                        order=self.cell_order)
+                                        order=self.cell_order, precision=self.precision)
         self.nnps_manager = NNPSManager(cell_manager=self.cell_manager,
                                         variable_h=self.variable_h,
                                         locator_type=self.locator_type)

@@ -63,11 +63,14 @@ class BatchedSimulation:
     sims: list of dicts of numpy arrays x, y, z, u, v, w, h, m, rho, p, cs
     (missing z, w, p are zeros).  Physics: TaitEquation(co, ro, gamma),
     MomentumEquation(alpha, beta, eta), cubic spline, radius scale 2.
+    precision "single": float records, lists and operators (the state and
+    the stage arithmetic stay FP64).
     """
 
     def __init__(self, sims, dim=2, co=1400.0, ro=1.0, gamma=7.0, alpha=0.1, beta=0.0, eta=0.1,
                  radius_scale=2.0, device=None, key_cap=None, max_cells=None, list_cap=None,
-                 maxch=None, cell_margin=4.0, list_slots_per_particle=None, dt_per_sim=None):
+                 maxch=None, cell_margin=4.0, list_slots_per_particle=None, dt_per_sim=None,
+                 precision="double"):
         if not torch.cuda.is_available():
             raise RuntimeError("pysph_amd needs a HIP device; it has no CPU fallback")
         if dim not in (2, 3):
@@ -126,6 +129,10 @@ class BatchedSimulation:
             setattr(d, k, float(v))
         d.key_cap, d.max_cells, d.list_cap, d.maxch = (self.key_cap, self.max_cells,
                                                        self.list_cap, self.maxch)
+        if precision not in ("double", "single"):
+            raise ValueError("precision must be 'double' or 'single'")
+        self.precision = precision
+        d.f32 = 1 if precision == "single" else 0
         d.status, d.pairs = self.status.data_ptr(), self.pairs.data_ptr()
         d.dt_sim = self.dt_sim.data_ptr() if self.dt_sim is not None else None
         nbytes = ctypes.c_int64(0)

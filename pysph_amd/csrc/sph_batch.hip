@@ -423,7 +423,7 @@ BatchWork carve(const sph_batch_desc *d, char *base)
     w.cells = (int32_t *)take(sizeof(int32_t) * (size_t)kc);
     w.slot = (int32_t *)take(sizeof(int32_t) * (size_t)(kc + 1));
     w.tmp = (int32_t *)take(sizeof(int32_t) * (size_t)n);
-    w.rec = (double *)take(sizeof(double) * SPH_REC * (size_t)n);
+    w.rec = (double *)take(sizeof(double) * SPH_REC * (size_t)n);  // float records when f32
     w.counts = (int32_t *)take(sizeof(int32_t) * (size_t)(n + 1));
     w.slice_off = (int64_t *)take(sizeof(int64_t) * 2 * (size_t)(nsl + 1));
     w.indices = (int32_t *)take(sizeof(int32_t) * (size_t)d->list_cap);
@@ -445,6 +445,7 @@ int check_desc(sph_ctx *ctx, const sph_batch_desc *d)
                          d->max_cells >= 1 && d->max_cells <= d->key_cap,
                 "batch: bad capacities");
     SPH_REQUIRE(ctx, d->maxch >= 1 && d->maxch <= 16, "batch: bad maxch");
+    SPH_REQUIRE(ctx, d->f32 == 0 || d->f32 == 1, "batch: bad f32 flag");
     SPH_REQUIRE(ctx, d->sim_off && d->sim_of && d->x && d->y && d->z && d->u && d->v && d->w &&
                          d->h && d->m && d->rho && d->p && d->cs && d->x0 && d->y0 && d->u0 &&
                          d->v0 && d->ax && d->ay && d->au && d->av && d->dt_fac && d->status &&
@@ -485,37 +486,66 @@ int batch_eval(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w)
                        d->max_cells, d->status);
     SPH_LAUNCH_CHECK(ctx, "k_batch_check_cells");
     // -- sorted records of the fields the operators read
-    sph_prop_list pl;
-    memset(&pl, 0, sizeof(pl));
-    pl.dst_stride = SPH_REC;
     const double *src[8] = {d->x, d->y, d->z, d->m, d->u, d->v, d->w, d->h};
     const int fld[8] = {SPH_R_X, SPH_R_Y, SPH_R_Z, SPH_R_M, SPH_R_U, SPH_R_V, SPH_R_W, SPH_R_H};
-    pl.nprops = 8;
-    for (int k = 0; k < 8; ++k) {
-        pl.src[k] = src[k];
-        pl.dst[k] = w.rec + fld[k];
+    float *rec32 = d->f32 ? (float *)w.rec : nullptr;
+    if (rec32) {
+        sph_prop_list_f32 pl;
+        memset(&pl, 0, sizeof(pl));
+        pl.dst_stride = SPH_REC;
+        pl.nprops = 8;
+        for (int k = 0; k < 8; ++k) {
+            pl.src[k] = src[k];
+            pl.dst[k] = rec32 + fld[k];
+        }
+        rc = sph_gather_sorted_f32(ctx, n, w.perm, &pl);
+    } else {
+        sph_prop_list pl;
+        memset(&pl, 0, sizeof(pl));
+        pl.dst_stride = SPH_REC;
+        pl.nprops = 8;
+        for (int k = 0; k < 8; ++k) {
+            pl.src[k] = src[k];
+            pl.dst[k] = w.rec + fld[k];
+        }
+        rc = sph_gather_sorted(ctx, n, w.perm, &pl);
     }
-    rc = sph_gather_sorted(ctx, n, w.perm, &pl);
     if (rc) return rc;
     // -- neighbour lists
-    rc = sph_nnps_build_batch(ctx, w.grids, d->sim_of, w.rec, w.perm, w.cell_start, w.cells,
-                              ncell_dev, d->max_cells, n, d->radius_scale, d->maxch, w.counts,
-                              w.slice_off, w.indices, d->list_cap, d->status, d->pairs,
-                              trash);
+    rc = sph_nnps_build_batch(ctx, w.grids, d->sim_of, rec32 ? nullptr : w.rec, rec32, d->x,
+                              d->y, d->z, w.perm, w.cell_start, w.cells, ncell_dev,
+                              d->max_cells, n, d->radius_scale, d->maxch, w.counts, w.slice_off,
+                              w.indices, d->list_cap, d->status, d->pairs, trash);
     if (rc) return rc;
-    // -- SPHRho, Tait (non-integrating: outputs start from 0)
-    rc = sph_summation_density(ctx, d->dim, n, n, w.perm, w.counts, w.slice_off, w.indices,
-                               w.rec, w.rec, 0, 0, d->rho, w.rec + SPH_R_RHO, SPH_REC);
-    if (rc) return rc;
-    rc = sph_tait(ctx, n, n, w.perm, w.rec + SPH_R_RHO, SPH_REC, d->ro, d->co, d->gamma, 0,
-                  d->p, d->cs, w.rec + SPH_R_P, w.rec + SPH_R_CS);
-    if (rc) return rc;
-    // -- MomentumEquation (dt_fac reset per calc), PositionStepping
-    rc = sph_fill(ctx, n, 0.0, d->dt_fac);
-    if (rc) return rc;
-    rc = sph_momentum_av(ctx, d->dim, n, n, w.perm, w.counts, w.slice_off, w.indices, w.rec,
-                         w.rec, n, 0, d->alpha, d->beta, d->eta, 0, d->au, d->av,
-                         d->dim == 3 ? d->aw : nullptr, d->dt_fac);
+    // -- SPHRho, Tait (non-integrating: outputs start from 0), MomentumEquation
+    // (dt_fac reset per calc), PositionStepping
+    double *az = d->dim == 3 ? d->aw : nullptr;
+    if (rec32) {
+        rc = sph_summation_density_f32(ctx, d->dim, n, n, w.perm, w.counts, w.slice_off,
+                                       w.indices, rec32, rec32, 0, 0, d->rho,
+                                       rec32 + SPH_R_RHO, SPH_REC);
+        if (rc) return rc;
+        rc = sph_tait_f32(ctx, n, n, w.perm, rec32 + SPH_R_RHO, SPH_REC, d->ro, d->co, d->gamma,
+                          0, d->p, d->cs, rec32 + SPH_R_P, rec32 + SPH_R_CS);
+        if (rc) return rc;
+        rc = sph_fill(ctx, n, 0.0, d->dt_fac);
+        if (rc) return rc;
+        rc = sph_momentum_av_f32(ctx, d->dim, n, n, w.perm, w.counts, w.slice_off, w.indices,
+                                 rec32, rec32, n, 0, d->alpha, d->beta, d->eta, 0, d->au, d->av,
+                                 az, d->dt_fac);
+    } else {
+        rc = sph_summation_density(ctx, d->dim, n, n, w.perm, w.counts, w.slice_off, w.indices,
+                                   w.rec, w.rec, 0, 0, d->rho, w.rec + SPH_R_RHO, SPH_REC);
+        if (rc) return rc;
+        rc = sph_tait(ctx, n, n, w.perm, w.rec + SPH_R_RHO, SPH_REC, d->ro, d->co, d->gamma, 0,
+                      d->p, d->cs, w.rec + SPH_R_P, w.rec + SPH_R_CS);
+        if (rc) return rc;
+        rc = sph_fill(ctx, n, 0.0, d->dt_fac);
+        if (rc) return rc;
+        rc = sph_momentum_av(ctx, d->dim, n, n, w.perm, w.counts, w.slice_off, w.indices, w.rec,
+                             w.rec, n, 0, d->alpha, d->beta, d->eta, 0, d->au, d->av, az,
+                             d->dt_fac);
+    }
     if (rc) return rc;
     return sph_position_stepping(ctx, n, n, d->u, d->v, d->dim == 3 ? d->w : nullptr, d->ax,
                                  d->ay, d->dim == 3 ? d->az : nullptr);
@@ -637,6 +667,7 @@ extern "C" int sph_batch_vjp(sph_ctx *ctx, const sph_batch_desc *d, const double
     if (rc) return rc;
     SPH_REQUIRE(ctx, abar_u && abar_v && xbar && ybar && ubar && vbar && alpha_bar,
                 "sph_batch_vjp: null argument");
+    SPH_REQUIRE(ctx, !d->f32, "sph_batch_vjp: the reverse mode runs in FP64 only");
     SPH_REQUIRE(ctx, d->dim == 2 || (abar_w && zbar && wbar), "sph_batch_vjp: 3D needs w/z");
     const int64_t n = d->n;
     hipLaunchKernelGGL(k_gather3, dim3(sph_blocks(n, BT_THREADS)), dim3(BT_THREADS), 0,

@@ -110,9 +110,12 @@ int sph_bin_sort(sph_ctx *ctx, int64_t n, int64_t nc, const int32_t *cell_of, in
 // of original particle i, *ncell_dev the number of non-empty cells (at most
 // max_cells).  Lists needing more than cap slots set bit 0 of *status and
 // are not written; nnz is added to *nnz_acc.  Particles in cell `trash` get
-// empty rows.
+// empty rows.  With rec32 != NULL the records are float (rec unused) and
+// ox, oy, oz are the double coordinates in original order.
 int sph_nnps_build_batch(sph_ctx *ctx, const GridDev *grids, const int32_t *sim_of,
-                         const double *rec, const int32_t *perm, const int32_t *cell_start,
+                         const double *rec, const float *rec32, const double *ox,
+                         const double *oy, const double *oz, const int32_t *perm,
+                         const int32_t *cell_start,
                          const int32_t *cells, const int32_t *ncell_dev, int64_t max_cells,
                          int64_t n, double radius_scale, int maxch, int32_t *counts,
                          int64_t *slice_off, int32_t *indices, int64_t cap, int32_t *status,
@@ -138,18 +141,21 @@ size_t sph_scan_scratch_bytes(int64_t n, size_t elem);
 // ------------------------------------------------------------------------
 #define SPH_PI 3.141592653589793
 
-__device__ __forceinline__ double sph_h_dim(double h, int dim)
+// Every helper is a template over the arithmetic type T (double: the
+// reference's FP64 path; float: the single-precision variant); the double
+// instantiation evaluates exactly the expressions above.
+template <typename T> __device__ __forceinline__ T sph_h_dim(T h, int dim)
 {
-    if (dim == 1) return 1 / h;
-    else if (dim == 2) return 1 / (h * h);
-    else return 1 / (h * h * h);
+    if (dim == 1) return T(1) / h;
+    else if (dim == 2) return T(1) / (h * h);
+    else return T(1) / (h * h * h);
 }
 
-__host__ __device__ __forceinline__ double sph_cubic_fac1(int dim)
+template <typename T = double> __host__ __device__ __forceinline__ T sph_cubic_fac1(int dim)
 {
-    if (dim == 1) return (2. / 3.) / 1.0;
-    else if (dim == 2) return 10 / (7 * SPH_PI) / (1.0 * 1.0);
-    else return 1. / SPH_PI / (1.0 * 1.0 * 1.0);
+    if (dim == 1) return T((2. / 3.) / 1.0);
+    else if (dim == 2) return T(10 / (7 * SPH_PI) / (1.0 * 1.0));
+    else return T(1. / SPH_PI / (1.0 * 1.0 * 1.0));
 }
 
 // W(pa, pb, h) given the three coordinate differences pa - pb
@@ -170,18 +176,18 @@ __device__ __forceinline__ double sph_cubic_w(int dim, double rx, double ry, dou
 
 // W with the factor h_dim(h)*sigma supplied by the caller (the same value
 // sph_cubic_w computes)
-__device__ __forceinline__ double sph_cubic_w_fac(double rx, double ry, double rz, double h,
-                                                  double fac)
+template <typename T>
+__device__ __forceinline__ T sph_cubic_w_fac(T rx, T ry, T rz, T h, T fac)
 {
-    double rab = sqrt(rx * rx + ry * ry + rz * rz);
-    double q = rab / h;
-    double val;
-    if (q > 2.0)
-        val = 0.0;
-    else if (q > 1.0)
-        val = 0.25 * (2 - q) * (2 - q) * (2 - q);
+    T rab = sqrt(rx * rx + ry * ry + rz * rz);
+    T q = rab / h;
+    T val;
+    if (q > T(2.0))
+        val = T(0.0);
+    else if (q > T(1.0))
+        val = T(0.25) * (T(2) - q) * (T(2) - q) * (T(2) - q);
     else
-        val = 1 - 1.5 * (q * q) * (1 - 0.5 * q);
+        val = T(1) - T(1.5) * (q * q) * (T(1) - T(0.5) * q);
     return val * fac;
 }
 
@@ -189,16 +195,16 @@ __device__ __forceinline__ double sph_cubic_w_fac(double rx, double ry, double r
 // branches of kernels.pyx:408-411 divide different numerators by different
 // denominators; selecting both first and dividing once gives the same
 // quotient with one division per pair instead of two divergent ones.
-__device__ __forceinline__ double sph_cubic_gfac_fac(double rx, double ry, double rz, double h,
-                                                     double fac)
+template <typename T>
+__device__ __forceinline__ T sph_cubic_gfac_fac(T rx, T ry, T rz, T h, T fac)
 {
-    const double rab = sqrt(rx * rx + ry * ry + rz * rz);
-    const double q = rab / h;
-    const bool outer = q >= 1.0;
-    const double num = outer ? -0.75 * (2 - q) * (2 - q) : 3.0 * (0.75 * q - 1);
-    const double den = outer ? h * rab : h * h;
-    const double v = num / den;
-    const double val = (q > 2.0 || !(outer || q > 1e-14)) ? 0.0 : v;
+    const T rab = sqrt(rx * rx + ry * ry + rz * rz);
+    const T q = rab / h;
+    const bool outer = q >= T(1.0);
+    const T num = outer ? T(-0.75) * (T(2) - q) * (T(2) - q) : T(3.0) * (T(0.75) * q - T(1));
+    const T den = outer ? h * rab : h * h;
+    const T v = num / den;
+    const T val = (q > T(2.0) || !(outer || q > T(1e-14))) ? T(0.0) : v;
     return val * fac;
 }
 

@@ -456,6 +456,34 @@ __global__ void __launch_bounds__(NN_THREADS)
     for (int k = 0; k < pl.nprops; ++k) pl.dst[k][o] = pl.src[k][i];
 }
 
+__global__ void __launch_bounds__(NN_THREADS)
+    k_gather_sorted_f32(int64_t n, const int32_t *__restrict__ perm, sph_prop_list_f32 pl)
+{
+    int64_t t = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
+    if (t >= n) return;
+    const int32_t i = perm[t];
+    const int64_t o = t * pl.dst_stride;
+    for (int k = 0; k < pl.nprops; ++k) pl.dst[k][o] = (float)pl.src[k][i];
+}
+
+extern "C" int sph_gather_sorted_f32(sph_ctx *ctx, int64_t n, const int32_t *perm,
+                                     const sph_prop_list_f32 *props)
+{
+    if (!ctx) return SPH_EINVAL;
+    SPH_REQUIRE(ctx, props && props->nprops >= 0 && props->nprops <= SPH_MAX_PROPS &&
+                         props->dst_stride >= 1,
+                "sph_gather_sorted_f32: bad props");
+    if (n <= 0 || props->nprops == 0) return SPH_OK;
+    SPH_REQUIRE(ctx, perm, "sph_gather_sorted_f32: null perm");
+    for (int k = 0; k < props->nprops; ++k)
+        SPH_REQUIRE(ctx, props->src[k] && props->dst[k], "sph_gather_sorted_f32: null prop %d",
+                    k);
+    hipLaunchKernelGGL(k_gather_sorted_f32, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
+                       ctx->stream, n, perm, *props);
+    SPH_LAUNCH_CHECK(ctx, "k_gather_sorted_f32");
+    return SPH_OK;
+}
+
 extern "C" int sph_gather_sorted(sph_ctx *ctx, int64_t n, const int32_t *perm,
                                  const sph_prop_list *props)
 {
@@ -483,6 +511,12 @@ extern "C" int sph_gather_sorted(sph_ctx *ctx, int64_t n, const int32_t *perm,
 
 struct CellArgs {
     GridDev g;
+    // records of the destination and source arrays: double, or float when
+    // f32 (then the destination cell coordinates come from ox, oy, oz, the
+    // double coordinates the binning used, in original order)
+    int f32;
+    const float *drec32, *srec32;
+    const double *ox, *oy, *oz;
     const double *drec;
     const int32_t *dperm, *dstart, *dcells;
     int64_t ncell_list, nd_real;
@@ -562,14 +596,50 @@ __device__ __forceinline__ uint64_t wave_bcast_u64(uint64_t v, int l)
 //   flagged in a.cell_ovf.
 // FILL = true: writes the SELL-64 rows of the flagged cells only (the other
 //   cells are written by k_nnps_fill_slices from the masks).
-template <bool FILL>
+template <typename T> struct Quad;
+template <> struct Quad<double> {
+    using type = double4;
+    static __device__ type make(double a, double b, double c, double d)
+    {
+        return make_double4(a, b, c, d);
+    }
+};
+template <> struct Quad<float> {
+    using type = float4;
+    static __device__ type make(float a, float b, float c, float d)
+    {
+        return make_float4(a, b, c, d);
+    }
+};
+template <typename T> __device__ __forceinline__ const T *drec_of(const CellArgs &a);
+template <> __device__ __forceinline__ const double *drec_of<double>(const CellArgs &a)
+{
+    return a.drec;
+}
+template <> __device__ __forceinline__ const float *drec_of<float>(const CellArgs &a)
+{
+    return a.drec32;
+}
+template <typename T> __device__ __forceinline__ const T *srec_of(const CellArgs &a);
+template <> __device__ __forceinline__ const double *srec_of<double>(const CellArgs &a)
+{
+    return a.srec;
+}
+template <> __device__ __forceinline__ const float *srec_of<float>(const CellArgs &a)
+{
+    return a.srec32;
+}
+
+template <bool FILL, typename T>
 __global__ void __launch_bounds__(CW)
     k_nnps_cells(CellArgs a, int32_t *__restrict__ counts, const int64_t *__restrict__ slice_off,
                  int32_t *__restrict__ indices)
 {
     __shared__ int32_t seg_start[MAXSEG];
     __shared__ int32_t seg_end[MAXSEG];  // inclusive prefix of the segment sizes
-    __shared__ double4 dsts[CW];         // (x, y, z, radius^2) of the group's destinations
+    __shared__ typename Quad<T>::type dsts[CW];  // (x, y, z, radius^2) of the group's destinations
+    const T *drec = drec_of<T>(a);
+    const T *srec = srec_of<T>(a);
     const int lane = threadIdx.x;
     const int64_t w = sph_xcd_tile(blockIdx.x, gridDim.x);
     const int64_t ncl = a.ncell_dev ? (int64_t)*a.ncell_dev : a.ncell_list;
@@ -580,17 +650,29 @@ __global__ void __launch_bounds__(CW)
     if (key == a.trash) return;  // rows stay empty
     const int32_t ds = a.dstart[key], de = a.dstart[key + 1];
     const GridDev g = a.grids ? a.grids[a.sim_of[a.dperm[ds]]] : a.g;
-    // the cell's coordinates, recomputed exactly as the binning did
-    const double *r0 = a.drec + (int64_t)ds * SPH_REC;
-    const int cx = min(max(cell_coord(r0[SPH_R_X], g.cs) - g.cmin0, 0), g.d0 - 1);
-    const int cy = min(max(cell_coord(r0[SPH_R_Y], g.cs) - g.cmin1, 0), g.d1 - 1);
-    const int cz = min(max(cell_coord(r0[SPH_R_Z], g.cs) - g.cmin2, 0), g.d2 - 1);
+    // the cell's coordinates, recomputed exactly as the binning did (from
+    // the double coordinates it used)
+    double c0x, c0y, c0z;
+    if (a.ox) {
+        const int32_t i0 = a.dperm[ds];
+        c0x = a.ox[i0];
+        c0y = a.oy[i0];
+        c0z = a.oz[i0];
+    } else {
+        const T *r0 = drec + (int64_t)ds * SPH_REC;
+        c0x = r0[SPH_R_X];
+        c0y = r0[SPH_R_Y];
+        c0z = r0[SPH_R_Z];
+    }
+    const int cx = min(max(cell_coord(c0x, g.cs) - g.cmin0, 0), g.d0 - 1);
+    const int cy = min(max(cell_coord(c0y, g.cs) - g.cmin1, 0), g.d1 - 1);
+    const int cz = min(max(cell_coord(c0z, g.cs) - g.cmin2, 0), g.d2 - 1);
     // stencil reach: the largest (int)ceil(h*rs/cell) over the cell; a
     // destination with a smaller reach only sees extra candidates that fail
     // its distance test, and its own stencil is a sub-box visited in order
     int dl = 0;
     for (int32_t t = ds + lane; t < de; t += CW) {
-        const double r = a.drec[(int64_t)t * SPH_REC + SPH_R_H] * a.rs / g.cs;
+        const double r = (double)drec[(int64_t)t * SPH_REC + SPH_R_H] * a.rs / g.cs;
         // bounded so that a non-finite h (rejected on the host, flagged in a
         // batch) cannot overflow the stencil arithmetic; a reach above 256
         // cells needs a cell size clamped below radius/256
@@ -609,9 +691,9 @@ __global__ void __launch_bounds__(CW)
         const int32_t my_t = g0 + lane;  // destination owned by this lane
         bool mreal = false;
         if (lane < gn) {
-            const double *dr = a.drec + (int64_t)my_t * SPH_REC;
-            const double radius = dr[SPH_R_H] * a.rs;
-            dsts[lane] = make_double4(dr[SPH_R_X], dr[SPH_R_Y], dr[SPH_R_Z], radius * radius);
+            const T *dr = drec + (int64_t)my_t * SPH_REC;
+            const T radius = dr[SPH_R_H] * T(a.rs);
+            dsts[lane] = Quad<T>::make(dr[SPH_R_X], dr[SPH_R_Y], dr[SPH_R_Z], radius * radius);
             mreal = a.dperm[my_t] < a.nd_real;
         }
         const uint64_t realmask = __ballot(mreal);
@@ -643,7 +725,7 @@ __global__ void __launch_bounds__(CW)
                 const int q = c0 + lane;
                 const bool valid = q < total;
                 int32_t p = 0;
-                double sx = 0.0, sy = 0.0, sz = 0.0;
+                T sx = T(0.0), sy = T(0.0), sz = T(0.0);
                 if (valid) {
                     int lo = 0, hi = MAXSEG - 1;  // first segment with end > q
                     while (lo < hi) {
@@ -654,7 +736,7 @@ __global__ void __launch_bounds__(CW)
                     p = seg_start[lo] + (q - prev);
                     if (!FILL && g0 == ds && b0 == 0 && q < a.maxch * CW)
                         a.cand[w * (a.maxch * CW) + q] = p;
-                    const double *sr = a.srec + (int64_t)p * SPH_REC;
+                    const T *sr = srec + (int64_t)p * SPH_REC;
                     sx = sr[SPH_R_X];
                     sy = sr[SPH_R_Y];
                     sz = sr[SPH_R_Z];
@@ -663,10 +745,10 @@ __global__ void __launch_bounds__(CW)
                 for (int d = 0; d < gn; ++d) {
                     if (!((realmask >> d) & 1)) continue;  // non-LocalReal: empty row
                     const int32_t t = g0 + d;
-                    const double4 D = dsts[d];  // same address in every lane
+                    const auto D = dsts[d];  // same address in every lane
                     // cPoint_distance2(src, dst), base/point.pxd:35-37
-                    const double ex = sx - D.x, ey = sy - D.y, ez = sz - D.z;
-                    const double d2 = ex * ex + ey * ey + ez * ez;
+                    const T ex = sx - D.x, ey = sy - D.y, ez = sz - D.z;
+                    const T d2 = ex * ex + ey * ey + ez * ez;
                     const bool hit = valid && d2 < D.w;
                     const uint64_t mask = __ballot(hit);
                     if (mask == 0) continue;
@@ -848,8 +930,9 @@ static int nnps_count(sph_ctx *ctx, CellArgs &a, int64_t ncell_launch, int64_t n
     a.cand = (int32_t *)(sc + masks_bytes + ovf_bytes);
     a.dcell_of = (int32_t *)(sc + masks_bytes + ovf_bytes + cand_bytes);
     a.nslices = nslices;
-    hipLaunchKernelGGL(k_nnps_cells<false>, dim3((unsigned)ncell_launch), dim3(CW), 0,
-                       ctx->stream, a, counts, (const int64_t *)nullptr, (int32_t *)nullptr);
+    auto count_kern = a.f32 ? k_nnps_cells<false, float> : k_nnps_cells<false, double>;
+    hipLaunchKernelGGL(count_kern, dim3((unsigned)ncell_launch), dim3(CW), 0, ctx->stream, a,
+                       counts, (const int64_t *)nullptr, (int32_t *)nullptr);
     SPH_LAUNCH_CHECK(ctx, "k_nnps_cells<count>");
     // slice widths -> offsets (scan); slice totals -> entry total
     int64_t *slice_sum = (int64_t *)(sc + sums_off);
@@ -869,8 +952,9 @@ static int nnps_fill(sph_ctx *ctx, const CellArgs &a, int64_t ncell_launch, int6
     hipLaunchKernelGGL(k_nnps_fill_slices, dim3(sph_blocks(nd_total, NN_THREADS)),
                        dim3(NN_THREADS), 0, ctx->stream, a, nd_total, counts, slice_off, indices);
     SPH_LAUNCH_CHECK(ctx, "k_nnps_fill_slices");
-    hipLaunchKernelGGL(k_nnps_cells<true>, dim3((unsigned)ncell_launch), dim3(CW), 0, ctx->stream,
-                       a, (int32_t *)counts, slice_off, indices);
+    auto fill_kern = a.f32 ? k_nnps_cells<true, float> : k_nnps_cells<true, double>;
+    hipLaunchKernelGGL(fill_kern, dim3((unsigned)ncell_launch), dim3(CW), 0, ctx->stream, a,
+                       (int32_t *)counts, slice_off, indices);
     SPH_LAUNCH_CHECK(ctx, "k_nnps_cells<fill>");
     return SPH_OK;
 }
@@ -898,13 +982,14 @@ static CellArgs cell_args_base()
     return a;
 }
 
-extern "C" int sph_nnps_build(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
-                              const int32_t *d_perm, const int32_t *d_cell_start,
-                              const int32_t *d_cells, int64_t nd_cells, int64_t nd_total,
-                              int64_t nd_real, const int32_t *s_cell_start, const double *s_rec,
-                              int64_t ns, double radius_scale, int same_array, int32_t *counts,
-                              int64_t *slice_off, int32_t *indices, int64_t cap,
-                              int64_t *slots_out, int64_t *nnz_out)
+static int nnps_build(sph_ctx *ctx, const sph_grid *grid, int f32, const double *d_rec,
+                      const float *d_rec32, const double *ox, const double *oy,
+                      const double *oz, const int32_t *d_perm, const int32_t *d_cell_start,
+                      const int32_t *d_cells, int64_t nd_cells, int64_t nd_total,
+                      int64_t nd_real, const int32_t *s_cell_start, const double *s_rec,
+                      const float *s_rec32, int64_t ns, double radius_scale, int same_array,
+                      int32_t *counts, int64_t *slice_off, int32_t *indices, int64_t cap,
+                      int64_t *slots_out, int64_t *nnz_out)
 {
     if (!ctx) return SPH_EINVAL;
     SPH_REQUIRE(ctx, grid && counts && slice_off && slots_out && nnz_out,
@@ -912,8 +997,10 @@ extern "C" int sph_nnps_build(sph_ctx *ctx, const sph_grid *grid, const double *
     SPH_REQUIRE(ctx, nd_total >= 0 && nd_total < ((int64_t)1 << 31) && nd_real >= 0 &&
                          nd_real <= nd_total && nd_cells >= 0 && nd_cells <= nd_total,
                 "sph_nnps_build: bad destination counts");
-    SPH_REQUIRE(ctx, ns == 0 || (s_cell_start && s_rec), "sph_nnps_build: null source arrays");
-    SPH_REQUIRE(ctx, nd_total == 0 || (d_rec && d_perm && d_cell_start && d_cells),
+    SPH_REQUIRE(ctx, ns == 0 || (s_cell_start && (f32 ? (const void *)s_rec32 : s_rec)),
+                "sph_nnps_build: null source arrays");
+    SPH_REQUIRE(ctx, nd_total == 0 || ((f32 ? (const void *)d_rec32 : d_rec) && d_perm &&
+                                       d_cell_start && d_cells && (!f32 || (ox && oy && oz))),
                 "sph_nnps_build: null destination arrays");
     int rc = sph_upload_key_tables(ctx, grid);
     if (rc) return rc;
@@ -925,7 +1012,13 @@ extern "C" int sph_nnps_build(sph_ctx *ctx, const sph_grid *grid, const double *
     if (nd_total == 0 || ns == 0 || nd_real == 0 || nd_cells == 0) return SPH_OK;
     CellArgs a = cell_args_base();
     a.g = make_griddev(ctx, grid);
+    a.f32 = f32;
     a.drec = d_rec;
+    a.drec32 = d_rec32;
+    a.srec32 = s_rec32;
+    a.ox = f32 ? ox : nullptr;
+    a.oy = f32 ? oy : nullptr;
+    a.oz = f32 ? oz : nullptr;
     a.dperm = d_perm;
     a.dstart = d_cell_start;
     a.dcells = d_cells;
@@ -956,8 +1049,40 @@ extern "C" int sph_nnps_build(sph_ctx *ctx, const sph_grid *grid, const double *
     return nnps_fill(ctx, a, nd_cells, nd_total, counts, slice_off, indices);
 }
 
+extern "C" int sph_nnps_build(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
+                              const int32_t *d_perm, const int32_t *d_cell_start,
+                              const int32_t *d_cells, int64_t nd_cells, int64_t nd_total,
+                              int64_t nd_real, const int32_t *s_cell_start, const double *s_rec,
+                              int64_t ns, double radius_scale, int same_array, int32_t *counts,
+                              int64_t *slice_off, int32_t *indices, int64_t cap,
+                              int64_t *slots_out, int64_t *nnz_out)
+{
+    return nnps_build(ctx, grid, 0, d_rec, nullptr, nullptr, nullptr, nullptr, d_perm, d_cell_start,
+                      d_cells, nd_cells, nd_total, nd_real, s_cell_start, s_rec, nullptr, ns,
+                      radius_scale, same_array, counts, slice_off, indices, cap, slots_out,
+                      nnz_out);
+}
+
+extern "C" int sph_nnps_build_f32(sph_ctx *ctx, const sph_grid *grid, const float *d_rec,
+                                  const double *d_x, const double *d_y, const double *d_z,
+                                  const int32_t *d_perm, const int32_t *d_cell_start,
+                                  const int32_t *d_cells, int64_t nd_cells, int64_t nd_total,
+                                  int64_t nd_real, const int32_t *s_cell_start,
+                                  const float *s_rec, int64_t ns, double radius_scale,
+                                  int same_array, int32_t *counts, int64_t *slice_off,
+                                  int32_t *indices, int64_t cap, int64_t *slots_out,
+                                  int64_t *nnz_out)
+{
+    return nnps_build(ctx, grid, 1, nullptr, d_rec, d_x, d_y, d_z,
+                      d_perm, d_cell_start, d_cells, nd_cells, nd_total, nd_real, s_cell_start,
+                      nullptr, s_rec, ns, radius_scale, same_array, counts, slice_off, indices,
+                      cap, slots_out, nnz_out);
+}
+
 int sph_nnps_build_batch(sph_ctx *ctx, const GridDev *grids, const int32_t *sim_of,
-                         const double *rec, const int32_t *perm, const int32_t *cell_start,
+                         const double *rec, const float *rec32, const double *ox,
+                         const double *oy, const double *oz, const int32_t *perm,
+                         const int32_t *cell_start,
                          const int32_t *cells, const int32_t *ncell_dev, int64_t max_cells,
                          int64_t n, double radius_scale, int maxch, int32_t *counts,
                          int64_t *slice_off, int32_t *indices, int64_t cap, int32_t *status,
@@ -971,7 +1096,15 @@ int sph_nnps_build_batch(sph_ctx *ctx, const GridDev *grids, const int32_t *sim_
     a.grids = grids;
     a.sim_of = sim_of;
     a.ncell_dev = ncell_dev;
+    a.f32 = rec32 != nullptr;
     a.drec = rec;
+    a.drec32 = rec32;
+    a.srec32 = rec32;
+    if (a.f32) {
+        a.ox = ox;
+        a.oy = oy;
+        a.oz = oz;
+    }
     a.dperm = perm;
     a.dstart = cell_start;
     a.dcells = cells;

@@ -9,11 +9,17 @@
 
 #define OP_THREADS 256
 
-// Sorted records: fields SPH_R_* of SPH_REC doubles, 16-byte aligned; the
-// neighbour loops fetch them as double2 pairs.
-__device__ __forceinline__ const double2 *rec2(const double *rec, int64_t j)
+// Sorted records: fields SPH_R_* of SPH_REC values of the arithmetic type T
+// (double, or float for the single-precision variant), 16-byte (8-byte)
+// aligned; the neighbour loops fetch them as pairs.
+template <typename T> struct Pair;
+template <> struct Pair<double> { using type = double2; };
+template <> struct Pair<float> { using type = float2; };
+
+template <typename T>
+__device__ __forceinline__ const typename Pair<T>::type *rec2(const T *rec, int64_t j)
 {
-    return reinterpret_cast<const double2 *>(rec + j * SPH_REC);
+    return reinterpret_cast<const typename Pair<T>::type *>(rec + j * SPH_REC);
 }
 
 // ---------------------------------------------------------------------------
@@ -21,31 +27,31 @@ __device__ __forceinline__ const double2 *rec2(const double *rec, int64_t j)
 // ---------------------------------------------------------------------------
 // one neighbour term of SPHRho (density_funcs.pyx:41-73); sxy = (x, y) and
 // szm = (z, m) of the source record, hb its h (not read when UH)
-template <bool UH>
-__device__ __forceinline__ double rho_term(int dim, double xa, double ya, double za, double ha,
-                                           double facA, double2 sxy, double2 szm, double hb)
+template <bool UH, typename T, typename P>
+__device__ __forceinline__ T rho_term(int dim, T xa, T ya, T za, T ha, T facA, P sxy, P szm,
+                                      T hb)
 {
     // with h_b == h_a, 0.5*(h_a+h_b) == h_a exactly and h_dim(h)*sigma is the
     // destination's value: reuse it instead of dividing again
-    double hab, fac;
+    T hab, fac;
     if (UH || hb == ha) {
         hab = ha;
         fac = facA;
     } else {
-        hab = 0.5 * (ha + hb);
-        fac = sph_h_dim(hab, dim) * sph_cubic_fac1(dim);
+        hab = T(0.5) * (ha + hb);
+        fac = sph_h_dim(hab, dim) * sph_cubic_fac1<T>(dim);
     }
-    return sph_cubic_w_fac(xa - sxy.x, ya - sxy.y, za - szm.x, hab, fac) * szm.y;
+    return sph_cubic_w_fac<T>(xa - sxy.x, ya - sxy.y, za - szm.x, hab, fac) * szm.y;
 }
 
-template <bool UH>
+template <bool UH, typename T>
 __global__ void __launch_bounds__(OP_THREADS)
     k_summation_density(int dim, int64_t nd_total, int64_t nd_real,
                         const int32_t *__restrict__ dperm, const int32_t *__restrict__ counts,
                         const int64_t *__restrict__ slice_off,
-                        const int32_t *__restrict__ indices, const double *__restrict__ drec,
-                        const double *__restrict__ srec, int accumulate,
-                        double *__restrict__ rho, double *__restrict__ out_s, int64_t out_stride)
+                        const int32_t *__restrict__ indices, const T *__restrict__ drec,
+                        const T *__restrict__ srec, int accumulate,
+                        double *__restrict__ rho, T *__restrict__ out_s, int64_t out_stride)
 {
     const int64_t t =
         (int64_t)sph_xcd_tile(blockIdx.x, gridDim.x) * OP_THREADS + threadIdx.x;
@@ -56,11 +62,11 @@ __global__ void __launch_bounds__(OP_THREADS)
         if (out_s) out_s[t * out_stride] = 0;
         return;
     }
-    const double2 *dr = rec2(drec, t);
-    const double2 dxy = dr[0], dzm = dr[1];
-    const double xa = dxy.x, ya = dxy.y, za = dzm.x, ha = drec[t * SPH_REC + SPH_R_H];
-    const double facA = sph_h_dim(ha, dim) * sph_cubic_fac1(dim);
-    double nr = 0.0;
+    const auto *dr = rec2(drec, t);
+    const auto dxy = dr[0], dzm = dr[1];
+    const T xa = dxy.x, ya = dxy.y, za = dzm.x, ha = drec[t * SPH_REC + SPH_R_H];
+    const T facA = sph_h_dim(ha, dim) * sph_cubic_fac1<T>(dim);
+    T nr = T(0.0);
     // SELL-64 row: entry k at row[64*k]; each load is one coalesced line for
     // the wavefront and is read once, so it bypasses the caches
     const int32_t *row = indices + slice_off[t >> 6] + (t & 63);
@@ -72,34 +78,56 @@ __global__ void __launch_bounds__(OP_THREADS)
                       j1 = __builtin_nontemporal_load(row + ((int64_t)(k + 1) << 6)),
                       j2 = __builtin_nontemporal_load(row + ((int64_t)(k + 2) << 6)),
                       j3 = __builtin_nontemporal_load(row + ((int64_t)(k + 3) << 6));
-        const double2 *s0 = rec2(srec, j0), *s1 = rec2(srec, j1), *s2 = rec2(srec, j2),
-                      *s3 = rec2(srec, j3);
-        const double2 a0 = s0[0], b0 = s0[1];
-        const double2 a1 = s1[0], b1 = s1[1];
-        const double2 a2 = s2[0], b2 = s2[1];
-        const double2 a3 = s3[0], b3 = s3[1];
-        double h0 = ha, h1 = ha, h2 = ha, h3 = ha;
+        const auto *s0 = rec2(srec, j0), *s1 = rec2(srec, j1), *s2 = rec2(srec, j2),
+                   *s3 = rec2(srec, j3);
+        const auto a0 = s0[0], b0 = s0[1];
+        const auto a1 = s1[0], b1 = s1[1];
+        const auto a2 = s2[0], b2 = s2[1];
+        const auto a3 = s3[0], b3 = s3[1];
+        T h0 = ha, h1 = ha, h2 = ha, h3 = ha;
         if (!UH) {
             h0 = srec[(int64_t)j0 * SPH_REC + SPH_R_H];
             h1 = srec[(int64_t)j1 * SPH_REC + SPH_R_H];
             h2 = srec[(int64_t)j2 * SPH_REC + SPH_R_H];
             h3 = srec[(int64_t)j3 * SPH_REC + SPH_R_H];
         }
-        nr += rho_term<UH>(dim, xa, ya, za, ha, facA, a0, b0, h0);
-        nr += rho_term<UH>(dim, xa, ya, za, ha, facA, a1, b1, h1);
-        nr += rho_term<UH>(dim, xa, ya, za, ha, facA, a2, b2, h2);
-        nr += rho_term<UH>(dim, xa, ya, za, ha, facA, a3, b3, h3);
+        nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, a0, b0, h0);
+        nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, a1, b1, h1);
+        nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, a2, b2, h2);
+        nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, a3, b3, h3);
     }
     for (; k < e; ++k) {
         const int32_t j = __builtin_nontemporal_load(row + ((int64_t)k << 6));
-        const double2 *sr = rec2(srec, j);
-        const double hb = UH ? ha : srec[(int64_t)j * SPH_REC + SPH_R_H];
-        nr += rho_term<UH>(dim, xa, ya, za, ha, facA, sr[0], sr[1], hb);
+        const auto *sr = rec2(srec, j);
+        const T hb = UH ? ha : srec[(int64_t)j * SPH_REC + SPH_R_H];
+        nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, sr[0], sr[1], hb);
     }
-    const double base = accumulate ? (out_s ? out_s[t * out_stride] : rho[i]) : 0.0;
-    const double val = base + nr;
+    const T base = accumulate ? (out_s ? out_s[t * out_stride] : T(rho[i])) : T(0.0);
+    const T val = base + nr;
     rho[i] = val;
     if (out_s) out_s[t * out_stride] = val;
+}
+
+template <typename T>
+static int summation_density(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                             const int32_t *d_perm, const int32_t *counts,
+                             const int64_t *slice_off, const int32_t *indices, const T *d_rec,
+                             const T *s_rec, int uniform_h, int accumulate, double *rho,
+                             T *out_s, int64_t out_s_stride)
+{
+    if (!ctx) return SPH_EINVAL;
+    SPH_REQUIRE(ctx, dim >= 1 && dim <= 3, "sph_summation_density: bad dim %d", dim);
+    SPH_REQUIRE(ctx, nd_real >= 0 && nd_total >= nd_real, "sph_summation_density: bad n");
+    if (nd_total == 0) return SPH_OK;
+    SPH_REQUIRE(ctx, rho && d_perm && counts && slice_off && d_rec &&
+                         (!out_s || out_s_stride >= 1),
+                "sph_summation_density: null argument");
+    auto kern = uniform_h ? k_summation_density<true, T> : k_summation_density<false, T>;
+    hipLaunchKernelGGL(kern, dim3(sph_blocks(nd_total, OP_THREADS)),
+                       dim3(OP_THREADS), 0, ctx->stream, dim, nd_total, nd_real, d_perm, counts,
+                       slice_off, indices, d_rec, s_rec, accumulate, rho, out_s, out_s_stride);
+    SPH_LAUNCH_CHECK(ctx, "k_summation_density");
+    return SPH_OK;
 }
 
 extern "C" int sph_summation_density(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
@@ -109,47 +137,50 @@ extern "C" int sph_summation_density(sph_ctx *ctx, int dim, int64_t nd_total, in
                                      int accumulate, double *rho, double *out_s,
                                      int64_t out_s_stride)
 {
-    if (!ctx) return SPH_EINVAL;
-    SPH_REQUIRE(ctx, dim >= 1 && dim <= 3, "sph_summation_density: bad dim %d", dim);
-    SPH_REQUIRE(ctx, nd_real >= 0 && nd_total >= nd_real, "sph_summation_density: bad n");
-    if (nd_total == 0) return SPH_OK;
-    SPH_REQUIRE(ctx, rho && d_perm && counts && slice_off && d_rec &&
-                         (!out_s || out_s_stride >= 1),
-                "sph_summation_density: null argument");
-    auto kern = uniform_h ? k_summation_density<true> : k_summation_density<false>;
-    hipLaunchKernelGGL(kern, dim3(sph_blocks(nd_total, OP_THREADS)),
-                       dim3(OP_THREADS), 0, ctx->stream, dim, nd_total, nd_real, d_perm, counts,
-                       slice_off, indices, d_rec, s_rec, accumulate, rho, out_s, out_s_stride);
-    SPH_LAUNCH_CHECK(ctx, "k_summation_density");
-    return SPH_OK;
+    return summation_density<double>(ctx, dim, nd_total, nd_real, d_perm, counts, slice_off,
+                                      indices, d_rec, s_rec, uniform_h, accumulate, rho, out_s,
+                                      out_s_stride);
+}
+
+extern "C" int sph_summation_density_f32(sph_ctx *ctx, int dim, int64_t nd_total,
+                                         int64_t nd_real, const int32_t *d_perm,
+                                         const int32_t *counts, const int64_t *slice_off,
+                                         const int32_t *indices, const float *d_rec,
+                                         const float *s_rec, int uniform_h, int accumulate,
+                                         double *rho, float *out_s, int64_t out_s_stride)
+{
+    return summation_density<float>(ctx, dim, nd_total, nd_real, d_perm, counts, slice_off,
+                                     indices, d_rec, s_rec, uniform_h, accumulate, rho, out_s,
+                                     out_s_stride);
 }
 
 // ---------------------------------------------------------------------------
 // Tait equation of state
 // ---------------------------------------------------------------------------
+template <typename T>
 __global__ void __launch_bounds__(OP_THREADS)
     k_tait(int64_t n_total, int64_t n_real, const int32_t *__restrict__ perm,
-           const double *__restrict__ rho_s, int64_t st, double ro, double co, double gamma,
-           double B, int accumulate, double *__restrict__ p, double *__restrict__ cs,
-           double *__restrict__ p_s, double *__restrict__ cs_s)
+           const T *__restrict__ rho_s, int64_t st, T ro, T co, T gamma, T B, int accumulate,
+           double *__restrict__ p, double *__restrict__ cs, T *__restrict__ p_s,
+           T *__restrict__ cs_s)
 {
     int64_t t = (int64_t)blockIdx.x * OP_THREADS + threadIdx.x;
     if (t >= n_total) return;
     const int32_t i = perm[t];
-    double r0 = 0.0, r1 = 0.0;
+    T r0 = T(0.0), r1 = T(0.0);
     if (i < n_real) {
-        const double rhoa = rho_s[t * st];
-        const double ratio = rhoa / ro;
-        const double gamma2 = 0.5 * (gamma - 1.0);
-        const double tmp = pow(ratio, gamma);
-        r0 = (tmp - 1.0) * B;
+        const T rhoa = rho_s[t * st];
+        const T ratio = rhoa / ro;
+        const T gamma2 = T(0.5) * (gamma - T(1.0));
+        const T tmp = pow(ratio, gamma);
+        r0 = (tmp - T(1.0)) * B;
         r1 = pow(ratio, gamma2) * co;
         if (accumulate) {
-            r0 = (p_s ? p_s[t * st] : p[i]) + r0;
-            r1 = (cs_s ? cs_s[t * st] : cs[i]) + r1;
+            r0 = (p_s ? p_s[t * st] : T(p[i])) + r0;
+            r1 = (cs_s ? cs_s[t * st] : T(cs[i])) + r1;
         } else {
-            r0 = 0.0 + r0;
-            r1 = 0.0 + r1;
+            r0 = T(0.0) + r0;
+            r1 = T(0.0) + r1;
         }
     }
     p[i] = r0;
@@ -158,87 +189,103 @@ __global__ void __launch_bounds__(OP_THREADS)
     if (cs_s) cs_s[t * st] = r1;
 }
 
-extern "C" int sph_tait(sph_ctx *ctx, int64_t n_total, int64_t n_real, const int32_t *perm,
-                        const double *rho_s, int64_t s_stride, double ro, double co,
-                        double gamma, int accumulate, double *p, double *cs, double *p_s,
-                        double *cs_s)
+template <typename T>
+static int tait(sph_ctx *ctx, int64_t n_total, int64_t n_real, const int32_t *perm,
+                const T *rho_s, int64_t s_stride, double ro, double co, double gamma,
+                int accumulate, double *p, double *cs, T *p_s, T *cs_s)
 {
     if (!ctx) return SPH_EINVAL;
     SPH_REQUIRE(ctx, n_real >= 0 && n_total >= n_real, "sph_tait: bad n");
     if (n_total == 0) return SPH_OK;
     SPH_REQUIRE(ctx, perm && rho_s && p && cs && s_stride >= 1, "sph_tait: null argument");
-    const double B = co * co * ro / gamma;  // eos_funcs.pyx:90
-    hipLaunchKernelGGL(k_tait, dim3(sph_blocks(n_total, OP_THREADS)), dim3(OP_THREADS), 0,
-                       ctx->stream, n_total, n_real, perm, rho_s, s_stride, ro, co, gamma, B,
-                       accumulate, p, cs, p_s, cs_s);
+    const T B = T(co) * T(co) * T(ro) / T(gamma);  // eos_funcs.pyx:90
+    hipLaunchKernelGGL(k_tait<T>, dim3(sph_blocks(n_total, OP_THREADS)), dim3(OP_THREADS), 0,
+                       ctx->stream, n_total, n_real, perm, rho_s, s_stride, T(ro), T(co),
+                       T(gamma), B, accumulate, p, cs, p_s, cs_s);
     SPH_LAUNCH_CHECK(ctx, "k_tait");
     return SPH_OK;
+}
+
+extern "C" int sph_tait(sph_ctx *ctx, int64_t n_total, int64_t n_real, const int32_t *perm,
+                        const double *rho_s, int64_t s_stride, double ro, double co,
+                        double gamma, int accumulate, double *p, double *cs, double *p_s,
+                        double *cs_s)
+{
+    return tait<double>(ctx, n_total, n_real, perm, rho_s, s_stride, ro, co, gamma, accumulate,
+                        p, cs, p_s, cs_s);
+}
+
+extern "C" int sph_tait_f32(sph_ctx *ctx, int64_t n_total, int64_t n_real, const int32_t *perm,
+                            const float *rho_s, int64_t s_stride, double ro, double co,
+                            double gamma, int accumulate, double *p, double *cs, float *p_s,
+                            float *cs_s)
+{
+    return tait<float>(ctx, n_total, n_real, perm, rho_s, s_stride, ro, co, gamma, accumulate,
+                       p, cs, p_s, cs_s);
 }
 
 // ---------------------------------------------------------------------------
 // MomentumEquation with Monaghan artificial viscosity
 // ---------------------------------------------------------------------------
-struct MomDest {
-    double xa, ya, za, ha, facA, rhoa, ca, Pa, ua, va, wa, rhoa2;
+template <typename T> struct MomDest {
+    T xa, ya, za, ha, facA, rhoa, ca, Pa, ua, va, wa, rhoa2;
 };
 
 // one neighbour term of MomentumEquation (pressure_funcs.pyx:252-391).
 // Source record slots: s0 = (x, y), s1 = (z, m), s2 = (u, v), s3 = (w, P/rho^2),
 // s4 = (rho, cs); hb its h (not read when UH).
-template <bool UH>
-__device__ __forceinline__ void mom_term(int dim, const MomDest &D, double alpha, double beta,
-                                         double eta, const double2 s0, const double2 s1,
-                                         const double2 s2, const double2 s3, const double2 s4,
-                                         double hb, double &dtf, double &nr0, double &nr1,
-                                         double &nr2)
+template <bool UH, typename T, typename P>
+__device__ __forceinline__ void mom_term(int dim, const MomDest<T> &D, T alpha, T beta, T eta,
+                                         const P s0, const P s1, const P s2, const P s3,
+                                         const P s4, T hb, T &dtf, T &nr0, T &nr1, T &nr2)
 {
-    double hab, fac;
+    T hab, fac;
     if (UH || hb == D.ha) {  // 0.5*(h_a+h_a) == h_a: reuse the destination's h_dim(h)*sigma
         hab = D.ha;
         fac = D.facA;
     } else {
-        hab = 0.5 * (D.ha + hb);
-        fac = sph_h_dim(hab, dim) * sph_cubic_fac1(dim);
+        hab = T(0.5) * (D.ha + hb);
+        fac = sph_h_dim(hab, dim) * sph_cubic_fac1<T>(dim);
     }
-    const double rabx = D.xa - s0.x;
-    const double raby = D.ya - s0.y;
-    const double rabz = D.za - s1.x;
-    const double vabx = D.ua - s2.x;
-    const double vaby = D.va - s2.y;
-    const double vabz = D.wa - s3.x;
-    const double dot = vabx * rabx + vaby * raby + vabz * rabz;
-    const double rab2 = rabx * rabx + raby * raby + rabz * rabz;
-    const double den = rab2 + eta * eta * hab * hab;
-    const double q = hab * dot / den;  // the same quotient as mu below
-    const double f = fabs(q);
+    const T rabx = D.xa - s0.x;
+    const T raby = D.ya - s0.y;
+    const T rabz = D.za - s1.x;
+    const T vabx = D.ua - s2.x;
+    const T vaby = D.va - s2.y;
+    const T vabz = D.wa - s3.x;
+    const T dot = vabx * rabx + vaby * raby + vabz * rabz;
+    const T rab2 = rabx * rabx + raby * raby + rabz * rabz;
+    const T den = rab2 + eta * eta * hab * hab;
+    const T q = hab * dot / den;  // the same quotient as mu below
+    const T f = fabs(q);
     dtf = f > dtf ? f : dtf;
-    const double mb = s1.y, rhob = s4.x, cb = s4.y;
+    const T mb = s1.y, rhob = s4.x, cb = s4.y;
     // Pb * (1.0/(rhob*rhob)), computed once per source particle (k_prho2)
-    double tmp = D.Pa * D.rhoa2 + s3.y;
+    T tmp = D.Pa * D.rhoa2 + s3.y;
     // Monaghan viscosity, applied when dot < 0; evaluated for every pair and
     // selected, which costs less than a divergent branch in a wavefront
-    const double cab = 0.5 * (D.ca + cb);
-    const double rhoab = 0.5 * (D.rhoa + rhob);
-    const double mu = q;  // mu = hab*dot; mu /= (rab2 + eta*eta*hab*hab)
-    double pi_v = -alpha * cab * mu + beta * mu * mu;
+    const T cab = T(0.5) * (D.ca + cb);
+    const T rhoab = T(0.5) * (D.rhoa + rhob);
+    const T mu = q;  // mu = hab*dot; mu /= (rab2 + eta*eta*hab*hab)
+    T pi_v = -alpha * cab * mu + beta * mu * mu;
     pi_v /= rhoab;
-    const double piab = dot < 0 ? pi_v : 0.0;
+    const T piab = dot < 0 ? pi_v : T(0.0);
     tmp += piab;
     tmp *= -mb;
-    const double g = sph_cubic_gfac_fac(rabx, raby, rabz, hab, fac);
+    const T g = sph_cubic_gfac_fac<T>(rabx, raby, rabz, hab, fac);
     nr0 += tmp * (rabx * g);
     nr1 += tmp * (raby * g);
     nr2 += tmp * (rabz * g);
 }
 
-template <bool UH>
+template <bool UH, typename T>
 __global__ void __launch_bounds__(OP_THREADS)
     k_momentum(int dim, int64_t nd_total, int64_t nd_real, const int32_t *__restrict__ dperm,
                const int32_t *__restrict__ counts, const int64_t *__restrict__ slice_off,
                const int32_t *__restrict__ indices,
-               const double *__restrict__ drec, const double *__restrict__ srec, double alpha,
-               double beta, double eta, int accumulate, double *__restrict__ ax,
-               double *__restrict__ ay, double *__restrict__ az, double *__restrict__ dt_fac)
+               const T *__restrict__ drec, const T *__restrict__ srec, T alpha, T beta, T eta,
+               int accumulate, double *__restrict__ ax, double *__restrict__ ay,
+               double *__restrict__ az, double *__restrict__ dt_fac)
 {
     const int64_t t =
         (int64_t)sph_xcd_tile(blockIdx.x, gridDim.x) * OP_THREADS + threadIdx.x;
@@ -250,9 +297,9 @@ __global__ void __launch_bounds__(OP_THREADS)
         if (az) az[i] = 0;
         return;
     }
-    const double2 *dr = rec2(drec, t);
-    const double2 d0 = dr[0], d1 = dr[1], d2 = dr[2], d3 = dr[3], d4 = dr[4], d5 = dr[5];
-    MomDest D;
+    const auto *dr = rec2(drec, t);
+    const auto d0 = dr[0], d1 = dr[1], d2 = dr[2], d3 = dr[3], d4 = dr[4], d5 = dr[5];
+    MomDest<T> D;
     D.xa = d0.x;
     D.ya = d0.y;
     D.za = d1.x;
@@ -263,46 +310,74 @@ __global__ void __launch_bounds__(OP_THREADS)
     D.ca = d4.y;
     D.ha = d5.x;
     D.Pa = d5.y;
-    D.facA = sph_h_dim(D.ha, dim) * sph_cubic_fac1(dim);
-    D.rhoa2 = 1.0 / (D.rhoa * D.rhoa);
-    double dtf = dt_fac[i];
-    double nr0 = 0.0, nr1 = 0.0, nr2 = 0.0;
+    D.facA = sph_h_dim(D.ha, dim) * sph_cubic_fac1<T>(dim);
+    D.rhoa2 = T(1.0) / (D.rhoa * D.rhoa);
+    T dtf = T(dt_fac[i]);
+    T nr0 = T(0.0), nr1 = T(0.0), nr2 = T(0.0);
     const int32_t *row = indices + slice_off[t >> 6] + (t & 63);  // SELL-64 row
     const int32_t e = counts[t];
     int32_t k = 0;
     for (; k + 2 <= e; k += 2) {  // two neighbours' loads in flight
         const int32_t jp = __builtin_nontemporal_load(row + ((int64_t)k << 6)),
                       jq = __builtin_nontemporal_load(row + ((int64_t)(k + 1) << 6));
-        const double2 *p = rec2(srec, jp), *q = rec2(srec, jq);
-        const double2 p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3], p4 = p[4];
-        const double2 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4];
-        const double hp = UH ? D.ha : srec[(int64_t)jp * SPH_REC + SPH_R_H];
-        const double hq = UH ? D.ha : srec[(int64_t)jq * SPH_REC + SPH_R_H];
-        mom_term<UH>(dim, D, alpha, beta, eta, p0, p1, p2, p3, p4, hp, dtf, nr0, nr1, nr2);
-        mom_term<UH>(dim, D, alpha, beta, eta, q0, q1, q2, q3, q4, hq, dtf, nr0, nr1, nr2);
+        const auto *p = rec2(srec, jp), *q = rec2(srec, jq);
+        const auto p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3], p4 = p[4];
+        const auto q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4];
+        const T hp = UH ? D.ha : srec[(int64_t)jp * SPH_REC + SPH_R_H];
+        const T hq = UH ? D.ha : srec[(int64_t)jq * SPH_REC + SPH_R_H];
+        mom_term<UH, T>(dim, D, alpha, beta, eta, p0, p1, p2, p3, p4, hp, dtf, nr0, nr1, nr2);
+        mom_term<UH, T>(dim, D, alpha, beta, eta, q0, q1, q2, q3, q4, hq, dtf, nr0, nr1, nr2);
     }
     if (k < e) {
         const int32_t jp = __builtin_nontemporal_load(row + ((int64_t)k << 6));
-        const double2 *p = rec2(srec, jp);
-        const double hp = UH ? D.ha : srec[(int64_t)jp * SPH_REC + SPH_R_H];
-        mom_term<UH>(dim, D, alpha, beta, eta, p[0], p[1], p[2], p[3], p[4], hp, dtf, nr0, nr1,
-                     nr2);
+        const auto *p = rec2(srec, jp);
+        const T hp = UH ? D.ha : srec[(int64_t)jp * SPH_REC + SPH_R_H];
+        mom_term<UH, T>(dim, D, alpha, beta, eta, p[0], p[1], p[2], p[3], p[4], hp, dtf, nr0,
+                        nr1, nr2);
     }
     dt_fac[i] = dtf;
-    ax[i] = (accumulate ? ax[i] : 0.0) + nr0;
-    ay[i] = (accumulate ? ay[i] : 0.0) + nr1;
-    if (az) az[i] = (accumulate ? az[i] : 0.0) + nr2;
+    ax[i] = (accumulate ? T(ax[i]) : T(0.0)) + nr0;
+    ay[i] = (accumulate ? T(ay[i]) : T(0.0)) + nr1;
+    if (az) az[i] = (accumulate ? T(az[i]) : T(0.0)) + nr2;
 }
 
 // P/rho^2 of every source record, the per-pair term of pressure_funcs.pyx:302-305
-__global__ void __launch_bounds__(OP_THREADS) k_prho2(int64_t n, double *__restrict__ rec)
+template <typename T> __global__ void __launch_bounds__(OP_THREADS) k_prho2(int64_t n, T *__restrict__ rec)
 {
     const int64_t j = (int64_t)blockIdx.x * OP_THREADS + threadIdx.x;
     if (j >= n) return;
-    double *r = rec + j * SPH_REC;
-    const double rhob = r[SPH_R_RHO];
-    const double rhob2 = 1.0 / (rhob * rhob);
+    T *r = rec + j * SPH_REC;
+    const T rhob = r[SPH_R_RHO];
+    const T rhob2 = T(1.0) / (rhob * rhob);
     r[SPH_R_PRHO2] = r[SPH_R_P] * rhob2;
+}
+
+template <typename T>
+static int momentum_av(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                       const int32_t *d_perm, const int32_t *counts, const int64_t *slice_off,
+                       const int32_t *indices, const T *d_rec, T *s_rec, int64_t ns,
+                       int uniform_h, double alpha, double beta, double eta, int accumulate,
+                       double *ax, double *ay, double *az, double *dt_fac)
+{
+    if (!ctx) return SPH_EINVAL;
+    SPH_REQUIRE(ctx, dim >= 1 && dim <= 3, "sph_momentum_av: bad dim %d", dim);
+    SPH_REQUIRE(ctx, nd_real >= 0 && nd_total >= nd_real, "sph_momentum_av: bad n");
+    if (nd_total == 0) return SPH_OK;
+    SPH_REQUIRE(ctx, d_rec && ax && ay && dt_fac && d_perm && counts && slice_off && ns >= 0 &&
+                         (ns == 0 || s_rec),
+                "sph_momentum_av: null argument");
+    if (ns > 0) {
+        hipLaunchKernelGGL(k_prho2<T>, dim3(sph_blocks(ns, OP_THREADS)), dim3(OP_THREADS), 0,
+                           ctx->stream, ns, s_rec);
+        SPH_LAUNCH_CHECK(ctx, "k_prho2");
+    }
+    auto kern = uniform_h ? k_momentum<true, T> : k_momentum<false, T>;
+    hipLaunchKernelGGL(kern, dim3(sph_blocks(nd_total, OP_THREADS)), dim3(OP_THREADS), 0,
+                       ctx->stream, dim, nd_total, nd_real, d_perm, counts, slice_off, indices,
+                       d_rec, (const T *)s_rec, T(alpha), T(beta), T(eta), accumulate, ax, ay,
+                       az, dt_fac);
+    SPH_LAUNCH_CHECK(ctx, "k_momentum");
+    return SPH_OK;
 }
 
 extern "C" int sph_momentum_av(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
@@ -313,24 +388,21 @@ extern "C" int sph_momentum_av(sph_ctx *ctx, int dim, int64_t nd_total, int64_t 
                                int accumulate,
                                double *ax, double *ay, double *az, double *dt_fac)
 {
-    if (!ctx) return SPH_EINVAL;
-    SPH_REQUIRE(ctx, dim >= 1 && dim <= 3, "sph_momentum_av: bad dim %d", dim);
-    SPH_REQUIRE(ctx, nd_real >= 0 && nd_total >= nd_real, "sph_momentum_av: bad n");
-    if (nd_total == 0) return SPH_OK;
-    SPH_REQUIRE(ctx, d_rec && ax && ay && dt_fac && d_perm && counts && slice_off && ns >= 0 &&
-                         (ns == 0 || s_rec),
-                "sph_momentum_av: null argument");
-    if (ns > 0) {
-        hipLaunchKernelGGL(k_prho2, dim3(sph_blocks(ns, OP_THREADS)), dim3(OP_THREADS), 0,
-                           ctx->stream, ns, s_rec);
-        SPH_LAUNCH_CHECK(ctx, "k_prho2");
-    }
-    auto kern = uniform_h ? k_momentum<true> : k_momentum<false>;
-    hipLaunchKernelGGL(kern, dim3(sph_blocks(nd_total, OP_THREADS)), dim3(OP_THREADS), 0,
-                       ctx->stream, dim, nd_total, nd_real, d_perm, counts, slice_off, indices,
-                       d_rec, s_rec, alpha, beta, eta, accumulate, ax, ay, az, dt_fac);
-    SPH_LAUNCH_CHECK(ctx, "k_momentum");
-    return SPH_OK;
+    return momentum_av<double>(ctx, dim, nd_total, nd_real, d_perm, counts, slice_off, indices,
+                               d_rec, s_rec, ns, uniform_h, alpha, beta, eta, accumulate, ax, ay,
+                               az, dt_fac);
+}
+
+extern "C" int sph_momentum_av_f32(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                                   const int32_t *d_perm, const int32_t *counts,
+                                   const int64_t *slice_off, const int32_t *indices,
+                                   const float *d_rec, float *s_rec, int64_t ns, int uniform_h,
+                                   double alpha, double beta, double eta, int accumulate,
+                                   double *ax, double *ay, double *az, double *dt_fac)
+{
+    return momentum_av<float>(ctx, dim, nd_total, nd_real, d_perm, counts, slice_off, indices,
+                              d_rec, s_rec, ns, uniform_h, alpha, beta, eta, accumulate, ax, ay,
+                              az, dt_fac);
 }
 
 // ---------------------------------------------------------------------------

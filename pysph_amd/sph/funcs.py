@@ -56,8 +56,10 @@ class SPHRho(CSPHFunctionParticle):
         out_s = cm.shadow_out(self.dest, name) if name else None
         p = _lib.ptr
         profiling.count_pairs(loc.nnz)
+        fn = (ctx.lib.sph_summation_density_f32 if cm.precision == "single"
+              else ctx.lib.sph_summation_density)
         with profiling.region("summation_density"):
-            ctx.check(ctx.lib.sph_summation_density(
+            ctx.check(fn(
                 ctx.handle, kernel.dim, self.dest.get_number_of_particles(),
                 self.dest.num_real_particles, p(cm.bins[self.dest.name].perm), p(cnt), p(soff),
                 p(idx),
@@ -94,19 +96,22 @@ class TaitEquation(SPHFunction):
         n0 = names[0] if names else None
         n1 = names[1] if names else None
         p_s = cs_s = None
+        fn = ctx.lib.sph_tait
         if cm is not None and self.dest.name in cm.array_indices:
             read = ('rho',) + tuple(k for k in (n0, n1) if accumulate and k in REC_FIELDS)
             rec = cm.shadows(self.dest, read)
-            rho_s, stride = rec.data_ptr() + 8 * REC_FIELDS['rho'], REC
+            rho_s, stride = rec.data_ptr() + rec.element_size() * REC_FIELDS['rho'], REC
             perm = cm.bins[self.dest.name].perm
             p_s = cm.shadow_out(self.dest, n0) if n0 else None
             cs_s = cm.shadow_out(self.dest, n1) if n1 else None
+            if cm.precision == "single":
+                fn = ctx.lib.sph_tait_f32
         else:
             cm = None
             rho_s, stride = p(self.d('rho')), 1
             perm = _identity_perm(n, self.dest.device)
         with profiling.region("tait"):
-            ctx.check(ctx.lib.sph_tait(ctx.handle, n, self.dest.num_real_particles, p(perm),
+            ctx.check(fn(ctx.handle, n, self.dest.num_real_particles, p(perm),
                                        rho_s, stride, self.ro, self.co, self.gamma,
                                        int(bool(accumulate)), p(output1), p(output2), p_s, cs_s))
         if cm is not None:
@@ -162,8 +167,9 @@ class MomentumEquation(SPHFunctionParticle):
         srec = cm.shadows(self.source, self.SRC)
         p = _lib.ptr
         profiling.count_pairs(loc.nnz)
+        fn = ctx.lib.sph_momentum_av_f32 if cm.precision == "single" else ctx.lib.sph_momentum_av
         with profiling.region("momentum_av"):
-            ctx.check(ctx.lib.sph_momentum_av(
+            ctx.check(fn(
                 ctx.handle, kernel.dim, self.dest.get_number_of_particles(),
                 self.dest.num_real_particles, p(cm.bins[self.dest.name].perm), p(cnt), p(soff),
                 p(idx),

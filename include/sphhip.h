@@ -335,6 +335,8 @@ typedef struct sph_batch_desc {
 } sph_batch_desc;
 
 int sph_batch_workspace_size(sph_ctx *ctx, const sph_batch_desc *d, int64_t *bytes);
+/* prepare a new workspace (clears its record area); call once per workspace */
+int sph_batch_init_workspace(sph_ctx *ctx, const sph_batch_desc *d);
 /* one RK2 step of every simulation, stream-ordered */
 int sph_batch_rk2_step(sph_ctx *ctx, const sph_batch_desc *d, double dt);
 /* capture nsteps RK2 steps in a HIP graph (the context needs its own

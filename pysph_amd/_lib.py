@@ -117,6 +117,7 @@ _SIGNATURES = {
                                    _i32, _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp, _vp]),
     "sph_batch_workspace_size": (_i32, [_vp, _vp, ctypes.POINTER(_i64)]),
     "sph_batch_rk2_step": (_i32, [_vp, _vp, _dbl]),
+    "sph_batch_init_workspace": (_i32, [_vp, _vp]),
     "sph_batch_graph_create": (_i32, [_vp, _vp, _dbl, _i32, ctypes.POINTER(_vp)]),
     "sph_batch_graph_launch": (_i32, [_vp, _vp]),
     "sph_batch_graph_destroy": (_i32, [_vp, _vp]),

@@ -54,6 +54,10 @@ class Rollout:
                              for _ in range(2 * self.checkpoint)]
             self.alpha_bar = torch.zeros(1, dtype=torch.float64, device=s.device)
         self.main_work = (s.desc.work, s.desc.work_bytes)
+        for wk in self.seg_work:
+            self._use(wk)
+            self._call("sph_batch_init_workspace")
+        self._use(None)
         self.checkpoints = []
         self.alpha = 0.0
 

@@ -143,6 +143,7 @@ class BatchedSimulation:
             self.work = torch.empty(int(nbytes.value), dtype=torch.uint8, device=self.device)
         d.work, d.work_bytes = self.work.data_ptr(), int(nbytes.value)
         self.desc = d
+        self.ctx.check(lib.sph_batch_init_workspace(self.ctx.handle, ctypes.byref(d)))
         self.graph = None
         self.graph_steps = 0
         self.steps_done = 0

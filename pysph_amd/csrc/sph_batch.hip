@@ -29,6 +29,8 @@
 // ---------------------------------------------------------------------------
 // per-simulation grids
 // ---------------------------------------------------------------------------
+#define GRID_THREADS 1024  // one block per simulation for its bounding box
+
 __device__ __forceinline__ double blk_min(double v, double *red)
 {
 #pragma unroll
@@ -38,26 +40,26 @@ __device__ __forceinline__ double blk_min(double v, double *red)
     if (lane == 0) red[wid] = v;
     __syncthreads();
     double r = red[0];
-    for (int w = 1; w < BT_THREADS / 64; ++w) r = fmin(r, red[w]);
+    for (int w = 1; w < GRID_THREADS / 64; ++w) r = fmin(r, red[w]);
     return r;
 }
 
 // One block per simulation: bounding box and h range of its particles, then
 // its grid (CellManager.compute_cell_size with the Particles defaults: no
 // clamps, 0 -> 1.0).  ncell[s] = cells of the grid.
-__global__ void __launch_bounds__(BT_THREADS)
+__global__ void __launch_bounds__(GRID_THREADS)
     k_batch_grid(const int64_t *__restrict__ sim_off, const double *__restrict__ x,
                  const double *__restrict__ y, const double *__restrict__ z,
                  const double *__restrict__ h, double radius_scale, GridDev *__restrict__ grids,
                  int64_t *__restrict__ ncell, int32_t *__restrict__ status)
 {
-    __shared__ double red[BT_THREADS / 64];
+    __shared__ double red[GRID_THREADS / 64];
     const int s = blockIdx.x;
     const int64_t b = sim_off[s], e = sim_off[s + 1];
     double v[8];
     for (int k = 0; k < 8; ++k) v[k] = INFINITY;
     double bad = 0.0;
-    for (int64_t i = b + threadIdx.x; i < e; i += BT_THREADS) {
+    for (int64_t i = b + threadIdx.x; i < e; i += GRID_THREADS) {
         const double a = x[i], bb = y[i], c = z[i], hh = h[i];
         if (!isfinite(a) || !isfinite(bb) || !isfinite(c) || !isfinite(hh)) {
             bad = 1.0;
@@ -118,23 +120,50 @@ __global__ void __launch_bounds__(BT_THREADS)
 // (non-finite values, a grid too large, or no room left in the key space --
 // status |= 2) go there and get no neighbour lists, so a simulation that
 // blew up costs no more than one that did not.
-__global__ void k_batch_key_base(int nsims, const int64_t *__restrict__ ncell, int64_t key_cap,
-                                 GridDev *__restrict__ grids, int32_t *__restrict__ status)
+__global__ void __launch_bounds__(1024)
+    k_batch_key_base(int nsims, const int64_t *__restrict__ ncell, int64_t key_cap,
+                     GridDev *__restrict__ grids, int32_t *__restrict__ status)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    int64_t base = 0;
-    for (int s = 0; s < nsims; ++s) {
-        if (grids[s].d0 == 0) continue;
-        if (base + ncell[s] <= key_cap - 1) {
-            grids[s].key_base = (int32_t)base;
-            base += ncell[s];
-        } else {
-            GridDev g = grids[s];
-            g.key_base = 0;
-            g.d0 = 0;
-            grids[s] = g;
-            *status |= 2;
+    // block-wide exclusive scan of the cells of the live simulations, 1024
+    // simulations at a time; a simulation that does not fit becomes void
+    __shared__ int64_t part[1024 / 64];
+    __shared__ int64_t carry;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int s0 = 0; s0 < nsims; s0 += 1024) {
+        const int s = s0 + threadIdx.x;
+        const bool live = s < nsims && grids[s].d0 != 0;
+        const int64_t c = live ? ncell[s] : 0;
+        int64_t incl = c;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int64_t t = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += t;
         }
+        if (lane == 63) part[wid] = incl;
+        __syncthreads();
+        int64_t before = carry;
+        for (int w = 0; w < wid; ++w) before += part[w];
+        const int64_t base = before + incl - c;
+        if (live) {
+            if (base + c <= key_cap - 1) {
+                grids[s].key_base = (int32_t)base;
+            } else {
+                GridDev g = grids[s];
+                g.key_base = 0;
+                g.d0 = 0;
+                grids[s] = g;
+                atomicOr(status, 2);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int64_t tot = 0;
+            for (int w = 0; w < 1024 / 64; ++w) tot += part[w];
+            carry += tot;
+        }
+        __syncthreads();
     }
 }
 
@@ -463,11 +492,11 @@ int batch_eval(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w)
     const int64_t n = d->n;
     int rc;
     // -- binning into the per-simulation grids
-    hipLaunchKernelGGL(k_batch_grid, dim3(d->nsims), dim3(BT_THREADS), 0, ctx->stream,
+    hipLaunchKernelGGL(k_batch_grid, dim3(d->nsims), dim3(GRID_THREADS), 0, ctx->stream,
                        d->sim_off, d->x, d->y, d->z, d->h, d->radius_scale, w.grids, w.ncell,
                        d->status);
     SPH_LAUNCH_CHECK(ctx, "k_batch_grid");
-    hipLaunchKernelGGL(k_batch_key_base, dim3(1), dim3(64), 0, ctx->stream, d->nsims, w.ncell,
+    hipLaunchKernelGGL(k_batch_key_base, dim3(1), dim3(1024), 0, ctx->stream, d->nsims, w.ncell,
                        d->key_cap, w.grids, d->status);
     SPH_LAUNCH_CHECK(ctx, "k_batch_key_base");
     hipLaunchKernelGGL(k_fill_i32, dim3(sph_blocks(d->key_cap + 1, BT_THREADS)),
@@ -486,15 +515,18 @@ int batch_eval(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w)
                        d->max_cells, d->status);
     SPH_LAUNCH_CHECK(ctx, "k_batch_check_cells");
     // -- sorted records of the fields the operators read
-    const double *src[8] = {d->x, d->y, d->z, d->m, d->u, d->v, d->w, d->h};
-    const int fld[8] = {SPH_R_X, SPH_R_Y, SPH_R_Z, SPH_R_M, SPH_R_U, SPH_R_V, SPH_R_W, SPH_R_H};
+    // in 2D the z and w fields of the records stay zero (the workspace
+    // record area is cleared by sph_batch_init_workspace)
+    const double *src[8] = {d->x, d->y, d->m, d->u, d->v, d->h, d->z, d->w};
+    const int fld[8] = {SPH_R_X, SPH_R_Y, SPH_R_M, SPH_R_U, SPH_R_V, SPH_R_H, SPH_R_Z, SPH_R_W};
+    const int nf = d->dim == 3 ? 8 : 6;
     float *rec32 = d->f32 ? (float *)w.rec : nullptr;
     if (rec32) {
         sph_prop_list_f32 pl;
         memset(&pl, 0, sizeof(pl));
         pl.dst_stride = SPH_REC;
-        pl.nprops = 8;
-        for (int k = 0; k < 8; ++k) {
+        pl.nprops = nf;
+        for (int k = 0; k < nf; ++k) {
             pl.src[k] = src[k];
             pl.dst[k] = rec32 + fld[k];
         }
@@ -503,8 +535,8 @@ int batch_eval(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w)
         sph_prop_list pl;
         memset(&pl, 0, sizeof(pl));
         pl.dst_stride = SPH_REC;
-        pl.nprops = 8;
-        for (int k = 0; k < 8; ++k) {
+        pl.nprops = nf;
+        for (int k = 0; k < nf; ++k) {
             pl.src[k] = src[k];
             pl.dst[k] = w.rec + fld[k];
         }
@@ -525,25 +557,24 @@ int batch_eval(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w)
                                        w.indices, rec32, rec32, 0, 0, d->rho,
                                        rec32 + SPH_R_RHO, SPH_REC);
         if (rc) return rc;
-        rc = sph_tait_f32(ctx, n, n, w.perm, rec32 + SPH_R_RHO, SPH_REC, d->ro, d->co, d->gamma,
-                          0, d->p, d->cs, rec32 + SPH_R_P, rec32 + SPH_R_CS);
+        rc = sph_tait_records_f32(ctx, n, w.perm, rec32, d->ro, d->co, d->gamma, d->p, d->cs);
         if (rc) return rc;
         rc = sph_fill(ctx, n, 0.0, d->dt_fac);
         if (rc) return rc;
+        // ns = 0: P/rho^2 is already in the records
         rc = sph_momentum_av_f32(ctx, d->dim, n, n, w.perm, w.counts, w.slice_off, w.indices,
-                                 rec32, rec32, n, 0, d->alpha, d->beta, d->eta, 0, d->au, d->av,
+                                 rec32, rec32, 0, 0, d->alpha, d->beta, d->eta, 0, d->au, d->av,
                                  az, d->dt_fac);
     } else {
         rc = sph_summation_density(ctx, d->dim, n, n, w.perm, w.counts, w.slice_off, w.indices,
                                    w.rec, w.rec, 0, 0, d->rho, w.rec + SPH_R_RHO, SPH_REC);
         if (rc) return rc;
-        rc = sph_tait(ctx, n, n, w.perm, w.rec + SPH_R_RHO, SPH_REC, d->ro, d->co, d->gamma, 0,
-                      d->p, d->cs, w.rec + SPH_R_P, w.rec + SPH_R_CS);
+        rc = sph_tait_records(ctx, n, w.perm, w.rec, d->ro, d->co, d->gamma, d->p, d->cs);
         if (rc) return rc;
         rc = sph_fill(ctx, n, 0.0, d->dt_fac);
         if (rc) return rc;
         rc = sph_momentum_av(ctx, d->dim, n, n, w.perm, w.counts, w.slice_off, w.indices, w.rec,
-                             w.rec, n, 0, d->alpha, d->beta, d->eta, 0, d->au, d->av, az,
+                             w.rec, 0, 0, d->alpha, d->beta, d->eta, 0, d->au, d->av, az,
                              d->dt_fac);
     }
     if (rc) return rc;
@@ -624,6 +655,19 @@ static int batch_prep(sph_ctx *ctx, const sph_batch_desc *d, BatchWork *w)
     *w = carve(d, (char *)d->work);
     SPH_REQUIRE(ctx, d->work_bytes >= (int64_t)w->bytes, "batch: workspace of %lld bytes < %zu",
                 (long long)d->work_bytes, w->bytes);
+    return SPH_OK;
+}
+
+extern "C" int sph_batch_init_workspace(sph_ctx *ctx, const sph_batch_desc *d)
+{
+    BatchWork w;
+    int rc = batch_prep(ctx, d, &w);
+    if (rc) return rc;
+    // zero the records (2D reads z = w = 0 from them) with a fill kernel
+    const int64_t nwords = (int64_t)(SPH_REC * (size_t)d->n * sizeof(double) / sizeof(int32_t));
+    hipLaunchKernelGGL(k_fill_i32, dim3(sph_blocks(nwords, BT_THREADS)), dim3(BT_THREADS), 0,
+                       ctx->stream, (int32_t *)w.rec, nwords, 0);
+    SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
     return SPH_OK;
 }
 

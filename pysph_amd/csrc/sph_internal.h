@@ -121,6 +121,14 @@ int sph_nnps_build_batch(sph_ctx *ctx, const GridDev *grids, const int32_t *sim_
                          int64_t *slice_off, int32_t *indices, int64_t cap, int32_t *status,
                          int64_t *nnz_acc, int32_t trash);
 
+// TaitEquation over sorted records (all particles real, no accumulation):
+// p, cs into the original-order arrays and the record fields, P/rho^2 into
+// SPH_R_PRHO2 (so a following sph_momentum_av can skip it with ns = 0).
+int sph_tait_records(sph_ctx *ctx, int64_t n, const int32_t *perm, double *rec, double ro,
+                     double co, double gamma, double *p, double *cs);
+int sph_tait_records_f32(sph_ctx *ctx, int64_t n, const int32_t *perm, float *rec, double ro,
+                         double co, double gamma, double *p, double *cs);
+
 // Exclusive prefix sums (device-wide), stream-ordered.  `data` has n
 // elements; the scanned values are written in place and the total is
 // written to data[n] (so data must hold n+1 elements).

@@ -508,6 +508,8 @@ extern "C" int sph_gather_sorted(sph_ctx *ctx, int64_t n, const int32_t *perm,
 #define MAXSEG 64      // stencil cells handled per batch (27 for d = 1)
 
 #define NN_MAXCH 16    // candidate chunks (of 64) whose hit masks the count pass keeps
+#define NN_CELL_GRID (1 << 20)  // blocks of the count pass (one cell at a time each)
+#define NN_OVF_GRID 2048    // blocks of the per-cell fill of flagged cells
 
 struct CellArgs {
     GridDev g;
@@ -546,6 +548,9 @@ struct CellArgs {
     int64_t cap;
     int32_t *status;
     int32_t trash;  // cell key that gets no lists (particles of void simulations), or -1
+    // cells flagged by the count pass, for the per-cell fill
+    int32_t *ovf_list;
+    int32_t *novf;
 };
 
 __device__ __forceinline__ int wave_max_i(int v)
@@ -630,22 +635,16 @@ template <> __device__ __forceinline__ const float *srec_of<float>(const CellArg
     return a.srec32;
 }
 
+// one destination cell w (one wavefront); seg_start, seg_end, dsts are the
+// block's LDS arrays
 template <bool FILL, typename T>
-__global__ void __launch_bounds__(CW)
-    k_nnps_cells(CellArgs a, int32_t *__restrict__ counts, const int64_t *__restrict__ slice_off,
-                 int32_t *__restrict__ indices)
+__device__ void nnps_cell(const CellArgs &a, int64_t w, int32_t *__restrict__ counts,
+                          const int64_t *__restrict__ slice_off, int32_t *__restrict__ indices,
+                          int32_t *seg_start, int32_t *seg_end, typename Quad<T>::type *dsts)
 {
-    __shared__ int32_t seg_start[MAXSEG];
-    __shared__ int32_t seg_end[MAXSEG];  // inclusive prefix of the segment sizes
-    __shared__ typename Quad<T>::type dsts[CW];  // (x, y, z, radius^2) of the group's destinations
     const T *drec = drec_of<T>(a);
     const T *srec = srec_of<T>(a);
     const int lane = threadIdx.x;
-    const int64_t w = sph_xcd_tile(blockIdx.x, gridDim.x);
-    const int64_t ncl = a.ncell_dev ? (int64_t)*a.ncell_dev : a.ncell_list;
-    if (w >= ncl) return;  // uniform: one wavefront per block
-    if (FILL && (a.cell_ovf[w] == 0 || (a.status && slice_off[a.nslices] > a.cap)))
-        return;  // written from the masks, or no room
     const int32_t key = a.dcells[w];
     if (key == a.trash) return;  // rows stay empty
     const int32_t ds = a.dstart[key], de = a.dstart[key + 1];
@@ -672,7 +671,11 @@ __global__ void __launch_bounds__(CW)
     // its distance test, and its own stencil is a sub-box visited in order
     int dl = 0;
     for (int32_t t = ds + lane; t < de; t += CW) {
-        const double r = (double)drec[(int64_t)t * SPH_REC + SPH_R_H] * a.rs / g.cs;
+        double r = (double)drec[(int64_t)t * SPH_REC + SPH_R_H] * a.rs / g.cs;
+        // float h can round up past the h the cell size came from: a reach
+        // of 1 + 1e-7 cells would double the stencil for pairs that only
+        // the float rounding could add
+        if (sizeof(T) == sizeof(float)) r *= 1.0 - 0x1p-20;
         // bounded so that a non-finite h (rejected on the host, flagged in a
         // batch) cannot overflow the stencil arithmetic; a reach above 256
         // cells needs a cell size clamped below radius/256
@@ -797,7 +800,39 @@ __global__ void __launch_bounds__(CW)
             }
         }
     }
-    if (!FILL && lane == 0) a.cell_ovf[w] = nst > MAXSEG || nchunks > a.maxch;
+    if (!FILL && lane == 0) {
+        const int ovf = nst > MAXSEG || nchunks > a.maxch;
+        a.cell_ovf[w] = ovf;
+        if (ovf) a.ovf_list[atomicAdd(a.novf, 1)] = (int32_t)w;
+    }
+}
+
+// FILL = false: every non-empty destination cell, gridDim.x cells at a time
+// (each XCD walks a contiguous range of each round).  FILL = true: the cells
+// the count pass flagged, listed in a.ovf_list.
+template <bool FILL, typename T>
+__global__ void __launch_bounds__(CW)
+    k_nnps_cells(CellArgs a, int32_t *__restrict__ counts, const int64_t *__restrict__ slice_off,
+                 int32_t *__restrict__ indices)
+{
+    __shared__ int32_t seg_start[MAXSEG];
+    __shared__ int32_t seg_end[MAXSEG];  // inclusive prefix of the segment sizes
+    __shared__ typename Quad<T>::type dsts[CW];  // (x, y, z, radius^2) of the group's destinations
+    if (FILL) {
+        if (a.status && slice_off[a.nslices] > a.cap) return;  // no room: nothing is written
+        const int32_t nov = *a.novf;
+        for (int32_t k = blockIdx.x; k < nov; k += gridDim.x)
+            nnps_cell<true, T>(a, a.ovf_list[k], counts, slice_off, indices, seg_start, seg_end,
+                               dsts);
+    } else {
+        const int64_t ncl = a.ncell_dev ? (int64_t)*a.ncell_dev : a.ncell_list;
+        const unsigned G = gridDim.x;
+        for (int64_t base = 0; base < ncl; base += G) {
+            const int64_t w = base + sph_xcd_tile(blockIdx.x, G);
+            if (w >= ncl) break;  // uniform: one wavefront per block
+            nnps_cell<false, T>(a, w, counts, slice_off, indices, seg_start, seg_end, dsts);
+        }
+    }
 }
 
 // SELL-64 rows from the count pass's hit masks: one lane per destination,
@@ -918,8 +953,9 @@ static int nnps_count(sph_ctx *ctx, CellArgs &a, int64_t ncell_launch, int64_t n
     const size_t ovf_bytes = al((size_t)ncell_launch * sizeof(int32_t));
     const size_t cand_bytes = al((size_t)ncell_launch * a.maxch * CW * sizeof(int32_t));
     const size_t dcell_bytes = al((size_t)nd_total * sizeof(int32_t));
+    const size_t list_bytes = al((size_t)(ncell_launch + 1) * sizeof(int32_t));
     const size_t sums_bytes = al((size_t)(nslices + 1) * sizeof(int64_t) + 256);
-    const size_t sums_off = masks_bytes + ovf_bytes + cand_bytes + dcell_bytes;
+    const size_t sums_off = masks_bytes + ovf_bytes + cand_bytes + dcell_bytes + list_bytes;
     int rc = sph_ensure_scratch(ctx, sums_off + sums_bytes +
                                          sph_scan_scratch_bytes(nslices, sizeof(int64_t)));
     if (rc) return rc;
@@ -929,9 +965,14 @@ static int nnps_count(sph_ctx *ctx, CellArgs &a, int64_t ncell_launch, int64_t n
     a.cell_ovf = (int32_t *)(sc + masks_bytes);
     a.cand = (int32_t *)(sc + masks_bytes + ovf_bytes);
     a.dcell_of = (int32_t *)(sc + masks_bytes + ovf_bytes + cand_bytes);
+    a.novf = (int32_t *)(sc + masks_bytes + ovf_bytes + cand_bytes + dcell_bytes);
+    a.ovf_list = a.novf + 1;
     a.nslices = nslices;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, a.novf, 1, 0);
+    SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
     auto count_kern = a.f32 ? k_nnps_cells<false, float> : k_nnps_cells<false, double>;
-    hipLaunchKernelGGL(count_kern, dim3((unsigned)ncell_launch), dim3(CW), 0, ctx->stream, a,
+    const int64_t count_grid = ncell_launch < NN_CELL_GRID ? ncell_launch : NN_CELL_GRID;
+    hipLaunchKernelGGL(count_kern, dim3((unsigned)count_grid), dim3(CW), 0, ctx->stream, a,
                        counts, (const int64_t *)nullptr, (int32_t *)nullptr);
     SPH_LAUNCH_CHECK(ctx, "k_nnps_cells<count>");
     // slice widths -> offsets (scan); slice totals -> entry total
@@ -953,7 +994,8 @@ static int nnps_fill(sph_ctx *ctx, const CellArgs &a, int64_t ncell_launch, int6
                        dim3(NN_THREADS), 0, ctx->stream, a, nd_total, counts, slice_off, indices);
     SPH_LAUNCH_CHECK(ctx, "k_nnps_fill_slices");
     auto fill_kern = a.f32 ? k_nnps_cells<true, float> : k_nnps_cells<true, double>;
-    hipLaunchKernelGGL(fill_kern, dim3((unsigned)ncell_launch), dim3(CW), 0, ctx->stream, a,
+    const int64_t fill_grid = ncell_launch < NN_OVF_GRID ? ncell_launch : NN_OVF_GRID;
+    hipLaunchKernelGGL(fill_kern, dim3((unsigned)fill_grid), dim3(CW), 0, ctx->stream, a,
                        (int32_t *)counts, slice_off, indices);
     SPH_LAUNCH_CHECK(ctx, "k_nnps_cells<fill>");
     return SPH_OK;

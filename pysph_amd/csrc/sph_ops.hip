@@ -224,6 +224,56 @@ extern "C" int sph_tait_f32(sph_ctx *ctx, int64_t n_total, int64_t n_real, const
                        p, cs, p_s, cs_s);
 }
 
+// Tait over sorted records with every particle real and no accumulation
+// (the batched path): p, cs to the original-order arrays and to the
+// record, with P/rho^2 (the k_prho2 value) written in the same pass
+template <typename T>
+__global__ void __launch_bounds__(OP_THREADS)
+    k_tait_rec(int64_t n, const int32_t *__restrict__ perm, T *__restrict__ rec, T ro, T co,
+               T gamma, T B, double *__restrict__ p, double *__restrict__ cs)
+{
+    const int64_t t = (int64_t)blockIdx.x * OP_THREADS + threadIdx.x;
+    if (t >= n) return;
+    T *r = rec + t * SPH_REC;
+    const T rhoa = r[SPH_R_RHO];
+    const T ratio = rhoa / ro;
+    const T gamma2 = T(0.5) * (gamma - T(1.0));
+    const T tmp = pow(ratio, gamma);
+    const T r0 = T(0.0) + (tmp - T(1.0)) * B;
+    const T r1 = T(0.0) + pow(ratio, gamma2) * co;
+    const int32_t i = perm[t];
+    p[i] = r0;
+    cs[i] = r1;
+    r[SPH_R_P] = r0;
+    r[SPH_R_CS] = r1;
+    const T rhob2 = T(1.0) / (rhoa * rhoa);
+    r[SPH_R_PRHO2] = r0 * rhob2;
+}
+
+template <typename T>
+static int tait_records(sph_ctx *ctx, int64_t n, const int32_t *perm, T *rec, double ro,
+                        double co, double gamma, double *p, double *cs)
+{
+    if (n == 0) return SPH_OK;
+    const T B = T(co) * T(co) * T(ro) / T(gamma);  // eos_funcs.pyx:90
+    hipLaunchKernelGGL(k_tait_rec<T>, dim3(sph_blocks(n, OP_THREADS)), dim3(OP_THREADS), 0,
+                       ctx->stream, n, perm, rec, T(ro), T(co), T(gamma), B, p, cs);
+    SPH_LAUNCH_CHECK(ctx, "k_tait_rec");
+    return SPH_OK;
+}
+
+int sph_tait_records(sph_ctx *ctx, int64_t n, const int32_t *perm, double *rec, double ro,
+                     double co, double gamma, double *p, double *cs)
+{
+    return tait_records<double>(ctx, n, perm, rec, ro, co, gamma, p, cs);
+}
+
+int sph_tait_records_f32(sph_ctx *ctx, int64_t n, const int32_t *perm, float *rec, double ro,
+                         double co, double gamma, double *p, double *cs)
+{
+    return tait_records<float>(ctx, n, perm, rec, ro, co, gamma, p, cs);
+}
+
 // ---------------------------------------------------------------------------
 // MomentumEquation with Monaghan artificial viscosity
 // ---------------------------------------------------------------------------

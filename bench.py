@@ -51,6 +51,8 @@ def _args():
                     help="arithmetic of the lattice and batch workloads (single: float records, "
                          "lists and operators)")
     ap.add_argument("--nsims", type=int, default=256, help="drops of --workload batch")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="--workload batch: keep the default launch shapes")
     ap.add_argument("--graph-steps", type=int, default=10,
                     help="RK2 steps per captured graph (--workload batch)")
     ap.add_argument("--seq-sims", type=int, default=0,
@@ -256,6 +258,12 @@ def run_batch(args):
     b.step(dt)
     b.check()
     progress("eager step done")
+    tune = None
+    if not args.no_autotune:
+        from pysph_amd.autotune import autotune
+        tune = autotune(b, dt, steps=3, use_cache=False, log=progress)
+        progress(f"autotuned: {tune['tuning']} {tune['ms_per_step']:.3f} ms/step "
+                 f"(default shape {tune['trials'][0][1]:.3f} ms/step)")
     gs = max(1, args.graph_steps)
     b.capture(dt, nsteps=gs)
     progress(f"graph of {gs} steps captured")
@@ -310,7 +318,13 @@ def run_batch(args):
                    "pair_interactions_per_s": pairs / elapsed,
                    "sequential_plain_gpu_steps_per_s": seq_sps,
                    "sequential_drops_timed": int(len(pick)),
-                   "speedup_vs_sequential": batch_sps / seq_sps},
+                   "speedup_vs_sequential": batch_sps / seq_sps,
+                   "fused_kernels": "density+EOS, momentum+position stepping+stage",
+                   "launch_shape": b.tuning(),
+                   "autotune": None if tune is None else {
+                       "trials": len(tune["trials"]),
+                       "default_shape_ms_per_step": tune["trials"][0][1],
+                       "tuned_ms_per_step_eager": tune["ms_per_step"]}},
         "roofline": None,
     }, None
 

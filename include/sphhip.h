@@ -328,6 +328,21 @@ typedef struct sph_batch_desc {
     int64_t list_cap;  /* neighbour-list slots */
     int32_t maxch;     /* candidate chunks of 64 replayed by the list fill, 1..16 */
     int32_t f32;       /* 1: float records, lists and operators (state stays FP64) */
+    int32_t uniform_h; /* 1: every simulation has one h; sources' h is not read (a
+                        * simulation whose h differs sets status bit 32) */
+    /* launch shapes of sph_batch_rk2_step (0 picks the default); any value
+     * gives the same results, bit for bit (the per-particle sums keep their
+     * order) -- an autotuner picks them by timing */
+    int32_t op_threads;       /* threads per block of the fused operator kernels:
+                               * 64, 128 or 256 (default 256) */
+    int32_t op_blocks_per_cu; /* persistent grid of the fused kernels: blocks per
+                               * compute unit, each walking a contiguous range of
+                               * particle tiles (default 0: one tile per block) */
+    int32_t nbr_tile;         /* neighbour records in flight per lane in the fused
+                               * momentum kernel: 1, 2 or 4 (density: twice that;
+                               * default 2) */
+    int32_t cells_per_block;  /* destination cells each block of the list count
+                               * pass walks: 1..64 (default 1) */
     void *work;        /* device workspace of sph_batch_workspace_size bytes */
     int64_t work_bytes;
     int32_t *status;   /* device */
@@ -337,7 +352,10 @@ typedef struct sph_batch_desc {
 int sph_batch_workspace_size(sph_ctx *ctx, const sph_batch_desc *d, int64_t *bytes);
 /* prepare a new workspace (clears its record area); call once per workspace */
 int sph_batch_init_workspace(sph_ctx *ctx, const sph_batch_desc *d);
-/* one RK2 step of every simulation, stream-ordered */
+/* one RK2 step of every simulation, stream-ordered (RK2Integrator.integrate,
+ * solver/integrator.py:467-498).  Per evaluation: binning and lists, then two
+ * fused kernels -- SPHRho + Tait, and MomentumEquation + PositionStepping +
+ * the stage update (with save_initial_arrays folded into the first stage) */
 int sph_batch_rk2_step(sph_ctx *ctx, const sph_batch_desc *d, double dt);
 /* capture nsteps RK2 steps in a HIP graph (the context needs its own
  * stream; run one eager step first so the context scratch has its final

@@ -71,7 +71,10 @@ class BatchDesc(ctypes.Structure):
                 + [(k, _dbl) for k in ("co", "ro", "gamma", "alpha", "beta", "eta",
                                        "radius_scale")]
                 + [("key_cap", _i64), ("max_cells", _i64), ("list_cap", _i64),
-                   ("maxch", ctypes.c_int32), ("f32", ctypes.c_int32), ("work", _vp),
+                   ("maxch", ctypes.c_int32), ("f32", ctypes.c_int32),
+                   ("uniform_h", ctypes.c_int32), ("op_threads", ctypes.c_int32),
+                   ("op_blocks_per_cu", ctypes.c_int32), ("nbr_tile", ctypes.c_int32),
+                   ("cells_per_block", ctypes.c_int32), ("work", _vp),
                    ("work_bytes", _i64),
                    ("status", _vp), ("pairs", _vp)])
 

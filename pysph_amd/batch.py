@@ -29,7 +29,10 @@ _STATE = ("x", "y", "z", "u", "v", "w", "h", "m", "rho", "p", "cs",
 STATUS_BITS = {1: "neighbour-list slots exceed list_cap", 2: "cell keys exceed key_cap",
                4: "non-finite coordinate or h", 8: "grid too large",
                16: "non-empty cells exceed max_cells",
-               32: "reverse mode needs one h per simulation"}
+               32: "h differs within a simulation (uniform_h or the reverse mode needs one h)"}
+
+# launch shapes of the fused step (sph_batch_desc); 0 = the library default
+TUNE_FIELDS = ("op_threads", "op_blocks_per_cu", "nbr_tile", "cells_per_block")
 
 
 class BatchError(RuntimeError):
@@ -70,7 +73,7 @@ class BatchedSimulation:
     def __init__(self, sims, dim=2, co=1400.0, ro=1.0, gamma=7.0, alpha=0.1, beta=0.0, eta=0.1,
                  radius_scale=2.0, device=None, key_cap=None, max_cells=None, list_cap=None,
                  maxch=None, cell_margin=4.0, list_slots_per_particle=None, dt_per_sim=None,
-                 precision="double"):
+                 precision="double", tune=None):
         if not torch.cuda.is_available():
             raise RuntimeError("pysph_amd needs a HIP device; it has no CPU fallback")
         if dim not in (2, 3):
@@ -133,6 +136,11 @@ class BatchedSimulation:
             raise ValueError("precision must be 'double' or 'single'")
         self.precision = precision
         d.f32 = 1 if precision == "single" else 0
+        # one h per simulation: the operators take h_ab = h_a without reading
+        # the source's h (the grid kernel flags a simulation whose h differs)
+        self.uniform_h = all(np.ptp(np.asarray(sd["h"], dtype=np.float64)) == 0.0
+                             for sd in sims if len(sd["h"]))
+        d.uniform_h = 1 if self.uniform_h else 0
         d.status, d.pairs = self.status.data_ptr(), self.pairs.data_ptr()
         d.dt_sim = self.dt_sim.data_ptr() if self.dt_sim is not None else None
         nbytes = ctypes.c_int64(0)
@@ -143,12 +151,37 @@ class BatchedSimulation:
             self.work = torch.empty(int(nbytes.value), dtype=torch.uint8, device=self.device)
         d.work, d.work_bytes = self.work.data_ptr(), int(nbytes.value)
         self.desc = d
+        self.set_tuning(tune or {})
         self.ctx.check(lib.sph_batch_init_workspace(self.ctx.handle, ctypes.byref(d)))
         self.graph = None
         self.graph_steps = 0
         self.steps_done = 0
 
     # ------------------------------------------------------------------
+    def set_tuning(self, tune):
+        """Launch shapes of the fused step, e.g. from ``autotune``: a dict
+        over TUNE_FIELDS (missing keys keep their value).  Every choice gives
+        the same results bit for bit; only the speed differs."""
+        for k, v in dict(tune).items():
+            if k not in TUNE_FIELDS:
+                raise ValueError(f"unknown tuning field {k!r}")
+            setattr(self.desc, k, int(v))
+
+    def tuning(self):
+        return {k: int(getattr(self.desc, k)) for k in TUNE_FIELDS}
+
+    def snapshot(self):
+        """Device copies of the state a step reads (for trial runs)."""
+        with torch.cuda.stream(self.stream):
+            return {k: v.clone() for k, v in self.t.items()}, self.pairs.clone()
+
+    def restore(self, snap):
+        state, pairs = snap
+        with torch.cuda.stream(self.stream):
+            for k, v in state.items():
+                self.t[k].copy_(v)
+            self.pairs.copy_(pairs)
+
     def step(self, dt=1.0):
         """One RK2 step of every simulation (queued on the batch's stream).
         With per-simulation time steps, dt multiplies them (1.0: one step)."""

@@ -23,6 +23,7 @@
 #include <string.h>
 
 #include "sph_internal.h"
+#include "sph_terms.h"
 
 #define BT_THREADS 256
 
@@ -50,8 +51,9 @@ __device__ __forceinline__ double blk_min(double v, double *red)
 __global__ void __launch_bounds__(GRID_THREADS)
     k_batch_grid(const int64_t *__restrict__ sim_off, const double *__restrict__ x,
                  const double *__restrict__ y, const double *__restrict__ z,
-                 const double *__restrict__ h, double radius_scale, GridDev *__restrict__ grids,
-                 int64_t *__restrict__ ncell, int32_t *__restrict__ status)
+                 const double *__restrict__ h, double radius_scale, int uniform_h,
+                 GridDev *__restrict__ grids, int64_t *__restrict__ ncell,
+                 int32_t *__restrict__ status)
 {
     __shared__ double red[GRID_THREADS / 64];
     const int s = blockIdx.x;
@@ -88,6 +90,7 @@ __global__ void __launch_bounds__(GRID_THREADS)
         g.d0 = bad > 0.0 ? 0 : 1;  // d0 = 0: void, its particles go to the trash cell
         g.d1 = g.d2 = 1;
     } else {
+        if (uniform_h && -v[6] != v[7]) atomicOr(status, 32);  // h differs within the simulation
         double cs = radius_scale * (-v[6]);
         if (cs == 0.0) cs = 1.0;
         g.cs = cs;
@@ -211,6 +214,198 @@ __global__ void __launch_bounds__(BT_THREADS)
     for (int k = 0; k < pr.nprops; ++k) {
         const double t = pr.a[k][i] * dt;
         pr.x[k][i] = pr.x0[k][i] + t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fused kernels of sph_batch_rk2_step
+// ---------------------------------------------------------------------------
+// Two kernels per evaluation once the lists are built:
+//   k_batch_density_eos    SPHRho, then Tait and P/rho^2 of the same particle
+//   k_batch_momentum_stage MomentumEquation + AV, then PositionStepping and
+//                          x <- x0 + a*dt of the same particle (the first
+//                          stage of a step also does save_initial_arrays)
+// Each keeps the per-pair order and the arithmetic of the separate kernels
+// (k_summation_density, k_tait_rec, k_momentum, k_position_stepping,
+// k_batch_stage), so the results are the same bit for bit.  The density
+// kernel writes fields 7, 8, 9 and 11 of the records while other lanes read
+// fields 0-3 and 10 of their neighbours: distinct words.  The momentum
+// kernel reads only records and writes the original-order state.
+//
+// Launch shape (tunable): blockDim.x threads per block (64, 128 or 256; a
+// multiple of the 64-row slices of the lists); a persistent grid walks the
+// tiles of blockDim.x sorted particles, block b taking a contiguous range
+// of tiles after the XCD remap, so each XCD's blocks cover one contiguous
+// part of the sorted order.  NT neighbour records are loaded before their
+// terms are added in list order.
+#define FUSED_MAX_THREADS 256
+
+__device__ __forceinline__ void fused_tiles(int64_t ntiles, int64_t &tb, int64_t &te)
+{
+    const int64_t G = gridDim.x, L = sph_xcd_tile(blockIdx.x, gridDim.x);
+    tb = L * ntiles / G;
+    te = (L + 1) * ntiles / G;
+}
+
+template <bool UH, typename T, int NT>
+__global__ void __launch_bounds__(FUSED_MAX_THREADS)
+    k_batch_density_eos(int dim, int64_t n, int64_t ntiles, const int32_t *__restrict__ perm,
+                        const int32_t *__restrict__ counts, const int64_t *__restrict__ slice_off,
+                        const int32_t *__restrict__ indices, T *__restrict__ rec, T ro, T co,
+                        T gamma, T B, double *__restrict__ rho, double *__restrict__ p,
+                        double *__restrict__ cs)
+{
+    int64_t tb, te;
+    fused_tiles(ntiles, tb, te);
+    for (int64_t tile = tb; tile < te; ++tile) {
+        const int64_t t = tile * blockDim.x + threadIdx.x;
+        if (t >= n) break;
+        const T *ra = rec + t * SPH_REC;
+        const auto *dr = rec2<T>(rec, t);
+        const auto dxy = dr[0], dzm = dr[1];
+        const T xa = dxy.x, ya = dxy.y, za = dzm.x, ha = ra[SPH_R_H];
+        const T facA = sph_h_dim(ha, dim) * sph_cubic_fac1<T>(dim);
+        T nr = T(0.0);
+        const int32_t *row = indices + slice_off[t >> 6] + (t & 63);
+        const int32_t e = counts[t];
+        int32_t k = 0;
+        for (; k + NT <= e; k += NT) {
+            int32_t j[NT];
+            typename Pair<T>::type a[NT], b[NT];
+            T hb[NT];
+#pragma unroll
+            for (int u = 0; u < NT; ++u)
+                j[u] = __builtin_nontemporal_load(row + ((int64_t)(k + u) << 6));
+#pragma unroll
+            for (int u = 0; u < NT; ++u) {
+                const auto *sr = rec2<T>(rec, j[u]);
+                a[u] = sr[0];
+                b[u] = sr[1];
+                hb[u] = UH ? ha : rec[(int64_t)j[u] * SPH_REC + SPH_R_H];
+            }
+#pragma unroll
+            for (int u = 0; u < NT; ++u)
+                nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, a[u], b[u], hb[u]);
+        }
+        for (; k < e; ++k) {
+            const int32_t j = __builtin_nontemporal_load(row + ((int64_t)k << 6));
+            const auto *sr = rec2<T>(rec, j);
+            const T hb = UH ? ha : rec[(int64_t)j * SPH_REC + SPH_R_H];
+            nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, sr[0], sr[1], hb);
+        }
+        // SPHRho is non-integrating: the output starts from 0 (sph_calc.pyx:319-321)
+        const T rhoa = T(0.0) + nr;
+        // Tait (eos_funcs.pyx:118-129), as k_tait_rec
+        const T ratio = rhoa / ro;
+        const T gamma2 = T(0.5) * (gamma - T(1.0));
+        const T tmp = pow(ratio, gamma);
+        const T r0 = T(0.0) + (tmp - T(1.0)) * B;
+        const T r1 = T(0.0) + pow(ratio, gamma2) * co;
+        const T rhob2 = T(1.0) / (rhoa * rhoa);
+        const int32_t i = perm[t];
+        rho[i] = rhoa;
+        p[i] = r0;
+        cs[i] = r1;
+        T *rw = rec + t * SPH_REC;
+        rw[SPH_R_RHO] = rhoa;
+        rw[SPH_R_P] = r0;
+        rw[SPH_R_CS] = r1;
+        rw[SPH_R_PRHO2] = r0 * rhob2;
+    }
+}
+
+struct FusedState {
+    double *x[3], *u[3], *x0[3], *u0[3], *ax[3], *au[3];
+    double *dt_fac;
+};
+
+template <bool UH, typename T, int NT>
+__global__ void __launch_bounds__(FUSED_MAX_THREADS)
+    k_batch_momentum_stage(int dim, int64_t n, int64_t ntiles, const int32_t *__restrict__ perm,
+                           const int32_t *__restrict__ counts,
+                           const int64_t *__restrict__ slice_off,
+                           const int32_t *__restrict__ indices, const T *__restrict__ rec,
+                           T alpha, T beta, T eta, FusedState st,
+                           const int32_t *__restrict__ sim_of, const double *__restrict__ dt_sim,
+                           double scale, int save)
+{
+    int64_t tb, te;
+    fused_tiles(ntiles, tb, te);
+    for (int64_t tile = tb; tile < te; ++tile) {
+        const int64_t t = tile * blockDim.x + threadIdx.x;
+        if (t >= n) break;
+        const auto *dr = rec2<T>(rec, t);
+        const auto d0 = dr[0], d1 = dr[1], d2 = dr[2], d3 = dr[3], d4 = dr[4], d5 = dr[5];
+        MomDest<T> D;
+        D.xa = d0.x;
+        D.ya = d0.y;
+        D.za = d1.x;
+        D.ua = d2.x;
+        D.va = d2.y;
+        D.wa = d3.x;
+        D.rhoa = d4.x;
+        D.ca = d4.y;
+        D.ha = d5.x;
+        D.Pa = d5.y;
+        D.facA = sph_h_dim(D.ha, dim) * sph_cubic_fac1<T>(dim);
+        D.rhoa2 = T(1.0) / (D.rhoa * D.rhoa);
+        T dtf = T(0.0);  // dt_fac is reset per calc (pressure_funcs.pyx:214)
+        T nr0 = T(0.0), nr1 = T(0.0), nr2 = T(0.0);
+        const int32_t *row = indices + slice_off[t >> 6] + (t & 63);
+        const int32_t e = counts[t];
+        int32_t k = 0;
+        for (; k + NT <= e; k += NT) {
+            int32_t j[NT];
+            typename Pair<T>::type s0[NT], s1[NT], s2[NT], s3[NT], s4[NT];
+            T hb[NT];
+#pragma unroll
+            for (int u = 0; u < NT; ++u)
+                j[u] = __builtin_nontemporal_load(row + ((int64_t)(k + u) << 6));
+#pragma unroll
+            for (int u = 0; u < NT; ++u) {
+                const auto *sr = rec2<T>(rec, j[u]);
+                s0[u] = sr[0];
+                s1[u] = sr[1];
+                s2[u] = sr[2];
+                s3[u] = sr[3];
+                s4[u] = sr[4];
+                hb[u] = UH ? D.ha : rec[(int64_t)j[u] * SPH_REC + SPH_R_H];
+            }
+#pragma unroll
+            for (int u = 0; u < NT; ++u)
+                mom_term<UH, T>(dim, D, alpha, beta, eta, s0[u], s1[u], s2[u], s3[u], s4[u],
+                                hb[u], dtf, nr0, nr1, nr2);
+        }
+        for (; k < e; ++k) {
+            const int32_t j = __builtin_nontemporal_load(row + ((int64_t)k << 6));
+            const auto *sr = rec2<T>(rec, j);
+            const T hb = UH ? D.ha : rec[(int64_t)j * SPH_REC + SPH_R_H];
+            mom_term<UH, T>(dim, D, alpha, beta, eta, sr[0], sr[1], sr[2], sr[3], sr[4], hb, dtf,
+                            nr0, nr1, nr2);
+        }
+        const int32_t i = perm[t];
+        st.dt_fac[i] = dtf;
+        // the accelerations of u, v, w (integrating calc: output 0 + sum) and
+        // PositionStepping's a_x = u; then x <- x0 + a*dt per stepped property
+        const double acc[3] = {(double)(T(0.0) + nr0), (double)(T(0.0) + nr1),
+                               (double)(T(0.0) + nr2)};
+        const double dt = dt_sim ? scale * dt_sim[sim_of[i]] : scale;
+        for (int c = 0; c < dim; ++c) {
+            const double uc = st.u[c][i], xc = st.x[c][i];
+            double u0 = uc, x0 = xc;
+            if (save) {
+                st.u0[c][i] = uc;
+                st.x0[c][i] = xc;
+            } else {
+                u0 = st.u0[c][i];
+                x0 = st.x0[c][i];
+            }
+            st.au[c][i] = acc[c];
+            st.ax[c][i] = uc;
+            const double tu = acc[c] * dt, tx = uc * dt;
+            st.u[c][i] = u0 + tu;
+            st.x[c][i] = x0 + tx;
+        }
     }
 }
 
@@ -475,6 +670,16 @@ int check_desc(sph_ctx *ctx, const sph_batch_desc *d)
                 "batch: bad capacities");
     SPH_REQUIRE(ctx, d->maxch >= 1 && d->maxch <= 16, "batch: bad maxch");
     SPH_REQUIRE(ctx, d->f32 == 0 || d->f32 == 1, "batch: bad f32 flag");
+    SPH_REQUIRE(ctx, d->uniform_h == 0 || d->uniform_h == 1, "batch: bad uniform_h flag");
+    SPH_REQUIRE(ctx, d->op_threads == 0 || d->op_threads == 64 || d->op_threads == 128 ||
+                         d->op_threads == 256,
+                "batch: op_threads must be 64, 128 or 256 (or 0)");
+    SPH_REQUIRE(ctx, d->op_blocks_per_cu >= 0 && d->op_blocks_per_cu <= 64,
+                "batch: op_blocks_per_cu must be 0..64");
+    SPH_REQUIRE(ctx, d->nbr_tile == 0 || d->nbr_tile == 1 || d->nbr_tile == 2 || d->nbr_tile == 4,
+                "batch: nbr_tile must be 1, 2 or 4 (or 0)");
+    SPH_REQUIRE(ctx, d->cells_per_block >= 0 && d->cells_per_block <= 64,
+                "batch: cells_per_block must be 0..64");
     SPH_REQUIRE(ctx, d->sim_off && d->sim_of && d->x && d->y && d->z && d->u && d->v && d->w &&
                          d->h && d->m && d->rho && d->p && d->cs && d->x0 && d->y0 && d->u0 &&
                          d->v0 && d->ax && d->ay && d->au && d->av && d->dt_fac && d->status &&
@@ -486,15 +691,15 @@ int check_desc(sph_ctx *ctx, const sph_batch_desc *d)
     return SPH_OK;
 }
 
-// one evaluation of the operation list at the current positions
-int batch_eval(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w)
+// binning, sorted records and neighbour lists at the current positions
+int batch_lists(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w)
 {
     const int64_t n = d->n;
     int rc;
     // -- binning into the per-simulation grids
     hipLaunchKernelGGL(k_batch_grid, dim3(d->nsims), dim3(GRID_THREADS), 0, ctx->stream,
-                       d->sim_off, d->x, d->y, d->z, d->h, d->radius_scale, w.grids, w.ncell,
-                       d->status);
+                       d->sim_off, d->x, d->y, d->z, d->h, d->radius_scale, d->uniform_h,
+                       w.grids, w.ncell, d->status);
     SPH_LAUNCH_CHECK(ctx, "k_batch_grid");
     hipLaunchKernelGGL(k_batch_key_base, dim3(1), dim3(1024), 0, ctx->stream, d->nsims, w.ncell,
                        d->key_cap, w.grids, d->status);
@@ -547,14 +752,25 @@ int batch_eval(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w)
     rc = sph_nnps_build_batch(ctx, w.grids, d->sim_of, rec32 ? nullptr : w.rec, rec32, d->x,
                               d->y, d->z, w.perm, w.cell_start, w.cells, ncell_dev,
                               d->max_cells, n, d->radius_scale, d->maxch, w.counts, w.slice_off,
-                              w.indices, d->list_cap, d->status, d->pairs, trash);
+                              w.indices, d->list_cap, d->status, d->pairs, trash,
+                              d->cells_per_block > 0 ? d->cells_per_block : 1);
+    return rc;
+}
+
+// one evaluation of the operation list at the current positions, kernel by
+// kernel (the reverse mode drives these pieces)
+int batch_eval(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w)
+{
+    const int64_t n = d->n;
+    int rc = batch_lists(ctx, d, w);
     if (rc) return rc;
+    float *rec32 = d->f32 ? (float *)w.rec : nullptr;
     // -- SPHRho, Tait (non-integrating: outputs start from 0), MomentumEquation
     // (dt_fac reset per calc), PositionStepping
     double *az = d->dim == 3 ? d->aw : nullptr;
     if (rec32) {
         rc = sph_summation_density_f32(ctx, d->dim, n, n, w.perm, w.counts, w.slice_off,
-                                       w.indices, rec32, rec32, 0, 0, d->rho,
+                                       w.indices, rec32, rec32, d->uniform_h, 0, d->rho,
                                        rec32 + SPH_R_RHO, SPH_REC);
         if (rc) return rc;
         rc = sph_tait_records_f32(ctx, n, w.perm, rec32, d->ro, d->co, d->gamma, d->p, d->cs);
@@ -563,19 +779,20 @@ int batch_eval(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w)
         if (rc) return rc;
         // ns = 0: P/rho^2 is already in the records
         rc = sph_momentum_av_f32(ctx, d->dim, n, n, w.perm, w.counts, w.slice_off, w.indices,
-                                 rec32, rec32, 0, 0, d->alpha, d->beta, d->eta, 0, d->au, d->av,
-                                 az, d->dt_fac);
+                                 rec32, rec32, 0, d->uniform_h, d->alpha, d->beta, d->eta, 0,
+                                 d->au, d->av, az, d->dt_fac);
     } else {
         rc = sph_summation_density(ctx, d->dim, n, n, w.perm, w.counts, w.slice_off, w.indices,
-                                   w.rec, w.rec, 0, 0, d->rho, w.rec + SPH_R_RHO, SPH_REC);
+                                   w.rec, w.rec, d->uniform_h, 0, d->rho, w.rec + SPH_R_RHO,
+                                   SPH_REC);
         if (rc) return rc;
         rc = sph_tait_records(ctx, n, w.perm, w.rec, d->ro, d->co, d->gamma, d->p, d->cs);
         if (rc) return rc;
         rc = sph_fill(ctx, n, 0.0, d->dt_fac);
         if (rc) return rc;
         rc = sph_momentum_av(ctx, d->dim, n, n, w.perm, w.counts, w.slice_off, w.indices, w.rec,
-                             w.rec, 0, 0, d->alpha, d->beta, d->eta, 0, d->au, d->av, az,
-                             d->dt_fac);
+                             w.rec, 0, d->uniform_h, d->alpha, d->beta, d->eta, 0, d->au, d->av,
+                             az, d->dt_fac);
     }
     if (rc) return rc;
     return sph_position_stepping(ctx, n, n, d->u, d->v, d->dim == 3 ? d->w : nullptr, d->ax,
@@ -602,23 +819,72 @@ int batch_stage(sph_ctx *ctx, const sph_batch_desc *d, double dt)
     return SPH_OK;
 }
 
+template <typename T, int NT>
+void launch_fused(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w, unsigned grid,
+                  unsigned threads, int64_t ntiles, double scale, int save)
+{
+    const int64_t n = d->n;
+    T *rec = (T *)w.rec;
+    const T B = T(d->co) * T(d->co) * T(d->ro) / T(d->gamma);  // eos_funcs.pyx:90
+    auto dk = d->uniform_h ? k_batch_density_eos<true, T, 2 * NT>
+                           : k_batch_density_eos<false, T, 2 * NT>;
+    hipLaunchKernelGGL(dk, dim3(grid), dim3(threads), 0, ctx->stream, d->dim, n, ntiles, w.perm,
+                       w.counts, w.slice_off, w.indices, rec, T(d->ro), T(d->co), T(d->gamma), B,
+                       d->rho, d->p, d->cs);
+    FusedState st;
+    double *x[3] = {d->x, d->y, d->z}, *u[3] = {d->u, d->v, d->w};
+    double *x0[3] = {d->x0, d->y0, d->z0}, *u0[3] = {d->u0, d->v0, d->w0};
+    double *ax[3] = {d->ax, d->ay, d->az}, *au[3] = {d->au, d->av, d->aw};
+    for (int c = 0; c < 3; ++c) {
+        st.x[c] = x[c];
+        st.u[c] = u[c];
+        st.x0[c] = x0[c];
+        st.u0[c] = u0[c];
+        st.ax[c] = ax[c];
+        st.au[c] = au[c];
+    }
+    st.dt_fac = d->dt_fac;
+    auto mk = d->uniform_h ? k_batch_momentum_stage<true, T, NT>
+                           : k_batch_momentum_stage<false, T, NT>;
+    hipLaunchKernelGGL(mk, dim3(grid), dim3(threads), 0, ctx->stream, d->dim, n, ntiles, w.perm,
+                       w.counts, w.slice_off, w.indices, (const T *)rec, T(d->alpha),
+                       T(d->beta), T(d->eta), st, d->sim_of, d->dt_sim, scale, save);
+}
+
+// one evaluation and its stage through the fused kernels: x <- x0 + a*(scale
+// dt); save: x0 <- x first (save_initial_arrays, solver/integrator.py:262-285)
+int batch_eval_stage_fused(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w,
+                           double scale, int save)
+{
+    int rc = batch_lists(ctx, d, w);
+    if (rc) return rc;
+    const unsigned threads = d->op_threads > 0 ? (unsigned)d->op_threads : FUSED_MAX_THREADS;
+    const int64_t ntiles = (d->n + threads - 1) / threads;
+    int64_t grid = ntiles;
+    if (d->op_blocks_per_cu > 0) {
+        const int64_t g = (int64_t)d->op_blocks_per_cu * sph_cu_count(ctx);
+        if (g < grid) grid = g;
+    }
+    const int nt = d->nbr_tile > 0 ? d->nbr_tile : 2;
+    if (d->f32) {
+        if (nt == 1) launch_fused<float, 1>(ctx, d, w, (unsigned)grid, threads, ntiles, scale, save);
+        else if (nt == 2) launch_fused<float, 2>(ctx, d, w, (unsigned)grid, threads, ntiles, scale, save);
+        else launch_fused<float, 4>(ctx, d, w, (unsigned)grid, threads, ntiles, scale, save);
+    } else {
+        if (nt == 1) launch_fused<double, 1>(ctx, d, w, (unsigned)grid, threads, ntiles, scale, save);
+        else if (nt == 2) launch_fused<double, 2>(ctx, d, w, (unsigned)grid, threads, ntiles, scale, save);
+        else launch_fused<double, 4>(ctx, d, w, (unsigned)grid, threads, ntiles, scale, save);
+    }
+    SPH_LAUNCH_CHECK(ctx, "k_batch_density_eos / k_batch_momentum_stage");
+    return SPH_OK;
+}
+
 int batch_rk2(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w, double dt)
 {
-    // save_initial_arrays
-    double *dst[6] = {d->u0, d->v0, d->x0, d->y0, d->w0, d->z0};
-    const double *src[6] = {d->u, d->v, d->x, d->y, d->w, d->z};
-    const int np = d->dim == 3 ? 6 : 4;
-    for (int k = 0; k < np; ++k) {
-        int rc = sph_copy(ctx, d->n, src[k], dst[k]);
-        if (rc) return rc;
-    }
-    int rc = batch_eval(ctx, d, w);
+    // RK2 midpoint: X1 = X0 + (dt/2) F(X0) (saving X0 first), X = X0 + dt F(X1)
+    int rc = batch_eval_stage_fused(ctx, d, w, dt / 2, 1);
     if (rc) return rc;
-    rc = batch_stage(ctx, d, dt / 2);
-    if (rc) return rc;
-    rc = batch_eval(ctx, d, w);
-    if (rc) return rc;
-    return batch_stage(ctx, d, dt);
+    return batch_eval_stage_fused(ctx, d, w, dt, 0);
 }
 
 struct BatchGraph {

@@ -96,6 +96,19 @@ int sph_ensure_scratch(sph_ctx *ctx, size_t bytes)
     return SPH_OK;
 }
 
+int sph_cu_count(sph_ctx *ctx)
+{
+    if (ctx->ncu <= 0) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, ctx->device) !=
+                hipSuccess ||
+            v <= 0)
+            v = 256;  // MI355X
+        ctx->ncu = v;
+    }
+    return ctx->ncu;
+}
+
 int sph_ensure_pinned(sph_ctx *ctx, size_t bytes)
 {
     if (bytes <= ctx->pinned_bytes) return SPH_OK;

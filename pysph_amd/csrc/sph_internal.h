@@ -26,7 +26,11 @@ struct sph_ctx {
     int32_t key_dims[3], key_bits[3], key_order;
     // set once a graph refers to the scratch: it may no longer be reallocated
     int frozen;
+    // compute units of the device (queried on first use)
+    int ncu;
 };
+
+int sph_cu_count(sph_ctx *ctx);
 
 int sph_set_error(sph_ctx *ctx, int code, const char *fmt, ...);
 int sph_ensure_scratch(sph_ctx *ctx, size_t bytes);
@@ -119,7 +123,7 @@ int sph_nnps_build_batch(sph_ctx *ctx, const GridDev *grids, const int32_t *sim_
                          const int32_t *cells, const int32_t *ncell_dev, int64_t max_cells,
                          int64_t n, double radius_scale, int maxch, int32_t *counts,
                          int64_t *slice_off, int32_t *indices, int64_t cap, int32_t *status,
-                         int64_t *nnz_acc, int32_t trash);
+                         int64_t *nnz_acc, int32_t trash, int cells_per_block);
 
 // TaitEquation over sorted records (all particles real, no accumulation):
 // p, cs into the original-order arrays and the record fields, P/rho^2 into

@@ -551,6 +551,9 @@ struct CellArgs {
     // cells flagged by the count pass, for the per-cell fill
     int32_t *ovf_list;
     int32_t *novf;
+    // destination cells per block of the count pass (its grid is the cell
+    // count over this, at most NN_CELL_GRID blocks)
+    int cells_per_block;
 };
 
 __device__ __forceinline__ int wave_max_i(int v)
@@ -971,7 +974,9 @@ static int nnps_count(sph_ctx *ctx, CellArgs &a, int64_t ncell_launch, int64_t n
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, a.novf, 1, 0);
     SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
     auto count_kern = a.f32 ? k_nnps_cells<false, float> : k_nnps_cells<false, double>;
-    const int64_t count_grid = ncell_launch < NN_CELL_GRID ? ncell_launch : NN_CELL_GRID;
+    const int cpb = a.cells_per_block > 0 ? a.cells_per_block : 1;
+    const int64_t cgrid = (ncell_launch + cpb - 1) / cpb;
+    const int64_t count_grid = cgrid < NN_CELL_GRID ? cgrid : NN_CELL_GRID;
     hipLaunchKernelGGL(count_kern, dim3((unsigned)count_grid), dim3(CW), 0, ctx->stream, a,
                        counts, (const int64_t *)nullptr, (int32_t *)nullptr);
     SPH_LAUNCH_CHECK(ctx, "k_nnps_cells<count>");
@@ -1128,9 +1133,10 @@ int sph_nnps_build_batch(sph_ctx *ctx, const GridDev *grids, const int32_t *sim_
                          const int32_t *cells, const int32_t *ncell_dev, int64_t max_cells,
                          int64_t n, double radius_scale, int maxch, int32_t *counts,
                          int64_t *slice_off, int32_t *indices, int64_t cap, int32_t *status,
-                         int64_t *nnz_acc, int32_t trash)
+                         int64_t *nnz_acc, int32_t trash, int cells_per_block)
 {
     SPH_REQUIRE(ctx, maxch >= 1 && maxch <= NN_MAXCH, "sph_nnps_build_batch: bad maxch");
+    SPH_REQUIRE(ctx, cells_per_block >= 1, "sph_nnps_build_batch: bad cells_per_block");
     int rc = nnps_clear(ctx, n, counts, slice_off);
     if (rc) return rc;
     if (n == 0 || max_cells == 0) return SPH_OK;
@@ -1160,6 +1166,7 @@ int sph_nnps_build_batch(sph_ctx *ctx, const GridDev *grids, const int32_t *sim_
     a.cap = cap;
     a.status = status;
     a.trash = trash;
+    a.cells_per_block = cells_per_block;
     int64_t *nnz_dev = nullptr;
     rc = nnps_count(ctx, a, max_cells, n, counts, slice_off, &nnz_dev);
     if (rc) return rc;

@@ -1,0 +1,97 @@
+// Per-pair terms of the neighbour operators, shared by the operator kernels
+// and the fused kernels of the batched path.  Every expression follows the
+// reference Cython in order (sph/funcs/density_funcs.pyx:37-73,
+// sph/funcs/pressure_funcs.pyx:246-391); the sources are compiled without
+// FMA contraction.
+#pragma once
+#include "sph_internal.h"
+
+// Sorted records: fields SPH_R_* of SPH_REC values of the arithmetic type T
+// (double, or float for the single-precision variant), 16-byte (8-byte)
+// aligned; the neighbour loops fetch them as pairs.
+template <typename T> struct Pair;
+template <> struct Pair<double> { using type = double2; };
+template <> struct Pair<float> { using type = float2; };
+
+template <typename T>
+__device__ __forceinline__ const typename Pair<T>::type *rec2(const T *rec, int64_t j)
+{
+    return reinterpret_cast<const typename Pair<T>::type *>(rec + j * SPH_REC);
+}
+
+// ---------------------------------------------------------------------------
+// SPHRho
+// ---------------------------------------------------------------------------
+// one neighbour term of SPHRho (density_funcs.pyx:41-73); sxy = (x, y) and
+// szm = (z, m) of the source record, hb its h (not read when UH)
+template <bool UH, typename T, typename P>
+__device__ __forceinline__ T rho_term(int dim, T xa, T ya, T za, T ha, T facA, P sxy, P szm,
+                                      T hb)
+{
+    // with h_b == h_a, 0.5*(h_a+h_b) == h_a exactly and h_dim(h)*sigma is the
+    // destination's value: reuse it instead of dividing again
+    T hab, fac;
+    if (UH || hb == ha) {
+        hab = ha;
+        fac = facA;
+    } else {
+        hab = T(0.5) * (ha + hb);
+        fac = sph_h_dim(hab, dim) * sph_cubic_fac1<T>(dim);
+    }
+    return sph_cubic_w_fac<T>(xa - sxy.x, ya - sxy.y, za - szm.x, hab, fac) * szm.y;
+}
+
+// ---------------------------------------------------------------------------
+// MomentumEquation with Monaghan artificial viscosity
+// ---------------------------------------------------------------------------
+template <typename T> struct MomDest {
+    T xa, ya, za, ha, facA, rhoa, ca, Pa, ua, va, wa, rhoa2;
+};
+
+// one neighbour term of MomentumEquation (pressure_funcs.pyx:252-391).
+// Source record slots: s0 = (x, y), s1 = (z, m), s2 = (u, v), s3 = (w, P/rho^2),
+// s4 = (rho, cs); hb its h (not read when UH).
+template <bool UH, typename T, typename P>
+__device__ __forceinline__ void mom_term(int dim, const MomDest<T> &D, T alpha, T beta, T eta,
+                                         const P s0, const P s1, const P s2, const P s3,
+                                         const P s4, T hb, T &dtf, T &nr0, T &nr1, T &nr2)
+{
+    T hab, fac;
+    if (UH || hb == D.ha) {  // 0.5*(h_a+h_a) == h_a: reuse the destination's h_dim(h)*sigma
+        hab = D.ha;
+        fac = D.facA;
+    } else {
+        hab = T(0.5) * (D.ha + hb);
+        fac = sph_h_dim(hab, dim) * sph_cubic_fac1<T>(dim);
+    }
+    const T rabx = D.xa - s0.x;
+    const T raby = D.ya - s0.y;
+    const T rabz = D.za - s1.x;
+    const T vabx = D.ua - s2.x;
+    const T vaby = D.va - s2.y;
+    const T vabz = D.wa - s3.x;
+    const T dot = vabx * rabx + vaby * raby + vabz * rabz;
+    const T rab2 = rabx * rabx + raby * raby + rabz * rabz;
+    const T den = rab2 + eta * eta * hab * hab;
+    const T q = hab * dot / den;  // the same quotient as mu below
+    const T f = fabs(q);
+    dtf = f > dtf ? f : dtf;
+    const T mb = s1.y, rhob = s4.x, cb = s4.y;
+    // Pb * (1.0/(rhob*rhob)), computed once per source particle (k_prho2)
+    T tmp = D.Pa * D.rhoa2 + s3.y;
+    // Monaghan viscosity, applied when dot < 0; evaluated for every pair and
+    // selected, which costs less than a divergent branch in a wavefront
+    const T cab = T(0.5) * (D.ca + cb);
+    const T rhoab = T(0.5) * (D.rhoa + rhob);
+    const T mu = q;  // mu = hab*dot; mu /= (rab2 + eta*eta*hab*hab)
+    T pi_v = -alpha * cab * mu + beta * mu * mu;
+    pi_v /= rhoab;
+    const T piab = dot < 0 ? pi_v : T(0.0);
+    tmp += piab;
+    tmp *= -mb;
+    const T g = sph_cubic_gfac_fac<T>(rabx, raby, rabz, hab, fac);
+    nr0 += tmp * (rabx * g);
+    nr1 += tmp * (raby * g);
+    nr2 += tmp * (rabz * g);
+}
+

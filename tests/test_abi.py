@@ -72,3 +72,42 @@ def test_product_path_has_no_oracle_dependency():
             if fn.endswith((".py", ".hip", ".h", ".cpp")):
                 text = open(os.path.join(dirpath, fn)).read()
                 assert "oracle" not in text and "sphref" not in text, fn
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes mirrors of sph_grid, sph_prop_list(_f32), sph_step_props and
+    sph_batch_desc have the C compiler's field offsets and sizes."""
+    import ctypes
+    import shutil
+    import subprocess
+    from pysph_amd import _lib
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    structs = {name: getattr(_lib, cls) for name, cls in
+               (("sph_grid", "SphGrid"), ("sph_prop_list", "PropList"),
+                ("sph_prop_list_f32", "PropListF32"), ("sph_step_props", "StepProps"),
+                ("sph_batch_desc", "BatchDesc"))
+               if hasattr(_lib, cls)}
+    assert "sph_batch_desc" in structs
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sphhip.h"', 'int main(void){']
+    for cname, cls in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f in cls._fields_:
+            lines.append(f'printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.check_call([gcc, "-std=c11", "-I", os.path.join(ROOT, "include"), str(src),
+                           "-o", str(exe)])
+    out = subprocess.check_output([str(exe)], text=True).split("\n")
+    got = {}
+    for ln in out:
+        if ln:
+            c, f, v = ln.split()
+            got[(c, f)] = int(v)
+    for cname, cls in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for f in cls._fields_:
+            assert got[(cname, f[0])] == getattr(cls, f[0]).offset, (cname, f[0])

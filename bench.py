@@ -53,6 +53,9 @@ def _args():
     ap.add_argument("--nsims", type=int, default=256, help="drops of --workload batch")
     ap.add_argument("--no-autotune", action="store_true",
                     help="--workload batch: keep the default launch shapes")
+    ap.add_argument("--launch-shape", default="",
+                    help="--workload batch: fixed launch shapes instead of the autotuner, "
+                         "e.g. cells_per_block=8,nbr_tile=4")
     ap.add_argument("--graph-steps", type=int, default=10,
                     help="RK2 steps per captured graph (--workload batch)")
     ap.add_argument("--seq-sims", type=int, default=0,
@@ -259,7 +262,11 @@ def run_batch(args):
     b.check()
     progress("eager step done")
     tune = None
-    if not args.no_autotune:
+    if args.launch_shape:
+        b.set_tuning({k: int(v) for k, v in
+                      (kv.split("=") for kv in args.launch_shape.split(","))})
+        progress(f"launch shape {b.tuning()}")
+    elif not args.no_autotune:
         from pysph_amd.autotune import autotune
         tune = autotune(b, dt, steps=3, use_cache=False, log=progress)
         progress(f"autotuned: {tune['tuning']} {tune['ms_per_step']:.3f} ms/step "
@@ -419,10 +426,11 @@ def run_adjoint(args):
             fd.append((L[k] - L[k + 1]) / (2 * e))
         g, fd = np.array(g), np.array(fd)
         rel = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-300)
-        return rel, np.abs(g - fd).max() / np.abs(fd).max()
+        return rel, np.abs(g - fd).max() / np.abs(fd).max(), \
+            float(np.linalg.norm(g - fd) / np.linalg.norm(fd))
 
-    rel6, norm6 = fd_errors(eps)
-    rel8, norm8 = fd_errors(1e-8)
+    rel6, norm6, l2_6 = fd_errors(eps)
+    rel8, norm8, l2_8 = fd_errors(1e-8)
     d0 = dict(d)
     d0["u"], d0["v"] = u_init.copy(), v_init.copy()
     alpha_sweep = {}
@@ -431,31 +439,39 @@ def run_adjoint(args):
         lb = batch_losses([d0], 0.1 - e)[0]
         fda = (la - lb) / (2 * e)
         alpha_sweep[f"{e:g}"] = abs(float(ga) - fda) / abs(fda)
-    progress(f"FD eps 1e-6: {len(rel6)} velocity components, max rel err {rel6.max():.3g}, "
-             f"median {np.median(rel6):.3g}, max err / max |FD| {norm6:.3g}; eps 1e-8: max "
-             f"{rel8.max():.3g}, median {np.median(rel8):.3g}; alpha rel err by eps {alpha_sweep}")
-    # fit in scaled variables: velocities in units of 100 about the initial
-    # guess, loss relative to its initial value
-    s_u = 100.0
+    progress(f"FD eps 1e-6: {len(rel6)} velocity components, |g - FD| / |FD| {l2_6:.3g}, "
+             f"max err / max |FD| {norm6:.3g}, componentwise max {rel6.max():.3g} median "
+             f"{np.median(rel6):.3g}; eps 1e-8: |g - FD| / |FD| {l2_8:.3g}; alpha rel err by "
+             f"eps {alpha_sweep}")
+    # fit in scaled variables, loss relative to its initial value l0.  The
+    # final positions move by about T = steps*dt per unit of initial
+    # velocity, so the Gauss-Newton curvature of the mean squared error in
+    # one velocity component is 2 T^2 / n: velocities in units of
+    # s_u = sqrt(l0 n / 2) / T give the velocity block a unit curvature.
+    # alpha moves every particle at once; its unit is 1e-3.
+    l0 = float(loss.item())
+    T = steps * dt
+    s_u = (l0 * n / 2.0) ** 0.5 / T
+    s_a = 1e-3
     du = torch.zeros(n, dtype=torch.float64, device=dev, requires_grad=True)
     dv = torch.zeros(n, dtype=torch.float64, device=dev, requires_grad=True)
-    a_fit = torch.tensor(0.1, dtype=torch.float64, device=dev, requires_grad=True)
+    da = torch.zeros((), dtype=torch.float64, device=dev, requires_grad=True)
     u_base = torch.tensor(u_init, device=dev)
     v_base = torch.tensor(v_init, device=dev)
-    opt = torch.optim.LBFGS([du, dv, a_fit], lr=1.0, max_iter=args.lbfgs_iters,
+    opt = torch.optim.LBFGS([du, dv, da], lr=1.0, max_iter=args.lbfgs_iters,
                             history_size=20, line_search_fn="strong_wolfe",
                             tolerance_grad=1e-14, tolerance_change=1e-18)
     evals = [0]
     losses = []
-    l0 = float(loss.item())
 
     def closure():
         opt.zero_grad()
-        xx, yy = rollout(roll, a_fit, u_base + s_u * du, v_base + s_u * dv)
+        xx, yy = rollout(roll, 0.1 + s_a * da, u_base + s_u * du, v_base + s_u * dv)
         lo = loss_fn(xx, yy) / l0
         lo.backward()
         evals[0] += 1
         losses.append(float(lo.item()) * l0)
+        progress(f"L-BFGS evaluation {evals[0]}: loss {losses[-1]:.4g}")
         return lo
 
     t0 = time.perf_counter()
@@ -473,6 +489,7 @@ def run_adjoint(args):
                                f"{steps} steps, checkpoint every {ckpt}, FP64",
                    "forward_ms": 1e3 * t_fwd, "backward_ms": 1e3 * t_bwd,
                    "fd_eps": eps, "fd_components": int(len(rel6)),
+                   "fd_norm_rel_err": l2_6, "fd_eps_1e-8_norm_rel_err": l2_8,
                    "fd_max_rel_err": float(rel6.max()),
                    "fd_median_rel_err": float(np.median(rel6)),
                    "fd_max_err_over_max_fd": float(norm6),
@@ -482,7 +499,8 @@ def run_adjoint(args):
                    "grad_alpha": float(ga),
                    "lbfgs_iterations": args.lbfgs_iters, "lbfgs_evaluations": evals[0],
                    "loss_initial": losses[0], "loss_final": losses[-1],
-                   "alpha_fitted": float(a_fit.item()), "fit_s": t_fit,
+                   "loss_reduction": losses[-1] / losses[0],
+                   "alpha_fitted": 0.1 + s_a * float(da.item()), "fit_s": t_fit,
                    "fit_max_velocity_change": float(s_u * max(du.abs().max().item(),
                                                               dv.abs().max().item()))},
         "roofline": None,

@@ -9,6 +9,11 @@
 #include "sph_terms.h"
 
 #define OP_THREADS 256
+// neighbour records whose loads are in flight before their terms are added
+// (measured on the 100^3 lattice with 75 neighbours per particle: density
+// 0.34 ms at 4, 0.31 ms at 8; momentum 0.66 ms at 2, 0.60 ms at 4)
+#define DENS_NT 8
+#define MOM_NT 4
 
 // ---------------------------------------------------------------------------
 // SPHRho (rho_term in sph_terms.h)
@@ -41,29 +46,24 @@ __global__ void __launch_bounds__(OP_THREADS)
     const int32_t *row = indices + slice_off[t >> 6] + (t & 63);
     const int32_t e = counts[t];
     int32_t k = 0;
-    // four neighbours' loads in flight, then their terms added in list order
-    for (; k + 4 <= e; k += 4) {
-        const int32_t j0 = __builtin_nontemporal_load(row + ((int64_t)k << 6)),
-                      j1 = __builtin_nontemporal_load(row + ((int64_t)(k + 1) << 6)),
-                      j2 = __builtin_nontemporal_load(row + ((int64_t)(k + 2) << 6)),
-                      j3 = __builtin_nontemporal_load(row + ((int64_t)(k + 3) << 6));
-        const auto *s0 = rec2(srec, j0), *s1 = rec2(srec, j1), *s2 = rec2(srec, j2),
-                   *s3 = rec2(srec, j3);
-        const auto a0 = s0[0], b0 = s0[1];
-        const auto a1 = s1[0], b1 = s1[1];
-        const auto a2 = s2[0], b2 = s2[1];
-        const auto a3 = s3[0], b3 = s3[1];
-        T h0 = ha, h1 = ha, h2 = ha, h3 = ha;
-        if (!UH) {
-            h0 = srec[(int64_t)j0 * SPH_REC + SPH_R_H];
-            h1 = srec[(int64_t)j1 * SPH_REC + SPH_R_H];
-            h2 = srec[(int64_t)j2 * SPH_REC + SPH_R_H];
-            h3 = srec[(int64_t)j3 * SPH_REC + SPH_R_H];
+    // DENS_NT neighbours' loads in flight, then their terms added in list order
+    for (; k + DENS_NT <= e; k += DENS_NT) {
+        int32_t j[DENS_NT];
+        typename Pair<T>::type a[DENS_NT], b[DENS_NT];
+        T hb[DENS_NT];
+#pragma unroll
+        for (int u = 0; u < DENS_NT; ++u)
+            j[u] = __builtin_nontemporal_load(row + ((int64_t)(k + u) << 6));
+#pragma unroll
+        for (int u = 0; u < DENS_NT; ++u) {
+            const auto *sr = rec2(srec, j[u]);
+            a[u] = sr[0];
+            b[u] = sr[1];
+            hb[u] = UH ? ha : srec[(int64_t)j[u] * SPH_REC + SPH_R_H];
         }
-        nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, a0, b0, h0);
-        nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, a1, b1, h1);
-        nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, a2, b2, h2);
-        nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, a3, b3, h3);
+#pragma unroll
+        for (int u = 0; u < DENS_NT; ++u)
+            nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, a[u], b[u], hb[u]);
     }
     for (; k < e; ++k) {
         const int32_t j = __builtin_nontemporal_load(row + ((int64_t)k << 6));
@@ -285,23 +285,35 @@ __global__ void __launch_bounds__(OP_THREADS)
     const int32_t *row = indices + slice_off[t >> 6] + (t & 63);  // SELL-64 row
     const int32_t e = counts[t];
     int32_t k = 0;
-    for (; k + 2 <= e; k += 2) {  // two neighbours' loads in flight
-        const int32_t jp = __builtin_nontemporal_load(row + ((int64_t)k << 6)),
-                      jq = __builtin_nontemporal_load(row + ((int64_t)(k + 1) << 6));
-        const auto *p = rec2(srec, jp), *q = rec2(srec, jq);
-        const auto p0 = p[0], p1 = p[1], p2 = p[2], p3 = p[3], p4 = p[4];
-        const auto q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4];
-        const T hp = UH ? D.ha : srec[(int64_t)jp * SPH_REC + SPH_R_H];
-        const T hq = UH ? D.ha : srec[(int64_t)jq * SPH_REC + SPH_R_H];
-        mom_term<UH, T>(dim, D, alpha, beta, eta, p0, p1, p2, p3, p4, hp, dtf, nr0, nr1, nr2);
-        mom_term<UH, T>(dim, D, alpha, beta, eta, q0, q1, q2, q3, q4, hq, dtf, nr0, nr1, nr2);
+    // MOM_NT neighbours' records in flight, then their terms added in list order
+    for (; k + MOM_NT <= e; k += MOM_NT) {
+        int32_t j[MOM_NT];
+        typename Pair<T>::type s0[MOM_NT], s1[MOM_NT], s2[MOM_NT], s3[MOM_NT], s4[MOM_NT];
+        T hb[MOM_NT];
+#pragma unroll
+        for (int u = 0; u < MOM_NT; ++u)
+            j[u] = __builtin_nontemporal_load(row + ((int64_t)(k + u) << 6));
+#pragma unroll
+        for (int u = 0; u < MOM_NT; ++u) {
+            const auto *sr = rec2(srec, j[u]);
+            s0[u] = sr[0];
+            s1[u] = sr[1];
+            s2[u] = sr[2];
+            s3[u] = sr[3];
+            s4[u] = sr[4];
+            hb[u] = UH ? D.ha : srec[(int64_t)j[u] * SPH_REC + SPH_R_H];
+        }
+#pragma unroll
+        for (int u = 0; u < MOM_NT; ++u)
+            mom_term<UH, T>(dim, D, alpha, beta, eta, s0[u], s1[u], s2[u], s3[u], s4[u], hb[u],
+                            dtf, nr0, nr1, nr2);
     }
-    if (k < e) {
-        const int32_t jp = __builtin_nontemporal_load(row + ((int64_t)k << 6));
-        const auto *p = rec2(srec, jp);
-        const T hp = UH ? D.ha : srec[(int64_t)jp * SPH_REC + SPH_R_H];
-        mom_term<UH, T>(dim, D, alpha, beta, eta, p[0], p[1], p[2], p[3], p[4], hp, dtf, nr0,
-                        nr1, nr2);
+    for (; k < e; ++k) {
+        const int32_t j = __builtin_nontemporal_load(row + ((int64_t)k << 6));
+        const auto *sr = rec2(srec, j);
+        const T hb = UH ? D.ha : srec[(int64_t)j * SPH_REC + SPH_R_H];
+        mom_term<UH, T>(dim, D, alpha, beta, eta, sr[0], sr[1], sr[2], sr[3], sr[4], hb, dtf,
+                        nr0, nr1, nr2);
     }
     dt_fac[i] = dtf;
     ax[i] = (accumulate ? T(ax[i]) : T(0.0)) + nr0;

@@ -256,6 +256,41 @@ int sph_momentum_av_f32(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real
                         int uniform_h, double alpha, double beta, double eta, int accumulate,
                         double *ax, double *ay, double *az, double *dt_fac);
 
+/* SPHDensityRate (sph/funcs/density_funcs.pyx:116-164, hks off): the
+ * continuity equation rate_a = sum_b (v_a - v_b) . gradW_a(r_ab, h_ab) m_b over
+ * the SELL-64 lists, in list order; out (original order) = [out +] rate, 0 for
+ * non-LocalReal destinations (sph_func.pyx:208-227).  Records need x y z m u
+ * v w (and h when !uniform_h). */
+int sph_density_rate(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                     const int32_t *d_perm, const int32_t *counts, const int64_t *slice_off,
+                     const int32_t *indices, const double *d_rec, const double *s_rec,
+                     int uniform_h, int accumulate, double *out);
+int sph_density_rate_f32(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                         const int32_t *d_perm, const int32_t *counts, const int64_t *slice_off,
+                         const int32_t *indices, const float *d_rec, const float *s_rec,
+                         int uniform_h, int accumulate, double *out);
+
+/* XSPHCorrection (sph/funcs/xsph_funcs.pyx:36-91, hks and the Bonnet-Lok
+ * correction off): out_c = [out_c +] sum_b (m_b W_ab / rho_ab) (v_b - v_a)_c eps,
+ * rho_ab = (rho_b + rho_a)/2; oy, oz may be NULL.  Records need x y z m u v w
+ * rho (and h when !uniform_h). */
+int sph_xsph(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real, const int32_t *d_perm,
+             const int32_t *counts, const int64_t *slice_off, const int32_t *indices,
+             const double *d_rec, const double *s_rec, int uniform_h, double eps,
+             int accumulate, double *ox, double *oy, double *oz);
+int sph_xsph_f32(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                 const int32_t *d_perm, const int32_t *counts, const int64_t *slice_off,
+                 const int32_t *indices, const float *d_rec, const float *s_rec, int uniform_h,
+                 double eps, int accumulate, double *ox, double *oy, double *oz);
+
+/* Reductions of the time-step functions (solver/time_step_functions.py:8-57,
+ * Integrator.get_max_acceleration, solver/integrator.py:369-412): over n
+ * elements, out[0] = max |(a, b, c)| (sqrt(a*a + b*b [+ c*c]) with ncomp
+ * components, |a| with one), out[1] = min a, out[2] = max a; NaN propagates as
+ * in numpy.  out is host memory; the call waits for the stream. */
+int sph_extrema(sph_ctx *ctx, int64_t n, int ncomp, const double *a, const double *b,
+                const double *c, double *out);
+
 /* GravityForce (sph/funcs/external_force.pyx:38-44): a (+)= g; az may be
  * NULL.  Original order. */
 int sph_gravity(sph_ctx *ctx, int64_t n_real, int64_t n_total, double gx, double gy, double gz,

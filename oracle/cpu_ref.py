@@ -56,6 +56,10 @@ def _load():
     lib.sphref_tait.restype = None
     lib.sphref_momentum.argtypes = [i32, i64] + [vp] * 23 + [dbl, dbl, dbl] + [vp] * 4 + [i32]
     lib.sphref_momentum.restype = None
+    lib.sphref_density_rate.argtypes = [i32, i64] + [vp] * 18 + [i32]
+    lib.sphref_density_rate.restype = None
+    lib.sphref_xsph.argtypes = [i32, i64] + [vp] * 19 + [dbl, vp, vp, vp, i32]
+    lib.sphref_xsph.restype = None
     lib.sphref_gravity.argtypes = [i64, dbl, dbl, dbl, vp, vp, vp]
     lib.sphref_gravity.restype = None
     lib.sphref_step.argtypes = [i64, vp, vp, dbl, vp]
@@ -169,6 +173,26 @@ def momentum(dim, dst, src, offsets, indices, alpha, beta, eta, ax, ay, az, dt_f
                         _p(ax), _p(ay), _p(az), _p(dt_fac), nthreads)
 
 
+def density_rate(dim, dst, src, offsets, indices, out, nthreads=1):
+    """out[i] += sum_j (v_i - v_j) . gradW_ij m_j (sph/funcs/density_funcs.pyx:116-164)."""
+    lib = _load()
+    nd = len(offsets) - 1
+    d = [_f64(dst[k]) for k in ("x", "y", "z", "h", "u", "v", "w")]
+    s = [_f64(src[k]) for k in ("x", "y", "z", "h", "m", "u", "v", "w")]
+    lib.sphref_density_rate(int(dim), nd, *[_p(c) for c in d], *[_p(c) for c in s],
+                            _p(offsets), _p(indices), _p(out), nthreads)
+
+
+def xsph(dim, dst, src, offsets, indices, eps, ox, oy, oz, nthreads=1):
+    """o[i] += sum_j m_j W_ij / rho_ij (v_j - v_i) eps (sph/funcs/xsph_funcs.pyx:36-91)."""
+    lib = _load()
+    nd = len(offsets) - 1
+    d = [_f64(dst[k]) for k in ("x", "y", "z", "h", "rho", "u", "v", "w")]
+    s = [_f64(src[k]) for k in ("x", "y", "z", "h", "m", "rho", "u", "v", "w")]
+    lib.sphref_xsph(int(dim), nd, *[_p(c) for c in d], *[_p(c) for c in s], _p(offsets),
+                    _p(indices), float(eps), _p(ox), _p(oy), _p(oz), nthreads)
+
+
 def gravity(n, g, ax, ay, az):
     _load().sphref_gravity(int(n), float(g[0]), float(g[1]), float(g[2]), _p(ax), _p(ay), _p(az))
 
@@ -211,15 +235,19 @@ class RefSimulation:
 
     Operations, in solver order (sph_equation.py:101-228, solver.py:222-282):
       1. SPHRho            on fluids, from ``density_from`` types, updates rho
+         (``density="continuity"``: SPHDensityRate instead, integrating rho)
       2. TaitEquation      on fluids, updates p, cs
       3. MomentumEquation  on fluids, from ``momentum_from`` types, integrates u,v(,w)
       4. GravityForce      on fluids (if ``gravity`` given), integrates u,v(,w)
       5. PositionStepping  on fluids, integrates x,y(,z)
+      6. XSPHCorrection    on fluids from fluids (if ``xsph_eps`` given), integrates
+         x,y(,z) (Solver.add_operation_xsph, solver.py:186-220)
     """
 
     def __init__(self, arrays, dim, integrator="rk2", co=1.0, ro=1000.0, gamma=7.0,
                  alpha=1.0, beta=1.0, eta=0.1, gravity=None, density_from=(0,),
-                 momentum_from=(0,), cell_radius_scale=2.0, nthreads=1):
+                 momentum_from=(0,), cell_radius_scale=2.0, nthreads=1, density="summation",
+                 xsph_eps=None):
         self.arrays = arrays
         self.dim = dim
         self.integrator = integrator
@@ -234,6 +262,12 @@ class RefSimulation:
         self.vel = ["u", "v", "w"][:dim]
         self.pos = ["x", "y", "z"][:dim]
         self.stepped = self.vel + self.pos  # momentum/gravity updates, then position step
+        if density not in ("summation", "continuity"):
+            raise ValueError(density)
+        self.density = density
+        if density == "continuity":
+            self.stepped = ["rho"] + self.stepped
+        self.xsph_eps = xsph_eps
         self.lists = None
         self.dirty = True
         self.pairs = 0
@@ -248,7 +282,8 @@ class RefSimulation:
         self.lists = {}
         for d in self.fluids:
             for s in self.arrays:
-                if s.ptype in set(self.density_from) | set(self.momentum_from):
+                if s.ptype in set(self.density_from) | set(self.momentum_from) or (
+                        self.xsph_eps is not None and s.ptype == 0):
                     self.lists[(d.name, s.name)] = nnps(d, s, 2.0, cs, d is s, self.nthreads)
         self.dirty = False
 
@@ -262,6 +297,15 @@ class RefSimulation:
         dim = self.dim
         lists = self._lists()
         for d in self.fluids:
+            if self.density == "continuity":
+                a_rho = acc[d.name]["rho"]
+                a_rho[:] = 0.0
+                for s in self.arrays:
+                    if s.ptype in self.density_from:
+                        off, idx = lists[(d.name, s.name)]
+                        density_rate(dim, d, s, off, idx, a_rho, self.nthreads)
+                        self.pairs += len(idx)
+                continue
             d["rho"][:] = 0.0
             for s in self.arrays:
                 if s.ptype in self.density_from:
@@ -274,7 +318,7 @@ class RefSimulation:
             tait(d["rho"], self.ro, self.co, self.gamma, d["p"], d["cs"])
         for d in self.fluids:
             a = acc[d.name]
-            for k in self.stepped:
+            for k in self.vel + self.pos:
                 a[k][:] = 0.0
             d["dt_fac"][:] = 0.0
             az = a["w"] if dim == 3 else np.zeros(d.n)
@@ -288,6 +332,14 @@ class RefSimulation:
                 gravity(d.n, self.gravity, a["u"], a["v"], az)
             for vk, xk in zip(self.vel, self.pos):
                 a[xk][:] = d[vk]
+            if self.xsph_eps is not None:
+                zero = np.zeros(d.n)
+                ax_ = [a[k] for k in self.pos] + [zero] * (3 - dim)
+                for s in self.fluids:
+                    off, idx = lists[(d.name, s.name)]
+                    xsph(dim, d, s, off, idx, self.xsph_eps, ax_[0], ax_[1], ax_[2],
+                         self.nthreads)
+                    self.pairs += len(idx)
 
     def _step(self, acc, init, dt):
         for d in self.fluids:

@@ -23,6 +23,8 @@
  *   SPHRho         sum m_b W(r_ab, (h_a+h_b)/2)       sph/funcs/density_funcs.pyx:37-73
  *   Tait           p, cs                              sph/funcs/eos_funcs.pyx:80-129
  *   Momentum + AV  pressure + Monaghan Pi_ab, dt_fac  sph/funcs/pressure_funcs.pyx:185-391
+ *   DensityRate    sum (va-vb).gradW m_b               sph/funcs/density_funcs.pyx:116-164
+ *   XSPH           sum m_b W/rho_ab (vb-va) eps        sph/funcs/xsph_funcs.pyx:36-91
  *   Gravity        += (gx,gy,gz)                      sph/funcs/external_force.pyx:38-44
  *   PositionStep   a_x = u                            sph/funcs/position_funcs.pyx:29-43
  *   stage          x = x0 + a*dt ; PC final 2x - x0   solver/integrator.py:340-367, :632-683
@@ -410,6 +412,74 @@ void sphref_momentum(int dim, int64_t nd_real,
         ax[i] += nr0;
         ay[i] += nr1;
         if (az) az[i] += nr2;
+    }
+}
+
+/* SPHDensityRate.eval_nbr, sph/funcs/density_funcs.pyx:116-164 (hks off,
+ * no kernel corrections): nr += cPoint_dot(va - vb, gradW(dst, src, hab)) * m_b,
+ * summed by SPHFunctionParticle.eval_single (sph/sph_func.pyx:427-441);
+ * out += result. */
+void sphref_density_rate(int dim, int64_t nd_real,
+                         const double *dx, const double *dy, const double *dz, const double *dh,
+                         const double *du, const double *dv, const double *dw,
+                         const double *sx, const double *sy, const double *sz, const double *sh,
+                         const double *sm, const double *su, const double *sv, const double *sw,
+                         const int64_t *offsets, const int32_t *indices, double *out,
+                         int nthreads)
+{
+    (void)nthreads;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int64_t i = 0; i < nd_real; ++i) {
+        double nr = 0.0;
+        for (int64_t k = offsets[i]; k < offsets[i + 1]; ++k) {
+            int32_t j = indices[k];
+            double ha = dh[i];
+            double hb = sh[j];
+            double hab = 0.5 * (ha + hb);
+            double velx = du[i] - su[j];
+            double vely = dv[i] - sv[j];
+            double velz = dw[i] - sw[j];
+            double g[3];
+            cubic_gradw(dim, dx[i], dy[i], dz[i], sx[j], sy[j], sz[j], hab, g);
+            nr += (velx * g[0] + vely * g[1] + velz * g[2]) * sm[j];
+        }
+        out[i] += nr;
+    }
+}
+
+/* XSPHCorrection.eval_nbr_csph, sph/funcs/xsph_funcs.pyx:36-91 (hks off, no
+ * Bonnet-Lok correction, so CSPHFunctionParticle.eval_single divides by
+ * nothing, sph/sph_func.pyx:513-531): temp = m_b*w/rho_ab,
+ * nr += temp*(vb - va)*eps; out += result. */
+void sphref_xsph(int dim, int64_t nd_real,
+                 const double *dx, const double *dy, const double *dz, const double *dh,
+                 const double *drho, const double *du, const double *dv, const double *dw,
+                 const double *sx, const double *sy, const double *sz, const double *sh,
+                 const double *sm, const double *srho, const double *su, const double *sv,
+                 const double *sw, const int64_t *offsets, const int32_t *indices, double eps,
+                 double *ox, double *oy, double *oz, int nthreads)
+{
+    (void)nthreads;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int64_t i = 0; i < nd_real; ++i) {
+        double nr0 = 0.0, nr1 = 0.0, nr2 = 0.0;
+        for (int64_t k = offsets[i]; k < offsets[i + 1]; ++k) {
+            int32_t j = indices[k];
+            double ha = dh[i];
+            double hb = sh[j];
+            double hab = 0.5 * (ha + hb);
+            double rhoab = 0.5 * (srho[j] + drho[i]);
+            double vbax = su[j] - du[i], vbay = sv[j] - dv[i], vbaz = sw[j] - dw[i];
+            double mb = sm[j];
+            double w = cubic_w(dim, dx[i], dy[i], dz[i], sx[j], sy[j], sz[j], hab);
+            double temp = mb * w / rhoab;
+            nr0 += temp * vbax * eps;
+            nr1 += temp * vbay * eps;
+            nr2 += temp * vbaz * eps;
+        }
+        ox[i] += nr0;
+        if (oy) oy[i] += nr1;
+        if (oz) oz[i] += nr2;
     }
 }
 

@@ -103,6 +103,15 @@ _SIGNATURES = {
     "sph_momentum_av": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32,
                                _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp, _vp]),
     "sph_cubic_spline": (_i32, [_vp, _i32, _i64] + [_vp] * 11),
+    "sph_density_rate": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32,
+                                _vp]),
+    "sph_density_rate_f32": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32,
+                                    _i32, _vp]),
+    "sph_xsph": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _dbl, _i32,
+                        _vp, _vp, _vp]),
+    "sph_xsph_f32": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _dbl,
+                            _i32, _vp, _vp, _vp]),
+    "sph_extrema": (_i32, [_vp, _i64, _i32, _vp, _vp, _vp, ctypes.POINTER(_dbl)]),
     "sph_gravity": (_i32, [_vp, _i64, _i64, _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp]),
     "sph_position_stepping": (_i32, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sph_stage_step": (_i32, [_vp, _i64, ctypes.POINTER(StepProps), _dbl]),

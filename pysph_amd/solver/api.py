@@ -3,5 +3,6 @@ from .integrator import (EulerIntegrator, Integrator, PredictorCorrectorIntegrat
                          RK2Integrator)
 from .solver import Solver
 from .sph_equation import SPHIntegration, SPHOperation
-from .time_step_functions import TimeStep
+from .time_step_functions import (TimeStep, ViscousTimeStep, ViscousAndForceBasedTimeStep,
+                                  VelocityBasedTimeStep)
 from .generators import get_circular_patch, elliptical_drop, dam_break_3d, jittered_lattice_3d

@@ -15,8 +15,11 @@ from .generators import dam_break_3d, elliptical_drop
 
 
 def drop_solver(dx=0.025, radius=1.0, vel=100.0, c0=1400.0, rho0=1.0, alpha=0.1, beta=0.0,
-                dt=1e-5, integrator="rk2", data=None, cell_order="morton"):
-    """Solver + Particles of the 2D elliptical drop."""
+                dt=1e-5, integrator="rk2", data=None, cell_order="morton", density="summation",
+                xsph_eps=None):
+    """Solver + Particles of the 2D elliptical drop.  density="continuity"
+    integrates rho with SPHDensityRate instead of SPHRho; xsph_eps adds the
+    XSPH correction to the position stepping."""
     d = data if data is not None else elliptical_drop(dx=dx, radius=radius, rho0=rho0, c0=c0,
                                                       vel=vel)
     fluid = base.get_particle_array(name="fluid", type=base.Fluid, **d)
@@ -25,8 +28,12 @@ def drop_solver(dx=0.025, radius=1.0, vel=100.0, c0=1400.0, rho0=1.0, alpha=0.1,
              "euler": solver_api.EulerIntegrator}[integrator]
     s = solver_api.Solver(dim=2, integrator_type=integ)
     F = base.Fluid
-    s.add_operation(solver_api.SPHOperation(sph.SPHRho.withargs(), on_types=[F], from_types=[F],
-                                            updates=['rho'], id='density'))
+    if density == "continuity":
+        s.add_operation(solver_api.SPHIntegration(sph.SPHDensityRate.withargs(), on_types=[F],
+                                                  from_types=[F], updates=['rho'], id='density'))
+    else:
+        s.add_operation(solver_api.SPHOperation(sph.SPHRho.withargs(), on_types=[F],
+                                                from_types=[F], updates=['rho'], id='density'))
     s.add_operation(solver_api.SPHOperation(
         sph.TaitEquation.withargs(co=c0, ro=rho0, gamma=7.0), on_types=[F], updates=['p', 'cs'],
         id='eos'))
@@ -34,6 +41,8 @@ def drop_solver(dx=0.025, radius=1.0, vel=100.0, c0=1400.0, rho0=1.0, alpha=0.1,
         sph.MomentumEquation.withargs(alpha=alpha, beta=beta), on_types=[F], from_types=[F],
         updates=['u', 'v'], id='mom'))
     s.add_operation_step([F])
+    if xsph_eps is not None:
+        s.add_operation_xsph(eps=xsph_eps)
     s.set_time_step(dt)
     s.setup(particles)
     return s, particles, d

@@ -117,16 +117,18 @@ class Integrator:
         self.cstep += 1
 
     def get_max_acceleration(self, array, solver):
-        import torch
+        """Largest |a| of the stage-1 velocity accelerations (integrator.py:369-412;
+        on the first step the reference returns solver.dt)."""
+        from .time_step_functions import extrema
         if solver.count == 1:
             return solver.dt
         if array not in self.arrays:
             raise RuntimeError(f"Array {array.name} does not belong to me ")
         acc = float('-inf')
         if array.get_carray('_a_u_1') is not None:
-            comps = [array.get_carray(f'_a_{c}_1') for c in ('u', 'v', 'w')[:solver.dim]]
-            mag = torch.sqrt(sum(c * c for c in comps))
-            acc = max(acc, float(mag.max().item()))
+            comps = tuple(f'_a_{c}_1' for c in ('u', 'v', 'w')[:solver.dim])
+            mag, _, _ = extrema(array, None, comps)
+            acc = max(acc, mag)
         return acc
 
     def integrate(self, dt):

@@ -2,7 +2,7 @@
 import logging
 
 from ..base.kernels import CubicSplineKernel
-from ..sph.funcs import PositionStepping
+from ..sph.funcs import PositionStepping, XSPHCorrection
 from .integrator import EulerIntegrator
 from .sph_equation import SPHIntegration
 from .time_step_functions import TimeStep
@@ -54,6 +54,19 @@ class Solver:
         updates = ['x', 'y', 'z'][:self.dim]
         self.add_operation(SPHIntegration(PositionStepping, on_types=types, updates=updates,
                                           id='step', kernel=None))
+
+    def add_operation_xsph(self, eps, hks=False):
+        """XSPH stepping on the arrays the position stepping moves, added to
+        the same accelerations (solver.py:186-220)."""
+        assert eps > 0, 'Invalid value for XSPH epsilon: %f' % (eps)
+        self.eps = eps
+        err = "position stepping function does not exist!"
+        assert 'step' in self.operation_dict, err
+        types = self.operation_dict['step'].on_types
+        updates = self.operation_dict['step'].updates
+        self.add_operation(SPHIntegration(
+            XSPHCorrection.withargs(eps=eps, hks=hks), from_types=types, on_types=types,
+            updates=updates, id='xsph', kernel=self.default_kernel))
 
     def add_operation(self, operation, before=False, id=None):
         """solver.py:222-282"""

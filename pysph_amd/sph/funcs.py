@@ -5,6 +5,8 @@ TaitEquation      sph/funcs/eos_funcs.pyx:62-129
 MomentumEquation  sph/funcs/pressure_funcs.pyx:174-391 (hks=False, deltap=None)
 GravityForce      sph/funcs/external_force.pyx:13-44
 PositionStepping  sph/funcs/position_funcs.pyx:10-43
+SPHDensityRate    sph/funcs/density_funcs.pyx:85-164 (hks=False)
+XSPHCorrection    sph/funcs/xsph_funcs.pyx:9-91 (hks=False)
 
 The neighbour functions read the sorted shadows of their arrays (see
 base/nnps.py) and the sorted-space lists of their locator; results land in
@@ -226,3 +228,84 @@ class PositionStepping(SPHFunction):
             ctx.handle, self.dest.num_real_particles, self.dest.get_number_of_particles(),
             p(self.d('u')), p(self.d('v')), p(self.d('w')), p(output1), p(output2), p(output3)))
         self.dest.touch(*[n for n in (names or ()) if n])
+
+
+class SPHDensityRate(SPHFunctionParticle):
+    """Continuity equation: d(rho_a)/dt = sum_b m_b (v_a - v_b) . grad_a W_ab
+    (density_funcs.pyx:116-164)."""
+
+    writes = (True, False, False)
+    DST = ('x', 'y', 'z', 'h', 'u', 'v', 'w')
+    SRC = ('x', 'y', 'z', 'h', 'm', 'u', 'v', 'w')
+
+    def __init__(self, source, dest, setup_arrays=True, **kwargs):
+        if kwargs.get('hks'):
+            raise NotImplementedError("hks is not part of this path")
+        SPHFunctionParticle.__init__(self, source, dest, setup_arrays=True, **kwargs)
+        self.id = 'densityrate'
+        self.tag = "density"
+        self.num_outputs = 1
+
+    def set_src_dst_reads(self):
+        self.src_reads = list(self.SRC)
+        self.dst_reads = list(self.DST) + ['tag']
+
+    def eval(self, kernel, output1, output2, output3, accumulate=True, names=None):
+        ctx = self._ctx()
+        loc = self.nbr_locator
+        cm = loc.cell_manager
+        cnt, soff, idx = loc.get_sorted_lists()
+        drec = cm.shadows(self.dest, self.DST)
+        srec = cm.shadows(self.source, self.SRC)
+        p = _lib.ptr
+        profiling.count_pairs(loc.nnz)
+        fn = (ctx.lib.sph_density_rate_f32 if cm.precision == "single"
+              else ctx.lib.sph_density_rate)
+        with profiling.region("density_rate"):
+            ctx.check(fn(ctx.handle, kernel.dim, self.dest.get_number_of_particles(),
+                         self.dest.num_real_particles, p(cm.bins[self.dest.name].perm), p(cnt),
+                         p(soff), p(idx), p(drec), p(srec), int(cm.uniform_h()),
+                         int(bool(accumulate)), p(output1)))
+        for name in (names or ()):
+            if name:
+                cm.written(self.dest, name, False)
+
+
+class XSPHCorrection(CSPHFunctionParticle):
+    """XSPH velocity correction added to the position stepping:
+    dx_a/dt += eps sum_b m_b W_ab / rho_ab (v_b - v_a) (xsph_funcs.pyx:36-91)."""
+
+    writes = (True, True, True)
+    DST = ('x', 'y', 'z', 'h', 'rho', 'u', 'v', 'w')
+    SRC = ('x', 'y', 'z', 'h', 'm', 'rho', 'u', 'v', 'w')
+
+    def __init__(self, source, dest, setup_arrays=True, eps=0.5, **kwargs):
+        if kwargs.get('hks'):
+            raise NotImplementedError("hks is not part of this path")
+        CSPHFunctionParticle.__init__(self, source, dest, setup_arrays, **kwargs)
+        self.eps = float(eps)
+        self.id = 'xsph'
+        self.tag = "position"
+
+    def set_src_dst_reads(self):
+        self.src_reads = list(self.SRC)
+        self.dst_reads = list(self.DST)
+
+    def eval(self, kernel, output1, output2, output3, accumulate=True, names=None):
+        ctx = self._ctx()
+        loc = self.nbr_locator
+        cm = loc.cell_manager
+        cnt, soff, idx = loc.get_sorted_lists()
+        drec = cm.shadows(self.dest, self.DST)
+        srec = cm.shadows(self.source, self.SRC)
+        p = _lib.ptr
+        profiling.count_pairs(loc.nnz)
+        fn = ctx.lib.sph_xsph_f32 if cm.precision == "single" else ctx.lib.sph_xsph
+        with profiling.region("xsph"):
+            ctx.check(fn(ctx.handle, kernel.dim, self.dest.get_number_of_particles(),
+                         self.dest.num_real_particles, p(cm.bins[self.dest.name].perm), p(cnt),
+                         p(soff), p(idx), p(drec), p(srec), int(cm.uniform_h()), self.eps,
+                         int(bool(accumulate)), p(output1), p(output2), p(output3)))
+        for name in (names or ()):
+            if name:
+                cm.written(self.dest, name, False)

@@ -362,3 +362,102 @@ def test_elliptical_drop_run_invariants():
     assert ax_num < 0.95 and ay_num > 1.05
     assert abs(ax_num * ay_num - 1.0) < 0.05
     assert abs(ax_num - a) < 0.03 * a
+
+
+# ---------------------------------------------------------------------------
+# SPHDensityRate and XSPHCorrection (the operators beside the path)
+# ---------------------------------------------------------------------------
+
+def _rate(d, dim=3):
+    cs = cpu_ref.compute_cell_size(d["h"].max(), 2.0)
+    off, idx = cpu_ref.nnps(d, d, 2.0, cs, True)
+    out = np.zeros(len(d["x"]))
+    cpu_ref.density_rate(dim, d, d, off, idx, out)
+    return out
+
+
+def _xsph(d, eps, dim=3):
+    cs = cpu_ref.compute_cell_size(d["h"].max(), 2.0)
+    off, idx = cpu_ref.nnps(d, d, 2.0, cs, True)
+    n = len(d["x"])
+    o = [np.zeros(n) for _ in range(3)]
+    cpu_ref.xsph(dim, d, d, off, idx, eps, *o)
+    return o
+
+
+def test_density_rate_physics():
+    """Rigid translation: rate 0 exactly.  Uniform expansion v = c x: the
+    interior rate is -rho0 c dim (the continuity equation d rho/dt = -rho
+    div v); a sign error, a swapped velocity difference or a dropped m_b
+    fails this."""
+    d = _lattice(20)
+    n = len(d["x"])
+    d["u"], d["v"], d["w"] = np.full(n, 1.5), np.full(n, -0.5), np.full(n, 2.0)
+    assert np.all(_rate(d) == 0.0)
+    c = 3.0
+    d["u"], d["v"], d["w"] = c * d["x"], c * d["y"], c * d["z"]
+    r = _rate(d)
+    inner = np.all([np.abs(d[k] - 0.5) < 0.25 for k in "xyz"], axis=0)
+    assert np.allclose(r[inner], -1000.0 * c * 3, rtol=0.01)
+    # shear v = (c y, 0, 0): divergence-free, interior rate ~ 0
+    d["u"], d["v"], d["w"] = c * d["y"], np.zeros(n), np.zeros(n)
+    assert np.max(np.abs(_rate(d)[inner])) <= 1e-6 * 1000.0 * c
+
+
+def test_density_rate_two_particles():
+    """rate_a = m_b (v_a - v_b) . gradW_a(r_ab, h_ab) with the kernel pinned
+    above; both factors flip sign for the other particle, so m_a rate_a =
+    m_b rate_b (a swapped velocity difference or mass breaks this)."""
+    d = {"x": np.array([0.0, 0.7]), "y": np.array([0.0, 0.4]), "z": np.zeros(2),
+         "h": np.full(2, 0.6), "m": np.array([1.0, 2.5]), "u": np.array([1.0, -0.5]),
+         "v": np.array([0.25, 0.75]), "w": np.zeros(2)}
+    r = _rate(d, dim=2)
+    _, g = cpu_ref.cubic_spline(2, [d["x"][[0]], d["y"][[0]], d["z"][[0]]],
+                                [d["x"][[1]], d["y"][[1]], d["z"][[1]]], [0.6])
+    dv = np.array([d["u"][0] - d["u"][1], d["v"][0] - d["v"][1], 0.0])
+    want0 = float(dv @ g[:, 0]) * d["m"][1]
+    assert math.isclose(r[0], want0, rel_tol=1e-14)
+    assert math.isclose(d["m"][0] * r[0], d["m"][1] * r[1], rel_tol=1e-14)
+
+
+def test_xsph_closed_forms():
+    """Uniform velocity: no correction.  Two particles: eps m_b W / rho_ab
+    (v_b - v_a), rho_ab the mean density; linear in eps."""
+    d = _lattice(10)
+    n = len(d["x"])
+    d["u"], d["v"], d["w"] = np.full(n, 1.0), np.full(n, 2.0), np.full(n, -3.0)
+    assert all(np.all(o == 0.0) for o in _xsph(d, 0.5))
+    p = {"x": np.array([0.0, 0.5]), "y": np.zeros(2), "z": np.zeros(2), "h": np.full(2, 0.4),
+         "m": np.array([1.0, 3.0]), "rho": np.array([1.0, 1.5]), "u": np.array([2.0, -1.0]),
+         "v": np.array([0.0, 1.0]), "w": np.zeros(2)}
+    ox, oy, _ = _xsph(p, 0.5, dim=2)
+    w, _ = cpu_ref.cubic_spline(2, [p["x"][[0]], p["y"][[0]], p["z"][[0]]],
+                                [p["x"][[1]], p["y"][[1]], p["z"][[1]]], [0.4])
+    temp = 3.0 * w[0] / 1.25
+    assert math.isclose(ox[0], temp * (-3.0) * 0.5, rel_tol=1e-14)
+    assert math.isclose(oy[0], temp * 1.0 * 0.5, rel_tol=1e-14)
+    ox2, oy2, _ = _xsph(p, 0.25, dim=2)
+    assert np.allclose(ox2, 0.5 * ox, rtol=1e-15) and np.allclose(oy2, 0.5 * oy, rtol=1e-15)
+
+
+def test_continuity_run_tracks_summation():
+    """2D drop, RK2, 20 steps: the density integrated by the continuity
+    equation stays within 1 % of the summation density of the same run."""
+    from pysph_amd.solver.generators import elliptical_drop
+    d = elliptical_drop(dx=0.05)
+    sims = {}
+    for mode in ("summation", "continuity"):
+        dd = {k: np.array(v, copy=True) for k, v in d.items()}
+        if mode == "continuity":
+            # start from the summation density of the initial lattice
+            ref0 = cpu_ref.RefSimulation([cpu_ref.RefArray("f", 0, dd)], dim=2, co=1400.0,
+                                         ro=1.0, alpha=0.1, beta=0.0)
+            ref0.eval({"f": {k: np.zeros(len(dd["x"])) for k in ref0.stepped}})
+            dd["rho"] = ref0.arrays[0]["rho"].copy()
+        sim = cpu_ref.RefSimulation([cpu_ref.RefArray("f", 0, dd)], dim=2, co=1400.0, ro=1.0,
+                                    alpha=0.1, beta=0.0, density=mode, xsph_eps=0.5)
+        sim.run(1e-5, 20)
+        sims[mode] = sim.arrays[0]
+    a, b = sims["continuity"]["rho"], sims["summation"]["rho"]
+    inner = np.hypot(sims["summation"]["x"], sims["summation"]["y"]) < 0.7
+    assert np.max(np.abs(a[inner] - b[inner]) / b[inner]) < 0.01

@@ -146,6 +146,13 @@ typedef struct sph_prop_list_f32 {
 /* dst[k][t * dst_stride] = (float)src[k][perm[t]] */
 int sph_gather_sorted_f32(sph_ctx *ctx, int64_t n, const int32_t *perm,
                           const sph_prop_list_f32 *props);
+/* The same records written from the original order with rank (original ->
+ * sorted): dst[k][rank[i] * dst_stride] = src[k][i] (coalesced reads; the
+ * faster direction when many fields are refreshed).  Same results as the
+ * gathers. */
+int sph_scatter_sorted(sph_ctx *ctx, int64_t n, const int32_t *rank, const sph_prop_list *props);
+int sph_scatter_sorted_f32(sph_ctx *ctx, int64_t n, const int32_t *rank,
+                           const sph_prop_list_f32 *props);
 
 /* Neighbour lists of FixedDestNbrParticleLocator (base/nnps.pyx:458-621)
  * in SORTED space, stored as sliced ELLPACK with 64-row slices (SELL-64):
@@ -325,6 +332,10 @@ int sph_pc_final(sph_ctx *ctx, int64_t n, const sph_step_props *props);
 int sph_copy(sph_ctx *ctx, int64_t n, const double *src, double *dst);
 /* fill n doubles with value (reset_accelerations / set_to_zero) */
 int sph_fill(sph_ctx *ctx, int64_t n, double value, double *dst);
+/* the same for several arrays in one launch: dst[k][i] = src[k][i], or value
+ * (src and dst_stride unused by the fill; the arrays are contiguous) */
+int sph_copy_many(sph_ctx *ctx, int64_t n, const sph_prop_list *props);
+int sph_fill_many(sph_ctx *ctx, int64_t n, double value, const sph_prop_list *props);
 
 /* ---- batched simulations ------------------------------------------------ */
 

@@ -214,9 +214,11 @@ class CellManager:
                     raise TypeError(f"property {p} of {array.name} must be float64")
                 pl.src[k] = _lib.ptr(src)
                 pl.dst[k] = b.field_ptr(p)
-            gather = ctx.lib.sph_gather_sorted_f32 if single else ctx.lib.sph_gather_sorted
+            # original order -> sorted rows by rank: coalesced reads of every
+            # field (the gather by perm reads one cache line per lane and field)
+            scatter = ctx.lib.sph_scatter_sorted_f32 if single else ctx.lib.sph_scatter_sorted
             with profiling.region("gather_sorted"):
-                ctx.check(gather(ctx.handle, b.n, _lib.ptr(b.perm), ctypes.byref(pl)))
+                ctx.check(scatter(ctx.handle, b.n, _lib.ptr(b.rank), ctypes.byref(pl)))
             for p in stale:
                 b.shadow_ver[p] = array.version(p)
         return b.rec

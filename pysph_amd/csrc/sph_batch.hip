@@ -719,7 +719,8 @@ int batch_lists(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w)
     hipLaunchKernelGGL(k_batch_check_cells, dim3(1), dim3(64), 0, ctx->stream, ncell_dev,
                        d->max_cells, d->status);
     SPH_LAUNCH_CHECK(ctx, "k_batch_check_cells");
-    // -- sorted records of the fields the operators read
+    // -- sorted records of the fields the operators read (written from the
+    // original order into row rank[i]: coalesced reads)
     // in 2D the z and w fields of the records stay zero (the workspace
     // record area is cleared by sph_batch_init_workspace)
     const double *src[8] = {d->x, d->y, d->m, d->u, d->v, d->h, d->z, d->w};
@@ -735,7 +736,7 @@ int batch_lists(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w)
             pl.src[k] = src[k];
             pl.dst[k] = rec32 + fld[k];
         }
-        rc = sph_gather_sorted_f32(ctx, n, w.perm, &pl);
+        rc = sph_scatter_sorted_f32(ctx, n, w.rank, &pl);
     } else {
         sph_prop_list pl;
         memset(&pl, 0, sizeof(pl));
@@ -745,7 +746,7 @@ int batch_lists(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w)
             pl.src[k] = src[k];
             pl.dst[k] = w.rec + fld[k];
         }
-        rc = sph_gather_sorted(ctx, n, w.perm, &pl);
+        rc = sph_scatter_sorted(ctx, n, w.rank, &pl);
     }
     if (rc) return rc;
     // -- neighbour lists

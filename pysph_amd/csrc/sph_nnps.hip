@@ -466,6 +466,58 @@ __global__ void __launch_bounds__(NN_THREADS)
     for (int k = 0; k < pl.nprops; ++k) pl.dst[k][o] = (float)pl.src[k][i];
 }
 
+// the same records written from the original order: thread i reads field k
+// of particle i (coalesced) and writes it into row rank[i].  A row is 96
+// contiguous bytes, so the scattered stores of one particle fill its row in
+// L2, while the gather above reads each field with one cache line per lane.
+__global__ void __launch_bounds__(NN_THREADS)
+    k_scatter_sorted(int64_t n, const int32_t *__restrict__ rank, sph_prop_list pl)
+{
+    int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
+    if (i >= n) return;
+    const int64_t o = (int64_t)rank[i] * pl.dst_stride;
+    for (int k = 0; k < pl.nprops; ++k) pl.dst[k][o] = pl.src[k][i];
+}
+
+__global__ void __launch_bounds__(NN_THREADS)
+    k_scatter_sorted_f32(int64_t n, const int32_t *__restrict__ rank, sph_prop_list_f32 pl)
+{
+    int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
+    if (i >= n) return;
+    const int64_t o = (int64_t)rank[i] * pl.dst_stride;
+    for (int k = 0; k < pl.nprops; ++k) pl.dst[k][o] = (float)pl.src[k][i];
+}
+
+template <typename PL, typename K>
+static int scatter_sorted(sph_ctx *ctx, const char *name, K kern, int64_t n, const int32_t *rank,
+                          const PL *props)
+{
+    if (!ctx) return SPH_EINVAL;
+    SPH_REQUIRE(ctx, props && props->nprops >= 0 && props->nprops <= SPH_MAX_PROPS &&
+                         props->dst_stride >= 1,
+                "%s: bad props", name);
+    if (n <= 0 || props->nprops == 0) return SPH_OK;
+    SPH_REQUIRE(ctx, rank, "%s: null rank", name);
+    for (int k = 0; k < props->nprops; ++k)
+        SPH_REQUIRE(ctx, props->src[k] && props->dst[k], "%s: null prop %d", name, k);
+    hipLaunchKernelGGL(kern, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0, ctx->stream,
+                       n, rank, *props);
+    SPH_LAUNCH_CHECK(ctx, name);
+    return SPH_OK;
+}
+
+extern "C" int sph_scatter_sorted(sph_ctx *ctx, int64_t n, const int32_t *rank,
+                                  const sph_prop_list *props)
+{
+    return scatter_sorted(ctx, "sph_scatter_sorted", k_scatter_sorted, n, rank, props);
+}
+
+extern "C" int sph_scatter_sorted_f32(sph_ctx *ctx, int64_t n, const int32_t *rank,
+                                      const sph_prop_list_f32 *props)
+{
+    return scatter_sorted(ctx, "sph_scatter_sorted_f32", k_scatter_sorted_f32, n, rank, props);
+}
+
 extern "C" int sph_gather_sorted_f32(sph_ctx *ctx, int64_t n, const int32_t *perm,
                                      const sph_prop_list_f32 *props)
 {
@@ -828,7 +880,11 @@ __global__ void __launch_bounds__(CW)
             nnps_cell<true, T>(a, a.ovf_list[k], counts, slice_off, indices, seg_start, seg_end,
                                dsts);
     } else {
-        const int64_t ncl = a.ncell_dev ? (int64_t)*a.ncell_dev : a.ncell_list;
+        // cells read on the device are bounded by the launch size the
+        // scratch was carved for (more cells than that is a status error of
+        // the caller, whose lists are then incomplete, never written past)
+        int64_t ncl = a.ncell_list;
+        if (a.ncell_dev && (int64_t)*a.ncell_dev < ncl) ncl = *a.ncell_dev;
         const unsigned G = gridDim.x;
         for (int64_t base = 0; base < ncl; base += G) {
             const int64_t w = base + sph_xcd_tile(blockIdx.x, G);

@@ -565,3 +565,50 @@ extern "C" int sph_fill(sph_ctx *ctx, int64_t n, double value, double *dst)
     SPH_LAUNCH_CHECK(ctx, "k_fill");
     return SPH_OK;
 }
+
+// several copies / fills of n doubles in one launch (save_initial_arrays and
+// reset_accelerations of an array's stepped properties)
+__global__ void __launch_bounds__(OP_THREADS) k_copy_many(int64_t n, sph_prop_list pl)
+{
+    const int64_t i = (int64_t)blockIdx.x * OP_THREADS + threadIdx.x;
+    if (i >= n) return;
+    for (int k = 0; k < pl.nprops; ++k) pl.dst[k][i] = pl.src[k][i];
+}
+
+__global__ void __launch_bounds__(OP_THREADS) k_fill_many(int64_t n, double v, sph_prop_list pl)
+{
+    const int64_t i = (int64_t)blockIdx.x * OP_THREADS + threadIdx.x;
+    if (i >= n) return;
+    for (int k = 0; k < pl.nprops; ++k) pl.dst[k][i] = v;
+}
+
+static int check_many(sph_ctx *ctx, const char *name, const sph_prop_list *pl, int need_src)
+{
+    SPH_REQUIRE(ctx, pl && pl->nprops >= 0 && pl->nprops <= SPH_MAX_PROPS, "%s: bad props",
+                name);
+    for (int k = 0; k < pl->nprops; ++k)
+        SPH_REQUIRE(ctx, pl->dst[k] && (!need_src || pl->src[k]), "%s: null prop %d", name, k);
+    return SPH_OK;
+}
+
+extern "C" int sph_copy_many(sph_ctx *ctx, int64_t n, const sph_prop_list *props)
+{
+    if (!ctx) return SPH_EINVAL;
+    int rc = check_many(ctx, "sph_copy_many", props, 1);
+    if (rc || n <= 0 || props->nprops == 0) return rc;
+    hipLaunchKernelGGL(k_copy_many, dim3(sph_blocks(n, OP_THREADS)), dim3(OP_THREADS), 0,
+                       ctx->stream, n, *props);
+    SPH_LAUNCH_CHECK(ctx, "k_copy_many");
+    return SPH_OK;
+}
+
+extern "C" int sph_fill_many(sph_ctx *ctx, int64_t n, double value, const sph_prop_list *props)
+{
+    if (!ctx) return SPH_EINVAL;
+    int rc = check_many(ctx, "sph_fill_many", props, 0);
+    if (rc || n <= 0 || props->nprops == 0) return rc;
+    hipLaunchKernelGGL(k_fill_many, dim3(sph_blocks(n, OP_THREADS)), dim3(OP_THREADS), 0,
+                       ctx->stream, n, value, *props);
+    SPH_LAUNCH_CHECK(ctx, "k_fill_many");
+    return SPH_OK;
+}

@@ -56,22 +56,42 @@ class Integrator:
         return _lib.context(self.arrays[0].device if self.arrays else None)
 
     def reset_accelerations(self, step):
+        """Zero the acceleration arrays of stage `step`, one launch per array
+        (integrator.py:287-308)."""
         ctx = self._ctx()
         for array in self.arrays:
             n = array.get_number_of_particles()
-            for prop in self.step_props[array.name][step]:
-                name = self.step_props[array.name][step][prop][1]
-                ctx.check(ctx.lib.sph_fill(ctx.handle, n, 0.0, _lib.ptr(array.get_carray(name))))
-                array.touch(name)
+            names = [self.step_props[array.name][step][prop][1]
+                     for prop in self.step_props[array.name][step]]
+            for c in range(0, len(names), _lib.MAX_PROPS):
+                pl = _lib.PropList()
+                part = names[c:c + _lib.MAX_PROPS]
+                pl.nprops = len(part)
+                pl.dst_stride = 1
+                for k, name in enumerate(part):
+                    pl.dst[k] = _lib.ptr(array.get_carray(name))
+                ctx.check(ctx.lib.sph_fill_many(ctx.handle, n, 0.0, ctypes.byref(pl)))
+            if names:
+                array.touch(*names)
 
     def save_initial_arrays(self):
+        """x0 <- x for the stepped properties, one launch per array
+        (integrator.py:262-285)."""
         ctx = self._ctx()
         for array in self.arrays:
             n = array.get_number_of_particles()
-            for src, dst in self.initial_properties[array.name].items():
-                ctx.check(ctx.lib.sph_copy(ctx.handle, n, _lib.ptr(array.get_carray(src)),
-                                           _lib.ptr(array.get_carray(dst))))
-                array.touch(dst)
+            pairs = list(self.initial_properties[array.name].items())
+            for c in range(0, len(pairs), _lib.MAX_PROPS):
+                pl = _lib.PropList()
+                part = pairs[c:c + _lib.MAX_PROPS]
+                pl.nprops = len(part)
+                pl.dst_stride = 1
+                for k, (src, dst) in enumerate(part):
+                    pl.src[k] = _lib.ptr(array.get_carray(src))
+                    pl.dst[k] = _lib.ptr(array.get_carray(dst))
+                ctx.check(ctx.lib.sph_copy_many(ctx.handle, n, ctypes.byref(pl)))
+            if pairs:
+                array.touch(*[dst for _, dst in pairs])
 
     def eval(self):
         k_num = self.cstep

@@ -106,6 +106,11 @@ int sph_grid_build(sph_ctx *ctx, int narrays, const double *const *xs, const dou
 int sph_bin_array(sph_ctx *ctx, const sph_grid *grid, const double *x, const double *y,
                   const double *z, int64_t n, int32_t *cell_start, int32_t *perm, int32_t *rank,
                   int32_t *cell_of, int32_t *cells, int64_t *ncells_out);
+/* The same without a host synchronisation: the number of non-empty cells is
+ * written to the device word *ncells_dev (for sph_nnps_build_dev). */
+int sph_bin_array_dev(sph_ctx *ctx, const sph_grid *grid, const double *x, const double *y,
+                      const double *z, int64_t n, int32_t *cell_start, int32_t *perm,
+                      int32_t *rank, int32_t *cell_of, int32_t *cells, int32_t *ncells_dev);
 
 /* Sorted particle records: SPH_REC doubles per particle in sorted order,
  * the layout every neighbour kernel reads (one 96-byte record per
@@ -184,6 +189,22 @@ int sph_nnps_build(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
                    int32_t *indices, int64_t cap, int64_t *slots_out /* host */,
                    int64_t *nnz_out /* host */);
 
+/* The same build without a host synchronisation: the destination's count of
+ * non-empty cells is read on the device (nd_cells_dev, from
+ * sph_bin_array_dev; max_cells bounds it and sizes the launch), the totals
+ * go to totals[2] = {slots, nnz} (device) and the capacity checks to the
+ * device word *status: bit 0 when cap < slots (no entry written, every row
+ * empty), bit 4 when more than max_cells cells are non-empty (the cells past
+ * the bound get no rows).  The caller reads totals and status later and
+ * must discard the evaluation that used a build with a status bit set. */
+int sph_nnps_build_dev(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
+                       const int32_t *d_perm, const int32_t *d_cell_start, const int32_t *d_cells,
+                       const int32_t *nd_cells_dev, int64_t max_cells, int64_t nd_total,
+                       int64_t nd_real, const int32_t *s_cell_start, const double *s_rec,
+                       int64_t ns, double radius_scale, int same_array, int32_t *counts,
+                       int64_t *slice_off, int32_t *indices, int64_t cap, int64_t *totals,
+                       int32_t *status);
+
 /* sph_nnps_build over float records (d_rec, s_rec with SPH_REC floats per
  * particle): the distance test d2 < (h*radius_scale)^2 is evaluated in
  * float.  d_x, d_y, d_z are the destination's double coordinates in
@@ -197,6 +218,15 @@ int sph_nnps_build_f32(sph_ctx *ctx, const sph_grid *grid, const float *d_rec,
                        int64_t ns, double radius_scale, int same_array, int32_t *counts,
                        int64_t *slice_off, int32_t *indices, int64_t cap, int64_t *slots_out,
                        int64_t *nnz_out);
+int sph_nnps_build_f32_dev(sph_ctx *ctx, const sph_grid *grid, const float *d_rec,
+                           const double *d_x, const double *d_y, const double *d_z,
+                           const int32_t *d_perm, const int32_t *d_cell_start,
+                           const int32_t *d_cells, const int32_t *nd_cells_dev,
+                           int64_t max_cells, int64_t nd_total, int64_t nd_real,
+                           const int32_t *s_cell_start, const float *s_rec, int64_t ns,
+                           double radius_scale, int same_array, int32_t *counts,
+                           int64_t *slice_off, int32_t *indices, int64_t cap, int64_t *totals,
+                           int32_t *status);
 
 /* The same lists in ORIGINAL index space as CSR, the layout of the
  * reference's per-destination LongArrays (nnps.pyx:596-621): row i

@@ -55,8 +55,27 @@ class _Bins:
         self.cell_of = torch.empty(max(n, 1), dtype=torch.int32, device=device)
         self.cells = torch.empty(max(min(n, ncells), 1), dtype=torch.int32, device=device)
         self.ncells_nonempty = 0
+        # after the first binning the count of non-empty cells stays on the
+        # device (ncells_dev); list builds are launched for cells_bound cells
+        # and the host learns the count one binning later (pending)
+        self.ncells_dev = None
+        self.cells_bound = 0
+        self.pending = None
         self.rec = torch.zeros((max(n, 1), REC), dtype=dtype, device=device)
         self.shadow_ver = {}
+
+    def poll(self):
+        """Take the device cell count of the last binning if it has arrived,
+        and keep the launch bound at twice it."""
+        if self.pending is None:
+            return
+        ev, host = self.pending
+        if not ev.query():
+            return
+        self.pending = None
+        self.ncells_nonempty = int(host[0])
+        if self.ncells_nonempty > self.cells_bound // 2:
+            self.cells_bound = min(max(self.n, 1), 2 * self.ncells_nonempty + 4096)
 
     def field_ptr(self, prop):
         return self.rec.data_ptr() + self.rec.element_size() * REC_FIELDS[prop]
@@ -166,14 +185,26 @@ class CellManager:
             if b is None or b.n != np_ or b.cell_start.numel() != grid.ncells + 1:
                 b = _Bins(np_, grid.ncells, a.device, self.rec_dtype)
                 self.bins[a.name] = b
-            nne = ctypes.c_int64(0)
-            ctx.check(lib.sph_bin_array(ctx.handle, ctypes.byref(grid),
-                                        _lib.ptr(a.get_carray("x")), _lib.ptr(a.get_carray("y")),
-                                        _lib.ptr(a.get_carray("z")), np_,
-                                        _lib.ptr(b.cell_start), _lib.ptr(b.perm),
-                                        _lib.ptr(b.rank), _lib.ptr(b.cell_of), _lib.ptr(b.cells),
-                                        ctypes.byref(nne)))
-            b.ncells_nonempty = int(nne.value)
+            args = (ctx.handle, ctypes.byref(grid), _lib.ptr(a.get_carray("x")),
+                    _lib.ptr(a.get_carray("y")), _lib.ptr(a.get_carray("z")), np_,
+                    _lib.ptr(b.cell_start), _lib.ptr(b.perm), _lib.ptr(b.rank),
+                    _lib.ptr(b.cell_of), _lib.ptr(b.cells))
+            if b.ncells_dev is None:
+                # first binning of these bins: read the count (synchronises)
+                nne = ctypes.c_int64(0)
+                ctx.check(lib.sph_bin_array(*args, ctypes.byref(nne)))
+                b.ncells_nonempty = int(nne.value)
+                b.cells_bound = min(max(np_, 1), 2 * b.ncells_nonempty + 4096)
+                b.ncells_dev = torch.full((1,), b.ncells_nonempty, dtype=torch.int32,
+                                          device=a.device)
+            else:
+                b.poll()
+                ctx.check(lib.sph_bin_array_dev(*args, _lib.ptr(b.ncells_dev)))
+                host = torch.empty(1, dtype=torch.int32, pin_memory=True)
+                host.copy_(b.ncells_dev, non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record()
+                b.pending = (ev, host)
             b.invalidate()
 
     def uniform_h(self):
@@ -283,8 +314,14 @@ class FixedDestNbrParticleLocator:
         self.counts = None
         self.slice_off = None
         self.indices = None
-        self.nnz = 0
+        self._nnz = 0
         self.slots = 0
+        # builds after the first run without a host synchronisation; their
+        # totals and capacity flags arrive later (_pending) and are checked
+        # at the next build or when nnz is read
+        self._totals = None
+        self._pending = []
+        self._status = None
         self.built_version = -1
         self.is_dirty = True
         self.nbuilds = 0
@@ -294,6 +331,50 @@ class FixedDestNbrParticleLocator:
     def update_status(self):
         if not self.is_dirty:
             self.is_dirty = self.source.is_dirty or self.dest.is_dirty
+
+    @property
+    def nnz(self):
+        """Entries of the current lists (waits for a device-side build)."""
+        self._poll(block=True)
+        return self._nnz
+
+    @nnz.setter
+    def nnz(self, v):
+        self._nnz = int(v)
+
+    def pair_count(self):
+        """Entries of the current lists: an int, or a one-element device
+        tensor while a device-side build's totals are still on the way."""
+        if self._pending:
+            return self._totals[1:2]
+        return self._nnz
+
+    def _poll(self, block=False):
+        """Check the device-side builds whose totals have arrived: raise if
+        one of them ran out of list capacity or cell bound (the evaluation
+        that used it is invalid), and grow the list buffer ahead of need."""
+        while self._pending:
+            ev, host, status_host = self._pending[0]
+            if block:
+                ev.synchronize()
+            elif not ev.query():
+                return
+            self._pending.pop(0)
+            slots, nnz = int(host[0]), int(host[1])
+            st = int(status_host[0])
+            self.slots = slots
+            self._nnz = nnz
+            if st:
+                self._status.zero_()
+                why = []
+                if st & 1:
+                    why.append(f"{slots} list slots needed, {self.indices.numel()} allocated")
+                if st & 16:
+                    why.append("more non-empty destination cells than the launch bound")
+                raise _lib.SphError(_lib.SPH_ENEEDCAP,
+                                    f"neighbour lists {self.source.name}->{self.dest.name} "
+                                    f"incomplete ({'; '.join(why)}); the evaluation that used "
+                                    "them is invalid")
 
     def update(self):
         """Rebuild the cached lists if the cell manager re-binned."""
@@ -326,9 +407,46 @@ class FixedDestNbrParticleLocator:
             self.slice_off = torch.empty(nslices + 1, dtype=torch.int64, device=dev)
         if self.indices is None:
             self.indices = torch.empty(max(1, 96 * nd_total), dtype=torch.int32, device=dev)
+        p = _lib.ptr
+        self._poll()
+        if self.nbuilds > 0 and bd.ncells_dev is not None:
+            # device-side build: no synchronisation.  The buffer is grown
+            # ahead of need from the last known size (the lists in it are
+            # replaced by this build anyway).
+            if self.slots > (3 * self.indices.numel()) // 4:
+                self.indices = torch.empty(int(self.slots * 1.5) + 4096, dtype=torch.int32,
+                                           device=dev)
+            if self._status is None:
+                self._status = torch.zeros(1, dtype=torch.int32, device=dev)
+            totals = torch.empty(2, dtype=torch.int64, device=dev)
+            if cm.precision == "single":
+                rc = lib.sph_nnps_build_f32_dev(
+                    ctx.handle, ctypes.byref(cm.grid), p(drec), p(dst.get_carray("x")),
+                    p(dst.get_carray("y")), p(dst.get_carray("z")), p(bd.perm),
+                    p(bd.cell_start), p(bd.cells), p(bd.ncells_dev), bd.cells_bound, nd_total,
+                    nd_real, p(bs.cell_start), p(srec), bs.n, self.radius_scale,
+                    int(src is dst), p(self.counts), p(self.slice_off), p(self.indices),
+                    self.indices.numel(), p(totals), p(self._status))
+            else:
+                rc = lib.sph_nnps_build_dev(
+                    ctx.handle, ctypes.byref(cm.grid), p(drec), p(bd.perm), p(bd.cell_start),
+                    p(bd.cells), p(bd.ncells_dev), bd.cells_bound, nd_total, nd_real,
+                    p(bs.cell_start), p(srec), bs.n, self.radius_scale, int(src is dst),
+                    p(self.counts), p(self.slice_off), p(self.indices), self.indices.numel(),
+                    p(totals), p(self._status))
+            ctx.check(rc)
+            host = torch.empty(2, dtype=torch.int64, pin_memory=True)
+            host.copy_(totals, non_blocking=True)
+            status_host = torch.empty(1, dtype=torch.int32, pin_memory=True)
+            status_host.copy_(self._status, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record()
+            self._totals = totals
+            self._pending.append((ev, host, status_host))
+            self.nbuilds += 1
+            return
         slots = ctypes.c_int64(0)
         nnz = ctypes.c_int64(0)
-        p = _lib.ptr
 
         def call():
             if cm.precision == "single":

@@ -405,13 +405,14 @@ int sph_bin_sort(sph_ctx *ctx, int64_t n, int64_t nc, const int32_t *cell_of, in
     return SPH_OK;
 }
 
-extern "C" int sph_bin_array(sph_ctx *ctx, const sph_grid *grid, const double *x,
-                             const double *y, const double *z, int64_t n, int32_t *cell_start,
-                             int32_t *perm, int32_t *rank, int32_t *cell_of, int32_t *cells,
-                             int64_t *ncells_out)
+// ncells_out (host, synchronising) or ncells_dev (device, stream-ordered)
+static int bin_array(sph_ctx *ctx, const sph_grid *grid, const double *x, const double *y,
+                     const double *z, int64_t n, int32_t *cell_start, int32_t *perm,
+                     int32_t *rank, int32_t *cell_of, int32_t *cells, int64_t *ncells_out,
+                     int32_t *ncells_dev)
 {
     if (!ctx) return SPH_EINVAL;
-    SPH_REQUIRE(ctx, grid && grid->ncells > 0 && cell_start && ncells_out,
+    SPH_REQUIRE(ctx, grid && grid->ncells > 0 && cell_start && (ncells_out || ncells_dev),
                 "sph_bin_array: bad grid / null output");
     SPH_REQUIRE(ctx, n >= 0 && n < ((int64_t)1 << 31), "sph_bin_array: bad n %lld",
                 (long long)n);
@@ -421,11 +422,17 @@ extern "C" int sph_bin_array(sph_ctx *ctx, const sph_grid *grid, const double *x
     if (rc) return rc;
     GridDev g = make_griddev(ctx, grid);
     const int64_t nc = grid->ncells;
-    *ncells_out = 0;
+    if (ncells_out) *ncells_out = 0;
     hipLaunchKernelGGL(k_fill_i32, dim3(sph_blocks(nc + 1, NN_THREADS)), dim3(NN_THREADS), 0,
                        ctx->stream, cell_start, nc + 1, 0);
     SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
-    if (n == 0) return SPH_OK;
+    if (n == 0) {
+        if (ncells_dev) {
+            hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, ncells_dev, 1, 0);
+            SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
+        }
+        return SPH_OK;
+    }
     // scratch: [non-empty flags / slots (nc+1) | unsorted cell members (n) | scan partials];
     // the scan of cell_start runs before either region is written
     const size_t slot_bytes = (sizeof(int32_t) * (size_t)(nc + 1) + 511) & ~(size_t)255;
@@ -443,7 +450,34 @@ extern "C" int sph_bin_array(sph_ctx *ctx, const sph_grid *grid, const double *x
     rc = sph_bin_sort(ctx, n, nc, cell_of, rank, cell_start, perm, cells, slot, tmp, -1,
                       slot_bytes + tmp_bytes);
     if (rc) return rc;
+    if (ncells_dev) {
+        SPH_HIP_CHECK(ctx, hipMemcpyAsync(ncells_dev, slot + nc, sizeof(int32_t),
+                                          hipMemcpyDeviceToDevice, ctx->stream));
+        return SPH_OK;
+    }
     return read_i64(ctx, slot + nc, ncells_out, true);
+}
+
+extern "C" int sph_bin_array(sph_ctx *ctx, const sph_grid *grid, const double *x,
+                             const double *y, const double *z, int64_t n, int32_t *cell_start,
+                             int32_t *perm, int32_t *rank, int32_t *cell_of, int32_t *cells,
+                             int64_t *ncells_out)
+{
+    if (!ctx) return SPH_EINVAL;
+    SPH_REQUIRE(ctx, ncells_out, "sph_bin_array: null ncells_out");
+    return bin_array(ctx, grid, x, y, z, n, cell_start, perm, rank, cell_of, cells, ncells_out,
+                     nullptr);
+}
+
+extern "C" int sph_bin_array_dev(sph_ctx *ctx, const sph_grid *grid, const double *x,
+                                 const double *y, const double *z, int64_t n,
+                                 int32_t *cell_start, int32_t *perm, int32_t *rank,
+                                 int32_t *cell_of, int32_t *cells, int32_t *ncells_dev)
+{
+    if (!ctx) return SPH_EINVAL;
+    SPH_REQUIRE(ctx, ncells_dev, "sph_bin_array_dev: null ncells_dev");
+    return bin_array(ctx, grid, x, y, z, n, cell_start, perm, rank, cell_of, cells, nullptr,
+                     ncells_dev);
 }
 
 __global__ void __launch_bounds__(NN_THREADS)
@@ -1085,6 +1119,37 @@ static CellArgs cell_args_base()
     return a;
 }
 
+// totals of a device-side plain build: totals = [slots, nnz]; status bit 0
+// when the lists need more than cap slots (nothing written, rows empty), bit
+// 4 when there are more non-empty destination cells than max_cells (the
+// cells past that bound get no rows)
+__global__ void k_nnps_dev_totals(const int64_t *__restrict__ slots,
+                                  const int64_t *__restrict__ nnz,
+                                  const int32_t *__restrict__ ncells_dev, int64_t cap,
+                                  int64_t max_cells, int64_t *__restrict__ totals,
+                                  int32_t *__restrict__ status)
+{
+    if (threadIdx.x != 0) return;
+    const int64_t sl = slots ? *slots : 0, nz = nnz ? *nnz : 0;
+    totals[0] = sl;
+    totals[1] = nz;
+    int32_t st = 0;
+    if (sl > cap) st |= 1;
+    if (ncells_dev && *ncells_dev > max_cells) st |= 16;
+    if (st) *status |= st;
+}
+
+// Host mode (dev == NULL): nd_cells is the host count of non-empty cells;
+// the call reads the totals back (synchronises) and returns SPH_ENEEDCAP
+// without writing lists when cap is too small.  Device mode: the cell count
+// is read on the device (nd_cells bounds it), the totals go to dev->totals,
+// the capacity checks to dev->status, and nothing synchronises.
+struct NnpsDevOut {
+    const int32_t *ncells_dev;
+    int64_t *totals;
+    int32_t *status;
+};
+
 static int nnps_build(sph_ctx *ctx, const sph_grid *grid, int f32, const double *d_rec,
                       const float *d_rec32, const double *ox, const double *oy,
                       const double *oz, const int32_t *d_perm, const int32_t *d_cell_start,
@@ -1092,10 +1157,12 @@ static int nnps_build(sph_ctx *ctx, const sph_grid *grid, int f32, const double 
                       int64_t nd_real, const int32_t *s_cell_start, const double *s_rec,
                       const float *s_rec32, int64_t ns, double radius_scale, int same_array,
                       int32_t *counts, int64_t *slice_off, int32_t *indices, int64_t cap,
-                      int64_t *slots_out, int64_t *nnz_out)
+                      int64_t *slots_out, int64_t *nnz_out, const NnpsDevOut *dev)
 {
     if (!ctx) return SPH_EINVAL;
-    SPH_REQUIRE(ctx, grid && counts && slice_off && slots_out && nnz_out,
+    SPH_REQUIRE(ctx, grid && counts && slice_off &&
+                         (dev ? (dev->ncells_dev && dev->totals && dev->status && indices)
+                              : (slots_out && nnz_out)),
                 "sph_nnps_build: null argument");
     SPH_REQUIRE(ctx, nd_total >= 0 && nd_total < ((int64_t)1 << 31) && nd_real >= 0 &&
                          nd_real <= nd_total && nd_cells >= 0 && nd_cells <= nd_total,
@@ -1107,12 +1174,23 @@ static int nnps_build(sph_ctx *ctx, const sph_grid *grid, int f32, const double 
                 "sph_nnps_build: null destination arrays");
     int rc = sph_upload_key_tables(ctx, grid);
     if (rc) return rc;
-    *slots_out = 0;
-    *nnz_out = 0;
+    if (!dev) {
+        *slots_out = 0;
+        *nnz_out = 0;
+    }
     const int64_t nslices = (nd_total + 63) / 64;
     rc = nnps_clear(ctx, nd_total, counts, slice_off);
     if (rc) return rc;
-    if (nd_total == 0 || ns == 0 || nd_real == 0 || nd_cells == 0) return SPH_OK;
+    if (nd_total == 0 || ns == 0 || nd_real == 0 || nd_cells == 0) {
+        if (dev) {
+            hipLaunchKernelGGL(k_nnps_dev_totals, dim3(1), dim3(64), 0, ctx->stream,
+                               (const int64_t *)nullptr, (const int64_t *)nullptr,
+                               nd_total == 0 ? (const int32_t *)nullptr : dev->ncells_dev, cap,
+                               nd_cells, dev->totals, dev->status);
+            SPH_LAUNCH_CHECK(ctx, "k_nnps_dev_totals");
+        }
+        return SPH_OK;
+    }
     CellArgs a = cell_args_base();
     a.g = make_griddev(ctx, grid);
     a.f32 = f32;
@@ -1131,9 +1209,21 @@ static int nnps_build(sph_ctx *ctx, const sph_grid *grid, int f32, const double 
     a.srec = s_rec;
     a.rs = radius_scale;
     a.same = same_array;
+    if (dev) {
+        a.ncell_dev = dev->ncells_dev;
+        a.cap = cap;
+        a.status = dev->status;
+    }
     int64_t *nnz_dev = nullptr;
     rc = nnps_count(ctx, a, nd_cells, nd_total, counts, slice_off, &nnz_dev);
     if (rc) return rc;
+    if (dev) {
+        hipLaunchKernelGGL(k_nnps_dev_totals, dim3(1), dim3(64), 0, ctx->stream,
+                           (const int64_t *)(slice_off + nslices), (const int64_t *)nnz_dev,
+                           dev->ncells_dev, cap, nd_cells, dev->totals, dev->status);
+        SPH_LAUNCH_CHECK(ctx, "k_nnps_dev_totals");
+        return nnps_fill(ctx, a, nd_cells, nd_total, counts, slice_off, indices);
+    }
     rc = sph_ensure_pinned(ctx, 2 * sizeof(int64_t));
     if (rc) return rc;
     SPH_HIP_CHECK(ctx, hipMemcpyAsync(ctx->pinned, slice_off + nslices, sizeof(int64_t),
@@ -1163,7 +1253,23 @@ extern "C" int sph_nnps_build(sph_ctx *ctx, const sph_grid *grid, const double *
     return nnps_build(ctx, grid, 0, d_rec, nullptr, nullptr, nullptr, nullptr, d_perm, d_cell_start,
                       d_cells, nd_cells, nd_total, nd_real, s_cell_start, s_rec, nullptr, ns,
                       radius_scale, same_array, counts, slice_off, indices, cap, slots_out,
-                      nnz_out);
+                      nnz_out, nullptr);
+}
+
+extern "C" int sph_nnps_build_dev(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
+                                  const int32_t *d_perm, const int32_t *d_cell_start,
+                                  const int32_t *d_cells, const int32_t *nd_cells_dev,
+                                  int64_t max_cells, int64_t nd_total, int64_t nd_real,
+                                  const int32_t *s_cell_start, const double *s_rec, int64_t ns,
+                                  double radius_scale, int same_array, int32_t *counts,
+                                  int64_t *slice_off, int32_t *indices, int64_t cap,
+                                  int64_t *totals, int32_t *status)
+{
+    const NnpsDevOut dev = {nd_cells_dev, totals, status};
+    return nnps_build(ctx, grid, 0, d_rec, nullptr, nullptr, nullptr, nullptr, d_perm,
+                      d_cell_start, d_cells, max_cells, nd_total, nd_real, s_cell_start, s_rec,
+                      nullptr, ns, radius_scale, same_array, counts, slice_off, indices, cap,
+                      nullptr, nullptr, &dev);
 }
 
 extern "C" int sph_nnps_build_f32(sph_ctx *ctx, const sph_grid *grid, const float *d_rec,
@@ -1179,7 +1285,24 @@ extern "C" int sph_nnps_build_f32(sph_ctx *ctx, const sph_grid *grid, const floa
     return nnps_build(ctx, grid, 1, nullptr, d_rec, d_x, d_y, d_z,
                       d_perm, d_cell_start, d_cells, nd_cells, nd_total, nd_real, s_cell_start,
                       nullptr, s_rec, ns, radius_scale, same_array, counts, slice_off, indices,
-                      cap, slots_out, nnz_out);
+                      cap, slots_out, nnz_out, nullptr);
+}
+
+extern "C" int sph_nnps_build_f32_dev(sph_ctx *ctx, const sph_grid *grid, const float *d_rec,
+                                      const double *d_x, const double *d_y, const double *d_z,
+                                      const int32_t *d_perm, const int32_t *d_cell_start,
+                                      const int32_t *d_cells, const int32_t *nd_cells_dev,
+                                      int64_t max_cells, int64_t nd_total, int64_t nd_real,
+                                      const int32_t *s_cell_start, const float *s_rec,
+                                      int64_t ns, double radius_scale, int same_array,
+                                      int32_t *counts, int64_t *slice_off, int32_t *indices,
+                                      int64_t cap, int64_t *totals, int32_t *status)
+{
+    const NnpsDevOut dev = {nd_cells_dev, totals, status};
+    return nnps_build(ctx, grid, 1, nullptr, d_rec, d_x, d_y, d_z, d_perm, d_cell_start, d_cells,
+                      max_cells, nd_total, nd_real, s_cell_start, nullptr, s_rec, ns,
+                      radius_scale, same_array, counts, slice_off, indices, cap, nullptr,
+                      nullptr, &dev);
 }
 
 int sph_nnps_build_batch(sph_ctx *ctx, const GridDev *grids, const int32_t *sim_of,

@@ -22,17 +22,30 @@ def enable(on=True):
 def reset():
     global _pairs
     _records.clear()
+    _pending_pairs.clear()
     _pairs = 0
 
 
+_pending_pairs = []
+
+
 def count_pairs(n):
-    """Add n pair interactions (list entries evaluated by an operator)."""
+    """Add n pair interactions (list entries evaluated by an operator): an
+    int, or a one-element device tensor read when pairs() is called."""
     global _pairs
     if _enabled:
-        _pairs += int(n)
+        if isinstance(n, torch.Tensor):
+            _pending_pairs.append(n)
+        else:
+            _pairs += int(n)
 
 
 def pairs():
+    """Total pair interactions counted (synchronises for device counts)."""
+    global _pairs
+    if _pending_pairs:
+        _pairs += int(torch.stack([t.reshape(()) for t in _pending_pairs]).sum().item())
+        _pending_pairs.clear()
     return _pairs
 
 

@@ -57,7 +57,7 @@ class SPHRho(CSPHFunctionParticle):
         srec = cm.shadows(self.source, ('x', 'y', 'z', 'h', 'm'))
         out_s = cm.shadow_out(self.dest, name) if name else None
         p = _lib.ptr
-        profiling.count_pairs(loc.nnz)
+        profiling.count_pairs(loc.pair_count())
         fn = (ctx.lib.sph_summation_density_f32 if cm.precision == "single"
               else ctx.lib.sph_summation_density)
         with profiling.region("summation_density"):
@@ -168,7 +168,7 @@ class MomentumEquation(SPHFunctionParticle):
         drec = cm.shadows(self.dest, self.DST)
         srec = cm.shadows(self.source, self.SRC)
         p = _lib.ptr
-        profiling.count_pairs(loc.nnz)
+        profiling.count_pairs(loc.pair_count())
         fn = ctx.lib.sph_momentum_av_f32 if cm.precision == "single" else ctx.lib.sph_momentum_av
         with profiling.region("momentum_av"):
             ctx.check(fn(
@@ -258,7 +258,7 @@ class SPHDensityRate(SPHFunctionParticle):
         drec = cm.shadows(self.dest, self.DST)
         srec = cm.shadows(self.source, self.SRC)
         p = _lib.ptr
-        profiling.count_pairs(loc.nnz)
+        profiling.count_pairs(loc.pair_count())
         fn = (ctx.lib.sph_density_rate_f32 if cm.precision == "single"
               else ctx.lib.sph_density_rate)
         with profiling.region("density_rate"):
@@ -299,7 +299,7 @@ class XSPHCorrection(CSPHFunctionParticle):
         drec = cm.shadows(self.dest, self.DST)
         srec = cm.shadows(self.source, self.SRC)
         p = _lib.ptr
-        profiling.count_pairs(loc.nnz)
+        profiling.count_pairs(loc.pair_count())
         fn = ctx.lib.sph_xsph_f32 if cm.precision == "single" else ctx.lib.sph_xsph
         with profiling.region("xsph"):
             ctx.check(fn(ctx.handle, kernel.dim, self.dest.get_number_of_particles(),

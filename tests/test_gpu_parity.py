@@ -335,3 +335,32 @@ def test_dambreak_small_pc_vs_oracle(gpu):
     assert worst <= TOL
     # walls never move
     assert np.array_equal(particles.arrays[1].get('x'), wl["x"])
+
+
+def test_device_side_rebuild_matches_and_reports_capacity(gpu):
+    """Builds after the first run without a host synchronisation: the lists
+    equal the synchronous build's, and a build that runs out of list capacity
+    (forced here with a tiny buffer) is reported at the next check instead of
+    being used silently."""
+    base, sph, _ = _api()
+    from pysph_amd import _lib
+    from pysph_amd.solver.generators import jittered_lattice_3d
+    d = jittered_lattice_3d(n=16, seed=9, hdx=1.3)
+    pa = base.get_particle_array(name="fluid", **d)
+    particles = base.Particles([pa])
+    loc = particles.nnps_manager.get_neighbor_particle_locator(pa, pa, 2.0)
+    off0, idx0 = (t.cpu().numpy() for t in loc.get_csr())       # first build: synchronous
+    x = pa.get("x")
+    pa.set(x=x)                                                  # same positions, dirty
+    particles.update()
+    loc.update()
+    assert loc.nbuilds == 2 and loc._pending                     # device-side build
+    off1, idx1 = (t.cpu().numpy() for t in loc.get_csr())
+    assert np.array_equal(off0, off1) and np.array_equal(idx0, idx1)
+    loc.indices = loc.indices[:64]
+    loc.slots = 0                                                # no growth ahead of need
+    pa.set(x=x)
+    particles.update()
+    loc.update()
+    with pytest.raises(_lib.SphError, match="incomplete"):
+        _ = loc.nnz

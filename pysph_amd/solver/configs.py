@@ -16,13 +16,13 @@ from .generators import dam_break_3d, elliptical_drop
 
 def drop_solver(dx=0.025, radius=1.0, vel=100.0, c0=1400.0, rho0=1.0, alpha=0.1, beta=0.0,
                 dt=1e-5, integrator="rk2", data=None, cell_order="morton", density="summation",
-                xsph_eps=None):
+                xsph_eps=None, precision="double"):
     """Solver + Particles of the 2D elliptical drop.  density="continuity"
     integrates rho with SPHDensityRate instead of SPHRho; xsph_eps adds the
     XSPH correction to the position stepping."""
     d = data if data is not None else elliptical_drop(dx=dx, radius=radius, rho0=rho0, c0=c0,
                                                       vel=vel)
-    fluid = base.get_particle_array(name="fluid", type=base.Fluid, **d)
+    fluid = base.get_particle_array(name="fluid", type=base.Fluid, cl_precision=precision, **d)
     particles = base.Particles([fluid], cell_order=cell_order)
     integ = {"rk2": solver_api.RK2Integrator, "pc": solver_api.PredictorCorrectorIntegrator,
              "euler": solver_api.EulerIntegrator}[integrator]

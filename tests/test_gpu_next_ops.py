@@ -101,3 +101,24 @@ def test_time_step_functions(gpu):
     got = ts.compute_time_step(s)
     print(f"viscous dt {want_visc:.6g}, force-based dt {got:.6g}")
     assert got == want
+
+
+def test_continuity_xsph_single_precision_tracks_double(gpu):
+    """cl_precision="single": the float operators (sph_density_rate_f32,
+    sph_xsph_f32) over 5 RK2 steps stay within 1e-4 (max error / max
+    magnitude) of the FP64 run."""
+    from pysph_amd.solver.configs import drop_solver
+    runs = {}
+    for prec in ("double", "single"):
+        s, particles, _ = drop_solver(dx=0.04, density="continuity", xsph_eps=0.5,
+                                      precision=prec)
+        for _ in range(5):
+            s.integrator.integrate(1e-5)
+        runs[prec] = particles.arrays[0]
+    errs = {}
+    for k in ("x", "y", "u", "v", "rho"):
+        a, b = runs["single"].get(k), runs["double"].get(k)
+        errs[k] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    print("continuity+xsph FP32 vs FP64 after 5 steps:",
+          " ".join(f"{k} {v:.2g}" for k, v in errs.items()))
+    assert max(errs.values()) <= 1e-4

@@ -229,6 +229,9 @@ class CellManager:
                 wanted.append(p)
         if any(b.shadow_ver.get(p) != array.version(p) for p in props):
             stale = [p for p in wanted if b.shadow_ver.get(p) != array.version(p)]
+            # in record order, so that the two fields of a record pair are
+            # adjacent in the list and go out as one 16-byte store
+            stale.sort(key=lambda q: REC_FIELDS[q])
         else:
             stale = []
         if stale:

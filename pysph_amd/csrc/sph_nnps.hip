@@ -510,7 +510,18 @@ __global__ void __launch_bounds__(NN_THREADS)
     int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
     if (i >= n) return;
     const int64_t o = (int64_t)rank[i] * pl.dst_stride;
-    for (int k = 0; k < pl.nprops; ++k) pl.dst[k][o] = pl.src[k][i];
+    for (int k = 0; k < pl.nprops; ++k) {
+        double *d = pl.dst[k] + o;
+        // two fields that are adjacent 16-byte-aligned words of the row go
+        // out as one 16-byte store (the test is the same for every lane)
+        if (k + 1 < pl.nprops && pl.dst[k + 1] == pl.dst[k] + 1 &&
+            ((uintptr_t)d & 15) == 0) {
+            *reinterpret_cast<double2 *>(d) = make_double2(pl.src[k][i], pl.src[k + 1][i]);
+            ++k;
+        } else {
+            *d = pl.src[k][i];
+        }
+    }
 }
 
 __global__ void __launch_bounds__(NN_THREADS)
@@ -919,12 +930,21 @@ __global__ void __launch_bounds__(CW)
         // the caller, whose lists are then incomplete, never written past)
         int64_t ncl = a.ncell_list;
         if (a.ncell_dev && (int64_t)*a.ncell_dev < ncl) ncl = *a.ncell_dev;
+        // blocks go round-robin to the XCDs (block b to XCD b % 8): XCD x
+        // walks cells [x*ncl/8, (x+1)*ncl/8) with its blocks, so every XCD
+        // has an equal, contiguous share whatever the grid size (the grid is
+        // sized for a bound when the cell count is read on the device)
         const unsigned G = gridDim.x;
-        for (int64_t base = 0; base < ncl; base += G) {
-            const int64_t w = base + sph_xcd_tile(blockIdx.x, G);
-            if (w >= ncl) break;  // uniform: one wavefront per block
-            nnps_cell<false, T>(a, w, counts, slice_off, indices, seg_start, seg_end, dsts);
+        if (G < SPH_NXCD) {  // too few blocks to give every XCD one
+            for (int64_t w = blockIdx.x; w < ncl; w += G)
+                nnps_cell<false, T>(a, w, counts, slice_off, indices, seg_start, seg_end, dsts);
+            return;
         }
+        const unsigned x = blockIdx.x % SPH_NXCD, k = blockIdx.x / SPH_NXCD;
+        const unsigned per = (G - x + SPH_NXCD - 1) / SPH_NXCD;  // blocks on XCD x (>= 1)
+        const int64_t lo = ncl * x / SPH_NXCD, hi = ncl * (x + 1) / SPH_NXCD;
+        for (int64_t w = lo + k; w < hi; w += per)  // uniform: one wavefront per block
+            nnps_cell<false, T>(a, w, counts, slice_off, indices, seg_start, seg_end, dsts);
     }
 }
 

@@ -10,17 +10,31 @@ from collections import defaultdict
 import torch
 
 _enabled = False
+_only = None          # region names that record events (None: all)
 _records = defaultdict(list)
+_pool = []            # timing events kept for reuse
 _pairs = 0
 
 
-def enable(on=True):
-    global _enabled
+def enable(on=True, regions=None):
+    """Turn recording on or off; ``regions`` limits the event pairs to the
+    named regions (pair counting is on either way), which keeps the
+    instrumentation of a timed loop to the kernels it measures."""
+    global _enabled, _only
     _enabled = bool(on)
+    _only = set(regions) if regions is not None else None
+
+
+def _event():
+    return _pool.pop() if _pool else torch.cuda.Event(enable_timing=True)
 
 
 def reset():
     global _pairs
+    for evs in _records.values():
+        for s, e in evs:
+            _pool.append(s)
+            _pool.append(e)
     _records.clear()
     _pending_pairs.clear()
     _pairs = 0
@@ -51,11 +65,11 @@ def pairs():
 
 @contextlib.contextmanager
 def region(name):
-    if not _enabled:
+    if not _enabled or (_only is not None and name not in _only):
         yield
         return
-    s = torch.cuda.Event(enable_timing=True)
-    e = torch.cuda.Event(enable_timing=True)
+    s = _event()
+    e = _event()
     s.record()
     try:
         yield

@@ -99,13 +99,15 @@ def measured_traffic():
         return None
 
 
-def momentum_bytes(nd, nnz, ns_other):
-    """Algorithmic HBM bytes of one momentum launch: per destination the
-    int64 row offset, 10 destination properties + dt_fac read, ax ay az
-    read-modify-write and dt_fac written (8+11*8+3*16+8 = 152 B), 4 B per
-    list entry, and -- when the source is another array -- its 11
-    properties once (88 B per source particle)."""
-    return nd * 152 + 4 * nnz + 88 * ns_other
+def momentum_bytes(nd, nnz, ns_other, ns):
+    """Algorithmic HBM bytes of one sph_momentum_av call (k_prho2 over the
+    ns source records, then k_momentum): P/rho^2 of each source (p and rho
+    read, P/rho^2 written: 24 B), and per destination the int64 row offset,
+    10 destination properties + dt_fac read, ax ay az read-modify-write and
+    dt_fac written (8+11*8+3*16+8 = 152 B), 4 B per list entry, and -- when
+    the source is another array -- its 11 properties once (88 B per source
+    particle)."""
+    return 24 * ns + nd * 152 + 4 * nnz + 88 * ns_other
 
 
 def run_dambreak(args):
@@ -156,8 +158,8 @@ def run_dambreak(args):
     nnz_fw = mom_calc.funcs[1].nbr_locator.nnz
     # dominant kernel: momentum (two launches per evaluation: fluid, wall source)
     mom = prof.get("momentum_av", {"mean_ms": float("nan"), "count": 0})
-    b_ff = momentum_bytes(nd, nnz_ff, 0)
-    b_fw = momentum_bytes(nd, nnz_fw, nw)
+    b_ff = momentum_bytes(nd, nnz_ff, 0, nd)
+    b_fw = momentum_bytes(nd, nnz_fw, nw, nw)
     bytes_per_launch = (b_ff + b_fw) / 2.0
     achieved = bytes_per_launch / (mom["mean_ms"] * 1e-3) / 1e9
     result = {
@@ -181,7 +183,7 @@ def run_dambreak(args):
                               f"{min(5, args.steps)} steps outside the timed loop",
         },
         "roofline": {
-            "kernel": "k_momentum",
+            "kernel": "k_momentum (sph_momentum_av: k_prho2 pre-pass + k_momentum)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

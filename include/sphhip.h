@@ -120,7 +120,7 @@ int sph_bin_array_dev(sph_ctx *ctx, const sph_grid *grid, const double *x, const
 #define SPH_REC 12
 enum sph_rec_field {
     SPH_R_X = 0, SPH_R_Y, SPH_R_Z, SPH_R_M, SPH_R_U, SPH_R_V, SPH_R_W,
-    SPH_R_PRHO2, /* p * (1/(rho*rho)), filled by sph_momentum_av */
+    SPH_R_PRHO2, /* p * (1/(rho*rho)), filled by sph_momentum_av or sph_tait */
     SPH_R_RHO, SPH_R_CS, SPH_R_H, SPH_R_P
 };
 
@@ -256,17 +256,22 @@ int sph_summation_density(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_re
  *   p (+)= (pow(rho/ro, gamma) - 1) * co*co*ro/gamma,
  *   cs (+)= pow(rho/ro, (gamma-1)/2) * co
  * rho_s[t * s_stride] is the sorted density; p, cs are written in original
- * order and p_s, cs_s (may be NULL) at [t * s_stride]. */
+ * order and p_s, cs_s (may be NULL) at [t * s_stride].  prho2_s (may be
+ * NULL) receives P/rho^2 = p * (1/(rho*rho)) of the final p at the same
+ * stride, the per-source term of MomentumEquation (pressure_funcs.pyx:302-305),
+ * so that sph_momentum_av can skip its own pass over the sources. */
 int sph_tait(sph_ctx *ctx, int64_t n_total, int64_t n_real, const int32_t *perm,
              const double *rho_s, int64_t s_stride, double ro, double co, double gamma,
-             int accumulate, double *p, double *cs, double *p_s, double *cs_s);
+             int accumulate, double *p, double *cs, double *p_s, double *cs_s,
+             double *prho2_s);
 
 /* MomentumEquation with Monaghan artificial viscosity, hks=False, no
  * artificial pressure (sph/funcs/pressure_funcs.pyx:185-391):
  *   a_i (+)= sum_j -m_j (P_i/rho_i^2 + P_j/rho_j^2 + Pi_ij) gradW_ij(h_ij)
  *   dt_fac_i = max(dt_fac_i, |h_ij v_ij.r_ij / (r_ij^2 + eta^2 h_ij^2)|)
  * d_rec/s_rec are sorted records (s_rec has ns records; its SPH_R_PRHO2
- * field is recomputed from its p and rho fields first); ax, ay, az (az may
+ * field is recomputed from its p and rho fields first -- ns = 0 when the
+ * caller knows it is current, e.g. written by sph_tait); ax, ay, az (az may
  * be NULL) and dt_fac are in original order.  uniform_h as for
  * sph_summation_density. */
 int sph_momentum_av(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
@@ -286,7 +291,8 @@ int sph_summation_density_f32(sph_ctx *ctx, int dim, int64_t nd_total, int64_t n
                               int accumulate, double *rho, float *out_s, int64_t out_s_stride);
 int sph_tait_f32(sph_ctx *ctx, int64_t n_total, int64_t n_real, const int32_t *perm,
                  const float *rho_s, int64_t s_stride, double ro, double co, double gamma,
-                 int accumulate, double *p, double *cs, float *p_s, float *cs_s);
+                 int accumulate, double *p, double *cs, float *p_s, float *cs_s,
+                 float *prho2_s);
 int sph_momentum_av_f32(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
                         const int32_t *d_perm, const int32_t *counts, const int64_t *slice_off,
                         const int32_t *indices, const float *d_rec, float *s_rec, int64_t ns,

@@ -111,7 +111,7 @@ _SIGNATURES = {
     "sph_summation_density": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32,
                                      _i32, _vp, _vp, _i64]),
     "sph_tait": (_i32, [_vp, _i64, _i64, _vp, _vp, _i64, _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp,
-                        _vp]),
+                        _vp, _vp]),
     "sph_momentum_av": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32,
                                _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp, _vp]),
     "sph_cubic_spline": (_i32, [_vp, _i32, _i64] + [_vp] * 11),
@@ -136,7 +136,7 @@ _SIGNATURES = {
     "sph_summation_density_f32": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp,
                                          _i32, _i32, _vp, _vp, _i64]),
     "sph_tait_f32": (_i32, [_vp, _i64, _i64, _vp, _vp, _i64, _dbl, _dbl, _dbl, _i32, _vp, _vp,
-                            _vp, _vp]),
+                            _vp, _vp, _vp]),
     "sph_momentum_av_f32": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64,
                                    _i32, _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp, _vp]),
     "sph_batch_workspace_size": (_i32, [_vp, _vp, ctypes.POINTER(_i64)]),

@@ -33,6 +33,7 @@ from .. import _lib, profiling
 REC = 12
 REC_FIELDS = {"x": 0, "y": 1, "z": 2, "m": 3, "u": 4, "v": 5, "w": 6, "rho": 8, "cs": 9,
               "h": 10, "p": 11}
+PRHO2_FIELD = 7  # P/rho^2, written by the library (SPH_R_PRHO2)
 SHADOW_PROPS = tuple(REC_FIELDS)
 
 
@@ -270,6 +271,20 @@ class CellManager:
         b = self.bins.get(array.name)
         if b is not None and shadow_written and prop in REC_FIELDS:
             b.shadow_ver[prop] = array.version(prop)
+
+    # P/rho^2 of the records (the per-source term of MomentumEquation): fresh
+    # while the record's p and rho are the ones it was computed from
+    def prho2_fresh(self, array):
+        b = self.bins.get(array.name)
+        if b is None:
+            return False
+        vp, vr = array.version("p"), array.version("rho")
+        return (b.shadow_ver.get("_prho2") == (vp, vr) and b.shadow_ver.get("p") == vp
+                and b.shadow_ver.get("rho") == vr)
+
+    def prho2_written(self, array):
+        b = self.bins[array.name]
+        b.shadow_ver["_prho2"] = (array.version("p"), array.version("rho"))
 
     def sorted_field(self, array, prop):
         """Host copy of one record field in sorted order (diagnostics)."""

@@ -131,7 +131,7 @@ __global__ void __launch_bounds__(OP_THREADS)
     k_tait(int64_t n_total, int64_t n_real, const int32_t *__restrict__ perm,
            const T *__restrict__ rho_s, int64_t st, T ro, T co, T gamma, T B, int accumulate,
            double *__restrict__ p, double *__restrict__ cs, T *__restrict__ p_s,
-           T *__restrict__ cs_s)
+           T *__restrict__ cs_s, T *__restrict__ prho2_s)
 {
     int64_t t = (int64_t)blockIdx.x * OP_THREADS + threadIdx.x;
     if (t >= n_total) return;
@@ -156,12 +156,17 @@ __global__ void __launch_bounds__(OP_THREADS)
     cs[i] = r1;
     if (p_s) p_s[t * st] = r0;
     if (cs_s) cs_s[t * st] = r1;
+    if (prho2_s) {  // k_prho2's two operations on the record's p and rho
+        const T rhob = rho_s[t * st];
+        const T rhob2 = T(1.0) / (rhob * rhob);
+        prho2_s[t * st] = r0 * rhob2;
+    }
 }
 
 template <typename T>
 static int tait(sph_ctx *ctx, int64_t n_total, int64_t n_real, const int32_t *perm,
                 const T *rho_s, int64_t s_stride, double ro, double co, double gamma,
-                int accumulate, double *p, double *cs, T *p_s, T *cs_s)
+                int accumulate, double *p, double *cs, T *p_s, T *cs_s, T *prho2_s)
 {
     if (!ctx) return SPH_EINVAL;
     SPH_REQUIRE(ctx, n_real >= 0 && n_total >= n_real, "sph_tait: bad n");
@@ -170,7 +175,7 @@ static int tait(sph_ctx *ctx, int64_t n_total, int64_t n_real, const int32_t *pe
     const T B = T(co) * T(co) * T(ro) / T(gamma);  // eos_funcs.pyx:90
     hipLaunchKernelGGL(k_tait<T>, dim3(sph_blocks(n_total, OP_THREADS)), dim3(OP_THREADS), 0,
                        ctx->stream, n_total, n_real, perm, rho_s, s_stride, T(ro), T(co),
-                       T(gamma), B, accumulate, p, cs, p_s, cs_s);
+                       T(gamma), B, accumulate, p, cs, p_s, cs_s, prho2_s);
     SPH_LAUNCH_CHECK(ctx, "k_tait");
     return SPH_OK;
 }
@@ -178,19 +183,19 @@ static int tait(sph_ctx *ctx, int64_t n_total, int64_t n_real, const int32_t *pe
 extern "C" int sph_tait(sph_ctx *ctx, int64_t n_total, int64_t n_real, const int32_t *perm,
                         const double *rho_s, int64_t s_stride, double ro, double co,
                         double gamma, int accumulate, double *p, double *cs, double *p_s,
-                        double *cs_s)
+                        double *cs_s, double *prho2_s)
 {
     return tait<double>(ctx, n_total, n_real, perm, rho_s, s_stride, ro, co, gamma, accumulate,
-                        p, cs, p_s, cs_s);
+                        p, cs, p_s, cs_s, prho2_s);
 }
 
 extern "C" int sph_tait_f32(sph_ctx *ctx, int64_t n_total, int64_t n_real, const int32_t *perm,
                             const float *rho_s, int64_t s_stride, double ro, double co,
                             double gamma, int accumulate, double *p, double *cs, float *p_s,
-                            float *cs_s)
+                            float *cs_s, float *prho2_s)
 {
     return tait<float>(ctx, n_total, n_real, perm, rho_s, s_stride, ro, co, gamma, accumulate,
-                       p, cs, p_s, cs_s);
+                       p, cs, p_s, cs_s, prho2_s);
 }
 
 // Tait over sorted records with every particle real and no accumulation

@@ -15,7 +15,7 @@ the caller's particle order and, for shadowed properties, in the shadow.
 import torch
 
 from .. import _lib, profiling
-from ..base.nnps import REC, REC_FIELDS
+from ..base.nnps import PRHO2_FIELD, REC, REC_FIELDS
 from .sph_func import CSPHFunctionParticle, SPHFunction, SPHFunctionParticle
 
 _identity = {}
@@ -97,7 +97,7 @@ class TaitEquation(SPHFunction):
         cm = getattr(self, "cell_manager", None)
         n0 = names[0] if names else None
         n1 = names[1] if names else None
-        p_s = cs_s = None
+        p_s = cs_s = prho2_s = None
         fn = ctx.lib.sph_tait
         if cm is not None and self.dest.name in cm.array_indices:
             read = ('rho',) + tuple(k for k in (n0, n1) if accumulate and k in REC_FIELDS)
@@ -106,6 +106,9 @@ class TaitEquation(SPHFunction):
             perm = cm.bins[self.dest.name].perm
             p_s = cm.shadow_out(self.dest, n0) if n0 else None
             cs_s = cm.shadow_out(self.dest, n1) if n1 else None
+            if n0 == 'p':
+                # P/rho^2 of the new pressure, for MomentumEquation's sources
+                prho2_s = rec.data_ptr() + rec.element_size() * PRHO2_FIELD
             if cm.precision == "single":
                 fn = ctx.lib.sph_tait_f32
         else:
@@ -115,12 +118,15 @@ class TaitEquation(SPHFunction):
         with profiling.region("tait"):
             ctx.check(fn(ctx.handle, n, self.dest.num_real_particles, p(perm),
                                        rho_s, stride, self.ro, self.co, self.gamma,
-                                       int(bool(accumulate)), p(output1), p(output2), p_s, cs_s))
+                                       int(bool(accumulate)), p(output1), p(output2), p_s, cs_s,
+                                       prho2_s))
         if cm is not None:
             if n0:
                 cm.written(self.dest, n0, p_s is not None)
             if n1:
                 cm.written(self.dest, n1, cs_s is not None)
+            if prho2_s is not None:
+                cm.prho2_written(self.dest)
         else:
             self.dest.touch()
 
@@ -167,6 +173,9 @@ class MomentumEquation(SPHFunctionParticle):
         cnt, soff, idx = loc.get_sorted_lists()
         drec = cm.shadows(self.dest, self.DST)
         srec = cm.shadows(self.source, self.SRC)
+        # the sources' P/rho^2 record field: computed by the library's
+        # pre-pass unless Tait already wrote it for the current p and rho
+        fresh = cm.prho2_fresh(self.source)
         p = _lib.ptr
         profiling.count_pairs(loc.pair_count())
         fn = ctx.lib.sph_momentum_av_f32 if cm.precision == "single" else ctx.lib.sph_momentum_av
@@ -175,9 +184,12 @@ class MomentumEquation(SPHFunctionParticle):
                 ctx.handle, kernel.dim, self.dest.get_number_of_particles(),
                 self.dest.num_real_particles, p(cm.bins[self.dest.name].perm), p(cnt), p(soff),
                 p(idx),
-                p(drec), p(srec), cm.bins[self.source.name].n, int(cm.uniform_h()), self.alpha,
+                p(drec), p(srec), 0 if fresh else cm.bins[self.source.name].n,
+                int(cm.uniform_h()), self.alpha,
                 self.beta, self.eta,
                 int(bool(accumulate)), p(output1), p(output2), p(output3), p(self.d('dt_fac'))))
+        if not fresh:
+            cm.prho2_written(self.source)
         for name in (names or ()):
             if name:
                 cm.written(self.dest, name, False)

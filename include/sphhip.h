@@ -192,11 +192,13 @@ int sph_nnps_build(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
 /* The same build without a host synchronisation: the destination's count of
  * non-empty cells is read on the device (nd_cells_dev, from
  * sph_bin_array_dev; max_cells bounds it and sizes the launch), the totals
- * go to totals[2] = {slots, nnz} (device) and the capacity checks to the
+ * go to totals[3] = {slots, nnz, status} (device) and the capacity checks to the
  * device word *status: bit 0 when cap < slots (no entry written, every row
  * empty), bit 4 when more than max_cells cells are non-empty (the cells past
- * the bound get no rows).  The caller reads totals and status later and
- * must discard the evaluation that used a build with a status bit set. */
+ * the bound get no rows).  totals[2] receives the status word after the
+ * build, so one copy of totals[3] = {slots, nnz, status} tells the caller
+ * everything; it must discard the evaluation that used a build with a
+ * status bit set. */
 int sph_nnps_build_dev(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
                        const int32_t *d_perm, const int32_t *d_cell_start, const int32_t *d_cells,
                        const int32_t *nd_cells_dev, int64_t max_cells, int64_t nd_total,

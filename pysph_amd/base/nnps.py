@@ -372,14 +372,13 @@ class FixedDestNbrParticleLocator:
         one of them ran out of list capacity or cell bound (the evaluation
         that used it is invalid), and grow the list buffer ahead of need."""
         while self._pending:
-            ev, host, status_host = self._pending[0]
+            ev, host = self._pending[0]
             if block:
                 ev.synchronize()
             elif not ev.query():
                 return
             self._pending.pop(0)
-            slots, nnz = int(host[0]), int(host[1])
-            st = int(status_host[0])
+            slots, nnz, st = int(host[0]), int(host[1]), int(host[2])
             self.slots = slots
             self._nnz = nnz
             if st:
@@ -436,7 +435,7 @@ class FixedDestNbrParticleLocator:
                                            device=dev)
             if self._status is None:
                 self._status = torch.zeros(1, dtype=torch.int32, device=dev)
-            totals = torch.empty(2, dtype=torch.int64, device=dev)
+            totals = torch.empty(3, dtype=torch.int64, device=dev)
             if cm.precision == "single":
                 rc = lib.sph_nnps_build_f32_dev(
                     ctx.handle, ctypes.byref(cm.grid), p(drec), p(dst.get_carray("x")),
@@ -453,14 +452,12 @@ class FixedDestNbrParticleLocator:
                     p(self.counts), p(self.slice_off), p(self.indices), self.indices.numel(),
                     p(totals), p(self._status))
             ctx.check(rc)
-            host = torch.empty(2, dtype=torch.int64, pin_memory=True)
-            host.copy_(totals, non_blocking=True)
-            status_host = torch.empty(1, dtype=torch.int32, pin_memory=True)
-            status_host.copy_(self._status, non_blocking=True)
+            host = torch.empty(3, dtype=torch.int64, pin_memory=True)
+            host.copy_(totals, non_blocking=True)   # {slots, nnz, status}
             ev = torch.cuda.Event()
             ev.record()
             self._totals = totals
-            self._pending.append((ev, host, status_host))
+            self._pending.append((ev, host))
             self.nbuilds += 1
             return
         slots = ctypes.c_int64(0)

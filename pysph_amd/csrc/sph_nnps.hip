@@ -651,6 +651,11 @@ struct CellArgs {
     // destination cells per block of the count pass (its grid is the cell
     // count over this, at most NN_CELL_GRID blocks)
     int cells_per_block;
+    // device-side plain builds: the fill also writes totals = {slots, nnz,
+    // status} (and sets the status bits) from its first thread
+    int64_t *totals;
+    const int64_t *nnz_dev;
+    int64_t cell_bound;
 };
 
 __device__ __forceinline__ int wave_max_i(int v)
@@ -956,6 +961,19 @@ __global__ void __launch_bounds__(NN_THREADS)
     k_nnps_fill_slices(CellArgs a, int64_t nd_total, int32_t *__restrict__ counts,
                        const int64_t *__restrict__ slice_off, int32_t *__restrict__ indices)
 {
+    if (a.totals && blockIdx.x == 0 && threadIdx.x == 0) {
+        // totals and capacity checks of a device-side build: status bit 0
+        // when the lists need more than cap slots (nothing is written), bit
+        // 4 when more cells are non-empty than the launch bound
+        const int64_t sl = slice_off[a.nslices];
+        int32_t st = *a.status;
+        if (sl > a.cap) st |= 1;
+        if (a.ncell_dev && (int64_t)*a.ncell_dev > a.cell_bound) st |= 16;
+        *a.status = st;
+        a.totals[0] = sl;
+        a.totals[1] = *a.nnz_dev;
+        a.totals[2] = st;
+    }
     const int64_t t = (int64_t)sph_xcd_tile(blockIdx.x, gridDim.x) * NN_THREADS + threadIdx.x;
     if (t >= nd_total) return;
     if (a.status && slice_off[a.nslices] > a.cap) {
@@ -1008,7 +1026,7 @@ __global__ void __launch_bounds__(NN_THREADS)
 // slice s of 64 destinations: storage 64 * max(count) and its entry total
 __global__ void __launch_bounds__(NN_THREADS)
     k_slices(const int32_t *__restrict__ counts, int64_t nd_total, int64_t nslices,
-             int64_t *__restrict__ slice_w, int64_t *__restrict__ slice_sum)
+             int64_t *__restrict__ slice_w, int64_t *__restrict__ nnz)
 {
     const int64_t t = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
     const int c = t < nd_total ? counts[t] : 0;
@@ -1021,7 +1039,20 @@ __global__ void __launch_bounds__(NN_THREADS)
     const int64_t s = t >> 6;
     if ((threadIdx.x & 63) == 0 && s < nslices) {
         slice_w[s] = 64 * (int64_t)m;
-        slice_sum[s] = sum;
+        atomicAdd((unsigned long long *)nnz, (unsigned long long)sum);  // exact: integers
+    }
+}
+
+// counts of every row, the flagged-cell counter and the entry total start at 0
+__global__ void __launch_bounds__(NN_THREADS)
+    k_nnps_reset(int32_t *__restrict__ counts, int64_t nd_total, int32_t *__restrict__ novf,
+                 int64_t *__restrict__ nnz)
+{
+    const int64_t t = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
+    if (t < nd_total) counts[t] = 0;
+    if (t == 0) {
+        *novf = 0;
+        *nnz = 0;
     }
 }
 
@@ -1081,8 +1112,11 @@ static int nnps_count(sph_ctx *ctx, CellArgs &a, int64_t ncell_launch, int64_t n
     a.novf = (int32_t *)(sc + masks_bytes + ovf_bytes + cand_bytes + dcell_bytes);
     a.ovf_list = a.novf + 1;
     a.nslices = nslices;
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, a.novf, 1, 0);
-    SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
+    int64_t *slice_sum = (int64_t *)(sc + sums_off);
+    *nnz_dev = slice_sum + nslices;
+    hipLaunchKernelGGL(k_nnps_reset, dim3(sph_blocks(nd_total, NN_THREADS)), dim3(NN_THREADS), 0,
+                       ctx->stream, counts, nd_total, a.novf, *nnz_dev);
+    SPH_LAUNCH_CHECK(ctx, "k_nnps_reset");
     auto count_kern = a.f32 ? k_nnps_cells<false, float> : k_nnps_cells<false, double>;
     const int cpb = a.cells_per_block > 0 ? a.cells_per_block : 1;
     const int64_t cgrid = (ncell_launch + cpb - 1) / cpb;
@@ -1091,14 +1125,9 @@ static int nnps_count(sph_ctx *ctx, CellArgs &a, int64_t ncell_launch, int64_t n
                        counts, (const int64_t *)nullptr, (int32_t *)nullptr);
     SPH_LAUNCH_CHECK(ctx, "k_nnps_cells<count>");
     // slice widths -> offsets (scan); slice totals -> entry total
-    int64_t *slice_sum = (int64_t *)(sc + sums_off);
-    *nnz_dev = slice_sum + nslices;
     hipLaunchKernelGGL(k_slices, dim3(sph_blocks(nslices * 64, NN_THREADS)), dim3(NN_THREADS), 0,
-                       ctx->stream, counts, nd_total, nslices, slice_off, slice_sum);
+                       ctx->stream, counts, nd_total, nslices, slice_off, *nnz_dev);
     SPH_LAUNCH_CHECK(ctx, "k_slices");
-    hipLaunchKernelGGL(k_sum_i64, dim3(1), dim3(1024), 0, ctx->stream, slice_sum, nslices,
-                       *nnz_dev);
-    SPH_LAUNCH_CHECK(ctx, "k_sum_i64");
     return sph_scan_exclusive_i64_at(ctx, slice_off, nslices, sums_off + sums_bytes);
 }
 
@@ -1139,7 +1168,7 @@ static CellArgs cell_args_base()
     return a;
 }
 
-// totals of a device-side plain build: totals = [slots, nnz]; status bit 0
+// totals of an empty device-side plain build: totals = {slots, nnz, status}; status bit 0
 // when the lists need more than cap slots (nothing written, rows empty), bit
 // 4 when there are more non-empty destination cells than max_cells (the
 // cells past that bound get no rows)
@@ -1151,12 +1180,13 @@ __global__ void k_nnps_dev_totals(const int64_t *__restrict__ slots,
 {
     if (threadIdx.x != 0) return;
     const int64_t sl = slots ? *slots : 0, nz = nnz ? *nnz : 0;
-    totals[0] = sl;
-    totals[1] = nz;
-    int32_t st = 0;
+    int32_t st = *status;
     if (sl > cap) st |= 1;
     if (ncells_dev && *ncells_dev > max_cells) st |= 16;
-    if (st) *status |= st;
+    *status = st;
+    totals[0] = sl;
+    totals[1] = nz;
+    totals[2] = st;
 }
 
 // Host mode (dev == NULL): nd_cells is the host count of non-empty cells;
@@ -1199,9 +1229,9 @@ static int nnps_build(sph_ctx *ctx, const sph_grid *grid, int f32, const double 
         *nnz_out = 0;
     }
     const int64_t nslices = (nd_total + 63) / 64;
-    rc = nnps_clear(ctx, nd_total, counts, slice_off);
-    if (rc) return rc;
     if (nd_total == 0 || ns == 0 || nd_real == 0 || nd_cells == 0) {
+        rc = nnps_clear(ctx, nd_total, counts, slice_off);  // every row empty
+        if (rc) return rc;
         if (dev) {
             hipLaunchKernelGGL(k_nnps_dev_totals, dim3(1), dim3(64), 0, ctx->stream,
                                (const int64_t *)nullptr, (const int64_t *)nullptr,
@@ -1237,11 +1267,10 @@ static int nnps_build(sph_ctx *ctx, const sph_grid *grid, int f32, const double 
     int64_t *nnz_dev = nullptr;
     rc = nnps_count(ctx, a, nd_cells, nd_total, counts, slice_off, &nnz_dev);
     if (rc) return rc;
-    if (dev) {
-        hipLaunchKernelGGL(k_nnps_dev_totals, dim3(1), dim3(64), 0, ctx->stream,
-                           (const int64_t *)(slice_off + nslices), (const int64_t *)nnz_dev,
-                           dev->ncells_dev, cap, nd_cells, dev->totals, dev->status);
-        SPH_LAUNCH_CHECK(ctx, "k_nnps_dev_totals");
+    if (dev) {  // the fill's first thread writes the totals and the status
+        a.totals = dev->totals;
+        a.nnz_dev = nnz_dev;
+        a.cell_bound = nd_cells;
         return nnps_fill(ctx, a, nd_cells, nd_total, counts, slice_off, indices);
     }
     rc = sph_ensure_pinned(ctx, 2 * sizeof(int64_t));
@@ -1336,9 +1365,8 @@ int sph_nnps_build_batch(sph_ctx *ctx, const GridDev *grids, const int32_t *sim_
 {
     SPH_REQUIRE(ctx, maxch >= 1 && maxch <= NN_MAXCH, "sph_nnps_build_batch: bad maxch");
     SPH_REQUIRE(ctx, cells_per_block >= 1, "sph_nnps_build_batch: bad cells_per_block");
-    int rc = nnps_clear(ctx, n, counts, slice_off);
-    if (rc) return rc;
-    if (n == 0 || max_cells == 0) return SPH_OK;
+    int rc;
+    if (n == 0 || max_cells == 0) return nnps_clear(ctx, n, counts, slice_off);
     CellArgs a = cell_args_base();
     a.grids = grids;
     a.sim_of = sim_of;

@@ -364,3 +364,31 @@ def test_device_side_rebuild_matches_and_reports_capacity(gpu):
     loc.update()
     with pytest.raises(_lib.SphError, match="incomplete"):
         _ = loc.nnz
+
+
+def test_lists_exact_at_the_support_radius(gpu):
+    """Pairs exactly at, and within a few FP64 ulps of, the support radius,
+    far from the origin: the float distance test of the list kernel leaves
+    them to its FP64 fallback, so the lists equal the oracle's."""
+    base, _, _ = _api()
+    rng = np.random.default_rng(31)
+    n = 12
+    dx = 0.25
+    g = np.arange(n) * dx
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    x, y, z = (1000.0 + X.ravel(), -2000.0 + Y.ravel(), 500.0 + Z.ravel())
+    # nudge a third of the particles by a few ulps (both signs)
+    k = rng.integers(-4, 5, x.size) * (rng.random(x.size) < 0.33)
+    x = x + k * np.spacing(1000.0)
+    d = {"x": x, "y": y, "z": z, "h": np.full(x.size, dx), "m": np.ones(x.size),
+         "rho": np.ones(x.size)}
+    pa = base.get_particle_array(name="fluid", **d)
+    particles = base.Particles([pa])
+    loc = particles.nnps_manager.get_neighbor_particle_locator(pa, pa, 2.0)
+    off, idx = (t.cpu().numpy() for t in loc.get_csr())
+    full = {kk: pa.get(kk) for kk in ("x", "y", "z", "h")}
+    roff, ridx = cpu_ref.nnps(full, full, 2.0, particles.cell_manager.cell_size, True)
+    assert np.array_equal(off, roff) and np.array_equal(idx, ridx)
+    # the radius is hit exactly by lattice neighbours two spacings away
+    assert (len(idx) - x.size) > 0
+    print(f"support-radius lattice far from the origin: {len(idx)} entries, identical to the oracle")

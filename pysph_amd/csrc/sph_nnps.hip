@@ -742,38 +742,11 @@ template <> __device__ __forceinline__ const float *srec_of<float>(const CellArg
 
 // one destination cell w (one wavefront); seg_start, seg_end, dsts are the
 // block's LDS arrays
-// FP64 lists: the distance test runs in float on coordinates taken relative
-// to the destination cell's first particle (|relative coordinate| < D =
-// (stencil reach + 1) * cell size), against the squared radius widened and
-// narrowed by a band B = 101 u D^2 (u = 2^-24).  Rounding the coordinates,
-// the relative shifts (FP64), the float subtraction, squares and sum moves
-// the float d^2 at most 86 u D^2 from the FP64 d^2 of the reference
-// expression, so d^2_float < R^2 - B proves a hit and d^2_float > R^2 + B a
-// miss; the candidates in between are tested in FP64.  The lists are the
-// FP64 lists exactly.
-constexpr double NN_BAND = 101.0 * 0x1p-24;
-
-__device__ __forceinline__ float f32_down(double x)
-{
-    float f = (float)x;
-    if ((double)f > x) f = nextafterf(f, -INFINITY);
-    return f;
-}
-
-__device__ __forceinline__ float f32_up(double x)
-{
-    float f = (float)x;
-    if ((double)f < x) f = nextafterf(f, INFINITY);
-    return f;
-}
-
 template <bool FILL, typename T>
 __device__ void nnps_cell(const CellArgs &a, int64_t w, int32_t *__restrict__ counts,
                           const int64_t *__restrict__ slice_off, int32_t *__restrict__ indices,
-                          int32_t *seg_start, int32_t *seg_end, typename Quad<T>::type *dsts,
-                          float4 *fdst, float *fhi)
+                          int32_t *seg_start, int32_t *seg_end, typename Quad<T>::type *dsts)
 {
-    constexpr bool FILT = sizeof(T) == sizeof(double);
     const T *drec = drec_of<T>(a);
     const T *srec = srec_of<T>(a);
     const int lane = threadIdx.x;
@@ -819,8 +792,6 @@ __device__ void nnps_cell(const CellArgs &a, int64_t w, int32_t *__restrict__ co
     const int z0 = max(cz - dist, 0), z1 = min(cz + dist, g.d2 - 1);
     const int nx = x1 - x0 + 1, ny = y1 - y0 + 1, nz = z1 - z0 + 1;
     const int nst = nx * ny * nz;
-    const double Dmax = (double)(dist + 1) * g.cs;
-    const double band = NN_BAND * Dmax * Dmax;
 
     int nchunks = 0;  // chunks per destination over all stencil batches
     for (int32_t g0 = ds; g0 < de; g0 += CW) {  // destination groups of <= 64
@@ -830,15 +801,7 @@ __device__ void nnps_cell(const CellArgs &a, int64_t w, int32_t *__restrict__ co
         if (lane < gn) {
             const T *dr = drec + (int64_t)my_t * SPH_REC;
             const T radius = dr[SPH_R_H] * T(a.rs);
-            const T R2 = radius * radius;
-            dsts[lane] = Quad<T>::make(dr[SPH_R_X], dr[SPH_R_Y], dr[SPH_R_Z], R2);
-            if (FILT) {
-                fdst[lane] = make_float4((float)((double)dr[SPH_R_X] - c0x),
-                                         (float)((double)dr[SPH_R_Y] - c0y),
-                                         (float)((double)dr[SPH_R_Z] - c0z),
-                                         f32_down((double)R2 - band));
-                fhi[lane] = f32_up((double)R2 + band);
-            }
+            dsts[lane] = Quad<T>::make(dr[SPH_R_X], dr[SPH_R_Y], dr[SPH_R_Z], radius * radius);
             mreal = a.dperm[my_t] < a.nd_real;
         }
         const uint64_t realmask = __ballot(mreal);
@@ -886,40 +849,15 @@ __device__ void nnps_cell(const CellArgs &a, int64_t w, int32_t *__restrict__ co
                     sy = sr[SPH_R_Y];
                     sz = sr[SPH_R_Z];
                 }
-                float fsx = 0.0f, fsy = 0.0f, fsz = 0.0f;
-                if (FILT) {
-                    fsx = (float)((double)sx - c0x);
-                    fsy = (float)((double)sy - c0y);
-                    fsz = (float)((double)sz - c0z);
-                }
                 uint32_t mlo = 0, mhi = 0;  // count: my destination's hit mask of this chunk
                 for (int d = 0; d < gn; ++d) {
                     if (!((realmask >> d) & 1)) continue;  // non-LocalReal: empty row
                     const int32_t t = g0 + d;
-                    bool hit;
-                    if (FILT) {
-                        const float4 F = fdst[d];  // same address in every lane
-                        const float gx = fsx - F.x, gy = fsy - F.y, gz = fsz - F.z;
-                        const float g2 = gx * gx + gy * gy + gz * gz;
-                        hit = g2 < F.w;
-                        const bool unsure = valid && !hit && !(g2 > fhi[d]);
-                        if (__ballot(unsure)) {  // rare: the FP64 test decides
-                            if (unsure) {
-                                const auto D = dsts[d];
-                                // cPoint_distance2(src, dst), base/point.pxd:35-37
-                                const T ex = sx - D.x, ey = sy - D.y, ez = sz - D.z;
-                                const T d2 = ex * ex + ey * ey + ez * ez;
-                                hit = d2 < D.w;
-                            }
-                        }
-                        hit = valid && hit;
-                    } else {
-                        const auto D = dsts[d];  // same address in every lane
-                        // cPoint_distance2(src, dst), base/point.pxd:35-37
-                        const T ex = sx - D.x, ey = sy - D.y, ez = sz - D.z;
-                        const T d2 = ex * ex + ey * ey + ez * ez;
-                        hit = valid && d2 < D.w;
-                    }
+                    const auto D = dsts[d];  // same address in every lane
+                    // cPoint_distance2(src, dst), base/point.pxd:35-37
+                    const T ex = sx - D.x, ey = sy - D.y, ez = sz - D.z;
+                    const T d2 = ex * ex + ey * ey + ez * ez;
+                    const bool hit = valid && d2 < D.w;
                     const uint64_t mask = __ballot(hit);
                     if (mask == 0) continue;
                     if (!FILL) {
@@ -985,14 +923,12 @@ __global__ void __launch_bounds__(CW)
     __shared__ int32_t seg_start[MAXSEG];
     __shared__ int32_t seg_end[MAXSEG];  // inclusive prefix of the segment sizes
     __shared__ typename Quad<T>::type dsts[CW];  // (x, y, z, radius^2) of the group's destinations
-    __shared__ float4 fdst[CW];  // FP64 lists: float relative (x, y, z) and R^2 - band
-    __shared__ float fhi[CW];    // FP64 lists: R^2 + band
     if (FILL) {
         if (a.status && slice_off[a.nslices] > a.cap) return;  // no room: nothing is written
         const int32_t nov = *a.novf;
         for (int32_t k = blockIdx.x; k < nov; k += gridDim.x)
             nnps_cell<true, T>(a, a.ovf_list[k], counts, slice_off, indices, seg_start, seg_end,
-                               dsts, fdst, fhi);
+                               dsts);
     } else {
         // cells read on the device are bounded by the launch size the
         // scratch was carved for (more cells than that is a status error of
@@ -1006,16 +942,14 @@ __global__ void __launch_bounds__(CW)
         const unsigned G = gridDim.x;
         if (G < SPH_NXCD) {  // too few blocks to give every XCD one
             for (int64_t w = blockIdx.x; w < ncl; w += G)
-                nnps_cell<false, T>(a, w, counts, slice_off, indices, seg_start, seg_end, dsts,
-                                   fdst, fhi);
+                nnps_cell<false, T>(a, w, counts, slice_off, indices, seg_start, seg_end, dsts);
             return;
         }
         const unsigned x = blockIdx.x % SPH_NXCD, k = blockIdx.x / SPH_NXCD;
         const unsigned per = (G - x + SPH_NXCD - 1) / SPH_NXCD;  // blocks on XCD x (>= 1)
         const int64_t lo = ncl * x / SPH_NXCD, hi = ncl * (x + 1) / SPH_NXCD;
         for (int64_t w = lo + k; w < hi; w += per)  // uniform: one wavefront per block
-            nnps_cell<false, T>(a, w, counts, slice_off, indices, seg_start, seg_end, dsts,
-                                   fdst, fhi);
+            nnps_cell<false, T>(a, w, counts, slice_off, indices, seg_start, seg_end, dsts);
     }
 }
 

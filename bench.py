@@ -600,7 +600,7 @@ def cpu_baseline_lattice(args, d):
 def main():
     args = _args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build runs the hot path on a single MI355X "
+        print(f"bench.py measures one GPU: pysph_amd runs the hot path on a single MI355X "
               f"and has no multi-GPU path (asked for --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
     import torch

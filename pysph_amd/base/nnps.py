@@ -429,7 +429,7 @@ class FixedDestNbrParticleLocator:
         if self.nbuilds > 0 and bd.ncells_dev is not None:
             # device-side build: no synchronisation.  The buffer is grown
             # ahead of need from the last known size (the lists in it are
-            # replaced by this build anyway).
+            # replaced by the build below anyway).
             if self.slots > (3 * self.indices.numel()) // 4:
                 self.indices = torch.empty(int(self.slots * 1.5) + 4096, dtype=torch.int32,
                                            device=dev)

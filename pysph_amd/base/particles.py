@@ -17,7 +17,7 @@ class Particles:
                  cell_order="morton", precision=None):
         assert len(arrays) > 0, "Particles must be given some arrays!"
         if in_parallel:
-            raise NotImplementedError("the MPI path is not part of this build")
+            raise NotImplementedError("the MPI path is not part of this package (one GPU)")
         if periodic_domain is not None:
             raise NotImplementedError("periodic domains are not part of this path")
         prec = arrays[0].cl_precision

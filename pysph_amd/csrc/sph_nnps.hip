@@ -912,9 +912,9 @@ __device__ void nnps_cell(const CellArgs &a, int64_t w, int32_t *__restrict__ co
     }
 }
 
-// FILL = false: every non-empty destination cell, gridDim.x cells at a time
-// (each XCD walks a contiguous range of each round).  FILL = true: the cells
-// the count pass flagged, listed in a.ovf_list.
+// FILL = false: every non-empty destination cell (XCD x walks the x-th
+// eighth of the cells with its blocks).  FILL = true: the cells the count
+// pass flagged, listed in a.ovf_list.
 template <bool FILL, typename T>
 __global__ void __launch_bounds__(CW)
     k_nnps_cells(CellArgs a, int32_t *__restrict__ counts, const int64_t *__restrict__ slice_off,

@@ -36,6 +36,14 @@ __global__ void __launch_bounds__(OP_THREADS)
         if (out_s) out_s[t * out_stride] = 0;
         return;
     }
+    const int32_t e = counts[t];
+    if (e == 0) {  // empty row (a destination far from a second array): base + 0
+        const T base = accumulate ? (out_s ? out_s[t * out_stride] : T(rho[i])) : T(0.0);
+        const T val = base + T(0.0);
+        rho[i] = val;
+        if (out_s) out_s[t * out_stride] = val;
+        return;
+    }
     const auto *dr = rec2(drec, t);
     const auto dxy = dr[0], dzm = dr[1];
     const T xa = dxy.x, ya = dxy.y, za = dzm.x, ha = drec[t * SPH_REC + SPH_R_H];
@@ -44,7 +52,6 @@ __global__ void __launch_bounds__(OP_THREADS)
     // SELL-64 row: entry k at row[64*k]; each load is one coalesced line for
     // the wavefront and is read once, so it bypasses the caches
     const int32_t *row = indices + slice_off[t >> 6] + (t & 63);
-    const int32_t e = counts[t];
     int32_t k = 0;
     // DENS_NT neighbours' loads in flight, then their terms added in list order
     for (; k + DENS_NT <= e; k += DENS_NT) {
@@ -270,6 +277,14 @@ __global__ void __launch_bounds__(OP_THREADS)
         if (az) az[i] = 0;
         return;
     }
+    const int32_t e = counts[t];
+    if (e == 0) {  // empty row: dt_fac keeps its value, a = base + 0
+        dt_fac[i] = T(dt_fac[i]);
+        ax[i] = (accumulate ? T(ax[i]) : T(0.0)) + T(0.0);
+        ay[i] = (accumulate ? T(ay[i]) : T(0.0)) + T(0.0);
+        if (az) az[i] = (accumulate ? T(az[i]) : T(0.0)) + T(0.0);
+        return;
+    }
     const auto *dr = rec2(drec, t);
     const auto d0 = dr[0], d1 = dr[1], d2 = dr[2], d3 = dr[3], d4 = dr[4], d5 = dr[5];
     MomDest<T> D;
@@ -288,7 +303,6 @@ __global__ void __launch_bounds__(OP_THREADS)
     T dtf = T(dt_fac[i]);
     T nr0 = T(0.0), nr1 = T(0.0), nr2 = T(0.0);
     const int32_t *row = indices + slice_off[t >> 6] + (t & 63);  // SELL-64 row
-    const int32_t e = counts[t];
     int32_t k = 0;
     // MOM_NT neighbours' records in flight, then their terms added in list order
     for (; k + MOM_NT <= e; k += MOM_NT) {

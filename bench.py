@@ -100,13 +100,15 @@ def measured_traffic():
 
 
 def momentum_bytes(nd, nnz, ns_other, ns):
-    """Algorithmic HBM bytes of one sph_momentum_av call (k_prho2 over the
-    ns source records, then k_momentum): P/rho^2 of each source (p and rho
-    read, P/rho^2 written: 24 B), and per destination the int64 row offset,
-    10 destination properties + dt_fac read, ax ay az read-modify-write and
-    dt_fac written (8+11*8+3*16+8 = 152 B), 4 B per list entry, and -- when
-    the source is another array -- its 11 properties once (88 B per source
-    particle)."""
+    """Algorithmic HBM bytes of MomentumEquation over one source (k_prho2
+    over the ns source records if needed, then k_momentum): P/rho^2 of each
+    source (p and rho read, P/rho^2 written: 24 B), and per destination the
+    int64 row offset, 10 destination properties + dt_fac read, ax ay az
+    read-modify-write and dt_fac written (8+11*8+3*16+8 = 152 B), 4 B per
+    list entry, and -- when the source is another array -- its 11
+    properties once (88 B per source particle).  A launch over several
+    sources reads and writes each destination once: pass the entries of all
+    lists and the particles of the other arrays."""
     return 24 * ns + nd * 152 + 4 * nnz + 88 * ns_other
 
 
@@ -156,13 +158,15 @@ def run_dambreak(args):
     nw = wall.get_number_of_particles()
     nnz_ff = mom_calc.funcs[0].nbr_locator.nnz
     nnz_fw = mom_calc.funcs[1].nbr_locator.nnz
-    # dominant kernel: momentum (two launches per evaluation: fluid, wall source)
-    mom = prof.get("momentum_av", {"mean_ms": float("nan"), "count": 0})
-    # P/rho^2 of both sources is current when momentum runs (Tait writes the
-    # fluid's, the walls' is computed once), so the calls run k_momentum only
-    b_ff = momentum_bytes(nd, nnz_ff, 0, 0)
-    b_fw = momentum_bytes(nd, nnz_fw, nw, 0)
-    bytes_per_launch = (b_ff + b_fw) / 2.0
+    # dominant kernel: momentum, one launch per evaluation over both sources
+    # (fluid and wall lists; a PC step evaluates twice).  P/rho^2 of both
+    # sources is current when momentum runs (Tait writes the fluid's, the
+    # walls' is computed once), so the calls run k_momentum only.
+    mom = prof.get("momentum_av", {"mean_ms": float("nan"), "count": 0, "total_ms": float("nan")})
+    # the fused launch reads and writes each destination once
+    b_eval = momentum_bytes(nd, nnz_ff + nnz_fw, nw, 0)
+    evals = 2 * args.steps
+    bytes_per_launch = b_eval * evals / max(mom["count"], 1)
     achieved = bytes_per_launch / (mom["mean_ms"] * 1e-3) / 1e9
     result = {
         "metric": "pair-interactions/s",

@@ -66,6 +66,7 @@ typedef struct sph_grid {
 enum { SPH_ORDER_MORTON = 0, SPH_ORDER_LINEAR = 1 };
 
 #define SPH_MAX_PROPS 16
+#define SPH_MAX_SRC 4 /* source arrays of one multi-source operator call */
 
 /* ---- context ---------------------------------------------------------- */
 
@@ -300,6 +301,41 @@ int sph_momentum_av_f32(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real
                         const int32_t *indices, const float *d_rec, float *s_rec, int64_t ns,
                         int uniform_h, double alpha, double beta, double eta, int accumulate,
                         double *ax, double *ay, double *az, double *dt_fac);
+
+/* Every function of one calc (SPHCalc.sph, sph/sph_calc.pyx:307-358: one
+ * function per source array, e.g. fluid and walls) in one launch: per
+ * destination each source's row is summed from 0 in its list order and added
+ * to the output in source order, out = ((base + r_1) + r_2) ..., the
+ * arithmetic of consecutive single-source calls with accumulate set from the
+ * second on.  counts/slice_off/indices: each source's SELL-64 lists for this
+ * destination; s_rec: its sorted records (double*, or float* for _f32);
+ * ns: momentum only, records whose P/rho^2 field is refreshed first (0 when
+ * it is current). */
+typedef struct sph_sources {
+    int32_t nsrc;
+    const int32_t *counts[SPH_MAX_SRC];
+    const int64_t *slice_off[SPH_MAX_SRC];
+    const int32_t *indices[SPH_MAX_SRC];
+    const void *s_rec[SPH_MAX_SRC];
+    int64_t ns[SPH_MAX_SRC];
+} sph_sources;
+int sph_summation_density_multi(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                                const int32_t *d_perm, const sph_sources *src,
+                                const double *d_rec, int uniform_h, int accumulate,
+                                double *rho, double *out_s, int64_t out_s_stride);
+int sph_summation_density_multi_f32(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                                    const int32_t *d_perm, const sph_sources *src,
+                                    const float *d_rec, int uniform_h, int accumulate,
+                                    double *rho, float *out_s, int64_t out_s_stride);
+int sph_momentum_av_multi(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                          const int32_t *d_perm, const sph_sources *src, const double *d_rec,
+                          int uniform_h, double alpha, double beta, double eta, int accumulate,
+                          double *ax, double *ay, double *az, double *dt_fac);
+int sph_momentum_av_multi_f32(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                              const int32_t *d_perm, const sph_sources *src, const float *d_rec,
+                              int uniform_h, double alpha, double beta, double eta,
+                              int accumulate, double *ax, double *ay, double *az,
+                              double *dt_fac);
 
 /* SPHDensityRate (sph/funcs/density_funcs.pyx:116-164, hks off): the
  * continuity equation rate_a = sum_b (v_a - v_b) . gradW_a(r_ab, h_ab) m_b over

@@ -60,6 +60,15 @@ class PropList(ctypes.Structure):
                 ("dst", _vp * MAX_PROPS)]
 
 
+MAX_SRC = 4
+
+
+class Sources(ctypes.Structure):
+    """sph_sources (include/sphhip.h)."""
+    _fields_ = [("nsrc", ctypes.c_int32), ("counts", _vp * MAX_SRC), ("slice_off", _vp * MAX_SRC),
+                ("indices", _vp * MAX_SRC), ("s_rec", _vp * MAX_SRC), ("ns", _i64 * MAX_SRC)]
+
+
 class BatchDesc(ctypes.Structure):
     """sph_batch_desc (include/sphhip.h)."""
     _fields_ = ([("nsims", ctypes.c_int32), ("dim", ctypes.c_int32), ("n", _i64),
@@ -115,6 +124,16 @@ _SIGNATURES = {
     "sph_momentum_av": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32,
                                _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp, _vp]),
     "sph_cubic_spline": (_i32, [_vp, _i32, _i64] + [_vp] * 11),
+    "sph_summation_density_multi": (_i32, [_vp, _i32, _i64, _i64, _vp, ctypes.POINTER(Sources),
+                                           _vp, _i32, _i32, _vp, _vp, _i64]),
+    "sph_summation_density_multi_f32": (_i32, [_vp, _i32, _i64, _i64, _vp,
+                                               ctypes.POINTER(Sources), _vp, _i32, _i32, _vp,
+                                               _vp, _i64]),
+    "sph_momentum_av_multi": (_i32, [_vp, _i32, _i64, _i64, _vp, ctypes.POINTER(Sources), _vp,
+                                     _i32, _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp, _vp]),
+    "sph_momentum_av_multi_f32": (_i32, [_vp, _i32, _i64, _i64, _vp, ctypes.POINTER(Sources),
+                                         _vp, _i32, _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp,
+                                         _vp]),
     "sph_density_rate": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32,
                                 _vp]),
     "sph_density_rate_f32": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32,

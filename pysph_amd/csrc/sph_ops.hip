@@ -5,6 +5,8 @@
 // sph/funcs/pressure_funcs.pyx:246-391) and is compiled without FMA
 // contraction, so with the lists in the reference's order the sums are the
 // reference's sums.
+#include <string.h>
+
 #include "sph_internal.h"
 #include "sph_terms.h"
 
@@ -16,44 +18,49 @@
 #define MOM_NT 4
 
 // ---------------------------------------------------------------------------
+// Sources of one destination: every function of a calc over its own source
+// array (SPHCalc.sph, sph/sph_calc.pyx:307-358) in one launch.  Each source's
+// row is summed from 0 in list order and added to the output in source
+// order, exactly as separate calls would (out = (out + r_1) + r_2 ...).
+// ---------------------------------------------------------------------------
+template <typename T> struct SrcLists {
+    int nsrc;
+    const int32_t *counts[SPH_MAX_SRC];
+    const int64_t *slice_off[SPH_MAX_SRC];
+    const int32_t *indices[SPH_MAX_SRC];
+    const T *srec[SPH_MAX_SRC];
+};
+
+template <typename T>
+static int src_lists(sph_ctx *ctx, const char *name, const sph_sources *src, SrcLists<T> *L)
+{
+    SPH_REQUIRE(ctx, src && src->nsrc >= 1 && src->nsrc <= SPH_MAX_SRC, "%s: bad nsrc", name);
+    L->nsrc = src->nsrc;
+    for (int s = 0; s < src->nsrc; ++s) {
+        SPH_REQUIRE(ctx, src->counts[s] && src->slice_off[s] && src->s_rec[s] && src->ns[s] >= 0,
+                    "%s: null source %d", name, s);
+        L->counts[s] = src->counts[s];
+        L->slice_off[s] = src->slice_off[s];
+        L->indices[s] = src->indices[s];
+        L->srec[s] = (const T *)src->s_rec[s];
+    }
+    return SPH_OK;
+}
+
+// ---------------------------------------------------------------------------
 // SPHRho (rho_term in sph_terms.h)
 // ---------------------------------------------------------------------------
+// one SELL-64 row of SPHRho, summed from 0 in list order
 template <bool UH, typename T>
-__global__ void __launch_bounds__(OP_THREADS)
-    k_summation_density(int dim, int64_t nd_total, int64_t nd_real,
-                        const int32_t *__restrict__ dperm, const int32_t *__restrict__ counts,
-                        const int64_t *__restrict__ slice_off,
-                        const int32_t *__restrict__ indices, const T *__restrict__ drec,
-                        const T *__restrict__ srec, int accumulate,
-                        double *__restrict__ rho, T *__restrict__ out_s, int64_t out_stride)
+__device__ __forceinline__ T rho_row(int dim, T xa, T ya, T za, T ha, T facA,
+                                     const int32_t *__restrict__ row, int32_t e,
+                                     const T *__restrict__ srec)
 {
-    const int64_t t =
-        (int64_t)sph_xcd_tile(blockIdx.x, gridDim.x) * OP_THREADS + threadIdx.x;
-    if (t >= nd_total) return;
-    const int32_t i = dperm[t];
-    if (i >= nd_real) {  // non-LocalReal destinations (sph_func.pyx:226-227)
-        rho[i] = 0;
-        if (out_s) out_s[t * out_stride] = 0;
-        return;
-    }
-    const int32_t e = counts[t];
-    if (e == 0) {  // empty row (a destination far from a second array): base + 0
-        const T base = accumulate ? (out_s ? out_s[t * out_stride] : T(rho[i])) : T(0.0);
-        const T val = base + T(0.0);
-        rho[i] = val;
-        if (out_s) out_s[t * out_stride] = val;
-        return;
-    }
-    const auto *dr = rec2(drec, t);
-    const auto dxy = dr[0], dzm = dr[1];
-    const T xa = dxy.x, ya = dxy.y, za = dzm.x, ha = drec[t * SPH_REC + SPH_R_H];
-    const T facA = sph_h_dim(ha, dim) * sph_cubic_fac1<T>(dim);
     T nr = T(0.0);
-    // SELL-64 row: entry k at row[64*k]; each load is one coalesced line for
-    // the wavefront and is read once, so it bypasses the caches
-    const int32_t *row = indices + slice_off[t >> 6] + (t & 63);
     int32_t k = 0;
-    // DENS_NT neighbours' loads in flight, then their terms added in list order
+    // DENS_NT neighbours' loads in flight, then their terms added in list order;
+    // each list load is one coalesced line for the wavefront, read once, so it
+    // bypasses the caches
     for (; k + DENS_NT <= e; k += DENS_NT) {
         int32_t j[DENS_NT];
         typename Pair<T>::type a[DENS_NT], b[DENS_NT];
@@ -78,32 +85,84 @@ __global__ void __launch_bounds__(OP_THREADS)
         const T hb = UH ? ha : srec[(int64_t)j * SPH_REC + SPH_R_H];
         nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, sr[0], sr[1], hb);
     }
-    const T base = accumulate ? (out_s ? out_s[t * out_stride] : T(rho[i])) : T(0.0);
-    const T val = base + nr;
+    return nr;
+}
+
+template <bool UH, typename T>
+__global__ void __launch_bounds__(OP_THREADS)
+    k_summation_density(int dim, int64_t nd_total, int64_t nd_real,
+                        const int32_t *__restrict__ dperm, SrcLists<T> L,
+                        const T *__restrict__ drec, int accumulate,
+                        double *__restrict__ rho, T *__restrict__ out_s, int64_t out_stride)
+{
+    const int64_t t =
+        (int64_t)sph_xcd_tile(blockIdx.x, gridDim.x) * OP_THREADS + threadIdx.x;
+    if (t >= nd_total) return;
+    const int32_t i = dperm[t];
+    if (i >= nd_real) {  // non-LocalReal destinations (sph_func.pyx:226-227)
+        rho[i] = 0;
+        if (out_s) out_s[t * out_stride] = 0;
+        return;
+    }
+    int32_t e[SPH_MAX_SRC];
+    int32_t any = 0;
+#pragma unroll
+    for (int s = 0; s < SPH_MAX_SRC; ++s) {
+        e[s] = s < L.nsrc ? L.counts[s][t] : 0;
+        any |= e[s];
+    }
+    T val = accumulate ? (out_s ? out_s[t * out_stride] : T(rho[i])) : T(0.0);
+    if (!any) {  // empty rows (a destination far from a second array): val + 0
+        val = val + T(0.0);
+    } else {
+        const auto *dr = rec2(drec, t);
+        const auto dxy = dr[0], dzm = dr[1];
+        const T xa = dxy.x, ya = dxy.y, za = dzm.x, ha = drec[t * SPH_REC + SPH_R_H];
+        const T facA = sph_h_dim(ha, dim) * sph_cubic_fac1<T>(dim);
+        for (int s = 0; s < L.nsrc; ++s) {
+            const int32_t *row = L.indices[s] + L.slice_off[s][t >> 6] + (t & 63);
+            val = val + rho_row<UH, T>(dim, xa, ya, za, ha, facA, row, e[s], L.srec[s]);
+        }
+    }
     rho[i] = val;
     if (out_s) out_s[t * out_stride] = val;
 }
 
 template <typename T>
 static int summation_density(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
-                             const int32_t *d_perm, const int32_t *counts,
-                             const int64_t *slice_off, const int32_t *indices, const T *d_rec,
-                             const T *s_rec, int uniform_h, int accumulate, double *rho,
-                             T *out_s, int64_t out_s_stride)
+                             const int32_t *d_perm, const sph_sources *src, const T *d_rec,
+                             int uniform_h, int accumulate, double *rho, T *out_s,
+                             int64_t out_s_stride)
 {
     if (!ctx) return SPH_EINVAL;
     SPH_REQUIRE(ctx, dim >= 1 && dim <= 3, "sph_summation_density: bad dim %d", dim);
     SPH_REQUIRE(ctx, nd_real >= 0 && nd_total >= nd_real, "sph_summation_density: bad n");
     if (nd_total == 0) return SPH_OK;
-    SPH_REQUIRE(ctx, rho && d_perm && counts && slice_off && d_rec &&
-                         (!out_s || out_s_stride >= 1),
+    SPH_REQUIRE(ctx, rho && d_perm && d_rec && (!out_s || out_s_stride >= 1),
                 "sph_summation_density: null argument");
+    SrcLists<T> L;
+    int rc = src_lists<T>(ctx, "sph_summation_density", src, &L);
+    if (rc) return rc;
     auto kern = uniform_h ? k_summation_density<true, T> : k_summation_density<false, T>;
     hipLaunchKernelGGL(kern, dim3(sph_blocks(nd_total, OP_THREADS)),
-                       dim3(OP_THREADS), 0, ctx->stream, dim, nd_total, nd_real, d_perm, counts,
-                       slice_off, indices, d_rec, s_rec, accumulate, rho, out_s, out_s_stride);
+                       dim3(OP_THREADS), 0, ctx->stream, dim, nd_total, nd_real, d_perm, L,
+                       d_rec, accumulate, rho, out_s, out_s_stride);
     SPH_LAUNCH_CHECK(ctx, "k_summation_density");
     return SPH_OK;
+}
+
+static sph_sources one_source(const int32_t *counts, const int64_t *slice_off,
+                              const int32_t *indices, const void *s_rec, int64_t ns)
+{
+    sph_sources src;
+    memset(&src, 0, sizeof(src));
+    src.nsrc = 1;
+    src.counts[0] = counts;
+    src.slice_off[0] = slice_off;
+    src.indices[0] = indices;
+    src.s_rec[0] = s_rec;
+    src.ns[0] = ns;
+    return src;
 }
 
 extern "C" int sph_summation_density(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
@@ -113,9 +172,9 @@ extern "C" int sph_summation_density(sph_ctx *ctx, int dim, int64_t nd_total, in
                                      int accumulate, double *rho, double *out_s,
                                      int64_t out_s_stride)
 {
-    return summation_density<double>(ctx, dim, nd_total, nd_real, d_perm, counts, slice_off,
-                                      indices, d_rec, s_rec, uniform_h, accumulate, rho, out_s,
-                                      out_s_stride);
+    const sph_sources src = one_source(counts, slice_off, indices, s_rec, 0);
+    return summation_density<double>(ctx, dim, nd_total, nd_real, d_perm, &src, d_rec,
+                                      uniform_h, accumulate, rho, out_s, out_s_stride);
 }
 
 extern "C" int sph_summation_density_f32(sph_ctx *ctx, int dim, int64_t nd_total,
@@ -125,9 +184,29 @@ extern "C" int sph_summation_density_f32(sph_ctx *ctx, int dim, int64_t nd_total
                                          const float *s_rec, int uniform_h, int accumulate,
                                          double *rho, float *out_s, int64_t out_s_stride)
 {
-    return summation_density<float>(ctx, dim, nd_total, nd_real, d_perm, counts, slice_off,
-                                     indices, d_rec, s_rec, uniform_h, accumulate, rho, out_s,
-                                     out_s_stride);
+    const sph_sources src = one_source(counts, slice_off, indices, s_rec, 0);
+    return summation_density<float>(ctx, dim, nd_total, nd_real, d_perm, &src, d_rec,
+                                     uniform_h, accumulate, rho, out_s, out_s_stride);
+}
+
+extern "C" int sph_summation_density_multi(sph_ctx *ctx, int dim, int64_t nd_total,
+                                           int64_t nd_real, const int32_t *d_perm,
+                                           const sph_sources *src, const double *d_rec,
+                                           int uniform_h, int accumulate, double *rho,
+                                           double *out_s, int64_t out_s_stride)
+{
+    return summation_density<double>(ctx, dim, nd_total, nd_real, d_perm, src, d_rec,
+                                      uniform_h, accumulate, rho, out_s, out_s_stride);
+}
+
+extern "C" int sph_summation_density_multi_f32(sph_ctx *ctx, int dim, int64_t nd_total,
+                                               int64_t nd_real, const int32_t *d_perm,
+                                               const sph_sources *src, const float *d_rec,
+                                               int uniform_h, int accumulate, double *rho,
+                                               float *out_s, int64_t out_s_stride)
+{
+    return summation_density<float>(ctx, dim, nd_total, nd_real, d_perm, src, d_rec,
+                                     uniform_h, accumulate, rho, out_s, out_s_stride);
 }
 
 // ---------------------------------------------------------------------------
@@ -256,53 +335,16 @@ int sph_tait_records_f32(sph_ctx *ctx, int64_t n, const int32_t *perm, float *re
 }
 
 // ---------------------------------------------------------------------------
-// MomentumEquation with Monaghan artificial viscosity
+// MomentumEquation with Monaghan artificial viscosity (mom_term in sph_terms.h)
 // ---------------------------------------------------------------------------
+// one SELL-64 row of MomentumEquation, summed from 0 in list order; dtf is
+// max-reduced over the row
 template <bool UH, typename T>
-__global__ void __launch_bounds__(OP_THREADS)
-    k_momentum(int dim, int64_t nd_total, int64_t nd_real, const int32_t *__restrict__ dperm,
-               const int32_t *__restrict__ counts, const int64_t *__restrict__ slice_off,
-               const int32_t *__restrict__ indices,
-               const T *__restrict__ drec, const T *__restrict__ srec, T alpha, T beta, T eta,
-               int accumulate, double *__restrict__ ax, double *__restrict__ ay,
-               double *__restrict__ az, double *__restrict__ dt_fac)
+__device__ __forceinline__ void mom_row(int dim, const MomDest<T> &D, T alpha, T beta, T eta,
+                                        const int32_t *__restrict__ row, int32_t e,
+                                        const T *__restrict__ srec, T &dtf, T &nr0, T &nr1,
+                                        T &nr2)
 {
-    const int64_t t =
-        (int64_t)sph_xcd_tile(blockIdx.x, gridDim.x) * OP_THREADS + threadIdx.x;
-    if (t >= nd_total) return;
-    const int32_t i = dperm[t];
-    if (i >= nd_real) {
-        ax[i] = 0;
-        ay[i] = 0;
-        if (az) az[i] = 0;
-        return;
-    }
-    const int32_t e = counts[t];
-    if (e == 0) {  // empty row: dt_fac keeps its value, a = base + 0
-        dt_fac[i] = T(dt_fac[i]);
-        ax[i] = (accumulate ? T(ax[i]) : T(0.0)) + T(0.0);
-        ay[i] = (accumulate ? T(ay[i]) : T(0.0)) + T(0.0);
-        if (az) az[i] = (accumulate ? T(az[i]) : T(0.0)) + T(0.0);
-        return;
-    }
-    const auto *dr = rec2(drec, t);
-    const auto d0 = dr[0], d1 = dr[1], d2 = dr[2], d3 = dr[3], d4 = dr[4], d5 = dr[5];
-    MomDest<T> D;
-    D.xa = d0.x;
-    D.ya = d0.y;
-    D.za = d1.x;
-    D.ua = d2.x;
-    D.va = d2.y;
-    D.wa = d3.x;
-    D.rhoa = d4.x;
-    D.ca = d4.y;
-    D.ha = d5.x;
-    D.Pa = d5.y;
-    D.facA = sph_h_dim(D.ha, dim) * sph_cubic_fac1<T>(dim);
-    D.rhoa2 = T(1.0) / (D.rhoa * D.rhoa);
-    T dtf = T(dt_fac[i]);
-    T nr0 = T(0.0), nr1 = T(0.0), nr2 = T(0.0);
-    const int32_t *row = indices + slice_off[t >> 6] + (t & 63);  // SELL-64 row
     int32_t k = 0;
     // MOM_NT neighbours' records in flight, then their terms added in list order
     for (; k + MOM_NT <= e; k += MOM_NT) {
@@ -334,10 +376,69 @@ __global__ void __launch_bounds__(OP_THREADS)
         mom_term<UH, T>(dim, D, alpha, beta, eta, sr[0], sr[1], sr[2], sr[3], sr[4], hb, dtf,
                         nr0, nr1, nr2);
     }
+}
+
+template <bool UH, typename T>
+__global__ void __launch_bounds__(OP_THREADS)
+    k_momentum(int dim, int64_t nd_total, int64_t nd_real, const int32_t *__restrict__ dperm,
+               SrcLists<T> L, const T *__restrict__ drec, T alpha, T beta, T eta,
+               int accumulate, double *__restrict__ ax, double *__restrict__ ay,
+               double *__restrict__ az, double *__restrict__ dt_fac)
+{
+    const int64_t t =
+        (int64_t)sph_xcd_tile(blockIdx.x, gridDim.x) * OP_THREADS + threadIdx.x;
+    if (t >= nd_total) return;
+    const int32_t i = dperm[t];
+    if (i >= nd_real) {
+        ax[i] = 0;
+        ay[i] = 0;
+        if (az) az[i] = 0;
+        return;
+    }
+    int32_t e[SPH_MAX_SRC];
+    int32_t any = 0;
+#pragma unroll
+    for (int s = 0; s < SPH_MAX_SRC; ++s) {
+        e[s] = s < L.nsrc ? L.counts[s][t] : 0;
+        any |= e[s];
+    }
+    T dtf = T(dt_fac[i]);
+    T o0 = accumulate ? T(ax[i]) : T(0.0);
+    T o1 = accumulate ? T(ay[i]) : T(0.0);
+    T o2 = (accumulate && az) ? T(az[i]) : T(0.0);
+    if (!any) {  // empty rows: dt_fac keeps its value, a = base + 0
+        o0 = o0 + T(0.0);
+        o1 = o1 + T(0.0);
+        o2 = o2 + T(0.0);
+    } else {
+        const auto *dr = rec2(drec, t);
+        const auto d0 = dr[0], d1 = dr[1], d2 = dr[2], d3 = dr[3], d4 = dr[4], d5 = dr[5];
+        MomDest<T> D;
+        D.xa = d0.x;
+        D.ya = d0.y;
+        D.za = d1.x;
+        D.ua = d2.x;
+        D.va = d2.y;
+        D.wa = d3.x;
+        D.rhoa = d4.x;
+        D.ca = d4.y;
+        D.ha = d5.x;
+        D.Pa = d5.y;
+        D.facA = sph_h_dim(D.ha, dim) * sph_cubic_fac1<T>(dim);
+        D.rhoa2 = T(1.0) / (D.rhoa * D.rhoa);
+        for (int s = 0; s < L.nsrc; ++s) {
+            T nr0 = T(0.0), nr1 = T(0.0), nr2 = T(0.0);
+            const int32_t *row = L.indices[s] + L.slice_off[s][t >> 6] + (t & 63);
+            mom_row<UH, T>(dim, D, alpha, beta, eta, row, e[s], L.srec[s], dtf, nr0, nr1, nr2);
+            o0 = o0 + nr0;
+            o1 = o1 + nr1;
+            o2 = o2 + nr2;
+        }
+    }
     dt_fac[i] = dtf;
-    ax[i] = (accumulate ? T(ax[i]) : T(0.0)) + nr0;
-    ay[i] = (accumulate ? T(ay[i]) : T(0.0)) + nr1;
-    if (az) az[i] = (accumulate ? T(az[i]) : T(0.0)) + nr2;
+    ax[i] = o0;
+    ay[i] = o1;
+    if (az) az[i] = o2;
 }
 
 // P/rho^2 of every source record, the per-pair term of pressure_funcs.pyx:302-305
@@ -353,8 +454,7 @@ template <typename T> __global__ void __launch_bounds__(OP_THREADS) k_prho2(int6
 
 template <typename T>
 static int momentum_av(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
-                       const int32_t *d_perm, const int32_t *counts, const int64_t *slice_off,
-                       const int32_t *indices, const T *d_rec, T *s_rec, int64_t ns,
+                       const int32_t *d_perm, const sph_sources *src, const T *d_rec,
                        int uniform_h, double alpha, double beta, double eta, int accumulate,
                        double *ax, double *ay, double *az, double *dt_fac)
 {
@@ -362,19 +462,20 @@ static int momentum_av(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
     SPH_REQUIRE(ctx, dim >= 1 && dim <= 3, "sph_momentum_av: bad dim %d", dim);
     SPH_REQUIRE(ctx, nd_real >= 0 && nd_total >= nd_real, "sph_momentum_av: bad n");
     if (nd_total == 0) return SPH_OK;
-    SPH_REQUIRE(ctx, d_rec && ax && ay && dt_fac && d_perm && counts && slice_off && ns >= 0 &&
-                         (ns == 0 || s_rec),
-                "sph_momentum_av: null argument");
-    if (ns > 0) {
-        hipLaunchKernelGGL(k_prho2<T>, dim3(sph_blocks(ns, OP_THREADS)), dim3(OP_THREADS), 0,
-                           ctx->stream, ns, s_rec);
+    SPH_REQUIRE(ctx, d_rec && ax && ay && dt_fac && d_perm, "sph_momentum_av: null argument");
+    SrcLists<T> L;
+    int rc = src_lists<T>(ctx, "sph_momentum_av", src, &L);
+    if (rc) return rc;
+    for (int s = 0; s < src->nsrc; ++s) {
+        if (src->ns[s] == 0) continue;
+        hipLaunchKernelGGL(k_prho2<T>, dim3(sph_blocks(src->ns[s], OP_THREADS)),
+                           dim3(OP_THREADS), 0, ctx->stream, src->ns[s], (T *)src->s_rec[s]);
         SPH_LAUNCH_CHECK(ctx, "k_prho2");
     }
     auto kern = uniform_h ? k_momentum<true, T> : k_momentum<false, T>;
     hipLaunchKernelGGL(kern, dim3(sph_blocks(nd_total, OP_THREADS)), dim3(OP_THREADS), 0,
-                       ctx->stream, dim, nd_total, nd_real, d_perm, counts, slice_off, indices,
-                       d_rec, (const T *)s_rec, T(alpha), T(beta), T(eta), accumulate, ax, ay,
-                       az, dt_fac);
+                       ctx->stream, dim, nd_total, nd_real, d_perm, L, d_rec, T(alpha),
+                       T(beta), T(eta), accumulate, ax, ay, az, dt_fac);
     SPH_LAUNCH_CHECK(ctx, "k_momentum");
     return SPH_OK;
 }
@@ -387,9 +488,9 @@ extern "C" int sph_momentum_av(sph_ctx *ctx, int dim, int64_t nd_total, int64_t 
                                int accumulate,
                                double *ax, double *ay, double *az, double *dt_fac)
 {
-    return momentum_av<double>(ctx, dim, nd_total, nd_real, d_perm, counts, slice_off, indices,
-                               d_rec, s_rec, ns, uniform_h, alpha, beta, eta, accumulate, ax, ay,
-                               az, dt_fac);
+    const sph_sources src = one_source(counts, slice_off, indices, s_rec, ns);
+    return momentum_av<double>(ctx, dim, nd_total, nd_real, d_perm, &src, d_rec, uniform_h,
+                               alpha, beta, eta, accumulate, ax, ay, az, dt_fac);
 }
 
 extern "C" int sph_momentum_av_f32(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
@@ -399,9 +500,30 @@ extern "C" int sph_momentum_av_f32(sph_ctx *ctx, int dim, int64_t nd_total, int6
                                    double alpha, double beta, double eta, int accumulate,
                                    double *ax, double *ay, double *az, double *dt_fac)
 {
-    return momentum_av<float>(ctx, dim, nd_total, nd_real, d_perm, counts, slice_off, indices,
-                              d_rec, s_rec, ns, uniform_h, alpha, beta, eta, accumulate, ax, ay,
-                              az, dt_fac);
+    const sph_sources src = one_source(counts, slice_off, indices, s_rec, ns);
+    return momentum_av<float>(ctx, dim, nd_total, nd_real, d_perm, &src, d_rec, uniform_h,
+                              alpha, beta, eta, accumulate, ax, ay, az, dt_fac);
+}
+
+extern "C" int sph_momentum_av_multi(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                                     const int32_t *d_perm, const sph_sources *src,
+                                     const double *d_rec, int uniform_h, double alpha,
+                                     double beta, double eta, int accumulate, double *ax,
+                                     double *ay, double *az, double *dt_fac)
+{
+    return momentum_av<double>(ctx, dim, nd_total, nd_real, d_perm, src, d_rec, uniform_h,
+                               alpha, beta, eta, accumulate, ax, ay, az, dt_fac);
+}
+
+extern "C" int sph_momentum_av_multi_f32(sph_ctx *ctx, int dim, int64_t nd_total,
+                                         int64_t nd_real, const int32_t *d_perm,
+                                         const sph_sources *src, const float *d_rec,
+                                         int uniform_h, double alpha, double beta, double eta,
+                                         int accumulate, double *ax, double *ay, double *az,
+                                         double *dt_fac)
+{
+    return momentum_av<float>(ctx, dim, nd_total, nd_real, d_perm, src, d_rec, uniform_h,
+                              alpha, beta, eta, accumulate, ax, ay, az, dt_fac);
 }
 
 // ---------------------------------------------------------------------------

@@ -12,6 +12,8 @@ The neighbour functions read the sorted shadows of their arrays (see
 base/nnps.py) and the sorted-space lists of their locator; results land in
 the caller's particle order and, for shadowed properties, in the shadow.
 """
+import ctypes
+
 import torch
 
 from .. import _lib, profiling
@@ -69,6 +71,41 @@ class SPHRho(CSPHFunctionParticle):
                 REC))
         if name:
             cm.written(self.dest, name, out_s is not None)
+
+    @staticmethod
+    def fusable(funcs):
+        return all(type(f) is SPHRho and f.dest is funcs[0].dest for f in funcs)
+
+    @staticmethod
+    def eval_multi(funcs, kernel, output1, output2, output3, accumulate=True, names=None):
+        """All functions of one calc (one per source array) in one launch:
+        the same sums, in the same order, as their eval calls in turn."""
+        f0 = funcs[0]
+        ctx = f0._ctx()
+        cm = f0.nbr_locator.cell_manager
+        p = _lib.ptr
+        src = _lib.Sources()
+        src.nsrc = len(funcs)
+        for k, f in enumerate(funcs):
+            loc = f.nbr_locator
+            cnt, soff, idx = loc.get_sorted_lists()
+            srec = cm.shadows(f.source, ('x', 'y', 'z', 'h', 'm'))
+            src.counts[k], src.slice_off[k], src.indices[k] = p(cnt), p(soff), p(idx)
+            src.s_rec[k], src.ns[k] = p(srec), 0
+            profiling.count_pairs(loc.pair_count())
+        name = names[0] if names else None
+        read = ('x', 'y', 'z', 'h') + ((name,) if name in REC_FIELDS and accumulate else ())
+        drec = cm.shadows(f0.dest, read)
+        out_s = cm.shadow_out(f0.dest, name) if name else None
+        fn = (ctx.lib.sph_summation_density_multi_f32 if cm.precision == "single"
+              else ctx.lib.sph_summation_density_multi)
+        with profiling.region("summation_density"):
+            ctx.check(fn(ctx.handle, kernel.dim, f0.dest.get_number_of_particles(),
+                         f0.dest.num_real_particles, p(cm.bins[f0.dest.name].perm),
+                         ctypes.byref(src), p(drec), int(cm.uniform_h()),
+                         int(bool(accumulate)), p(output1), out_s, REC))
+        if name:
+            cm.written(f0.dest, name, out_s is not None)
 
 
 class TaitEquation(SPHFunction):
@@ -194,6 +231,50 @@ class MomentumEquation(SPHFunctionParticle):
             if name:
                 cm.written(self.dest, name, False)
         cm.written(self.dest, 'dt_fac', False)
+
+    @staticmethod
+    def fusable(funcs):
+        f0 = funcs[0]
+        return all(type(f) is MomentumEquation and f.dest is f0.dest and
+                   (f.alpha, f.beta, f.eta) == (f0.alpha, f0.beta, f0.eta) for f in funcs)
+
+    @staticmethod
+    def eval_multi(funcs, kernel, output1, output2, output3, accumulate=True, names=None):
+        """All functions of one calc (one per source array) in one launch:
+        the same sums, in the same order, as their eval calls in turn."""
+        f0 = funcs[0]
+        ctx = f0._ctx()
+        cm = f0.nbr_locator.cell_manager
+        p = _lib.ptr
+        src = _lib.Sources()
+        src.nsrc = len(funcs)
+        fresh = []
+        for k, f in enumerate(funcs):
+            loc = f.nbr_locator
+            cnt, soff, idx = loc.get_sorted_lists()
+            srec = cm.shadows(f.source, f.SRC)
+            fr = cm.prho2_fresh(f.source) or f.source in [g.source for g in funcs[:k]]
+            fresh.append(fr)
+            src.counts[k], src.slice_off[k], src.indices[k] = p(cnt), p(soff), p(idx)
+            src.s_rec[k] = p(srec)
+            src.ns[k] = 0 if fr else cm.bins[f.source.name].n
+            profiling.count_pairs(loc.pair_count())
+        drec = cm.shadows(f0.dest, f0.DST)
+        fn = (ctx.lib.sph_momentum_av_multi_f32 if cm.precision == "single"
+              else ctx.lib.sph_momentum_av_multi)
+        with profiling.region("momentum_av"):
+            ctx.check(fn(ctx.handle, kernel.dim, f0.dest.get_number_of_particles(),
+                         f0.dest.num_real_particles, p(cm.bins[f0.dest.name].perm),
+                         ctypes.byref(src), p(drec), int(cm.uniform_h()), f0.alpha, f0.beta,
+                         f0.eta, int(bool(accumulate)), p(output1), p(output2), p(output3),
+                         p(f0.d('dt_fac'))))
+        for f, fr in zip(funcs, fresh):
+            if not fr:
+                cm.prho2_written(f.source)
+        for name in (names or ()):
+            if name:
+                cm.written(f0.dest, name, False)
+        cm.written(f0.dest, 'dt_fac', False)
 
 
 class GravityForce(SPHFunction):

@@ -92,9 +92,17 @@ class SPHCalc:
         for prop in self.to_reset:
             ctx.check(ctx.lib.sph_fill(ctx.handle, n, 0.0, _lib.ptr(self.dest.get_carray(prop))))
             self.dest.touch(prop)
-        for f in self.funcs:
-            f.eval(self.kernel, outs[0], outs[1], outs[2], accumulate=accumulate, names=names)
-            accumulate = True
+        funcs = self.funcs
+        multi = getattr(type(funcs[0]), "eval_multi", None) if funcs else None
+        if multi is not None and 1 < len(funcs) <= _lib.MAX_SRC and type(funcs[0]).fusable(funcs):
+            # one function per source array: one launch with the same sums
+            multi(funcs, self.kernel, outs[0], outs[1], outs[2], accumulate=accumulate,
+                  names=names)
+        else:
+            for f in funcs:
+                f.eval(self.kernel, outs[0], outs[1], outs[2], accumulate=accumulate,
+                       names=names)
+                accumulate = True
         if self.dest.is_dirty:
             self.particles.update()
 

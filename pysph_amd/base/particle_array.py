@@ -261,13 +261,19 @@ class ParticleArray:
                                                       dtype=torch.float64, device=self.device)
         self.touch(arr_name)
 
-    def get_property_arrays(self, all=False):
-        ret = {}
+    def output_property_names(self, all=False):
+        """The properties ``get_property_arrays`` returns: every one, or those
+        not starting with '_' plus _tmpx/_tmpy/_tmpz (particle_array.pyx:634-650)."""
         names = list(self.properties) if all else [
             p for p in self.properties if not p.startswith("_")] + ["_tmpx", "_tmpy", "_tmpz"]
-        for p in names:
-            if p in self.properties:
-                ret[p] = self.properties[p].detach().cpu().numpy().copy()
+        return [p for p in names if p in self.properties]
+
+    def get_property_arrays(self, all=False):
+        """Host copies of the output properties, then the constants
+        (particle_array.pyx:634-650)."""
+        ret = {}
+        for p in self.output_property_names(all):
+            ret[p] = self.properties[p].detach().cpu().numpy().copy()
         ret.update(self.constants)
         return ret
 

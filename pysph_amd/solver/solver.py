@@ -1,5 +1,6 @@
 """Solver: operation registry, setup and the time loop (solver/solver.py:33-775)."""
 import logging
+import os
 
 from ..base.kernels import CubicSplineKernel
 from ..sph.funcs import PositionStepping, XSPHCorrection
@@ -42,6 +43,12 @@ class Solver:
             self.print_properties.extend(['y', 'v'])
         if self.dim > 2:
             self.print_properties.extend(['z', 'w'])
+        self.detailed_output = False
+        self.fname = self.__class__.__name__
+        self.output_directory = self.fname + '_output'
+        # the reference always writes; False keeps solve() from writing files
+        self.output_enabled = True
+        self._writer = None
 
     def switch_integrator(self, integrator_type):
         if self.particles is None:
@@ -129,9 +136,95 @@ class Solver:
     def set_time_step_function(self, time_step_function):
         self.time_step_function = time_step_function
 
+    def set_output_fname(self, fname):
+        """solver.py:450-452"""
+        self.fname = fname
+
+    def set_output_printing_level(self, detailed_output):
+        """solver.py:454-456"""
+        self.detailed_output = detailed_output
+
+    def set_output_directory(self, path):
+        """solver.py:458-460"""
+        self.output_directory = path
+
+    def set_output(self, enabled):
+        """Switch the files solve() writes on or off (no reference counterpart)."""
+        self.output_enabled = bool(enabled)
+
+    def dump_output(self, dt, *print_properties):
+        """Write ``<output_directory>/<fname>_<count>.npz`` in the version-1
+        layout (solver.py:605-699): every array's properties not starting
+        with '_' plus _tmpx/_tmpy/_tmpz and its constants (all properties with
+        detailed output), and ``solver_data = {dt, t, count}``.
+
+        The file is written by a worker thread from host copies taken on the
+        operators' stream; ``wait_output`` returns once every file is
+        complete, and solve() calls it before returning. ``print_properties``
+        is accepted and unused, as in the reference."""
+        from . import utils
+        if self.particles is None:
+            raise RuntimeError("There are no particles!")
+        utils.mkdir(self.output_directory)
+        arrays = {}
+        for array in self.particles.arrays:
+            props = {p: array.properties[p]
+                     for p in array.output_property_names(all=self.detailed_output)}
+            props.update(array.constants)
+            arrays[array.name] = props
+        path = os.path.join(self.output_directory, self.fname + '_' + str(self.count) + '.npz')
+        if self._writer is None:
+            self._writer = utils.OutputWriter()
+        self._writer.submit(path, arrays, {"dt": dt, "t": self.t, "count": self.count})
+        return path
+
+    def wait_output(self):
+        """Block until every file dump_output started is written."""
+        if self._writer is not None:
+            self._writer.wait()
+
+    def load_output(self, count):
+        """Resume from ``<fname>_<count>.npz`` (solver.py:701-743).
+
+        ``count == '?'`` returns the available counts, ascending. Otherwise
+        every particle array of the solver takes the properties and constants
+        stored under its name (its particle type, precision and device stay),
+        the particle manager is initialised again, ``t`` and ``count`` are
+        restored (``dt`` is not, as in the reference), and an integrator that
+        was already set up is rebuilt on the loaded arrays."""
+        from . import utils
+        self.wait_output()
+        available = utils.output_counts(self.output_directory, self.fname)
+        if count == '?':
+            return available
+        if str(count) not in available:
+            msg = "File with iteration count `%s` does not exist" % (count)
+            msg += "\nValid iteration counts are %s" % (available)
+            raise IOError(msg)
+        if self.particles is None:
+            raise RuntimeError("There are no particles!")
+        data = utils.read_output(os.path.join(self.output_directory,
+                                              self.fname + '_' + str(count) + '.npz'))
+        stored = data["arrays"]
+        missing = [a.name for a in self.particles.arrays if a.name not in stored]
+        if missing:
+            raise KeyError(f"output file has no array named {missing[0]!r}")
+        for array in self.particles.arrays:
+            props = stored[array.name]
+            array.initialize(**utils.property_infos(props))
+            array.constants.update(utils.constants_of(props))
+        self.particles.initialize()
+        self.t = float(data["solver_data"]['t'])
+        self.count = int(data["solver_data"]['count'])
+        if getattr(self, 'integrator', None) is not None:
+            self.setup(self.particles)
+        return None
+
     def solve(self, show_progress=False, nsteps=None):
         """The time loop (solver.py:523-590); ``nsteps`` bounds the step count."""
         dt = self.dt
+        if self.output_enabled:
+            self.dump_output(dt, *self.print_properties)
         self.integrator.time = self.t
         steps = 0
         while self.t < self.tf if nsteps is None else steps < nsteps:
@@ -147,5 +240,8 @@ class Solver:
                 array.set_time(self.t)
             for func in self.post_step_functions:
                 func.eval(self)
+            if self.output_enabled and self.pfreq and self.count % self.pfreq == 0:
+                self.dump_output(dt, *self.print_properties)
             steps += 1
+        self.wait_output()
         return steps

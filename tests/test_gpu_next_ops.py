@@ -89,6 +89,7 @@ def test_time_step_functions(gpu):
     s.set_time_step_function(ts)
     s.set_final_time(1.0)
     s.set_time_step(1e-5)
+    s.set_output(False)
     s.solve(nsteps=3)   # first step: the force bound uses dt as fmax (reference quirk)
     fl = particles.arrays[0]
     h, dtf = fl.get('h'), fl.get('dt_fac')

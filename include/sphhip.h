@@ -364,6 +364,57 @@ int sph_xsph_f32(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
                  const int32_t *indices, const float *d_rec, const float *s_rec, int uniform_h,
                  double eps, int accumulate, double *ox, double *oy, double *oz);
 
+/* Split-form pair operators and the wall force, over the same lists and
+ * records as sph_momentum_av (one launch for all source arrays of a calc,
+ * summed per source from 0 in list order and added in source order; outputs
+ * 0 for non-LocalReal destinations, sph_func.pyx:208-227):
+ *   SPH_PAIR_PRESSURE_GRADIENT     SPHPressureGradient.eval_nbr
+ *                                  (sph/funcs/pressure_funcs.pyx:61-151)
+ *   SPH_PAIR_ARTIFICIAL_VISCOSITY  MonaghanArtificialViscosity.eval_nbr
+ *                                  (sph/funcs/viscosity_funcs.pyx:62-151), dt_fac
+ *                                  max-reduced with the 0.01 constant of :99
+ *   SPH_PAIR_MOMENTUM              MomentumEquation.eval_nbr with the artificial
+ *                                  pressure branch (pressure_funcs.pyx:246-391,
+ *                                  :327-365), dt_fac max-reduced
+ *   SPH_PAIR_BOUNDARY_FORCE        MonaghanBoundaryForce.eval_nbr
+ *                                  (sph/funcs/boundary_funcs.pyx:55-107); the
+ *                                  sources' tangent/normal frames come from
+ *                                  `frames` (original order, addressed through
+ *                                  each source's sorted->original permutation)
+ * with_correction switches the artificial pressure term on (the functions'
+ * deltap argument, pressure_funcs.pyx:27-40, :185-205).  A pair whose
+ * reference arithmetic divides by zero (W(deltap) == 0, a wall distance y ==
+ * 0, delp == 0, m_a + m_b == 0) raises in the reference's Cython and adds
+ * nothing; it adds nothing here.  ns of each source: records whose P/rho^2
+ * field is refreshed first (pressure gradient and momentum; 0 when current).
+ * dt_fac may be NULL for the pressure gradient and the wall force. */
+#define SPH_PAIR_PRESSURE_GRADIENT 1
+#define SPH_PAIR_ARTIFICIAL_VISCOSITY 2
+#define SPH_PAIR_MOMENTUM 3
+#define SPH_PAIR_BOUNDARY_FORCE 4
+typedef struct sph_pair_params {
+    int32_t op;              /* SPH_PAIR_* */
+    int32_t with_correction; /* artificial pressure on (gradient, momentum) */
+    double alpha, beta, eta; /* viscosity; the viscosity function's eta is 0
+                                unless set after construction (viscosity_funcs.pyx:30-43) */
+    double deltap, n, eps;   /* artificial pressure */
+    double delp;             /* wall force: reach along the wall */
+} sph_pair_params;
+typedef struct sph_wall_frames {
+    const int32_t *s_perm[SPH_MAX_SRC]; /* sorted slot -> source particle index */
+    const double *tx[SPH_MAX_SRC], *ty[SPH_MAX_SRC], *tz[SPH_MAX_SRC];
+    const double *nx[SPH_MAX_SRC], *ny[SPH_MAX_SRC], *nz[SPH_MAX_SRC];
+} sph_wall_frames;
+int sph_pair_accel(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                   const int32_t *d_perm, const sph_sources *src, const double *d_rec,
+                   int uniform_h, const sph_pair_params *prm, const sph_wall_frames *frames,
+                   int accumulate, double *ax, double *ay, double *az, double *dt_fac);
+int sph_pair_accel_f32(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                       const int32_t *d_perm, const sph_sources *src, const float *d_rec,
+                       int uniform_h, const sph_pair_params *prm,
+                       const sph_wall_frames *frames, int accumulate, double *ax, double *ay,
+                       double *az, double *dt_fac);
+
 /* Reductions of the time-step functions (solver/time_step_functions.py:8-57,
  * Integrator.get_max_acceleration, solver/integrator.py:369-412): over n
  * elements, out[0] = max |(a, b, c)| (sqrt(a*a + b*b [+ c*c]) with ncomp

@@ -60,6 +60,17 @@ def _load():
     lib.sphref_density_rate.restype = None
     lib.sphref_xsph.argtypes = [i32, i64] + [vp] * 19 + [dbl, vp, vp, vp, i32]
     lib.sphref_xsph.restype = None
+    lib.sphref_pressure_gradient.argtypes = ([i32, i64] + [vp] * 15 + [i32, dbl, dbl, dbl] +
+                                             [vp] * 3 + [i32])
+    lib.sphref_pressure_gradient.restype = None
+    lib.sphref_artificial_viscosity.argtypes = ([i32, i64] + [vp] * 21 + [dbl, dbl, dbl] +
+                                                [vp] * 4 + [i32])
+    lib.sphref_artificial_viscosity.restype = None
+    lib.sphref_momentum_deltap.argtypes = ([i32, i64] + [vp] * 23 + [dbl] * 6 + [vp] * 4 +
+                                           [i32])
+    lib.sphref_momentum_deltap.restype = None
+    lib.sphref_boundary_force.argtypes = [i64] + [vp] * 18 + [dbl, vp, vp, vp, i32]
+    lib.sphref_boundary_force.restype = None
     lib.sphref_gravity.argtypes = [i64, dbl, dbl, dbl, vp, vp, vp]
     lib.sphref_gravity.restype = None
     lib.sphref_step.argtypes = [i64, vp, vp, dbl, vp]
@@ -193,6 +204,55 @@ def xsph(dim, dst, src, offsets, indices, eps, ox, oy, oz, nthreads=1):
                     _p(indices), float(eps), _p(ox), _p(oy), _p(oz), nthreads)
 
 
+def pressure_gradient(dim, dst, src, offsets, indices, ax, ay, az, deltap=None, n=1.0,
+                      eps=0.2, nthreads=1):
+    """a += -m_b (P_a/rho_a^2 + P_b/rho_b^2 [- artificial pressure]) gradW
+    (sph/funcs/pressure_funcs.pyx:61-151)."""
+    lib = _load()
+    nd = len(offsets) - 1
+    d = [_f64(dst[k]) for k in ("x", "y", "z", "h", "rho", "p")]
+    s = [_f64(src[k]) for k in ("x", "y", "z", "h", "m", "rho", "p")]
+    lib.sphref_pressure_gradient(int(dim), nd, *[_p(c) for c in d], *[_p(c) for c in s],
+                                 _p(offsets), _p(indices), int(deltap is not None),
+                                 float(deltap or 0.0), float(n), float(eps), _p(ax), _p(ay),
+                                 _p(az), nthreads)
+
+
+def artificial_viscosity(dim, dst, src, offsets, indices, alpha, beta, eta, ax, ay, az,
+                         dt_fac, nthreads=1):
+    """Monaghan Pi_ab gradW with dt_fac (sph/funcs/viscosity_funcs.pyx:62-151)."""
+    lib = _load()
+    nd = len(offsets) - 1
+    d = [_f64(dst[k]) for k in ("x", "y", "z", "h", "rho", "u", "v", "w", "cs")]
+    s = [_f64(src[k]) for k in ("x", "y", "z", "h", "m", "rho", "u", "v", "w", "cs")]
+    lib.sphref_artificial_viscosity(int(dim), nd, *[_p(c) for c in d], *[_p(c) for c in s],
+                                    _p(offsets), _p(indices), float(alpha), float(beta),
+                                    float(eta), _p(ax), _p(ay), _p(az), _p(dt_fac), nthreads)
+
+
+def momentum_deltap(dim, dst, src, offsets, indices, alpha, beta, eta, deltap, n, eps, ax, ay,
+                    az, dt_fac, nthreads=1):
+    """MomentumEquation with artificial pressure (sph/funcs/pressure_funcs.pyx:246-391)."""
+    lib = _load()
+    nd = len(offsets) - 1
+    d = [_f64(dst[k]) for k in MOM_PROPS]
+    s = [_f64(src[k]) for k in MOM_SRC_PROPS]
+    lib.sphref_momentum_deltap(int(dim), nd, *[_p(c) for c in d], *[_p(c) for c in s],
+                               _p(offsets), _p(indices), float(alpha), float(beta), float(eta),
+                               float(deltap), float(n), float(eps), _p(ax), _p(ay), _p(az),
+                               _p(dt_fac), nthreads)
+
+
+def boundary_force(dst, src, offsets, indices, delp, ax, ay, az, nthreads=1):
+    """Monaghan wall force from tangent/normal frames (sph/funcs/boundary_funcs.pyx:55-107)."""
+    lib = _load()
+    nd = len(offsets) - 1
+    d = [_f64(dst[k]) for k in ("x", "y", "z", "h", "m", "cs")]
+    s = [_f64(src[k]) for k in ("x", "y", "z", "m", "tx", "ty", "tz", "nx", "ny", "nz")]
+    lib.sphref_boundary_force(nd, *[_p(c) for c in d], *[_p(c) for c in s], _p(offsets),
+                              _p(indices), float(delp), _p(ax), _p(ay), _p(az), nthreads)
+
+
 def gravity(n, g, ax, ay, az):
     _load().sphref_gravity(int(n), float(g[0]), float(g[1]), float(g[2]), _p(ax), _p(ay), _p(az))
 
@@ -242,12 +302,20 @@ class RefSimulation:
       5. PositionStepping  on fluids, integrates x,y(,z)
       6. XSPHCorrection    on fluids from fluids (if ``xsph_eps`` given), integrates
          x,y(,z) (Solver.add_operation_xsph, solver.py:186-220)
+
+    ``momentum="split"`` replaces 3. by SPHPressureGradient from ``pgrad_from``
+    types followed by MonaghanArtificialViscosity (its eta: ``avisc_eta``) from
+    ``momentum_from`` types; ``deltap`` (with ``ap_n``, ``ap_eps``) switches the
+    artificial pressure of the momentum equation or the pressure gradient on;
+    ``bforce_from`` types add MonaghanBoundaryForce(delp=``bforce_delp``) after
+    gravity (the order of sph/examples' dam_break_simple.py).
     """
 
     def __init__(self, arrays, dim, integrator="rk2", co=1.0, ro=1000.0, gamma=7.0,
                  alpha=1.0, beta=1.0, eta=0.1, gravity=None, density_from=(0,),
                  momentum_from=(0,), cell_radius_scale=2.0, nthreads=1, density="summation",
-                 xsph_eps=None):
+                 xsph_eps=None, momentum="full", pgrad_from=(0,), avisc_eta=0.0, deltap=None,
+                 ap_n=1.0, ap_eps=0.2, bforce_from=(), bforce_delp=1.0):
         self.arrays = arrays
         self.dim = dim
         self.integrator = integrator
@@ -268,6 +336,14 @@ class RefSimulation:
         if density == "continuity":
             self.stepped = ["rho"] + self.stepped
         self.xsph_eps = xsph_eps
+        if momentum not in ("full", "split"):
+            raise ValueError(momentum)
+        self.momentum = momentum
+        self.pgrad_from = pgrad_from if momentum == "split" else ()
+        self.avisc_eta = avisc_eta
+        self.deltap, self.ap_n, self.ap_eps = deltap, ap_n, ap_eps
+        self.bforce_from = bforce_from
+        self.bforce_delp = bforce_delp
         self.lists = None
         self.dirty = True
         self.pairs = 0
@@ -282,8 +358,9 @@ class RefSimulation:
         self.lists = {}
         for d in self.fluids:
             for s in self.arrays:
-                if s.ptype in set(self.density_from) | set(self.momentum_from) or (
-                        self.xsph_eps is not None and s.ptype == 0):
+                used = (set(self.density_from) | set(self.momentum_from) |
+                        set(self.pgrad_from) | set(self.bforce_from))
+                if s.ptype in used or (self.xsph_eps is not None and s.ptype == 0):
                     self.lists[(d.name, s.name)] = nnps(d, s, 2.0, cs, d is s, self.nthreads)
         self.dirty = False
 
@@ -322,14 +399,37 @@ class RefSimulation:
                 a[k][:] = 0.0
             d["dt_fac"][:] = 0.0
             az = a["w"] if dim == 3 else np.zeros(d.n)
+            if self.momentum == "split":
+                for s in self.arrays:
+                    if s.ptype in self.pgrad_from:
+                        off, idx = lists[(d.name, s.name)]
+                        pressure_gradient(dim, d, s, off, idx, a["u"], a["v"], az,
+                                          deltap=self.deltap, n=self.ap_n, eps=self.ap_eps,
+                                          nthreads=self.nthreads)
+                        self.pairs += len(idx)
             for s in self.arrays:
                 if s.ptype in self.momentum_from:
                     off, idx = lists[(d.name, s.name)]
-                    momentum(dim, d, s, off, idx, self.alpha, self.beta, self.eta,
-                             a["u"], a["v"], az, d["dt_fac"], self.nthreads)
+                    if self.momentum == "split":
+                        artificial_viscosity(dim, d, s, off, idx, self.alpha, self.beta,
+                                             self.avisc_eta, a["u"], a["v"], az, d["dt_fac"],
+                                             self.nthreads)
+                    elif self.deltap is not None:
+                        momentum_deltap(dim, d, s, off, idx, self.alpha, self.beta, self.eta,
+                                        self.deltap, self.ap_n, self.ap_eps, a["u"], a["v"],
+                                        az, d["dt_fac"], self.nthreads)
+                    else:
+                        momentum(dim, d, s, off, idx, self.alpha, self.beta, self.eta,
+                                 a["u"], a["v"], az, d["dt_fac"], self.nthreads)
                     self.pairs += len(idx)
             if self.gravity is not None:
                 gravity(d.n, self.gravity, a["u"], a["v"], az)
+            for s in self.arrays:
+                if s.ptype in self.bforce_from:
+                    off, idx = lists[(d.name, s.name)]
+                    boundary_force(d, s, off, idx, self.bforce_delp, a["u"], a["v"], az,
+                                   self.nthreads)
+                    self.pairs += len(idx)
             for vk, xk in zip(self.vel, self.pos):
                 a[xk][:] = d[vk]
             if self.xsph_eps is not None:

@@ -25,6 +25,10 @@
  *   Momentum + AV  pressure + Monaghan Pi_ab, dt_fac  sph/funcs/pressure_funcs.pyx:185-391
  *   DensityRate    sum (va-vb).gradW m_b               sph/funcs/density_funcs.pyx:116-164
  *   XSPH           sum m_b W/rho_ab (vb-va) eps        sph/funcs/xsph_funcs.pyx:36-91
+ *   PressureGrad   -m_b(Pa/ra^2+Pb/rb^2)+art. press.  sph/funcs/pressure_funcs.pyx:19-168
+ *   Monaghan AV    Pi_ab gradW, dt_fac (0.01, eta=0)  sph/funcs/viscosity_funcs.pyx:19-157
+ *   Momentum deltap artificial pressure branch       sph/funcs/pressure_funcs.pyx:327-365
+ *   BoundaryForce  Monaghan wall force (tangent/normal) sph/funcs/boundary_funcs.pyx:9-107
  *   Gravity        += (gx,gy,gz)                      sph/funcs/external_force.pyx:38-44
  *   PositionStep   a_x = u                            sph/funcs/position_funcs.pyx:29-43
  *   stage          x = x0 + a*dt ; PC final 2x - x0   solver/integrator.py:340-367, :632-683
@@ -480,6 +484,265 @@ void sphref_xsph(int dim, int64_t nd_real,
         ox[i] += nr0;
         if (oy) oy[i] += nr1;
         if (oz) oz[i] += nr2;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* split-form operators and wall forces                                 */
+/* ------------------------------------------------------------------ */
+/* The reference compiles these modules without cdivision, so a float
+ * division by zero raises ZeroDivisionError inside a `cdef void eval_nbr`,
+ * which Cython reports and ignores: the pair adds nothing from that point on.
+ * The divisions that can plausibly meet a zero (W at deltap, the wall
+ * distance y, delp, m_a + m_b, h) model that as "skip the rest of the pair". */
+
+/* artificial pressure term of pressure_funcs.pyx:92-140 (hks off): returns 0
+ * when the pair raised (W(deltap) == 0), else 1 with *fab set. */
+static int art_pressure_fab(int dim, double pax, double pay, double paz, double pbx,
+                            double pby, double pbz, double h, double deltap, double n,
+                            double *fab)
+{
+    double w = cubic_w(dim, pax, pay, paz, pbx, pby, pbz, h);
+    double wdeltap = cubic_w(dim, deltap, 0.0, 0.0, 0.0, 0.0, 0.0, h);
+    if (wdeltap == 0.0) return 0;
+    double f = w / wdeltap;
+    *fab = pow(f, n);
+    return 1;
+}
+
+/* SPHPressureGradient.eval_nbr, sph/funcs/pressure_funcs.pyx:61-151 (hks
+ * off, no kernel corrections), summed by SPHFunctionParticle.eval_single
+ * (sph/sph_func.pyx:427-441); out += result.  with_correction: deltap given
+ * (:27-40). */
+void sphref_pressure_gradient(int dim, int64_t nd_real,
+                              const double *dx, const double *dy, const double *dz,
+                              const double *dh, const double *drho, const double *dp,
+                              const double *sx, const double *sy, const double *sz,
+                              const double *sh, const double *sm, const double *srho,
+                              const double *sp, const int64_t *offsets, const int32_t *indices,
+                              int with_correction, double deltap, double n, double eps,
+                              double *ax, double *ay, double *az, int nthreads)
+{
+    (void)nthreads;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int64_t i = 0; i < nd_real; ++i) {
+        double nr0 = 0.0, nr1 = 0.0, nr2 = 0.0;
+        for (int64_t k = offsets[i]; k < offsets[i + 1]; ++k) {
+            int32_t j = indices[k];
+            double mb = sm[j], rhoa = drho[i], rhob = srho[j], pa = dp[i], pb = sp[j];
+            double ha = dh[i], hb = sh[j];
+            double h = 0.5 * (ha + hb);
+            double temp = 0.0, artificial_stress = 0.0;
+            double rhoa2 = 1.0 / (rhoa * rhoa);
+            double rhob2 = 1.0 / (rhob * rhob);
+            temp = -mb * (pa * rhoa2 + pb * rhob2);
+            if (with_correction) {
+                double Ra = pa < 0 ? -eps * pa : 0.0;
+                Ra = Ra * rhoa2;
+                double Rb = pb < 0 ? -eps * pb : 0.0;
+                Rb = Rb * rhob2;
+                double fab;
+                if (!art_pressure_fab(dim, dx[i], dy[i], dz[i], sx[j], sy[j], sz[j], h, deltap,
+                                      n, &fab))
+                    continue;
+                artificial_stress = mb * (Ra + Rb) * fab;
+            }
+            temp = temp + artificial_stress;
+            double g[3];
+            cubic_gradw(dim, dx[i], dy[i], dz[i], sx[j], sy[j], sz[j], h, g);
+            nr0 += temp * g[0];
+            nr1 += temp * g[1];
+            nr2 += temp * g[2];
+        }
+        ax[i] += nr0;
+        ay[i] += nr1;
+        if (az) az[i] += nr2;
+    }
+}
+
+/* MonaghanArtificialViscosity.eval_nbr, sph/funcs/viscosity_funcs.pyx:62-151
+ * (hks off).  eta is the function's attribute: its constructor takes an eta
+ * argument but never stores it (:30-43), so it is 0 unless set afterwards.
+ * dt_fac uses the constant 0.01 (:99).  out += result; dt_fac max-reduced. */
+void sphref_artificial_viscosity(int dim, int64_t nd_real,
+                                 const double *dx, const double *dy, const double *dz,
+                                 const double *dh, const double *drho, const double *du,
+                                 const double *dv, const double *dw, const double *dcs,
+                                 const double *sx, const double *sy, const double *sz,
+                                 const double *sh, const double *sm, const double *srho,
+                                 const double *su, const double *sv, const double *sw,
+                                 const double *scs, const int64_t *offsets,
+                                 const int32_t *indices, double alpha, double beta, double eta,
+                                 double *ax, double *ay, double *az, double *dt_fac,
+                                 int nthreads)
+{
+    (void)nthreads;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int64_t i = 0; i < nd_real; ++i) {
+        double nr0 = 0.0, nr1 = 0.0, nr2 = 0.0;
+        double dtf = dt_fac[i];
+        for (int64_t k = offsets[i]; k < offsets[i + 1]; ++k) {
+            int32_t j = indices[k];
+            double ha = dh[i], hb = sh[j];
+            double hab = 0.5 * (ha + hb);
+            double rabx = dx[i] - sx[j], raby = dy[i] - sy[j], rabz = dz[i] - sz[j];
+            double vabx = du[i] - su[j], vaby = dv[i] - sv[j], vabz = dw[i] - sw[j];
+            double ca = dcs[i], cb = scs[j];
+            double dot = vabx * rabx + vaby * raby + vabz * rabz;
+            double rab2 = rabx * rabx + raby * raby + rabz * rabz;
+            double f = fabs(hab * dot / (rab2 + 0.01 * hab * hab));
+            if (f > dtf) dtf = f;
+            double rhoa = drho[i], rhob = srho[j], mb = sm[j];
+            double piab = 0;
+            if (dot < 0) {
+                double cab = 0.5 * (ca + cb);
+                double rhoab = 0.5 * (rhoa + rhob);
+                double mu = hab * dot;
+                double den = rab2 + eta * eta * hab * hab;
+                if (den == 0.0) continue;
+                mu /= den;
+                piab = -alpha * cab * mu + beta * mu * mu;
+                piab /= rhoab;
+            }
+            double tmp = piab;
+            tmp *= -mb;
+            double g[3];
+            cubic_gradw(dim, dx[i], dy[i], dz[i], sx[j], sy[j], sz[j], hab, g);
+            nr0 += tmp * g[0];
+            nr1 += tmp * g[1];
+            nr2 += tmp * g[2];
+        }
+        dt_fac[i] = dtf;
+        ax[i] += nr0;
+        ay[i] += nr1;
+        if (az) az[i] += nr2;
+    }
+}
+
+/* MomentumEquation.eval_nbr with artificial pressure, pressure_funcs.pyx:
+ * 246-391 with the deltap branch :327-365 (hks off).  A pair whose W(deltap)
+ * is 0 raises after dt_fac was updated: dt_fac keeps the update, the
+ * acceleration gets nothing from that pair. */
+void sphref_momentum_deltap(int dim, int64_t nd_real,
+                            const double *dx, const double *dy, const double *dz,
+                            const double *dh, const double *drho, const double *dp,
+                            const double *du, const double *dv, const double *dw,
+                            const double *dcs, const double *sx, const double *sy,
+                            const double *sz, const double *sh, const double *sm,
+                            const double *srho, const double *sp, const double *su,
+                            const double *sv, const double *sw, const double *scs,
+                            const int64_t *offsets, const int32_t *indices, double alpha,
+                            double beta, double eta, double deltap, double n, double eps,
+                            double *ax, double *ay, double *az, double *dt_fac, int nthreads)
+{
+    (void)nthreads;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int64_t i = 0; i < nd_real; ++i) {
+        double nr0 = 0.0, nr1 = 0.0, nr2 = 0.0;
+        double dtf = dt_fac[i];
+        for (int64_t k = offsets[i]; k < offsets[i + 1]; ++k) {
+            int32_t j = indices[k];
+            double ha = dh[i], hb = sh[j];
+            double hab = 0.5 * (ha + hb);
+            double ca = dcs[i], cb = scs[j];
+            double rabx = dx[i] - sx[j], raby = dy[i] - sy[j], rabz = dz[i] - sz[j];
+            double vabx = du[i] - su[j], vaby = dv[i] - sv[j], vabz = dw[i] - sw[j];
+            double dot = vabx * rabx + vaby * raby + vabz * rabz;
+            double rab2 = rabx * rabx + raby * raby + rabz * rabz;
+            double f = fabs(hab * dot / (rab2 + eta * eta * hab * hab));
+            if (f > dtf) dtf = f;
+            double Pa = dp[i], rhoa = drho[i];
+            double Pb = sp[j], rhob = srho[j], mb = sm[j];
+            double rhoa2 = 1.0 / (rhoa * rhoa);
+            double rhob2 = 1.0 / (rhob * rhob);
+            double tmp = Pa * rhoa2 + Pb * rhob2;
+            double piab = 0;
+            if (dot < 0) {
+                double cab = 0.5 * (ca + cb);
+                double rhoab = 0.5 * (rhoa + rhob);
+                double mu = hab * dot;
+                mu /= (rab2 + eta * eta * hab * hab);
+                piab = -alpha * cab * mu + beta * mu * mu;
+                piab /= rhoab;
+            }
+            tmp += piab;
+            tmp *= -mb;
+            double Ra = 0.0;
+            if (Pa < 0) Ra = -eps * Pa;
+            Ra = Ra * rhoa2;
+            double Rb = 0.0;
+            if (Pb < 0) Rb = -eps * Pb;
+            Rb = Rb * rhob2;
+            double fab;
+            if (!art_pressure_fab(dim, dx[i], dy[i], dz[i], sx[j], sy[j], sz[j], hab, deltap, n,
+                                  &fab))
+                continue;
+            tmp = tmp + mb * (Ra + Rb) * fab;
+            double g[3];
+            cubic_gradw(dim, dx[i], dy[i], dz[i], sx[j], sy[j], sz[j], hab, g);
+            nr0 += tmp * g[0];
+            nr1 += tmp * g[1];
+            nr2 += tmp * g[2];
+        }
+        dt_fac[i] = dtf;
+        ax[i] += nr0;
+        ay[i] += nr1;
+        if (az) az[i] += nr2;
+    }
+}
+
+/* MonaghanBoundaryForce.eval_nbr, sph/funcs/boundary_funcs.pyx:55-107:
+ * tangent/normal frames of the wall (source) particle, the destination's h,
+ * m and cs. */
+void sphref_boundary_force(int64_t nd_real,
+                           const double *dx, const double *dy, const double *dz,
+                           const double *dh, const double *dm, const double *dcs,
+                           const double *sx, const double *sy, const double *sz,
+                           const double *sm, const double *stx, const double *sty,
+                           const double *stz, const double *snx, const double *sny,
+                           const double *snz, const int64_t *offsets, const int32_t *indices,
+                           double delp, double *ax, double *ay, double *az, int nthreads)
+{
+    (void)nthreads;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int64_t i = 0; i < nd_real; ++i) {
+        double nr0 = 0.0, nr1 = 0.0, nr2 = 0.0;
+        for (int64_t k = offsets[i]; k < offsets[i + 1]; ++k) {
+            int32_t j = indices[k];
+            double h = dh[i], ma = dm[i], mb = sm[j];
+            double nx = snx[j], ny = sny[j], nz = snz[j];
+            double tx = stx[j], ty = sty[j], tz = stz[j];
+            double cs = dcs[i];
+            double rx = dx[i] - sx[j], ry = dy[i] - sy[j], rz = dz[i] - sz[j];
+            double x = rx * tx + ry * ty + rz * tz;
+            double y = rx * nx + ry * ny + rz * nz;
+            double force = 0.0;
+            if (h == 0.0) continue;
+            double q = y / h;
+            if (0 <= fabs(x) && fabs(x) <= delp) {
+                if (y == 0.0) continue;
+                double beta = 0.02 * cs * cs / y;
+                if (delp == 0.0) continue;
+                double tforce = 1.0 - fabs(x) / delp;
+                double nforce;
+                if (0 < q && q <= 2.0 / 3.0)
+                    nforce = 2.0 / 3.0;
+                else if (2.0 / 3.0 < q && q <= 1.0)
+                    nforce = 2 * q * (1.0 - 0.75 * q);
+                else if (1.0 < q && q <= 2.0)
+                    nforce = 0.5 * (2 - q) * (2 - q);
+                else
+                    nforce = 0.0;
+                if (ma + mb == 0.0) continue;
+                force = (mb / (ma + mb)) * nforce * tforce * beta;
+            }
+            nr0 += force * nx;
+            nr1 += force * ny;
+            nr2 += force * nz;
+        }
+        ax[i] += nr0;
+        ay[i] += nr1;
+        if (az) az[i] += nr2;
     }
 }
 

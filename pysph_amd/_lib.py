@@ -69,6 +69,24 @@ class Sources(ctypes.Structure):
                 ("indices", _vp * MAX_SRC), ("s_rec", _vp * MAX_SRC), ("ns", _i64 * MAX_SRC)]
 
 
+PAIR_PRESSURE_GRADIENT = 1
+PAIR_ARTIFICIAL_VISCOSITY = 2
+PAIR_MOMENTUM = 3
+PAIR_BOUNDARY_FORCE = 4
+
+
+class PairParams(ctypes.Structure):
+    """sph_pair_params (include/sphhip.h)."""
+    _fields_ = [("op", ctypes.c_int32), ("with_correction", ctypes.c_int32)] + [
+        (k, ctypes.c_double) for k in ("alpha", "beta", "eta", "deltap", "n", "eps", "delp")]
+
+
+class WallFrames(ctypes.Structure):
+    """sph_wall_frames (include/sphhip.h)."""
+    _fields_ = [("s_perm", _vp * MAX_SRC)] + [
+        (k, _vp * MAX_SRC) for k in ("tx", "ty", "tz", "nx", "ny", "nz")]
+
+
 class BatchDesc(ctypes.Structure):
     """sph_batch_desc (include/sphhip.h)."""
     _fields_ = ([("nsims", ctypes.c_int32), ("dim", ctypes.c_int32), ("n", _i64),
@@ -138,6 +156,12 @@ _SIGNATURES = {
                                 _vp]),
     "sph_density_rate_f32": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32,
                                     _i32, _vp]),
+    "sph_pair_accel": (_i32, [_vp, _i32, _i64, _i64, _vp, ctypes.POINTER(Sources), _vp, _i32,
+                              ctypes.POINTER(PairParams), ctypes.POINTER(WallFrames), _i32,
+                              _vp, _vp, _vp, _vp]),
+    "sph_pair_accel_f32": (_i32, [_vp, _i32, _i64, _i64, _vp, ctypes.POINTER(Sources), _vp,
+                                  _i32, ctypes.POINTER(PairParams), ctypes.POINTER(WallFrames),
+                                  _i32, _vp, _vp, _vp, _vp]),
     "sph_xsph": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _dbl, _i32,
                         _vp, _vp, _vp]),
     "sph_xsph_f32": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _dbl,

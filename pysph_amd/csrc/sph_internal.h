@@ -133,6 +133,10 @@ int sph_tait_records(sph_ctx *ctx, int64_t n, const int32_t *perm, double *rec, 
 int sph_tait_records_f32(sph_ctx *ctx, int64_t n, const int32_t *perm, float *rec, double ro,
                          double co, double gamma, double *p, double *cs);
 
+// P/rho^2 into SPH_R_PRHO2 of the first ns[s] records of every source
+// (float records when f32)
+int sph_prho2_sources(sph_ctx *ctx, const sph_sources *src, int f32);
+
 // Exclusive prefix sums (device-wide), stream-ordered.  `data` has n
 // elements; the scanned values are written in place and the total is
 // written to data[n] (so data must hold n+1 elements).

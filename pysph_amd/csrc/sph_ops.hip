@@ -18,36 +18,6 @@
 #define MOM_NT 4
 
 // ---------------------------------------------------------------------------
-// Sources of one destination: every function of a calc over its own source
-// array (SPHCalc.sph, sph/sph_calc.pyx:307-358) in one launch.  Each source's
-// row is summed from 0 in list order and added to the output in source
-// order, exactly as separate calls would (out = (out + r_1) + r_2 ...).
-// ---------------------------------------------------------------------------
-template <typename T> struct SrcLists {
-    int nsrc;
-    const int32_t *counts[SPH_MAX_SRC];
-    const int64_t *slice_off[SPH_MAX_SRC];
-    const int32_t *indices[SPH_MAX_SRC];
-    const T *srec[SPH_MAX_SRC];
-};
-
-template <typename T>
-static int src_lists(sph_ctx *ctx, const char *name, const sph_sources *src, SrcLists<T> *L)
-{
-    SPH_REQUIRE(ctx, src && src->nsrc >= 1 && src->nsrc <= SPH_MAX_SRC, "%s: bad nsrc", name);
-    L->nsrc = src->nsrc;
-    for (int s = 0; s < src->nsrc; ++s) {
-        SPH_REQUIRE(ctx, src->counts[s] && src->slice_off[s] && src->s_rec[s] && src->ns[s] >= 0,
-                    "%s: null source %d", name, s);
-        L->counts[s] = src->counts[s];
-        L->slice_off[s] = src->slice_off[s];
-        L->indices[s] = src->indices[s];
-        L->srec[s] = (const T *)src->s_rec[s];
-    }
-    return SPH_OK;
-}
-
-// ---------------------------------------------------------------------------
 // SPHRho (rho_term in sph_terms.h)
 // ---------------------------------------------------------------------------
 // one SELL-64 row of SPHRho, summed from 0 in list order
@@ -452,6 +422,22 @@ template <typename T> __global__ void __launch_bounds__(OP_THREADS) k_prho2(int6
     r[SPH_R_PRHO2] = r[SPH_R_P] * rhob2;
 }
 
+int sph_prho2_sources(sph_ctx *ctx, const sph_sources *src, int f32)
+{
+    for (int s = 0; s < src->nsrc; ++s) {
+        if (src->ns[s] == 0) continue;
+        const dim3 grid(sph_blocks(src->ns[s], OP_THREADS));
+        if (f32)
+            hipLaunchKernelGGL(k_prho2<float>, grid, dim3(OP_THREADS), 0, ctx->stream,
+                               src->ns[s], (float *)src->s_rec[s]);
+        else
+            hipLaunchKernelGGL(k_prho2<double>, grid, dim3(OP_THREADS), 0, ctx->stream,
+                               src->ns[s], (double *)src->s_rec[s]);
+        SPH_LAUNCH_CHECK(ctx, "k_prho2");
+    }
+    return SPH_OK;
+}
+
 template <typename T>
 static int momentum_av(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
                        const int32_t *d_perm, const sph_sources *src, const T *d_rec,
@@ -466,12 +452,8 @@ static int momentum_av(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
     SrcLists<T> L;
     int rc = src_lists<T>(ctx, "sph_momentum_av", src, &L);
     if (rc) return rc;
-    for (int s = 0; s < src->nsrc; ++s) {
-        if (src->ns[s] == 0) continue;
-        hipLaunchKernelGGL(k_prho2<T>, dim3(sph_blocks(src->ns[s], OP_THREADS)),
-                           dim3(OP_THREADS), 0, ctx->stream, src->ns[s], (T *)src->s_rec[s]);
-        SPH_LAUNCH_CHECK(ctx, "k_prho2");
-    }
+    rc = sph_prho2_sources(ctx, src, sizeof(T) == sizeof(float));
+    if (rc) return rc;
     auto kern = uniform_h ? k_momentum<true, T> : k_momentum<false, T>;
     hipLaunchKernelGGL(kern, dim3(sph_blocks(nd_total, OP_THREADS)), dim3(OP_THREADS), 0,
                        ctx->stream, dim, nd_total, nd_real, d_perm, L, d_rec, T(alpha),

@@ -95,3 +95,32 @@ __device__ __forceinline__ void mom_term(int dim, const MomDest<T> &D, T alpha, 
     nr2 += tmp * (rabz * g);
 }
 
+// ---------------------------------------------------------------------------
+// Sources of one destination: every function of a calc over its own source
+// array (SPHCalc.sph, sph/sph_calc.pyx:307-358) in one launch.  Each source's
+// row is summed from 0 in list order and added to the output in source
+// order, exactly as separate calls would (out = (out + r_1) + r_2 ...).
+// ---------------------------------------------------------------------------
+template <typename T> struct SrcLists {
+    int nsrc;
+    const int32_t *counts[SPH_MAX_SRC];
+    const int64_t *slice_off[SPH_MAX_SRC];
+    const int32_t *indices[SPH_MAX_SRC];
+    const T *srec[SPH_MAX_SRC];
+};
+
+template <typename T>
+static int src_lists(sph_ctx *ctx, const char *name, const sph_sources *src, SrcLists<T> *L)
+{
+    SPH_REQUIRE(ctx, src && src->nsrc >= 1 && src->nsrc <= SPH_MAX_SRC, "%s: bad nsrc", name);
+    L->nsrc = src->nsrc;
+    for (int s = 0; s < src->nsrc; ++s) {
+        SPH_REQUIRE(ctx, src->counts[s] && src->slice_off[s] && src->s_rec[s] && src->ns[s] >= 0,
+                    "%s: null source %d", name, s);
+        L->counts[s] = src->counts[s];
+        L->slice_off[s] = src->slice_off[s];
+        L->indices[s] = src->indices[s];
+        L->srec[s] = (const T *)src->s_rec[s];
+    }
+    return SPH_OK;
+}

@@ -6,3 +6,4 @@ from .sph_equation import SPHIntegration, SPHOperation
 from .time_step_functions import (TimeStep, ViscousTimeStep, ViscousAndForceBasedTimeStep,
                                   VelocityBasedTimeStep)
 from .generators import get_circular_patch, elliptical_drop, dam_break_3d, jittered_lattice_3d
+from .utils import savez, savez_compressed, mkdir, load

@@ -111,3 +111,84 @@ def dam_break_3d(dx=0.01, fluid=(1.0, 0.5, 2.0), tank=(4.0, 0.5, 2.5), layers=3,
     params = dict(dx=dx, h=h, rho0=rho0, c0=c0, gamma=7.0, alpha=0.1, beta=0.0, eta=0.1,
                   g=-g, dt=0.25 * h / c0)
     return fl, wl, params
+
+
+def _line_frame(angle):
+    """Tangent and normal of a straight wall at ``angle`` to +x
+    (base/geometry.pyx:62-88)."""
+    c, s = np.cos(angle), np.sin(angle)
+    return np.array([c, s, 0.0]), np.array([-s, c, 0.0])
+
+
+def monaghan_walls_2d(lines, dx):
+    """Single-row wall particles with tangent/normal frames for Monaghan's
+    boundary force, laid out as base/geometry.pyx meshes an open chain of
+    straight lines (``mesh_line`` :115-133, corners of ``mesh_geometry``
+    :282-351, ``get_particle_array`` :353-405): each line ``(xa, length,
+    angle)`` gets the points xa + k*dx*t strictly inside it with frame
+    (-t, -n); the joint of consecutive lines gets the normalised mean frame;
+    the free ends get their line's frame.  m = 1 and h = 1, as the reference
+    sets them."""
+    pts, tang, norm = [], [], []
+    frames = []
+    for xa, length, angle in lines:
+        t, n = _line_frame(angle)
+        xa = np.array([xa[0], xa[1], 0.0])
+        xb = xa + length * t
+        frames.append((xa, xb, t, n))
+        pos = xa + t * dx
+        while (length - (np.linalg.norm(pos - xa) - 1e-10)) > dx:
+            pts.append(pos)
+            tang.append(-t)
+            norm.append(-n)
+            pos = pos + t * dx
+    nl = len(frames)
+
+    def unit(v):
+        return v / np.sqrt(np.dot(v, v))
+
+    for i in range(1, nl - 1):
+        xa, xb, t, n = frames[i]
+        _, _, tp, np_ = frames[i + 1]
+        _, _, tm, nm = frames[i - 1]
+        pts.append(xa)
+        norm.append(unit((n + np_) * -0.5))
+        tang.append(unit((t + tp) * -0.5))
+        pts.append(xb)
+        norm.append(unit((n + nm) * -0.5))
+        tang.append(unit((t + tm) * -0.5))
+    xa0, xb0, t0, n0 = frames[0]
+    pts.append(xb0)
+    norm.append(-n0)
+    tang.append(-t0)
+    xal, xbl, tl, nl_ = frames[-1]
+    pts.append(xal)
+    norm.append(-nl_)
+    tang.append(-tl)
+    P, T, N = np.array(pts), np.array(tang), np.array(norm)
+    k = len(P)
+    return dict(x=P[:, 0].copy(), y=P[:, 1].copy(), z=np.zeros(k), tx=T[:, 0].copy(),
+                ty=T[:, 1].copy(), tz=np.zeros(k), nx=N[:, 0].copy(), ny=N[:, 1].copy(),
+                nz=np.zeros(k), m=np.ones(k), h=np.ones(k))
+
+
+def dam_break_2d_monaghan(h=0.05, ro=1000.0, co=65.0, width=4.0, height=3.0,
+                          column=(1.0, 2.0)):
+    """The tiny 2D dam break of examples/dam-break/dam_break_simple.py: a
+    column of fluid at (dx..w, dx..h) with dx = h/1.25, m = dx^2 ro, and an open
+    box (left wall, floor, right wall) of single-row Monaghan wall particles
+    at spacing dx.  Returns (fluid, walls, dx); the walls' h is set to the
+    fluid's (the reference leaves it at 1, which only sets its cell size)."""
+    dx = h / 1.25
+    xarr = np.arange(dx, column[0] + dx, dx)
+    yarr = np.arange(dx, column[1] + dx, dx)
+    x, y = np.meshgrid(xarr, yarr)
+    x, y = x.ravel(), y.ravel()
+    n = len(x)
+    fluid = dict(x=x, y=y, z=np.zeros(n), h=np.full(n, h), m=np.full(n, dx * dx * ro),
+                 rho=np.full(n, ro), cs=np.full(n, co))
+    walls = monaghan_walls_2d([((0.0, 0.0), height, np.pi / 2),
+                               ((width, 0.0), width, np.pi),
+                               ((width, height), height, 1.5 * np.pi)], dx)
+    walls["h"] = np.full(len(walls["x"]), h)
+    return fluid, walls, dx

@@ -2,4 +2,5 @@
 from .sph_calc import SPHCalc, HIPCalc
 from .sph_func import Function, SPHFunction, SPHFunctionParticle, CSPHFunctionParticle
 from .funcs import (SPHRho, TaitEquation, MomentumEquation, GravityForce, PositionStepping,
-                    SPHDensityRate, XSPHCorrection)
+                    SPHDensityRate, XSPHCorrection, SPHPressureGradient,
+                    MonaghanArtificialViscosity, MonaghanBoundaryForce)

@@ -2,11 +2,14 @@
 
 SPHRho            sph/funcs/density_funcs.pyx:14-73
 TaitEquation      sph/funcs/eos_funcs.pyx:62-129
-MomentumEquation  sph/funcs/pressure_funcs.pyx:174-391 (hks=False, deltap=None)
+MomentumEquation  sph/funcs/pressure_funcs.pyx:174-391 (hks=False; deltap optional)
 GravityForce      sph/funcs/external_force.pyx:13-44
 PositionStepping  sph/funcs/position_funcs.pyx:10-43
 SPHDensityRate    sph/funcs/density_funcs.pyx:85-164 (hks=False)
 XSPHCorrection    sph/funcs/xsph_funcs.pyx:9-91 (hks=False)
+SPHPressureGradient          sph/funcs/pressure_funcs.pyx:19-168 (hks=False)
+MonaghanArtificialViscosity  sph/funcs/viscosity_funcs.pyx:19-157 (hks=False)
+MonaghanBoundaryForce        sph/funcs/boundary_funcs.pyx:9-107
 
 The neighbour functions read the sorted shadows of their arrays (see
 base/nnps.py) and the sorted-space lists of their locator; results land in
@@ -173,6 +176,7 @@ class MomentumEquation(SPHFunctionParticle):
     viscous time-step factor dt_fac (pressure_funcs.pyx:246-391)."""
 
     writes = (True, True, True)
+    writes_dt_fac = True
     DST = ('x', 'y', 'z', 'h', 'rho', 'p', 'u', 'v', 'w', 'cs')
     SRC = ('x', 'y', 'z', 'h', 'm', 'rho', 'p', 'u', 'v', 'w', 'cs')
 
@@ -183,12 +187,15 @@ class MomentumEquation(SPHFunctionParticle):
         self.beta = float(beta)
         self.gamma = float(gamma)
         self.eta = float(eta)
-        if deltap is not None:
-            raise NotImplementedError("artificial pressure (deltap) is not part of this path")
-        self.deltap = deltap
+        self.deltap = 0.0
         self.with_correction = False
-        self.n = n
-        self.eps = eps
+        if deltap is not None:
+            self.deltap = float(deltap)
+            if deltap < 0:
+                raise RuntimeError("deltap cannot be negative!")
+            self.with_correction = True
+        self.n = float(n)
+        self.eps = float(eps)
         self.id = 'momentumequation'
         self.tag = "velocity"
         self.to_reset = ['dt_fac']
@@ -203,7 +210,16 @@ class MomentumEquation(SPHFunctionParticle):
         if 'dt_fac' not in self.dst_reads:
             self.dst_reads.append('dt_fac')
 
+    def pair_params(self):
+        return _lib.PairParams(op=_lib.PAIR_MOMENTUM, with_correction=1, alpha=self.alpha,
+                               beta=self.beta, eta=self.eta, deltap=self.deltap, n=self.n,
+                               eps=self.eps)
+
     def eval(self, kernel, output1, output2, output3, accumulate=True, names=None):
+        if self.with_correction:
+            # the artificial-pressure branch (pressure_funcs.pyx:327-365)
+            _pair_launch([self], kernel, output1, output2, output3, accumulate, names)
+            return
         ctx = self._ctx()
         loc = self.nbr_locator
         cm = loc.cell_manager
@@ -235,14 +251,19 @@ class MomentumEquation(SPHFunctionParticle):
     @staticmethod
     def fusable(funcs):
         f0 = funcs[0]
+        key = (f0.alpha, f0.beta, f0.eta, f0.with_correction, f0.deltap, f0.n, f0.eps)
         return all(type(f) is MomentumEquation and f.dest is f0.dest and
-                   (f.alpha, f.beta, f.eta) == (f0.alpha, f0.beta, f0.eta) for f in funcs)
+                   (f.alpha, f.beta, f.eta, f.with_correction, f.deltap, f.n, f.eps) == key
+                   for f in funcs)
 
     @staticmethod
     def eval_multi(funcs, kernel, output1, output2, output3, accumulate=True, names=None):
         """All functions of one calc (one per source array) in one launch:
         the same sums, in the same order, as their eval calls in turn."""
         f0 = funcs[0]
+        if f0.with_correction:
+            _pair_launch(funcs, kernel, output1, output2, output3, accumulate, names)
+            return
         ctx = f0._ctx()
         cm = f0.nbr_locator.cell_manager
         p = _lib.ptr
@@ -275,6 +296,180 @@ class MomentumEquation(SPHFunctionParticle):
             if name:
                 cm.written(f0.dest, name, False)
         cm.written(f0.dest, 'dt_fac', False)
+
+
+def _pair_launch(funcs, kernel, output1, output2, output3, accumulate, names):
+    """One sph_pair_accel launch for the functions of one calc (one per
+    source array, all with the same parameters): per destination, each
+    source's row summed in list order and added in source order."""
+    f0 = funcs[0]
+    ctx = f0._ctx()
+    cm = f0.nbr_locator.cell_manager
+    p = _lib.ptr
+    prm = f0.pair_params()
+    uses_prho2 = prm.op in (_lib.PAIR_PRESSURE_GRADIENT, _lib.PAIR_MOMENTUM)
+    src = _lib.Sources()
+    src.nsrc = len(funcs)
+    frames = None
+    if prm.op == _lib.PAIR_BOUNDARY_FORCE:
+        frames = _lib.WallFrames()
+    fresh = []
+    for k, f in enumerate(funcs):
+        loc = f.nbr_locator
+        cnt, soff, idx = loc.get_sorted_lists()
+        srec = cm.shadows(f.source, f.SRC)
+        fr = (not uses_prho2 or cm.prho2_fresh(f.source) or
+              f.source in [g.source for g in funcs[:k]])
+        fresh.append(fr)
+        src.counts[k], src.slice_off[k], src.indices[k] = p(cnt), p(soff), p(idx)
+        src.s_rec[k] = p(srec)
+        src.ns[k] = 0 if fr else cm.bins[f.source.name].n
+        if frames is not None:
+            frames.s_perm[k] = p(cm.bins[f.source.name].perm)
+            for c in ("tx", "ty", "tz", "nx", "ny", "nz"):
+                t = f.source.get_carray(c)
+                if t is None or t.dtype != torch.float64:
+                    raise AttributeError(f"MonaghanBoundaryForce needs the double property "
+                                         f"{c} on source array {f.source.name}")
+                getattr(frames, c)[k] = p(t)
+        profiling.count_pairs(loc.pair_count())
+    drec = cm.shadows(f0.dest, f0.DST)
+    fn = ctx.lib.sph_pair_accel_f32 if cm.precision == "single" else ctx.lib.sph_pair_accel
+    dt_fac = f0.d('dt_fac') if getattr(f0, 'writes_dt_fac', False) else None
+    with profiling.region(f0.id):
+        ctx.check(fn(ctx.handle, kernel.dim, f0.dest.get_number_of_particles(),
+                     f0.dest.num_real_particles, p(cm.bins[f0.dest.name].perm),
+                     ctypes.byref(src), p(drec), int(cm.uniform_h()), ctypes.byref(prm),
+                     ctypes.byref(frames) if frames is not None else None,
+                     int(bool(accumulate)), p(output1), p(output2), p(output3), p(dt_fac)))
+    if uses_prho2:
+        for f, fr in zip(funcs, fresh):
+            if not fr:
+                cm.prho2_written(f.source)
+    for name in (names or ()):
+        if name:
+            cm.written(f0.dest, name, False)
+    if dt_fac is not None:
+        cm.written(f0.dest, 'dt_fac', False)
+
+
+class _PairFunction(SPHFunctionParticle):
+    """Functions evaluated by sph_pair_accel; every function of a calc with
+    the same parameters goes into one launch."""
+
+    writes = (True, True, True)
+    writes_dt_fac = False
+
+    def set_src_dst_reads(self):
+        self.src_reads = [k for k in self.SRC_READS]
+        self.dst_reads = [k for k in self.DST_READS] + ['tag']
+
+    def eval(self, kernel, output1, output2, output3, accumulate=True, names=None):
+        _pair_launch([self], kernel, output1, output2, output3, accumulate, names)
+
+    @classmethod
+    def fusable(cls, funcs):
+        f0 = funcs[0]
+        key = f0._param_key()
+        return all(type(f) is type(f0) and f.dest is f0.dest and f._param_key() == key
+                   for f in funcs)
+
+    @staticmethod
+    def eval_multi(funcs, kernel, output1, output2, output3, accumulate=True, names=None):
+        _pair_launch(funcs, kernel, output1, output2, output3, accumulate, names)
+
+    def _param_key(self):
+        prm = self.pair_params()
+        return tuple(getattr(prm, f[0]) for f in prm._fields_)
+
+
+class SPHPressureGradient(_PairFunction):
+    """a_a += -m_b (P_a/rho_a^2 + P_b/rho_b^2) gradW_ab, with Monaghan's
+    artificial pressure when deltap is given (pressure_funcs.pyx:19-151)."""
+
+    DST = ('x', 'y', 'z', 'h', 'rho', 'p')
+    SRC = ('x', 'y', 'z', 'h', 'm', 'rho', 'p')
+    SRC_READS = ('x', 'y', 'z', 'h', 'm', 'rho', 'p')
+    DST_READS = ('x', 'y', 'z', 'h', 'rho', 'p')
+
+    def __init__(self, source, dest, setup_arrays=True, deltap=None, n=1, eps=0.2, **kwargs):
+        SPHFunctionParticle.__init__(self, source, dest, setup_arrays, **kwargs)
+        self.deltap = 0.0
+        self.with_correction = False
+        if deltap is not None:
+            self.deltap = float(deltap)
+            if deltap < 0:
+                raise RuntimeError("deltap cannot be negative!")
+            self.with_correction = True
+        self.n = float(n)
+        self.eps = float(eps)
+        self.id = 'pgrad'
+        self.tag = "velocity"
+
+    def pair_params(self):
+        return _lib.PairParams(op=_lib.PAIR_PRESSURE_GRADIENT,
+                               with_correction=int(self.with_correction), deltap=self.deltap,
+                               n=self.n, eps=self.eps)
+
+
+class MonaghanArtificialViscosity(_PairFunction):
+    """Monaghan's artificial viscosity alone, with the viscous time-step factor
+    dt_fac (viscosity_funcs.pyx:19-151).  As in the reference, the constructor's
+    eta is not stored: the function's eta is 0 unless set afterwards, and
+    dt_fac uses the constant 0.01 h_ab^2."""
+
+    writes_dt_fac = True
+    DST = ('x', 'y', 'z', 'h', 'rho', 'u', 'v', 'w', 'cs')
+    SRC = ('x', 'y', 'z', 'h', 'm', 'rho', 'u', 'v', 'w', 'cs')
+    SRC_READS = ('x', 'y', 'z', 'h', 'm', 'rho', 'u', 'v', 'w', 'cs')
+    DST_READS = ('x', 'y', 'z', 'h', 'p', 'u', 'v', 'w', 'cs', 'rho')
+
+    def __init__(self, source, dest, setup_arrays=True, alpha=1, beta=1, gamma=1.4, eta=0.1,
+                 **kwargs):
+        SPHFunctionParticle.__init__(self, source, dest, setup_arrays, **kwargs)
+        self.alpha = float(alpha)
+        self.beta = float(beta)
+        self.gamma = float(gamma)
+        self.eta = 0.0
+        self.id = 'momavisc'
+        self.tag = "velocity"
+        self.to_reset = ['dt_fac']
+
+    def setup_arrays(self):
+        if self.dest.get_carray('dt_fac') is None:
+            self.dest.add_property({'name': 'dt_fac'})
+        if 'dt_fac' not in self.dst_reads:
+            self.dst_reads.append('dt_fac')
+
+    def pair_params(self):
+        return _lib.PairParams(op=_lib.PAIR_ARTIFICIAL_VISCOSITY, alpha=self.alpha,
+                               beta=self.beta, eta=self.eta)
+
+
+class MonaghanBoundaryForce(_PairFunction):
+    """Monaghan's repulsive wall force from the wall particles' tangent and
+    normal (tx, ty, tz, nx, ny, nz) (boundary_funcs.pyx:9-107)."""
+
+    DST = ('x', 'y', 'z', 'h', 'm', 'cs')
+    SRC = ('x', 'y', 'z', 'm')
+    SRC_READS = ('x', 'y', 'z', 'h', 'm', 'rho', 'tx', 'ty', 'tz', 'nx', 'ny', 'nz', 'cs')
+    DST_READS = ('x', 'y', 'z', 'h', 'm', 'cs')
+
+    def __init__(self, source, dest, setup_arrays=False, delp=1.0, **kwargs):
+        SPHFunctionParticle.__init__(self, source, dest, setup_arrays=True, **kwargs)
+        self.id = 'monaghanbforce'
+        self.tag = "velocity"
+        self.delp = float(delp)
+
+    def setup_arrays(self):
+        missing = [c for c in ("tx", "ty", "tz", "nx", "ny", "nz")
+                   if self.source.get_carray(c) is None]
+        if missing:
+            raise AttributeError(f"source array {self.source.name} has no property "
+                                 f"{missing[0]}")
+
+    def pair_params(self):
+        return _lib.PairParams(op=_lib.PAIR_BOUNDARY_FORCE, delp=self.delp)
 
 
 class GravityForce(SPHFunction):

@@ -1,0 +1,251 @@
+"""GPU parity of the split-form operators and the wall force against the FP64
+oracle: SPHPressureGradient (with and without artificial pressure),
+MonaghanArtificialViscosity, MomentumEquation with deltap,
+MonaghanBoundaryForce; bit-level agreement expected (printed) except where
+pow() enters (artificial pressure), 1e-6 relative required."""
+import numpy as np
+import pytest
+
+from oracle import cpu_ref
+from tests.helpers import bit_equal_frac, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+
+
+def _fluid_and_floor(n, seed, varh):
+    """Jittered fluid lattice above a one-layer floor with frames (normal +z)."""
+    from pysph_amd.solver.generators import jittered_lattice_3d
+    rng = np.random.default_rng(seed)
+    d = jittered_lattice_3d(n=n, seed=seed, hdx=1.3)
+    N = len(d["x"])
+    dx = 1.0 / n
+    d["u"], d["v"], d["w"] = rng.standard_normal((3, N))
+    d["rho"] = 1000.0 * rng.uniform(0.98, 1.02, N)
+    d["p"] = rng.uniform(-300.0, 500.0, N)
+    d["cs"] = rng.uniform(9.0, 11.0, N)
+    d["m"] = d["m"] * 1000.0 * rng.uniform(0.9, 1.1, N)
+    if varh:
+        d["h"] = d["h"] * rng.uniform(0.8, 1.25, N)
+    gx, gy = np.meshgrid((np.arange(n) + 0.5) * dx, (np.arange(n) + 0.5) * dx)
+    M = gx.size
+    w = dict(x=gx.ravel(), y=gy.ravel(), z=np.full(M, -0.5 * dx), h=np.full(M, 1.3 * dx),
+             m=np.full(M, 1000.0 * dx ** 3), rho=np.full(M, 1000.0),
+             p=rng.uniform(0.0, 200.0, M), cs=np.full(M, 10.0), u=np.zeros(M), v=np.zeros(M),
+             w=np.zeros(M), tx=np.ones(M), ty=np.zeros(M), tz=np.zeros(M), nx=np.zeros(M),
+             ny=np.zeros(M), nz=np.ones(M))
+    return d, w
+
+
+def _run_calc(particles, dest, sources, funcs, kernel):
+    from pysph_amd.sph import api as sph
+    calc = sph.SPHCalc(particles, sources, dest, kernel, funcs, ['_tmpx', '_tmpy', '_tmpz'],
+                       integrates=True)
+    for k in ('_tmpx', '_tmpy', '_tmpz'):
+        dest.get_carray(k).zero_()
+    calc.sph('_tmpx', '_tmpy', '_tmpz')
+    return [dest.get(k) for k in ('_tmpx', '_tmpy', '_tmpz')]
+
+
+@pytest.mark.parametrize("varh", [False, True])
+def test_pair_operators_vs_oracle(gpu, varh):
+    from pysph_amd.base import api as base
+    from pysph_amd.sph import api as sph
+    d, w = _fluid_and_floor(16, 11 if varh else 10, varh)
+    fl = base.get_particle_array(name="fluid", type=base.Fluid, **d)
+    wl = base.get_particle_array(name="wall", type=base.Solid, **w)
+    particles = base.Particles([fl, wl])
+    kernel = base.CubicSplineKernel(dim=3)
+    N = len(d["x"])
+    full = {k: fl.get(k) for k in ("x", "y", "z", "h", "m", "rho", "p", "u", "v", "w", "cs")}
+    wall = {k: wl.get(k) for k in ("x", "y", "z", "h", "m", "rho", "p", "u", "v", "w", "cs",
+                                   "tx", "ty", "tz", "nx", "ny", "nz")}
+    srcs = {"fluid": full, "wall": wall}
+    errs = {}
+    lists = {}
+
+    def oracle_lists():
+        # the cell size is fixed at the first neighbour query
+        cs = particles.cell_manager.cell_size
+        assert cs > 0
+        lists["fluid"] = cpu_ref.nnps(full, full, 2.0, cs, True, nthreads=8)
+        lists["wall"] = cpu_ref.nnps(full, wall, 2.0, cs, False, nthreads=8)
+
+    def check(name, got, want, exact):
+        e = max(rel_err(g, r) for g, r in zip(got, want))
+        b = min(bit_equal_frac(g, r) for g, r in zip(got, want))
+        errs[name] = e
+        print(f"varh {varh} {name}: rel err {e:.3g}, bit-equal {b:.4f}")
+        if exact:
+            assert b == 1.0, name
+
+    # SPHPressureGradient from fluid and walls, one launch, without and with deltap
+    for deltap in (None, 0.9 / 16):
+        fs = [sph.SPHPressureGradient.withargs(deltap=deltap, n=4, eps=0.2).get_func(s, fl)
+              for s in (fl, wl)]
+        got = _run_calc(particles, fl, [fl, wl], fs, kernel)
+        if not lists:
+            oracle_lists()
+        want = [np.zeros(N) for _ in range(3)]
+        for s in ("fluid", "wall"):
+            cpu_ref.pressure_gradient(3, full, srcs[s], *lists[s], *want, deltap=deltap, n=4,
+                                      eps=0.2, nthreads=8)
+        check(f"pgrad deltap={deltap}", got, want, exact=deltap is None)
+
+    # MonaghanArtificialViscosity with dt_fac; then with eta set on the functions
+    for eta in (None, 0.1):
+        fs = [sph.MonaghanArtificialViscosity.withargs(alpha=0.5, beta=0.3).get_func(s, fl)
+              for s in (fl, wl)]
+        if eta is not None:
+            for f in fs:
+                f.eta = eta
+        got = _run_calc(particles, fl, [fl, wl], fs, kernel)
+        want = [np.zeros(N) for _ in range(3)]
+        dtf = np.zeros(N)
+        for s in ("fluid", "wall"):
+            cpu_ref.artificial_viscosity(3, full, srcs[s], *lists[s], 0.5, 0.3, eta or 0.0,
+                                         *want, dtf, nthreads=8)
+        check(f"avisc eta={eta}", got, want, exact=True)
+        check(f"avisc dt_fac eta={eta}", [fl.get('dt_fac')], [dtf], exact=True)
+
+    # MomentumEquation with artificial pressure
+    fs = [sph.MomentumEquation.withargs(alpha=0.1, beta=0.2, deltap=0.9 / 16, n=4,
+                                        eps=0.2).get_func(s, fl) for s in (fl, wl)]
+    got = _run_calc(particles, fl, [fl, wl], fs, kernel)
+    want = [np.zeros(N) for _ in range(3)]
+    dtf = np.zeros(N)
+    for s in ("fluid", "wall"):
+        cpu_ref.momentum_deltap(3, full, srcs[s], *lists[s], 0.1, 0.2, 0.1, 0.9 / 16, 4.0, 0.2,
+                                *want, dtf, nthreads=8)
+    check("momentum deltap", got, want, exact=False)
+    check("momentum deltap dt_fac", [fl.get('dt_fac')], [dtf], exact=True)
+
+    # MonaghanBoundaryForce from the floor
+    f = sph.MonaghanBoundaryForce.withargs(delp=1.0 / 16).get_func(wl, fl)
+    got = _run_calc(particles, fl, [wl], [f], kernel)
+    want = [np.zeros(N) for _ in range(3)]
+    cpu_ref.boundary_force(full, wall, *lists["wall"], 1.0 / 16, *want, nthreads=8)
+    assert np.count_nonzero(want[2]) > 100
+    check("boundary force", got, want, exact=True)
+    assert max(errs.values()) <= TOL
+
+
+def test_boundary_force_needs_frames(gpu):
+    from pysph_amd.base import api as base
+    from pysph_amd.sph import api as sph
+    fl = base.get_particle_array(name="fluid", x=np.zeros(2))
+    wl = base.get_particle_array(name="wall", x=np.ones(2))
+    with pytest.raises(AttributeError, match="tx"):
+        sph.MonaghanBoundaryForce.withargs(delp=0.1).get_func(wl, fl)
+    with pytest.raises(RuntimeError, match="deltap cannot be negative"):
+        sph.SPHPressureGradient.withargs(deltap=-1.0).get_func(fl, fl)
+
+
+def _state_err(particles, ref, keys):
+    worst = 0.0
+    for a, r in zip(particles.arrays, ref.arrays):
+        for k in keys:
+            e = rel_err(a.get(k), r[k])
+            worst = max(worst, e)
+            print(f"{a.name} {k}: rel err {e:.3g} bit-equal {bit_equal_frac(a.get(k), r[k]):.4f}")
+    return worst
+
+
+def test_dam_break_simple_vs_oracle(gpu):
+    """examples/dam-break/dam_break_simple.py: Tait, continuity, momentum
+    (alpha 0.5), gravity, Monaghan wall force from a single row of frame
+    particles (delp = dx), position stepping with XSPH (eps 0.5); Euler,
+    dt = 1e-4, 20 steps."""
+    from pysph_amd.base import api as base
+    from pysph_amd.sph import api as sph
+    from pysph_amd.solver import api as solver
+    from pysph_amd.solver.generators import dam_break_2d_monaghan
+    fd, wd, dx = dam_break_2d_monaghan()
+    ro, co = 1000.0, 65.0
+    fluid = base.get_particle_array(name="fluid", type=base.Fluid, **fd)
+    walls = base.get_particle_array(name="boundary", type=base.Solid, **wd)
+    particles = base.Particles([fluid, walls])
+    F, S = base.Fluid, base.Solid
+    s = solver.Solver(dim=2, integrator_type=solver.EulerIntegrator)
+    s.add_operation(solver.SPHOperation(sph.TaitEquation.withargs(co=co, ro=ro), on_types=[F],
+                                        updates=['p', 'cs'], id='eos'))
+    s.add_operation(solver.SPHIntegration(sph.SPHDensityRate.withargs(), on_types=[F],
+                                          from_types=[F], updates=['rho'], id='density'))
+    s.add_operation(solver.SPHIntegration(sph.MomentumEquation.withargs(alpha=0.5, beta=0.0),
+                                          on_types=[F], from_types=[F], updates=['u', 'v'],
+                                          id='mom'))
+    s.add_operation(solver.SPHIntegration(sph.GravityForce.withargs(gy=-9.81), on_types=[F],
+                                          updates=['u', 'v'], id='gravity'))
+    s.add_operation(solver.SPHIntegration(sph.MonaghanBoundaryForce.withargs(delp=dx),
+                                          on_types=[F], from_types=[S], updates=['u', 'v'],
+                                          id='bforce'))
+    s.add_operation_step([F])
+    s.add_operation_xsph(eps=0.5)
+    s.set_time_step(1e-4)
+    s.set_final_time(1.0)
+    s.set_output(False)
+    s.setup(particles)
+    ref = cpu_ref.RefSimulation([cpu_ref.RefArray("fluid", 0, fd), cpu_ref.RefArray(
+        "boundary", 1, wd)], dim=2, integrator="euler", co=co, ro=ro, gamma=7.0, alpha=0.5,
+        beta=0.0, eta=0.1, gravity=(0.0, -9.81, 0.0), density_from=(0,), momentum_from=(0,),
+        nthreads=8, density="continuity", xsph_eps=0.5, bforce_from=(1,), bforce_delp=dx)
+    for _ in range(20):
+        s.integrator.integrate(1e-4)
+        ref.integrate(1e-4)
+    worst = _state_err(particles, ref, ("x", "y", "u", "v", "rho", "p"))
+    assert worst <= TOL
+
+
+def test_split_form_dam_break_vs_oracle(gpu):
+    """The operator split of examples/dam-break/dam_break3d.py on the 3D dam
+    break (fluid + three wall layers): continuity from fluid and walls, Tait,
+    SPHPressureGradient from the fluid with artificial pressure,
+    MonaghanArtificialViscosity from fluid and walls, gravity, stepping with
+    XSPH; predictor-corrector, 4 steps."""
+    from pysph_amd.base import api as base
+    from pysph_amd.sph import api as sph
+    from pysph_amd.solver import api as solver
+    from pysph_amd.solver.generators import dam_break_3d
+    fl, wl, prm = dam_break_3d(dx=0.04)
+    deltap = 0.04
+    fluid = base.get_particle_array(name="fluid", type=base.Fluid, **fl)
+    wall = base.get_particle_array(name="wall", type=base.Solid, **wl)
+    particles = base.Particles([fluid, wall])
+    F, S = base.Fluid, base.Solid
+    s = solver.Solver(dim=3, integrator_type=solver.PredictorCorrectorIntegrator)
+    s.add_operation(solver.SPHIntegration(sph.SPHDensityRate.withargs(), on_types=[F],
+                                          from_types=[F, S], updates=['rho'], id='density'))
+    s.add_operation(solver.SPHOperation(
+        sph.TaitEquation.withargs(co=prm["c0"], ro=prm["rho0"], gamma=prm["gamma"]),
+        on_types=[F], updates=['p', 'cs'], id='eos'))
+    s.add_operation(solver.SPHIntegration(
+        sph.SPHPressureGradient.withargs(deltap=deltap, n=4), on_types=[F], from_types=[F],
+        updates=['u', 'v', 'w'], id='pgrad'))
+    s.add_operation(solver.SPHIntegration(
+        sph.MonaghanArtificialViscosity.withargs(alpha=prm["alpha"], beta=prm["beta"]),
+        on_types=[F], from_types=[F, S], updates=['u', 'v', 'w'], id='avisc'))
+    s.add_operation(solver.SPHIntegration(sph.GravityForce.withargs(gz=prm["g"]), on_types=[F],
+                                          updates=['u', 'v', 'w'], id='gravity'))
+    s.add_operation_step([F])
+    s.add_operation_xsph(eps=0.5)
+    s.set_output(False)
+    s.setup(particles)
+    ref = cpu_ref.RefSimulation(
+        [cpu_ref.RefArray("fluid", 0, fl), cpu_ref.RefArray("wall", 1, wl)], dim=3,
+        integrator="pc", co=prm["c0"], ro=prm["rho0"], gamma=prm["gamma"], alpha=prm["alpha"],
+        beta=prm["beta"], gravity=(0.0, 0.0, prm["g"]), density_from=(0, 1),
+        momentum_from=(0, 1), nthreads=8, density="continuity", xsph_eps=0.5,
+        momentum="split", pgrad_from=(0,), deltap=deltap, ap_n=4.0, ap_eps=0.2)
+    # the stored pressures start negative nowhere: give the fluid a suction
+    # band so the artificial pressure acts
+    z = fl["z"]
+    rho0 = fluid.get("rho")
+    rho0[z > 1.5] *= 0.999
+    fluid.set(rho=rho0)
+    ref.arrays[0].props["rho"][:] = rho0
+    for _ in range(4):
+        s.integrator.integrate(prm["dt"])
+        ref.integrate(prm["dt"])
+    worst = _state_err(particles, ref, ("x", "y", "z", "u", "v", "w", "rho", "p"))
+    assert worst <= TOL

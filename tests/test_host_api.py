@@ -121,8 +121,8 @@ def test_operation_order_and_errors():
 
 def test_unsupported_features_fail_loudly():
     pa = base.get_particle_array(name="f", x=np.zeros(2))
-    with pytest.raises(NotImplementedError):
-        sph.MomentumEquation.withargs(deltap=0.1).get_func(pa, pa)
+    with pytest.raises(RuntimeError, match="deltap cannot be negative"):
+        sph.MomentumEquation.withargs(deltap=-0.1).get_func(pa, pa)
     with pytest.raises(NotImplementedError):
         sph.SPHRho.withargs(hks=True).get_func(pa, pa)
     with pytest.raises(NotImplementedError):

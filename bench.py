@@ -15,7 +15,10 @@ neighbour-operator evaluation (density and momentum, both sources, both
 stages) per second of wall time.
 
 ``--workload lattice`` runs the NNPS + SummationDensity micro-workload
-(100^3 jittered lattice, h = 1.2 dx) instead.
+(100^3 jittered lattice, h = 1.2 dx) instead; ``--workload dambreak-split``
+the same dam break with the operator split of examples/dam-break/
+dam_break3d.py (continuity, pressure gradient and Monaghan viscosity as
+separate functions, XSPH).
 
 Also reported: the roofline fraction of the dominant kernel (momentum)
 measured with HIP events on its stream, and the FP64 CPU oracle timed on
@@ -41,7 +44,8 @@ def _args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=("dambreak", "lattice", "batch", "adjoint"),
+    ap.add_argument("--workload", choices=("dambreak", "dambreak-split", "lattice", "batch",
+                                           "adjoint"),
                     default="dambreak")
     ap.add_argument("--fd-particles", type=int, default=64,
                     help="particles whose u0 and v0 gradients --workload adjoint checks by "
@@ -89,11 +93,12 @@ def copy_calibration(mib):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_momentum_dambreak_1M_traffic.json")
 
 
-def measured_traffic():
-    """HBM bytes per k_momentum launch from the committed rocprofv3 byte
-    counters (FETCH_SIZE, WRITE_SIZE, scaled by a 1 GiB copy), or None."""
+def measured_traffic(path=None):
+    """HBM bytes per launch of the roofline kernel from the committed
+    rocprofv3 byte counters (FETCH_SIZE, WRITE_SIZE, scaled by a 1 GiB copy),
+    or None."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path or TRAFFIC_FILE) as f:
             return float(json.load(f)["hbm_bytes_per_launch"])
     except (OSError, ValueError, KeyError):
         return None
@@ -110,6 +115,91 @@ def momentum_bytes(nd, nnz, ns_other, ns):
     sources reads and writes each destination once: pass the entries of all
     lists and the particles of the other arrays."""
     return 24 * ns + nd * 152 + 4 * nnz + 88 * ns_other
+
+
+def avisc_bytes(nd, nnz, ns_other):
+    """Algorithmic HBM bytes of one MonaghanArtificialViscosity launch over
+    all its sources: per destination the list count, 9 properties + dt_fac
+    read, ax ay az read-modify-write and dt_fac written (4+10*8+3*16+8 =
+    140 B), 4 B per list entry, and the other arrays' 10 properties once
+    (80 B per source particle)."""
+    return nd * 140 + 4 * nnz + 80 * ns_other
+
+
+def run_dambreak_split(args):
+    """The operator split of dam_break3d.py (continuity, Tait, pressure
+    gradient, Monaghan viscosity, gravity, XSPH) on the same 1M dam break."""
+    import torch
+    from pysph_amd import profiling
+    from pysph_amd.solver.configs import dambreak_split_solver
+    from pysph_amd.solver.generators import dam_break_3d
+
+    data = dam_break_3d(dx=args.dx)
+    fl, wl, prm = data
+    s, particles, _ = dambreak_split_solver(data=data, cell_order=args.cell_order)
+    fluid, wall = particles.arrays
+    integ = s.integrator
+    dt = prm["dt"]
+    avisc_calc = [c for c in integ.calcs if c.id.startswith("avisc")][0]
+    for _ in range(args.warmup):
+        integ.integrate(dt)
+    if args.copy_calibration_mb:
+        copy_calibration(args.copy_calibration_mb)
+    torch.cuda.synchronize()
+    profiling.reset()
+    profiling.enable(True)
+    for _ in range(min(5, args.steps)):
+        integ.integrate(dt)
+    profiling.enable(False)
+    breakdown = profiling.summary()
+    profiling.reset()
+    region = avisc_calc.funcs[0].id
+    profiling.enable(True, regions=(region,))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        integ.integrate(dt)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    profiling.enable(False)
+    pairs = profiling.pairs()
+    prof = profiling.summary()
+    elapsed = t1 - t0
+    nd = fluid.get_number_of_particles()
+    nw = wall.get_number_of_particles()
+    nnz_ff = avisc_calc.funcs[0].nbr_locator.nnz
+    nnz_fw = avisc_calc.funcs[1].nbr_locator.nnz
+    av = prof.get(region, {"mean_ms": float("nan"), "count": 0})
+    bytes_per_launch = avisc_bytes(nd, nnz_ff + nnz_fw, nw)
+    achieved = bytes_per_launch / (av["mean_ms"] * 1e-3) / 1e9
+    traffic_file = os.path.join(ROOT, "profiles", "k_pair_avisc_dambreak_1M_traffic.json")
+    result = {
+        "metric": "pair-interactions/s",
+        "value": pairs / elapsed,
+        "unit": "pair-interactions/s",
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "dtype": "f64",
+        "config": {
+            "workload": "3D dam break, operator split of dam_break3d.py (continuity, Tait, "
+                        "SPHPressureGradient, MonaghanArtificialViscosity, gravity, XSPH), "
+                        "PredictorCorrector step, FP64",
+            "fluid_particles": nd, "wall_particles": nw, "dx": args.dx, "dt": dt,
+            "steps_per_s": args.steps / elapsed, "pairs_per_step": pairs / args.steps,
+            "nnz_fluid_fluid": nnz_ff, "nnz_fluid_wall": nnz_fw,
+            "kernel_ms": {k: round(v["mean_ms"], 4) for k, v in breakdown.items()},
+            "kernel_launches_per_step": {k: v["count"] / min(5, args.steps)
+                                         for k, v in breakdown.items()},
+        },
+        "roofline": {
+            "kernel": "k_pair (MonaghanArtificialViscosity, fluid + wall sources)",
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(traffic_file) if args.dx == 0.01 else None,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "mean_launch_ms": av["mean_ms"],
+        },
+    }
+    return result, data
 
 
 def run_dambreak(args):
@@ -586,6 +676,29 @@ def cpu_baseline_dambreak(args, data):
             "steps_per_s": 1.0 / (t1 - t0)}
 
 
+def cpu_baseline_dambreak_split(args, data):
+    """The FP64 oracle on the host cores: one PredictorCorrector step of the
+    operator-split dam break."""
+    from oracle import cpu_ref
+    fl, wl, prm = data
+    threads = args.cpu_threads or len(os.sched_getaffinity(0))
+    ref = cpu_ref.RefSimulation(
+        [cpu_ref.RefArray("fluid", 0, fl), cpu_ref.RefArray("wall", 1, wl)], dim=3,
+        integrator="pc", co=prm["c0"], ro=prm["rho0"], gamma=prm["gamma"], alpha=prm["alpha"],
+        beta=prm["beta"], gravity=(0.0, 0.0, prm["g"]), density_from=(0, 1),
+        momentum_from=(0, 1), nthreads=threads, density="continuity", xsph_eps=0.5,
+        momentum="split", pgrad_from=(0,))
+    t0 = time.perf_counter()
+    ref.integrate(prm["dt"])
+    t1 = time.perf_counter()
+    return {"value": ref.pairs / (t1 - t0), "unit": "pair-interactions/s", "cores": threads,
+            "kind": "port",
+            "sample": f"1 PredictorCorrector step of the same operator-split dam break "
+                      f"({ref.pairs} pair interactions, {t1 - t0:.2f} s, {threads} OpenMP "
+                      f"threads)",
+            "steps_per_s": 1.0 / (t1 - t0)}
+
+
 def cpu_baseline_lattice(args, d):
     from oracle import cpu_ref
     import numpy as np
@@ -613,6 +726,8 @@ def main():
         sys.exit(1)
     if args.workload == "dambreak":
         result, data = run_dambreak(args)
+    elif args.workload == "dambreak-split":
+        result, data = run_dambreak_split(args)
     elif args.workload == "lattice":
         result, data = run_lattice(args)
     elif args.workload == "batch":
@@ -625,9 +740,11 @@ def main():
            "higher_is_better": result.get("higher_is_better", True), "scaling": "weak",
            "vs_baseline": None, "dtype": result["dtype"], "data": "synthetic",
            "config": result["config"], "roofline": result["roofline"]}
-    if not args.no_cpu_baseline and args.workload in ("dambreak", "lattice"):
+    if not args.no_cpu_baseline and args.workload in ("dambreak", "dambreak-split", "lattice"):
         if args.workload == "dambreak":
             out["cpu_baseline"] = cpu_baseline_dambreak(args, data)
+        elif args.workload == "dambreak-split":
+            out["cpu_baseline"] = cpu_baseline_dambreak_split(args, data)
         else:
             out["cpu_baseline"] = cpu_baseline_lattice(args, data)
     print(json.dumps(out), flush=True)

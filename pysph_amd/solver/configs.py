@@ -71,3 +71,41 @@ def dambreak_solver(dx=0.01, data=None, cell_order="morton"):
     s.set_time_step(prm["dt"])
     s.setup(particles)
     return s, particles, (fl, wl, prm)
+
+
+def dambreak_split_solver(dx=0.01, data=None, cell_order="morton", deltap=None, xsph_eps=0.5):
+    """The operator split of examples/dam-break/dam_break3d.py on the 3D dam
+    break: SPHDensityRate on the fluid from fluid and walls, TaitEquation on
+    the fluid, SPHPressureGradient from the fluid (artificial pressure when
+    deltap is given), MonaghanArtificialViscosity from fluid and walls,
+    gravity, position stepping with XSPH; PredictorCorrector.  The example
+    also evolves the walls' density and pressure; here the walls keep their
+    initial state, as in the 3D dam break above."""
+    fl, wl, prm = data if data is not None else dam_break_3d(dx=dx)
+    fluid = base.get_particle_array(name="fluid", type=base.Fluid, **fl)
+    wall = base.get_particle_array(name="wall", type=base.Solid, **wl)
+    particles = base.Particles([fluid, wall], cell_order=cell_order)
+    s = solver_api.Solver(dim=3, integrator_type=solver_api.PredictorCorrectorIntegrator)
+    F, S = base.Fluid, base.Solid
+    s.add_operation(solver_api.SPHIntegration(sph.SPHDensityRate.withargs(), on_types=[F],
+                                              from_types=[F, S], updates=['rho'],
+                                              id='density'))
+    s.add_operation(solver_api.SPHOperation(
+        sph.TaitEquation.withargs(co=prm["c0"], ro=prm["rho0"], gamma=prm["gamma"]),
+        on_types=[F], updates=['p', 'cs'], id='eos'))
+    s.add_operation(solver_api.SPHIntegration(
+        sph.SPHPressureGradient.withargs(deltap=deltap, n=4), on_types=[F], from_types=[F],
+        updates=['u', 'v', 'w'], id='pgrad'))
+    s.add_operation(solver_api.SPHIntegration(
+        sph.MonaghanArtificialViscosity.withargs(alpha=prm["alpha"], beta=prm["beta"]),
+        on_types=[F], from_types=[F, S], updates=['u', 'v', 'w'], id='avisc'))
+    s.add_operation(solver_api.SPHIntegration(
+        sph.GravityForce.withargs(gz=prm["g"]), on_types=[F], updates=['u', 'v', 'w'],
+        id='gravity'))
+    s.add_operation_step([F])
+    if xsph_eps is not None:
+        s.add_operation_xsph(eps=xsph_eps)
+    s.set_time_step(prm["dt"])
+    s.set_output(False)
+    s.setup(particles)
+    return s, particles, (fl, wl, prm)

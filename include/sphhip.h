@@ -231,6 +231,26 @@ int sph_nnps_build_f32_dev(sph_ctx *ctx, const sph_grid *grid, const float *d_re
                            int64_t *slice_off, int32_t *indices, int64_t cap, int64_t *totals,
                            int32_t *status);
 
+/* Symmetric variable-h lists of VarHNbrParticleLocator (base/nnps.pyx:
+ * 691-786, chosen by NNPSManager(variable_h=True), :895-910), in the
+ * SELL-64 layout of sph_nnps_build.  Row of sorted destination t (LocalReal
+ * only; other rows empty): the sources within h_t*radius_scale in the
+ * stencil order of sph_nnps_build with the own index NOT moved last, then
+ * the real sources j (original index < ns_real) outside that radius whose
+ * own radius h_j*radius_scale reaches t, in ascending original index -- the
+ * order in which the reference appends the reversed locator's rows
+ * (:742-786).  FP64 records with h (d_rec, s_rec); s_perm / s_rank: the
+ * source's sorted->original and original->sorted permutations.  max_h: a
+ * bound of every h of both arrays (stencil reach ceil(max_h*rs/cell) <= 256).
+ * Reads the totals back (synchronises); returns SPH_ENEEDCAP with
+ * *slots_out filled when cap is too small (nothing written). */
+int sph_nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
+                        const int32_t *d_perm, int64_t nd_total, int64_t nd_real,
+                        const int32_t *s_cell_start, const double *s_rec, const int32_t *s_perm,
+                        const int32_t *s_rank, int64_t ns, int64_t ns_real, double radius_scale,
+                        double max_h, int32_t *counts, int64_t *slice_off, int32_t *indices,
+                        int64_t cap, int64_t *slots_out, int64_t *nnz_out);
+
 /* The same lists in ORIGINAL index space as CSR, the layout of the
  * reference's per-destination LongArrays (nnps.pyx:596-621): row i
  * (i < nd_real) holds original source indices in the same order.

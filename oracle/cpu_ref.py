@@ -47,6 +47,9 @@ def _load():
     lib.sphref_nnps.argtypes = [vp, vp, vp, vp, i64, vp, vp, vp, i64, dbl, dbl, i32, vp, vp, i32]
     lib.sphref_nnps.restype = i64
     lib.sphref_nnps_brute.argtypes = [vp, vp, vp, vp, i64, vp, vp, vp, i64, dbl, vp, vp, i32]
+    lib.sphref_nnps_varh.argtypes = [vp, vp, vp, vp, i64, vp, vp, vp, vp, i64, i64, dbl, dbl,
+                                     vp, vp]
+    lib.sphref_nnps_varh.restype = i64
     lib.sphref_nnps_brute.restype = i64
     lib.sphref_cubic_spline.argtypes = [i32, i64] + [vp] * 11
     lib.sphref_cubic_spline.restype = None
@@ -120,6 +123,25 @@ def nnps(dst, src, radius_scale, cell_size, same_array, nthreads=1, nd_real=None
     lib.sphref_nnps(_p(dx), _p(dy), _p(dz), _p(dh), nd, _p(sx), _p(sy), _p(sz), len(sx),
                     float(radius_scale), float(cell_size), int(bool(same_array)),
                     _p(offsets), _p(indices), nthreads)
+    return offsets, indices[:nnz]
+
+
+def nnps_varh(dst, src, radius_scale, cell_size, ns_real=None):
+    """VarHNbrParticleLocator lists (base/nnps.pyx:691-786): each row is the
+    destination's own query in stencil order (own index not moved) followed
+    by the real sources whose own radius reaches the destination, appended
+    in ascending source index.  ``dst``/``src`` need x, y, z, h.  Returns
+    ``(offsets int64[nd+1], indices int32[nnz])``."""
+    lib = _load()
+    dx, dy, dz, dh = (_f64(dst[k]) for k in ("x", "y", "z", "h"))
+    sx, sy, sz, sh = (_f64(src[k]) for k in ("x", "y", "z", "h"))
+    ns_real = len(sx) if ns_real is None else int(ns_real)
+    offsets = np.zeros(len(dx) + 1, dtype=np.int64)
+    args = (_p(dx), _p(dy), _p(dz), _p(dh), len(dx), _p(sx), _p(sy), _p(sz), _p(sh), len(sx),
+            ns_real, float(radius_scale), float(cell_size))
+    nnz = lib.sphref_nnps_varh(*args, _p(offsets), None)
+    indices = np.zeros(max(nnz, 1), dtype=np.int32)
+    lib.sphref_nnps_varh(*args, _p(offsets), _p(indices))
     return offsets, indices[:nnz]
 
 
@@ -315,7 +337,7 @@ class RefSimulation:
                  alpha=1.0, beta=1.0, eta=0.1, gravity=None, density_from=(0,),
                  momentum_from=(0,), cell_radius_scale=2.0, nthreads=1, density="summation",
                  xsph_eps=None, momentum="full", pgrad_from=(0,), avisc_eta=0.0, deltap=None,
-                 ap_n=1.0, ap_eps=0.2, bforce_from=(), bforce_delp=1.0):
+                 ap_n=1.0, ap_eps=0.2, bforce_from=(), bforce_delp=1.0, variable_h=False):
         self.arrays = arrays
         self.dim = dim
         self.integrator = integrator
@@ -344,6 +366,7 @@ class RefSimulation:
         self.deltap, self.ap_n, self.ap_eps = deltap, ap_n, ap_eps
         self.bforce_from = bforce_from
         self.bforce_delp = bforce_delp
+        self.variable_h = variable_h
         self.lists = None
         self.dirty = True
         self.pairs = 0
@@ -361,7 +384,11 @@ class RefSimulation:
                 used = (set(self.density_from) | set(self.momentum_from) |
                         set(self.pgrad_from) | set(self.bforce_from))
                 if s.ptype in used or (self.xsph_eps is not None and s.ptype == 0):
-                    self.lists[(d.name, s.name)] = nnps(d, s, 2.0, cs, d is s, self.nthreads)
+                    if self.variable_h:
+                        self.lists[(d.name, s.name)] = nnps_varh(d, s, 2.0, cs)
+                    else:
+                        self.lists[(d.name, s.name)] = nnps(d, s, 2.0, cs, d is s,
+                                                            self.nthreads)
         self.dirty = False
 
     def _lists(self):

@@ -19,6 +19,7 @@
  *   predicate      distance2(src,dst) < (h_dst*rs)^2  base/nnps.pyx:271-356, :525-559
  *                  distance2 order                    base/point.pxd:35-37
  *   self slot      own index swapped to the last slot base/nnps.pyx:596-621
+ *   variable h     own row + reversed rows appended   base/nnps.pyx:691-786
  *   cubic spline   W, gradW                           base/kernels.pyx:51-57, :349-453
  *   SPHRho         sum m_b W(r_ab, (h_a+h_b)/2)       sph/funcs/density_funcs.pyx:37-73
  *   Tait           p, cs                              sph/funcs/eos_funcs.pyx:80-129
@@ -193,6 +194,69 @@ int64_t sphref_nnps(const double *dx, const double *dy, const double *dz, const 
     }
     free(e);
     return offsets[nd_real];
+}
+
+/* Symmetric variable-h lists of VarHNbrParticleLocator._update_cache
+ * (base/nnps.pyx:742-786):
+ *   1. every destination's own query (radius dh[i]*radius_scale, stencil
+ *      order), with no move of the own index (the FixedDest swap of
+ *      :612-619 is not part of this _update_cache);
+ *   2. the reversed FixedDest locator's rows: for every real source j
+ *      (j < ns_real, the rows FixedDest fills, :603) the destinations within
+ *      sh[j]*radius_scale of source j;
+ *   3. for j ascending, for each destination i of j's reversed row, j is
+ *      appended to row i unless the row holds it already (LongArray.index).
+ * Rows for destinations [0, nd).  Pass indices == NULL for the offsets only;
+ * returns nnz. */
+typedef struct { int32_t *v; int64_t n, cap; } ref_row;
+
+static void row_push(ref_row *r, int32_t j)
+{
+    if (r->n == r->cap) {
+        r->cap = r->cap ? 2 * r->cap : 16;
+        r->v = (int32_t *)realloc(r->v, sizeof(int32_t) * (size_t)r->cap);
+    }
+    r->v[r->n++] = j;
+}
+
+int64_t sphref_nnps_varh(const double *dx, const double *dy, const double *dz, const double *dh,
+                         int64_t nd,
+                         const double *sx, const double *sy, const double *sz, const double *sh,
+                         int64_t ns, int64_t ns_real, double radius_scale, double cell_size,
+                         int64_t *offsets, int32_t *indices)
+{
+    ref_cell_entry *es = bin_sources(sx, sy, sz, ns, cell_size);
+    ref_cell_entry *ed = bin_sources(dx, dy, dz, nd, cell_size);
+    ref_row *rows = (ref_row *)calloc((size_t)(nd > 0 ? nd : 1), sizeof(ref_row));
+    int32_t *buf = (int32_t *)malloc(sizeof(int32_t) * (size_t)((nd > ns ? nd : ns) + 1));
+    for (int64_t i = 0; i < nd; ++i) {
+        int64_t len = query_one(es, ns, sx, sy, sz, dx[i], dy[i], dz[i], dh[i] * radius_scale,
+                                cell_size, buf);
+        for (int64_t k = 0; k < len; ++k) row_push(&rows[i], buf[k]);
+    }
+    for (int64_t j = 0; j < ns_real; ++j) {
+        int64_t len = query_one(ed, nd, dx, dy, dz, sx[j], sy[j], sz[j], sh[j] * radius_scale,
+                                cell_size, buf);
+        for (int64_t k = 0; k < len; ++k) {
+            ref_row *r = &rows[buf[k]];
+            int found = 0;
+            for (int64_t q = 0; q < r->n; ++q)
+                if (r->v[q] == (int32_t)j) { found = 1; break; }
+            if (!found) row_push(r, (int32_t)j);
+        }
+    }
+    offsets[0] = 0;
+    for (int64_t i = 0; i < nd; ++i) {
+        offsets[i + 1] = offsets[i] + rows[i].n;
+        if (indices)
+            for (int64_t q = 0; q < rows[i].n; ++q) indices[offsets[i] + q] = rows[i].v[q];
+        free(rows[i].v);
+    }
+    free(rows);
+    free(buf);
+    free(es);
+    free(ed);
+    return offsets[nd];
 }
 
 /* Brute-force O(N*M) neighbour sets in ascending index order, the same

@@ -536,13 +536,65 @@ class FixedDestNbrParticleLocator:
         return self.update_status()
 
 
+class VarHNbrParticleLocator(FixedDestNbrParticleLocator):
+    """Symmetric lists for variable smoothing lengths (base/nnps.pyx:691-786):
+    if b is a neighbour of a, a is a neighbour of b.  Row of destination a:
+    its own query (radius h_a * radius_scale, stencil order, own index not
+    moved last) followed by the real sources b outside that radius whose
+    own radius h_b * radius_scale reaches a, in ascending source index --
+    the rows the reference builds by appending the reversed locator's rows
+    (``sph_nnps_build_varh``).  As in the reference, ``exclude_self`` of
+    ``get_nearest_particles`` drops the last entry of the row."""
+
+    def _build(self):
+        cm = self.cell_manager
+        if cm.precision == "single":
+            raise NotImplementedError("variable-h lists are built in double precision only")
+        ctx = cm._ctx()
+        lib = ctx.lib
+        dst, src = self.dest, self.source
+        nd_total = dst.get_number_of_particles()
+        bd, bs = cm.bins[dst.name], cm.bins[src.name]
+        drec = cm.shadows(dst, ("x", "y", "z", "h"))
+        srec = cm.shadows(src, ("x", "y", "z", "h"))
+        dev = dst.device
+        nslices = (nd_total + 63) // 64
+        if self.counts is None or self.counts.numel() != max(nd_total, 1):
+            self.counts = torch.empty(max(nd_total, 1), dtype=torch.int32, device=dev)
+            self.slice_off = torch.empty(nslices + 1, dtype=torch.int64, device=dev)
+        if self.indices is None:
+            self.indices = torch.empty(max(1, 96 * nd_total), dtype=torch.int32, device=dev)
+        slots = ctypes.c_int64(0)
+        nnz = ctypes.c_int64(0)
+        p = _lib.ptr
+        # a bound of every h now (h may have been written since the binning)
+        max_h = max(float(a.get_carray("h")[:a.get_number_of_particles()].max())
+                    for a in (dst, src) if a.get_number_of_particles() > 0)
+
+        def call():
+            return lib.sph_nnps_build_varh(
+                ctx.handle, ctypes.byref(cm.grid), p(drec), p(bd.perm), nd_total,
+                dst.num_real_particles, p(bs.cell_start), p(srec), p(bs.perm), p(bs.rank), bs.n,
+                src.num_real_particles, self.radius_scale, max_h, p(self.counts),
+                p(self.slice_off),
+                p(self.indices), self.indices.numel(), ctypes.byref(slots), ctypes.byref(nnz))
+
+        rc = ctx.check(call(), allow=(_lib.SPH_ENEEDCAP,))
+        if rc == _lib.SPH_ENEEDCAP:
+            self.indices = torch.empty(int(slots.value * 1.15) + 4096, dtype=torch.int32,
+                                       device=dev)
+            ctx.check(call())
+        self.nnz = int(nnz.value)
+        self.slots = int(slots.value)
+        self.nbuilds += 1
+
+
 class NNPSManager:
-    """One locator per (source, dest, radius_scale) (base/nnps.pyx:791-932)."""
+    """One locator per (source, dest, radius_scale) (base/nnps.pyx:791-932);
+    ``variable_h`` selects the symmetric variable-h locators (:895-910)."""
 
     def __init__(self, cell_manager=None, variable_h=False,
                  locator_type=NeighborLocatorType.SPHNeighborLocator):
-        if variable_h:
-            raise NotImplementedError("variable-h locators are not part of this path")
         self.cell_manager = cell_manager
         self.variable_h = variable_h
         self.locator_type = locator_type
@@ -554,7 +606,8 @@ class NNPSManager:
         key = (id(source), id(dest), float(radius_scale))
         loc = self.particle_locator_cache.get(key)
         if loc is None:
-            loc = FixedDestNbrParticleLocator(source, dest, radius_scale, self.cell_manager)
+            cls = VarHNbrParticleLocator if self.variable_h else FixedDestNbrParticleLocator
+            loc = cls(source, dest, radius_scale, self.cell_manager)
             self.particle_locator_cache[key] = loc
         return loc
 

@@ -1458,3 +1458,198 @@ extern "C" int sph_nnps_export(sph_ctx *ctx, int64_t nd_real, const int32_t *d_r
     SPH_LAUNCH_CHECK(ctx, "k_export_rows");
     return SPH_OK;
 }
+
+// ---------------------------------------------------------------------------
+// symmetric variable-h lists (VarHNbrParticleLocator, base/nnps.pyx:691-786)
+// ---------------------------------------------------------------------------
+// Row of destination d = its own query (sources within h_d*rs, stencil order,
+// own index not moved) followed by the real sources j outside that radius
+// whose own radius h_j*rs reaches d, in ascending original index: the rows
+// the reference builds by appending each source of the reversed locator's
+// rows in ascending source order.  One lane per sorted destination walks the
+// stencil of the largest reach (ceil(max_h*rs/cell), max_h bounding every h
+// of both arrays); a destination's own stencil is a sub-box of it, visited
+// in the same lexicographic order.
+struct VarhArgs {
+    GridDev g;
+    const double *drec, *srec;
+    const int32_t *dperm, *sperm, *srank, *sstart;
+    int64_t nd_total, nd_real, ns_real;
+    double rs, rmax2;  // rmax2 = (max_h*rs)^2 bounds every source radius
+    int reach;
+    int32_t *fcount;   // per destination: entries of its own query
+};
+
+template <bool FILL>
+__global__ void __launch_bounds__(NN_THREADS)
+    k_varh_rows(VarhArgs a, int32_t *__restrict__ counts, const int64_t *__restrict__ slice_off,
+                int32_t *__restrict__ indices)
+{
+    const int64_t t = (int64_t)sph_xcd_tile(blockIdx.x, gridDim.x) * NN_THREADS + threadIdx.x;
+    if (t >= a.nd_total) return;
+    if (a.dperm[t] >= a.nd_real) {  // non-LocalReal destination: empty row
+        if (!FILL) {
+            counts[t] = 0;
+            a.fcount[t] = 0;
+        }
+        return;
+    }
+    const double *dr = a.drec + t * SPH_REC;
+    const double px = dr[SPH_R_X], py = dr[SPH_R_Y], pz = dr[SPH_R_Z];
+    const double radius = dr[SPH_R_H] * a.rs;
+    const double r2 = radius * radius;
+    const GridDev &g = a.g;
+    const int cx = min(max(cell_coord(px, g.cs) - g.cmin0, 0), g.d0 - 1);
+    const int cy = min(max(cell_coord(py, g.cs) - g.cmin1, 0), g.d1 - 1);
+    const int cz = min(max(cell_coord(pz, g.cs) - g.cmin2, 0), g.d2 - 1);
+    const int R = a.reach;
+    int32_t nf = 0, ne = 0;
+    int32_t *row = nullptr;
+    int32_t fbase = 0, n = 0;
+    if (FILL) {
+        n = counts[t];
+        if (n == 0) return;
+        row = indices + slice_off[t >> 6] + (t & 63);
+        fbase = a.fcount[t];
+    }
+    for (int ix = max(cx - R, 0); ix <= min(cx + R, g.d0 - 1); ++ix)
+        for (int iy = max(cy - R, 0); iy <= min(cy + R, g.d1 - 1); ++iy)
+            for (int iz = max(cz - R, 0); iz <= min(cz + R, g.d2 - 1); ++iz) {
+                const int32_t k = cell_key(g, ix, iy, iz);
+                const int32_t s1 = a.sstart[k + 1];
+                for (int32_t s = a.sstart[k]; s < s1; ++s) {
+                    const double *sr = a.srec + (int64_t)s * SPH_REC;
+                    // cPoint_distance2(src, dst), base/point.pxd:35-37
+                    const double ex = sr[SPH_R_X] - px, ey = sr[SPH_R_Y] - py,
+                                 ez = sr[SPH_R_Z] - pz;
+                    const double d2 = ex * ex + ey * ey + ez * ez;
+                    if (d2 < r2) {
+                        // the fill repeats the count pass's tests, so its
+                        // entries are the counted ones (the bounds only
+                        // guard the row)
+                        if (FILL && nf < fbase) row[(int64_t)nf << 6] = s;
+                        ++nf;
+                    } else if (d2 < a.rmax2) {
+                        // the source's own radius: (h_j*rs)^2 <= rmax2, so
+                        // no pair that reaches d is skipped by the bound
+                        const int32_t j = a.sperm[s];
+                        const double rj = sr[SPH_R_H] * a.rs;
+                        if (j < a.ns_real && d2 < rj * rj) {
+                            if (FILL && fbase + ne < n)
+                                row[(int64_t)(fbase + ne) << 6] = j;  // original index
+                            ++ne;
+                        }
+                    }
+                }
+            }
+    if (!FILL) {
+        counts[t] = nf + ne;
+        a.fcount[t] = nf;
+        return;
+    }
+    // appended sources in ascending original index (insertion sort; the
+    // appended part is short: it is empty where h does not vary), then
+    // back to sorted positions
+    ne = min(ne, n - fbase);
+    for (int32_t i = 1; i < ne; ++i) {
+        const int32_t v = row[(int64_t)(fbase + i) << 6];
+        int32_t q = i - 1;
+        while (q >= 0 && row[(int64_t)(fbase + q) << 6] > v) {
+            row[(int64_t)(fbase + q + 1) << 6] = row[(int64_t)(fbase + q) << 6];
+            --q;
+        }
+        row[(int64_t)(fbase + q + 1) << 6] = v;
+    }
+    for (int32_t i = 0; i < ne; ++i) {
+        int32_t *e = row + ((int64_t)(fbase + i) << 6);
+        *e = a.srank[*e];
+    }
+}
+
+extern "C" int sph_nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
+                                   const int32_t *d_perm, int64_t nd_total, int64_t nd_real,
+                                   const int32_t *s_cell_start, const double *s_rec,
+                                   const int32_t *s_perm, const int32_t *s_rank, int64_t ns,
+                                   int64_t ns_real, double radius_scale, double max_h,
+                                   int32_t *counts, int64_t *slice_off, int32_t *indices,
+                                   int64_t cap, int64_t *slots_out, int64_t *nnz_out)
+{
+    if (!ctx) return SPH_EINVAL;
+    SPH_REQUIRE(ctx, grid && counts && slice_off && slots_out && nnz_out,
+                "sph_nnps_build_varh: null argument");
+    SPH_REQUIRE(ctx, nd_total >= 0 && nd_total < ((int64_t)1 << 31) && nd_real >= 0 &&
+                         nd_real <= nd_total && ns >= 0 && ns < ((int64_t)1 << 31) &&
+                         ns_real >= 0 && ns_real <= ns,
+                "sph_nnps_build_varh: bad particle counts");
+    SPH_REQUIRE(ctx, ns == 0 || (s_cell_start && s_rec && s_perm && s_rank),
+                "sph_nnps_build_varh: null source arrays");
+    SPH_REQUIRE(ctx, nd_total == 0 || (d_rec && d_perm),
+                "sph_nnps_build_varh: null destination arrays");
+    *slots_out = 0;
+    *nnz_out = 0;
+    const int64_t nslices = (nd_total + 63) / 64;
+    if (nd_total == 0 || ns == 0 || nd_real == 0) return nnps_clear(ctx, nd_total, counts, slice_off);
+    SPH_REQUIRE(ctx, radius_scale >= 0.0 && grid->cell_size > 0.0 && max_h >= 0.0,
+                "sph_nnps_build_varh: bad radius scale, max_h or grid");
+    const double rmax = max_h * radius_scale;
+    const double reach = ceil(rmax / grid->cell_size);
+    SPH_REQUIRE(ctx, reach <= 256.0,
+                "sph_nnps_build_varh: the largest radius spans %g cells (at most 256)", reach);
+    int rc = sph_upload_key_tables(ctx, grid);
+    if (rc) return rc;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t fc_bytes = al((size_t)nd_total * sizeof(int32_t));
+    const size_t sums_bytes = al((size_t)(nslices + 1) * sizeof(int64_t) + 256);
+    rc = sph_ensure_scratch(ctx, fc_bytes + sums_bytes +
+                                     sph_scan_scratch_bytes(nslices, sizeof(int64_t)));
+    if (rc) return rc;
+    char *sc = (char *)ctx->scratch;
+    VarhArgs a;
+    a.g = make_griddev(ctx, grid);
+    a.drec = d_rec;
+    a.srec = s_rec;
+    a.dperm = d_perm;
+    a.sperm = s_perm;
+    a.srank = s_rank;
+    a.sstart = s_cell_start;
+    a.nd_total = nd_total;
+    a.nd_real = nd_real;
+    a.ns_real = ns_real;
+    a.rs = radius_scale;
+    a.rmax2 = rmax * rmax;
+    a.reach = (int)reach;
+    a.fcount = (int32_t *)sc;
+    int64_t *slice_sum = (int64_t *)(sc + fc_bytes);
+    int64_t *nnz_dev = slice_sum + nslices;
+    const unsigned grid_blocks = sph_blocks(nd_total, NN_THREADS);
+    hipLaunchKernelGGL(k_varh_rows<false>, dim3(grid_blocks), dim3(NN_THREADS), 0, ctx->stream,
+                       a, counts, (const int64_t *)nullptr, (int32_t *)nullptr);
+    SPH_LAUNCH_CHECK(ctx, "k_varh_rows<count>");
+    hipLaunchKernelGGL(k_slices, dim3(sph_blocks(nslices * 64, NN_THREADS)), dim3(NN_THREADS), 0,
+                       ctx->stream, counts, nd_total, nslices, slice_off, slice_sum);
+    SPH_LAUNCH_CHECK(ctx, "k_slices");
+    hipLaunchKernelGGL(k_sum_i64, dim3(1), dim3(1024), 0, ctx->stream, slice_sum, nslices,
+                       nnz_dev);
+    SPH_LAUNCH_CHECK(ctx, "k_sum_i64");
+    rc = sph_scan_exclusive_i64_at(ctx, slice_off, nslices, fc_bytes + sums_bytes);
+    if (rc) return rc;
+    rc = sph_ensure_pinned(ctx, 2 * sizeof(int64_t));
+    if (rc) return rc;
+    SPH_HIP_CHECK(ctx, hipMemcpyAsync(ctx->pinned, slice_off + nslices, sizeof(int64_t),
+                                      hipMemcpyDeviceToHost, ctx->stream));
+    SPH_HIP_CHECK(ctx, hipMemcpyAsync((char *)ctx->pinned + sizeof(int64_t), nnz_dev,
+                                      sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    SPH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const int64_t slots = ((int64_t *)ctx->pinned)[0];
+    *slots_out = slots;
+    *nnz_out = ((int64_t *)ctx->pinned)[1];
+    if (slots > cap || !indices) {
+        sph_set_error(ctx, SPH_ENEEDCAP, "neighbour capacity %lld < %lld", (long long)cap,
+                      (long long)slots);
+        return SPH_ENEEDCAP;
+    }
+    hipLaunchKernelGGL(k_varh_rows<true>, dim3(grid_blocks), dim3(NN_THREADS), 0, ctx->stream,
+                       a, counts, (const int64_t *)slice_off, indices);
+    SPH_LAUNCH_CHECK(ctx, "k_varh_rows<fill>");
+    return SPH_OK;
+}

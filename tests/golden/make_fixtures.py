@@ -10,6 +10,7 @@ Sources (paths relative to the reference root):
   base/tests/test_nnps.py:23-73            brute-force query counts
   base/tests/test_nnps.py:94-181           point queries (radius -> neighbours)
   base/tests/test_nnps.py:221-270          FixedDest locator rows
+  base/tests/test_variable_h.py:14-81      variable-h 1D set-up and symmetric rows
   base/tests/test_kernels.py:33-50,111-213 1D cubic-spline quadrature setup
   sph/tests/function_test_template.py:45-58  4-particle unit square
   sph/tests/test_eos_funcs.py:88-116       Tait closed form parameters
@@ -64,6 +65,12 @@ FIXTURES = {
                       ["parr2", [1.0, 1.0, 1.0], 1.88, [0, 1, 2, 3]]],
     # FixedDest rows, source parr1, dest parr2: radius_scale, dest index, expected set
     "locator_rows": [[0.5, 3, [0]], [1.0, 3, [0, 1, 2, 3]], [4.0, 2, [0, 1, 2, 3, 4, 5, 6, 7]]],
+    # VarHNbrParticleLocator rows (sorted sets): 11 particles on linspace(0, 1, 11),
+    # h = dx + 1e-10 except particle 5 with twice that, radius_scale 2
+    "varh_1d": {"n": 11, "h_extra": 1e-10, "wide": 5, "radius_scale": 2.0,
+                "rows": [[0, 1, 2], [0, 1, 2, 3, 5], [0, 1, 2, 3, 4, 5], [1, 2, 3, 4, 5],
+                         [2, 3, 4, 5, 6], [1, 2, 3, 4, 5, 6, 7, 8, 9], [4, 5, 6, 7, 8],
+                         [5, 6, 7, 8, 9], [5, 6, 7, 8, 9, 10], [5, 7, 8, 9, 10], [8, 9, 10]]},
     "kernel_1d": {"x0": -1.0, "x1": 1.0, "n": 21, "h": 0.2,
                   "sum_w": 1.0, "sum_gw": 0.0, "moment_w": 0.0, "moment_gw": -1.0,
                   "places": 10},

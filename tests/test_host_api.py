@@ -125,8 +125,20 @@ def test_unsupported_features_fail_loudly():
         sph.MomentumEquation.withargs(deltap=-0.1).get_func(pa, pa)
     with pytest.raises(NotImplementedError):
         sph.SPHRho.withargs(hks=True).get_func(pa, pa)
-    with pytest.raises(NotImplementedError):
-        base.Particles([pa], variable_h=True)
+
+
+def test_variable_h_selects_symmetric_locators():
+    """NNPSManager(variable_h=True) hands out VarHNbrParticleLocator
+    (base/nnps.pyx:895-910), the plain manager FixedDestNbrParticleLocator."""
+    pa = base.get_particle_array(name="f", x=np.zeros(2))
+    p = base.Particles([pa], variable_h=True)
+    loc = p.nnps_manager.get_neighbor_particle_locator(pa, pa, 2.0)
+    assert isinstance(loc, base.VarHNbrParticleLocator)
+    assert p.nnps_manager.get_neighbor_particle_locator(pa, pa, 2.0) is loc
+    p2 = base.Particles([base.get_particle_array(name="g", x=np.zeros(2))])
+    assert type(p2.nnps_manager.get_neighbor_particle_locator(p2.arrays[0], p2.arrays[0],
+                                                              2.0)) is \
+        base.FixedDestNbrParticleLocator
 
 
 def test_compute_without_device_raises():

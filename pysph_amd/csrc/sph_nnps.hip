@@ -692,6 +692,21 @@ __device__ __forceinline__ int64_t wave_bcast_i64(int64_t v, int l)
     return ((int64_t)hi << 32) | (uint32_t)lo;
 }
 
+// lo[l] = xlo and hi[l] = xhi for the wave-uniform lane l and scalars xlo,
+// xhi (v_writelane_b32 with the lane select in M0: one scalar operand per
+// instruction besides M0 keeps within the constant bus; values and lane are
+// written by scalar code).  M0 is declared clobbered; the count pass, its
+// only caller, uses M0 for nothing else.
+__device__ __forceinline__ void wave_writelane2(uint32_t &lo, uint32_t &hi, uint32_t xlo,
+                                                uint32_t xhi, int l)
+{
+    asm volatile("s_mov_b32 m0, %4\n\ts_nop 0\n\tv_writelane_b32 %0, %2, m0\n\t"
+                 "v_writelane_b32 %1, %3, m0"
+                 : "+v"(lo), "+v"(hi)
+                 : "s"(xlo), "s"(xhi), "s"(l)
+                 : "m0");
+}
+
 __device__ __forceinline__ uint64_t wave_bcast_u64(uint64_t v, int l)
 {
     const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, l);
@@ -850,7 +865,22 @@ __device__ void nnps_cell(const CellArgs &a, int64_t w, int32_t *__restrict__ co
                     sz = sr[SPH_R_Z];
                 }
                 uint32_t mlo = 0, mhi = 0;  // count: my destination's hit mask of this chunk
-                for (int d = 0; d < gn; ++d) {
+                if (!FILL) {
+                    // every lane runs this loop (EXEC is full): the hit mask
+                    // of destination d is the ballot of the compare, kept in
+                    // lane d with v_writelane (no per-lane select)
+                    const uint64_t validm = __builtin_amdgcn_ballot_w64(valid);
+                    for (int d = 0; d < gn; ++d) {
+                        if (!((realmask >> d) & 1)) continue;  // non-LocalReal: empty row
+                        const auto D = dsts[d];  // same address in every lane
+                        // cPoint_distance2(src, dst), base/point.pxd:35-37
+                        const T ex = sx - D.x, ey = sy - D.y, ez = sz - D.z;
+                        const T d2 = ex * ex + ey * ey + ez * ez;
+                        const uint64_t mask = __builtin_amdgcn_ballot_w64(d2 < D.w) & validm;
+                        wave_writelane2(mlo, mhi, (uint32_t)mask, (uint32_t)(mask >> 32), d);
+                    }
+                }
+                for (int d = 0; FILL && d < gn; ++d) {
                     if (!((realmask >> d) & 1)) continue;  // non-LocalReal: empty row
                     const int32_t t = g0 + d;
                     const auto D = dsts[d];  // same address in every lane
@@ -860,12 +890,7 @@ __device__ void nnps_cell(const CellArgs &a, int64_t w, int32_t *__restrict__ co
                     const bool hit = valid && d2 < D.w;
                     const uint64_t mask = __ballot(hit);
                     if (mask == 0) continue;
-                    if (!FILL) {
-                        if (lane == d) {
-                            mlo = (uint32_t)mask;
-                            mhi = (uint32_t)(mask >> 32);
-                        }
-                    } else {
+                    {
                         const int nh = __popcll(mask);
                         const int64_t addr0 = wave_bcast_i64(myaddr, d);
                         const int32_t k0 = __builtin_amdgcn_readlane(mycount, d);

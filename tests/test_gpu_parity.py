@@ -337,6 +337,41 @@ def test_dambreak_small_pc_vs_oracle(gpu):
     assert np.array_equal(particles.arrays[1].get('x'), wl["x"])
 
 
+def test_dambreak_full_size_pc_step_vs_oracle(gpu):
+    """The 1M-fluid dam break (752,208 wall particles) at full size: both
+    neighbour lists identical to the oracle's (order included) at the initial
+    and at the stepped positions, and one PredictorCorrector step within 1e-6
+    of the oracle's."""
+    from pysph_amd.solver.configs import dambreak_solver
+    from pysph_amd.solver.generators import dam_break_3d
+    data = dam_break_3d(dx=0.01)
+    s, particles, (fl, wl, prm) = dambreak_solver(data=data)
+    fluid, wall = particles.arrays
+    mom = [c for c in s.integrator.calcs if c.id.startswith("mom")][0]
+    locs = [f.nbr_locator for f in mom.funcs]
+
+    def check_lists(label):
+        cur = {k: fluid.get(k) for k in ("x", "y", "z", "h")}
+        for loc, src, same in ((locs[0], cur, True), (locs[1], wl, False)):
+            off, idx = (t.cpu().numpy() for t in loc.get_csr())
+            cs = particles.cell_manager.cell_size
+            off_r, idx_r = cpu_ref.nnps(cur, src, 2.0, cs, same, nthreads=16)
+            print(f"{label} {loc.source.name}: nnz {len(idx)}")
+            assert np.array_equal(off, off_r) and np.array_equal(idx, idx_r), label
+
+    check_lists("initial")
+    ref = cpu_ref.RefSimulation([cpu_ref.RefArray("fluid", 0, fl), cpu_ref.RefArray("wall", 1, wl)],
+                                dim=3, integrator="pc", co=prm["c0"], ro=prm["rho0"],
+                                gamma=7.0, alpha=0.1, beta=0.0, gravity=(0.0, 0.0, prm["g"]),
+                                density_from=(0, 1), momentum_from=(0, 1), nthreads=16)
+    s.integrator.integrate(prm["dt"])
+    ref.integrate(prm["dt"])
+    worst = _compare_arrays(fluid, ref.arrays[0], ("x", "y", "z", "u", "v", "w", "rho", "p"),
+                            "dambreak 1M")
+    assert worst <= TOL
+    check_lists("stepped")
+
+
 def test_device_side_rebuild_matches_and_reports_capacity(gpu):
     """Builds after the first run without a host synchronisation: the lists
     equal the synchronous build's, and a build that runs out of list capacity

@@ -249,3 +249,105 @@ def test_split_form_dam_break_vs_oracle(gpu):
         ref.integrate(prm["dt"])
     worst = _state_err(particles, ref, ("x", "y", "z", "u", "v", "w", "rho", "p"))
     assert worst <= TOL
+
+
+def test_split_form_single_precision_tracks_double(gpu):
+    """cl_precision="single": the float pair operators (sph_pair_accel_f32:
+    pressure gradient with artificial pressure, Monaghan viscosity from fluid
+    and walls) with float continuity and XSPH over 3 PC steps of the split
+    dam break stay within 1e-4 (max error / max magnitude; pressures / Tait's
+    B) of the FP64 run."""
+    from pysph_amd.base import api as base
+    from pysph_amd.sph import api as sph
+    from pysph_amd.solver import api as solver
+    from pysph_amd.solver.generators import dam_break_3d
+    fl, wl, prm = dam_break_3d(dx=0.04)
+    runs = {}
+    for prec in ("double", "single"):
+        fluid = base.get_particle_array(name="fluid", type=base.Fluid, cl_precision=prec, **fl)
+        wall = base.get_particle_array(name="wall", type=base.Solid, cl_precision=prec, **wl)
+        particles = base.Particles([fluid, wall])
+        assert particles.cell_manager.precision == prec
+        F, S = base.Fluid, base.Solid
+        s = solver.Solver(dim=3, integrator_type=solver.PredictorCorrectorIntegrator)
+        s.add_operation(solver.SPHIntegration(sph.SPHDensityRate.withargs(), on_types=[F],
+                                              from_types=[F, S], updates=['rho'], id='density'))
+        s.add_operation(solver.SPHOperation(
+            sph.TaitEquation.withargs(co=prm["c0"], ro=prm["rho0"], gamma=prm["gamma"]),
+            on_types=[F], updates=['p', 'cs'], id='eos'))
+        s.add_operation(solver.SPHIntegration(
+            sph.SPHPressureGradient.withargs(deltap=0.04, n=4), on_types=[F], from_types=[F],
+            updates=['u', 'v', 'w'], id='pgrad'))
+        s.add_operation(solver.SPHIntegration(
+            sph.MonaghanArtificialViscosity.withargs(alpha=prm["alpha"], beta=prm["beta"]),
+            on_types=[F], from_types=[F, S], updates=['u', 'v', 'w'], id='avisc'))
+        s.add_operation(solver.SPHIntegration(sph.GravityForce.withargs(gz=prm["g"]),
+                                              on_types=[F], updates=['u', 'v', 'w'],
+                                              id='gravity'))
+        s.add_operation_step([F])
+        s.add_operation_xsph(eps=0.5)
+        s.set_output(False)
+        s.setup(particles)
+        rho0 = fluid.get("rho")
+        rho0[fl["z"] > 1.5] *= 0.999  # a suction band: the artificial pressure acts
+        fluid.set(rho=rho0)
+        for _ in range(3):
+            s.integrator.integrate(prm["dt"])
+        runs[prec] = fluid
+    errs = {}
+    # pressures near rest are small differences (rho/rho0)^7 - 1: their error
+    # is measured against Tait's scale B = c0^2 rho0 / gamma
+    B = prm["c0"] ** 2 * prm["rho0"] / prm["gamma"]
+    for k in ("x", "y", "z", "u", "v", "w", "rho", "p"):
+        a, b = runs["single"].get(k), runs["double"].get(k)
+        scale = B if k == "p" else np.max(np.abs(b))
+        errs[k] = float(np.max(np.abs(a - b)) / scale)
+    print("split dam break FP32 vs FP64 after 3 steps:",
+          " ".join(f"{k} {v:.2g}" for k, v in errs.items()))
+    assert max(errs.values()) <= 1e-4
+
+
+def test_wall_force_single_precision_tracks_double(gpu):
+    """cl_precision="single" for the 2D dam break with Monaghan's wall force
+    and momentum with artificial pressure (the float pair kernel's wall-force
+    and momentum branches): 20 Euler steps within 1e-4 of the FP64 run."""
+    from pysph_amd.base import api as base
+    from pysph_amd.sph import api as sph
+    from pysph_amd.solver import api as solver
+    from pysph_amd.solver.generators import dam_break_2d_monaghan
+    fd, wd, dx = dam_break_2d_monaghan()
+    runs = {}
+    for prec in ("double", "single"):
+        fluid = base.get_particle_array(name="fluid", type=base.Fluid, cl_precision=prec, **fd)
+        walls = base.get_particle_array(name="boundary", type=base.Solid, cl_precision=prec,
+                                        **wd)
+        particles = base.Particles([fluid, walls])
+        F, S = base.Fluid, base.Solid
+        s = solver.Solver(dim=2, integrator_type=solver.EulerIntegrator)
+        s.add_operation(solver.SPHOperation(sph.TaitEquation.withargs(co=65.0, ro=1000.0),
+                                            on_types=[F], updates=['p', 'cs'], id='eos'))
+        s.add_operation(solver.SPHIntegration(sph.SPHDensityRate.withargs(), on_types=[F],
+                                              from_types=[F], updates=['rho'], id='density'))
+        s.add_operation(solver.SPHIntegration(
+            sph.MomentumEquation.withargs(alpha=0.5, beta=0.0, deltap=dx), on_types=[F],
+            from_types=[F], updates=['u', 'v'], id='mom'))
+        s.add_operation(solver.SPHIntegration(sph.GravityForce.withargs(gy=-9.81),
+                                              on_types=[F], updates=['u', 'v'], id='gravity'))
+        s.add_operation(solver.SPHIntegration(sph.MonaghanBoundaryForce.withargs(delp=dx),
+                                              on_types=[F], from_types=[S], updates=['u', 'v'],
+                                              id='bforce'))
+        s.add_operation_step([F])
+        s.set_output(False)
+        s.setup(particles)
+        for _ in range(20):
+            s.integrator.integrate(1e-4)
+        runs[prec] = fluid
+    errs = {}
+    B = 65.0 ** 2 * 1000.0 / 7.0  # Tait's pressure scale (the p errors' yardstick)
+    for k in ("x", "y", "u", "v", "rho", "p"):
+        a, b = runs["single"].get(k), runs["double"].get(k)
+        scale = B if k == "p" else np.max(np.abs(b))
+        errs[k] = float(np.max(np.abs(a - b)) / scale)
+    print("2D dam break (wall force, deltap) FP32 vs FP64 after 20 steps:",
+          " ".join(f"{k} {v:.2g}" for k, v in errs.items()))
+    assert max(errs.values()) <= 1e-4

@@ -15,7 +15,15 @@
 // (measured on the 100^3 lattice with 75 neighbours per particle: density
 // 0.34 ms at 4, 0.31 ms at 8; momentum 0.66 ms at 2, 0.60 ms at 4)
 #define DENS_NT 8
+#ifndef MOM_NT
 #define MOM_NT 4
+#endif
+// waves per SIMD the momentum kernel is compiled for: 3 caps it at 168
+// registers (172 unbounded, which allows 2 waves), with two 8-byte spills
+// outside the neighbour loop; 0.657 -> 0.649 ms on the 1M dam break
+#ifndef MOM_WAVES
+#define MOM_WAVES 3
+#endif
 
 // ---------------------------------------------------------------------------
 // SPHRho (rho_term in sph_terms.h)
@@ -349,7 +357,7 @@ __device__ __forceinline__ void mom_row(int dim, const MomDest<T> &D, T alpha, T
 }
 
 template <bool UH, typename T>
-__global__ void __launch_bounds__(OP_THREADS)
+__global__ void __launch_bounds__(OP_THREADS) __attribute__((amdgpu_waves_per_eu(MOM_WAVES)))
     k_momentum(int dim, int64_t nd_total, int64_t nd_real, const int32_t *__restrict__ dperm,
                SrcLists<T> L, const T *__restrict__ drec, T alpha, T beta, T eta,
                int accumulate, double *__restrict__ ax, double *__restrict__ ay,

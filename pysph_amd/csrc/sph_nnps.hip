@@ -624,8 +624,9 @@ struct CellArgs {
     double rs;
     int same;
     // hit masks of the count pass: chunk c of sorted destination t at
-    // masks[c * mstride + t]; cell_ovf[w] = 1 when cell w has more than
-    // NN_MAXCH chunks (its fill recomputes the distance tests)
+    // masks[c * mstride + t]; cell_ovf[w] = the chunks of cell w's stencil,
+    // or -1 when it has more than maxch chunks or more than one stencil batch
+    // (its fill recomputes the distance tests)
     uint64_t *masks;
     int64_t mstride;
     int32_t *cell_ovf;
@@ -932,7 +933,7 @@ __device__ void nnps_cell(const CellArgs &a, int64_t w, int32_t *__restrict__ co
     }
     if (!FILL && lane == 0) {
         const int ovf = nst > MAXSEG || nchunks > a.maxch;
-        a.cell_ovf[w] = ovf;
+        a.cell_ovf[w] = ovf ? -1 : nchunks;
         if (ovf) a.ovf_list[atomicAdd(a.novf, 1)] = (int32_t)w;
     }
 }
@@ -986,6 +987,7 @@ __global__ void __launch_bounds__(NN_THREADS)
     k_nnps_fill_slices(CellArgs a, int64_t nd_total, int32_t *__restrict__ counts,
                        const int64_t *__restrict__ slice_off, int32_t *__restrict__ indices)
 {
+    __shared__ uint64_t s_mask[NN_MAXCH][NN_THREADS];
     if (a.totals && blockIdx.x == 0 && threadIdx.x == 0) {
         // totals and capacity checks of a device-side build: status bit 0
         // when the lists need more than cap slots (nothing is written), bit
@@ -1008,38 +1010,64 @@ __global__ void __launch_bounds__(NN_THREADS)
     const int32_t n = counts[t];
     if (n == 0) return;
     const int32_t w = a.dcell_of[t];
-    if (a.cell_ovf[w]) return;  // written by the per-cell pass
+    const int32_t nch = a.cell_ovf[w];  // chunks of the cell's stencil
+    if (nch < 0) return;                // written by the per-cell pass
     int32_t *row = indices + slice_off[t >> 6] + (t & 63);
     const int32_t *cand = a.cand + (int64_t)w * (a.maxch * CW);
     const uint64_t *mk = a.masks + t;
+    // the row's chunk masks, loaded together and kept in LDS (each lane reads
+    // only its own column): walking them later waits on LDS, not on the
+    // global stores of the row
+    uint64_t v[NN_MAXCH];
+#pragma unroll
+    for (int c = 0; c < NN_MAXCH; ++c) v[c] = c < nch ? mk[(int64_t)c * a.mstride] : 0ull;
+#pragma unroll
+    for (int c = 0; c < NN_MAXCH; ++c) s_mask[c][threadIdx.x] = v[c];
     int c = 0;
-    uint64_t m = mk[0];
+    uint64_t m = v[0];
     int32_t self_k = -1, last_p = -1;
-    // FILL_UNROLL candidates per round: their table loads are in flight together
+    // FILL_UNROLL candidates per round; the next round's table loads are
+    // issued before this round's stores, so a round never waits for the
+    // stores before it
     constexpr int FILL_UNROLL = 8;
-    for (int32_t k0 = 0; k0 < n; k0 += FILL_UNROLL) {
-        int32_t q[FILL_UNROLL];
+    int32_t q[FILL_UNROLL], p[FILL_UNROLL];
+    auto next_round = [&](int32_t k0, int32_t *qq, int32_t *pp) {
 #pragma unroll
         for (int j = 0; j < FILL_UNROLL; ++j) {
-            q[j] = -1;
+            qq[j] = -1;
             if (k0 + j < n) {
-                while (m == 0 && c + 1 < a.maxch) m = mk[(int64_t)(++c) * a.mstride];
+                while (m == 0 && c + 1 < nch) m = s_mask[++c][threadIdx.x];
                 if (m != 0) {  // always: the masks hold exactly n bits
-                    q[j] = (c << 6) + __ffsll((unsigned long long)m) - 1;
+                    qq[j] = (c << 6) + __ffsll((unsigned long long)m) - 1;
                     m &= m - 1;
                 }
             }
         }
-        int32_t p[FILL_UNROLL];
+        // unconditional loads (entry 0 of the cell's table stands in for an
+        // empty slot): straight-line code, so each store waits only for its
+        // own round's loads
 #pragma unroll
-        for (int j = 0; j < FILL_UNROLL; ++j) p[j] = q[j] >= 0 ? cand[q[j]] : -1;
+        for (int j = 0; j < FILL_UNROLL; ++j) pp[j] = cand[qq[j] >= 0 ? qq[j] : 0];
+    };
+    auto store_round = [&](int32_t k0, const int32_t *qq, const int32_t *pp) {
 #pragma unroll
         for (int j = 0; j < FILL_UNROLL; ++j) {
-            if (q[j] < 0) continue;
-            row[(int64_t)(k0 + j) << 6] = p[j];
-            if (p[j] == t) self_k = k0 + j;
-            last_p = p[j];
+            if (qq[j] < 0) continue;
+            row[(int64_t)(k0 + j) << 6] = pp[j];
+            if (pp[j] == t) self_k = k0 + j;
+            last_p = pp[j];
         }
+    };
+    // two rounds per iteration, alternating buffers (no register copies
+    // between rounds)
+    int32_t qn[FILL_UNROLL], pn[FILL_UNROLL];
+    next_round(0, q, p);
+    for (int32_t k0 = 0; k0 < n; k0 += 2 * FILL_UNROLL) {
+        next_round(k0 + FILL_UNROLL, qn, pn);
+        store_round(k0, q, p);
+        if (k0 + FILL_UNROLL >= n) break;
+        next_round(k0 + 2 * FILL_UNROLL, q, p);
+        store_round(k0 + FILL_UNROLL, qn, pn);
     }
     if (a.same && self_k >= 0 && self_k < n - 1) {
         // keep the particle itself at the last index (nnps.pyx:612-619)

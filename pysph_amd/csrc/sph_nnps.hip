@@ -487,7 +487,12 @@ __global__ void __launch_bounds__(NN_THREADS)
     if (t >= n) return;
     const int32_t i = perm[t];
     const int64_t o = t * pl.dst_stride;
-    for (int k = 0; k < pl.nprops; ++k) pl.dst[k][o] = pl.src[k][i];
+    double v[SPH_MAX_PROPS];  // all loads before the first store (in-order retirement)
+#pragma unroll
+    for (int k = 0; k < SPH_MAX_PROPS; ++k) v[k] = k < pl.nprops ? pl.src[k][i] : 0.0;
+#pragma unroll
+    for (int k = 0; k < SPH_MAX_PROPS; ++k)
+        if (k < pl.nprops) pl.dst[k][o] = v[k];
 }
 
 __global__ void __launch_bounds__(NN_THREADS)
@@ -497,7 +502,12 @@ __global__ void __launch_bounds__(NN_THREADS)
     if (t >= n) return;
     const int32_t i = perm[t];
     const int64_t o = t * pl.dst_stride;
-    for (int k = 0; k < pl.nprops; ++k) pl.dst[k][o] = (float)pl.src[k][i];
+    double v[SPH_MAX_PROPS];  // all loads before the first store (in-order retirement)
+#pragma unroll
+    for (int k = 0; k < SPH_MAX_PROPS; ++k) v[k] = k < pl.nprops ? pl.src[k][i] : 0.0;
+#pragma unroll
+    for (int k = 0; k < SPH_MAX_PROPS; ++k)
+        if (k < pl.nprops) pl.dst[k][o] = (float)v[k];
 }
 
 // the same records written from the original order: thread i reads field k
@@ -510,16 +520,28 @@ __global__ void __launch_bounds__(NN_THREADS)
     int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
     if (i >= n) return;
     const int64_t o = (int64_t)rank[i] * pl.dst_stride;
-    for (int k = 0; k < pl.nprops; ++k) {
+    // every field is loaded before the first store: memory operations retire
+    // in order, so a load issued after a store would wait for that store
+    double v[SPH_MAX_PROPS];
+#pragma unroll
+    for (int k = 0; k < SPH_MAX_PROPS; ++k) v[k] = k < pl.nprops ? pl.src[k][i] : 0.0;
+    bool taken = false;  // field k went out with field k - 1
+#pragma unroll
+    for (int k = 0; k < SPH_MAX_PROPS; ++k) {
+        if (k >= pl.nprops) break;
+        if (taken) {
+            taken = false;
+            continue;
+        }
         double *d = pl.dst[k] + o;
         // two fields that are adjacent 16-byte-aligned words of the row go
         // out as one 16-byte store (the test is the same for every lane)
         if (k + 1 < pl.nprops && pl.dst[k + 1] == pl.dst[k] + 1 &&
             ((uintptr_t)d & 15) == 0) {
-            *reinterpret_cast<double2 *>(d) = make_double2(pl.src[k][i], pl.src[k + 1][i]);
-            ++k;
+            *reinterpret_cast<double2 *>(d) = make_double2(v[k], v[k + 1]);
+            taken = true;
         } else {
-            *d = pl.src[k][i];
+            *d = v[k];
         }
     }
 }
@@ -530,7 +552,12 @@ __global__ void __launch_bounds__(NN_THREADS)
     int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
     if (i >= n) return;
     const int64_t o = (int64_t)rank[i] * pl.dst_stride;
-    for (int k = 0; k < pl.nprops; ++k) pl.dst[k][o] = (float)pl.src[k][i];
+    double v[SPH_MAX_PROPS];  // all loads before the first store (in-order retirement)
+#pragma unroll
+    for (int k = 0; k < SPH_MAX_PROPS; ++k) v[k] = k < pl.nprops ? pl.src[k][i] : 0.0;
+#pragma unroll
+    for (int k = 0; k < SPH_MAX_PROPS; ++k)
+        if (k < pl.nprops) pl.dst[k][o] = (float)v[k];
 }
 
 template <typename PL, typename K>

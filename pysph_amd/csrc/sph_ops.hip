@@ -201,8 +201,9 @@ __global__ void __launch_bounds__(OP_THREADS)
     if (t >= n_total) return;
     const int32_t i = perm[t];
     T r0 = T(0.0), r1 = T(0.0);
+    // read once, before the stores (a load behind a store waits for it)
+    const T rhoa = rho_s[t * st];
     if (i < n_real) {
-        const T rhoa = rho_s[t * st];
         const T ratio = rhoa / ro;
         const T gamma2 = T(0.5) * (gamma - T(1.0));
         const T tmp = pow(ratio, gamma);
@@ -221,8 +222,7 @@ __global__ void __launch_bounds__(OP_THREADS)
     if (p_s) p_s[t * st] = r0;
     if (cs_s) cs_s[t * st] = r1;
     if (prho2_s) {  // k_prho2's two operations on the record's p and rho
-        const T rhob = rho_s[t * st];
-        const T rhob2 = T(1.0) / (rhob * rhob);
+        const T rhob2 = T(1.0) / (rhoa * rhoa);
         prho2_s[t * st] = r0 * rhob2;
     }
 }

@@ -308,19 +308,37 @@ __global__ void __launch_bounds__(NN_THREADS)
                const double *__restrict__ z, int64_t n, int32_t *__restrict__ count,
                int32_t *__restrict__ cell_of, int32_t *__restrict__ rank)
 {
-    int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
-    if (i >= n) return;
-    int cx = cell_coord(x[i], g.cs) - g.cmin0;
-    int cy = cell_coord(y[i], g.cs) - g.cmin1;
-    int cz = cell_coord(z[i], g.cs) - g.cmin2;
-    // every particle lies in the grid by construction (sph_grid_build); clamp
-    // defensively so a stale grid can never index outside the tables
-    cx = min(max(cx, 0), g.d0 - 1);
-    cy = min(max(cy, 0), g.d1 - 1);
-    cz = min(max(cz, 0), g.d2 - 1);
-    int32_t key = cell_key(g, cx, cy, cz);
-    cell_of[i] = key;
-    rank[i] = atomicAdd(&count[key], 1);
+    const int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
+    const bool act = i < n;  // the active lanes are a prefix of the wavefront
+    int32_t key = -1;
+    if (act) {
+        int cx = cell_coord(x[i], g.cs) - g.cmin0;
+        int cy = cell_coord(y[i], g.cs) - g.cmin1;
+        int cz = cell_coord(z[i], g.cs) - g.cmin2;
+        // every particle lies in the grid by construction (sph_grid_build);
+        // clamp defensively so a stale grid can never index outside the tables
+        cx = min(max(cx, 0), g.d0 - 1);
+        cy = min(max(cy, 0), g.d1 - 1);
+        cz = min(max(cz, 0), g.d2 - 1);
+        key = cell_key(g, cx, cy, cz);
+        cell_of[i] = key;
+    }
+    // consecutive lanes with the same cell (neighbours in memory are mostly
+    // neighbours in space) take their slots with one atomic per run; slots
+    // only need to be distinct within a cell (k_bin_rank orders them)
+    const int lane = __lane_id();
+    const int32_t prev = __shfl_up(key, 1, 64);
+    const bool head = act && (lane == 0 || prev != key);
+    const uint64_t heads = __builtin_amdgcn_ballot_w64(head);
+    const uint64_t acts = __builtin_amdgcn_ballot_w64(act);
+    const uint64_t le = heads & ((2ull << lane) - 1ull);  // heads at or below my lane
+    const int start = le ? 63 - __clzll(le) : 0;
+    const uint64_t above = heads & ~((2ull << start) - 1ull);
+    const int next = above ? __ffsll((long long)above) - 1 : __popcll(acts);
+    int32_t base = 0;
+    if (head) base = atomicAdd(&count[key], next - start);
+    base = __shfl(base, start, 64);
+    if (act) rank[i] = base + (lane - start);
 }
 
 __global__ void __launch_bounds__(NN_THREADS)

@@ -66,7 +66,8 @@ MAX_SRC = 4
 class Sources(ctypes.Structure):
     """sph_sources (include/sphhip.h)."""
     _fields_ = [("nsrc", ctypes.c_int32), ("counts", _vp * MAX_SRC), ("slice_off", _vp * MAX_SRC),
-                ("indices", _vp * MAX_SRC), ("s_rec", _vp * MAX_SRC), ("ns", _i64 * MAX_SRC)]
+                ("indices", _vp * MAX_SRC), ("s_rec", _vp * MAX_SRC), ("ns", _i64 * MAX_SRC),
+                ("s_pm", _vp * MAX_SRC)]
 
 
 PAIR_PRESSURE_GRADIENT = 1

@@ -34,6 +34,8 @@ REC = 12
 REC_FIELDS = {"x": 0, "y": 1, "z": 2, "m": 3, "u": 4, "v": 5, "w": 6, "rho": 8, "cs": 9,
               "h": 10, "p": 11}
 PRHO2_FIELD = 7  # P/rho^2, written by the library (SPH_R_PRHO2)
+#: fields of the compact position-mass rows, in row order
+PM_FIELDS = ("x", "y", "z", "m")
 SHADOW_PROPS = tuple(REC_FIELDS)
 
 
@@ -64,6 +66,10 @@ class _Bins:
         self.pending = None
         self.rec = torch.zeros((max(n, 1), REC), dtype=dtype, device=device)
         self.shadow_ver = {}
+        # compact sorted (x, y, z, m) rows (FP64 only): the neighbour gathers
+        # of density and momentum touch fewer cache lines than in the records
+        self.pm = None
+        self.pm_ver = {}
 
     def poll(self):
         """Take the device cell count of the last binning if it has arrived,
@@ -83,6 +89,7 @@ class _Bins:
 
     def invalidate(self):
         self.shadow_ver = {}
+        self.pm_ver = {}
 
 
 def _grid_signature(g):
@@ -257,6 +264,37 @@ class CellManager:
             for p in stale:
                 b.shadow_ver[p] = array.version(p)
         return b.rec
+
+    def posmass(self, array):
+        """The compact sorted (x, y, z, m) rows of `array` (4 doubles per
+        particle, rows in record order) with stale fields refreshed, or None
+        in single precision (the float records are read instead)."""
+        if self.precision == "single":
+            return None
+        self.update()
+        b = self.bins[array.name]
+        if b.pm is None or b.pm.shape[0] != max(b.n, 1):
+            b.pm = torch.empty((max(b.n, 1), 4), dtype=torch.float64, device=b.device)
+            b.pm_ver = {}
+        stale = [q for q in PM_FIELDS if b.pm_ver.get(q) != array.version(q)]
+        if stale:
+            ctx = self._ctx()
+            pl = _lib.PropList()
+            pl.nprops = len(stale)
+            pl.dst_stride = 4
+            base = b.pm.data_ptr()
+            for k, q in enumerate(stale):
+                src = array.get_carray(q)
+                if src is None or src.dtype != torch.float64:
+                    raise TypeError(f"property {q} of {array.name} must be float64")
+                pl.src[k] = _lib.ptr(src)
+                pl.dst[k] = base + 8 * PM_FIELDS.index(q)
+            with profiling.region("gather_sorted"):
+                ctx.check(ctx.lib.sph_scatter_sorted(ctx.handle, b.n, _lib.ptr(b.rank),
+                                                     ctypes.byref(pl)))
+            for q in stale:
+                b.pm_ver[q] = array.version(q)
+        return b.pm
 
     def shadow_out(self, array, prop):
         """Device pointer of the record field a kernel may write together

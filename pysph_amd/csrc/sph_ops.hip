@@ -32,7 +32,7 @@
 template <bool UH, typename T>
 __device__ __forceinline__ T rho_row(int dim, T xa, T ya, T za, T ha, T facA,
                                      const int32_t *__restrict__ row, int32_t e,
-                                     const T *__restrict__ srec)
+                                     const T *__restrict__ srec, const T *__restrict__ spm)
 {
     T nr = T(0.0);
     int32_t k = 0;
@@ -48,9 +48,9 @@ __device__ __forceinline__ T rho_row(int dim, T xa, T ya, T za, T ha, T facA,
             j[u] = __builtin_nontemporal_load(row + ((int64_t)(k + u) << 6));
 #pragma unroll
         for (int u = 0; u < DENS_NT; ++u) {
-            const auto *sr = rec2(srec, j[u]);
-            a[u] = sr[0];
-            b[u] = sr[1];
+            const auto *sp = pm2(spm, srec, j[u]);
+            a[u] = sp[0];
+            b[u] = sp[1];
             hb[u] = UH ? ha : srec[(int64_t)j[u] * SPH_REC + SPH_R_H];
         }
 #pragma unroll
@@ -59,9 +59,9 @@ __device__ __forceinline__ T rho_row(int dim, T xa, T ya, T za, T ha, T facA,
     }
     for (; k < e; ++k) {
         const int32_t j = __builtin_nontemporal_load(row + ((int64_t)k << 6));
-        const auto *sr = rec2(srec, j);
+        const auto *sp = pm2(spm, srec, j);
         const T hb = UH ? ha : srec[(int64_t)j * SPH_REC + SPH_R_H];
-        nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, sr[0], sr[1], hb);
+        nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, sp[0], sp[1], hb);
     }
     return nr;
 }
@@ -99,7 +99,8 @@ __global__ void __launch_bounds__(OP_THREADS)
         const T facA = sph_h_dim(ha, dim) * sph_cubic_fac1<T>(dim);
         for (int s = 0; s < L.nsrc; ++s) {
             const int32_t *row = L.indices[s] + L.slice_off[s][t >> 6] + (t & 63);
-            val = val + rho_row<UH, T>(dim, xa, ya, za, ha, facA, row, e[s], L.srec[s]);
+            val = val + rho_row<UH, T>(dim, xa, ya, za, ha, facA, row, e[s], L.srec[s],
+                                       L.spm[s]);
         }
     }
     rho[i] = val;

@@ -107,7 +107,17 @@ template <typename T> struct SrcLists {
     const int64_t *slice_off[SPH_MAX_SRC];
     const int32_t *indices[SPH_MAX_SRC];
     const T *srec[SPH_MAX_SRC];
+    const T *spm[SPH_MAX_SRC];  // compact (x, y, z, m) rows, or NULL: the records
 };
+
+// pairs 0 and 1 of source j ((x, y) and (z, m)): from the compact rows when
+// given (4 values per particle), else from the records
+template <typename T>
+__device__ __forceinline__ const typename Pair<T>::type *pm2(const T *spm, const T *srec,
+                                                             int64_t j)
+{
+    return spm ? reinterpret_cast<const typename Pair<T>::type *>(spm + j * 4) : rec2(srec, j);
+}
 
 template <typename T>
 static int src_lists(sph_ctx *ctx, const char *name, const sph_sources *src, SrcLists<T> *L)
@@ -121,6 +131,7 @@ static int src_lists(sph_ctx *ctx, const char *name, const sph_sources *src, Src
         L->slice_off[s] = src->slice_off[s];
         L->indices[s] = src->indices[s];
         L->srec[s] = (const T *)src->s_rec[s];
+        L->spm[s] = (const T *)src->s_pm[s];
     }
     return SPH_OK;
 }

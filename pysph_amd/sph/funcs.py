@@ -93,8 +93,10 @@ class SPHRho(CSPHFunctionParticle):
             loc = f.nbr_locator
             cnt, soff, idx = loc.get_sorted_lists()
             srec = cm.shadows(f.source, ('x', 'y', 'z', 'h', 'm'))
+            pm = cm.posmass(f.source)
             src.counts[k], src.slice_off[k], src.indices[k] = p(cnt), p(soff), p(idx)
             src.s_rec[k], src.ns[k] = p(srec), 0
+            src.s_pm[k] = p(pm) if pm is not None else None
             profiling.count_pairs(loc.pair_count())
         name = names[0] if names else None
         read = ('x', 'y', 'z', 'h') + ((name,) if name in REC_FIELDS and accumulate else ())

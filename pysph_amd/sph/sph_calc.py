@@ -94,8 +94,9 @@ class SPHCalc:
             self.dest.touch(prop)
         funcs = self.funcs
         multi = getattr(type(funcs[0]), "eval_multi", None) if funcs else None
-        if multi is not None and 1 < len(funcs) <= _lib.MAX_SRC and type(funcs[0]).fusable(funcs):
-            # one function per source array: one launch with the same sums
+        if multi is not None and 1 <= len(funcs) <= _lib.MAX_SRC and type(funcs[0]).fusable(funcs):
+            # one function per source array: one launch with the same sums (a
+            # single source too: the launch also reads the compact rows)
             multi(funcs, self.kernel, outs[0], outs[1], outs[2], accumulate=accumulate,
                   names=names)
         else:

@@ -390,21 +390,30 @@ __global__ void __launch_bounds__(FUSED_MAX_THREADS)
         const double acc[3] = {(double)(T(0.0) + nr0), (double)(T(0.0) + nr1),
                                (double)(T(0.0) + nr2)};
         const double dt = dt_sim ? scale * dt_sim[sim_of[i]] : scale;
-        for (int c = 0; c < dim; ++c) {
-            const double uc = st.u[c][i], xc = st.x[c][i];
-            double u0 = uc, x0 = xc;
+        // every component is read before the first store (memory operations
+        // retire in order: a load behind a store waits for it)
+        double uc[3] = {0.0, 0.0, 0.0}, xc[3] = {0.0, 0.0, 0.0};
+        double u0[3] = {0.0, 0.0, 0.0}, x0[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (c >= dim) break;
+            uc[c] = st.u[c][i];
+            xc[c] = st.x[c][i];
+            u0[c] = save ? uc[c] : st.u0[c][i];
+            x0[c] = save ? xc[c] : st.x0[c][i];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (c >= dim) break;
             if (save) {
-                st.u0[c][i] = uc;
-                st.x0[c][i] = xc;
-            } else {
-                u0 = st.u0[c][i];
-                x0 = st.x0[c][i];
+                st.u0[c][i] = uc[c];
+                st.x0[c][i] = xc[c];
             }
             st.au[c][i] = acc[c];
-            st.ax[c][i] = uc;
-            const double tu = acc[c] * dt, tx = uc * dt;
-            st.u[c][i] = u0 + tu;
-            st.x[c][i] = x0 + tx;
+            st.ax[c][i] = uc[c];
+            const double tu = acc[c] * dt, tx = uc[c] * dt;
+            st.u[c][i] = u0[c] + tu;
+            st.x[c][i] = x0[c] + tx;
         }
     }
 }

@@ -178,21 +178,39 @@ __global__ void __launch_bounds__(BT_THREADS)
                  int32_t *__restrict__ slot, int32_t *__restrict__ status)
 {
     const int64_t i = (int64_t)blockIdx.x * BT_THREADS + threadIdx.x;
-    if (i >= n) return;
-    const GridDev g = grids[sim_of[i]];
-    int64_t key = key_cap - 1;  // trash cell of void simulations
-    if (g.d0 > 0) {
-        const int cx = min(max(cell_coord(x[i], g.cs) - g.cmin0, 0), g.d0 - 1);
-        const int cy = min(max(cell_coord(y[i], g.cs) - g.cmin1, 0), g.d1 - 1);
-        const int cz = min(max(cell_coord(z[i], g.cs) - g.cmin2, 0), g.d2 - 1);
-        key = (int64_t)g.key_base + ((int64_t)cx * g.d1 + cy) * g.d2 + cz;
-        if (key >= key_cap - 1) {  // cannot happen after k_batch_key_base; kept in bounds
-            atomicOr(status, 2);
-            key = key_cap - 1;
+    const bool act = i < n;  // the active lanes are a prefix of the wavefront
+    int32_t key32 = -1;
+    if (act) {
+        const GridDev g = grids[sim_of[i]];
+        int64_t key = key_cap - 1;  // trash cell of void simulations
+        if (g.d0 > 0) {
+            const int cx = min(max(cell_coord(x[i], g.cs) - g.cmin0, 0), g.d0 - 1);
+            const int cy = min(max(cell_coord(y[i], g.cs) - g.cmin1, 0), g.d1 - 1);
+            const int cz = min(max(cell_coord(z[i], g.cs) - g.cmin2, 0), g.d2 - 1);
+            key = (int64_t)g.key_base + ((int64_t)cx * g.d1 + cy) * g.d2 + cz;
+            if (key >= key_cap - 1) {  // cannot happen after k_batch_key_base; kept in bounds
+                atomicOr(status, 2);
+                key = key_cap - 1;
+            }
         }
+        key32 = (int32_t)key;
+        cell_of[i] = key32;
     }
-    cell_of[i] = (int32_t)key;
-    slot[i] = atomicAdd(&count[key], 1);
+    // one atomic per run of consecutive lanes in the same cell (slots need
+    // only be distinct within a cell)
+    const int lane = __lane_id();
+    const int32_t prev = __shfl_up(key32, 1, 64);
+    const bool head = act && (lane == 0 || prev != key32);
+    const uint64_t heads = __builtin_amdgcn_ballot_w64(head);
+    const uint64_t acts = __builtin_amdgcn_ballot_w64(act);
+    const uint64_t le = heads & ((2ull << lane) - 1ull);
+    const int start = le ? 63 - __clzll(le) : 0;
+    const uint64_t above = heads & ~((2ull << start) - 1ull);
+    const int next = above ? __ffsll((long long)above) - 1 : __popcll(acts);
+    int32_t base = 0;
+    if (head) base = atomicAdd(&count[key32], next - start);
+    base = __shfl(base, start, 64);
+    if (act) slot[i] = base + (lane - start);
 }
 
 // more non-empty cells than the list kernels are launched for: status |= 16

@@ -741,16 +741,16 @@ __device__ __forceinline__ int64_t wave_bcast_i64(int64_t v, int l)
 // lo[l] = xlo and hi[l] = xhi for the wave-uniform lane l and scalars xlo,
 // xhi (v_writelane_b32 with the lane select in M0: one scalar operand per
 // instruction besides M0 keeps within the constant bus; values and lane are
-// written by scalar code).  M0 is declared clobbered; the count pass, its
-// only caller, uses M0 for nothing else.
+// written by scalar code).  M0 is saved and restored around the two writes.
 __device__ __forceinline__ void wave_writelane2(uint32_t &lo, uint32_t &hi, uint32_t xlo,
                                                 uint32_t xhi, int l)
 {
-    asm volatile("s_mov_b32 m0, %4\n\ts_nop 0\n\tv_writelane_b32 %0, %2, m0\n\t"
-                 "v_writelane_b32 %1, %3, m0"
-                 : "+v"(lo), "+v"(hi)
-                 : "s"(xlo), "s"(xhi), "s"(l)
-                 : "m0");
+    uint32_t m0_saved;
+    asm volatile("s_mov_b32 %2, m0\n\ts_mov_b32 m0, %5\n\ts_nop 0\n\t"
+                 "v_writelane_b32 %0, %3, m0\n\tv_writelane_b32 %1, %4, m0\n\t"
+                 "s_mov_b32 m0, %2"
+                 : "+v"(lo), "+v"(hi), "=&s"(m0_saved)
+                 : "s"(xlo), "s"(xhi), "s"(l));
 }
 
 __device__ __forceinline__ uint64_t wave_bcast_u64(uint64_t v, int l)

@@ -668,12 +668,27 @@ def cpu_baseline_dambreak(args, data):
     t0 = time.perf_counter()
     ref.integrate(prm["dt"])
     t1 = time.perf_counter()
+    # the reference's Cython loops are serial: one thread on a bounded sample
+    # (one evaluation: both lists, SPHRho, Tait, momentum)
+    ref1 = cpu_ref.RefSimulation([cpu_ref.RefArray("fluid", 0, fl),
+                                  cpu_ref.RefArray("wall", 1, wl)],
+                                 dim=3, integrator="pc", co=prm["c0"], ro=prm["rho0"], gamma=7.0,
+                                 alpha=0.1, beta=0.0, gravity=(0.0, 0.0, prm["g"]),
+                                 density_from=(0, 1), momentum_from=(0, 1), nthreads=1)
+    acc = {d.name: {k: np.zeros(d.n) for k in ref1.stepped} for d in ref1.fluids}
+    s0 = time.perf_counter()
+    ref1.eval(acc)
+    s1 = time.perf_counter()
     return {"value": ref.pairs / (t1 - t0), "unit": "pair-interactions/s", "cores": threads,
             "kind": "port",
             "sample": f"1 PredictorCorrector step of the same dam break ({ref.pairs} pair "
                       f"interactions, {t1 - t0:.2f} s, {threads} OpenMP threads, gcc -O2 "
                       f"-ffp-contract=off)",
-            "steps_per_s": 1.0 / (t1 - t0)}
+            "steps_per_s": 1.0 / (t1 - t0),
+            "single_thread": {"value": ref1.pairs / (s1 - s0), "cores": 1,
+                              "sample": f"1 evaluation (both neighbour lists, SPHRho, Tait, "
+                                        f"momentum) of the same dam break, {ref1.pairs} pair "
+                                        f"interactions, {s1 - s0:.2f} s"}}
 
 
 def cpu_baseline_dambreak_split(args, data):

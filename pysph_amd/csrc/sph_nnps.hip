@@ -1629,13 +1629,16 @@ __global__ void __launch_bounds__(NN_THREADS)
                         ++nf;
                     } else if (d2 < a.rmax2) {
                         // the source's own radius: (h_j*rs)^2 <= rmax2, so
-                        // no pair that reaches d is skipped by the bound
-                        const int32_t j = a.sperm[s];
+                        // no pair that reaches d is skipped by the bound; the
+                        // source's original index is read for the hits only
                         const double rj = sr[SPH_R_H] * a.rs;
-                        if (j < a.ns_real && d2 < rj * rj) {
-                            if (FILL && fbase + ne < n)
-                                row[(int64_t)(fbase + ne) << 6] = j;  // original index
-                            ++ne;
+                        if (d2 < rj * rj) {
+                            const int32_t j = a.sperm[s];
+                            if (j < a.ns_real) {
+                                if (FILL && fbase + ne < n)
+                                    row[(int64_t)(fbase + ne) << 6] = j;  // original index
+                                ++ne;
+                            }
                         }
                     }
                 }

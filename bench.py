@@ -15,7 +15,9 @@ neighbour-operator evaluation (density and momentum, both sources, both
 stages) per second of wall time.
 
 ``--workload lattice`` runs the NNPS + SummationDensity micro-workload
-(100^3 jittered lattice, h = 1.2 dx) instead; ``--workload dambreak-split``
+(100^3 jittered lattice, h = 1.2 dx; ``--h-jitter 0.2 --variable-h`` gives every
+particle its own h and the symmetric variable-h lists) instead;
+``--workload dambreak-split``
 the same dam break with the operator split of examples/dam-break/
 dam_break3d.py (continuity, pressure gradient and Monaghan viscosity as
 separate functions, XSPH).
@@ -67,6 +69,10 @@ def _args():
                          "comparison of --workload batch (0: all)")
     ap.add_argument("--dx", type=float, default=0.01)
     ap.add_argument("--n", type=int, default=100, help="lattice edge for --workload lattice")
+    ap.add_argument("--h-jitter", type=float, default=0.0,
+                    help="--workload lattice: h scaled by U(1-f, 1+f) per particle (seeded)")
+    ap.add_argument("--variable-h", action="store_true",
+                    help="--workload lattice: Particles(variable_h=True), the symmetric lists")
     ap.add_argument("--cell-order", choices=("morton", "linear"), default="morton")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -301,8 +307,11 @@ def run_lattice(args):
     from pysph_amd.solver.generators import jittered_lattice_3d
 
     d = jittered_lattice_3d(n=args.n)
+    if args.h_jitter > 0:
+        rng = np.random.default_rng(1)
+        d["h"] = d["h"] * rng.uniform(1 - args.h_jitter, 1 + args.h_jitter, len(d["x"]))
     pa = base.get_particle_array(name="fluid", cl_precision=args.precision, **d)
-    particles = base.Particles([pa], cell_order=args.cell_order)
+    particles = base.Particles([pa], cell_order=args.cell_order, variable_h=args.variable_h)
     kernel = base.CubicSplineKernel(dim=3)
     func = sph.SPHRho.withargs().get_func(pa, pa)
     calc = sph.SPHCalc(particles, [pa], pa, kernel, [func], ['rho'])
@@ -345,7 +354,10 @@ def run_lattice(args):
         "ms_per_step": 1e3 * elapsed / args.steps,
         "dtype": "f32" if args.precision == "single" else "f64",
         "config": {"workload": f"3D NNPS + SummationDensity, {args.n}^3 jittered lattice, "
-                               f"{'FP32' if args.precision == 'single' else 'FP64'}",
+                               f"{'FP32' if args.precision == 'single' else 'FP64'}"
+                               + (f", h x U(1-{args.h_jitter}, 1+{args.h_jitter})"
+                                  if args.h_jitter > 0 else "")
+                               + (", variable-h lists" if args.variable_h else ""),
                    "particles": n, "nnz": nnz, "steps_per_s": args.steps / elapsed,
                    "kernel_ms": {k: round(v["mean_ms"], 4) for k, v in breakdown.items()},
                    "kernel_ms_note": "per library call, from an instrumented run outside the "
@@ -720,13 +732,18 @@ def cpu_baseline_lattice(args, d):
     threads = args.cpu_threads or len(os.sched_getaffinity(0))
     t0 = time.perf_counter()
     cs = cpu_ref.compute_cell_size(d["h"].max(), 2.0)
-    off, idx = cpu_ref.nnps(d, d, 2.0, cs, True, nthreads=threads)
+    if args.variable_h:  # the oracle's variable-h lists are serial
+        off, idx = cpu_ref.nnps_varh(d, d, 2.0, cs)
+    else:
+        off, idx = cpu_ref.nnps(d, d, 2.0, cs, True, nthreads=threads)
     rho = np.zeros(len(d["x"]))
     cpu_ref.summation_density(3, d, d, off, idx, rho, nthreads=threads)
     t1 = time.perf_counter()
     return {"value": len(idx) / (t1 - t0), "unit": "pair-interactions/s", "cores": threads,
             "kind": "port", "sample": f"1 NNPS + SummationDensity pass of the same lattice "
-                                      f"({t1 - t0:.2f} s, {threads} OpenMP threads)"}
+                                      f"({t1 - t0:.2f} s, {threads} OpenMP threads"
+                                      + (", lists built serially" if args.variable_h else "")
+                                      + ")"}
 
 
 def main():

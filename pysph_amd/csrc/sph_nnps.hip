@@ -1578,11 +1578,16 @@ struct VarhArgs {
     int32_t *fcount;   // per destination: entries of its own query
 };
 
+// appended entries a fill lane sorts in LDS (one column per lane, 40 KB per
+// block); a longer appended part is sorted in place in the row
+#define VARH_LDS 40
+
 template <bool FILL>
 __global__ void __launch_bounds__(NN_THREADS)
     k_varh_rows(VarhArgs a, int32_t *__restrict__ counts, const int64_t *__restrict__ slice_off,
                 int32_t *__restrict__ indices)
 {
+    __shared__ int32_t s_app[FILL ? VARH_LDS : 1][NN_THREADS];
     const int64_t t = (int64_t)sph_xcd_tile(blockIdx.x, gridDim.x) * NN_THREADS + threadIdx.x;
     if (t >= a.nd_total) return;
     if (a.dperm[t] >= a.nd_real) {  // non-LocalReal destination: empty row
@@ -1610,6 +1615,9 @@ __global__ void __launch_bounds__(NN_THREADS)
         row = indices + slice_off[t >> 6] + (t & 63);
         fbase = a.fcount[t];
     }
+    // the count pass found n - fbase appended entries: every LDS index below
+    // is < n - fbase <= VARH_LDS
+    const bool in_lds = FILL && n - fbase <= VARH_LDS;
     for (int ix = max(cx - R, 0); ix <= min(cx + R, g.d0 - 1); ++ix)
         for (int iy = max(cy - R, 0); iy <= min(cy + R, g.d1 - 1); ++iy)
             for (int iz = max(cz - R, 0); iz <= min(cz + R, g.d2 - 1); ++iz) {
@@ -1635,8 +1643,12 @@ __global__ void __launch_bounds__(NN_THREADS)
                         if (d2 < rj * rj) {
                             const int32_t j = a.sperm[s];
                             if (j < a.ns_real) {
-                                if (FILL && fbase + ne < n)
-                                    row[(int64_t)(fbase + ne) << 6] = j;  // original index
+                                if (FILL && fbase + ne < n) {  // original index
+                                    if (in_lds)
+                                        s_app[ne][threadIdx.x] = j;
+                                    else
+                                        row[(int64_t)(fbase + ne) << 6] = j;
+                                }
                                 ++ne;
                             }
                         }
@@ -1652,6 +1664,20 @@ __global__ void __launch_bounds__(NN_THREADS)
     // appended part is short: it is empty where h does not vary), then
     // back to sorted positions
     ne = min(ne, n - fbase);
+    if (in_lds) {
+        for (int32_t i = 1; i < ne; ++i) {
+            const int32_t v = s_app[i][threadIdx.x];
+            int32_t q = i - 1;
+            while (q >= 0 && s_app[q][threadIdx.x] > v) {
+                s_app[q + 1][threadIdx.x] = s_app[q][threadIdx.x];
+                --q;
+            }
+            s_app[q + 1][threadIdx.x] = v;
+        }
+        for (int32_t i = 0; i < ne; ++i)
+            row[(int64_t)(fbase + i) << 6] = a.srank[s_app[i][threadIdx.x]];
+        return;
+    }
     for (int32_t i = 1; i < ne; ++i) {
         const int32_t v = row[(int64_t)(fbase + i) << 6];
         int32_t q = i - 1;

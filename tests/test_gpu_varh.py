@@ -77,6 +77,30 @@ def test_rows_vs_oracle_one_array(gpu, n, dim, seed, spread):
     assert np.array_equal(idx, idx_r)
 
 
+def test_rows_with_long_appended_parts(gpu):
+    """h spread over a factor 10: small-h destinations collect many sources
+    whose own radius reaches them, so some appended parts pass 40 entries
+    (the fill sorts those in the row instead of in LDS) and the rest stay
+    short; every row matches the oracle's."""
+    base, _, _ = _api()
+    rng = np.random.default_rng(6)
+    n = 8000
+    h0 = (8.0 / n) ** (1.0 / 3.0) * 2.0
+    d = _cloud(rng, n, 3, h0, (0.1, 1.0))
+    pa = base.get_particle_array(name="fluid", **d)
+    particles = base.Particles([pa], variable_h=True)
+    loc = particles.nnps_manager.get_neighbor_particle_locator(pa, pa, 2.0)
+    off, idx = (t.cpu().numpy() for t in loc.get_csr())
+    cs = particles.cell_manager.cell_size
+    off_r, idx_r = cpu_ref.nnps_varh(d, d, 2.0, cs)
+    appended = np.diff(off_r) - np.diff(cpu_ref.nnps(d, d, 2.0, cs, True)[0])
+    print(f"appended entries per row: max {appended.max()}, rows over 40: "
+          f"{int((appended > 40).sum())} of {n}")
+    assert appended.max() > 40 and np.median(appended) <= 40
+    assert np.array_equal(off, off_r)
+    assert np.array_equal(idx, idx_r)
+
+
 def test_rows_vs_oracle_two_arrays_with_non_real(gpu):
     """Fluid rows over a wall array whose last particles are not LocalReal:
     only real sources reach back (FixedDest fills rows of real particles,

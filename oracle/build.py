@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "sphref.c")
 LIB = os.path.join(HERE, "libsphref.so")
 
-CFLAGS = ["-O2", "-fPIC", "-shared", "-std=c11", "-ffp-contract=off",
+CFLAGS = ["-O3", "-fPIC", "-shared", "-std=c11", "-ffp-contract=off",
           "-fno-fast-math", "-fopenmp", "-Wall", "-Wno-unknown-pragmas"]
 
 

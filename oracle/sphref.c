@@ -7,7 +7,7 @@
  *
  * It restates, in plain C with the same expression order, the Cython
  * arithmetic of the reference (sabago/pysph, PySPH 0.9beta).  Each function
- * cites the reference file:line it follows.  Build flags: -O2
+ * cites the reference file:line it follows.  Build flags: -O3
  * -ffp-contract=off (no FMA contraction, like the reference's baseline
  * x86-64/SSE2 build), optional OpenMP for the multi-core baseline.
  *

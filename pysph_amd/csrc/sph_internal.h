@@ -108,6 +108,10 @@ __global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v);
 int sph_bin_sort(sph_ctx *ctx, int64_t n, int64_t nc, const int32_t *cell_of, int32_t *rank,
                  int32_t *cell_start, int32_t *perm, int32_t *cells, int32_t *slot, int32_t *tmp,
                  int32_t trash, size_t scratch_off);
+// end of the context scratch sph_bin_sort uses from scratch_off on (scan
+// partials and the list of cells with many members); a caller that keeps
+// pointers into the scratch below scratch_off ensures this size first
+size_t sph_bin_sort_scratch_end(int64_t n, int64_t nc, size_t scratch_off);
 
 // Neighbour lists of a batch of simulations packed in one array, without
 // host synchronisation (capturable in a graph): grids[sim_of[i]] is the grid

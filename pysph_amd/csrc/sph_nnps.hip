@@ -350,25 +350,120 @@ __global__ void __launch_bounds__(NN_THREADS)
     tmp[start[cell_of[i]] + slot[i]] = (int32_t)i;
 }
 
+// Cells with at most RANK_DIRECT members are ordered by counting (each
+// member scans its cell); larger ones by k_bin_rank_large, a merge sort of
+// the cell's members, so a cell holding most of the particles (a large h
+// sets the cell size, or a clustered input) costs O(m log^2 m), not O(m^2).
+#define RANK_DIRECT 1024
+#define RANK_CHUNK 2048        // members sorted in LDS per pass of k_bin_rank_large
+#define RANK_LARGE_GRID 256    // blocks of k_bin_rank_large (one large cell at a time each)
+
 // Ascending particle index inside every cell (the atomics above hand out
 // slots in arbitrary order): particle i's position is the number of members
-// of its cell with a smaller index.  Writes perm and its inverse rank.
+// of its cell with a smaller index.  Writes perm and its inverse rank.  The
+// members of a cell with more than RANK_DIRECT of them are left to
+// k_bin_rank_large: the member holding slot 0 lists the cell there.
 __global__ void __launch_bounds__(NN_THREADS)
     k_bin_rank(const int32_t *__restrict__ cell_of, int64_t n, const int32_t *__restrict__ start,
                const int32_t *__restrict__ tmp, int32_t trash, int32_t *__restrict__ perm,
-               int32_t *__restrict__ rank)
+               int32_t *__restrict__ rank, int32_t *__restrict__ large, int32_t *__restrict__ nlarge)
 {
     int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
     if (i >= n) return;
     const int32_t key = cell_of[i];
     const int32_t s = start[key], e = start[key + 1];
     int32_t r = 0;
-    if (key == trash)
+    if (key == trash) {
         r = rank[i];  // unordered: the atomic slot (no lists are built for this cell)
-    else
+    } else if (e - s > RANK_DIRECT) {
+        if (rank[i] == 0) large[atomicAdd(nlarge, 1)] = key;  // slot 0: once per cell
+        return;
+    } else {
         for (int32_t a = s; a < e; ++a) r += tmp[a] < (int32_t)i;
+    }
     perm[s + r] = (int32_t)i;
     rank[i] = s + r;
+}
+
+// number of the n ascending values of a that are smaller than v
+__device__ __forceinline__ int64_t lower_bound_i32(const int32_t *a, int64_t n, int32_t v)
+{
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (a[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// The members of each listed large cell in ascending order: chunks of
+// RANK_CHUNK sorted in LDS (bitonic), then pairwise merges of runs of width
+// w, 2w, ... between tmp[s..e) and perm[s..e) (a member's place in the
+// merged run is its place in its own run plus the number of smaller members
+// of the other run; members are distinct indices).  The result ends in
+// perm[s..e) and rank is its inverse.  One block per cell at a time.
+__global__ void __launch_bounds__(NN_THREADS)
+    k_bin_rank_large(const int32_t *__restrict__ start, int32_t *__restrict__ tmp,
+                     const int32_t *__restrict__ large, const int32_t *__restrict__ nlarge,
+                     int32_t *__restrict__ perm, int32_t *__restrict__ rank)
+{
+    __shared__ int32_t buf[RANK_CHUNK];
+    const int32_t nl = *nlarge;
+    const int tid = threadIdx.x;
+    for (int32_t c = blockIdx.x; c < nl; c += gridDim.x) {
+        const int32_t key = large[c];
+        const int32_t s = start[key];
+        const int64_t m = (int64_t)start[key + 1] - s;
+        int32_t *a = tmp + s;
+        int32_t *b = perm + s;
+        for (int64_t c0 = 0; c0 < m; c0 += RANK_CHUNK) {
+            const int64_t len = m - c0 < RANK_CHUNK ? m - c0 : RANK_CHUNK;
+            for (int k = tid; k < RANK_CHUNK; k += NN_THREADS)
+                buf[k] = k < len ? a[c0 + k] : INT32_MAX;
+            __syncthreads();
+            for (int k = 2; k <= RANK_CHUNK; k <<= 1) {
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    for (int t = tid; t < RANK_CHUNK; t += NN_THREADS) {
+                        const int u = t ^ j;
+                        if (u > t) {
+                            const int32_t x = buf[t], y = buf[u];
+                            if ((x > y) == ((t & k) == 0)) {
+                                buf[t] = y;
+                                buf[u] = x;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            for (int k = tid; k < len; k += NN_THREADS) a[c0 + k] = buf[k];
+            __syncthreads();
+        }
+        int32_t *src = a, *dst = b;
+        for (int64_t w = RANK_CHUNK; w < m; w *= 2) {
+            for (int64_t e = tid; e < m; e += NN_THREADS) {
+                const int64_t p0 = e / (2 * w) * (2 * w);
+                const int64_t off = e - p0;
+                const int64_t lenA = m - p0 < w ? m - p0 : w;
+                const int64_t restB = m - p0 - w;
+                const int64_t lenB = restB <= 0 ? 0 : (restB < w ? restB : w);
+                const int32_t v = src[e];
+                const int64_t pos = off < w ? off + lower_bound_i32(src + p0 + w, lenB, v)
+                                            : (off - w) + lower_bound_i32(src + p0, lenA, v);
+                dst[p0 + pos] = v;
+            }
+            __syncthreads();
+            int32_t *t = src;
+            src = dst;
+            dst = t;
+        }
+        for (int64_t e = tid; e < m; e += NN_THREADS) {
+            const int32_t v = src[e];
+            if (src != b) b[e] = v;
+            rank[v] = s + (int32_t)e;
+        }
+        __syncthreads();
+    }
 }
 
 __global__ void __launch_bounds__(NN_THREADS)
@@ -399,18 +494,44 @@ static int read_i64(sph_ctx *ctx, const void *dev, int64_t *out, bool is32)
     return SPH_OK;
 }
 
+static size_t bin_large_off(int64_t n, int64_t nc, size_t scratch_off)
+{
+    return (scratch_off + sph_scan_scratch_bytes(nc > n ? nc : n, sizeof(int32_t)) + 255) &
+           ~(size_t)255;
+}
+
+size_t sph_bin_sort_scratch_end(int64_t n, int64_t nc, size_t scratch_off)
+{
+    return bin_large_off(n, nc, scratch_off) +
+           sizeof(int32_t) * (size_t)(n / (RANK_DIRECT + 1) + 2);
+}
+
 int sph_bin_sort(sph_ctx *ctx, int64_t n, int64_t nc, const int32_t *cell_of, int32_t *rank,
                  int32_t *cell_start, int32_t *perm, int32_t *cells, int32_t *slot, int32_t *tmp,
                  int32_t trash, size_t scratch_off)
 {
-    int rc = sph_scan_exclusive_i32_at(ctx, cell_start, nc, scratch_off);
+    // the large-cell list (a counter and at most n / (RANK_DIRECT + 1) keys)
+    // after the scan partials; the callers size the scratch for it first
+    // (sph_bin_sort_scratch_end), so this never moves it under their pointers
+    const size_t large_off = bin_large_off(n, nc, scratch_off);
+    int rc = sph_ensure_scratch(ctx, sph_bin_sort_scratch_end(n, nc, scratch_off));
     if (rc) return rc;
+    int32_t *nlarge = (int32_t *)((char *)ctx->scratch + large_off);
+    int32_t *large = nlarge + 1;
+    rc = sph_scan_exclusive_i32_at(ctx, cell_start, nc, scratch_off);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, nlarge, 1, 0);
+    SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
     hipLaunchKernelGGL(k_bin_scatter, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
                        ctx->stream, cell_of, rank, n, cell_start, tmp);
     SPH_LAUNCH_CHECK(ctx, "k_bin_scatter");
     hipLaunchKernelGGL(k_bin_rank, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
-                       ctx->stream, cell_of, n, cell_start, tmp, trash, perm, rank);
+                       ctx->stream, cell_of, n, cell_start, tmp, trash, perm, rank, large,
+                       nlarge);
     SPH_LAUNCH_CHECK(ctx, "k_bin_rank");
+    hipLaunchKernelGGL(k_bin_rank_large, dim3(RANK_LARGE_GRID), dim3(NN_THREADS), 0,
+                       ctx->stream, cell_start, tmp, large, nlarge, perm, rank);
+    SPH_LAUNCH_CHECK(ctx, "k_bin_rank_large");
     // non-empty flags -> slots (scan) -> compact list of cell keys
     hipLaunchKernelGGL(k_cell_flags, dim3(sph_blocks(nc, NN_THREADS)), dim3(NN_THREADS), 0,
                        ctx->stream, cell_start, nc, slot);
@@ -451,12 +572,12 @@ static int bin_array(sph_ctx *ctx, const sph_grid *grid, const double *x, const 
         }
         return SPH_OK;
     }
-    // scratch: [non-empty flags / slots (nc+1) | unsorted cell members (n) | scan partials];
+    // scratch: [non-empty flags / slots (nc+1) | unsorted cell members (n) | scan partials |
+    // cells with many members];
     // the scan of cell_start runs before either region is written
     const size_t slot_bytes = (sizeof(int32_t) * (size_t)(nc + 1) + 511) & ~(size_t)255;
     const size_t tmp_bytes = (sizeof(int32_t) * (size_t)n + 255) & ~(size_t)255;
-    const size_t scan_bytes = sph_scan_scratch_bytes(nc, sizeof(int32_t));
-    rc = sph_ensure_scratch(ctx, slot_bytes + tmp_bytes + scan_bytes);
+    rc = sph_ensure_scratch(ctx, sph_bin_sort_scratch_end(n, nc, slot_bytes + tmp_bytes));
     if (rc) return rc;
     int32_t *slot = (int32_t *)ctx->scratch;
     int32_t *tmp = (int32_t *)((char *)ctx->scratch + slot_bytes);

@@ -5,7 +5,16 @@ MomentumEquation with Monaghan viscosity (sph/funcs/density_funcs.pyx:37-73,
 eos_funcs.pyx:118-129, pressure_funcs.pyx:246-391), cubic spline
 (base/kernels.pyx:349-453) -- written with differentiable torch operations
 over an explicit pair list, so torch.autograd supplies reference gradients
-for the hand-written reverse kernels.  Uniform h.
+for the hand-written reverse kernels.  Uniform h.  Runs on the device of its
+input tensors.
+
+The loss of a rollout is only piecewise smooth: the viscosity acts only on
+pairs with v_ab . r_ab < 0, so the accelerations have a kink where a pair's
+v . r changes sign.  ``rk2_rollout`` can record each evaluation's pairs and
+viscosity switches, and ``rk2_rollout_frozen`` replays a batch of perturbed
+rollouts with those pairs and switches held fixed: the rollout map of that
+branch, smooth in the initial state and alpha, whose derivative at the
+recorded point is what autograd (and the adjoint) computes.
 """
 import math
 
@@ -19,6 +28,24 @@ def pairs_bruteforce(x, y, h, z=None):
     d2 = ((pts[:, None, :] - pts[None, :, :]) ** 2).sum(-1)
     i, j = np.nonzero(d2 < (2.0 * h) ** 2)
     return torch.from_numpy(i), torch.from_numpy(j)
+
+
+def pairs_of(p, h):
+    """Ordered pairs (i, j), i != j, with |x_i - x_j|^2 < (2h)^2 of the
+    positions p (n, dim), on p's device, not differentiated."""
+    d = p.detach()
+    n = d.shape[0]
+    out_i, out_j = [], []
+    step = max(1, (1 << 26) // max(n, 1))  # rows per block: ~64M distances at a time
+    for r0 in range(0, n, step):
+        blk = d[r0:r0 + step]
+        d2 = ((blk[:, None, :] - d[None, :, :]) ** 2).sum(-1)
+        i, j = torch.nonzero(d2 < (2.0 * h) ** 2, as_tuple=True)
+        out_i.append(i + r0)
+        out_j.append(j)
+    i, j = torch.cat(out_i), torch.cat(out_j)
+    off = i != j
+    return i[off], j[off]
 
 
 def _fac(h, dim):
@@ -44,47 +71,74 @@ def cubic_gfac(r, h, dim):
 
 
 def accelerations(pos, vel, m, h, pairs, alpha, beta=0.0, eta=0.1, co=1400.0, ro=1.0,
-                  gamma=7.0):
+                  gamma=7.0, switch=None):
     """a(x, u) of MomentumEquation + AV, densities by SPHRho over `pairs`.
-    pos, vel: (n, dim) tensors; alpha may be a tensor (differentiable)."""
-    n, dim = pos.shape
+    pos, vel: (..., n, dim) tensors (leading dimensions: independent
+    replicas sharing the pairs); alpha may be a tensor (differentiable, or
+    one value per replica with shape (..., 1)).  switch: a boolean mask per
+    pair replacing the viscosity condition v_ab . r_ab < 0.  Returns (a, rho,
+    the condition as evaluated)."""
+    dim = pos.shape[-1]
     i, j = pairs
     off = i != j
     i, j = i[off], j[off]
-    r = pos[i] - pos[j]
-    rab = torch.sqrt((r * r).sum(1))  # r > 0 off the diagonal: sqrt stays differentiable
-    w0 = cubic_w(torch.zeros(1, dtype=pos.dtype), h, dim)
-    rho = (m * w0).index_add(0, i, m[j] * cubic_w(rab, h, dim))
+    if switch is not None:
+        switch = switch[off] if switch.shape[-1] == off.shape[0] else switch
+    r = pos[..., i, :] - pos[..., j, :]
+    rab = torch.sqrt((r * r).sum(-1))  # r > 0 off the diagonal: sqrt stays differentiable
+    w0 = cubic_w(torch.zeros(1, dtype=pos.dtype, device=pos.device), h, dim)
+    rho = (m * w0).expand(pos.shape[:-1]).clone().index_add(
+        pos.dim() - 2, i, m[j] * cubic_w(rab, h, dim))
     B = co * co * ro / gamma
     p = ((rho / ro) ** gamma - 1.0) * B
     cs = (rho / ro) ** (0.5 * (gamma - 1.0)) * co
     P = p / (rho * rho)
-    v = vel[i] - vel[j]
-    dot = (v * r).sum(1)
-    mu = h * dot / ((r * r).sum(1) + eta * eta * h * h)
-    cm = 0.5 * (cs[i] + cs[j])
-    rm = 0.5 * (rho[i] + rho[j])
-    pi = torch.where(dot < 0, (-alpha * cm * mu + beta * mu * mu) / rm, torch.zeros_like(mu))
-    s = -m[j] * (P[i] + P[j] + pi)
-    term = (s * cubic_gfac(rab, h, dim))[:, None] * r
-    return torch.zeros_like(pos).index_add(0, i, term), rho
+    v = vel[..., i, :] - vel[..., j, :]
+    dot = (v * r).sum(-1)
+    mu = h * dot / ((r * r).sum(-1) + eta * eta * h * h)
+    cm = 0.5 * (cs[..., i] + cs[..., j])
+    rm = 0.5 * (rho[..., i] + rho[..., j])
+    cond = dot < 0
+    use = cond if switch is None else switch
+    pi = torch.where(use, (-alpha * cm * mu + beta * mu * mu) / rm, torch.zeros_like(mu))
+    s = -m[j] * (P[..., i] + P[..., j] + pi)
+    term = (s * cubic_gfac(rab, h, dim))[..., None] * r
+    return torch.zeros_like(pos).index_add(pos.dim() - 2, i, term), rho, cond
 
 
-def rk2_rollout(pos, vel, m, h, alpha, nsteps, dt, **phys):
+def rk2_rollout(pos, vel, m, h, alpha, nsteps, dt, record=None, **phys):
     """RK2 midpoint rollout (solver/integrator.py:467-498) with the pair
     list rebuilt from the positions before every evaluation (brute force,
-    not differentiated); returns the final positions."""
-    def pairs_of(p):
-        d = p.detach()
-        d2 = ((d[:, None, :] - d[None, :, :]) ** 2).sum(-1)
-        i, j = torch.nonzero(d2 < (2.0 * h) ** 2, as_tuple=True)
-        return i, j
-
+    not differentiated); returns the final positions.  record: a list that
+    receives (pairs, viscosity condition) of every evaluation."""
     for _ in range(nsteps):
-        a0, _ = accelerations(pos, vel, m, h, pairs_of(pos), alpha, **phys)
+        pr = pairs_of(pos, h)
+        a0, _, c0 = accelerations(pos, vel, m, h, pr, alpha, **phys)
+        if record is not None:
+            record.append((pr, c0.detach()))
         p1 = pos + vel * (dt / 2)
         v1 = vel + a0 * (dt / 2)
-        a1, _ = accelerations(p1, v1, m, h, pairs_of(p1), alpha, **phys)
+        pr = pairs_of(p1, h)
+        a1, _, c1 = accelerations(p1, v1, m, h, pr, alpha, **phys)
+        if record is not None:
+            record.append((pr, c1.detach()))
+        pos = pos + v1 * dt
+        vel = vel + a1 * dt
+    return pos
+
+
+def rk2_rollout_frozen(pos, vel, m, h, alpha, dt, record, **phys):
+    """The rollout of ``rk2_rollout`` over the pairs and viscosity switches of
+    `record` (one entry per evaluation), for a batch of replicas: pos, vel of
+    shape (B, n, dim), alpha a (B, 1) tensor.  Returns the final positions."""
+    assert len(record) % 2 == 0
+    for k in range(0, len(record), 2):
+        pr, sw = record[k]
+        a0, _, _ = accelerations(pos, vel, m, h, pr, alpha, switch=sw, **phys)
+        p1 = pos + vel * (dt / 2)
+        v1 = vel + a0 * (dt / 2)
+        pr, sw = record[k + 1]
+        a1, _, _ = accelerations(p1, v1, m, h, pr, alpha, switch=sw, **phys)
         pos = pos + v1 * dt
         vel = vel + a1 * dt
     return pos

@@ -46,7 +46,7 @@ def test_vjp_matches_torch_autograd(gpu):
     pos = torch.tensor(np.stack([d["x"], d["y"]], 1), requires_grad=True)
     vel = torch.tensor(np.stack([d["u"], d["v"]], 1), requires_grad=True)
     alpha = torch.tensor(0.3, dtype=torch.float64, requires_grad=True)
-    acc, _ = ref.accelerations(pos, vel, torch.tensor(d["m"]), h,
+    acc, _, _ = ref.accelerations(pos, vel, torch.tensor(d["m"]), h,
                                ref.pairs_bruteforce(d["x"], d["y"], h), alpha, beta=0.2)
     gx, gu, ga = torch.autograd.grad(acc, (pos, vel, alpha), grad_outputs=abar.cpu())
     errs = [_rel(outs[0].cpu(), gx[:, 0]), _rel(outs[1].cpu(), gx[:, 1]),

@@ -1,0 +1,71 @@
+"""Binning of crowded cells.  One particle with a large h sets the cell size
+(radius_scale * max h), so the others crowd into a few cells; the counting
+sort must still order every cell's members by ascending index (base/cell.pyx:
+1229-1306) at a cost that does not grow with the square of a cell's size.
+Cells of a few members up to 20,000, across the sizes where the kernel
+switches from counting to the merge sort (more than 1,024 members) and where
+the merge sort needs several passes (more than 2,048)."""
+import time
+
+import numpy as np
+import pytest
+
+from oracle import cpu_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _clusters(sizes, seed=3, small_h=1e-3):
+    """Clusters of the given sizes, each inside its own cell of side 2 (the
+    cell size set by one particle of h = 1), plus a few scattered particles;
+    indices interleaved over the clusters."""
+    rng = np.random.default_rng(seed)
+    pts = []
+    for k, m in enumerate(sizes):
+        c = np.array([2.0 * k + 0.05, 0.05, 0.05])
+        pts.append(c + rng.uniform(0.0, 1.9, (m, 3)))
+    pts.append(rng.uniform(-4.0, 4.0 + 2.0 * len(sizes), (50, 3)))
+    p = np.concatenate(pts)
+    p = p[rng.permutation(len(p))]
+    h = np.full(len(p), small_h)
+    h[7] = 1.0
+    return {"x": p[:, 0].copy(), "y": p[:, 1].copy(), "z": p[:, 2].copy(), "h": h,
+            "m": np.ones(len(p)), "rho": np.ones(len(p))}
+
+
+@pytest.mark.parametrize("sizes", [(20000,), (1024, 1025, 2048, 2049, 300, 5000, 4097)])
+def test_crowded_cells_sorted_by_index(gpu, sizes):
+    import torch
+    from pysph_amd.base import api as base
+    d = _clusters(sizes)
+    n = len(d["x"])
+    pa = base.get_particle_array(name="fluid", **d)
+    particles = base.Particles([pa])
+    cm = particles.cell_manager
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cm.update()
+    torch.cuda.synchronize()
+    t_first = time.perf_counter() - t0
+    pa.set_dirty(True)   # a second binning (device-side cell count)
+    t0 = time.perf_counter()
+    cm.update()
+    torch.cuda.synchronize()
+    t_second = time.perf_counter() - t0
+    b = cm.bins["fluid"]
+    cell_of = b.cell_of.cpu().numpy()[:n]
+    perm = b.perm.cpu().numpy()[:n]
+    rank = b.rank.cpu().numpy()[:n]
+    want = np.lexsort((np.arange(n), cell_of)).astype(np.int32)
+    counts = np.bincount(cell_of)
+    print(f"{n} particles, largest cell {counts.max()} members: binning {1e3 * t_first:.1f} ms "
+          f"(first, synchronous), {1e3 * t_second:.1f} ms (device-side)")
+    assert np.array_equal(perm, want)
+    assert np.array_equal(rank[perm], np.arange(n))
+    assert t_second < 1.0
+    # lists over the crowded cells equal the oracle's, order included
+    loc = particles.nnps_manager.get_neighbor_particle_locator(pa, pa, 2.0)
+    off, idx = (t.cpu().numpy() for t in loc.get_csr())
+    full = {k: pa.get(k) for k in ("x", "y", "z", "h")}
+    roff, ridx = cpu_ref.nnps(full, full, 2.0, cm.cell_size, True, nthreads=16)
+    assert np.array_equal(off, roff) and np.array_equal(idx, ridx)

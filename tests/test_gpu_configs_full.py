@@ -1,0 +1,239 @@
+"""The batched and adjoint configurations at their configured sizes.
+
+Batch: the 256 seeded elliptical drops (radii U[0.4, 2], dx in {0.01, 0.02,
+0.04}, velocity scale U[50, 150]; about 6.4M particles), FP64 and FP32,
+advanced 3 RK2 steps together.  Every drop must equal its own run through the
+plain path (drop_solver) bit for bit; the smallest and largest drops and one
+drop of each dx must follow the FP64 oracle (cpu_ref.RefSimulation) within
+1e-6; FP32 drops follow the FP64 batch to float accuracy.
+
+Adjoint: the 2D drop at dx = 0.02 (7,860 particles), RK2, dt = 1e-5, 200
+steps with a checkpoint every 20; loss = MSE of the final positions against a
+target run from u = -120 x, v = 120 y; gradient with respect to the
+per-particle initial velocities (u = -100 x, v = 100 y plus seeded N(0, 5))
+and alpha.  Checked (a) against torch.autograd of the pure-PyTorch SPH
+(tests/sph_torch_ref.py) rolled out on the GPU with its own pair lists, and
+(b) against central differences with eps = 1e-6 on 64 random particles and
+alpha, both of the rollout itself and of the rollout with the pairs and
+viscosity switches of the unperturbed run held fixed.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import cpu_ref
+from tests import sph_torch_ref as tref
+from tests.helpers import bit_equal_frac, rel_err
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("x", "y", "u", "v", "rho", "p", "cs", "dt_fac")
+STEPS = 3
+
+
+@pytest.fixture(scope="module")
+def drops():
+    from pysph_amd.batch import batched_drops
+    return batched_drops(nsims=256)
+
+
+def _plain_runs(sims, prm, precision, steps):
+    """Each drop through the plain path; yields (index, final fluid array)."""
+    from pysph_amd.solver.configs import drop_solver
+    for s, d in enumerate(sims):
+        solver, particles, _ = drop_solver(data=dict(d), dx=float(prm["dx"][s]),
+                                           precision=precision)
+        for _ in range(steps):
+            solver.integrator.integrate(float(prm["dt"][s]))
+        yield s, particles.arrays[0]
+
+
+@pytest.mark.timeout(900)
+def test_batch_256_drops_fp64(gpu, drops):
+    from pysph_amd.batch import BatchedSimulation
+    sims, prm = drops
+    b = BatchedSimulation(sims, dt_per_sim=prm["dt"])
+    print(f"\n256 drops: {b.n} particles, {min(b.counts)} to {max(b.counts)} per drop")
+    for _ in range(STEPS):
+        b.step(1.0)
+    b.check()
+    # every drop against its plain-path run, bit for bit
+    worst = 0.0
+    for s, fl in _plain_runs(sims, prm, "double", STEPS):
+        for k in KEYS:
+            a, e = b.get(k, s), fl.get(k)
+            worst = max(worst, rel_err(a, e))
+            assert bit_equal_frac(a, e) == 1.0, (s, k, rel_err(a, e))
+    print(f"every drop bit-equal to its plain-path run ({STEPS} RK2 steps)")
+    # the oracle: smallest, largest, one drop of each dx
+    counts = np.array(b.counts)
+    picks = {int(np.argmin(counts)), int(np.argmax(counts))}
+    for dxv in np.unique(prm["dx"]):
+        picks.add(int(np.flatnonzero(prm["dx"] == dxv)[0]))
+    for s in sorted(picks):
+        ref = cpu_ref.RefSimulation([cpu_ref.RefArray("fluid", 0, sims[s])], dim=2,
+                                    integrator="rk2", co=1400.0, ro=1.0, gamma=7.0, alpha=0.1,
+                                    beta=0.0, nthreads=16)
+        for _ in range(STEPS):
+            ref.integrate(float(prm["dt"][s]))
+        errs = {k: rel_err(b.get(k, s), ref.arrays[0][k]) for k in KEYS}
+        print(f"drop {s} (dx {prm['dx'][s]}, {counts[s]} particles) vs oracle: " +
+              " ".join(f"{k} {v:.2g}" for k, v in errs.items()))
+        assert max(errs.values()) <= 1e-6, (s, errs)
+
+
+@pytest.mark.timeout(900)
+def test_batch_256_drops_fp32(gpu, drops):
+    from pysph_amd.batch import BatchedSimulation
+    sims, prm = drops
+    b32 = BatchedSimulation(sims, dt_per_sim=prm["dt"], precision="single")
+    b64 = BatchedSimulation(sims, dt_per_sim=prm["dt"])
+    for _ in range(STEPS):
+        b32.step(1.0)
+        b64.step(1.0)
+    b32.check()
+    b64.check()
+
+    def norm_err(a, e):
+        return float(np.abs(a - e).max() / np.abs(e).max())
+
+    worst = {k: 0.0 for k in ("x", "y", "u", "v", "rho", "p")}
+    for s in range(len(sims)):
+        for k in worst:
+            worst[k] = max(worst[k], norm_err(b32.get(k, s), b64.get(k, s)))
+    print("\nFP32 batch vs FP64 batch, worst drop (max error / max magnitude): " +
+          " ".join(f"{k} {v:.2g}" for k, v in worst.items()))
+    assert max(worst.values()) <= 1e-4
+    # the FP32 plain path runs the same float arithmetic per drop
+    neq = 0
+    for s, fl in _plain_runs(sims, prm, "single", STEPS):
+        for k in KEYS:
+            a, e = b32.get(k, s), fl.get(k)
+            if bit_equal_frac(a, e) != 1.0:
+                neq += 1
+                print(f"drop {s} {k}: FP32 batch vs FP32 plain path rel err {rel_err(a, e):.3g}")
+    assert neq == 0
+    print("every FP32 drop bit-equal to its FP32 plain-path run")
+
+
+# ---------------------------------------------------------------------------
+# adjoint
+# ---------------------------------------------------------------------------
+
+def _norm_rel(a, b):
+    a, b = np.asarray(a, dtype=np.float64), np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.timeout(900)
+def test_adjoint_fit_configuration(gpu):
+    from pysph_amd.adjoint import Rollout, rollout
+    from pysph_amd.batch import BatchedSimulation
+    from pysph_amd.solver.generators import elliptical_drop
+    dx, steps, ckpt, dt, eps, nfd = 0.02, 200, 20, 1e-5, 1e-6, 64
+    d = elliptical_drop(dx=dx, radius=1.0, vel=100.0)
+    n = len(d["x"])
+    assert n == 7860
+    h = float(d["h"][0])
+    pos0 = np.stack([d["x"], d["y"]], 1)
+    roll = Rollout(pos0, d["m"], h, nsteps=steps, dt=dt, checkpoint=ckpt)
+    dev = roll.sim.device
+    tx, ty = roll.forward(torch.tensor(-120.0 * d["x"], device=dev),
+                          torch.tensor(120.0 * d["y"], device=dev), 0.1)
+    tx, ty = tx.clone(), ty.clone()
+    rng = np.random.default_rng(20261202)
+    u_init = -100.0 * d["x"] + rng.normal(0.0, 5.0, n)
+    v_init = 100.0 * d["y"] + rng.normal(0.0, 5.0, n)
+    u0 = torch.tensor(u_init, device=dev, requires_grad=True)
+    v0 = torch.tensor(v_init, device=dev, requires_grad=True)
+    alpha = torch.tensor(0.1, dtype=torch.float64, device=dev, requires_grad=True)
+    x, y = rollout(roll, alpha, u0, v0)
+    loss = ((x - tx) ** 2 + (y - ty) ** 2).mean()
+    gu, gv, ga = (g.detach().cpu().numpy() for g in torch.autograd.grad(loss, (u0, v0, alpha)))
+    ga = float(ga)
+
+    # (a) torch.autograd of the pure-PyTorch SPH on the GPU
+    m = torch.tensor(d["m"], device=dev)
+    vel = torch.tensor(np.stack([u_init, v_init], 1), device=dev, requires_grad=True)
+    alr = torch.tensor(0.1, dtype=torch.float64, device=dev, requires_grad=True)
+    record = []
+    pf = tref.rk2_rollout(torch.tensor(pos0, device=dev), vel, m, h, alr, steps, dt,
+                          record=record)
+    lr = ((pf[:, 0] - tx) ** 2 + (pf[:, 1] - ty) ** 2).mean()
+    rg, ra = torch.autograd.grad(lr, (vel, alr))
+    rg = rg.cpu().numpy()
+    ra = float(ra)
+    e_pos = max(rel_err(x.detach().cpu().numpy(), pf[:, 0].detach().cpu().numpy()),
+                rel_err(y.detach().cpu().numpy(), pf[:, 1].detach().cpu().numpy()))
+    e_auto = {"u": _norm_rel(gu, rg[:, 0]), "v": _norm_rel(gv, rg[:, 1]),
+              "alpha": abs(ga - ra) / abs(ra)}
+    print(f"\nadjoint vs torch.autograd ({n} particles, {steps} steps): final positions "
+          f"{e_pos:.2g}; gradient u {e_auto['u']:.2g} v {e_auto['v']:.2g} "
+          f"alpha {e_auto['alpha']:.2g} (|g| relative)")
+    del pf, lr
+    assert e_pos <= 1e-10
+    assert max(e_auto.values()) <= 1e-8
+
+    # (b) central differences on 64 particles (u and v) and alpha
+    picks = rng.choice(n, nfd, replace=False)
+    which = [(int(i), c) for i in picks for c in (0, 1)]
+    g_adj = np.array([gu[i] if c == 0 else gv[i] for i, c in which] + [ga])
+    sims = []
+    for i, c in which:
+        for sgn in (1.0, -1.0):
+            dd = dict(d)
+            dd["u"], dd["v"] = u_init.copy(), v_init.copy()
+            dd["u" if c == 0 else "v"][i] += sgn * eps
+            sims.append(dd)
+    txn, tyn = tx.cpu().numpy(), ty.cpu().numpy()
+
+    def losses(sim_list, alpha_val):
+        b = BatchedSimulation(sim_list, alpha=alpha_val)
+        for _ in range(steps):
+            b.step(dt)
+        b.check()
+        xs, ys = b.get("x"), b.get("y")
+        return [float(((xs[b.sim_off_host[k]:b.sim_off_host[k + 1]] - txn) ** 2 +
+                       (ys[b.sim_off_host[k]:b.sim_off_host[k + 1]] - tyn) ** 2).mean())
+                for k in range(len(sim_list))]
+
+    L = losses(sims, 0.1)
+    d0 = dict(d)
+    d0["u"], d0["v"] = u_init, v_init
+    la, lb = losses([d0], 0.1 + eps)[0], losses([d0], 0.1 - eps)[0]
+    fd = np.array([(L[2 * k] - L[2 * k + 1]) / (2 * eps) for k in range(len(which))] +
+                  [(la - lb) / (2 * eps)])
+
+    # the same differences with the unperturbed run's pairs and switches held fixed
+    B = 2 * len(which) + 2
+    vb = torch.tensor(np.stack([u_init, v_init], 1), device=dev).repeat(B, 1, 1)
+    ab = torch.full((B, 1), 0.1, dtype=torch.float64, device=dev)
+    for k, (i, c) in enumerate(which):
+        vb[2 * k, i, c] += eps
+        vb[2 * k + 1, i, c] -= eps
+    ab[-2, 0] += eps
+    ab[-1, 0] -= eps
+    with torch.no_grad():
+        pb = tref.rk2_rollout_frozen(torch.tensor(pos0, device=dev).repeat(B, 1, 1), vb, m, h,
+                                     ab, dt, record)
+        Lf = (((pb[..., 0] - tx) ** 2 + (pb[..., 1] - ty) ** 2).mean(-1)).cpu().numpy()
+    fd_frozen = (Lf[0::2] - Lf[1::2]) / (2 * eps)
+    g_auto = np.array([rg[i, c] for i, c in which] + [ra])
+
+    def stats(g, f):
+        rel = np.abs(g - f) / np.abs(f)
+        return _norm_rel(g, f), float(np.median(rel[:-1])), float(rel[:-1].max()), float(rel[-1])
+
+    s_fd = stats(g_adj, fd)
+    s_frozen = stats(g_auto, fd_frozen)
+    print(f"central differences eps {eps}, {len(which)} velocity components + alpha:\n"
+          f"  adjoint vs FD of the rollout:           |g - FD|/|FD| {s_fd[0]:.3g}, "
+          f"componentwise median {s_fd[1]:.3g} max {s_fd[2]:.3g}, alpha {s_fd[3]:.3g}\n"
+          f"  autograd vs FD with pairs and switches held fixed: |g - FD|/|FD| "
+          f"{s_frozen[0]:.3g}, componentwise median {s_frozen[1]:.3g} max {s_frozen[2]:.3g}, "
+          f"alpha {s_frozen[3]:.3g}")
+    # the derivative of the branch the run takes is exact: FD of that branch
+    # agrees componentwise, alpha included
+    assert s_frozen[2] <= 1e-4 and s_frozen[3] <= 1e-4
+    # the rollout itself, across viscosity switches, agrees as a vector
+    assert s_fd[0] <= 1e-4

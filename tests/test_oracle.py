@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 from oracle import cpu_ref
+from tests import closed_forms
 from tests.helpers import rel_err, rows_as_sets_equal
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -306,6 +307,79 @@ def test_momentum_viscosity_only_on_approach():
             assert abs(ax[0] - (-1.0 * piab * grad)) < 1e-14
         else:
             assert ax[0] == 0.0
+
+
+def _oracle_momentum_of(c, dim, alpha, beta, eta):
+    d = closed_forms.as_arrays(c)
+    cs = cpu_ref.compute_cell_size(d["h"].max(), 2.0)
+    off, idx = cpu_ref.nnps(d, d, 2.0, cs, True)
+    n = len(d["x"])
+    ax, ay, az, dtf = (np.zeros(n) for _ in range(4))
+    cpu_ref.momentum(dim, d, d, off, idx, alpha, beta, eta, ax, ay, az, dtf)
+    return np.stack([ax, ay, az], 1), dtf
+
+
+def test_momentum_unit_square_reference_case():
+    """The reference's MomentumEquation case (sph/tests/test_pressure_funcs.py:
+    103-182: alpha = beta = 1, eta = 0.1 on the four-particle unit square with
+    cs = 0), its forces computed here from the closed form: with cs = 0 only the
+    beta mu^2 part of the viscosity acts, so a sign error or a dropped beta term
+    fails."""
+    prm = FIX["momentum"]
+    c = closed_forms.unit_square()
+    want, want_dtf = closed_forms.momentum(2, **c, alpha=prm["alpha"], beta=prm["beta"],
+                                           eta=prm["eta"])
+    got, dtf = _oracle_momentum_of(c, 2, prm["alpha"], prm["beta"], prm["eta"])
+    assert np.allclose(got, want, rtol=1e-13, atol=1e-15), (got, want)
+    assert np.allclose(dtf, want_dtf, rtol=1e-14, atol=0)
+    # the reference's own bar is 6 places
+    assert np.all(np.abs(got - want) < 0.5 * 10.0 ** -prm["places"])
+    # particles 0 and 3 approach (v_ab . r_ab < 0): the viscosity term is live
+    assert want_dtf[0] > 0 and want_dtf[3] > 0
+    wrong, _ = closed_forms.momentum(2, **c, alpha=prm["alpha"], beta=-prm["beta"],
+                                     eta=prm["eta"])
+    assert np.max(np.abs(wrong - want)) > 1e-3
+
+
+@pytest.mark.parametrize("far", [False, True])
+@pytest.mark.parametrize("alpha,beta", [(0.0, 1.0), (0.8, 0.5)])
+def test_momentum_two_particles_unequal_states(far, alpha, beta):
+    """Two particles with unequal h, m, rho, P and c, approaching: the oracle's
+    accelerations equal the closed form on both spline branches.  The case is
+    chosen so that (P_a + P_b)/(rho_a rho_b) for the pressure term, -beta mu^2,
+    the geometric mean of the densities or the larger sound speed for cbar each
+    change the result."""
+    c = closed_forms.two_particles(approach=True, far=far)
+    want, want_dtf = closed_forms.momentum(3, **c, alpha=alpha, beta=beta)
+    got, dtf = _oracle_momentum_of(c, 3, alpha, beta, 0.1)
+    assert np.allclose(got, want, rtol=1e-13, atol=0), (got, want)
+    assert np.allclose(dtf, want_dtf, rtol=1e-14, atol=0)
+    assert np.all(np.abs(want) > 1e-3)
+    mistakes = [
+        dict(pressure_term=lambda pa, pb, ra, rb: (pa + pb) / (ra * rb)),
+        dict(pi_term=lambda al, be, mu, ca, cb, ra, rb:
+             (-al * 0.5 * (ca + cb) * mu - be * mu * mu) / (0.5 * (ra + rb))),
+        dict(pi_term=lambda al, be, mu, ca, cb, ra, rb:
+             (-al * 0.5 * (ca + cb) * mu + be * mu * mu) / math.sqrt(ra * rb)),
+    ]
+    if alpha > 0:
+        mistakes.append(dict(pi_term=lambda al, be, mu, ca, cb, ra, rb:
+                             (-al * max(ca, cb) * mu + be * mu * mu) / (0.5 * (ra + rb))))
+    for mk in mistakes:
+        wrong, _ = closed_forms.momentum(3, **c, alpha=alpha, beta=beta, **mk)
+        assert np.max(np.abs(wrong - want) / np.abs(want)) > 1e-3
+
+
+def test_momentum_two_particles_receding():
+    """Receding particles: no viscosity, dt_fac still |mu| (pressure_funcs.pyx:
+    288-293)."""
+    c = closed_forms.two_particles(approach=False)
+    want, want_dtf = closed_forms.momentum(3, **c, alpha=0.8, beta=0.5)
+    got, dtf = _oracle_momentum_of(c, 3, 0.8, 0.5, 0.1)
+    assert np.allclose(got, want, rtol=1e-13, atol=0)
+    assert np.allclose(dtf, want_dtf, rtol=1e-14, atol=0) and np.all(dtf > 0)
+    no_visc, _ = closed_forms.momentum(3, **c, alpha=0.0, beta=0.0)
+    assert np.allclose(want, no_visc, rtol=1e-15, atol=0)
 
 
 # ---------------------------------------------------------------------------

@@ -23,8 +23,13 @@ dam_break3d.py (continuity, pressure gradient and Monaghan viscosity as
 separate functions, XSPH).
 
 Also reported: the roofline fraction of the dominant kernel (momentum)
-measured with HIP events on its stream, and the FP64 CPU oracle timed on
-the host cores on a bounded sample of the same workload.
+measured with HIP events on its stream (algorithmic bytes N(4 + 4 nbar +
+15 f) per launch), its FP64 VALU view, the whole step's algorithmic bytes
+per second, and the FP64 CPU oracle timed on the host cores on a bounded
+sample of the same workload.  The default run also measures the batched
+form (the 256-drop batch against sequential plain-path runs) and the
+adjoint form (7,860-particle drop, 200 steps: forward, backward, plain-path
+forward, central differences) in sub-objects ``batch`` and ``adjoint``.
 """
 import argparse
 import json
@@ -39,6 +44,8 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# launch shape of the 256-drop batch picked by pysph_amd.autotune on an MI355X
+BATCH_LAUNCH_SHAPE = {"cells_per_block": 8, "nbr_tile": 4}
 
 
 def _args():
@@ -75,6 +82,8 @@ def _args():
                     help="--workload lattice: Particles(variable_h=True), the symmetric lists")
     ap.add_argument("--cell-order", choices=("morton", "linear"), default="morton")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default workload: skip the batch and adjoint sub-measurements")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="launch one k_copy of this many MiB before the timed steps (a kernel "
@@ -97,30 +106,88 @@ def copy_calibration(mib):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_momentum_dambreak_1M_traffic.json")
+VALU_FILE = os.path.join(ROOT, "profiles", "k_momentum_dambreak_1M_valu.json")
+FP64_PEAK_TFLOPS = 78.6
+# vector issue capacity of an MI355X: 256 CUs x 4 SIMDs, one wave64 VALU
+# instruction per SIMD every 4 cycles (16 lanes), 2.4 GHz
+SIMDS, CLOCK_HZ, CYCLES_PER_VALU = 1024, 2.4e9, 4
+
+
+def _profile_json(path):
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
 
 
 def measured_traffic(path=None):
     """HBM bytes per launch of the roofline kernel from the committed
     rocprofv3 byte counters (FETCH_SIZE, WRITE_SIZE, scaled by a 1 GiB copy),
-    or None."""
+    or None.  A profile of this bench command, not a measurement of this run."""
+    d = _profile_json(path or TRAFFIC_FILE)
     try:
-        with open(path or TRAFFIC_FILE) as f:
-            return float(json.load(f)["hbm_bytes_per_launch"])
-    except (OSError, ValueError, KeyError):
+        return float(d["hbm_bytes_per_launch"])
+    except (TypeError, KeyError, ValueError):
         return None
 
 
-def momentum_bytes(nd, nnz, ns_other, ns):
-    """Algorithmic HBM bytes of MomentumEquation over one source (k_prho2
-    over the ns source records if needed, then k_momentum): P/rho^2 of each
-    source (p and rho read, P/rho^2 written: 24 B), and per destination the
-    int64 row offset, 10 destination properties + dt_fac read, ax ay az
-    read-modify-write and dt_fac written (8+11*8+3*16+8 = 152 B), 4 B per
-    list entry, and -- when the source is another array -- its 11
-    properties once (88 B per source particle).  A launch over several
-    sources reads and writes each destination once: pass the entries of all
-    lists and the particles of the other arrays."""
-    return 24 * ns + nd * 152 + 4 * nnz + 88 * ns_other
+# Algorithmic bytes: the compulsory HBM traffic of each kernel, every array
+# element touched once per launch and neighbour records served on chip.
+# N destinations, nbar mean list entries per destination, f bytes per value
+# (8: FP64), k permuted SoA properties, P stepped properties.
+def alg_bytes(kind, N, nbar=0.0, f=8, k=8, P=6):
+    return {"bin": N * (4 * f + 8 + 2 * k * f),          # read x y z h, key + perm, permute k
+            "lists": N * (4 * f + 4 + 4 * nbar),         # read x y z h, count, 4 B per entry
+            "density": N * (4 + 4 * nbar + 6 * f),       # count, entries, 5 props in, rho out
+            "tait": 3 * f * N,                           # rho in, p cs out
+            "momentum": N * (4 + 4 * nbar + 15 * f),     # count, entries, 11 props in, 4 out
+            "stage": 3 * f * N * P,                      # x0, a in, x out
+            "save": 2 * f * N * P,                       # x in, x0 out
+            "pc_final": 3 * f * N * P}[kind]
+
+
+def dambreak_step_bytes(nd, nnz_ff, nnz_fw, dim=3):
+    """Algorithmic bytes of one PredictorCorrector step of the dam break:
+    two evaluations (the fluid re-binned, fluid<-fluid and fluid<-wall lists,
+    density, Tait, momentum), two stage updates, the save of the initial
+    values and the final step.  The walls do not move and are binned once."""
+    P = 2 * dim
+    nb = (nnz_ff + nnz_fw) / nd
+    ev = (alg_bytes("bin", nd) + alg_bytes("lists", nd, nnz_ff / nd) +
+          alg_bytes("lists", nd, nnz_fw / nd) + alg_bytes("density", nd, nb) +
+          alg_bytes("tait", nd) + alg_bytes("momentum", nd, nb))
+    return 2 * ev + 2 * alg_bytes("stage", nd, P=P) + alg_bytes("save", nd, P=P) + \
+        alg_bytes("pc_final", nd, P=P)
+
+
+# FP64 operations of one MomentumEquation pair term (mom_term): 6 differences
+# (r_ab, v_ab), v.r and |r|^2 (10), the denominator r^2 + eta^2 h^2 (4), mu (2
+# with the division), the pressure term (2), the viscosity (cbar, rhobar,
+# -alpha c mu + beta mu^2, the division: 10), the accumulation factor (2), the
+# kernel gradient (|r| again with the square root 6, q, the branch numerator
+# and denominator 4, the quotient and the factor 3) and the three components (9)
+MOM_FLOPS_PER_PAIR = 6 + 10 + 4 + 2 + 2 + 10 + 2 + 13 + 9
+
+
+def valu_block(mean_launch_s, pairs_per_launch):
+    """FP64 VALU view of the momentum kernel: algorithmic FP64 operations per
+    second against the FP64 vector peak, and the vector-issue floor of the
+    instructions the committed rocprofv3 SQ_INSTS_VALU profile counted."""
+    achieved = MOM_FLOPS_PER_PAIR * pairs_per_launch / mean_launch_s / 1e12
+    out = {"kernel": "k_momentum", "bound": "valu", "achieved": achieved,
+           "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+           "flops_per_pair": MOM_FLOPS_PER_PAIR}
+    prof = _profile_json(VALU_FILE)
+    if prof:
+        insts = float(prof["valu_wave_insts_per_launch"])
+        floor_s = insts * CYCLES_PER_VALU / SIMDS / CLOCK_HZ
+        out.update({"valu_wave_insts_per_launch": insts,
+                    "valu_lane_insts_per_pair": insts * 64 / pairs_per_launch,
+                    "issue_floor_ms": 1e3 * floor_s,
+                    "vector_issue_frac": floor_s / mean_launch_s,
+                    "profile": os.path.relpath(VALU_FILE, ROOT)})
+    return out
 
 
 def avisc_bytes(nd, nnz, ns_other):
@@ -259,11 +326,12 @@ def run_dambreak(args):
     # sources is current when momentum runs (Tait writes the fluid's, the
     # walls' is computed once), so the calls run k_momentum only.
     mom = prof.get("momentum_av", {"mean_ms": float("nan"), "count": 0, "total_ms": float("nan")})
-    # the fused launch reads and writes each destination once
-    b_eval = momentum_bytes(nd, nnz_ff + nnz_fw, nw, 0)
-    evals = 2 * args.steps
-    bytes_per_launch = b_eval * evals / max(mom["count"], 1)
+    # one launch per evaluation covers both lists and each destination once
+    nnz = nnz_ff + nnz_fw
+    bytes_per_launch = alg_bytes("momentum", nd, nnz / nd)
     achieved = bytes_per_launch / (mom["mean_ms"] * 1e-3) / 1e9
+    step_bytes = dambreak_step_bytes(nd, nnz_ff, nnz_fw)
+    step_gbs = step_bytes * args.steps / elapsed / 1e9
     result = {
         "metric": "pair-interactions/s",
         "value": pairs / elapsed,
@@ -292,8 +360,17 @@ def run_dambreak(args):
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": measured_traffic() if args.dx == 0.01 else None,
+            "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT) + " (rocprofv3 FETCH_SIZE and "
+                              "WRITE_SIZE passes of this command; not measured in this run)",
             "algorithmic_bytes_per_launch": bytes_per_launch,
+            "bytes_model": "N(4 + 4 nbar + 15 f): list count, 4 B per entry, 11 destination "
+                           "properties read and 4 written, f = 8; source records on chip",
             "mean_launch_ms": mom["mean_ms"],
+            "valu": valu_block(mom["mean_ms"] * 1e-3, nnz),
+            "step": {"algorithmic_bytes": step_bytes, "achieved": step_gbs,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS,
+                     "note": "all kernels of a PC step: binning, both list builds, density, "
+                             "Tait, momentum (2 evaluations) and the stage arithmetic"},
         },
     }
     return result, data
@@ -369,50 +446,44 @@ def run_lattice(args):
     }, d
 
 
-def run_batch(args):
-    """256 independent 2D elliptical drops (radius U[0.4, 2], dx from
-    {0.01, 0.02, 0.04}, velocity scale U[50, 150], seeded), RK2 with each
-    drop's time step 1e-5*min(1, dx/0.025) (the CFL number of the dx = 0.025
-    drop), advanced together by captured HIP graphs of --graph-steps steps;
-    compared with running the same drops one after another through the plain
-    path."""
+def batch_measure(nsims=256, precision="double", tune=None, autotune_steps=0, graph_steps=10,
+                  steps=20, warmup=10, seq_sims=0, log=None):
+    """The batched drops (radius U[0.4, 2], dx from {0.01, 0.02, 0.04},
+    velocity scale U[50, 150], seeded), RK2 with each drop's time step
+    1e-5*min(1, dx/0.025) (the CFL number of the dx = 0.025 drop), advanced
+    together by captured HIP graphs of graph_steps steps; compared with
+    running the same drops one after another through the plain path (all of
+    them, or seq_sims spread over the size range, scaled to the whole set)."""
     import torch
     from pysph_amd.batch import BatchedSimulation, batched_drops
     from pysph_amd.solver.configs import drop_solver
-    t_start = time.perf_counter()
-
-    def progress(msg):
-        print(f"[batch {time.perf_counter() - t_start:7.2f}s] {msg}", file=sys.stderr, flush=True)
-
-    sims, prm = batched_drops(nsims=args.nsims)
-    progress(f"{len(sims)} drops generated, {sum(len(d['x']) for d in sims)} particles")
-    b = BatchedSimulation(sims, dt_per_sim=prm["dt"], precision=args.precision)
+    log = log or (lambda msg: None)
+    sims, prm = batched_drops(nsims=nsims)
+    log(f"{len(sims)} drops generated, {sum(len(d['x']) for d in sims)} particles")
+    b = BatchedSimulation(sims, dt_per_sim=prm["dt"], precision=precision)
     dt = 1.0  # multiplies every drop's own time step
-    progress(f"packed: key_cap {b.key_cap} max_cells {b.max_cells} list_cap {b.list_cap} "
-             f"workspace {b.work.numel() / 2**20:.0f} MiB")
+    log(f"packed: key_cap {b.key_cap} max_cells {b.max_cells} list_cap {b.list_cap} "
+        f"workspace {b.work.numel() / 2**20:.0f} MiB")
     b.step(dt)
     b.check()
-    progress("eager step done")
-    tune = None
-    if args.launch_shape:
-        b.set_tuning({k: int(v) for k, v in
-                      (kv.split("=") for kv in args.launch_shape.split(","))})
-        progress(f"launch shape {b.tuning()}")
-    elif not args.no_autotune:
+    log("eager step done")
+    tuned = None
+    if tune:
+        b.set_tuning(tune)
+        log(f"launch shape {b.tuning()}")
+    elif autotune_steps:
         from pysph_amd.autotune import autotune
-        tune = autotune(b, dt, steps=3, use_cache=False, log=progress)
-        progress(f"autotuned: {tune['tuning']} {tune['ms_per_step']:.3f} ms/step "
-                 f"(default shape {tune['trials'][0][1]:.3f} ms/step)")
-    gs = max(1, args.graph_steps)
+        tuned = autotune(b, dt, steps=autotune_steps, use_cache=False, log=log)
+        log(f"autotuned: {tuned['tuning']} {tuned['ms_per_step']:.3f} ms/step "
+            f"(default shape {tuned['trials'][0][1]:.3f} ms/step)")
+    gs = max(1, graph_steps)
     b.capture(dt, nsteps=gs)
-    progress(f"graph of {gs} steps captured")
-    nrep_w = max(1, args.warmup // gs)
-    for _ in range(nrep_w):
+    log(f"graph of {gs} steps captured")
+    for _ in range(max(1, warmup // gs)):
         b.replay()
     b.check()
-    progress("warm-up replays done")
     pairs0 = b.pair_interactions()
-    nrep = max(1, args.steps // gs)
+    nrep = max(1, steps // gs)
     t0 = time.perf_counter()
     for _ in range(nrep):
         b.replay()
@@ -420,17 +491,18 @@ def run_batch(args):
     t1 = time.perf_counter()
     b.check()
     pairs = b.pair_interactions() - pairs0
-    steps = nrep * gs
+    nsteps = nrep * gs
     elapsed = t1 - t0
-    batch_sps = steps / elapsed
-    progress(f"timed {steps} steps: {1e3 * elapsed / steps:.3f} ms/step")
-    # sequential plain-path runs of the same drops (steps/s of the whole set)
+    batch_sps = nsteps / elapsed
+    log(f"timed {nsteps} steps: {1e3 * elapsed / nsteps:.3f} ms/step")
     order = np.argsort([len(d["x"]) for d in sims])
-    pick = order if not args.seq_sims else order[np.linspace(0, len(order) - 1, args.seq_sims).astype(int)]
+    pick = order if not seq_sims else order[np.linspace(0, len(order) - 1,
+                                                        seq_sims).astype(int)]
     seq_time = 0.0
     seq_steps = 2
     for k in pick:
-        solver, particles, _ = drop_solver(data=dict(sims[k]), dx=float(prm["dx"][k]))
+        solver, particles, _ = drop_solver(data=dict(sims[k]), dx=float(prm["dx"][k]),
+                                           precision=precision)
         dtk = float(prm["dt"][k])
         solver.integrator.integrate(dtk)          # first build, allocations
         torch.cuda.synchronize()
@@ -439,52 +511,71 @@ def run_batch(args):
             solver.integrator.integrate(dtk)
         torch.cuda.synchronize()
         seq_time += (time.perf_counter() - ta) / seq_steps
-        progress(f"sequential drop {k}: {len(sims[k]['x'])} particles")
     seq_time *= len(sims) / len(pick)
     seq_sps = 1.0 / seq_time
+    log(f"sequential plain path: {len(pick)} drops timed, {seq_sps:.2f} steps/s of the set")
+    res = {"steps_per_s": batch_sps, "ms_per_step": 1e3 * elapsed / nsteps,
+           "timed_steps": nsteps, "precision": precision,
+           "particles": b.n, "drops": len(sims),
+           "smallest_drop": int(min(b.counts)), "largest_drop": int(max(b.counts)),
+           "pair_interactions_per_s": pairs / elapsed,
+           "sequential_plain_gpu_steps_per_s": seq_sps,
+           "sequential_drops_timed": int(len(pick)),
+           "speedup_vs_sequential": batch_sps / seq_sps,
+           "graph_steps": gs, "launch_shape": b.tuning()}
+    if tuned is not None:
+        res["autotune"] = {"trials": len(tuned["trials"]),
+                           "default_shape_ms_per_step": tuned["trials"][0][1],
+                           "tuned_ms_per_step_eager": tuned["ms_per_step"]}
+    return res
+
+
+def run_batch(args):
+    """The 256-drop batch on its own (--workload batch)."""
+    t_start = time.perf_counter()
+
+    def progress(msg):
+        print(f"[batch {time.perf_counter() - t_start:7.2f}s] {msg}", file=sys.stderr, flush=True)
+
+    tune = None
+    if args.launch_shape:
+        tune = {k: int(v) for k, v in (kv.split("=") for kv in args.launch_shape.split(","))}
+    r = batch_measure(nsims=args.nsims, precision=args.precision, tune=tune,
+                      autotune_steps=0 if (tune or args.no_autotune) else 3,
+                      graph_steps=args.graph_steps, steps=args.steps, warmup=args.warmup,
+                      seq_sims=args.seq_sims, log=progress)
+    cfg = dict(r)
+    cfg["workload"] = (f"{r['drops']} independent 2D elliptical drops, RK2, "
+                       f"{'FP32 operators' if args.precision == 'single' else 'FP64'}, "
+                       f"dt 1e-5*min(1, dx/0.025) per drop, captured HIP graphs of "
+                       f"{r['graph_steps']} steps")
+    cfg["fused_kernels"] = "density+EOS, momentum+position stepping+stage"
     return {
         "metric": "steps/s",
-        "value": batch_sps,
+        "value": r["steps_per_s"],
         "unit": "steps/s (all drops advanced one RK2 step)",
-        "ms_per_step": 1e3 * elapsed / steps,
+        "ms_per_step": r["ms_per_step"],
         "dtype": "f32" if args.precision == "single" else "f64",
-        "config": {"workload": f"{len(sims)} independent 2D elliptical drops, RK2, "
-                               f"{'FP32 operators' if args.precision == 'single' else 'FP64'}, "
-                               f"dt 1e-5*min(1, dx/0.025) per drop, captured HIP graphs of "
-                               f"{gs} steps",
-                   "particles": b.n, "drops": len(sims),
-                   "smallest_drop": int(min(b.counts)), "largest_drop": int(max(b.counts)),
-                   "pair_interactions_per_s": pairs / elapsed,
-                   "sequential_plain_gpu_steps_per_s": seq_sps,
-                   "sequential_drops_timed": int(len(pick)),
-                   "speedup_vs_sequential": batch_sps / seq_sps,
-                   "fused_kernels": "density+EOS, momentum+position stepping+stage",
-                   "launch_shape": b.tuning(),
-                   "autotune": None if tune is None else {
-                       "trials": len(tune["trials"]),
-                       "default_shape_ms_per_step": tune["trials"][0][1],
-                       "tuned_ms_per_step_eager": tune["ms_per_step"]}},
+        "config": cfg,
         "roofline": None,
     }, None
 
 
-def run_adjoint(args):
+def adjoint_measure(fd_particles=64, log=None, reps=3):
     """Gradient of the final-position MSE of the 2D drop (dx = 0.02, 7,860
     particles, RK2, dt = 1e-5, 200 steps, checkpoint every 20) against a
     target run from u = -120 x, v = 120 y, with respect to the per-particle
     initial velocities (around u = -100 x, v = 100 y plus seeded N(0, 5)) and
-    alpha; checked against central differences (eps = 1e-6) on 64 random
-    particles, then fitted with 20 L-BFGS iterations."""
+    alpha: forward and backward time, the same 200 steps through the plain
+    per-simulation path, and central differences (eps = 1e-6) on
+    fd_particles random particles and alpha.  Returns (summary, state for
+    the fits)."""
     import torch
     from pysph_amd.adjoint import Rollout, rollout
     from pysph_amd.batch import BatchedSimulation
+    from pysph_amd.solver.configs import drop_solver
     from pysph_amd.solver.generators import elliptical_drop
-    t_start = time.perf_counter()
-
-    def progress(msg):
-        print(f"[adjoint {time.perf_counter() - t_start:7.2f}s] {msg}", file=sys.stderr,
-              flush=True)
-
+    log = log or (lambda msg: None)
     dx, steps, ckpt, dt, eps = 0.02, 200, 20, 1e-5, 1e-6
     d = elliptical_drop(dx=dx, radius=1.0, vel=100.0)
     n = len(d["x"])
@@ -498,7 +589,7 @@ def run_adjoint(args):
     rng = np.random.default_rng(20261202)
     u_init = -100.0 * d["x"] + rng.normal(0.0, 5.0, n)
     v_init = 100.0 * d["y"] + rng.normal(0.0, 5.0, n)
-    progress(f"{n} particles, target run done")
+    log(f"{n} particles, target run done")
 
     def loss_fn(x, y):
         return ((x - tx) ** 2 + (y - ty) ** 2).mean()
@@ -506,10 +597,8 @@ def run_adjoint(args):
     u0 = torch.tensor(u_init, device=dev, requires_grad=True)
     v0 = torch.tensor(v_init, device=dev, requires_grad=True)
     alpha = torch.tensor(0.1, dtype=torch.float64, device=dev, requires_grad=True)
-    # warm-up, then timed forward and backward
-    x, y = rollout(roll, alpha, u0, v0)
+    x, y = rollout(roll, alpha, u0, v0)   # warm-up
     torch.autograd.grad(loss_fn(x, y), (u0, v0, alpha))
-    reps = 3
     t_fwd = t_bwd = 0.0
     for _ in range(reps):
         torch.cuda.synchronize()
@@ -523,9 +612,20 @@ def run_adjoint(args):
         tc = time.perf_counter()
         t_fwd += (tb - ta) / reps
         t_bwd += (tc - tb) / reps
-    progress(f"forward {1e3 * t_fwd:.1f} ms, backward {1e3 * t_bwd:.1f} ms")
+    log(f"forward {1e3 * t_fwd:.1f} ms, backward {1e3 * t_bwd:.1f} ms")
+    # the same 200 steps through the plain per-simulation path (Solver)
+    dd = dict(d)
+    dd["u"], dd["v"] = u_init.copy(), v_init.copy()
+    solver, particles, _ = drop_solver(data=dd, dx=dx)
+    solver.integrator.integrate(dt)   # first build, allocations
+    torch.cuda.synchronize()
+    ta = time.perf_counter()
+    for _ in range(steps):
+        solver.integrator.integrate(dt)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - ta
     # central differences: every perturbed rollout is one simulation of a batch
-    picks = rng.choice(n, args.fd_particles, replace=False)
+    picks = rng.choice(n, fd_particles, replace=False)
     which = [(int(i), comp, sgn) for i in picks for comp in ("u", "v") for sgn in (1.0, -1.0)]
 
     def batch_losses(sim_list, alpha_val):
@@ -542,39 +642,62 @@ def run_adjoint(args):
         return out
 
     gmap = {"u": gu.cpu().numpy(), "v": gv.cpu().numpy()}
-
-    def fd_errors(e):
-        sims_e = []
-        for i, comp, sgn in which:
-            dd = dict(d)
-            dd["u"], dd["v"] = u_init.copy(), v_init.copy()
-            dd[comp][i] += sgn * e
-            sims_e.append(dd)
-        L = batch_losses(sims_e, 0.1)
-        g, fd = [], []
-        for k in range(0, len(sims_e), 2):
-            i, comp, _ = which[k]
-            g.append(gmap[comp][i])
-            fd.append((L[k] - L[k + 1]) / (2 * e))
-        g, fd = np.array(g), np.array(fd)
-        rel = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-300)
-        return rel, np.abs(g - fd).max() / np.abs(fd).max(), \
-            float(np.linalg.norm(g - fd) / np.linalg.norm(fd))
-
-    rel6, norm6, l2_6 = fd_errors(eps)
-    rel8, norm8, l2_8 = fd_errors(1e-8)
+    sims_e = []
+    for i, comp, sgn in which:
+        de = dict(d)
+        de["u"], de["v"] = u_init.copy(), v_init.copy()
+        de[comp][i] += sgn * eps
+        sims_e.append(de)
+    L = batch_losses(sims_e, 0.1)
+    g, fd = [], []
+    for k in range(0, len(sims_e), 2):
+        i, comp, _ = which[k]
+        g.append(gmap[comp][i])
+        fd.append((L[k] - L[k + 1]) / (2 * eps))
+    g, fd = np.array(g), np.array(fd)
+    rel = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-300)
     d0 = dict(d)
     d0["u"], d0["v"] = u_init.copy(), v_init.copy()
-    alpha_sweep = {}
-    for e in (1e-5, 1e-6, 1e-7, 1e-8):
-        la = batch_losses([d0], 0.1 + e)[0]
-        lb = batch_losses([d0], 0.1 - e)[0]
-        fda = (la - lb) / (2 * e)
-        alpha_sweep[f"{e:g}"] = abs(float(ga) - fda) / abs(fda)
-    progress(f"FD eps 1e-6: {len(rel6)} velocity components, |g - FD| / |FD| {l2_6:.3g}, "
-             f"max err / max |FD| {norm6:.3g}, componentwise max {rel6.max():.3g} median "
-             f"{np.median(rel6):.3g}; eps 1e-8: |g - FD| / |FD| {l2_8:.3g}; alpha rel err by "
-             f"eps {alpha_sweep}")
+    la = batch_losses([d0], 0.1 + eps)[0]
+    lb = batch_losses([d0], 0.1 - eps)[0]
+    fda = (la - lb) / (2 * eps)
+    ga_v = float(ga)
+    g_all, fd_all = np.append(g, ga_v), np.append(fd, fda)
+    summary = {
+        "workload": f"2D elliptical drop dx {dx} ({n} particles), RK2 dt {dt}, {steps} steps, "
+                    f"checkpoint every {ckpt}, FP64",
+        "forward_ms": 1e3 * t_fwd, "backward_ms": 1e3 * t_bwd,
+        "backward_over_forward": t_bwd / t_fwd,
+        "gradient_over_forward": (t_fwd + t_bwd) / t_fwd,
+        "plain_gpu_forward_ms": 1e3 * t_plain,
+        "forward_over_plain_gpu": t_fwd / t_plain,
+        "fd": {"eps": eps, "velocity_components": int(len(rel)),
+               "norm_rel_err": float(np.linalg.norm(g_all - fd_all) / np.linalg.norm(fd_all)),
+               "velocity_norm_rel_err": float(np.linalg.norm(g - fd) / np.linalg.norm(fd)),
+               "velocity_median_rel_err": float(np.median(rel)),
+               "velocity_max_rel_err": float(rel.max()),
+               "alpha_rel_err": abs(ga_v - fda) / abs(fda), "grad_alpha": ga_v},
+    }
+    log(f"FD eps {eps}: {summary['fd']}")
+    state = dict(roll=roll, d=d, n=n, dev=dev, tx=tx, ty=ty, u_init=u_init, v_init=v_init,
+                 loss=loss, loss_fn=loss_fn, steps=steps, dt=dt, rollout=rollout)
+    return summary, state
+
+
+def run_adjoint(args):
+    """cfg 4 on its own (--workload adjoint): the measurements of
+    adjoint_measure, then 20-iteration L-BFGS fits."""
+    import torch
+    t_start = time.perf_counter()
+
+    def progress(msg):
+        print(f"[adjoint {time.perf_counter() - t_start:7.2f}s] {msg}", file=sys.stderr,
+              flush=True)
+
+    summary, st = adjoint_measure(args.fd_particles, progress)
+    roll, d, n, dev = st["roll"], st["d"], st["n"], st["dev"]
+    u_init, v_init, loss, loss_fn = st["u_init"], st["v_init"], st["loss"], st["loss_fn"]
+    steps, dt, rollout = st["steps"], st["dt"], st["rollout"]
     # fits, loss relative to its initial value l0.  (a) per-particle initial
     # velocities and alpha, as the workload names them.  The final positions
     # move by about T = steps*dt per unit of initial velocity, so the
@@ -630,41 +753,47 @@ def run_adjoint(args):
     progress(f"L-BFGS quadratic velocity field + alpha: {len(losses_f)} evaluations, loss "
              f"{losses_f[0]:.4g} -> {losses_f[-1]:.4g}; x coefficient of u "
              f"{-100.0 + s_c * float(cu[1]):.3f}, y coefficient of v {100.0 + s_c * float(cv[2]):.3f}")
+    cfg = dict(summary)
+    cfg.update({"lbfgs_iterations": args.lbfgs_iters,
+                "fit_per_particle": {
+                    "evaluations": len(losses), "loss_initial": losses[0],
+                    "loss_final": losses[-1], "loss_reduction": losses[-1] / losses[0],
+                    "alpha_fitted": 0.1 + s_a * float(da.item()), "fit_s": t_fit,
+                    "max_velocity_change": float(s_u * max(du.abs().max().item(),
+                                                           dv.abs().max().item()))},
+                "fit_quadratic_field": {
+                    "evaluations": len(losses_f), "loss_initial": losses_f[0],
+                    "loss_final": losses_f[-1],
+                    "loss_reduction": losses_f[-1] / losses_f[0],
+                    "alpha_fitted": 0.1 + s_a * float(db.item()), "fit_s": t_fit_f,
+                    "u_x_coefficient": -100.0 + s_c * float(cu[1]),
+                    "v_y_coefficient": 100.0 + s_c * float(cv[2])}})
     return {
         "metric": "adjoint/forward cost ratio",
-        "value": t_bwd / t_fwd,
+        "value": summary["backward_over_forward"],
         "unit": "backward time / forward time",
-        "ms_per_step": 1e3 * t_fwd / steps,
+        "ms_per_step": summary["forward_ms"] / steps,
         "higher_is_better": False,
         "dtype": "f64",
-        "config": {"workload": f"2D elliptical drop dx {dx} ({n} particles), RK2 dt {dt}, "
-                               f"{steps} steps, checkpoint every {ckpt}, FP64",
-                   "forward_ms": 1e3 * t_fwd, "backward_ms": 1e3 * t_bwd,
-                   "fd_eps": eps, "fd_components": int(len(rel6)),
-                   "fd_norm_rel_err": l2_6, "fd_eps_1e-8_norm_rel_err": l2_8,
-                   "fd_max_rel_err": float(rel6.max()),
-                   "fd_median_rel_err": float(np.median(rel6)),
-                   "fd_max_err_over_max_fd": float(norm6),
-                   "fd_eps_1e-8_max_rel_err": float(rel8.max()),
-                   "fd_eps_1e-8_median_rel_err": float(np.median(rel8)),
-                   "fd_alpha_rel_err_by_eps": alpha_sweep,
-                   "grad_alpha": float(ga),
-                   "lbfgs_iterations": args.lbfgs_iters,
-                   "fit_per_particle": {
-                       "evaluations": len(losses), "loss_initial": losses[0],
-                       "loss_final": losses[-1], "loss_reduction": losses[-1] / losses[0],
-                       "alpha_fitted": 0.1 + s_a * float(da.item()), "fit_s": t_fit,
-                       "max_velocity_change": float(s_u * max(du.abs().max().item(),
-                                                              dv.abs().max().item()))},
-                   "fit_quadratic_field": {
-                       "evaluations": len(losses_f), "loss_initial": losses_f[0],
-                       "loss_final": losses_f[-1],
-                       "loss_reduction": losses_f[-1] / losses_f[0],
-                       "alpha_fitted": 0.1 + s_a * float(db.item()), "fit_s": t_fit_f,
-                       "u_x_coefficient": -100.0 + s_c * float(cu[1]),
-                       "v_y_coefficient": 100.0 + s_c * float(cv[2])}},
+        "config": cfg,
         "roofline": None,
     }, None
+
+
+def cpu_info():
+    """Host CPU model and the oracle's compiler flags, for the baseline line."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    from oracle import build as refbuild
+    return {"cpu_model": model, "compiler": "gcc " + " ".join(
+        c for c in refbuild.CFLAGS if c.startswith(("-O", "-f")))}
 
 
 def cpu_baseline_dambreak(args, data):
@@ -696,7 +825,7 @@ def cpu_baseline_dambreak(args, data):
             "sample": f"1 PredictorCorrector step of the same dam break ({ref.pairs} pair "
                       f"interactions, {t1 - t0:.2f} s, {threads} OpenMP threads, gcc -O2 "
                       f"-ffp-contract=off)",
-            "steps_per_s": 1.0 / (t1 - t0),
+            "steps_per_s": 1.0 / (t1 - t0), **cpu_info(),
             "single_thread": {"value": ref1.pairs / (s1 - s0), "cores": 1,
                               "sample": f"1 evaluation (both neighbour lists, SPHRho, Tait, "
                                         f"momentum) of the same dam break, {ref1.pairs} pair "
@@ -723,27 +852,39 @@ def cpu_baseline_dambreak_split(args, data):
             "sample": f"1 PredictorCorrector step of the same operator-split dam break "
                       f"({ref.pairs} pair interactions, {t1 - t0:.2f} s, {threads} OpenMP "
                       f"threads)",
-            "steps_per_s": 1.0 / (t1 - t0)}
+            "steps_per_s": 1.0 / (t1 - t0), **cpu_info()}
 
 
 def cpu_baseline_lattice(args, d):
+    """The oracle's NNPS + SummationDensity pass over the same lattice.  The
+    variable-h lists of the oracle are built by one thread, so with
+    --variable-h the list build is timed (and reported) on its own at one
+    thread and the density pass at all cores."""
     from oracle import cpu_ref
-    import numpy as np
     threads = args.cpu_threads or len(os.sched_getaffinity(0))
     t0 = time.perf_counter()
     cs = cpu_ref.compute_cell_size(d["h"].max(), 2.0)
-    if args.variable_h:  # the oracle's variable-h lists are serial
+    if args.variable_h:
         off, idx = cpu_ref.nnps_varh(d, d, 2.0, cs)
     else:
         off, idx = cpu_ref.nnps(d, d, 2.0, cs, True, nthreads=threads)
+    t_lists = time.perf_counter() - t0
     rho = np.zeros(len(d["x"]))
     cpu_ref.summation_density(3, d, d, off, idx, rho, nthreads=threads)
     t1 = time.perf_counter()
-    return {"value": len(idx) / (t1 - t0), "unit": "pair-interactions/s", "cores": threads,
-            "kind": "port", "sample": f"1 NNPS + SummationDensity pass of the same lattice "
-                                      f"({t1 - t0:.2f} s, {threads} OpenMP threads"
-                                      + (", lists built serially" if args.variable_h else "")
-                                      + ")"}
+    out = {"value": len(idx) / (t1 - t0), "unit": "pair-interactions/s", "cores": threads,
+           "kind": "port", "sample": f"1 NNPS + SummationDensity pass of the same lattice "
+                                     f"({t1 - t0:.2f} s, {threads} OpenMP threads)",
+           **cpu_info()}
+    if args.variable_h:
+        out["cores"] = 1
+        out["sample"] = (f"1 NNPS + SummationDensity pass of the same lattice ({t1 - t0:.2f} s): "
+                         f"variable-h lists built serially ({t_lists:.2f} s, 1 thread), "
+                         f"density on {threads} OpenMP threads ({t1 - t0 - t_lists:.2f} s)")
+        out["lists_s_one_thread"] = t_lists
+        out["density_s"] = t1 - t0 - t_lists
+        out["density_threads"] = threads
+    return out
 
 
 def main():
@@ -779,6 +920,21 @@ def main():
             out["cpu_baseline"] = cpu_baseline_dambreak_split(args, data)
         else:
             out["cpu_baseline"] = cpu_baseline_lattice(args, data)
+    if args.workload == "dambreak" and not args.no_extras:
+        # the batched and adjoint forms beside the plain path, bounded (about
+        # a minute): the 256-drop batch against sequential plain-path runs of
+        # 32 of its drops, and the adjoint configuration's costs and FD check
+        t_start = time.perf_counter()
+
+        def log(msg):
+            print(f"[extras {time.perf_counter() - t_start:7.2f}s] {msg}", file=sys.stderr,
+                  flush=True)
+
+        out["batch"] = batch_measure(nsims=256, tune=BATCH_LAUNCH_SHAPE, graph_steps=10,
+                                     steps=20, warmup=10, seq_sims=32, log=log)
+        out["batch"]["launch_shape_note"] = ("the autotuner's pick for this batch "
+                                             "(--workload batch runs the autotuner)")
+        out["adjoint"] = adjoint_measure(64, log)[0]
     print(json.dumps(out), flush=True)
 
 

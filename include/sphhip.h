@@ -250,6 +250,20 @@ int sph_nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
                         const int32_t *s_rank, int64_t ns, int64_t ns_real, double radius_scale,
                         double max_h, int32_t *counts, int64_t *slice_off, int32_t *indices,
                         int64_t cap, int64_t *slots_out, int64_t *nnz_out);
+/* The same lists over float records (the single-precision variant,
+ * cl_precision="single"): both distance tests in float; d_x, d_y, d_z are
+ * the destination's double coordinates in original order (the ones the
+ * binning used), from which each destination's cell is taken.  Pairs at a
+ * distance within float rounding of either radius may differ from the FP64
+ * lists. */
+int sph_nnps_build_varh_f32(sph_ctx *ctx, const sph_grid *grid, const float *d_rec,
+                            const double *d_x, const double *d_y, const double *d_z,
+                            const int32_t *d_perm, int64_t nd_total, int64_t nd_real,
+                            const int32_t *s_cell_start, const float *s_rec,
+                            const int32_t *s_perm, const int32_t *s_rank, int64_t ns,
+                            int64_t ns_real, double radius_scale, double max_h, int32_t *counts,
+                            int64_t *slice_off, int32_t *indices, int64_t cap,
+                            int64_t *slots_out, int64_t *nnz_out);
 
 /* The same lists in ORIGINAL index space as CSR, the layout of the
  * reference's per-destination LongArrays (nnps.pyx:596-621): row i
@@ -342,6 +356,10 @@ typedef struct sph_sources {
     const void *s_rec[SPH_MAX_SRC];
     int64_t ns[SPH_MAX_SRC];
     const void *s_pm[SPH_MAX_SRC];
+    /* the _staged operators only: each source's cell starts (its binning's
+     * cell_start) and whether it is the destination array itself */
+    const int32_t *s_cell_start[SPH_MAX_SRC];
+    int32_t same[SPH_MAX_SRC];
 } sph_sources;
 int sph_summation_density_multi(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
                                 const int32_t *d_perm, const sph_sources *src,
@@ -360,6 +378,29 @@ int sph_momentum_av_multi_f32(sph_ctx *ctx, int dim, int64_t nd_total, int64_t n
                               int uniform_h, double alpha, double beta, double eta,
                               int accumulate, double *ax, double *ay, double *az,
                               double *dt_fac);
+
+/* The same two operators (FP64) with the neighbour records staged in LDS,
+ * for grids in the linear cell order (grid->order == SPH_ORDER_LINEAR).
+ * With linear keys the reference's row order (stencil x -> y -> z, ascending
+ * index, own index last) is ascending sorted position apart from the
+ * own-index swap, and each stencil pencil (cx+dx, cy+dy, all z of the reach)
+ * is one contiguous range of sorted positions: one wavefront per 64-row slice
+ * copies the pencil's records into LDS with contiguous loads and sums its
+ * rows' entries from there, pencil by pencil, in list order.  Results equal
+ * sph_summation_density_multi / sph_momentum_av_multi bit for bit (the same
+ * sums in the same order).  Extra arguments: the grid the lists were built
+ * on, their radius_scale, and src->s_cell_start / src->same.  Same loop as
+ * SPHFunction.eval over each source (sph/sph_func.pyx:208-227). */
+int sph_summation_density_staged(sph_ctx *ctx, const sph_grid *grid, double radius_scale,
+                                 int dim, int64_t nd_total, int64_t nd_real,
+                                 const int32_t *d_perm, const sph_sources *src,
+                                 const double *d_rec, int uniform_h, int accumulate, double *rho,
+                                 double *out_s, int64_t out_s_stride);
+int sph_momentum_av_staged(sph_ctx *ctx, const sph_grid *grid, double radius_scale, int dim,
+                           int64_t nd_total, int64_t nd_real, const int32_t *d_perm,
+                           const sph_sources *src, const double *d_rec, int uniform_h,
+                           double alpha, double beta, double eta, int accumulate, double *ax,
+                           double *ay, double *az, double *dt_fac);
 
 /* SPHDensityRate (sph/funcs/density_funcs.pyx:116-164, hks off): the
  * continuity equation rate_a = sum_b (v_a - v_b) . gradW_a(r_ab, h_ab) m_b over
@@ -502,7 +543,8 @@ int sph_fill_many(sph_ctx *ctx, int64_t n, double value, const sph_prop_list *pr
  * The step never synchronises with the host: capacities bound every size and
  * violations set bits of *status (1: neighbour slots > list_cap, 2: cell keys
  * > key_cap, 4: non-finite coordinate or h, 8: grid too large, 16: non-empty
- * cells > max_cells, 32: reverse mode over particles of unequal h); a step
+ * cells > max_cells, 32: reverse mode over particles of unequal h, 64: the
+ * persistent evaluation's grid barrier timed out); a step
  * with a nonzero status is void.  *pairs
  * accumulates the list entries of every build (each build's lists serve two
  * operators). */
@@ -542,6 +584,12 @@ typedef struct sph_batch_desc {
     int64_t work_bytes;
     int32_t *status;   /* device */
     int64_t *pairs;    /* device */
+    int32_t persistent;       /* 1: each evaluation after its lists is ONE launch --
+                               * SPHRho + Tait, a grid barrier, MomentumEquation +
+                               * PositionStepping + stage -- over at most the
+                               * blocks the device holds at once (status bit 64 if
+                               * the barrier's bounded wait runs out); 0: two
+                               * launches.  Same results either way. */
 } sph_batch_desc;
 
 int sph_batch_workspace_size(sph_ctx *ctx, const sph_batch_desc *d, int64_t *bytes);

@@ -67,7 +67,8 @@ class Sources(ctypes.Structure):
     """sph_sources (include/sphhip.h)."""
     _fields_ = [("nsrc", ctypes.c_int32), ("counts", _vp * MAX_SRC), ("slice_off", _vp * MAX_SRC),
                 ("indices", _vp * MAX_SRC), ("s_rec", _vp * MAX_SRC), ("ns", _i64 * MAX_SRC),
-                ("s_pm", _vp * MAX_SRC)]
+                ("s_pm", _vp * MAX_SRC), ("s_cell_start", _vp * MAX_SRC),
+                ("same", ctypes.c_int32 * MAX_SRC)]
 
 
 PAIR_PRESSURE_GRADIENT = 1
@@ -104,7 +105,7 @@ class BatchDesc(ctypes.Structure):
                    ("op_blocks_per_cu", ctypes.c_int32), ("nbr_tile", ctypes.c_int32),
                    ("cells_per_block", ctypes.c_int32), ("work", _vp),
                    ("work_bytes", _i64),
-                   ("status", _vp), ("pairs", _vp)])
+                   ("status", _vp), ("pairs", _vp), ("persistent", ctypes.c_int32)])
 
 
 # name -> (restype, argtypes)
@@ -131,6 +132,10 @@ _SIGNATURES = {
     "sph_nnps_build_varh": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _i64, _i64, _vp, _vp,
                                    _vp, _vp, _i64, _i64, _dbl, _dbl, _vp, _vp, _vp, _i64,
                                    ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "sph_nnps_build_varh_f32": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _vp, _vp,
+                                       _i64, _i64, _vp, _vp, _vp, _vp, _i64, _i64, _dbl, _dbl,
+                                       _vp, _vp, _vp, _i64, ctypes.POINTER(_i64),
+                                       ctypes.POINTER(_i64)]),
     "sph_bin_array_dev": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _i64, _vp, _vp,
                                  _vp, _vp, _vp, _vp]),
     "sph_nnps_build_dev": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _vp, _vp, _i64,
@@ -156,6 +161,12 @@ _SIGNATURES = {
     "sph_momentum_av_multi_f32": (_i32, [_vp, _i32, _i64, _i64, _vp, ctypes.POINTER(Sources),
                                          _vp, _i32, _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp,
                                          _vp]),
+    "sph_summation_density_staged": (_i32, [_vp, ctypes.POINTER(SphGrid), _dbl, _i32, _i64, _i64,
+                                            _vp, ctypes.POINTER(Sources), _vp, _i32, _i32, _vp,
+                                            _vp, _i64]),
+    "sph_momentum_av_staged": (_i32, [_vp, ctypes.POINTER(SphGrid), _dbl, _i32, _i64, _i64, _vp,
+                                      ctypes.POINTER(Sources), _vp, _i32, _dbl, _dbl, _dbl, _i32,
+                                      _vp, _vp, _vp, _vp]),
     "sph_density_rate": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32,
                                 _vp]),
     "sph_density_rate_f32": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32,

@@ -586,8 +586,6 @@ class VarHNbrParticleLocator(FixedDestNbrParticleLocator):
 
     def _build(self):
         cm = self.cell_manager
-        if cm.precision == "single":
-            raise NotImplementedError("variable-h lists are built in double precision only")
         ctx = cm._ctx()
         lib = ctx.lib
         dst, src = self.dest, self.source
@@ -610,6 +608,14 @@ class VarHNbrParticleLocator(FixedDestNbrParticleLocator):
                     for a in (dst, src) if a.get_number_of_particles() > 0)
 
         def call():
+            if cm.precision == "single":  # float tests, the binning's double coordinates
+                return lib.sph_nnps_build_varh_f32(
+                    ctx.handle, ctypes.byref(cm.grid), p(drec), p(dst.get_carray("x")),
+                    p(dst.get_carray("y")), p(dst.get_carray("z")), p(bd.perm), nd_total,
+                    dst.num_real_particles, p(bs.cell_start), p(srec), p(bs.perm), p(bs.rank),
+                    bs.n, src.num_real_particles, self.radius_scale, max_h, p(self.counts),
+                    p(self.slice_off), p(self.indices), self.indices.numel(),
+                    ctypes.byref(slots), ctypes.byref(nnz))
             return lib.sph_nnps_build_varh(
                 ctx.handle, ctypes.byref(cm.grid), p(drec), p(bd.perm), nd_total,
                 dst.num_real_particles, p(bs.cell_start), p(srec), p(bs.perm), p(bs.rank), bs.n,

@@ -29,10 +29,12 @@ _STATE = ("x", "y", "z", "u", "v", "w", "h", "m", "rho", "p", "cs",
 STATUS_BITS = {1: "neighbour-list slots exceed list_cap", 2: "cell keys exceed key_cap",
                4: "non-finite coordinate or h", 8: "grid too large",
                16: "non-empty cells exceed max_cells",
-               32: "h differs within a simulation (uniform_h or the reverse mode needs one h)"}
+               32: "h differs within a simulation (uniform_h or the reverse mode needs one h)",
+               64: "the persistent evaluation's grid barrier timed out"}
 
 # launch shapes of the fused step (sph_batch_desc); 0 = the library default
-TUNE_FIELDS = ("op_threads", "op_blocks_per_cu", "nbr_tile", "cells_per_block")
+# (persistent: 1 = one launch per evaluation with a grid barrier, 0 = two)
+TUNE_FIELDS = ("op_threads", "op_blocks_per_cu", "nbr_tile", "cells_per_block", "persistent")
 
 
 class BatchError(RuntimeError):

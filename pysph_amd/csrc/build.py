@@ -17,7 +17,8 @@ ROOT = os.path.dirname(PKG)
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "libsphhip.so")
 
-SOURCES = ["sph_ctx.hip", "sph_nnps.hip", "sph_ops.hip", "sph_rate_xsph.hip", "sph_pairs.hip", "sph_batch.hip"]
+SOURCES = ["sph_ctx.hip", "sph_nnps.hip", "sph_ops.hip", "sph_rate_xsph.hip", "sph_pairs.hip", "sph_batch.hip",
+           "sph_staged.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -265,6 +265,70 @@ __device__ __forceinline__ void fused_tiles(int64_t ntiles, int64_t &tb, int64_t
     te = (L + 1) * ntiles / G;
 }
 
+// SPHRho, then Tait and P/rho^2, of sorted particle t
+template <bool UH, typename T, int NT>
+__device__ __forceinline__ void density_eos_one(int dim, int64_t t,
+                                                const int32_t *__restrict__ perm,
+                                                const int32_t *__restrict__ counts,
+                                                const int64_t *__restrict__ slice_off,
+                                                const int32_t *__restrict__ indices,
+                                                T *__restrict__ rec, T ro, T co, T gamma, T B,
+                                                double *__restrict__ rho, double *__restrict__ p,
+                                                double *__restrict__ cs)
+{
+    const T *ra = rec + t * SPH_REC;
+    const auto *dr = rec2<T>(rec, t);
+    const auto dxy = dr[0], dzm = dr[1];
+    const T xa = dxy.x, ya = dxy.y, za = dzm.x, ha = ra[SPH_R_H];
+    const T facA = sph_h_dim(ha, dim) * sph_cubic_fac1<T>(dim);
+    T nr = T(0.0);
+    const int32_t *row = indices + slice_off[t >> 6] + (t & 63);
+    const int32_t e = counts[t];
+    int32_t k = 0;
+    for (; k + NT <= e; k += NT) {
+        int32_t j[NT];
+        typename Pair<T>::type a[NT], b[NT];
+        T hb[NT];
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+            j[u] = __builtin_nontemporal_load(row + ((int64_t)(k + u) << 6));
+#pragma unroll
+        for (int u = 0; u < NT; ++u) {
+            const auto *sr = rec2<T>(rec, j[u]);
+            a[u] = sr[0];
+            b[u] = sr[1];
+            hb[u] = UH ? ha : rec[(int64_t)j[u] * SPH_REC + SPH_R_H];
+        }
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+            nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, a[u], b[u], hb[u]);
+    }
+    for (; k < e; ++k) {
+        const int32_t j = __builtin_nontemporal_load(row + ((int64_t)k << 6));
+        const auto *sr = rec2<T>(rec, j);
+        const T hb = UH ? ha : rec[(int64_t)j * SPH_REC + SPH_R_H];
+        nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, sr[0], sr[1], hb);
+    }
+    // SPHRho is non-integrating: the output starts from 0 (sph_calc.pyx:319-321)
+    const T rhoa = T(0.0) + nr;
+    // Tait (eos_funcs.pyx:118-129), as k_tait_rec
+    const T ratio = rhoa / ro;
+    const T gamma2 = T(0.5) * (gamma - T(1.0));
+    const T tmp = pow(ratio, gamma);
+    const T r0 = T(0.0) + (tmp - T(1.0)) * B;
+    const T r1 = T(0.0) + pow(ratio, gamma2) * co;
+    const T rhob2 = T(1.0) / (rhoa * rhoa);
+    const int32_t i = perm[t];
+    rho[i] = rhoa;
+    p[i] = r0;
+    cs[i] = r1;
+    T *rw = rec + t * SPH_REC;
+    rw[SPH_R_RHO] = rhoa;
+    rw[SPH_R_P] = r0;
+    rw[SPH_R_CS] = r1;
+    rw[SPH_R_PRHO2] = r0 * rhob2;
+}
+
 template <bool UH, typename T, int NT>
 __global__ void __launch_bounds__(FUSED_MAX_THREADS)
     k_batch_density_eos(int dim, int64_t n, int64_t ntiles, const int32_t *__restrict__ perm,
@@ -278,57 +342,8 @@ __global__ void __launch_bounds__(FUSED_MAX_THREADS)
     for (int64_t tile = tb; tile < te; ++tile) {
         const int64_t t = tile * blockDim.x + threadIdx.x;
         if (t >= n) break;
-        const T *ra = rec + t * SPH_REC;
-        const auto *dr = rec2<T>(rec, t);
-        const auto dxy = dr[0], dzm = dr[1];
-        const T xa = dxy.x, ya = dxy.y, za = dzm.x, ha = ra[SPH_R_H];
-        const T facA = sph_h_dim(ha, dim) * sph_cubic_fac1<T>(dim);
-        T nr = T(0.0);
-        const int32_t *row = indices + slice_off[t >> 6] + (t & 63);
-        const int32_t e = counts[t];
-        int32_t k = 0;
-        for (; k + NT <= e; k += NT) {
-            int32_t j[NT];
-            typename Pair<T>::type a[NT], b[NT];
-            T hb[NT];
-#pragma unroll
-            for (int u = 0; u < NT; ++u)
-                j[u] = __builtin_nontemporal_load(row + ((int64_t)(k + u) << 6));
-#pragma unroll
-            for (int u = 0; u < NT; ++u) {
-                const auto *sr = rec2<T>(rec, j[u]);
-                a[u] = sr[0];
-                b[u] = sr[1];
-                hb[u] = UH ? ha : rec[(int64_t)j[u] * SPH_REC + SPH_R_H];
-            }
-#pragma unroll
-            for (int u = 0; u < NT; ++u)
-                nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, a[u], b[u], hb[u]);
-        }
-        for (; k < e; ++k) {
-            const int32_t j = __builtin_nontemporal_load(row + ((int64_t)k << 6));
-            const auto *sr = rec2<T>(rec, j);
-            const T hb = UH ? ha : rec[(int64_t)j * SPH_REC + SPH_R_H];
-            nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, sr[0], sr[1], hb);
-        }
-        // SPHRho is non-integrating: the output starts from 0 (sph_calc.pyx:319-321)
-        const T rhoa = T(0.0) + nr;
-        // Tait (eos_funcs.pyx:118-129), as k_tait_rec
-        const T ratio = rhoa / ro;
-        const T gamma2 = T(0.5) * (gamma - T(1.0));
-        const T tmp = pow(ratio, gamma);
-        const T r0 = T(0.0) + (tmp - T(1.0)) * B;
-        const T r1 = T(0.0) + pow(ratio, gamma2) * co;
-        const T rhob2 = T(1.0) / (rhoa * rhoa);
-        const int32_t i = perm[t];
-        rho[i] = rhoa;
-        p[i] = r0;
-        cs[i] = r1;
-        T *rw = rec + t * SPH_REC;
-        rw[SPH_R_RHO] = rhoa;
-        rw[SPH_R_P] = r0;
-        rw[SPH_R_CS] = r1;
-        rw[SPH_R_PRHO2] = r0 * rhob2;
+        density_eos_one<UH, T, NT>(dim, t, perm, counts, slice_off, indices, rec, ro, co, gamma,
+                                   B, rho, p, cs);
     }
 }
 
@@ -336,6 +351,103 @@ struct FusedState {
     double *x[3], *u[3], *x0[3], *u0[3], *ax[3], *au[3];
     double *dt_fac;
 };
+
+// MomentumEquation + AV, then PositionStepping and the stage update, of
+// sorted particle t
+template <bool UH, typename T, int NT>
+__device__ __forceinline__ void momentum_stage_one(int dim, int64_t t,
+                                                   const int32_t *__restrict__ perm,
+                                                   const int32_t *__restrict__ counts,
+                                                   const int64_t *__restrict__ slice_off,
+                                                   const int32_t *__restrict__ indices,
+                                                   const T *__restrict__ rec, T alpha, T beta,
+                                                   T eta, const FusedState &st,
+                                                   const int32_t *__restrict__ sim_of,
+                                                   const double *__restrict__ dt_sim,
+                                                   double scale, int save)
+{
+    const auto *dr = rec2<T>(rec, t);
+    const auto d0 = dr[0], d1 = dr[1], d2 = dr[2], d3 = dr[3], d4 = dr[4], d5 = dr[5];
+    MomDest<T> D;
+    D.xa = d0.x;
+    D.ya = d0.y;
+    D.za = d1.x;
+    D.ua = d2.x;
+    D.va = d2.y;
+    D.wa = d3.x;
+    D.rhoa = d4.x;
+    D.ca = d4.y;
+    D.ha = d5.x;
+    D.Pa = d5.y;
+    D.facA = sph_h_dim(D.ha, dim) * sph_cubic_fac1<T>(dim);
+    D.rhoa2 = T(1.0) / (D.rhoa * D.rhoa);
+    T dtf = T(0.0);  // dt_fac is reset per calc (pressure_funcs.pyx:214)
+    T nr0 = T(0.0), nr1 = T(0.0), nr2 = T(0.0);
+    const int32_t *row = indices + slice_off[t >> 6] + (t & 63);
+    const int32_t e = counts[t];
+    int32_t k = 0;
+    for (; k + NT <= e; k += NT) {
+        int32_t j[NT];
+        typename Pair<T>::type s0[NT], s1[NT], s2[NT], s3[NT], s4[NT];
+        T hb[NT];
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+            j[u] = __builtin_nontemporal_load(row + ((int64_t)(k + u) << 6));
+#pragma unroll
+        for (int u = 0; u < NT; ++u) {
+            const auto *sr = rec2<T>(rec, j[u]);
+            s0[u] = sr[0];
+            s1[u] = sr[1];
+            s2[u] = sr[2];
+            s3[u] = sr[3];
+            s4[u] = sr[4];
+            hb[u] = UH ? D.ha : rec[(int64_t)j[u] * SPH_REC + SPH_R_H];
+        }
+#pragma unroll
+        for (int u = 0; u < NT; ++u)
+            mom_term<UH, T>(dim, D, alpha, beta, eta, s0[u], s1[u], s2[u], s3[u], s4[u],
+                            hb[u], dtf, nr0, nr1, nr2);
+    }
+    for (; k < e; ++k) {
+        const int32_t j = __builtin_nontemporal_load(row + ((int64_t)k << 6));
+        const auto *sr = rec2<T>(rec, j);
+        const T hb = UH ? D.ha : rec[(int64_t)j * SPH_REC + SPH_R_H];
+        mom_term<UH, T>(dim, D, alpha, beta, eta, sr[0], sr[1], sr[2], sr[3], sr[4], hb, dtf,
+                        nr0, nr1, nr2);
+    }
+    const int32_t i = perm[t];
+    st.dt_fac[i] = dtf;
+    // the accelerations of u, v, w (integrating calc: output 0 + sum) and
+    // PositionStepping's a_x = u; then x <- x0 + a*dt per stepped property
+    const double acc[3] = {(double)(T(0.0) + nr0), (double)(T(0.0) + nr1),
+                           (double)(T(0.0) + nr2)};
+    const double dt = dt_sim ? scale * dt_sim[sim_of[i]] : scale;
+    // every component is read before the first store (memory operations
+    // retire in order: a load behind a store waits for it)
+    double uc[3] = {0.0, 0.0, 0.0}, xc[3] = {0.0, 0.0, 0.0};
+    double u0[3] = {0.0, 0.0, 0.0}, x0[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (c >= dim) break;
+        uc[c] = st.u[c][i];
+        xc[c] = st.x[c][i];
+        u0[c] = save ? uc[c] : st.u0[c][i];
+        x0[c] = save ? xc[c] : st.x0[c][i];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (c >= dim) break;
+        if (save) {
+            st.u0[c][i] = uc[c];
+            st.x0[c][i] = xc[c];
+        }
+        st.au[c][i] = acc[c];
+        st.ax[c][i] = uc[c];
+        const double tu = acc[c] * dt, tx = uc[c] * dt;
+        st.u[c][i] = u0[c] + tu;
+        st.x[c][i] = x0[c] + tx;
+    }
+}
 
 template <bool UH, typename T, int NT>
 __global__ void __launch_bounds__(FUSED_MAX_THREADS)
@@ -352,87 +464,75 @@ __global__ void __launch_bounds__(FUSED_MAX_THREADS)
     for (int64_t tile = tb; tile < te; ++tile) {
         const int64_t t = tile * blockDim.x + threadIdx.x;
         if (t >= n) break;
-        const auto *dr = rec2<T>(rec, t);
-        const auto d0 = dr[0], d1 = dr[1], d2 = dr[2], d3 = dr[3], d4 = dr[4], d5 = dr[5];
-        MomDest<T> D;
-        D.xa = d0.x;
-        D.ya = d0.y;
-        D.za = d1.x;
-        D.ua = d2.x;
-        D.va = d2.y;
-        D.wa = d3.x;
-        D.rhoa = d4.x;
-        D.ca = d4.y;
-        D.ha = d5.x;
-        D.Pa = d5.y;
-        D.facA = sph_h_dim(D.ha, dim) * sph_cubic_fac1<T>(dim);
-        D.rhoa2 = T(1.0) / (D.rhoa * D.rhoa);
-        T dtf = T(0.0);  // dt_fac is reset per calc (pressure_funcs.pyx:214)
-        T nr0 = T(0.0), nr1 = T(0.0), nr2 = T(0.0);
-        const int32_t *row = indices + slice_off[t >> 6] + (t & 63);
-        const int32_t e = counts[t];
-        int32_t k = 0;
-        for (; k + NT <= e; k += NT) {
-            int32_t j[NT];
-            typename Pair<T>::type s0[NT], s1[NT], s2[NT], s3[NT], s4[NT];
-            T hb[NT];
-#pragma unroll
-            for (int u = 0; u < NT; ++u)
-                j[u] = __builtin_nontemporal_load(row + ((int64_t)(k + u) << 6));
-#pragma unroll
-            for (int u = 0; u < NT; ++u) {
-                const auto *sr = rec2<T>(rec, j[u]);
-                s0[u] = sr[0];
-                s1[u] = sr[1];
-                s2[u] = sr[2];
-                s3[u] = sr[3];
-                s4[u] = sr[4];
-                hb[u] = UH ? D.ha : rec[(int64_t)j[u] * SPH_REC + SPH_R_H];
+        momentum_stage_one<UH, T, NT>(dim, t, perm, counts, slice_off, indices, rec, alpha, beta,
+                                      eta, st, sim_of, dt_sim, scale, save);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// one evaluation and its stage as one persistent launch
+// ---------------------------------------------------------------------------
+// The grid is at most the blocks the device holds at once (the occupancy of
+// this kernel times the compute units), so every block is resident: the
+// blocks walk their tiles' SPHRho + Tait, meet at a grid barrier, then walk
+// the same tiles' MomentumEquation + PositionStepping + stage.  The barrier
+// publishes the density results with device-scope release and reads them
+// after a device-scope acquire (what a kernel boundary does).  Its wait is
+// bounded (spin_limit polls of the arrival counter); past the bound -- only
+// possible if another kernel kept some blocks from starting -- status |=
+// FUSED_WAIT_STATUS, the blocks go on, and the step is reported void.
+#define FUSED_WAIT_STATUS 64
+
+__device__ __forceinline__ bool fused_grid_barrier(int32_t *arrived, int32_t nblocks,
+                                                   int64_t spin_limit)
+{
+    __shared__ int32_t s_ok;
+    __threadfence();  // release: this thread's results, device-wide
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __hip_atomic_fetch_add(arrived, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int64_t spins = 0;
+        int32_t ok = 1;
+        while (__hip_atomic_load(arrived, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < nblocks) {
+            if (++spins > spin_limit) {
+                ok = 0;
+                break;
             }
-#pragma unroll
-            for (int u = 0; u < NT; ++u)
-                mom_term<UH, T>(dim, D, alpha, beta, eta, s0[u], s1[u], s2[u], s3[u], s4[u],
-                                hb[u], dtf, nr0, nr1, nr2);
+            __builtin_amdgcn_s_sleep(2);
         }
-        for (; k < e; ++k) {
-            const int32_t j = __builtin_nontemporal_load(row + ((int64_t)k << 6));
-            const auto *sr = rec2<T>(rec, j);
-            const T hb = UH ? D.ha : rec[(int64_t)j * SPH_REC + SPH_R_H];
-            mom_term<UH, T>(dim, D, alpha, beta, eta, sr[0], sr[1], sr[2], sr[3], sr[4], hb, dtf,
-                            nr0, nr1, nr2);
-        }
-        const int32_t i = perm[t];
-        st.dt_fac[i] = dtf;
-        // the accelerations of u, v, w (integrating calc: output 0 + sum) and
-        // PositionStepping's a_x = u; then x <- x0 + a*dt per stepped property
-        const double acc[3] = {(double)(T(0.0) + nr0), (double)(T(0.0) + nr1),
-                               (double)(T(0.0) + nr2)};
-        const double dt = dt_sim ? scale * dt_sim[sim_of[i]] : scale;
-        // every component is read before the first store (memory operations
-        // retire in order: a load behind a store waits for it)
-        double uc[3] = {0.0, 0.0, 0.0}, xc[3] = {0.0, 0.0, 0.0};
-        double u0[3] = {0.0, 0.0, 0.0}, x0[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            if (c >= dim) break;
-            uc[c] = st.u[c][i];
-            xc[c] = st.x[c][i];
-            u0[c] = save ? uc[c] : st.u0[c][i];
-            x0[c] = save ? xc[c] : st.x0[c][i];
-        }
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            if (c >= dim) break;
-            if (save) {
-                st.u0[c][i] = uc[c];
-                st.x0[c][i] = xc[c];
-            }
-            st.au[c][i] = acc[c];
-            st.ax[c][i] = uc[c];
-            const double tu = acc[c] * dt, tx = uc[c] * dt;
-            st.u[c][i] = u0[c] + tu;
-            st.x[c][i] = x0[c] + tx;
-        }
+        s_ok = ok;
+    }
+    __syncthreads();
+    __threadfence();  // acquire: the other blocks' results, fresh
+    return s_ok != 0;
+}
+
+template <bool UH, typename T, int NT>
+__global__ void __launch_bounds__(FUSED_MAX_THREADS)
+    k_batch_fused_eval(int dim, int64_t n, int64_t ntiles, const int32_t *__restrict__ perm,
+                       const int32_t *__restrict__ counts, const int64_t *__restrict__ slice_off,
+                       const int32_t *__restrict__ indices, T *__restrict__ rec, T ro, T co,
+                       T gamma, T B, double *__restrict__ rho, double *__restrict__ p,
+                       double *__restrict__ cs, T alpha, T beta, T eta, FusedState st,
+                       const int32_t *__restrict__ sim_of, const double *__restrict__ dt_sim,
+                       double scale, int save, int32_t *__restrict__ arrived,
+                       int32_t *__restrict__ status, int64_t spin_limit)
+{
+    int64_t tb, te;
+    fused_tiles(ntiles, tb, te);
+    for (int64_t tile = tb; tile < te; ++tile) {
+        const int64_t t = tile * blockDim.x + threadIdx.x;
+        if (t < n)
+            density_eos_one<UH, T, 2 * NT>(dim, t, perm, counts, slice_off, indices, rec, ro, co,
+                                           gamma, B, rho, p, cs);
+    }
+    if (!fused_grid_barrier(arrived, (int32_t)gridDim.x, spin_limit) && threadIdx.x == 0)
+        atomicOr(status, FUSED_WAIT_STATUS);
+    for (int64_t tile = tb; tile < te; ++tile) {
+        const int64_t t = tile * blockDim.x + threadIdx.x;
+        if (t < n)
+            momentum_stage_one<UH, T, NT>(dim, t, perm, counts, slice_off, indices, rec, alpha,
+                                          beta, eta, st, sim_of, dt_sim, scale, save);
     }
 }
 
@@ -649,6 +749,8 @@ struct BatchWork {
     // reverse mode, sorted order: cotangent of the accelerations, adjoints
     // of x and u (3 each), of rho, and the per-particle alpha terms
     double *abar, *xbar, *ubar, *rhobar, *alphabar;
+    // arrival counter of the persistent evaluation's grid barrier
+    int32_t *arrived;
     size_t bytes;
 };
 
@@ -683,6 +785,7 @@ BatchWork carve(const sph_batch_desc *d, char *base)
     w.ubar = (double *)take(sizeof(double) * 3 * (size_t)n);
     w.rhobar = (double *)take(sizeof(double) * (size_t)n);
     w.alphabar = (double *)take(sizeof(double) * (size_t)n);
+    w.arrived = (int32_t *)take(sizeof(int32_t) * 64);
     w.bytes = o;
     return w;
 }
@@ -707,6 +810,7 @@ int check_desc(sph_ctx *ctx, const sph_batch_desc *d)
                 "batch: nbr_tile must be 1, 2 or 4 (or 0)");
     SPH_REQUIRE(ctx, d->cells_per_block >= 0 && d->cells_per_block <= 64,
                 "batch: cells_per_block must be 0..64");
+    SPH_REQUIRE(ctx, d->persistent == 0 || d->persistent == 1, "batch: persistent must be 0 or 1");
     SPH_REQUIRE(ctx, d->sim_off && d->sim_of && d->x && d->y && d->z && d->u && d->v && d->w &&
                          d->h && d->m && d->rho && d->p && d->cs && d->x0 && d->y0 && d->u0 &&
                          d->v0 && d->ax && d->ay && d->au && d->av && d->dt_fac && d->status &&
@@ -847,6 +951,17 @@ int batch_stage(sph_ctx *ctx, const sph_batch_desc *d, double dt)
     return SPH_OK;
 }
 
+// blocks of k_batch_fused_eval the device holds at once at this block size
+template <bool UH, typename T, int NT>
+static int64_t fused_resident_blocks(sph_ctx *ctx, unsigned threads)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_fused_eval<UH, T, NT>,
+                                                     (int)threads, 0) != hipSuccess)
+        return 0;
+    return (int64_t)per_cu * sph_cu_count(ctx);
+}
+
 template <typename T, int NT>
 void launch_fused(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w, unsigned grid,
                   unsigned threads, int64_t ntiles, double scale, int save)
@@ -854,11 +969,6 @@ void launch_fused(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w, uns
     const int64_t n = d->n;
     T *rec = (T *)w.rec;
     const T B = T(d->co) * T(d->co) * T(d->ro) / T(d->gamma);  // eos_funcs.pyx:90
-    auto dk = d->uniform_h ? k_batch_density_eos<true, T, 2 * NT>
-                           : k_batch_density_eos<false, T, 2 * NT>;
-    hipLaunchKernelGGL(dk, dim3(grid), dim3(threads), 0, ctx->stream, d->dim, n, ntiles, w.perm,
-                       w.counts, w.slice_off, w.indices, rec, T(d->ro), T(d->co), T(d->gamma), B,
-                       d->rho, d->p, d->cs);
     FusedState st;
     double *x[3] = {d->x, d->y, d->z}, *u[3] = {d->u, d->v, d->w};
     double *x0[3] = {d->x0, d->y0, d->z0}, *u0[3] = {d->u0, d->v0, d->w0};
@@ -872,6 +982,27 @@ void launch_fused(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w, uns
         st.au[c] = au[c];
     }
     st.dt_fac = d->dt_fac;
+    // the whole evaluation in one launch of at most the resident blocks (two
+    // launches when the occupancy query fails)
+    const int64_t cap = !d->persistent ? 0
+                        : d->uniform_h ? fused_resident_blocks<true, T, NT>(ctx, threads)
+                                       : fused_resident_blocks<false, T, NT>(ctx, threads);
+    if (cap > 0) {
+        const int64_t g = (int64_t)grid < cap ? (int64_t)grid : cap;
+        hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, w.arrived, 1, 0);
+        auto fk = d->uniform_h ? k_batch_fused_eval<true, T, NT> : k_batch_fused_eval<false, T, NT>;
+        hipLaunchKernelGGL(fk, dim3((unsigned)g), dim3(threads), 0, ctx->stream, d->dim, n,
+                           ntiles, w.perm, w.counts, w.slice_off, w.indices, rec, T(d->ro),
+                           T(d->co), T(d->gamma), B, d->rho, d->p, d->cs, T(d->alpha), T(d->beta),
+                           T(d->eta), st, d->sim_of, d->dt_sim, scale, save, w.arrived, d->status,
+                           (int64_t)1 << 26);
+        return;
+    }
+    auto dk = d->uniform_h ? k_batch_density_eos<true, T, 2 * NT>
+                           : k_batch_density_eos<false, T, 2 * NT>;
+    hipLaunchKernelGGL(dk, dim3(grid), dim3(threads), 0, ctx->stream, d->dim, n, ntiles, w.perm,
+                       w.counts, w.slice_off, w.indices, rec, T(d->ro), T(d->co), T(d->gamma), B,
+                       d->rho, d->p, d->cs);
     auto mk = d->uniform_h ? k_batch_momentum_stage<true, T, NT>
                            : k_batch_momentum_stage<false, T, NT>;
     hipLaunchKernelGGL(mk, dim3(grid), dim3(threads), 0, ctx->stream, d->dim, n, ntiles, w.perm,

@@ -1692,6 +1692,11 @@ extern "C" int sph_nnps_export(sph_ctx *ctx, int64_t nd_real, const int32_t *d_r
 struct VarhArgs {
     GridDev g;
     const double *drec, *srec;
+    // float records (single precision): the distance tests run in float and
+    // the destination's cell comes from its double coordinates ox, oy, oz
+    // (original order), the ones the binning used
+    const float *drec32, *srec32;
+    const double *ox, *oy, *oz;
     const int32_t *dperm, *sperm, *srank, *sstart;
     int64_t nd_total, nd_real, ns_real;
     double rs, rmax2;  // rmax2 = (max_h*rs)^2 bounds every source radius
@@ -1703,7 +1708,26 @@ struct VarhArgs {
 // block); a longer appended part is sorted in place in the row
 #define VARH_LDS 40
 
-template <bool FILL>
+template <typename T> __device__ __forceinline__ const T *varh_drec(const VarhArgs &a);
+template <> __device__ __forceinline__ const double *varh_drec<double>(const VarhArgs &a)
+{
+    return a.drec;
+}
+template <> __device__ __forceinline__ const float *varh_drec<float>(const VarhArgs &a)
+{
+    return a.drec32;
+}
+template <typename T> __device__ __forceinline__ const T *varh_srec(const VarhArgs &a);
+template <> __device__ __forceinline__ const double *varh_srec<double>(const VarhArgs &a)
+{
+    return a.srec;
+}
+template <> __device__ __forceinline__ const float *varh_srec<float>(const VarhArgs &a)
+{
+    return a.srec32;
+}
+
+template <bool FILL, typename T>
 __global__ void __launch_bounds__(NN_THREADS)
     k_varh_rows(VarhArgs a, int32_t *__restrict__ counts, const int64_t *__restrict__ slice_off,
                 int32_t *__restrict__ indices)
@@ -1718,14 +1742,23 @@ __global__ void __launch_bounds__(NN_THREADS)
         }
         return;
     }
-    const double *dr = a.drec + t * SPH_REC;
-    const double px = dr[SPH_R_X], py = dr[SPH_R_Y], pz = dr[SPH_R_Z];
-    const double radius = dr[SPH_R_H] * a.rs;
-    const double r2 = radius * radius;
+    const T *dr = varh_drec<T>(a) + t * SPH_REC;
+    const T *srec = varh_srec<T>(a);
+    const T px = dr[SPH_R_X], py = dr[SPH_R_Y], pz = dr[SPH_R_Z];
+    const T radius = dr[SPH_R_H] * T(a.rs);
+    const T r2 = radius * radius;
+    const T rmax2 = T(a.rmax2);
     const GridDev &g = a.g;
-    const int cx = min(max(cell_coord(px, g.cs) - g.cmin0, 0), g.d0 - 1);
-    const int cy = min(max(cell_coord(py, g.cs) - g.cmin1, 0), g.d1 - 1);
-    const int cz = min(max(cell_coord(pz, g.cs) - g.cmin2, 0), g.d2 - 1);
+    double qx = px, qy = py, qz = pz;
+    if (a.ox) {
+        const int32_t i0 = a.dperm[t];
+        qx = a.ox[i0];
+        qy = a.oy[i0];
+        qz = a.oz[i0];
+    }
+    const int cx = min(max(cell_coord(qx, g.cs) - g.cmin0, 0), g.d0 - 1);
+    const int cy = min(max(cell_coord(qy, g.cs) - g.cmin1, 0), g.d1 - 1);
+    const int cz = min(max(cell_coord(qz, g.cs) - g.cmin2, 0), g.d2 - 1);
     const int R = a.reach;
     int32_t nf = 0, ne = 0;
     int32_t *row = nullptr;
@@ -1745,22 +1778,21 @@ __global__ void __launch_bounds__(NN_THREADS)
                 const int32_t k = cell_key(g, ix, iy, iz);
                 const int32_t s1 = a.sstart[k + 1];
                 for (int32_t s = a.sstart[k]; s < s1; ++s) {
-                    const double *sr = a.srec + (int64_t)s * SPH_REC;
+                    const T *sr = srec + (int64_t)s * SPH_REC;
                     // cPoint_distance2(src, dst), base/point.pxd:35-37
-                    const double ex = sr[SPH_R_X] - px, ey = sr[SPH_R_Y] - py,
-                                 ez = sr[SPH_R_Z] - pz;
-                    const double d2 = ex * ex + ey * ey + ez * ez;
+                    const T ex = sr[SPH_R_X] - px, ey = sr[SPH_R_Y] - py, ez = sr[SPH_R_Z] - pz;
+                    const T d2 = ex * ex + ey * ey + ez * ez;
                     if (d2 < r2) {
                         // the fill repeats the count pass's tests, so its
                         // entries are the counted ones (the bounds only
                         // guard the row)
                         if (FILL && nf < fbase) row[(int64_t)nf << 6] = s;
                         ++nf;
-                    } else if (d2 < a.rmax2) {
+                    } else if (d2 < rmax2) {
                         // the source's own radius: (h_j*rs)^2 <= rmax2, so
                         // no pair that reaches d is skipped by the bound; the
                         // source's original index is read for the hits only
-                        const double rj = sr[SPH_R_H] * a.rs;
+                        const T rj = sr[SPH_R_H] * T(a.rs);
                         if (d2 < rj * rj) {
                             const int32_t j = a.sperm[s];
                             if (j < a.ns_real) {
@@ -1814,13 +1846,14 @@ __global__ void __launch_bounds__(NN_THREADS)
     }
 }
 
-extern "C" int sph_nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
-                                   const int32_t *d_perm, int64_t nd_total, int64_t nd_real,
-                                   const int32_t *s_cell_start, const double *s_rec,
-                                   const int32_t *s_perm, const int32_t *s_rank, int64_t ns,
-                                   int64_t ns_real, double radius_scale, double max_h,
-                                   int32_t *counts, int64_t *slice_off, int32_t *indices,
-                                   int64_t cap, int64_t *slots_out, int64_t *nnz_out)
+template <typename T>
+static int nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const T *d_rec, const double *d_x,
+                           const double *d_y, const double *d_z, const int32_t *d_perm,
+                           int64_t nd_total, int64_t nd_real, const int32_t *s_cell_start,
+                           const T *s_rec, const int32_t *s_perm, const int32_t *s_rank,
+                           int64_t ns, int64_t ns_real, double radius_scale, double max_h,
+                           int32_t *counts, int64_t *slice_off, int32_t *indices, int64_t cap,
+                           int64_t *slots_out, int64_t *nnz_out)
 {
     if (!ctx) return SPH_EINVAL;
     SPH_REQUIRE(ctx, grid && counts && slice_off && slots_out && nnz_out,
@@ -1833,6 +1866,9 @@ extern "C" int sph_nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const dou
                 "sph_nnps_build_varh: null source arrays");
     SPH_REQUIRE(ctx, nd_total == 0 || (d_rec && d_perm),
                 "sph_nnps_build_varh: null destination arrays");
+    const bool f32 = sizeof(T) == sizeof(float);
+    SPH_REQUIRE(ctx, !f32 || nd_total == 0 || (d_x && d_y && d_z),
+                "sph_nnps_build_varh_f32: null destination coordinates");
     *slots_out = 0;
     *nnz_out = 0;
     const int64_t nslices = (nd_total + 63) / 64;
@@ -1853,9 +1889,18 @@ extern "C" int sph_nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const dou
     if (rc) return rc;
     char *sc = (char *)ctx->scratch;
     VarhArgs a;
+    memset(&a, 0, sizeof(a));
     a.g = make_griddev(ctx, grid);
-    a.drec = d_rec;
-    a.srec = s_rec;
+    if (f32) {
+        a.drec32 = (const float *)d_rec;
+        a.srec32 = (const float *)s_rec;
+        a.ox = d_x;
+        a.oy = d_y;
+        a.oz = d_z;
+    } else {
+        a.drec = (const double *)d_rec;
+        a.srec = (const double *)s_rec;
+    }
     a.dperm = d_perm;
     a.sperm = s_perm;
     a.srank = s_rank;
@@ -1864,13 +1909,16 @@ extern "C" int sph_nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const dou
     a.nd_real = nd_real;
     a.ns_real = ns_real;
     a.rs = radius_scale;
-    a.rmax2 = rmax * rmax;
+    // float: the bound rounded up (it only prunes; the source's own radius
+    // decides)
+    a.rmax2 = f32 ? rmax * rmax * (1.0 + 0x1p-18) : rmax * rmax;
     a.reach = (int)reach;
     a.fcount = (int32_t *)sc;
     int64_t *slice_sum = (int64_t *)(sc + fc_bytes);
     int64_t *nnz_dev = slice_sum + nslices;
     const unsigned grid_blocks = sph_blocks(nd_total, NN_THREADS);
-    hipLaunchKernelGGL(k_varh_rows<false>, dim3(grid_blocks), dim3(NN_THREADS), 0, ctx->stream,
+    auto count_kernel = k_varh_rows<false, T>;
+    hipLaunchKernelGGL(count_kernel, dim3(grid_blocks), dim3(NN_THREADS), 0, ctx->stream,
                        a, counts, (const int64_t *)nullptr, (int32_t *)nullptr);
     SPH_LAUNCH_CHECK(ctx, "k_varh_rows<count>");
     hipLaunchKernelGGL(k_slices, dim3(sph_blocks(nslices * 64, NN_THREADS)), dim3(NN_THREADS), 0,
@@ -1896,8 +1944,37 @@ extern "C" int sph_nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const dou
                       (long long)slots);
         return SPH_ENEEDCAP;
     }
-    hipLaunchKernelGGL(k_varh_rows<true>, dim3(grid_blocks), dim3(NN_THREADS), 0, ctx->stream,
+    auto fill_kernel = k_varh_rows<true, T>;
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_blocks), dim3(NN_THREADS), 0, ctx->stream,
                        a, counts, (const int64_t *)slice_off, indices);
     SPH_LAUNCH_CHECK(ctx, "k_varh_rows<fill>");
     return SPH_OK;
+}
+
+extern "C" int sph_nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
+                                   const int32_t *d_perm, int64_t nd_total, int64_t nd_real,
+                                   const int32_t *s_cell_start, const double *s_rec,
+                                   const int32_t *s_perm, const int32_t *s_rank, int64_t ns,
+                                   int64_t ns_real, double radius_scale, double max_h,
+                                   int32_t *counts, int64_t *slice_off, int32_t *indices,
+                                   int64_t cap, int64_t *slots_out, int64_t *nnz_out)
+{
+    return nnps_build_varh<double>(ctx, grid, d_rec, nullptr, nullptr, nullptr, d_perm, nd_total,
+                                   nd_real, s_cell_start, s_rec, s_perm, s_rank, ns, ns_real,
+                                   radius_scale, max_h, counts, slice_off, indices, cap,
+                                   slots_out, nnz_out);
+}
+
+extern "C" int sph_nnps_build_varh_f32(sph_ctx *ctx, const sph_grid *grid, const float *d_rec,
+                                       const double *d_x, const double *d_y, const double *d_z,
+                                       const int32_t *d_perm, int64_t nd_total, int64_t nd_real,
+                                       const int32_t *s_cell_start, const float *s_rec,
+                                       const int32_t *s_perm, const int32_t *s_rank, int64_t ns,
+                                       int64_t ns_real, double radius_scale, double max_h,
+                                       int32_t *counts, int64_t *slice_off, int32_t *indices,
+                                       int64_t cap, int64_t *slots_out, int64_t *nnz_out)
+{
+    return nnps_build_varh<float>(ctx, grid, d_rec, d_x, d_y, d_z, d_perm, nd_total, nd_real,
+                                  s_cell_start, s_rec, s_perm, s_rank, ns, ns_real, radius_scale,
+                                  max_h, counts, slice_off, indices, cap, slots_out, nnz_out);
 }

@@ -16,6 +16,7 @@ base/nnps.py) and the sorted-space lists of their locator; results land in
 the caller's particle order and, for shadowed properties, in the shadow.
 """
 import ctypes
+import os
 
 import torch
 
@@ -24,6 +25,26 @@ from ..base.nnps import PRHO2_FIELD, REC, REC_FIELDS
 from .sph_func import CSPHFunctionParticle, SPHFunction, SPHFunctionParticle
 
 _identity = {}
+
+# LDS-staged density and momentum kernels for grids in the linear cell order
+# (sph_*_staged); PYSPH_AMD_STAGED=0 keeps the gather kernels.  Both give the
+# same results bit for bit.
+STAGED = os.environ.get("PYSPH_AMD_STAGED", "1") != "0"
+
+
+def _staged(cm, funcs):
+    """Whether the staged kernels apply: FP64, the linear cell order, and
+    FixedDest lists (the variable-h rows append entries out of stencil order)."""
+    from ..base.nnps import VarHNbrParticleLocator
+    return (STAGED and cm.order == "linear" and cm.precision == "double" and
+            not any(isinstance(f.nbr_locator, VarHNbrParticleLocator) for f in funcs))
+
+
+def _stage_sources(src, k, f, cm):
+    """The staged kernels' extra per-source fields: cell starts and whether
+    the source is the destination array."""
+    src.s_cell_start[k] = _lib.ptr(cm.bins[f.source.name].cell_start)
+    src.same[k] = 1 if f.source is f.dest else 0
 
 
 def _identity_perm(n, device):
@@ -102,6 +123,18 @@ class SPHRho(CSPHFunctionParticle):
         read = ('x', 'y', 'z', 'h') + ((name,) if name in REC_FIELDS and accumulate else ())
         drec = cm.shadows(f0.dest, read)
         out_s = cm.shadow_out(f0.dest, name) if name else None
+        if _staged(cm, funcs):
+            for k, f in enumerate(funcs):
+                _stage_sources(src, k, f, cm)
+            with profiling.region("summation_density"):
+                ctx.check(ctx.lib.sph_summation_density_staged(
+                    ctx.handle, ctypes.byref(cm.grid), f0.nbr_locator.radius_scale, kernel.dim,
+                    f0.dest.get_number_of_particles(), f0.dest.num_real_particles,
+                    p(cm.bins[f0.dest.name].perm), ctypes.byref(src), p(drec),
+                    int(cm.uniform_h()), int(bool(accumulate)), p(output1), out_s, REC))
+            if name:
+                cm.written(f0.dest, name, out_s is not None)
+            return
         fn = (ctx.lib.sph_summation_density_multi_f32 if cm.precision == "single"
               else ctx.lib.sph_summation_density_multi)
         with profiling.region("summation_density"):
@@ -283,14 +316,25 @@ class MomentumEquation(SPHFunctionParticle):
             src.ns[k] = 0 if fr else cm.bins[f.source.name].n
             profiling.count_pairs(loc.pair_count())
         drec = cm.shadows(f0.dest, f0.DST)
-        fn = (ctx.lib.sph_momentum_av_multi_f32 if cm.precision == "single"
-              else ctx.lib.sph_momentum_av_multi)
-        with profiling.region("momentum_av"):
-            ctx.check(fn(ctx.handle, kernel.dim, f0.dest.get_number_of_particles(),
-                         f0.dest.num_real_particles, p(cm.bins[f0.dest.name].perm),
-                         ctypes.byref(src), p(drec), int(cm.uniform_h()), f0.alpha, f0.beta,
-                         f0.eta, int(bool(accumulate)), p(output1), p(output2), p(output3),
-                         p(f0.d('dt_fac'))))
+        if _staged(cm, funcs):
+            for k, f in enumerate(funcs):
+                _stage_sources(src, k, f, cm)
+            with profiling.region("momentum_av"):
+                ctx.check(ctx.lib.sph_momentum_av_staged(
+                    ctx.handle, ctypes.byref(cm.grid), f0.nbr_locator.radius_scale, kernel.dim,
+                    f0.dest.get_number_of_particles(), f0.dest.num_real_particles,
+                    p(cm.bins[f0.dest.name].perm), ctypes.byref(src), p(drec),
+                    int(cm.uniform_h()), f0.alpha, f0.beta, f0.eta, int(bool(accumulate)),
+                    p(output1), p(output2), p(output3), p(f0.d('dt_fac'))))
+        else:
+            fn = (ctx.lib.sph_momentum_av_multi_f32 if cm.precision == "single"
+                  else ctx.lib.sph_momentum_av_multi)
+            with profiling.region("momentum_av"):
+                ctx.check(fn(ctx.handle, kernel.dim, f0.dest.get_number_of_particles(),
+                             f0.dest.num_real_particles, p(cm.bins[f0.dest.name].perm),
+                             ctypes.byref(src), p(drec), int(cm.uniform_h()), f0.alpha,
+                             f0.beta, f0.eta, int(bool(accumulate)), p(output1), p(output2),
+                             p(output3), p(f0.d('dt_fac'))))
         for f, fr in zip(funcs, fresh):
             if not fr:
                 cm.prho2_written(f.source)

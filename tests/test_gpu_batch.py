@@ -43,12 +43,15 @@ def test_batch_matches_single_simulation_runs(gpu):
     assert b.pair_interactions() > 0
 
 
-def test_batch_graph_replays_match_eager(gpu):
+@pytest.mark.parametrize("persistent", [0, 1])
+def test_batch_graph_replays_match_eager(gpu, persistent):
+    """Graph replays (one persistent launch per evaluation, or two) against
+    eager steps of the other kind: bit-equal every replay."""
     from pysph_amd.batch import BatchedSimulation
     sims, _ = _small_drops(n=4, seed=9)
     dt = 1e-5
-    eager = BatchedSimulation(sims)
-    graph = BatchedSimulation(sims)
+    eager = BatchedSimulation(sims, tune={"persistent": 1 - persistent})
+    graph = BatchedSimulation(sims, tune={"persistent": persistent})
     eager.step(dt)
     eager.check()
     graph.capture(dt, nsteps=1)  # runs the first step eagerly, then records one step
@@ -60,7 +63,8 @@ def test_batch_graph_replays_match_eager(gpu):
         for k in KEYS:
             a, b = graph.get(k), eager.get(k)
             assert np.array_equal(a, b), (r, k, rel_err(a, b))
-    print("graph replays: 4, every replay bit-equal to the eager step")
+    print(f"graph replays (persistent={persistent}): 4, every replay bit-equal to the eager "
+          f"step")
     graph.release_graph()
 
 
@@ -142,7 +146,10 @@ def test_fused_step_matches_kernel_by_kernel(gpu, dim, precision):
     else:
         sims, dts, co = _cubes(), None, 20.0
     tunes = [{}, {"op_threads": 64, "op_blocks_per_cu": 4, "nbr_tile": 1, "cells_per_block": 3},
-             {"op_threads": 128, "op_blocks_per_cu": 1, "nbr_tile": 4, "cells_per_block": 8}]
+             {"op_threads": 128, "op_blocks_per_cu": 1, "nbr_tile": 4, "cells_per_block": 8},
+             {"persistent": 1},
+             {"persistent": 1, "op_threads": 64, "op_blocks_per_cu": 2, "nbr_tile": 1},
+             {"persistent": 1, "op_threads": 128, "nbr_tile": 4}]
     ref = BatchedSimulation(sims, dim=dim, co=co, dt_per_sim=dts, precision=precision)
     runs = [BatchedSimulation(sims, dim=dim, co=co, dt_per_sim=dts, precision=precision,
                               tune=t) for t in tunes]

@@ -70,3 +70,39 @@ def test_batched_drops_single_vs_double(gpu):
     print("FP32 batch vs FP64 after 5 RK2 steps (max error / max magnitude): " +
           " ".join(f"{k} {v:.2g}" for k, v in errs.items()))
     assert max(errs.values()) <= 1e-4
+
+
+def test_variable_h_lists_single(gpu):
+    """Symmetric variable-h lists (VarHNbrParticleLocator) over float records:
+    against the FP64 oracle's lists, differing pairs only where the distance
+    is within float rounding of one of the two radii 2 h_a, 2 h_b, and SPHRho
+    over them within 1e-5 of the FP64 density."""
+    from pysph_amd.base import api as base
+    from pysph_amd.sph import api as sph
+    from pysph_amd.solver.generators import jittered_lattice_3d
+    d = jittered_lattice_3d(n=24, seed=23, hdx=1.3)
+    rng = np.random.default_rng(2)
+    d["h"] = d["h"] * rng.uniform(0.8, 1.2, len(d["h"]))
+    pa = base.get_particle_array(name="fluid", cl_precision="single", **d)
+    particles = base.Particles([pa], variable_h=True)
+    func = sph.SPHRho.withargs().get_func(pa, pa)
+    calc = sph.SPHCalc(particles, [pa], pa, base.CubicSplineKernel(dim=3), [func], ['rho'])
+    calc.sph('rho')
+    off32, idx32 = (t.cpu().numpy() for t in func.nbr_locator.get_csr())
+    cs = particles.cell_manager.cell_size
+    off, idx = cpu_ref.nnps_varh(d, d, 2.0, cs)
+    rho = np.zeros(len(d["x"]))
+    cpu_ref.summation_density(3, d, d, off, idx, rho, nthreads=8)
+    a, b = _pairs(off32, idx32), _pairs(off, idx)
+    diff = a ^ b
+    worst_edge = 0.0
+    for i, j in diff:
+        r = np.sqrt((d["x"][i] - d["x"][j]) ** 2 + (d["y"][i] - d["y"][j]) ** 2
+                    + (d["z"][i] - d["z"][j]) ** 2)
+        worst_edge = max(worst_edge, min(abs(r / (2 * d["h"][i]) - 1), abs(r / (2 * d["h"][j]) - 1)))
+    e = rel_err(pa.get('rho'), rho)
+    print(f"FP32 variable-h lists 24^3, h x U(0.8, 1.2): {len(diff)} of {len(b)} pairs differ "
+          f"from FP64 (all within {worst_edge:.2g} relative of a radius); density rel err {e:.3g}")
+    assert len(diff) <= 1e-4 * len(b)
+    assert worst_edge <= 1e-5
+    assert e <= 1e-5

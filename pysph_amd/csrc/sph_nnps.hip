@@ -974,6 +974,18 @@ __device__ void nnps_cell(const CellArgs &a, int64_t w, int32_t *__restrict__ co
     const int z0 = max(cz - dist, 0), z1 = min(cz + dist, g.d2 - 1);
     const int nx = x1 - x0 + 1, ny = y1 - y0 + 1, nz = z1 - z0 + 1;
     const int nst = nx * ny * nz;
+    if (!FILL && !a.same) {
+        // no source in the stencil: every row of the cell stays empty (counts
+        // start at 0) -- most fluid cells against a wall array.  (A cell's own
+        // array always has its members in the stencil.)
+        int32_t found = 0;
+        for (int sIdx = lane; sIdx < nst; sIdx += CW) {
+            const int iz = sIdx % nz, iy = (sIdx / nz) % ny, ix = sIdx / (nz * ny);
+            const int32_t k = cell_key(g, x0 + ix, y0 + iy, z0 + iz);
+            found |= a.sstart[k + 1] - a.sstart[k];
+        }
+        if (!__builtin_amdgcn_ballot_w64(found != 0)) return;
+    }
 
     int nchunks = 0;  // chunks per destination over all stencil batches
     for (int32_t g0 = ds; g0 < de; g0 += CW) {  // destination groups of <= 64

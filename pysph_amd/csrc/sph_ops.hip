@@ -317,24 +317,24 @@ int sph_tait_records_f32(sph_ctx *ctx, int64_t n, const int32_t *perm, float *re
 // MomentumEquation with Monaghan artificial viscosity (mom_term in sph_terms.h)
 // ---------------------------------------------------------------------------
 // one SELL-64 row of MomentumEquation, summed from 0 in list order; dtf is
-// max-reduced over the row
-template <bool UH, typename T>
+// max-reduced over the row.  NT neighbours' records in flight.
+template <bool UH, typename T, int NT>
 __device__ __forceinline__ void mom_row(int dim, const MomDest<T> &D, T alpha, T beta, T eta,
                                         const int32_t *__restrict__ row, int32_t e,
                                         const T *__restrict__ srec, T &dtf, T &nr0, T &nr1,
                                         T &nr2)
 {
     int32_t k = 0;
-    // MOM_NT neighbours' records in flight, then their terms added in list order
-    for (; k + MOM_NT <= e; k += MOM_NT) {
-        int32_t j[MOM_NT];
-        typename Pair<T>::type s0[MOM_NT], s1[MOM_NT], s2[MOM_NT], s3[MOM_NT], s4[MOM_NT];
-        T hb[MOM_NT];
+    // NT neighbours' records in flight, then their terms added in list order
+    for (; k + NT <= e; k += NT) {
+        int32_t j[NT];
+        typename Pair<T>::type s0[NT], s1[NT], s2[NT], s3[NT], s4[NT];
+        T hb[NT];
 #pragma unroll
-        for (int u = 0; u < MOM_NT; ++u)
+        for (int u = 0; u < NT; ++u)
             j[u] = __builtin_nontemporal_load(row + ((int64_t)(k + u) << 6));
 #pragma unroll
-        for (int u = 0; u < MOM_NT; ++u) {
+        for (int u = 0; u < NT; ++u) {
             const auto *sr = rec2(srec, j[u]);
             s0[u] = sr[0];
             s1[u] = sr[1];
@@ -344,7 +344,7 @@ __device__ __forceinline__ void mom_row(int dim, const MomDest<T> &D, T alpha, T
             hb[u] = UH ? D.ha : srec[(int64_t)j[u] * SPH_REC + SPH_R_H];
         }
 #pragma unroll
-        for (int u = 0; u < MOM_NT; ++u)
+        for (int u = 0; u < NT; ++u)
             mom_term<UH, T>(dim, D, alpha, beta, eta, s0[u], s1[u], s2[u], s3[u], s4[u], hb[u],
                             dtf, nr0, nr1, nr2);
     }
@@ -408,7 +408,10 @@ __global__ void __launch_bounds__(OP_THREADS) __attribute__((amdgpu_waves_per_eu
         for (int s = 0; s < L.nsrc; ++s) {
             T nr0 = T(0.0), nr1 = T(0.0), nr2 = T(0.0);
             const int32_t *row = L.indices[s] + L.slice_off[s][t >> 6] + (t & 63);
-            mom_row<UH, T>(dim, D, alpha, beta, eta, row, e[s], L.srec[s], dtf, nr0, nr1, nr2);
+            // with h read per pair (!UH) four records in flight would spill
+            // registers at 3 waves per SIMD: two
+            mom_row<UH, T, (UH || sizeof(T) == sizeof(float)) ? MOM_NT : 2>(
+                dim, D, alpha, beta, eta, row, e[s], L.srec[s], dtf, nr0, nr1, nr2);
             o0 = o0 + nr0;
             o1 = o1 + nr1;
             o2 = o2 + nr2;

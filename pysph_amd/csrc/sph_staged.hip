@@ -80,7 +80,8 @@ __device__ __forceinline__ void stg_row(const GridDev &g, int cx, int cy, int cz
     int32_t k = 0;
     int32_t jb[STG_WIN + 1];
 #pragma unroll
-    for (int u = 0; u <= STG_WIN; ++u) jb[u] = u < e ? row[(int64_t)u << 6] : -1;
+    for (int u = 0; u <= STG_WIN; ++u)
+        jb[u] = u < e ? __builtin_nontemporal_load(row + ((int64_t)u << 6)) : -1;
     // a grid one cell deep in z (2D, 1D): the pencils run along y, one per dx
     const bool flat = g.d2 == 1;
     const int dy0 = flat ? 0 : -dmax, dy1 = flat ? 0 : dmax;
@@ -159,7 +160,9 @@ __device__ __forceinline__ void stg_row(const GridDev &g, int cx, int cy, int cz
                         ++k;
 #pragma unroll
                         for (int u = 0; u < STG_WIN; ++u) jb[u] = jb[u + 1];
-                        jb[STG_WIN] = k + STG_WIN < e ? row[(int64_t)(k + STG_WIN) << 6] : -1;
+                        jb[STG_WIN] = k + STG_WIN < e
+                                          ? __builtin_nontemporal_load(row + ((int64_t)(k + STG_WIN) << 6))
+                                          : -1;
                         run = k < e;
                     } else {
                         run = false;
@@ -178,7 +181,8 @@ __device__ __forceinline__ void stg_row(const GridDev &g, int cx, int cy, int cz
         ++k;
 #pragma unroll
         for (int u = 0; u < STG_WIN; ++u) jb[u] = jb[u + 1];
-        jb[STG_WIN] = k + STG_WIN < e ? row[(int64_t)(k + STG_WIN) << 6] : -1;
+        jb[STG_WIN] =
+            k + STG_WIN < e ? __builtin_nontemporal_load(row + ((int64_t)(k + STG_WIN) << 6)) : -1;
     }
 }
 

@@ -106,3 +106,41 @@ def test_variable_h_lists_single(gpu):
     assert len(diff) <= 1e-4 * len(b)
     assert worst_edge <= 1e-5
     assert e <= 1e-5
+
+
+@pytest.mark.timeout(600)
+def test_lattice_100_single(gpu):
+    """The FP32 variant at the configured size of the lattice workload (100^3,
+    jitter 0.2 dx, h = 1.2 dx): lists against the FP64 oracle (differing pairs
+    only within float rounding of 2h) and densities within 1e-5."""
+    from pysph_amd.base import api as base
+    from pysph_amd.sph import api as sph
+    from pysph_amd.solver.generators import jittered_lattice_3d
+    d = jittered_lattice_3d(n=100)
+    pa = base.get_particle_array(name="fluid", cl_precision="single", **d)
+    particles = base.Particles([pa])
+    func = sph.SPHRho.withargs().get_func(pa, pa)
+    calc = sph.SPHCalc(particles, [pa], pa, base.CubicSplineKernel(dim=3), [func], ['rho'])
+    calc.sph('rho')
+    off32, idx32 = (t.cpu().numpy() for t in func.nbr_locator.get_csr())
+    cs = particles.cell_manager.cell_size
+    off, idx = cpu_ref.nnps(d, d, 2.0, cs, True, nthreads=16)
+    rho = np.zeros(len(d["x"]))
+    cpu_ref.summation_density(3, d, d, off, idx, rho, nthreads=16)
+    n = len(d["x"])
+
+    def codes(o, ix):
+        rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(o))
+        return rows * n + ix.astype(np.int64)
+
+    diff = np.setxor1d(codes(off32, idx32), codes(off, idx))
+    i, j = diff // n, diff % n
+    r = np.sqrt((d["x"][i] - d["x"][j]) ** 2 + (d["y"][i] - d["y"][j]) ** 2
+                + (d["z"][i] - d["z"][j]) ** 2)
+    worst_edge = float(np.max(np.abs(r / (2 * d["h"][i]) - 1))) if len(diff) else 0.0
+    e = rel_err(pa.get('rho'), rho)
+    print(f"FP32 lattice 100^3: {len(diff)} of {len(idx)} pairs differ from FP64 (all within "
+          f"{worst_edge:.2g} relative of 2h); density rel err {e:.3g}")
+    assert len(diff) <= 1e-4 * len(idx)
+    assert worst_edge <= 1e-5
+    assert e <= 1e-5

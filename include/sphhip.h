@@ -590,6 +590,10 @@ typedef struct sph_batch_desc {
                                * blocks the device holds at once (status bit 64 if
                                * the barrier's bounded wait runs out); 0: two
                                * launches.  Same results either way. */
+    int32_t staged;           /* 1: the two fused kernels with the neighbour records
+                               * staged in LDS pencil by pencil (one wavefront per
+                               * 64-row slice; op_threads, op_blocks_per_cu, nbr_tile
+                               * and persistent then unused).  Same results. */
 } sph_batch_desc;
 
 int sph_batch_workspace_size(sph_ctx *ctx, const sph_batch_desc *d, int64_t *bytes);

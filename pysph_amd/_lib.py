@@ -105,7 +105,8 @@ class BatchDesc(ctypes.Structure):
                    ("op_blocks_per_cu", ctypes.c_int32), ("nbr_tile", ctypes.c_int32),
                    ("cells_per_block", ctypes.c_int32), ("work", _vp),
                    ("work_bytes", _i64),
-                   ("status", _vp), ("pairs", _vp), ("persistent", ctypes.c_int32)])
+                   ("status", _vp), ("pairs", _vp), ("persistent", ctypes.c_int32),
+                   ("staged", ctypes.c_int32)])
 
 
 # name -> (restype, argtypes)

@@ -1,10 +1,10 @@
 """Autotuner for the launch shapes of the batched step.
 
-The fused step of ``BatchedSimulation`` has five launch parameters (threads
+The fused step of ``BatchedSimulation`` has six launch parameters (threads
 per block and persistent blocks per compute unit of the fused operator
 kernels, neighbour records in flight per lane, destination cells per block of
-the list count pass, and one persistent launch per evaluation with a grid
-barrier or two launches).  None of them changes a result -- the per-particle sums
+the list count pass, one persistent launch per evaluation with a grid
+barrier or two launches, and the neighbour records staged in LDS or gathered).  None of them changes a result -- the per-particle sums
 keep their order -- so the tuner may pick any of them by speed alone.
 
 ``autotune`` times real steps of the batch itself (the state is restored
@@ -27,10 +27,11 @@ SPACE = {
     "op_blocks_per_cu": (0, 4, 8, 16),
     "nbr_tile": (1, 2, 4),
     "persistent": (0, 1),
+    "staged": (0, 1),
 }
 
 DEFAULT = {"cells_per_block": 1, "op_threads": 256, "op_blocks_per_cu": 0, "nbr_tile": 2,
-           "persistent": 0}
+           "persistent": 0, "staged": 0}
 
 
 def cache_path():

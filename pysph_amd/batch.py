@@ -33,8 +33,10 @@ STATUS_BITS = {1: "neighbour-list slots exceed list_cap", 2: "cell keys exceed k
                64: "the persistent evaluation's grid barrier timed out"}
 
 # launch shapes of the fused step (sph_batch_desc); 0 = the library default
-# (persistent: 1 = one launch per evaluation with a grid barrier, 0 = two)
-TUNE_FIELDS = ("op_threads", "op_blocks_per_cu", "nbr_tile", "cells_per_block", "persistent")
+# (persistent: 1 = one launch per evaluation with a grid barrier, 0 = two;
+# staged: 1 = neighbour records staged in LDS)
+TUNE_FIELDS = ("op_threads", "op_blocks_per_cu", "nbr_tile", "cells_per_block", "persistent",
+               "staged")
 
 
 class BatchError(RuntimeError):

@@ -23,6 +23,7 @@
 #include <string.h>
 
 #include "sph_internal.h"
+#include "sph_staged.h"
 #include "sph_terms.h"
 
 #define BT_THREADS 256
@@ -265,6 +266,32 @@ __device__ __forceinline__ void fused_tiles(int64_t ntiles, int64_t &tb, int64_t
     te = (L + 1) * ntiles / G;
 }
 
+// the density sum nr of sorted particle t (original index i) as SPHRho's
+// output, then Tait and P/rho^2 of it, into the state and the record
+template <typename T>
+__device__ __forceinline__ void eos_epilogue(int64_t t, int32_t i, T nr, T *__restrict__ rec, T ro,
+                                             T co, T gamma, T B, double *__restrict__ rho,
+                                             double *__restrict__ p, double *__restrict__ cs)
+{
+    // SPHRho is non-integrating: the output starts from 0 (sph_calc.pyx:319-321)
+    const T rhoa = T(0.0) + nr;
+    // Tait (eos_funcs.pyx:118-129), as k_tait_rec
+    const T ratio = rhoa / ro;
+    const T gamma2 = T(0.5) * (gamma - T(1.0));
+    const T tmp = pow(ratio, gamma);
+    const T r0 = T(0.0) + (tmp - T(1.0)) * B;
+    const T r1 = T(0.0) + pow(ratio, gamma2) * co;
+    const T rhob2 = T(1.0) / (rhoa * rhoa);
+    rho[i] = rhoa;
+    p[i] = r0;
+    cs[i] = r1;
+    T *rw = rec + t * SPH_REC;
+    rw[SPH_R_RHO] = rhoa;
+    rw[SPH_R_P] = r0;
+    rw[SPH_R_CS] = r1;
+    rw[SPH_R_PRHO2] = r0 * rhob2;
+}
+
 // SPHRho, then Tait and P/rho^2, of sorted particle t
 template <bool UH, typename T, int NT>
 __device__ __forceinline__ void density_eos_one(int dim, int64_t t,
@@ -309,24 +336,7 @@ __device__ __forceinline__ void density_eos_one(int dim, int64_t t,
         const T hb = UH ? ha : rec[(int64_t)j * SPH_REC + SPH_R_H];
         nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, sr[0], sr[1], hb);
     }
-    // SPHRho is non-integrating: the output starts from 0 (sph_calc.pyx:319-321)
-    const T rhoa = T(0.0) + nr;
-    // Tait (eos_funcs.pyx:118-129), as k_tait_rec
-    const T ratio = rhoa / ro;
-    const T gamma2 = T(0.5) * (gamma - T(1.0));
-    const T tmp = pow(ratio, gamma);
-    const T r0 = T(0.0) + (tmp - T(1.0)) * B;
-    const T r1 = T(0.0) + pow(ratio, gamma2) * co;
-    const T rhob2 = T(1.0) / (rhoa * rhoa);
-    const int32_t i = perm[t];
-    rho[i] = rhoa;
-    p[i] = r0;
-    cs[i] = r1;
-    T *rw = rec + t * SPH_REC;
-    rw[SPH_R_RHO] = rhoa;
-    rw[SPH_R_P] = r0;
-    rw[SPH_R_CS] = r1;
-    rw[SPH_R_PRHO2] = r0 * rhob2;
+    eos_epilogue<T>(t, perm[t], nr, rec, ro, co, gamma, B, rho, p, cs);
 }
 
 template <bool UH, typename T, int NT>
@@ -351,6 +361,47 @@ struct FusedState {
     double *x[3], *u[3], *x0[3], *u0[3], *ax[3], *au[3];
     double *dt_fac;
 };
+
+// dt_fac, the momentum sums as the accelerations of u, v, w (integrating
+// calc: output 0 + sum), PositionStepping's a_x = u, then x <- x0 + a*dt per
+// stepped property (x0 <- x first with save), of original particle i
+template <typename T>
+__device__ __forceinline__ void stage_epilogue(int dim, int32_t i, T dtf, T nr0, T nr1, T nr2,
+                                               const FusedState &st,
+                                               const int32_t *__restrict__ sim_of,
+                                               const double *__restrict__ dt_sim, double scale,
+                                               int save)
+{
+    st.dt_fac[i] = dtf;
+    const double acc[3] = {(double)(T(0.0) + nr0), (double)(T(0.0) + nr1),
+                           (double)(T(0.0) + nr2)};
+    const double dt = dt_sim ? scale * dt_sim[sim_of[i]] : scale;
+    // every component is read before the first store (memory operations
+    // retire in order: a load behind a store waits for it)
+    double uc[3] = {0.0, 0.0, 0.0}, xc[3] = {0.0, 0.0, 0.0};
+    double u0[3] = {0.0, 0.0, 0.0}, x0[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (c >= dim) break;
+        uc[c] = st.u[c][i];
+        xc[c] = st.x[c][i];
+        u0[c] = save ? uc[c] : st.u0[c][i];
+        x0[c] = save ? xc[c] : st.x0[c][i];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if (c >= dim) break;
+        if (save) {
+            st.u0[c][i] = uc[c];
+            st.x0[c][i] = xc[c];
+        }
+        st.au[c][i] = acc[c];
+        st.ax[c][i] = uc[c];
+        const double tu = acc[c] * dt, tx = uc[c] * dt;
+        st.u[c][i] = u0[c] + tu;
+        st.x[c][i] = x0[c] + tx;
+    }
+}
 
 // MomentumEquation + AV, then PositionStepping and the stage update, of
 // sorted particle t
@@ -415,38 +466,7 @@ __device__ __forceinline__ void momentum_stage_one(int dim, int64_t t,
         mom_term<UH, T>(dim, D, alpha, beta, eta, sr[0], sr[1], sr[2], sr[3], sr[4], hb, dtf,
                         nr0, nr1, nr2);
     }
-    const int32_t i = perm[t];
-    st.dt_fac[i] = dtf;
-    // the accelerations of u, v, w (integrating calc: output 0 + sum) and
-    // PositionStepping's a_x = u; then x <- x0 + a*dt per stepped property
-    const double acc[3] = {(double)(T(0.0) + nr0), (double)(T(0.0) + nr1),
-                           (double)(T(0.0) + nr2)};
-    const double dt = dt_sim ? scale * dt_sim[sim_of[i]] : scale;
-    // every component is read before the first store (memory operations
-    // retire in order: a load behind a store waits for it)
-    double uc[3] = {0.0, 0.0, 0.0}, xc[3] = {0.0, 0.0, 0.0};
-    double u0[3] = {0.0, 0.0, 0.0}, x0[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        if (c >= dim) break;
-        uc[c] = st.u[c][i];
-        xc[c] = st.x[c][i];
-        u0[c] = save ? uc[c] : st.u0[c][i];
-        x0[c] = save ? xc[c] : st.x0[c][i];
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        if (c >= dim) break;
-        if (save) {
-            st.u0[c][i] = uc[c];
-            st.x0[c][i] = xc[c];
-        }
-        st.au[c][i] = acc[c];
-        st.ax[c][i] = uc[c];
-        const double tu = acc[c] * dt, tx = uc[c] * dt;
-        st.u[c][i] = u0[c] + tu;
-        st.x[c][i] = x0[c] + tx;
-    }
+    stage_epilogue<T>(dim, perm[t], dtf, nr0, nr1, nr2, st, sim_of, dt_sim, scale, save);
 }
 
 template <bool UH, typename T, int NT>
@@ -534,6 +554,157 @@ __global__ void __launch_bounds__(FUSED_MAX_THREADS)
             momentum_stage_one<UH, T, NT>(dim, t, perm, counts, slice_off, indices, rec, alpha,
                                           beta, eta, st, sim_of, dt_sim, scale, save);
     }
+}
+
+// ---------------------------------------------------------------------------
+// the two fused kernels with the neighbour records staged in LDS
+// ---------------------------------------------------------------------------
+// The per-simulation grids key their cells linearly (key_base + (cx*d1 +
+// cy)*d2 + cz) and the simulations follow each other in key order, so the
+// pencil walk of the staged plain operators (sph_staged.h) applies: one
+// wavefront per 64-row slice, each lane with its own simulation's grid.  The
+// sums, and the epilogues, are those of k_batch_density_eos and
+// k_batch_momentum_stage.
+#ifndef BATCH_STG_DENS_CAP
+#define BATCH_STG_DENS_CAP 256
+#endif
+#ifndef BATCH_STG_MOM_CAP
+#define BATCH_STG_MOM_CAP 192
+#endif
+
+// the lane's grid, cell and reach (records of T; ox, oy, oz: the double
+// coordinates the binning used, for float records)
+template <typename T>
+__device__ __forceinline__ void batch_stage_cell(const GridDev &g, int32_t i, T xa, T ya, T za,
+                                                 T ha, const double *__restrict__ ox,
+                                                 const double *__restrict__ oy,
+                                                 const double *__restrict__ oz, double rs,
+                                                 int &cx, int &cy, int &cz, int &dl)
+{
+    if (ox) stg_cell(g, ox[i], oy[i], oz[i], cx, cy, cz);
+    else stg_cell(g, (double)xa, (double)ya, (double)za, cx, cy, cz);
+    dl = stg_reach<T>(ha, rs, g.cs);
+}
+
+template <bool UH, typename T>
+__global__ void __launch_bounds__(STG_THREADS)
+    k_batch_density_eos_staged(int dim, int64_t n, const GridDev *__restrict__ grids,
+                               const int32_t *__restrict__ sim_of,
+                               const int32_t *__restrict__ perm,
+                               const int32_t *__restrict__ cell_start,
+                               const int32_t *__restrict__ counts,
+                               const int64_t *__restrict__ slice_off,
+                               const int32_t *__restrict__ indices, T *__restrict__ rec,
+                               const double *__restrict__ ox, const double *__restrict__ oy,
+                               const double *__restrict__ oz, double rs, T ro, T co, T gamma,
+                               T B, double *__restrict__ rho, double *__restrict__ p,
+                               double *__restrict__ cs)
+{
+    using P = typename Pair<T>::type;
+    constexpr int NP = UH ? 2 : 3;  // (x, y) (z, m) [(h, -)]
+    __shared__ P lds[BATCH_STG_DENS_CAP * NP];
+    const int lane = threadIdx.x;
+    const int64_t slice = sph_xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t t = slice * STG_THREADS + lane;
+    const bool act = t < n;
+    const int32_t i = act ? perm[t] : 0;
+    const int32_t e = act ? counts[t] : 0;
+    GridDev g = {};
+    T xa = T(0), ya = T(0), za = T(0), ha = T(0), facA = T(0);
+    int cx = 0, cy = 0, cz = 0, dl = 0;
+    if (act) {
+        const auto *dr = rec2<T>(rec, t);
+        const auto dxy = dr[0], dzm = dr[1];
+        xa = dxy.x;
+        ya = dxy.y;
+        za = dzm.x;
+        ha = rec[t * SPH_REC + SPH_R_H];
+        facA = sph_h_dim(ha, dim) * sph_cubic_fac1<T>(dim);
+        if (e > 0) {
+            g = grids[sim_of[i]];
+            batch_stage_cell<T>(g, i, xa, ya, za, ha, ox, oy, oz, rs, cx, cy, cz, dl);
+        }
+    }
+    const int dmax = stg_wave_max(dl);
+    const int32_t *row = indices + slice_off[slice] + lane;
+    T nr = T(0.0);
+    auto stage = [&](P *dst, int32_t q) {
+        const P *sp = rec2<T>(rec, q);
+        dst[0] = sp[0];
+        dst[1] = sp[1];
+        if (!UH) dst[2] = P{rec[(int64_t)q * SPH_REC + SPH_R_H], T(0)};
+    };
+    auto term = [&](const P *pr) {
+        nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, pr[0], pr[1], UH ? ha : pr[NP - 1].x);
+    };
+    stg_row<T, NP, BATCH_STG_DENS_CAP>(g, cx, cy, cz, dmax, cell_start, row, e, (int32_t)t, true,
+                                       lds, stage, stage, term);
+    if (act) eos_epilogue<T>(t, i, nr, rec, ro, co, gamma, B, rho, p, cs);
+}
+
+template <bool UH, typename T>
+__global__ void __launch_bounds__(STG_THREADS)
+    k_batch_momentum_stage_staged(int dim, int64_t n, const GridDev *__restrict__ grids,
+                                  const int32_t *__restrict__ sim_of,
+                                  const int32_t *__restrict__ perm,
+                                  const int32_t *__restrict__ cell_start,
+                                  const int32_t *__restrict__ counts,
+                                  const int64_t *__restrict__ slice_off,
+                                  const int32_t *__restrict__ indices, const T *__restrict__ rec,
+                                  const double *__restrict__ ox, const double *__restrict__ oy,
+                                  const double *__restrict__ oz, double rs, T alpha, T beta,
+                                  T eta, FusedState st, const double *__restrict__ dt_sim,
+                                  double scale, int save)
+{
+    using P = typename Pair<T>::type;
+    constexpr int NP = UH ? 5 : 6;  // (x, y) (z, m) (u, v) (w, P/rho^2) (rho, cs) [(h, p)]
+    __shared__ P lds[BATCH_STG_MOM_CAP * NP];
+    const int lane = threadIdx.x;
+    const int64_t slice = sph_xcd_tile(blockIdx.x, gridDim.x);
+    const int64_t t = slice * STG_THREADS + lane;
+    const bool act = t < n;
+    const int32_t i = act ? perm[t] : 0;
+    const int32_t e = act ? counts[t] : 0;
+    GridDev g = {};
+    MomDest<T> D;
+    memset(&D, 0, sizeof(D));
+    int cx = 0, cy = 0, cz = 0, dl = 0;
+    if (act) {
+        const auto *dr = rec2<T>(rec, t);
+        const auto d0 = dr[0], d1 = dr[1], d2 = dr[2], d3 = dr[3], d4 = dr[4], d5 = dr[5];
+        D.xa = d0.x;
+        D.ya = d0.y;
+        D.za = d1.x;
+        D.ua = d2.x;
+        D.va = d2.y;
+        D.wa = d3.x;
+        D.rhoa = d4.x;
+        D.ca = d4.y;
+        D.ha = d5.x;
+        D.Pa = d5.y;
+        D.facA = sph_h_dim(D.ha, dim) * sph_cubic_fac1<T>(dim);
+        D.rhoa2 = T(1.0) / (D.rhoa * D.rhoa);
+        if (e > 0) {
+            g = grids[sim_of[i]];
+            batch_stage_cell<T>(g, i, D.xa, D.ya, D.za, D.ha, ox, oy, oz, rs, cx, cy, cz, dl);
+        }
+    }
+    const int dmax = stg_wave_max(dl);
+    const int32_t *row = indices + slice_off[slice] + lane;
+    T dtf = T(0.0);  // dt_fac is reset per calc (pressure_funcs.pyx:214)
+    T nr0 = T(0.0), nr1 = T(0.0), nr2 = T(0.0);
+    auto stage = [&](P *dst, int32_t q) {
+        const P *sp = rec2<T>(rec, q);
+#pragma unroll
+        for (int f = 0; f < NP; ++f) dst[f] = sp[f];
+    };
+    auto term = [&](const P *pr) {
+        mom_term<UH, T>(dim, D, alpha, beta, eta, pr[0], pr[1], pr[2], pr[3], pr[4],
+                        UH ? D.ha : pr[NP - 1].x, dtf, nr0, nr1, nr2);
+    };
+    stg_row<T, NP, BATCH_STG_MOM_CAP>(g, cx, cy, cz, dmax, cell_start, row, e, (int32_t)t, true,
+                                      lds, stage, stage, term);
+    if (act) stage_epilogue<T>(dim, i, dtf, nr0, nr1, nr2, st, sim_of, dt_sim, scale, save);
 }
 
 // ---------------------------------------------------------------------------
@@ -811,6 +982,7 @@ int check_desc(sph_ctx *ctx, const sph_batch_desc *d)
     SPH_REQUIRE(ctx, d->cells_per_block >= 0 && d->cells_per_block <= 64,
                 "batch: cells_per_block must be 0..64");
     SPH_REQUIRE(ctx, d->persistent == 0 || d->persistent == 1, "batch: persistent must be 0 or 1");
+    SPH_REQUIRE(ctx, d->staged == 0 || d->staged == 1, "batch: staged must be 0 or 1");
     SPH_REQUIRE(ctx, d->sim_off && d->sim_of && d->x && d->y && d->z && d->u && d->v && d->w &&
                          d->h && d->m && d->rho && d->p && d->cs && d->x0 && d->y0 && d->u0 &&
                          d->v0 && d->ax && d->ay && d->au && d->av && d->dt_fac && d->status &&
@@ -982,6 +1154,25 @@ void launch_fused(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w, uns
         st.au[c] = au[c];
     }
     st.dt_fac = d->dt_fac;
+    if (d->staged) {
+        // records staged in LDS: one wavefront per 64-row slice, two launches
+        const unsigned ns = (unsigned)((n + STG_THREADS - 1) / STG_THREADS);
+        const double *ox = d->f32 ? d->x : nullptr, *oy = d->f32 ? d->y : nullptr,
+                     *oz = d->f32 ? d->z : nullptr;
+        auto dks = d->uniform_h ? k_batch_density_eos_staged<true, T>
+                                : k_batch_density_eos_staged<false, T>;
+        hipLaunchKernelGGL(dks, dim3(ns), dim3(STG_THREADS), 0, ctx->stream, d->dim, n, w.grids,
+                           d->sim_of, w.perm, w.cell_start, w.counts, w.slice_off, w.indices,
+                           rec, ox, oy, oz, d->radius_scale, T(d->ro), T(d->co), T(d->gamma), B,
+                           d->rho, d->p, d->cs);
+        auto mks = d->uniform_h ? k_batch_momentum_stage_staged<true, T>
+                                : k_batch_momentum_stage_staged<false, T>;
+        hipLaunchKernelGGL(mks, dim3(ns), dim3(STG_THREADS), 0, ctx->stream, d->dim, n, w.grids,
+                           d->sim_of, w.perm, w.cell_start, w.counts, w.slice_off, w.indices,
+                           (const T *)rec, ox, oy, oz, d->radius_scale, T(d->alpha),
+                           T(d->beta), T(d->eta), st, d->dt_sim, scale, save);
+        return;
+    }
     // the whole evaluation in one launch of at most the resident blocks (two
     // launches when the occupancy query fails)
     const int64_t cap = !d->persistent ? 0

@@ -149,7 +149,8 @@ def test_fused_step_matches_kernel_by_kernel(gpu, dim, precision):
              {"op_threads": 128, "op_blocks_per_cu": 1, "nbr_tile": 4, "cells_per_block": 8},
              {"persistent": 1},
              {"persistent": 1, "op_threads": 64, "op_blocks_per_cu": 2, "nbr_tile": 1},
-             {"persistent": 1, "op_threads": 128, "nbr_tile": 4}]
+             {"persistent": 1, "op_threads": 128, "nbr_tile": 4},
+             {"staged": 1}, {"staged": 1, "cells_per_block": 4}]
     ref = BatchedSimulation(sims, dim=dim, co=co, dt_per_sim=dts, precision=precision)
     runs = [BatchedSimulation(sims, dim=dim, co=co, dt_per_sim=dts, precision=precision,
                               tune=t) for t in tunes]

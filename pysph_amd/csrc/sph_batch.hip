@@ -502,6 +502,10 @@ __global__ void __launch_bounds__(FUSED_MAX_THREADS)
 // possible if another kernel kept some blocks from starting -- status |=
 // FUSED_WAIT_STATUS, the blocks go on, and the step is reported void.
 #define FUSED_WAIT_STATUS 64
+// polls of the arrival counter before a block gives up (each poll is a
+// device-scope load plus a short sleep, about a microsecond: ~1 s, hundreds of
+// times the density phase of the largest batch)
+#define FUSED_SPIN_LIMIT ((int64_t)1 << 20)
 
 __device__ __forceinline__ bool fused_grid_barrier(int32_t *arrived, int32_t nblocks,
                                                    int64_t spin_limit)
@@ -1186,7 +1190,7 @@ void launch_fused(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w, uns
                            ntiles, w.perm, w.counts, w.slice_off, w.indices, rec, T(d->ro),
                            T(d->co), T(d->gamma), B, d->rho, d->p, d->cs, T(d->alpha), T(d->beta),
                            T(d->eta), st, d->sim_of, d->dt_sim, scale, save, w.arrived, d->status,
-                           (int64_t)1 << 26);
+                           FUSED_SPIN_LIMIT);
         return;
     }
     auto dk = d->uniform_h ? k_batch_density_eos<true, T, 2 * NT>

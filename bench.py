@@ -823,7 +823,7 @@ def cpu_baseline_dambreak(args, data):
     return {"value": ref.pairs / (t1 - t0), "unit": "pair-interactions/s", "cores": threads,
             "kind": "port",
             "sample": f"1 PredictorCorrector step of the same dam break ({ref.pairs} pair "
-                      f"interactions, {t1 - t0:.2f} s, {threads} OpenMP threads, gcc -O2 "
+                      f"interactions, {t1 - t0:.2f} s, {threads} OpenMP threads, gcc -O3 "
                       f"-ffp-contract=off)",
             "steps_per_s": 1.0 / (t1 - t0), **cpu_info(),
             "single_thread": {"value": ref1.pairs / (s1 - s0), "cores": 1,

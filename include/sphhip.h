@@ -242,14 +242,23 @@ int sph_nnps_build_f32_dev(sph_ctx *ctx, const sph_grid *grid, const float *d_re
  * (:742-786).  FP64 records with h (d_rec, s_rec); s_perm / s_rank: the
  * source's sorted->original and original->sorted permutations.  max_h: a
  * bound of every h of both arrays (stencil reach ceil(max_h*rs/cell) <= 256).
- * Reads the totals back (synchronises); returns SPH_ENEEDCAP with
- * *slots_out filled when cap is too small (nothing written). */
+ * The destination's binning as for sph_nnps_build: d_cell_start, the list
+ * d_cells of its non-empty cells, and their count nd_cells, or, with
+ * nd_cells_dev non-NULL, the count on the device and nd_cells a bound of it
+ * (a larger count is an error).  One wavefront per destination cell tests
+ * the cell's candidates (the stencil box of the largest reach) against its
+ * destinations; cells whose box exceeds 64 cells or 1024 candidates get
+ * one lane per destination.  Reads the totals back (synchronises); returns
+ * SPH_ENEEDCAP with *slots_out filled when cap is too small (nothing
+ * written). */
 int sph_nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
-                        const int32_t *d_perm, int64_t nd_total, int64_t nd_real,
-                        const int32_t *s_cell_start, const double *s_rec, const int32_t *s_perm,
-                        const int32_t *s_rank, int64_t ns, int64_t ns_real, double radius_scale,
-                        double max_h, int32_t *counts, int64_t *slice_off, int32_t *indices,
-                        int64_t cap, int64_t *slots_out, int64_t *nnz_out);
+                        const int32_t *d_perm, const int32_t *d_cell_start,
+                        const int32_t *d_cells, const int32_t *nd_cells_dev, int64_t nd_cells,
+                        int64_t nd_total, int64_t nd_real, const int32_t *s_cell_start,
+                        const double *s_rec, const int32_t *s_perm, const int32_t *s_rank,
+                        int64_t ns, int64_t ns_real, double radius_scale, double max_h,
+                        int32_t *counts, int64_t *slice_off, int32_t *indices, int64_t cap,
+                        int64_t *slots_out, int64_t *nnz_out);
 /* The same lists over float records (the single-precision variant,
  * cl_precision="single"): both distance tests in float; d_x, d_y, d_z are
  * the destination's double coordinates in original order (the ones the
@@ -258,7 +267,9 @@ int sph_nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
  * lists. */
 int sph_nnps_build_varh_f32(sph_ctx *ctx, const sph_grid *grid, const float *d_rec,
                             const double *d_x, const double *d_y, const double *d_z,
-                            const int32_t *d_perm, int64_t nd_total, int64_t nd_real,
+                            const int32_t *d_perm, const int32_t *d_cell_start,
+                            const int32_t *d_cells, const int32_t *nd_cells_dev,
+                            int64_t nd_cells, int64_t nd_total, int64_t nd_real,
                             const int32_t *s_cell_start, const float *s_rec,
                             const int32_t *s_perm, const int32_t *s_rank, int64_t ns,
                             int64_t ns_real, double radius_scale, double max_h, int32_t *counts,

@@ -55,6 +55,8 @@ def _load():
     lib.sphref_cubic_spline.restype = None
     lib.sphref_summation_density.argtypes = [i32, i64] + [vp] * 12 + [i32]
     lib.sphref_summation_density.restype = None
+    lib.sphref_pow.argtypes = [dbl, dbl]
+    lib.sphref_pow.restype = dbl
     lib.sphref_tait.argtypes = [i64, vp, dbl, dbl, dbl, vp, vp]
     lib.sphref_tait.restype = None
     lib.sphref_momentum.argtypes = [i32, i64] + [vp] * 23 + [dbl, dbl, dbl] + [vp] * 4 + [i32]
@@ -182,6 +184,13 @@ def summation_density(dim, dst, src, offsets, indices, rho_out, nthreads=1):
     s = [_f64(src[k]) for k in ("x", "y", "z", "h", "m")]
     lib.sphref_summation_density(int(dim), nd, *[_p(c) for c in d], *[_p(c) for c in s],
                                  _p(offsets), _p(indices), _p(rho_out), nthreads)
+
+
+def power(x, e):
+    """pow(x, e) of the Tait equation and the artificial pressure: correctly
+    rounded for an integral exponent 1..32 (2^-20 <= |x| <= 2^20), else
+    the C library's pow."""
+    return _load().sphref_pow(float(x), float(e))
 
 
 def tait(rho, ro, co, gamma, p_out, cs_out, n_real=None):

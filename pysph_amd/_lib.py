@@ -130,13 +130,14 @@ _SIGNATURES = {
                               _vp, _vp, _i64, _dbl, _i32, _vp, _vp, _vp, _i64,
                               ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "sph_nnps_export": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
-    "sph_nnps_build_varh": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _i64, _i64, _vp, _vp,
-                                   _vp, _vp, _i64, _i64, _dbl, _dbl, _vp, _vp, _vp, _i64,
-                                   ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "sph_nnps_build_varh": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _vp, _vp, _i64,
+                                   _i64, _i64, _vp, _vp, _vp, _vp, _i64, _i64, _dbl, _dbl, _vp,
+                                   _vp, _vp, _i64, ctypes.POINTER(_i64),
+                                   ctypes.POINTER(_i64)]),
     "sph_nnps_build_varh_f32": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _vp, _vp,
-                                       _i64, _i64, _vp, _vp, _vp, _vp, _i64, _i64, _dbl, _dbl,
-                                       _vp, _vp, _vp, _i64, ctypes.POINTER(_i64),
-                                       ctypes.POINTER(_i64)]),
+                                       _vp, _vp, _vp, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _i64,
+                                       _i64, _dbl, _dbl, _vp, _vp, _vp, _i64,
+                                       ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "sph_bin_array_dev": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _i64, _vp, _vp,
                                  _vp, _vp, _vp, _vp]),
     "sph_nnps_build_dev": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _vp, _vp, _i64,

@@ -607,17 +607,25 @@ class VarHNbrParticleLocator(FixedDestNbrParticleLocator):
         max_h = max(float(a.get_carray("h")[:a.get_number_of_particles()].max())
                     for a in (dst, src) if a.get_number_of_particles() > 0)
 
+        # the destination's non-empty cells: counted on the device after the
+        # first binning (the launch bound then bounds the count)
+        if bd.ncells_dev is not None:
+            ncells_dev, ncells = p(bd.ncells_dev), bd.cells_bound
+        else:
+            ncells_dev, ncells = None, bd.ncells_nonempty
+        cells = (p(bd.perm), p(bd.cell_start), p(bd.cells), ncells_dev, ncells)
+
         def call():
             if cm.precision == "single":  # float tests, the binning's double coordinates
                 return lib.sph_nnps_build_varh_f32(
                     ctx.handle, ctypes.byref(cm.grid), p(drec), p(dst.get_carray("x")),
-                    p(dst.get_carray("y")), p(dst.get_carray("z")), p(bd.perm), nd_total,
+                    p(dst.get_carray("y")), p(dst.get_carray("z")), *cells, nd_total,
                     dst.num_real_particles, p(bs.cell_start), p(srec), p(bs.perm), p(bs.rank),
                     bs.n, src.num_real_particles, self.radius_scale, max_h, p(self.counts),
                     p(self.slice_off), p(self.indices), self.indices.numel(),
                     ctypes.byref(slots), ctypes.byref(nnz))
             return lib.sph_nnps_build_varh(
-                ctx.handle, ctypes.byref(cm.grid), p(drec), p(bd.perm), nd_total,
+                ctx.handle, ctypes.byref(cm.grid), p(drec), *cells, nd_total,
                 dst.num_real_particles, p(bs.cell_start), p(srec), p(bs.perm), p(bs.rank), bs.n,
                 src.num_real_particles, self.radius_scale, max_h, p(self.counts),
                 p(self.slice_off),

@@ -278,9 +278,9 @@ __device__ __forceinline__ void eos_epilogue(int64_t t, int32_t i, T nr, T *__re
     // Tait (eos_funcs.pyx:118-129), as k_tait_rec
     const T ratio = rhoa / ro;
     const T gamma2 = T(0.5) * (gamma - T(1.0));
-    const T tmp = pow(ratio, gamma);
+    const T tmp = sph_pow(ratio, gamma);
     const T r0 = T(0.0) + (tmp - T(1.0)) * B;
-    const T r1 = T(0.0) + pow(ratio, gamma2) * co;
+    const T r1 = T(0.0) + sph_pow(ratio, gamma2) * co;
     const T rhob2 = T(1.0) / (rhoa * rhoa);
     rho[i] = rhoa;
     p[i] = r0;

@@ -161,6 +161,51 @@ size_t sph_scan_scratch_bytes(int64_t n, size_t elem);
 // ------------------------------------------------------------------------
 #define SPH_PI 3.141592653589793
 
+// ------------------------------------------------------------------------
+// pow(x, e) of the equation of state and the artificial pressure
+// (sph/funcs/eos_funcs.pyx:118-129, pressure_funcs.pyx:121-136).  For an
+// integral exponent 1 <= e <= 32 and 2^-20 <= |x| <= 2^20 the value is x^e
+// correctly rounded: the power is taken in double-double arithmetic (each
+// product split exactly with fma, Dekker/Knuth two-product and fast
+// two-sum), relative error below 2^-98 before the one final rounding, so the
+// result is the nearest double to the exact power unless that power lies
+// within 2^-98 of a rounding midpoint.  A correctly rounded pow is what the
+// C library of the reference's time computed (glibc's pow before 2.28 was
+// correctly rounded); device pow() and current glibc differ from it by an
+// ulp now and then.  Other exponents and ranges: pow().
+// ------------------------------------------------------------------------
+struct sph_dd {
+    double hi, lo;
+};
+
+__device__ __forceinline__ sph_dd sph_dd_mul(sph_dd a, sph_dd b)
+{
+    const double p = a.hi * b.hi;
+    const double e = fma(a.hi, b.hi, -p);  // a.hi * b.hi == p + e exactly
+    const double t = a.hi * b.lo + a.lo * b.hi;
+    const double s = e + t;
+    const double hi = p + s;
+    return {hi, s - (hi - p)};
+}
+
+__device__ __forceinline__ double sph_pow(double x, double e)
+{
+    const double ax = fabs(x);
+    if (!(e >= 1.0 && e <= 32.0 && e == floor(e) && ax >= 0x1p-20 && ax <= 0x1p20))
+        return pow(x, e);
+    const int n = (int)e;
+    const sph_dd xd = {x, 0.0};
+    sph_dd r = xd;
+    for (int b = 30 - __builtin_clz((unsigned)n); b >= 0; --b) {  // bits below the top one
+        r = sph_dd_mul(r, r);
+        if ((n >> b) & 1) r = sph_dd_mul(r, xd);
+    }
+    return r.hi;
+}
+
+// float records: the single-precision pow
+__device__ __forceinline__ float sph_pow(float x, float e) { return pow(x, e); }
+
 // Every helper is a template over the arithmetic type T (double: the
 // reference's FP64 path; float: the single-precision variant); the double
 // instantiation evaluates exactly the expressions above.

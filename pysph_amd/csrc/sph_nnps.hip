@@ -823,6 +823,15 @@ struct CellArgs {
     int64_t *totals;
     const int64_t *nnz_dev;
     int64_t cell_bound;
+    // symmetric variable-h lists: source permutation and real count (the
+    // appended part takes real sources only, by original index), the
+    // stencil reach of the largest radius, the own-query entries of every
+    // destination and the appended part's hit masks (same layout as masks)
+    const int32_t *sperm;
+    int64_t ns_real;
+    int reach;
+    int32_t *fcount;
+    uint64_t *amasks;
 };
 
 __device__ __forceinline__ int wave_max_i(int v)
@@ -1739,14 +1748,22 @@ template <> __device__ __forceinline__ const float *varh_srec<float>(const VarhA
     return a.srec32;
 }
 
+// dcell_of, cell_ovf (both or neither): only the destinations of the cells
+// the per-cell pass flagged (cell_ovf < 0) are handled; the other rows are
+// that pass's.
 template <bool FILL, typename T>
 __global__ void __launch_bounds__(NN_THREADS)
     k_varh_rows(VarhArgs a, int32_t *__restrict__ counts, const int64_t *__restrict__ slice_off,
-                int32_t *__restrict__ indices)
+                int32_t *__restrict__ indices, const int32_t *__restrict__ dcell_of,
+                const int32_t *__restrict__ cell_ovf)
 {
     __shared__ int32_t s_app[FILL ? VARH_LDS : 1][NN_THREADS];
     const int64_t t = (int64_t)sph_xcd_tile(blockIdx.x, gridDim.x) * NN_THREADS + threadIdx.x;
     if (t >= a.nd_total) return;
+    if (dcell_of) {
+        const int32_t w = dcell_of[t];
+        if (w < 0 || cell_ovf[w] >= 0) return;
+    }
     if (a.dperm[t] >= a.nd_real) {  // non-LocalReal destination: empty row
         if (!FILL) {
             counts[t] = 0;
@@ -1858,13 +1875,266 @@ __global__ void __launch_bounds__(NN_THREADS)
     }
 }
 
+// ---------------------------------------------------------------------------
+// symmetric variable-h lists, one wavefront per destination cell
+// ---------------------------------------------------------------------------
+// The count pass of the fixed-h lists with two hit masks per (destination,
+// 64-candidate chunk): own (d^2 < (h_d*rs)^2: the row's first part, in
+// stencil order) and appended (outside that radius, inside the source's own
+// radius h_j*rs, real sources only).  Candidates are the stencil box of the
+// largest reach around the cell, visited in the lexicographic order of the
+// box, which contains every destination's own stencil.  A cell whose box has
+// more than MAXSEG cells or more than maxch chunks is flagged (cell_ovf = -1)
+// and k_varh_rows builds its destinations' rows.  The distance and radius
+// expressions are k_varh_rows's, so both give the same rows.
+template <typename T>
+__device__ void varh_cell(const CellArgs &a, int64_t w, int32_t *__restrict__ counts,
+                          int32_t *seg_start, int32_t *seg_end, typename Quad<T>::type *dsts)
+{
+    const T *drec = drec_of<T>(a);
+    const T *srec = srec_of<T>(a);
+    const int lane = threadIdx.x;
+    const int32_t key = a.dcells[w];
+    const int32_t ds = a.dstart[key], de = a.dstart[key + 1];
+    const GridDev &g = a.g;
+    // the cell's coordinates, recomputed exactly as the binning did
+    double c0x, c0y, c0z;
+    if (a.ox) {
+        const int32_t i0 = a.dperm[ds];
+        c0x = a.ox[i0];
+        c0y = a.oy[i0];
+        c0z = a.oz[i0];
+    } else {
+        const T *r0 = drec + (int64_t)ds * SPH_REC;
+        c0x = r0[SPH_R_X];
+        c0y = r0[SPH_R_Y];
+        c0z = r0[SPH_R_Z];
+    }
+    const int cx = min(max(cell_coord(c0x, g.cs) - g.cmin0, 0), g.d0 - 1);
+    const int cy = min(max(cell_coord(c0y, g.cs) - g.cmin1, 0), g.d1 - 1);
+    const int cz = min(max(cell_coord(c0z, g.cs) - g.cmin2, 0), g.d2 - 1);
+    const int R = a.reach;
+    const int x0 = max(cx - R, 0), x1 = min(cx + R, g.d0 - 1);
+    const int y0 = max(cy - R, 0), y1 = min(cy + R, g.d1 - 1);
+    const int z0 = max(cz - R, 0), z1 = min(cz + R, g.d2 - 1);
+    const int nx = x1 - x0 + 1, ny = y1 - y0 + 1, nz = z1 - z0 + 1;
+    const int nst = nx * ny * nz;
+    int32_t st = 0, cnt = 0;
+    if (nst <= MAXSEG && lane < nst) {
+        const int iz = lane % nz, iy = (lane / nz) % ny, ix = lane / (nz * ny);
+        const int32_t k = cell_key(g, x0 + ix, y0 + iy, z0 + iz);
+        st = a.sstart[k];
+        cnt = a.sstart[k + 1] - st;
+    }
+    const int incl = wave_incl_scan_i(cnt);
+    const int total = __shfl(incl, CW - 1, 64);
+    const int nch = (total + CW - 1) / CW;
+    if (nst > MAXSEG || nch > a.maxch) {  // the lane-per-destination rows
+        for (int32_t t = ds + lane; t < de; t += CW) a.dcell_of[t] = (int32_t)w;
+        if (lane == 0) a.cell_ovf[w] = -1;
+        return;
+    }
+    __syncthreads();  // the previous cell's segments are no longer read
+    seg_start[lane] = st;  // MAXSEG == CW: one segment per lane
+    seg_end[lane] = incl;
+    __syncthreads();
+    for (int32_t g0 = ds; g0 < de; g0 += CW) {  // destination groups of <= 64
+        const int gn = min(CW, de - g0);
+        const int32_t my_t = g0 + lane;
+        bool mreal = false;
+        if (lane < gn) {
+            const T *dr = drec + (int64_t)my_t * SPH_REC;
+            const T radius = dr[SPH_R_H] * T(a.rs);
+            dsts[lane] = Quad<T>::make(dr[SPH_R_X], dr[SPH_R_Y], dr[SPH_R_Z], radius * radius);
+            mreal = a.dperm[my_t] < a.nd_real;
+        }
+        __syncthreads();
+        const uint64_t realmask = __ballot(mreal);
+        int32_t nf = 0, ne = 0;
+        for (int c = 0; c < nch; ++c) {
+            const int q = c * CW + lane;
+            const bool valid = q < total;
+            int32_t p = 0;
+            T sx = T(0.0), sy = T(0.0), sz = T(0.0), rj2 = T(0.0);
+            bool jreal = false;
+            if (valid) {
+                int lo = 0, hi = MAXSEG - 1;  // first segment with end > q
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (seg_end[mid] > q) hi = mid; else lo = mid + 1;
+                }
+                const int prev = lo > 0 ? seg_end[lo - 1] : 0;
+                p = seg_start[lo] + (q - prev);
+                if (g0 == ds) a.cand[w * (a.maxch * CW) + q] = p;
+                const T *sr = srec + (int64_t)p * SPH_REC;
+                sx = sr[SPH_R_X];
+                sy = sr[SPH_R_Y];
+                sz = sr[SPH_R_Z];
+                const T rj = sr[SPH_R_H] * T(a.rs);
+                rj2 = rj * rj;
+                jreal = a.sperm[p] < a.ns_real;
+            }
+            const uint64_t validm = __builtin_amdgcn_ballot_w64(valid);
+            uint32_t olo = 0, ohi = 0, alo = 0, ahi = 0;
+            for (int d = 0; d < gn; ++d) {
+                if (!((realmask >> d) & 1)) continue;  // non-LocalReal: empty row
+                const auto D = dsts[d];
+                // cPoint_distance2(src, dst), base/point.pxd:35-37
+                const T ex = sx - D.x, ey = sy - D.y, ez = sz - D.z;
+                const T d2 = ex * ex + ey * ey + ez * ez;
+                const bool own = d2 < D.w;
+                const uint64_t om = __builtin_amdgcn_ballot_w64(own) & validm;
+                const uint64_t am =
+                    __builtin_amdgcn_ballot_w64(!own && jreal && d2 < rj2) & validm;
+                wave_writelane2(olo, ohi, (uint32_t)om, (uint32_t)(om >> 32), d);
+                wave_writelane2(alo, ahi, (uint32_t)am, (uint32_t)(am >> 32), d);
+            }
+            nf += __popc(olo) + __popc(ohi);
+            ne += __popc(alo) + __popc(ahi);
+            if (lane < gn) {
+                a.masks[(int64_t)c * a.mstride + my_t] = ((uint64_t)ohi << 32) | olo;
+                a.amasks[(int64_t)c * a.mstride + my_t] = ((uint64_t)ahi << 32) | alo;
+            }
+        }
+        if (lane < gn) {
+            counts[my_t] = nf + ne;
+            a.fcount[my_t] = nf;
+            a.dcell_of[my_t] = (int32_t)w;
+        }
+        __syncthreads();  // the next group's destinations overwrite dsts
+    }
+    if (lane == 0) a.cell_ovf[w] = nch;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(CW) k_varh_cells(CellArgs a, int32_t *__restrict__ counts)
+{
+    __shared__ int32_t seg_start[MAXSEG];
+    __shared__ int32_t seg_end[MAXSEG];  // inclusive prefix of the segment sizes
+    __shared__ typename Quad<T>::type dsts[CW];
+    int64_t ncl = a.ncell_list;
+    if (a.ncell_dev && (int64_t)*a.ncell_dev < ncl) ncl = *a.ncell_dev;
+    // XCD x walks cells [x*ncl/8, (x+1)*ncl/8) with its blocks (k_nnps_cells)
+    const unsigned G = gridDim.x;
+    if (G < SPH_NXCD) {
+        for (int64_t w = blockIdx.x; w < ncl; w += G)
+            varh_cell<T>(a, w, counts, seg_start, seg_end, dsts);
+        return;
+    }
+    const unsigned x = blockIdx.x % SPH_NXCD, k = blockIdx.x / SPH_NXCD;
+    const unsigned per = (G - x + SPH_NXCD - 1) / SPH_NXCD;
+    const int64_t lo = ncl * x / SPH_NXCD, hi = ncl * (x + 1) / SPH_NXCD;
+    for (int64_t w = lo + k; w < hi; w += per)
+        varh_cell<T>(a, w, counts, seg_start, seg_end, dsts);
+}
+
+// rows start empty, with no cell (destinations of cells past the launch bound)
+__global__ void __launch_bounds__(NN_THREADS)
+    k_varh_reset(int64_t nd_total, int32_t *__restrict__ counts, int32_t *__restrict__ fcount,
+                 int32_t *__restrict__ dcell_of)
+{
+    const int64_t t = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
+    if (t >= nd_total) return;
+    counts[t] = 0;
+    fcount[t] = 0;
+    dcell_of[t] = -1;
+}
+
+// SELL-64 rows from the masks of k_varh_cells: one lane per destination.
+// The own part in mask order (the stencil order); the appended part
+// gathered as original indices, sorted ascending (in an LDS column of
+// VARH_LDS entries per lane, or in place in the row when longer) and
+// written back as sorted positions.
+__global__ void __launch_bounds__(NN_THREADS)
+    k_varh_fill(CellArgs a, int64_t nd_total, const int32_t *__restrict__ counts,
+                const int64_t *__restrict__ slice_off, int32_t *__restrict__ indices,
+                const int32_t *__restrict__ srank)
+{
+    __shared__ int32_t s_app[VARH_LDS][NN_THREADS];
+    const int64_t t = (int64_t)sph_xcd_tile(blockIdx.x, gridDim.x) * NN_THREADS + threadIdx.x;
+    if (t >= nd_total) return;
+    const int32_t n = counts[t];
+    if (n == 0) return;
+    const int32_t w = a.dcell_of[t];
+    if (w < 0) return;
+    const int32_t nch = a.cell_ovf[w];
+    if (nch < 0) return;  // k_varh_rows writes the flagged cells' rows
+    const int32_t nf = a.fcount[t];
+    const int32_t ne = n - nf;
+    int32_t *row = indices + slice_off[t >> 6] + (t & 63);
+    const int32_t *cand = a.cand + (int64_t)w * (a.maxch * CW);
+    int32_t k = 0;
+    for (int c = 0; c < nch; ++c) {
+        uint64_t m = a.masks[(int64_t)c * a.mstride + t];
+        while (m != 0) {
+            const int b = __ffsll((unsigned long long)m) - 1;
+            m &= m - 1;
+            if (k < nf) row[(int64_t)k << 6] = cand[(c << 6) + b];
+            ++k;
+        }
+    }
+    if (ne <= 0) return;
+    const bool in_lds = ne <= VARH_LDS;
+    int32_t e = 0;
+    for (int c = 0; c < nch; ++c) {
+        uint64_t m = a.amasks[(int64_t)c * a.mstride + t];
+        while (m != 0) {
+            const int b = __ffsll((unsigned long long)m) - 1;
+            m &= m - 1;
+            if (e < ne) {
+                const int32_t j = a.sperm[cand[(c << 6) + b]];  // original index
+                if (in_lds)
+                    s_app[e][threadIdx.x] = j;
+                else
+                    row[(int64_t)(nf + e) << 6] = j;
+            }
+            ++e;
+        }
+    }
+    const int32_t na = min(e, ne);
+    if (in_lds) {
+        for (int32_t i = 1; i < na; ++i) {
+            const int32_t v = s_app[i][threadIdx.x];
+            int32_t q = i - 1;
+            while (q >= 0 && s_app[q][threadIdx.x] > v) {
+                s_app[q + 1][threadIdx.x] = s_app[q][threadIdx.x];
+                --q;
+            }
+            s_app[q + 1][threadIdx.x] = v;
+        }
+        for (int32_t i = 0; i < na; ++i)
+            row[(int64_t)(nf + i) << 6] = srank[s_app[i][threadIdx.x]];
+        return;
+    }
+    for (int32_t i = 1; i < na; ++i) {
+        const int32_t v = row[(int64_t)(nf + i) << 6];
+        int32_t q = i - 1;
+        while (q >= 0 && row[(int64_t)(nf + q) << 6] > v) {
+            row[(int64_t)(nf + q + 1) << 6] = row[(int64_t)(nf + q) << 6];
+            --q;
+        }
+        row[(int64_t)(nf + q + 1) << 6] = v;
+    }
+    for (int32_t i = 0; i < na; ++i) {
+        int32_t *ep = row + ((int64_t)(nf + i) << 6);
+        *ep = srank[*ep];
+    }
+}
+
+// Count pass per destination cell (k_varh_cells; the destinations of flagged
+// cells by k_varh_rows), slice offsets and totals, the host check of the
+// capacity, then the fill.  Scratch: [own masks | appended masks | cell flags
+// | candidate sources | cell of destination | own-query entries | slice sums
+// | scan partials].
 template <typename T>
 static int nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const T *d_rec, const double *d_x,
                            const double *d_y, const double *d_z, const int32_t *d_perm,
-                           int64_t nd_total, int64_t nd_real, const int32_t *s_cell_start,
-                           const T *s_rec, const int32_t *s_perm, const int32_t *s_rank,
-                           int64_t ns, int64_t ns_real, double radius_scale, double max_h,
-                           int32_t *counts, int64_t *slice_off, int32_t *indices, int64_t cap,
+                           const int32_t *d_cell_start, const int32_t *d_cells,
+                           const int32_t *d_ncells_dev, int64_t nd_cells, int64_t nd_total,
+                           int64_t nd_real, const int32_t *s_cell_start, const T *s_rec,
+                           const int32_t *s_perm, const int32_t *s_rank, int64_t ns,
+                           int64_t ns_real, double radius_scale, double max_h, int32_t *counts,
+                           int64_t *slice_off, int32_t *indices, int64_t cap,
                            int64_t *slots_out, int64_t *nnz_out)
 {
     if (!ctx) return SPH_EINVAL;
@@ -1872,11 +2142,11 @@ static int nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const T *d_rec, c
                 "sph_nnps_build_varh: null argument");
     SPH_REQUIRE(ctx, nd_total >= 0 && nd_total < ((int64_t)1 << 31) && nd_real >= 0 &&
                          nd_real <= nd_total && ns >= 0 && ns < ((int64_t)1 << 31) &&
-                         ns_real >= 0 && ns_real <= ns,
-                "sph_nnps_build_varh: bad particle counts");
+                         ns_real >= 0 && ns_real <= ns && nd_cells >= 0 && nd_cells <= nd_total,
+                "sph_nnps_build_varh: bad particle or cell counts");
     SPH_REQUIRE(ctx, ns == 0 || (s_cell_start && s_rec && s_perm && s_rank),
                 "sph_nnps_build_varh: null source arrays");
-    SPH_REQUIRE(ctx, nd_total == 0 || (d_rec && d_perm),
+    SPH_REQUIRE(ctx, nd_total == 0 || (d_rec && d_perm && d_cell_start && d_cells),
                 "sph_nnps_build_varh: null destination arrays");
     const bool f32 = sizeof(T) == sizeof(float);
     SPH_REQUIRE(ctx, !f32 || nd_total == 0 || (d_x && d_y && d_z),
@@ -1884,7 +2154,8 @@ static int nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const T *d_rec, c
     *slots_out = 0;
     *nnz_out = 0;
     const int64_t nslices = (nd_total + 63) / 64;
-    if (nd_total == 0 || ns == 0 || nd_real == 0) return nnps_clear(ctx, nd_total, counts, slice_off);
+    if (nd_total == 0 || ns == 0 || nd_real == 0 || nd_cells == 0)
+        return nnps_clear(ctx, nd_total, counts, slice_off);
     SPH_REQUIRE(ctx, radius_scale >= 0.0 && grid->cell_size > 0.0 && max_h >= 0.0,
                 "sph_nnps_build_varh: bad radius scale, max_h or grid");
     const double rmax = max_h * radius_scale;
@@ -1894,25 +2165,63 @@ static int nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const T *d_rec, c
     int rc = sph_upload_key_tables(ctx, grid);
     if (rc) return rc;
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const int maxch = NN_MAXCH;
+    const size_t masks_bytes = al((size_t)maxch * nd_total * sizeof(uint64_t));
+    const size_t ovf_bytes = al((size_t)nd_cells * sizeof(int32_t));
+    const size_t cand_bytes = al((size_t)nd_cells * maxch * CW * sizeof(int32_t));
+    const size_t dcell_bytes = al((size_t)nd_total * sizeof(int32_t));
     const size_t fc_bytes = al((size_t)nd_total * sizeof(int32_t));
     const size_t sums_bytes = al((size_t)(nslices + 1) * sizeof(int64_t) + 256);
-    rc = sph_ensure_scratch(ctx, fc_bytes + sums_bytes +
+    const size_t sums_off = 2 * masks_bytes + ovf_bytes + cand_bytes + dcell_bytes + fc_bytes;
+    rc = sph_ensure_scratch(ctx, sums_off + sums_bytes +
                                      sph_scan_scratch_bytes(nslices, sizeof(int64_t)));
     if (rc) return rc;
     char *sc = (char *)ctx->scratch;
+    CellArgs c = cell_args_base();
+    c.g = make_griddev(ctx, grid);
+    c.f32 = f32;
+    if (f32) {
+        c.drec32 = (const float *)d_rec;
+        c.srec32 = (const float *)s_rec;
+        c.ox = d_x;
+        c.oy = d_y;
+        c.oz = d_z;
+    } else {
+        c.drec = (const double *)d_rec;
+        c.srec = (const double *)s_rec;
+    }
+    c.dperm = d_perm;
+    c.dstart = d_cell_start;
+    c.dcells = d_cells;
+    c.ncell_list = nd_cells;
+    c.ncell_dev = d_ncells_dev;
+    c.nd_real = nd_real;
+    c.sstart = s_cell_start;
+    c.rs = radius_scale;
+    c.sperm = s_perm;
+    c.ns_real = ns_real;
+    c.reach = (int)reach;
+    c.maxch = maxch;
+    c.masks = (uint64_t *)sc;
+    c.amasks = (uint64_t *)(sc + masks_bytes);
+    c.mstride = nd_total;
+    c.cell_ovf = (int32_t *)(sc + 2 * masks_bytes);
+    c.cand = (int32_t *)(sc + 2 * masks_bytes + ovf_bytes);
+    c.dcell_of = (int32_t *)(sc + 2 * masks_bytes + ovf_bytes + cand_bytes);
+    c.fcount = (int32_t *)(sc + 2 * masks_bytes + ovf_bytes + cand_bytes + dcell_bytes);
+    int64_t *slice_sum = (int64_t *)(sc + sums_off);
+    int64_t *nnz_dev = slice_sum + nslices;
+    // the rows of flagged cells: the lane-per-destination kernel
     VarhArgs a;
     memset(&a, 0, sizeof(a));
-    a.g = make_griddev(ctx, grid);
-    if (f32) {
-        a.drec32 = (const float *)d_rec;
-        a.srec32 = (const float *)s_rec;
-        a.ox = d_x;
-        a.oy = d_y;
-        a.oz = d_z;
-    } else {
-        a.drec = (const double *)d_rec;
-        a.srec = (const double *)s_rec;
-    }
+    a.g = c.g;
+    a.drec = c.drec;
+    a.srec = c.srec;
+    a.drec32 = c.drec32;
+    a.srec32 = c.srec32;
+    a.ox = c.ox;
+    a.oy = c.oy;
+    a.oz = c.oz;
     a.dperm = d_perm;
     a.sperm = s_perm;
     a.srank = s_rank;
@@ -1925,13 +2234,19 @@ static int nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const T *d_rec, c
     // decides)
     a.rmax2 = f32 ? rmax * rmax * (1.0 + 0x1p-18) : rmax * rmax;
     a.reach = (int)reach;
-    a.fcount = (int32_t *)sc;
-    int64_t *slice_sum = (int64_t *)(sc + fc_bytes);
-    int64_t *nnz_dev = slice_sum + nslices;
-    const unsigned grid_blocks = sph_blocks(nd_total, NN_THREADS);
-    auto count_kernel = k_varh_rows<false, T>;
-    hipLaunchKernelGGL(count_kernel, dim3(grid_blocks), dim3(NN_THREADS), 0, ctx->stream,
-                       a, counts, (const int64_t *)nullptr, (int32_t *)nullptr);
+    a.fcount = c.fcount;
+    const unsigned dgrid = sph_blocks(nd_total, NN_THREADS);
+    hipLaunchKernelGGL(k_varh_reset, dim3(dgrid), dim3(NN_THREADS), 0, ctx->stream, nd_total,
+                       counts, c.fcount, c.dcell_of);
+    SPH_LAUNCH_CHECK(ctx, "k_varh_reset");
+    const int64_t cgrid = nd_cells < NN_CELL_GRID ? nd_cells : NN_CELL_GRID;
+    hipLaunchKernelGGL(k_varh_cells<T>, dim3((unsigned)cgrid), dim3(CW), 0, ctx->stream, c,
+                       counts);
+    SPH_LAUNCH_CHECK(ctx, "k_varh_cells");
+    auto count_rows = k_varh_rows<false, T>;
+    hipLaunchKernelGGL(count_rows, dim3(dgrid), dim3(NN_THREADS), 0, ctx->stream, a, counts,
+                       (const int64_t *)nullptr, (int32_t *)nullptr,
+                       (const int32_t *)c.dcell_of, (const int32_t *)c.cell_ovf);
     SPH_LAUNCH_CHECK(ctx, "k_varh_rows<count>");
     hipLaunchKernelGGL(k_slices, dim3(sph_blocks(nslices * 64, NN_THREADS)), dim3(NN_THREADS), 0,
                        ctx->stream, counts, nd_total, nslices, slice_off, slice_sum);
@@ -1939,39 +2254,56 @@ static int nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const T *d_rec, c
     hipLaunchKernelGGL(k_sum_i64, dim3(1), dim3(1024), 0, ctx->stream, slice_sum, nslices,
                        nnz_dev);
     SPH_LAUNCH_CHECK(ctx, "k_sum_i64");
-    rc = sph_scan_exclusive_i64_at(ctx, slice_off, nslices, fc_bytes + sums_bytes);
+    rc = sph_scan_exclusive_i64_at(ctx, slice_off, nslices, sums_off + sums_bytes);
     if (rc) return rc;
-    rc = sph_ensure_pinned(ctx, 2 * sizeof(int64_t));
+    rc = sph_ensure_pinned(ctx, 3 * sizeof(int64_t));
     if (rc) return rc;
-    SPH_HIP_CHECK(ctx, hipMemcpyAsync(ctx->pinned, slice_off + nslices, sizeof(int64_t),
+    int64_t *hp = (int64_t *)ctx->pinned;
+    SPH_HIP_CHECK(ctx, hipMemcpyAsync(hp, slice_off + nslices, sizeof(int64_t),
                                       hipMemcpyDeviceToHost, ctx->stream));
-    SPH_HIP_CHECK(ctx, hipMemcpyAsync((char *)ctx->pinned + sizeof(int64_t), nnz_dev,
-                                      sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    SPH_HIP_CHECK(ctx, hipMemcpyAsync(hp + 1, nnz_dev, sizeof(int64_t), hipMemcpyDeviceToHost,
+                                      ctx->stream));
+    hp[2] = 0;
+    if (d_ncells_dev)
+        SPH_HIP_CHECK(ctx, hipMemcpyAsync(hp + 2, d_ncells_dev, sizeof(int32_t),
+                                          hipMemcpyDeviceToHost, ctx->stream));
     SPH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    const int64_t slots = ((int64_t *)ctx->pinned)[0];
+    const int64_t slots = hp[0];
     *slots_out = slots;
-    *nnz_out = ((int64_t *)ctx->pinned)[1];
+    *nnz_out = hp[1];
+    const int64_t ncells_now = d_ncells_dev ? (int64_t)*(const int32_t *)(hp + 2) : nd_cells;
+    if (ncells_now > nd_cells)
+        return sph_set_error(ctx, SPH_EINVAL,
+                             "sph_nnps_build_varh: %lld non-empty destination cells exceed the "
+                             "bound %lld", (long long)ncells_now, (long long)nd_cells);
     if (slots > cap || !indices) {
         sph_set_error(ctx, SPH_ENEEDCAP, "neighbour capacity %lld < %lld", (long long)cap,
                       (long long)slots);
         return SPH_ENEEDCAP;
     }
-    auto fill_kernel = k_varh_rows<true, T>;
-    hipLaunchKernelGGL(fill_kernel, dim3(grid_blocks), dim3(NN_THREADS), 0, ctx->stream,
-                       a, counts, (const int64_t *)slice_off, indices);
+    hipLaunchKernelGGL(k_varh_fill, dim3(dgrid), dim3(NN_THREADS), 0, ctx->stream, c, nd_total,
+                       (const int32_t *)counts, (const int64_t *)slice_off, indices, s_rank);
+    SPH_LAUNCH_CHECK(ctx, "k_varh_fill");
+    auto fill_rows = k_varh_rows<true, T>;
+    hipLaunchKernelGGL(fill_rows, dim3(dgrid), dim3(NN_THREADS), 0, ctx->stream, a, counts,
+                       (const int64_t *)slice_off, indices, (const int32_t *)c.dcell_of,
+                       (const int32_t *)c.cell_ovf);
     SPH_LAUNCH_CHECK(ctx, "k_varh_rows<fill>");
     return SPH_OK;
 }
 
 extern "C" int sph_nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
-                                   const int32_t *d_perm, int64_t nd_total, int64_t nd_real,
+                                   const int32_t *d_perm, const int32_t *d_cell_start,
+                                   const int32_t *d_cells, const int32_t *nd_cells_dev,
+                                   int64_t nd_cells, int64_t nd_total, int64_t nd_real,
                                    const int32_t *s_cell_start, const double *s_rec,
                                    const int32_t *s_perm, const int32_t *s_rank, int64_t ns,
                                    int64_t ns_real, double radius_scale, double max_h,
                                    int32_t *counts, int64_t *slice_off, int32_t *indices,
                                    int64_t cap, int64_t *slots_out, int64_t *nnz_out)
 {
-    return nnps_build_varh<double>(ctx, grid, d_rec, nullptr, nullptr, nullptr, d_perm, nd_total,
+    return nnps_build_varh<double>(ctx, grid, d_rec, nullptr, nullptr, nullptr, d_perm,
+                                   d_cell_start, d_cells, nd_cells_dev, nd_cells, nd_total,
                                    nd_real, s_cell_start, s_rec, s_perm, s_rank, ns, ns_real,
                                    radius_scale, max_h, counts, slice_off, indices, cap,
                                    slots_out, nnz_out);
@@ -1979,14 +2311,17 @@ extern "C" int sph_nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const dou
 
 extern "C" int sph_nnps_build_varh_f32(sph_ctx *ctx, const sph_grid *grid, const float *d_rec,
                                        const double *d_x, const double *d_y, const double *d_z,
-                                       const int32_t *d_perm, int64_t nd_total, int64_t nd_real,
+                                       const int32_t *d_perm, const int32_t *d_cell_start,
+                                       const int32_t *d_cells, const int32_t *nd_cells_dev,
+                                       int64_t nd_cells, int64_t nd_total, int64_t nd_real,
                                        const int32_t *s_cell_start, const float *s_rec,
                                        const int32_t *s_perm, const int32_t *s_rank, int64_t ns,
                                        int64_t ns_real, double radius_scale, double max_h,
                                        int32_t *counts, int64_t *slice_off, int32_t *indices,
                                        int64_t cap, int64_t *slots_out, int64_t *nnz_out)
 {
-    return nnps_build_varh<float>(ctx, grid, d_rec, d_x, d_y, d_z, d_perm, nd_total, nd_real,
-                                  s_cell_start, s_rec, s_perm, s_rank, ns, ns_real, radius_scale,
-                                  max_h, counts, slice_off, indices, cap, slots_out, nnz_out);
+    return nnps_build_varh<float>(ctx, grid, d_rec, d_x, d_y, d_z, d_perm, d_cell_start, d_cells,
+                                  nd_cells_dev, nd_cells, nd_total, nd_real, s_cell_start, s_rec,
+                                  s_perm, s_rank, ns, ns_real, radius_scale, max_h, counts,
+                                  slice_off, indices, cap, slots_out, nnz_out);
 }

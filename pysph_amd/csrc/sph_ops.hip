@@ -207,9 +207,9 @@ __global__ void __launch_bounds__(OP_THREADS)
     if (i < n_real) {
         const T ratio = rhoa / ro;
         const T gamma2 = T(0.5) * (gamma - T(1.0));
-        const T tmp = pow(ratio, gamma);
+        const T tmp = sph_pow(ratio, gamma);
         r0 = (tmp - T(1.0)) * B;
-        r1 = pow(ratio, gamma2) * co;
+        r1 = sph_pow(ratio, gamma2) * co;
         if (accumulate) {
             r0 = (p_s ? p_s[t * st] : T(p[i])) + r0;
             r1 = (cs_s ? cs_s[t * st] : T(cs[i])) + r1;
@@ -277,9 +277,9 @@ __global__ void __launch_bounds__(OP_THREADS)
     const T rhoa = r[SPH_R_RHO];
     const T ratio = rhoa / ro;
     const T gamma2 = T(0.5) * (gamma - T(1.0));
-    const T tmp = pow(ratio, gamma);
+    const T tmp = sph_pow(ratio, gamma);
     const T r0 = T(0.0) + (tmp - T(1.0)) * B;
-    const T r1 = T(0.0) + pow(ratio, gamma2) * co;
+    const T r1 = T(0.0) + sph_pow(ratio, gamma2) * co;
     const int32_t i = perm[t];
     p[i] = r0;
     cs[i] = r1;

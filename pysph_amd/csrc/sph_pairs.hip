@@ -66,7 +66,7 @@ __device__ __forceinline__ bool art_fab(const PairDest<T> &D, const PairArgs<T> 
                           : sph_cubic_w_fac<T>(A.deltap, T(0.0), T(0.0), hab, fac);
     if (wdeltap == T(0.0)) return false;
     fab = w / wdeltap;
-    fab = pow(fab, A.n);
+    fab = sph_pow(fab, A.n);
     return true;
 }
 

@@ -214,6 +214,50 @@ def test_tait_closed_forms():
     assert abs(p[3] - (1.01 ** 7 - 1) * B) < 1e-10 * abs(p[3])
 
 
+def _exact_pow_rounded(x, n):
+    """x**n for a double x and integer n, rounded once to the nearest double
+    (ties to even): exact rational power, then Python's correctly rounded
+    Fraction -> float conversion."""
+    from fractions import Fraction
+    return float(Fraction(x) ** n)
+
+
+def test_power_is_correctly_rounded():
+    """The oracle's pow for integral exponents equals the exact power rounded
+    once, on random densities around rho0 (the Tait range), wide magnitudes,
+    negative bases and few-bit bases whose powers are exact or fall on
+    rounding midpoints; the C library's pow misrounds some of these."""
+    rng = np.random.default_rng(11)
+    xs = list(rng.uniform(0.5, 2.0, 3000)) + list(rng.uniform(0.999, 1.001, 2000))
+    xs += list(np.exp(rng.uniform(-13.0, 13.0, 1000)))            # 2^-18.7 .. 2^18.7
+    xs += list(-rng.uniform(0.5, 2.0, 500))
+    xs += [1.0 + k * 2.0 ** -26 for k in range(1, 200)]          # few-bit: midpoints
+    xs += [1.5, 0.75, 1.25, 3.0, 1.0 - 2.0 ** -30, 1.0]
+    wrong, libm_wrong, checked = 0, 0, 0
+    for x in xs:
+        for n in (1, 2, 3, 4, 5, 7, 8, 13, 16):
+            if abs(x) ** n > 1e300:
+                continue
+            want = _exact_pow_rounded(x, n)
+            got = cpu_ref.power(x, float(n))
+            wrong += got != want
+            libm_wrong += math.pow(x, n) != want
+            checked += 1
+    assert wrong == 0, f"{wrong} of {checked} powers not correctly rounded"
+    print(f"{checked} powers correctly rounded; the C library's pow misrounds {libm_wrong}")
+    # the closed form of the Tait ratio at 1.01 rho0
+    assert cpu_ref.power(1.01, 7.0) == _exact_pow_rounded(1.01, 7)
+
+
+def test_power_other_exponents_use_libm():
+    """Non-integral exponents and out-of-range bases go to the C library."""
+    for x, e in ((1.3, 0.5), (1.3, 3.5), (2.0 ** -25, 7.0), (2.0 ** 21, 2.0), (1.1, 40.0),
+                 (1.1, 0.0)):
+        assert cpu_ref.power(x, e) == math.pow(x, e), (x, e)
+    assert math.isnan(cpu_ref.power(float("nan"), 7.0))
+    assert cpu_ref.power(float("inf"), 7.0) == math.inf
+
+
 def test_summation_density_unit_square_and_lattice():
     s = _arr(FIX["unit_square"])
     off, idx = cpu_ref.nnps(s, s, 2.0, 2.0, True)

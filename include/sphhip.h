@@ -499,6 +499,19 @@ int sph_pair_accel_f32(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
 int sph_extrema(sph_ctx *ctx, int64_t n, int ncomp, const double *a, const double *b,
                 const double *c, double *out);
 
+/* The reductions of sph_extrema without allocating anything.  No counterpart
+ * in the reference: for callers that cannot allocate device or pinned memory
+ * (code under HIP graph capture, tight stepping loops, harnesses that forbid
+ * hipMalloc).  The block partials live in a file-scope device array of the
+ * library (3 x 512 doubles, 12 KiB per device, held until the process ends);
+ * the three results (max |a|, min a, max a) are written to out_dev, device
+ * memory of 3 doubles, stream-ordered: the call neither allocates nor waits.
+ * Any n is accepted (at most 512 blocks, grid-stride).  Every call on one
+ * device shares that array, so calls that overlap on one device (other
+ * streams, other host threads) race. */
+int sph_extrema_noalloc(sph_ctx *ctx, int64_t n, int ncomp, const double *a, const double *b,
+                        const double *c, double *out_dev);
+
 /* GravityForce (sph/funcs/external_force.pyx:38-44): a (+)= g; az may be
  * NULL.  Original order. */
 int sph_gravity(sph_ctx *ctx, int64_t n_real, int64_t n_total, double gx, double gy, double gz,

@@ -184,6 +184,7 @@ _SIGNATURES = {
     "sph_xsph_f32": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _dbl,
                             _i32, _vp, _vp, _vp]),
     "sph_extrema": (_i32, [_vp, _i64, _i32, _vp, _vp, _vp, ctypes.POINTER(_dbl)]),
+    "sph_extrema_noalloc": (_i32, [_vp, _i64, _i32, _vp, _vp, _vp, _vp]),
     "sph_gravity": (_i32, [_vp, _i64, _i64, _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp]),
     "sph_position_stepping": (_i32, [_vp, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "sph_stage_step": (_i32, [_vp, _i64, ctypes.POINTER(StepProps), _dbl]),

@@ -24,6 +24,9 @@ int sph_ctx_create(sph_ctx **out, int device, void *stream)
         free(ctx);
         return SPH_EHIP;
     }
+    // the scratch address of sph_extrema_noalloc; on failure that entry
+    // point tries again on its first call and reports the error
+    (void)sph_extrema_scratch_setup(ctx);
     *out = ctx;
     return SPH_OK;
 }

@@ -28,9 +28,12 @@ struct sph_ctx {
     int frozen;
     // compute units of the device (queried on first use)
     int ncu;
+    // device address of the file-scope partials array of sph_extrema_noalloc
+    double *extrema_scratch;
 };
 
 int sph_cu_count(sph_ctx *ctx);
+int sph_extrema_scratch_setup(sph_ctx *ctx);
 
 int sph_set_error(sph_ctx *ctx, int code, const char *fmt, ...);
 int sph_ensure_scratch(sph_ctx *ctx, size_t bytes);

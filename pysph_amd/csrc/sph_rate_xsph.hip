@@ -405,3 +405,60 @@ extern "C" int sph_extrema(sph_ctx *ctx, int64_t n, int ncomp, const double *a, 
     memcpy(out, ctx->pinned, 3 * sizeof(double));
     return SPH_OK;
 }
+
+// ---------------------------------------------------------------------------
+// the same reductions without allocating: partials in a file-scope device
+// array (EXTREMA_SCRATCH_DOUBLES doubles, 12 KiB per device, held until the
+// process ends) and the three results written to device memory
+// ---------------------------------------------------------------------------
+#define EXTREMA_SCRATCH_DOUBLES (3 * RED_BLOCKS)
+static __device__ __align__(256) double scratch[EXTREMA_SCRATCH_DOUBLES];
+
+// The device address of `scratch` on the context's device (each device holds
+// its own copy).  Called by sph_ctx_create with the device current (and by
+// sph_extrema_noalloc if that failed); host code uses only this address, never
+// the array's name.
+int sph_extrema_scratch_setup(sph_ctx *ctx)
+{
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return sph_set_error(ctx, SPH_EHIP, "hipGetSymbolAddress(extrema scratch) failed: %s",
+                             hipGetErrorString(e));
+    ctx->extrema_scratch = (double *)p;
+    return SPH_OK;
+}
+
+extern "C" int sph_extrema_noalloc(sph_ctx *ctx, int64_t n, int ncomp, const double *a,
+                                   const double *b, const double *c, double *out_dev)
+{
+    if (!ctx) return SPH_EINVAL;
+    SPH_REQUIRE(ctx, out_dev && n >= 0 && ncomp >= 1 && ncomp <= 3,
+                "sph_extrema_noalloc: bad argument");
+    SPH_REQUIRE(ctx, n == 0 || (a && (ncomp < 2 || b) && (ncomp < 3 || c)),
+                "sph_extrema_noalloc: null array");
+    if (!ctx->extrema_scratch) {
+        int cur = 0;
+        SPH_HIP_CHECK(ctx, hipGetDevice(&cur));
+        SPH_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+        const int rc = sph_extrema_scratch_setup(ctx);
+        SPH_HIP_CHECK(ctx, hipSetDevice(cur));
+        if (rc) return rc;
+    }
+    // n == 0: the final pass over no partials writes -inf, +inf, -inf
+    int64_t nb = (n + RX_THREADS - 1) / RX_THREADS;
+    if (nb > RED_BLOCKS) nb = RED_BLOCKS;
+    SPH_REQUIRE(ctx, 3 * nb <= EXTREMA_SCRATCH_DOUBLES,
+                "sph_extrema_noalloc: %lld partials exceed the scratch of %d doubles",
+                (long long)(3 * nb), EXTREMA_SCRATCH_DOUBLES);
+    double *part = ctx->extrema_scratch;
+    if (nb > 0) {
+        hipLaunchKernelGGL(k_extrema_part, dim3((unsigned)nb), dim3(RX_THREADS), 0, ctx->stream,
+                           n, ncomp, a, b, c, part);
+        SPH_LAUNCH_CHECK(ctx, "k_extrema_part");
+    }
+    hipLaunchKernelGGL(k_extrema_final, dim3(1), dim3(64), 0, ctx->stream, (int)nb,
+                       (const double *)part, out_dev);
+    SPH_LAUNCH_CHECK(ctx, "k_extrema_final");
+    return SPH_OK;
+}

@@ -642,27 +642,36 @@ def adjoint_measure(fd_particles=64, log=None, reps=3):
         return out
 
     gmap = {"u": gu.cpu().numpy(), "v": gv.cpu().numpy()}
-    sims_e = []
-    for i, comp, sgn in which:
-        de = dict(d)
-        de["u"], de["v"] = u_init.copy(), v_init.copy()
-        de[comp][i] += sgn * eps
-        sims_e.append(de)
-    L = batch_losses(sims_e, 0.1)
-    g, fd = [], []
-    for k in range(0, len(sims_e), 2):
-        i, comp, _ = which[k]
-        g.append(gmap[comp][i])
-        fd.append((L[k] - L[k + 1]) / (2 * eps))
-    g, fd = np.array(g), np.array(fd)
-    rel = np.abs(g - fd) / np.maximum(np.abs(fd), 1e-300)
     d0 = dict(d)
     d0["u"], d0["v"] = u_init.copy(), v_init.copy()
-    la = batch_losses([d0], 0.1 + eps)[0]
-    lb = batch_losses([d0], 0.1 - eps)[0]
-    fda = (la - lb) / (2 * eps)
+
+    def central(e):
+        """central differences at step e: the velocity components, then alpha"""
+        sims_e = []
+        for i, comp, sgn in which:
+            de = dict(d)
+            de["u"], de["v"] = u_init.copy(), v_init.copy()
+            de[comp][i] += sgn * e
+            sims_e.append(de)
+        L = batch_losses(sims_e, 0.1)
+        fd = [(L[k] - L[k + 1]) / (2 * e) for k in range(0, len(sims_e), 2)]
+        la = batch_losses([d0], 0.1 + e)[0]
+        lb = batch_losses([d0], 0.1 - e)[0]
+        return np.array(fd + [(la - lb) / (2 * e)])
+
+    g_all = np.array([gmap[comp][i] for i, comp, _ in which[::2]] + [float(ga)])
+    fd1 = central(eps)
+    rel = np.abs(g_all - fd1) / np.maximum(np.abs(fd1), 1e-300)
     ga_v = float(ga)
-    g_all, fd_all = np.append(g, ga_v), np.append(fd, fda)
+    # alpha: the rollout's central difference at smaller steps (over 200
+    # steps the loss is strongly curved in alpha at the 1e-6 scale and kinked
+    # where viscosity switches flip)
+    alpha_sweep = {}
+    for e in (2.5e-7, 6.25e-8, 3.125e-8):
+        la = batch_losses([d0], 0.1 + e)[0]
+        lb = batch_losses([d0], 0.1 - e)[0]
+        fda = (la - lb) / (2 * e)
+        alpha_sweep[f"{e:g}"] = abs(ga_v - fda) / abs(fda)
     summary = {
         "workload": f"2D elliptical drop dx {dx} ({n} particles), RK2 dt {dt}, {steps} steps, "
                     f"checkpoint every {ckpt}, FP64",
@@ -671,12 +680,18 @@ def adjoint_measure(fd_particles=64, log=None, reps=3):
         "gradient_over_forward": (t_fwd + t_bwd) / t_fwd,
         "plain_gpu_forward_ms": 1e3 * t_plain,
         "forward_over_plain_gpu": t_fwd / t_plain,
-        "fd": {"eps": eps, "velocity_components": int(len(rel)),
-               "norm_rel_err": float(np.linalg.norm(g_all - fd_all) / np.linalg.norm(fd_all)),
-               "velocity_norm_rel_err": float(np.linalg.norm(g - fd) / np.linalg.norm(fd)),
-               "velocity_median_rel_err": float(np.median(rel)),
-               "velocity_max_rel_err": float(rel.max()),
-               "alpha_rel_err": abs(ga_v - fda) / abs(fda), "grad_alpha": ga_v},
+        "fd": {"eps": eps, "velocity_components": int(len(g_all) - 1), "grad_alpha": ga_v,
+               "velocity_norm_rel_err": float(np.linalg.norm(g_all[:-1] - fd1[:-1]) /
+                                              np.linalg.norm(fd1[:-1])),
+               "velocity_median_rel_err": float(np.median(rel[:-1])),
+               "velocity_max_rel_err": float(rel[:-1].max()),
+               "alpha_rel_err": float(rel[-1]),
+               "alpha_rel_err_smaller_eps": alpha_sweep,
+               "note": "adjoint vs central differences of the rollout; the velocity gradient "
+                       "agrees as a vector at eps 1e-6; for alpha the 200-step loss is "
+                       "strongly curved at that scale and kinked where viscosity switches "
+                       "flip, so the difference approaches the adjoint only at smaller eps "
+                       "(the tests compare it with the switches held fixed)"},
     }
     log(f"FD eps {eps}: {summary['fd']}")
     state = dict(roll=roll, d=d, n=n, dev=dev, tx=tx, ty=ty, u_init=u_init, v_init=v_init,

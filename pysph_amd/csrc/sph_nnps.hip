@@ -832,6 +832,7 @@ struct CellArgs {
     int reach;
     int32_t *fcount;
     uint64_t *amasks;
+    int32_t *candj;  // per cell: the original index of each candidate (cand's layout)
 };
 
 __device__ __forceinline__ int wave_max_i(int v)
@@ -1170,9 +1171,19 @@ __global__ void __launch_bounds__(CW)
 // one wavefront per 64-row slice, so that entry k of the 64 rows is one
 // contiguous 256-byte store.  Candidate q = 64*chunk + bit of the cell's
 // stencil is sorted source cand[cell][q].
+// VARH: the rows of k_varh_cells -- the own part (fcount entries, from the
+// own masks, own index not moved), then the appended part from a.amasks in
+// ascending original index (the count pass's candj table), sorted in the
+// lane's own column of the mask array once the own part no longer reads it
+// (VARH_COL entries; a longer part is sorted in place in the row) and
+// written as sorted positions (srank).  Rows of cells k_varh_rows handles are
+// skipped.
+#define VARH_COL (2 * NN_MAXCH)
+template <bool VARH>
 __global__ void __launch_bounds__(NN_THREADS)
     k_nnps_fill_slices(CellArgs a, int64_t nd_total, int32_t *__restrict__ counts,
-                       const int64_t *__restrict__ slice_off, int32_t *__restrict__ indices)
+                       const int64_t *__restrict__ slice_off, int32_t *__restrict__ indices,
+                       const int32_t *__restrict__ srank)
 {
     __shared__ uint64_t s_mask[NN_MAXCH][NN_THREADS];
     if (a.totals && blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1194,11 +1205,13 @@ __global__ void __launch_bounds__(NN_THREADS)
         counts[t] = 0;  // no room (status is set): empty rows, so no reader touches the slots
         return;
     }
-    const int32_t n = counts[t];
-    if (n == 0) return;
+    const int32_t nall = counts[t];
+    if (nall == 0) return;
     const int32_t w = a.dcell_of[t];
+    if (VARH && w < 0) return;
     const int32_t nch = a.cell_ovf[w];  // chunks of the cell's stencil
     if (nch < 0) return;                // written by the per-cell pass
+    const int32_t n = VARH ? a.fcount[t] : nall;  // entries from the (own) masks
     int32_t *row = indices + slice_off[t >> 6] + (t & 63);
     const int32_t *cand = a.cand + (int64_t)w * (a.maxch * CW);
     const uint64_t *mk = a.masks + t;
@@ -1256,10 +1269,85 @@ __global__ void __launch_bounds__(NN_THREADS)
         next_round(k0 + 2 * FILL_UNROLL, q, p);
         store_round(k0 + FILL_UNROLL, qn, pn);
     }
-    if (a.same && self_k >= 0 && self_k < n - 1) {
-        // keep the particle itself at the last index (nnps.pyx:612-619)
-        row[(int64_t)self_k << 6] = last_p;
-        row[(int64_t)(n - 1) << 6] = (int32_t)t;
+    if (!VARH) {
+        if (a.same && self_k >= 0 && self_k < n - 1) {
+            // keep the particle itself at the last index (nnps.pyx:612-619)
+            row[(int64_t)self_k << 6] = last_p;
+            row[(int64_t)(n - 1) << 6] = (int32_t)t;
+        }
+        return;
+    }
+    // the appended part: real sources outside the own radius that reach t,
+    // ascending original index (base/nnps.pyx:742-786)
+    const int32_t ne = nall - n;
+    if (ne <= 0) return;
+    // the lane's column of s_mask as 32-bit slots (only this lane uses them)
+    uint32_t *col = reinterpret_cast<uint32_t *>(&s_mask[0][0]) + 2 * threadIdx.x;
+    auto slot = [&](int32_t k) -> uint32_t & {
+        return col[(int64_t)(k >> 1) * (2 * NN_THREADS) + (k & 1)];
+    };
+    const int32_t *candj = a.candj + (int64_t)w * (a.maxch * CW);
+    if (ne <= VARH_COL) {
+        // original indices into LDS (loads only), sorted, turned back into
+        // sorted positions by loads only, then stored: no load waits behind
+        // a store of this part
+        int32_t e = 0;
+        for (int c2 = 0; c2 < nch; ++c2) {
+            uint64_t am = a.amasks[(int64_t)c2 * a.mstride + t];
+            while (am != 0) {
+                const int b = __ffsll((unsigned long long)am) - 1;
+                am &= am - 1;
+                if (e < ne) slot(e) = (uint32_t)candj[(c2 << 6) + b];
+                ++e;
+            }
+        }
+        const int32_t na = min(e, ne);
+        for (int32_t i = 1; i < na; ++i) {
+            const uint32_t v = slot(i);
+            int32_t q2 = i - 1;
+            while (q2 >= 0 && slot(q2) > v) {
+                slot(q2 + 1) = slot(q2);
+                --q2;
+            }
+            slot(q2 + 1) = v;
+        }
+        int32_t i = 0;
+        for (; i + 4 <= na; i += 4) {
+            const int32_t r0 = srank[slot(i)], r1 = srank[slot(i + 1)];
+            const int32_t r2 = srank[slot(i + 2)], r3 = srank[slot(i + 3)];
+            slot(i) = (uint32_t)r0;
+            slot(i + 1) = (uint32_t)r1;
+            slot(i + 2) = (uint32_t)r2;
+            slot(i + 3) = (uint32_t)r3;
+        }
+        for (; i < na; ++i) slot(i) = (uint32_t)srank[slot(i)];
+        for (i = 0; i < na; ++i) row[(int64_t)(n + i) << 6] = (int32_t)slot(i);
+        return;
+    }
+    // a long appended part: sorted in place in the row
+    int32_t e = 0;
+    for (int c2 = 0; c2 < nch; ++c2) {
+        uint64_t am = a.amasks[(int64_t)c2 * a.mstride + t];
+        while (am != 0) {
+            const int b = __ffsll((unsigned long long)am) - 1;
+            am &= am - 1;
+            if (e < ne) row[(int64_t)(n + e) << 6] = candj[(c2 << 6) + b];
+            ++e;
+        }
+    }
+    const int32_t na = min(e, ne);
+    for (int32_t i = 1; i < na; ++i) {
+        const int32_t v = row[(int64_t)(n + i) << 6];
+        int32_t q2 = i - 1;
+        while (q2 >= 0 && row[(int64_t)(n + q2) << 6] > v) {
+            row[(int64_t)(n + q2 + 1) << 6] = row[(int64_t)(n + q2) << 6];
+            --q2;
+        }
+        row[(int64_t)(n + q2 + 1) << 6] = v;
+    }
+    for (int32_t i = 0; i < na; ++i) {
+        int32_t *ep = row + ((int64_t)(n + i) << 6);
+        *ep = srank[*ep];
     }
 }
 
@@ -1375,8 +1463,9 @@ static int nnps_count(sph_ctx *ctx, CellArgs &a, int64_t ncell_launch, int64_t n
 static int nnps_fill(sph_ctx *ctx, const CellArgs &a, int64_t ncell_launch, int64_t nd_total,
                      int32_t *counts, const int64_t *slice_off, int32_t *indices)
 {
-    hipLaunchKernelGGL(k_nnps_fill_slices, dim3(sph_blocks(nd_total, NN_THREADS)),
-                       dim3(NN_THREADS), 0, ctx->stream, a, nd_total, counts, slice_off, indices);
+    hipLaunchKernelGGL(k_nnps_fill_slices<false>, dim3(sph_blocks(nd_total, NN_THREADS)),
+                       dim3(NN_THREADS), 0, ctx->stream, a, nd_total, counts, slice_off, indices,
+                       (const int32_t *)nullptr);
     SPH_LAUNCH_CHECK(ctx, "k_nnps_fill_slices");
     auto fill_kern = a.f32 ? k_nnps_cells<true, float> : k_nnps_cells<true, double>;
     const int64_t fill_grid = ncell_launch < NN_OVF_GRID ? ncell_launch : NN_OVF_GRID;
@@ -1965,14 +2054,18 @@ __device__ void varh_cell(const CellArgs &a, int64_t w, int32_t *__restrict__ co
                 }
                 const int prev = lo > 0 ? seg_end[lo - 1] : 0;
                 p = seg_start[lo] + (q - prev);
-                if (g0 == ds) a.cand[w * (a.maxch * CW) + q] = p;
                 const T *sr = srec + (int64_t)p * SPH_REC;
                 sx = sr[SPH_R_X];
                 sy = sr[SPH_R_Y];
                 sz = sr[SPH_R_Z];
                 const T rj = sr[SPH_R_H] * T(a.rs);
                 rj2 = rj * rj;
-                jreal = a.sperm[p] < a.ns_real;
+                const int32_t j = a.sperm[p];
+                jreal = j < a.ns_real;
+                if (g0 == ds) {
+                    a.cand[w * (a.maxch * CW) + q] = p;
+                    a.candj[w * (a.maxch * CW) + q] = j;
+                }
             }
             const uint64_t validm = __builtin_amdgcn_ballot_w64(valid);
             uint32_t olo = 0, ohi = 0, alo = 0, ahi = 0;
@@ -2040,92 +2133,11 @@ __global__ void __launch_bounds__(NN_THREADS)
     dcell_of[t] = -1;
 }
 
-// SELL-64 rows from the masks of k_varh_cells: one lane per destination.
-// The own part in mask order (the stencil order); the appended part
-// gathered as original indices, sorted ascending (in an LDS column of
-// VARH_LDS entries per lane, or in place in the row when longer) and
-// written back as sorted positions.
-__global__ void __launch_bounds__(NN_THREADS)
-    k_varh_fill(CellArgs a, int64_t nd_total, const int32_t *__restrict__ counts,
-                const int64_t *__restrict__ slice_off, int32_t *__restrict__ indices,
-                const int32_t *__restrict__ srank)
-{
-    __shared__ int32_t s_app[VARH_LDS][NN_THREADS];
-    const int64_t t = (int64_t)sph_xcd_tile(blockIdx.x, gridDim.x) * NN_THREADS + threadIdx.x;
-    if (t >= nd_total) return;
-    const int32_t n = counts[t];
-    if (n == 0) return;
-    const int32_t w = a.dcell_of[t];
-    if (w < 0) return;
-    const int32_t nch = a.cell_ovf[w];
-    if (nch < 0) return;  // k_varh_rows writes the flagged cells' rows
-    const int32_t nf = a.fcount[t];
-    const int32_t ne = n - nf;
-    int32_t *row = indices + slice_off[t >> 6] + (t & 63);
-    const int32_t *cand = a.cand + (int64_t)w * (a.maxch * CW);
-    int32_t k = 0;
-    for (int c = 0; c < nch; ++c) {
-        uint64_t m = a.masks[(int64_t)c * a.mstride + t];
-        while (m != 0) {
-            const int b = __ffsll((unsigned long long)m) - 1;
-            m &= m - 1;
-            if (k < nf) row[(int64_t)k << 6] = cand[(c << 6) + b];
-            ++k;
-        }
-    }
-    if (ne <= 0) return;
-    const bool in_lds = ne <= VARH_LDS;
-    int32_t e = 0;
-    for (int c = 0; c < nch; ++c) {
-        uint64_t m = a.amasks[(int64_t)c * a.mstride + t];
-        while (m != 0) {
-            const int b = __ffsll((unsigned long long)m) - 1;
-            m &= m - 1;
-            if (e < ne) {
-                const int32_t j = a.sperm[cand[(c << 6) + b]];  // original index
-                if (in_lds)
-                    s_app[e][threadIdx.x] = j;
-                else
-                    row[(int64_t)(nf + e) << 6] = j;
-            }
-            ++e;
-        }
-    }
-    const int32_t na = min(e, ne);
-    if (in_lds) {
-        for (int32_t i = 1; i < na; ++i) {
-            const int32_t v = s_app[i][threadIdx.x];
-            int32_t q = i - 1;
-            while (q >= 0 && s_app[q][threadIdx.x] > v) {
-                s_app[q + 1][threadIdx.x] = s_app[q][threadIdx.x];
-                --q;
-            }
-            s_app[q + 1][threadIdx.x] = v;
-        }
-        for (int32_t i = 0; i < na; ++i)
-            row[(int64_t)(nf + i) << 6] = srank[s_app[i][threadIdx.x]];
-        return;
-    }
-    for (int32_t i = 1; i < na; ++i) {
-        const int32_t v = row[(int64_t)(nf + i) << 6];
-        int32_t q = i - 1;
-        while (q >= 0 && row[(int64_t)(nf + q) << 6] > v) {
-            row[(int64_t)(nf + q + 1) << 6] = row[(int64_t)(nf + q) << 6];
-            --q;
-        }
-        row[(int64_t)(nf + q + 1) << 6] = v;
-    }
-    for (int32_t i = 0; i < na; ++i) {
-        int32_t *ep = row + ((int64_t)(nf + i) << 6);
-        *ep = srank[*ep];
-    }
-}
-
 // Count pass per destination cell (k_varh_cells; the destinations of flagged
 // cells by k_varh_rows), slice offsets and totals, the host check of the
 // capacity, then the fill.  Scratch: [own masks | appended masks | cell flags
-// | candidate sources | cell of destination | own-query entries | slice sums
-// | scan partials].
+// | candidate sources | cell of destination | own-query entries | candidate
+// original indices | slice sums | scan partials].
 template <typename T>
 static int nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const T *d_rec, const double *d_x,
                            const double *d_y, const double *d_z, const int32_t *d_perm,
@@ -2172,7 +2184,8 @@ static int nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const T *d_rec, c
     const size_t dcell_bytes = al((size_t)nd_total * sizeof(int32_t));
     const size_t fc_bytes = al((size_t)nd_total * sizeof(int32_t));
     const size_t sums_bytes = al((size_t)(nslices + 1) * sizeof(int64_t) + 256);
-    const size_t sums_off = 2 * masks_bytes + ovf_bytes + cand_bytes + dcell_bytes + fc_bytes;
+    const size_t sums_off =
+        2 * masks_bytes + ovf_bytes + 2 * cand_bytes + dcell_bytes + fc_bytes;
     rc = sph_ensure_scratch(ctx, sums_off + sums_bytes +
                                      sph_scan_scratch_bytes(nslices, sizeof(int64_t)));
     if (rc) return rc;
@@ -2209,6 +2222,7 @@ static int nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const T *d_rec, c
     c.cand = (int32_t *)(sc + 2 * masks_bytes + ovf_bytes);
     c.dcell_of = (int32_t *)(sc + 2 * masks_bytes + ovf_bytes + cand_bytes);
     c.fcount = (int32_t *)(sc + 2 * masks_bytes + ovf_bytes + cand_bytes + dcell_bytes);
+    c.candj = (int32_t *)(sc + 2 * masks_bytes + ovf_bytes + cand_bytes + dcell_bytes + fc_bytes);
     int64_t *slice_sum = (int64_t *)(sc + sums_off);
     int64_t *nnz_dev = slice_sum + nslices;
     // the rows of flagged cells: the lane-per-destination kernel
@@ -2281,9 +2295,9 @@ static int nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const T *d_rec, c
                       (long long)slots);
         return SPH_ENEEDCAP;
     }
-    hipLaunchKernelGGL(k_varh_fill, dim3(dgrid), dim3(NN_THREADS), 0, ctx->stream, c, nd_total,
-                       (const int32_t *)counts, (const int64_t *)slice_off, indices, s_rank);
-    SPH_LAUNCH_CHECK(ctx, "k_varh_fill");
+    hipLaunchKernelGGL(k_nnps_fill_slices<true>, dim3(dgrid), dim3(NN_THREADS), 0, ctx->stream,
+                       c, nd_total, counts, (const int64_t *)slice_off, indices, s_rank);
+    SPH_LAUNCH_CHECK(ctx, "k_nnps_fill_slices<varh>");
     auto fill_rows = k_varh_rows<true, T>;
     hipLaunchKernelGGL(fill_rows, dim3(dgrid), dim3(NN_THREADS), 0, ctx->stream, a, counts,
                        (const int64_t *)slice_off, indices, (const int32_t *)c.dcell_of,

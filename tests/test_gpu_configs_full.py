@@ -13,10 +13,12 @@ target run from u = -120 x, v = 120 y; gradient with respect to the
 per-particle initial velocities (u = -100 x, v = 100 y plus seeded N(0, 5))
 and alpha.  Checked (a) against torch.autograd of the pure-PyTorch SPH
 (tests/sph_torch_ref.py) rolled out on the GPU with its own pair lists, and
-(b) against central differences on 64 random particles and alpha at eps =
-1e-6 and 2e-6: the two show the O(eps^2) truncation error of the central
-difference (their error ratio is 4), and their Richardson combination agrees
-with the adjoint within 1e-4.
+(b) against central differences of the rollout with eps = 1e-6 on 64 random
+particles (u and v: within 1e-4 as a vector) and (c) for alpha, whose
+rollout difference at that eps is off by a few 1e-3 (the loss is strongly
+curved and kinked in alpha over 200 steps), against central differences
+with the unperturbed run's pairs and viscosity switches held fixed, which
+converge to the adjoint as eps^2 (within 1e-4 at eps = 6.25e-8).
 """
 import numpy as np
 import pytest
@@ -175,11 +177,8 @@ def test_adjoint_fit_configuration(gpu):
     assert e_pos <= 1e-10
     assert max(e_auto.values()) <= 1e-8
 
-    # (b) central differences on 64 particles (u and v) and alpha, at eps
-    # and 2 eps: their difference shows the truncation error of the central
-    # difference itself (O(eps^2): the 200-step rollout amplifies a velocity
-    # perturbation strongly, so its third derivative is large), and the
-    # Richardson combination (4 FD(eps) - FD(2 eps)) / 3 cancels that term.
+    # (b) central differences of the rollout (eps = 1e-6) on 64 random
+    # particles' u and v and on alpha
     picks = rng.choice(n, nfd, replace=False)
     which = [(int(i), c) for i in picks for c in (0, 1)]
     g_adj = np.array([gu[i] if c == 0 else gv[i] for i, c in which] + [ga])
@@ -197,36 +196,42 @@ def test_adjoint_fit_configuration(gpu):
                        (ys[b.sim_off_host[k]:b.sim_off_host[k + 1]] - tyn) ** 2).mean())
                 for k in range(len(sim_list))]
 
-    def central(e):
-        sims = []
-        for i, c in which:
-            for sgn in (1.0, -1.0):
-                dd = dict(d)
-                dd["u"], dd["v"] = u_init.copy(), v_init.copy()
-                dd["u" if c == 0 else "v"][i] += sgn * e
-                sims.append(dd)
-        L = losses(sims, 0.1)
-        la, lb = losses([d0], 0.1 + e)[0], losses([d0], 0.1 - e)[0]
-        return np.array([(L[2 * k] - L[2 * k + 1]) / (2 * e) for k in range(len(which))] +
-                        [(la - lb) / (2 * e)])
+    sims = []
+    for i, c in which:
+        for sgn in (1.0, -1.0):
+            dd = dict(d)
+            dd["u"], dd["v"] = u_init.copy(), v_init.copy()
+            dd["u" if c == 0 else "v"][i] += sgn * eps
+            sims.append(dd)
+    L = losses(sims, 0.1)
+    la, lb = losses([d0], 0.1 + eps)[0], losses([d0], 0.1 - eps)[0]
+    fd = np.array([(L[2 * k] - L[2 * k + 1]) / (2 * eps) for k in range(len(which))] +
+                  [(la - lb) / (2 * eps)])
+    rel = np.abs(g_adj - fd) / np.abs(fd)
+    e_vel = _norm_rel(g_adj[:-1], fd[:-1])
+    print(f"central differences eps {eps:g}: {len(which)} velocity components |g - FD|/|FD| "
+          f"{e_vel:.3g} (componentwise median {np.median(rel[:-1]):.3g}); alpha {rel[-1]:.3g}")
+    assert e_vel <= 1e-4
 
-    fd1, fd2 = central(eps), central(2 * eps)
-    fdr = (4.0 * fd1 - fd2) / 3.0
-
-    def stats(g, f):
-        rel = np.abs(g - f) / np.abs(f)
-        return _norm_rel(g, f), float(np.median(rel[:-1])), float(rel[:-1].max()), float(rel[-1])
-
-    s1, s2, sr = stats(g_adj, fd1), stats(g_adj, fd2), stats(g_adj, fdr)
-    print(f"central differences, {len(which)} velocity components + alpha "
-          f"(|g - FD|/|FD|; componentwise median, max; alpha):")
-    for lab, st in ((f"eps {eps:g}", s1), (f"eps {2 * eps:g}", s2),
-                    (f"Richardson (eps {eps:g}, {2 * eps:g})", sr)):
-        print(f"  {lab:32s} {st[0]:.3g}; {st[1]:.3g}, {st[2]:.3g}; alpha {st[3]:.3g}")
-    ratio = s2[0] / s1[0]
-    print(f"  error ratio FD(2 eps) / FD(eps): {ratio:.2f} (4 for an O(eps^2) truncation error)")
-    # the adjoint is the exact derivative of the rollout (autograd above, 1e-8);
-    # the central difference at eps carries its O(eps^2) error ...
-    assert 3.0 <= ratio <= 5.0
-    # ... and without it agrees to the 1e-4 bar, as a vector and for alpha
-    assert sr[0] <= 1e-4 and sr[3] <= 1e-4
+    # (c) alpha.  Over 200 steps the loss is strongly curved in alpha at the
+    # 1e-6 scale (the flow amplifies a perturbation about tenfold every 20
+    # steps) and has kinks where a pair's viscosity switch (v.r < 0;
+    # continuous, not differentiable) flips, so the rollout's central
+    # difference at eps = 1e-6 misses the derivative by a few 1e-3 and smaller
+    # steps cross kinks.  With the unperturbed run's pairs and switches held
+    # fixed -- the branch whose derivative the adjoint computes -- the central
+    # difference converges to the adjoint's alpha gradient as eps^2.
+    eps_a = (1e-6, 2.5e-7, 6.25e-8)
+    B = 2 * len(eps_a)
+    ab = torch.tensor([[0.1 + sg * e] for e in eps_a for sg in (1.0, -1.0)],
+                      dtype=torch.float64, device=dev)
+    vb = torch.tensor(np.stack([u_init, v_init], 1), device=dev).repeat(B, 1, 1)
+    with torch.no_grad():
+        pb = tref.rk2_rollout_frozen(torch.tensor(pos0, device=dev).repeat(B, 1, 1), vb, m, h,
+                                     ab, dt, record)
+        Lf = (((pb[..., 0] - tx) ** 2 + (pb[..., 1] - ty) ** 2).mean(-1)).cpu().numpy()
+    fd_a = np.array([(Lf[2 * k] - Lf[2 * k + 1]) / (2 * e) for k, e in enumerate(eps_a)])
+    err_a = np.abs(ga - fd_a) / np.abs(fd_a)
+    print("alpha, pairs and switches held fixed: " +
+          ", ".join(f"eps {e:g}: {x:.3g}" for e, x in zip(eps_a, err_a)))
+    assert err_a[0] > err_a[1] > err_a[2] and err_a[2] <= 1e-4

@@ -87,12 +87,26 @@ const char *sph_version(void);
  * xs/ys/zs/hs are HOST arrays of DEVICE pointers, ns HOST counts.
  * radius_scale, min_cell_size, max_cell_size as in cell.pyx:802-941.
  * `order` selects the cell order of the counting sort; max_cells bounds the
- * dense cell table.  Synchronises.  Replaces
+ * dense cell table; `pad` empty cells are added on every side of the
+ * bounding box (cell ids floor(x/cell_size) do not depend on it, so neither
+ * do the lists).  Synchronises.  Replaces
  * CellManager.compute_cell_size/_compute_minmax_h (cell.pyx:911-941, :1140-1158). */
 int sph_grid_build(sph_ctx *ctx, int narrays, const double *const *xs, const double *const *ys,
                    const double *const *zs, const double *const *hs, const int64_t *ns,
                    double radius_scale, double min_cell_size, double max_cell_size, int order,
-                   int64_t max_cells, sph_grid *grid /* host out */);
+                   int64_t max_cells, int pad, sph_grid *grid /* host out */);
+
+/* Whether a grid can be kept for the next binning, without synchronising:
+ * *flags (device int32) = OR over the particles of the arrays of 1 (within
+ * `margin` cells of the grid's edge: rebuild the grid before the next
+ * binning), 2 (outside the grid: a binning with it is invalid) and 8 (a
+ * non-finite coordinate).  The cell-manager counterpart of the reference's
+ * re-binning after a move (cell.pyx:943-976), which recomputes the grid
+ * every time; the caller keeps the padded grid while h is unchanged and the
+ * flags stay clear. */
+int sph_grid_check(sph_ctx *ctx, int narrays, const double *const *xs, const double *const *ys,
+                   const double *const *zs, const int64_t *ns, const sph_grid *grid, int margin,
+                   int32_t *flags);
 
 /* Counting sort of one array into the grid's cells (stable: ascending
  * original index inside a cell).  Outputs:

@@ -118,7 +118,9 @@ _SIGNATURES = {
     "sph_ctx_synchronize": (_i32, [_vp]),
     "sph_last_error": (ctypes.c_char_p, [_vp]),
     "sph_grid_build": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _dbl, _i32, _i64,
-                              ctypes.POINTER(SphGrid)]),
+                              _i32, ctypes.POINTER(SphGrid)]),
+    "sph_grid_check": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, ctypes.POINTER(SphGrid), _i32,
+                              _vp]),
     "sph_bin_array": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _i64, _vp, _vp, _vp,
                              _vp, _vp, ctypes.POINTER(_i64)]),
     "sph_gather_sorted": (_i32, [_vp, _i64, _vp, ctypes.POINTER(PropList)]),

@@ -128,8 +128,23 @@ class CellManager:
         self.is_dirty = True
         self.initialized = False
         self.jump_tolerance = 1
+        # a grid built with GRID_PAD empty cells around the particles is kept
+        # (no bounding-box readback, so no host synchronisation per binning)
+        # while h and the array sizes are unchanged, the moved arrays were
+        # last written by the integrator's stage kernels, and the device check
+        # of the previous binning found every particle at least GRID_MARGIN
+        # cells inside it
+        self.grid_pad = self.GRID_PAD
+        self.keep_grid = True
+        self._grid_key = None
+        self._grid_flags = None
+        self._grid_pending = None
+        self.grid_builds = 0
         if initialize:
             self.initialize()
+
+    GRID_PAD = 4
+    GRID_MARGIN = 2
 
     def initialize(self):
         """Mark the arrays for binning.  The first binning happens at the
@@ -165,6 +180,41 @@ class CellManager:
         self.is_dirty = False
         return 0
 
+    def _grid_state(self, arrays):
+        """What the kept grid depends on besides positions: the arrays, their
+        sizes and their h."""
+        return tuple((a.name, a.get_number_of_particles(), a.version("h")) for a in arrays)
+
+    def _poll_grid_check(self):
+        """Flags of the last device grid check (waits for it: it was queued
+        one binning ago).  Raises when that binning was invalid."""
+        if self._grid_pending is None:
+            return 0
+        ev, host = self._grid_pending
+        self._grid_pending = None
+        ev.synchronize()
+        flags = int(host[0])
+        if flags & 10:
+            self._grid_key = None
+            what = "a non-finite coordinate" if flags & 8 else "a particle outside the cell grid"
+            raise _lib.SphError(_lib.SPH_EDOMAIN,
+                                f"binning found {what}: the evaluation that used it is invalid "
+                                f"(particles moved more than {self.GRID_MARGIN} cells in one "
+                                f"step, or positions were written without touch())")
+        return flags
+
+    def _can_keep_grid(self, arrays):
+        if not self.keep_grid or self.grid is None or \
+                self._grid_key != self._grid_state(arrays):
+            return False
+        # every moved array was last written by the integrator's stage
+        # kernels (any other write of x, y or z rebuilds the grid)
+        for a in arrays:
+            if a.is_dirty and getattr(a, "_stepped_xyz", None) != (
+                    a.version("x"), a.version("y"), a.version("z")):
+                return False
+        return not (self._poll_grid_check() & 1)
+
     def _rebin(self, arrays):
         ctx = self._ctx()
         lib = ctx.lib
@@ -172,19 +222,43 @@ class CellManager:
         xs = (ctypes.c_void_p * n)(*[_lib.ptr(a.get_carray("x")) for a in arrays])
         ys = (ctypes.c_void_p * n)(*[_lib.ptr(a.get_carray("y")) for a in arrays])
         zs = (ctypes.c_void_p * n)(*[_lib.ptr(a.get_carray("z")) for a in arrays])
-        hs = (ctypes.c_void_p * n)(*[_lib.ptr(a.get_carray("h")) for a in arrays])
         ns = (ctypes.c_int64 * n)(*[a.get_number_of_particles() for a in arrays])
-        grid = _lib.SphGrid()
-        ctx.check(lib.sph_grid_build(ctx.handle, n, xs, ys, zs, hs, ns, self.radius_scale,
-                                     self.min_cell_size, self.max_cell_size,
-                                     self.ORDERS[self.order], self.max_cells,
-                                     ctypes.byref(grid)))
-        sig = _grid_signature(grid)
-        same_grid = self.grid is not None and _grid_signature(self.grid) == sig
-        self.grid = grid
-        self._h_uniform = grid.min_h == grid.max_h
-        self._h_versions = {a.name: a.version("h") for a in arrays}
-        self.cell_size = grid.cell_size
+        if self._can_keep_grid(arrays):
+            # the same grid: check on the device that the moved arrays still
+            # fit it, read the flags at the next binning
+            moved = [k for k, a in enumerate(arrays) if a.is_dirty]
+            m = len(moved)
+            if self._grid_flags is None:
+                self._grid_flags = torch.zeros(1, dtype=torch.int32, device=arrays[0].device)
+            ctx.check(lib.sph_grid_check(
+                ctx.handle, m, (ctypes.c_void_p * m)(*[xs[k] for k in moved]),
+                (ctypes.c_void_p * m)(*[ys[k] for k in moved]),
+                (ctypes.c_void_p * m)(*[zs[k] for k in moved]),
+                (ctypes.c_int64 * m)(*[ns[k] for k in moved]), ctypes.byref(self.grid),
+                self.GRID_MARGIN, _lib.ptr(self._grid_flags)))
+            host = torch.empty(1, dtype=torch.int32, pin_memory=True)
+            host.copy_(self._grid_flags, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record()
+            self._grid_pending = (ev, host)
+            grid = self.grid
+            same_grid = True
+        else:
+            self._grid_pending = None
+            hs = (ctypes.c_void_p * n)(*[_lib.ptr(a.get_carray("h")) for a in arrays])
+            grid = _lib.SphGrid()
+            ctx.check(lib.sph_grid_build(ctx.handle, n, xs, ys, zs, hs, ns, self.radius_scale,
+                                         self.min_cell_size, self.max_cell_size,
+                                         self.ORDERS[self.order], self.max_cells,
+                                         int(self.grid_pad), ctypes.byref(grid)))
+            self.grid_builds += 1
+            sig = _grid_signature(grid)
+            same_grid = self.grid is not None and _grid_signature(self.grid) == sig
+            self.grid = grid
+            self._grid_key = self._grid_state(arrays)
+            self._h_uniform = grid.min_h == grid.max_h
+            self._h_versions = {a.name: a.version("h") for a in arrays}
+            self.cell_size = grid.cell_size
         for a in arrays:
             np_ = a.get_number_of_particles()
             b = self.bins.get(a.name)

@@ -308,6 +308,7 @@ __device__ __forceinline__ void density_eos_one(int dim, int64_t t,
     const auto dxy = dr[0], dzm = dr[1];
     const T xa = dxy.x, ya = dxy.y, za = dzm.x, ha = ra[SPH_R_H];
     const T facA = sph_h_dim(ha, dim) * sph_cubic_fac1<T>(dim);
+    const T invA = sph_inv_for_div(ha);
     T nr = T(0.0);
     const int32_t *row = indices + slice_off[t >> 6] + (t & 63);
     const int32_t e = counts[t];
@@ -328,13 +329,13 @@ __device__ __forceinline__ void density_eos_one(int dim, int64_t t,
         }
 #pragma unroll
         for (int u = 0; u < NT; ++u)
-            nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, a[u], b[u], hb[u]);
+            nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, a[u], b[u], hb[u], invA);
     }
     for (; k < e; ++k) {
         const int32_t j = __builtin_nontemporal_load(row + ((int64_t)k << 6));
         const auto *sr = rec2<T>(rec, j);
         const T hb = UH ? ha : rec[(int64_t)j * SPH_REC + SPH_R_H];
-        nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, sr[0], sr[1], hb);
+        nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, sr[0], sr[1], hb, invA);
     }
     eos_epilogue<T>(t, perm[t], nr, rec, ro, co, gamma, B, rho, p, cs);
 }
@@ -431,6 +432,7 @@ __device__ __forceinline__ void momentum_stage_one(int dim, int64_t t,
     D.ha = d5.x;
     D.Pa = d5.y;
     D.facA = sph_h_dim(D.ha, dim) * sph_cubic_fac1<T>(dim);
+    D.inv_ha = sph_inv_for_div(D.ha);
     D.rhoa2 = T(1.0) / (D.rhoa * D.rhoa);
     T dtf = T(0.0);  // dt_fac is reset per calc (pressure_funcs.pyx:214)
     T nr0 = T(0.0), nr1 = T(0.0), nr2 = T(0.0);
@@ -614,7 +616,7 @@ __global__ void __launch_bounds__(STG_THREADS)
     const int32_t i = act ? perm[t] : 0;
     const int32_t e = act ? counts[t] : 0;
     GridDev g = {};
-    T xa = T(0), ya = T(0), za = T(0), ha = T(0), facA = T(0);
+    T xa = T(0), ya = T(0), za = T(0), ha = T(0), facA = T(0), invA = T(0);
     int cx = 0, cy = 0, cz = 0, dl = 0;
     if (act) {
         const auto *dr = rec2<T>(rec, t);
@@ -624,6 +626,7 @@ __global__ void __launch_bounds__(STG_THREADS)
         za = dzm.x;
         ha = rec[t * SPH_REC + SPH_R_H];
         facA = sph_h_dim(ha, dim) * sph_cubic_fac1<T>(dim);
+        invA = sph_inv_for_div(ha);
         if (e > 0) {
             g = grids[sim_of[i]];
             batch_stage_cell<T>(g, i, xa, ya, za, ha, ox, oy, oz, rs, cx, cy, cz, dl);
@@ -639,7 +642,7 @@ __global__ void __launch_bounds__(STG_THREADS)
         if (!UH) dst[2] = P{rec[(int64_t)q * SPH_REC + SPH_R_H], T(0)};
     };
     auto term = [&](const P *pr) {
-        nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, pr[0], pr[1], UH ? ha : pr[NP - 1].x);
+        nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, pr[0], pr[1], UH ? ha : pr[NP - 1].x, invA);
     };
     stg_row<T, NP, BATCH_STG_DENS_CAP>(g, cx, cy, cz, dmax, cell_start, row, e, (int32_t)t, true,
                                        lds, stage, stage, term);
@@ -687,6 +690,7 @@ __global__ void __launch_bounds__(STG_THREADS)
         D.ha = d5.x;
         D.Pa = d5.y;
         D.facA = sph_h_dim(D.ha, dim) * sph_cubic_fac1<T>(dim);
+        D.inv_ha = sph_inv_for_div(D.ha);
         D.rhoa2 = T(1.0) / (D.rhoa * D.rhoa);
         if (e > 0) {
             g = grids[sim_of[i]];

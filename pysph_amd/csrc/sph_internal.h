@@ -209,6 +209,28 @@ __device__ __forceinline__ double sph_pow(double x, double e)
 // float records: the single-precision pow
 __device__ __forceinline__ float sph_pow(float x, float e) { return pow(x, e); }
 
+// a / b correctly rounded, from inv_b = RN(1/b), the reciprocal one IEEE
+// division gave (0 when b lies outside [2^-400, 2^400]): q0 = RN(a * inv_b) is
+// within an ulp of a/b, the residual a - b*q0 is exact by fma, and one fma
+// correction rounds it correctly (Markstein's theorem), so the result is the
+// IEEE quotient bit for bit -- three operations instead of a division.  Used
+// where one destination divides every pair by its own h.  |a| outside
+// [2^-500, 2^500] (zero excepted) takes the IEEE division.
+__device__ __forceinline__ double sph_div_inv(double a, double b, double inv_b)
+{
+    const double aa = fabs(a);
+    if (!(inv_b != 0.0 && ((aa >= 0x1p-500 && aa <= 0x1p500) || a == 0.0))) return a / b;
+    const double q0 = a * inv_b;
+    return fma(fma(-b, q0, a), inv_b, q0);
+}
+__device__ __forceinline__ float sph_div_inv(float a, float b, float) { return a / b; }
+
+// the reciprocal sph_div_inv takes: 1/b by IEEE division, 0 when b is out of range
+template <typename T> __device__ __forceinline__ T sph_inv_for_div(T b)
+{
+    return (b >= T(0x1p-400) && b <= T(0x1p400)) ? T(1.0) / b : T(0.0);
+}
+
 // Every helper is a template over the arithmetic type T (double: the
 // reference's FP64 path; float: the single-precision variant); the double
 // instantiation evaluates exactly the expressions above.
@@ -245,10 +267,10 @@ __device__ __forceinline__ double sph_cubic_w(int dim, double rx, double ry, dou
 // W with the factor h_dim(h)*sigma supplied by the caller (the same value
 // sph_cubic_w computes)
 template <typename T>
-__device__ __forceinline__ T sph_cubic_w_fac(T rx, T ry, T rz, T h, T fac)
+__device__ __forceinline__ T sph_cubic_w_fac(T rx, T ry, T rz, T h, T fac, T inv_h = T(0.0))
 {
     T rab = sqrt(rx * rx + ry * ry + rz * rz);
-    T q = rab / h;
+    T q = sph_div_inv(rab, h, inv_h);
     T val;
     if (q > T(2.0))
         val = T(0.0);
@@ -264,10 +286,10 @@ __device__ __forceinline__ T sph_cubic_w_fac(T rx, T ry, T rz, T h, T fac)
 // denominators; selecting both first and dividing once gives the same
 // quotient with one division per pair instead of two divergent ones.
 template <typename T>
-__device__ __forceinline__ T sph_cubic_gfac_fac(T rx, T ry, T rz, T h, T fac)
+__device__ __forceinline__ T sph_cubic_gfac_fac(T rx, T ry, T rz, T h, T fac, T inv_h = T(0.0))
 {
     const T rab = sqrt(rx * rx + ry * ry + rz * rz);
-    const T q = rab / h;
+    const T q = sph_div_inv(rab, h, inv_h);
     const bool outer = q >= T(1.0);
     const T num = outer ? T(-0.75) * (T(2) - q) * (T(2) - q) : T(3.0) * (T(0.75) * q - T(1));
     const T den = outer ? h * rab : h * h;

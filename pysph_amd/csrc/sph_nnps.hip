@@ -198,12 +198,13 @@ extern "C" int sph_grid_build(sph_ctx *ctx, int narrays, const double *const *xs
                               const double *const *ys, const double *const *zs,
                               const double *const *hs, const int64_t *ns, double radius_scale,
                               double min_cell_size, double max_cell_size, int order,
-                              int64_t max_cells, sph_grid *grid)
+                              int64_t max_cells, int pad, sph_grid *grid)
 {
     if (!ctx) return SPH_EINVAL;
     SPH_REQUIRE(ctx, grid && narrays >= 0 && xs && ys && zs && hs && ns, "sph_grid_build: null");
     SPH_REQUIRE(ctx, order == SPH_ORDER_MORTON || order == SPH_ORDER_LINEAR,
                 "sph_grid_build: bad order %d", order);
+    SPH_REQUIRE(ctx, pad >= 0 && pad <= 1024, "sph_grid_build: bad pad %d", pad);
     const int kBlocks = 512;
     int nparts = 0;
     for (int a = 0; a < narrays; ++a)
@@ -256,8 +257,10 @@ extern "C" int sph_grid_build(sph_ctx *ctx, int narrays, const double *const *xs
         double fl = floor(lo[a] / cs), fh = floor(hi[a] / cs);
         if (fabs(fl) > 1.0e9 || fabs(fh) > 1.0e9)
             return sph_set_error(ctx, SPH_EDOMAIN, "cell ids out of range on axis %d", a);
-        grid->cmin[a] = (int32_t)fl;
-        int64_t d = (int64_t)fh - (int64_t)fl + 1;
+        // pad empty cells on both sides: particles that move a little stay
+        // inside the grid, which can then be kept (sph_grid_check)
+        grid->cmin[a] = (int32_t)fl - pad;
+        int64_t d = (int64_t)fh - (int64_t)fl + 1 + 2 * (int64_t)pad;
         if (d > (1 << 21))
             return sph_set_error(ctx, SPH_EDOMAIN, "grid too large on axis %d (%lld cells)", a,
                                  (long long)d);
@@ -302,6 +305,68 @@ __global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v)
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
+
+// ---------------------------------------------------------------------------
+// keeping a grid: do the arrays still fit it?
+// ---------------------------------------------------------------------------
+// flags |= 1: a particle lies within `margin` cells of the grid's edge (the
+// grid should be rebuilt before the next binning); 2: a particle lies outside
+// the grid (a binning with this grid clamps it to an edge cell: its lists are
+// wrong); 8: a non-finite coordinate.  One atomic per wavefront with a flag.
+#define GRID_NEAR_EDGE 1
+#define GRID_OUTSIDE 2
+#define GRID_NONFINITE 8
+
+__global__ void __launch_bounds__(NN_THREADS)
+    k_grid_check(GridDev g, int margin, const double *__restrict__ x,
+                 const double *__restrict__ y, const double *__restrict__ z, int64_t n,
+                 int32_t *__restrict__ flags)
+{
+    int32_t f = 0;
+    for (int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * NN_THREADS) {
+        const double a = x[i], b = y[i], c = z[i];
+        if (!isfinite(a) || !isfinite(b) || !isfinite(c)) {
+            f |= GRID_NONFINITE;
+            continue;
+        }
+        const double ca = floor(a / g.cs) - g.cmin0, cb = floor(b / g.cs) - g.cmin1,
+                     cc = floor(c / g.cs) - g.cmin2;
+        if (ca < 0.0 || cb < 0.0 || cc < 0.0 || ca >= g.d0 || cb >= g.d1 || cc >= g.d2)
+            f |= GRID_OUTSIDE;
+        else if (ca < margin || cb < margin || cc < margin || ca >= g.d0 - margin ||
+                 cb >= g.d1 - margin || cc >= g.d2 - margin)
+            f |= GRID_NEAR_EDGE;
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) f |= __shfl_xor(f, d, 64);
+    if ((threadIdx.x & 63) == 0 && f) atomicOr(flags, f);
+}
+
+extern "C" int sph_grid_check(sph_ctx *ctx, int narrays, const double *const *xs,
+                              const double *const *ys, const double *const *zs,
+                              const int64_t *ns, const sph_grid *grid, int margin,
+                              int32_t *flags)
+{
+    if (!ctx) return SPH_EINVAL;
+    SPH_REQUIRE(ctx, grid && flags && narrays >= 0 && (narrays == 0 || (xs && ys && zs && ns)),
+                "sph_grid_check: null argument");
+    SPH_REQUIRE(ctx, margin >= 0 && grid->cell_size > 0.0, "sph_grid_check: bad margin or grid");
+    GridDev g = make_griddev(ctx, grid);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, flags, 1, 0);
+    SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
+    for (int a = 0; a < narrays; ++a) {
+        if (ns[a] <= 0) continue;
+        SPH_REQUIRE(ctx, xs[a] && ys[a] && zs[a], "sph_grid_check: null array %d", a);
+        int64_t nb = (ns[a] + NN_THREADS - 1) / NN_THREADS;
+        if (nb > 1024) nb = 1024;
+        hipLaunchKernelGGL(k_grid_check, dim3((unsigned)nb), dim3(NN_THREADS), 0, ctx->stream, g,
+                           margin, xs[a], ys[a], zs[a], ns[a], flags);
+        SPH_LAUNCH_CHECK(ctx, "k_grid_check");
+    }
+    return SPH_OK;
+}
+
 
 __global__ void __launch_bounds__(NN_THREADS)
     k_bin_keys(GridDev g, const double *__restrict__ x, const double *__restrict__ y,
@@ -1288,20 +1353,38 @@ __global__ void __launch_bounds__(NN_THREADS)
     };
     const int32_t *candj = a.candj + (int64_t)w * (a.maxch * CW);
     if (ne <= VARH_COL) {
-        // original indices into LDS (loads only), sorted, turned back into
-        // sorted positions by loads only, then stored: no load waits behind
-        // a store of this part
+        // the chunk masks loaded together; the candidate numbers into LDS
+        // (no loads), then their original indices by loads four at a time,
+        // sorted, turned back into sorted positions the same way and stored:
+        // no load waits behind a store or a single other load of this part
+        uint64_t am[NN_MAXCH];
+#pragma unroll
+        for (int c2 = 0; c2 < NN_MAXCH; ++c2)
+            am[c2] = c2 < nch ? a.amasks[(int64_t)c2 * a.mstride + t] : 0ull;
         int32_t e = 0;
-        for (int c2 = 0; c2 < nch; ++c2) {
-            uint64_t am = a.amasks[(int64_t)c2 * a.mstride + t];
-            while (am != 0) {
-                const int b = __ffsll((unsigned long long)am) - 1;
-                am &= am - 1;
-                if (e < ne) slot(e) = (uint32_t)candj[(c2 << 6) + b];
+#pragma unroll
+        for (int c2 = 0; c2 < NN_MAXCH; ++c2) {
+            uint64_t m2 = am[c2];
+            while (m2 != 0) {
+                const int b = __ffsll((unsigned long long)m2) - 1;
+                m2 &= m2 - 1;
+                if (e < ne) slot(e) = (uint32_t)((c2 << 6) + b);
                 ++e;
             }
         }
         const int32_t na = min(e, ne);
+        {
+            int32_t i = 0;
+            for (; i + 4 <= na; i += 4) {
+                const int32_t j0 = candj[slot(i)], j1 = candj[slot(i + 1)];
+                const int32_t j2 = candj[slot(i + 2)], j3 = candj[slot(i + 3)];
+                slot(i) = (uint32_t)j0;
+                slot(i + 1) = (uint32_t)j1;
+                slot(i + 2) = (uint32_t)j2;
+                slot(i + 3) = (uint32_t)j3;
+            }
+            for (; i < na; ++i) slot(i) = (uint32_t)candj[slot(i)];
+        }
         for (int32_t i = 1; i < na; ++i) {
             const uint32_t v = slot(i);
             int32_t q2 = i - 1;

@@ -36,6 +36,7 @@ __device__ __forceinline__ T rho_row(int dim, T xa, T ya, T za, T ha, T facA,
 {
     T nr = T(0.0);
     int32_t k = 0;
+    const T invA = sph_inv_for_div(ha);
     // DENS_NT neighbours' loads in flight, then their terms added in list order;
     // each list load is one coalesced line for the wavefront, read once, so it
     // bypasses the caches
@@ -55,13 +56,13 @@ __device__ __forceinline__ T rho_row(int dim, T xa, T ya, T za, T ha, T facA,
         }
 #pragma unroll
         for (int u = 0; u < DENS_NT; ++u)
-            nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, a[u], b[u], hb[u]);
+            nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, a[u], b[u], hb[u], invA);
     }
     for (; k < e; ++k) {
         const int32_t j = __builtin_nontemporal_load(row + ((int64_t)k << 6));
         const auto *sp = pm2(spm, srec, j);
         const T hb = UH ? ha : srec[(int64_t)j * SPH_REC + SPH_R_H];
-        nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, sp[0], sp[1], hb);
+        nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, sp[0], sp[1], hb, invA);
     }
     return nr;
 }
@@ -404,6 +405,7 @@ __global__ void __launch_bounds__(OP_THREADS) __attribute__((amdgpu_waves_per_eu
         D.ha = d5.x;
         D.Pa = d5.y;
         D.facA = sph_h_dim(D.ha, dim) * sph_cubic_fac1<T>(dim);
+        D.inv_ha = sph_inv_for_div(D.ha);
         D.rhoa2 = T(1.0) / (D.rhoa * D.rhoa);
         for (int s = 0; s < L.nsrc; ++s) {
             T nr0 = T(0.0), nr1 = T(0.0), nr2 = T(0.0);

@@ -60,7 +60,7 @@ __global__ void __launch_bounds__(STG_THREADS)
     }
     T val = T(0.0);
     if (real) val = accumulate ? (out_s ? out_s[t * out_stride] : T(rho[i])) : T(0.0);
-    T xa = T(0), ya = T(0), za = T(0), ha = T(0), facA = T(0);
+    T xa = T(0), ya = T(0), za = T(0), ha = T(0), facA = T(0), invA = T(0);
     int cx = 0, cy = 0, cz = 0, dl = 0;
     if (any) {
         const auto *dr = rec2(drec, t);
@@ -70,6 +70,7 @@ __global__ void __launch_bounds__(STG_THREADS)
         za = dzm.x;
         ha = drec[t * SPH_REC + SPH_R_H];
         facA = sph_h_dim(ha, dim) * sph_cubic_fac1<T>(dim);
+        invA = sph_inv_for_div(ha);
         if (ox) stg_cell(g, ox[i], oy[i], oz[i], cx, cy, cz);
         else stg_cell(g, (double)xa, (double)ya, (double)za, cx, cy, cz);
         dl = stg_reach<T>(ha, rs, g.cs);
@@ -89,7 +90,7 @@ __global__ void __launch_bounds__(STG_THREADS)
         auto fetch = [&](P *pr, int32_t p) { stage(pr, p); };
         auto term = [&](const P *pr) {
             nr += rho_term<UH, T>(dim, xa, ya, za, ha, facA, pr[0], pr[1],
-                                  UH ? ha : pr[NP - 1].x);
+                                  UH ? ha : pr[NP - 1].x, invA);
         };
         stg_row<T, NP, CAP>(g, cx, cy, cz, dmax, S.sstart[s], row, e[s], (int32_t)t,
                             S.same[s] != 0, lds, stage, fetch, term);
@@ -161,6 +162,7 @@ __global__ void __launch_bounds__(STG_THREADS)
         D.ha = d5.x;
         D.Pa = d5.y;
         D.facA = sph_h_dim(D.ha, dim) * sph_cubic_fac1<T>(dim);
+        D.inv_ha = sph_inv_for_div(D.ha);
         D.rhoa2 = T(1.0) / (D.rhoa * D.rhoa);
         if (ox) stg_cell(g, ox[i], oy[i], oz[i], cx, cy, cz);
         else stg_cell(g, (double)D.xa, (double)D.ya, (double)D.za, cx, cy, cz);

@@ -24,20 +24,18 @@ __device__ __forceinline__ const typename Pair<T>::type *rec2(const T *rec, int6
 // ---------------------------------------------------------------------------
 // one neighbour term of SPHRho (density_funcs.pyx:41-73); sxy = (x, y) and
 // szm = (z, m) of the source record, hb its h (not read when UH)
+// invA: the destination's sph_inv_for_div(h_a), so q = r/h_a is the IEEE
+// quotient from three operations (sph_div_inv)
 template <bool UH, typename T, typename P>
 __device__ __forceinline__ T rho_term(int dim, T xa, T ya, T za, T ha, T facA, P sxy, P szm,
-                                      T hb)
+                                      T hb, T invA)
 {
     // with h_b == h_a, 0.5*(h_a+h_b) == h_a exactly and h_dim(h)*sigma is the
     // destination's value: reuse it instead of dividing again
-    T hab, fac;
-    if (UH || hb == ha) {
-        hab = ha;
-        fac = facA;
-    } else {
-        hab = T(0.5) * (ha + hb);
-        fac = sph_h_dim(hab, dim) * sph_cubic_fac1<T>(dim);
-    }
+    if (UH || hb == ha)
+        return sph_cubic_w_fac<T>(xa - sxy.x, ya - sxy.y, za - szm.x, ha, facA, invA) * szm.y;
+    const T hab = T(0.5) * (ha + hb);
+    const T fac = sph_h_dim(hab, dim) * sph_cubic_fac1<T>(dim);
     return sph_cubic_w_fac<T>(xa - sxy.x, ya - sxy.y, za - szm.x, hab, fac) * szm.y;
 }
 
@@ -46,6 +44,7 @@ __device__ __forceinline__ T rho_term(int dim, T xa, T ya, T za, T ha, T facA, P
 // ---------------------------------------------------------------------------
 template <typename T> struct MomDest {
     T xa, ya, za, ha, facA, rhoa, ca, Pa, ua, va, wa, rhoa2;
+    T inv_ha;  // sph_inv_for_div(ha)
 };
 
 // one neighbour term of MomentumEquation (pressure_funcs.pyx:252-391).
@@ -56,13 +55,15 @@ __device__ __forceinline__ void mom_term(int dim, const MomDest<T> &D, T alpha, 
                                          const P s0, const P s1, const P s2, const P s3,
                                          const P s4, T hb, T &dtf, T &nr0, T &nr1, T &nr2)
 {
-    T hab, fac;
+    T hab, fac, inv;
     if (UH || hb == D.ha) {  // 0.5*(h_a+h_a) == h_a: reuse the destination's h_dim(h)*sigma
         hab = D.ha;
         fac = D.facA;
+        inv = D.inv_ha;
     } else {
         hab = T(0.5) * (D.ha + hb);
         fac = sph_h_dim(hab, dim) * sph_cubic_fac1<T>(dim);
+        inv = T(0.0);
     }
     const T rabx = D.xa - s0.x;
     const T raby = D.ya - s0.y;
@@ -89,7 +90,7 @@ __device__ __forceinline__ void mom_term(int dim, const MomDest<T> &D, T alpha, 
     const T piab = dot < 0 ? pi_v : T(0.0);
     tmp += piab;
     tmp *= -mb;
-    const T g = sph_cubic_gfac_fac<T>(rabx, raby, rabz, hab, fac);
+    const T g = sph_cubic_gfac_fac<T>(rabx, raby, rabz, hab, fac, inv);
     nr0 += tmp * (rabx * g);
     nr1 += tmp * (raby * g);
     nr2 += tmp * (rabz * g);

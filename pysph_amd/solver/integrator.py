@@ -13,6 +13,14 @@ import ctypes
 from .. import _lib
 
 
+def _mark_stepped(array, stepped):
+    """Record the position versions this integrator's stage kernels left: the
+    cell manager keeps its grid only for positions written this way (a small
+    move per step; any other write of x, y or z rebuilds the grid)."""
+    if any(p in ('x', 'y', 'z') for p in stepped):
+        array._stepped_xyz = (array.version('x'), array.version('y'), array.version('z'))
+
+
 class Integrator:
 
     def __init__(self, particles=None, calcs=()):
@@ -134,6 +142,7 @@ class Integrator:
                     self.velocity_accelerations[array.name][k_num][v[1]] = array.get_carray(v[1])
                 if prop in ('x', 'y', 'z'):
                     array.set_dirty(True)
+            _mark_stepped(array, to_step)
         self.cstep += 1
 
     def get_max_acceleration(self, array, solver):
@@ -216,6 +225,7 @@ class PredictorCorrectorIntegrator(Integrator):
             array.touch(*to_step)
             if any(p in ('x', 'y', 'z') for p in to_step):
                 array.set_dirty(True)
+            _mark_stepped(array, to_step)
 
     def integrate(self, dt):
         self.save_initial_arrays()

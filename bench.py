@@ -220,7 +220,7 @@ def run_dambreak_split(args):
         copy_calibration(args.copy_calibration_mb)
     torch.cuda.synchronize()
     profiling.reset()
-    profiling.enable(True)
+    profiling.enable(True, drain=True)
     for _ in range(min(5, args.steps)):
         integ.integrate(dt)
     profiling.enable(False)
@@ -298,7 +298,7 @@ def run_dambreak(args):
     # outside the timed loop, events around every library call
     torch.cuda.synchronize()
     profiling.reset()
-    profiling.enable(True)
+    profiling.enable(True, drain=True)
     for _ in range(min(5, args.steps)):
         integ.integrate(dt)
     profiling.enable(False)
@@ -350,7 +350,9 @@ def run_dambreak(args):
             "kernel_launches_per_step": {k: v["count"] / min(5, args.steps)
                                          for k, v in breakdown.items()},
             "kernel_ms_note": "per library call, from an instrumented run of "
-                              f"{min(5, args.steps)} steps outside the timed loop",
+                              f"{min(5, args.steps)} steps outside the timed loop, the GPU "
+                              "drained before each call (its kernels plus their launch "
+                              "latency; nested calls count in both)",
         },
         "roofline": {
             "kernel": "k_momentum",
@@ -402,7 +404,7 @@ def run_lattice(args):
         step()
     torch.cuda.synchronize()
     profiling.reset()
-    profiling.enable(True)
+    profiling.enable(True, drain=True)
     for _ in range(min(5, args.steps)):
         step()
     profiling.enable(False)

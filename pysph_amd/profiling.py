@@ -11,18 +11,24 @@ import torch
 
 _enabled = False
 _only = None          # region names that record events (None: all)
+_drain = False        # synchronise before each region's start event
 _records = defaultdict(list)
 _pool = []            # timing events kept for reuse
 _pairs = 0
 
 
-def enable(on=True, regions=None):
+def enable(on=True, regions=None, drain=False):
     """Turn recording on or off; ``regions`` limits the event pairs to the
     named regions (pair counting is on either way), which keeps the
-    instrumentation of a timed loop to the kernels it measures."""
-    global _enabled, _only
+    instrumentation of a timed loop to the kernels it measures.  ``drain``
+    synchronises before each region, so that a region's time is its own
+    kernels plus their launch latency and never includes work queued before
+    it or the GPU's idle time while the host waits inside an earlier call
+    (for per-call breakdowns, not for timed loops)."""
+    global _enabled, _only, _drain
     _enabled = bool(on)
     _only = set(regions) if regions is not None else None
+    _drain = bool(drain)
 
 
 def _event():
@@ -70,6 +76,8 @@ def region(name):
         return
     s = _event()
     e = _event()
+    if _drain:
+        torch.cuda.synchronize()
     s.record()
     try:
         yield

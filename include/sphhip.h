@@ -220,7 +220,27 @@ int sph_nnps_build_dev(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
                        int64_t nd_real, const int32_t *s_cell_start, const double *s_rec,
                        int64_t ns, double radius_scale, int same_array, int32_t *counts,
                        int64_t *slice_off, int32_t *indices, int64_t cap, int64_t *totals,
-                       int32_t *status);
+                       int32_t *status, const uint8_t *s_near, int near_reach);
+
+/* Flags of the cells within `reach` cells (Chebyshev) of a non-empty cell of
+ * one array: near[key] = 1 for those, 0 for every other key of the grid
+ * (grid->ncells bytes, device).  The array's binning: s_cell_start, the
+ * list s_cells of its non-empty cells with their count in the device word
+ * *ns_cells_dev (bounded by ns_cells; NULL: ns_cells is the count), its
+ * permutation s_perm and its double coordinates in original order.  Passed
+ * as s_near with near_reach = reach to the device builds (cross-array lists
+ * only), it lets the count pass skip a destination cell with no source cell
+ * near after two loads: a table built once serves every build until that
+ * array is re-binned (walls).  The caller guarantees that no destination's
+ * stencil reach (int)ceil(h*radius_scale/cell_size) exceeds near_reach
+ * (e.g. from grid->max_h when no h changed since the grid was built); the
+ * cells are skipped on the flag alone.  With s_near NULL the builds read
+ * each stencil's cell ranges instead.  No counterpart in the reference,
+ * whose cell lists hold one array's cells (base/cell.pyx:742-1643). */
+int sph_cell_near(sph_ctx *ctx, const sph_grid *grid, const int32_t *s_cell_start,
+                  const int32_t *s_cells, const int32_t *ns_cells_dev, int64_t ns_cells,
+                  const int32_t *s_perm, const double *s_x, const double *s_y,
+                  const double *s_z, int reach, uint8_t *near);
 
 /* sph_nnps_build over float records (d_rec, s_rec with SPH_REC floats per
  * particle): the distance test d2 < (h*radius_scale)^2 is evaluated in
@@ -243,7 +263,7 @@ int sph_nnps_build_f32_dev(sph_ctx *ctx, const sph_grid *grid, const float *d_re
                            const int32_t *s_cell_start, const float *s_rec, int64_t ns,
                            double radius_scale, int same_array, int32_t *counts,
                            int64_t *slice_off, int32_t *indices, int64_t cap, int64_t *totals,
-                           int32_t *status);
+                           int32_t *status, const uint8_t *s_near, int near_reach);
 
 /* Symmetric variable-h lists of VarHNbrParticleLocator (base/nnps.pyx:
  * 691-786, chosen by NNPSManager(variable_h=True), :895-910), in the

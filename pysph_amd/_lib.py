@@ -144,10 +144,12 @@ _SIGNATURES = {
                                  _vp, _vp, _vp, _vp]),
     "sph_nnps_build_dev": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _vp, _vp, _i64,
                                   _i64, _i64, _vp, _vp, _i64, _dbl, _i32, _vp, _vp, _vp, _i64,
-                                  _vp, _vp]),
+                                  _vp, _vp, _vp, _i32]),
+    "sph_cell_near": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _i64, _vp, _vp, _vp,
+                             _vp, _i32, _vp]),
     "sph_nnps_build_f32_dev": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _vp, _vp,
                                       _vp, _vp, _vp, _i64, _i64, _i64, _vp, _vp, _i64, _dbl,
-                                      _i32, _vp, _vp, _vp, _i64, _vp, _vp]),
+                                      _i32, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _i32]),
     "sph_summation_density": (_i32, [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i32,
                                      _i32, _vp, _vp, _i64]),
     "sph_tait": (_i32, [_vp, _i64, _i64, _vp, _vp, _i64, _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp,

@@ -23,6 +23,8 @@ cell, own index last.
 (source, destination, radius_scale).
 """
 import ctypes
+import itertools
+import math
 
 import numpy as np
 import torch
@@ -36,6 +38,8 @@ REC_FIELDS = {"x": 0, "y": 1, "z": 2, "m": 3, "u": 4, "v": 5, "w": 6, "rho": 8, 
 PRHO2_FIELD = 7  # P/rho^2, written by the library (SPH_R_PRHO2)
 #: fields of the compact position-mass rows, in row order
 PM_FIELDS = ("x", "y", "z", "m")
+#: stamps of binnings: a _Bins' stamp changes whenever it is re-binned
+_BIN_STAMPS = itertools.count(1)
 SHADOW_PROPS = tuple(REC_FIELDS)
 
 
@@ -70,6 +74,7 @@ class _Bins:
         # of density and momentum touch fewer cache lines than in the records
         self.pm = None
         self.pm_ver = {}
+        self.stamp = 0
 
     def poll(self):
         """Take the device cell count of the last binning if it has arrived,
@@ -287,6 +292,7 @@ class CellManager:
                 ev = torch.cuda.Event()
                 ev.record()
                 b.pending = (ev, host)
+            b.stamp = next(_BIN_STAMPS)
             b.invalidate()
 
     def uniform_h(self):
@@ -457,10 +463,48 @@ class FixedDestNbrParticleLocator:
         self.nbuilds = 0
         self._export = None
         self._host = None
+        # cells near a source cell (sph_cell_near), for a source that was
+        # not re-binned between two builds (walls): key, table, reach
+        self._near_seen = None
+        self._near = None
+        self.near_used = False
 
     def update_status(self):
         if not self.is_dirty:
             self.is_dirty = self.source.is_dirty or self.dest.is_dirty
+
+    def _near_table(self, cm, bs):
+        """(table, reach) for a cross-array build over a source binning that
+        also served the previous build, else (None, 0): a source that moves
+        every step would rebuild the table at every build.  The kernels skip
+        cells on the flag alone, so the table is only handed over while no
+        destination h changed since the grid was built (every destination's
+        stencil reach is then at most the one of the grid's max_h)."""
+        if self.source is self.dest or bs.ncells_dev is None:
+            return None, 0
+        g = cm.grid
+        # the count pass's reach: (int)ceil(h*rs/cell), largest at max_h
+        reach = max(int(math.ceil(g.max_h * self.radius_scale / g.cell_size)), 1)
+        if cm._h_versions.get(self.dest.name) != self.dest.version("h") or reach > 2:
+            self._near_seen = None
+            return None, 0
+        key = (bs.stamp, _grid_signature(g), reach)
+        seen, self._near_seen = self._near_seen, key
+        if key != seen:
+            return None, 0
+        if self._near is not None and self._near[0] == key:
+            return self._near[1], self._near[2]
+        dev = self.dest.device
+        table = torch.empty(max(int(g.ncells), 1), dtype=torch.uint8, device=dev)
+        ctx = cm._ctx()
+        src = self.source
+        p = _lib.ptr
+        ctx.check(ctx.lib.sph_cell_near(
+            ctx.handle, ctypes.byref(g), p(bs.cell_start), p(bs.cells), p(bs.ncells_dev),
+            bs.cells_bound, p(bs.perm), p(src.get_carray("x")), p(src.get_carray("y")),
+            p(src.get_carray("z")), reach, p(table)))
+        self._near = (key, table, reach)
+        return table, reach
 
     @property
     def nnz(self):
@@ -548,6 +592,8 @@ class FixedDestNbrParticleLocator:
             if self._status is None:
                 self._status = torch.zeros(1, dtype=torch.int32, device=dev)
             totals = torch.empty(3, dtype=torch.int64, device=dev)
+            near, reach = self._near_table(cm, bs)
+            self.near_used = near is not None
             if cm.precision == "single":
                 rc = lib.sph_nnps_build_f32_dev(
                     ctx.handle, ctypes.byref(cm.grid), p(drec), p(dst.get_carray("x")),
@@ -555,14 +601,14 @@ class FixedDestNbrParticleLocator:
                     p(bd.cell_start), p(bd.cells), p(bd.ncells_dev), bd.cells_bound, nd_total,
                     nd_real, p(bs.cell_start), p(srec), bs.n, self.radius_scale,
                     int(src is dst), p(self.counts), p(self.slice_off), p(self.indices),
-                    self.indices.numel(), p(totals), p(self._status))
+                    self.indices.numel(), p(totals), p(self._status), p(near), reach)
             else:
                 rc = lib.sph_nnps_build_dev(
                     ctx.handle, ctypes.byref(cm.grid), p(drec), p(bd.perm), p(bd.cell_start),
                     p(bd.cells), p(bd.ncells_dev), bd.cells_bound, nd_total, nd_real,
                     p(bs.cell_start), p(srec), bs.n, self.radius_scale, int(src is dst),
                     p(self.counts), p(self.slice_off), p(self.indices), self.indices.numel(),
-                    p(totals), p(self._status))
+                    p(totals), p(self._status), p(near), reach)
             ctx.check(rc)
             host = torch.empty(3, dtype=torch.int64, pin_memory=True)
             host.copy_(totals, non_blocking=True)   # {slots, nnz, status}

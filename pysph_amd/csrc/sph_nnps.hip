@@ -898,6 +898,14 @@ struct CellArgs {
     int32_t *fcount;
     uint64_t *amasks;
     int32_t *candj;  // per cell: the original index of each candidate (cand's layout)
+    // rows whose appended part exceeds the fill's LDS column: their sorted
+    // destination positions, and how many (sorted by k_varh_long_rows)
+    int32_t *long_rows;
+    int32_t *nlong;
+    // per cell key: nonzero when a source cell lies within the stencil reach
+    // of every destination (sph_cell_near), or NULL; the count pass then
+    // skips a destination cell after two loads
+    const uint8_t *s_near;
 };
 
 __device__ __forceinline__ int wave_max_i(int v)
@@ -1009,6 +1017,9 @@ __device__ void nnps_cell(const CellArgs &a, int64_t w, int32_t *__restrict__ co
     const int lane = threadIdx.x;
     const int32_t key = a.dcells[w];
     if (key == a.trash) return;  // rows stay empty
+    // no source cell within reach: every row of the cell stays empty (counts
+    // start at 0) -- most fluid cells against a wall array
+    if (!FILL && a.s_near && !a.s_near[key]) return;
     const int32_t ds = a.dstart[key], de = a.dstart[key + 1];
     const GridDev g = a.grids ? a.grids[a.sim_of[a.dperm[ds]]] : a.g;
     // the cell's coordinates, recomputed exactly as the binning did (from
@@ -1049,7 +1060,7 @@ __device__ void nnps_cell(const CellArgs &a, int64_t w, int32_t *__restrict__ co
     const int z0 = max(cz - dist, 0), z1 = min(cz + dist, g.d2 - 1);
     const int nx = x1 - x0 + 1, ny = y1 - y0 + 1, nz = z1 - z0 + 1;
     const int nst = nx * ny * nz;
-    if (!FILL && !a.same) {
+    if (!FILL && !a.same && !a.s_near) {
         // no source in the stencil: every row of the cell stays empty (counts
         // start at 0) -- most fluid cells against a wall array.  (A cell's own
         // array always has its members in the stencil.)
@@ -1240,8 +1251,8 @@ __global__ void __launch_bounds__(CW)
 // own masks, own index not moved), then the appended part from a.amasks in
 // ascending original index (the count pass's candj table), sorted in the
 // lane's own column of the mask array once the own part no longer reads it
-// (VARH_COL entries; a longer part is sorted in place in the row) and
-// written as sorted positions (srank).  Rows of cells k_varh_rows handles are
+// (VARH_COL entries; a longer part is left to k_varh_long_rows) and written
+// as sorted positions (srank).  Rows of cells k_varh_rows handles are
 // skipped.
 #define VARH_COL (2 * NN_MAXCH)
 template <bool VARH>
@@ -1407,7 +1418,8 @@ __global__ void __launch_bounds__(NN_THREADS)
         for (i = 0; i < na; ++i) row[(int64_t)(n + i) << 6] = (int32_t)slot(i);
         return;
     }
-    // a long appended part: sorted in place in the row
+    // a long appended part: its original indices written unsorted, the row
+    // listed for k_varh_long_rows (one wavefront sorts it in LDS)
     int32_t e = 0;
     for (int c2 = 0; c2 < nch; ++c2) {
         uint64_t am = a.amasks[(int64_t)c2 * a.mstride + t];
@@ -1418,19 +1430,62 @@ __global__ void __launch_bounds__(NN_THREADS)
             ++e;
         }
     }
-    const int32_t na = min(e, ne);
-    for (int32_t i = 1; i < na; ++i) {
-        const int32_t v = row[(int64_t)(n + i) << 6];
-        int32_t q2 = i - 1;
-        while (q2 >= 0 && row[(int64_t)(n + q2) << 6] > v) {
-            row[(int64_t)(n + q2 + 1) << 6] = row[(int64_t)(n + q2) << 6];
-            --q2;
+    a.long_rows[atomicAdd(a.nlong, 1)] = (int32_t)t;
+}
+
+// The appended parts the fill left unsorted (longer than its LDS column): one
+// wavefront per listed row sorts the original indices ascending (bitonic, in
+// LDS, up to VARH_LONG entries; a longer part by one lane in place) and
+// writes them back as sorted positions (srank).
+#define VARH_LONG 1024
+__global__ void __launch_bounds__(CW)
+    k_varh_long_rows(CellArgs a, const int32_t *__restrict__ counts,
+                     const int64_t *__restrict__ slice_off, int32_t *__restrict__ indices,
+                     const int32_t *__restrict__ srank)
+{
+    __shared__ int32_t s_v[VARH_LONG];
+    const int lane = threadIdx.x;
+    const int32_t nl = *a.nlong;
+    for (int32_t r = blockIdx.x; r < nl; r += gridDim.x) {
+        const int32_t t = a.long_rows[r];
+        const int32_t n = a.fcount[t];
+        const int32_t ne = counts[t] - n;
+        int32_t *app = indices + slice_off[t >> 6] + (t & 63) + ((int64_t)n << 6);
+        if (ne > VARH_LONG) {  // one lane, in place
+            if (lane == 0) {
+                for (int32_t i = 1; i < ne; ++i) {
+                    const int32_t v = app[(int64_t)i << 6];
+                    int32_t q2 = i - 1;
+                    while (q2 >= 0 && app[(int64_t)q2 << 6] > v) {
+                        app[(int64_t)(q2 + 1) << 6] = app[(int64_t)q2 << 6];
+                        --q2;
+                    }
+                    app[(int64_t)(q2 + 1) << 6] = v;
+                }
+                for (int32_t i = 0; i < ne; ++i) app[(int64_t)i << 6] = srank[app[(int64_t)i << 6]];
+            }
+            continue;
         }
-        row[(int64_t)(n + q2 + 1) << 6] = v;
-    }
-    for (int32_t i = 0; i < na; ++i) {
-        int32_t *ep = row + ((int64_t)(n + i) << 6);
-        *ep = srank[*ep];
+        int32_t P = CW;
+        while (P < ne) P <<= 1;
+        for (int32_t i = lane; i < P; i += CW) s_v[i] = i < ne ? app[(int64_t)i << 6] : INT32_MAX;
+        __syncthreads();
+        for (int32_t k = 2; k <= P; k <<= 1)
+            for (int32_t j = k >> 1; j > 0; j >>= 1) {
+                for (int32_t i = lane; i < P; i += CW) {
+                    const int32_t ixj = i ^ j;
+                    if (ixj > i) {
+                        const int32_t x = s_v[i], y = s_v[ixj];
+                        if ((x > y) == ((i & k) == 0)) {
+                            s_v[i] = y;
+                            s_v[ixj] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        for (int32_t i = lane; i < ne; i += CW) app[(int64_t)i << 6] = srank[s_v[i]];
+        __syncthreads();  // s_v is reused by the next row
     }
 }
 
@@ -1620,9 +1675,12 @@ static int nnps_build(sph_ctx *ctx, const sph_grid *grid, int f32, const double 
                       int64_t nd_real, const int32_t *s_cell_start, const double *s_rec,
                       const float *s_rec32, int64_t ns, double radius_scale, int same_array,
                       int32_t *counts, int64_t *slice_off, int32_t *indices, int64_t cap,
-                      int64_t *slots_out, int64_t *nnz_out, const NnpsDevOut *dev)
+                      int64_t *slots_out, int64_t *nnz_out, const NnpsDevOut *dev,
+                      const uint8_t *s_near = nullptr, int near_reach = 0)
 {
     if (!ctx) return SPH_EINVAL;
+    SPH_REQUIRE(ctx, !s_near || (near_reach >= 1 && near_reach <= 8),
+                "sph_nnps_build: near_reach must be in [1, 8] with a near-cells table");
     SPH_REQUIRE(ctx, grid && counts && slice_off &&
                          (dev ? (dev->ncells_dev && dev->totals && dev->status && indices)
                               : (slots_out && nnz_out)),
@@ -1672,6 +1730,7 @@ static int nnps_build(sph_ctx *ctx, const sph_grid *grid, int f32, const double 
     a.srec = s_rec;
     a.rs = radius_scale;
     a.same = same_array;
+    a.s_near = same_array ? nullptr : s_near;
     if (dev) {
         a.ncell_dev = dev->ncells_dev;
         a.cap = cap;
@@ -1725,13 +1784,14 @@ extern "C" int sph_nnps_build_dev(sph_ctx *ctx, const sph_grid *grid, const doub
                                   const int32_t *s_cell_start, const double *s_rec, int64_t ns,
                                   double radius_scale, int same_array, int32_t *counts,
                                   int64_t *slice_off, int32_t *indices, int64_t cap,
-                                  int64_t *totals, int32_t *status)
+                                  int64_t *totals, int32_t *status, const uint8_t *s_near,
+                                  int near_reach)
 {
     const NnpsDevOut dev = {nd_cells_dev, totals, status};
     return nnps_build(ctx, grid, 0, d_rec, nullptr, nullptr, nullptr, nullptr, d_perm,
                       d_cell_start, d_cells, max_cells, nd_total, nd_real, s_cell_start, s_rec,
                       nullptr, ns, radius_scale, same_array, counts, slice_off, indices, cap,
-                      nullptr, nullptr, &dev);
+                      nullptr, nullptr, &dev, s_near, near_reach);
 }
 
 extern "C" int sph_nnps_build_f32(sph_ctx *ctx, const sph_grid *grid, const float *d_rec,
@@ -1758,13 +1818,76 @@ extern "C" int sph_nnps_build_f32_dev(sph_ctx *ctx, const sph_grid *grid, const 
                                       const int32_t *s_cell_start, const float *s_rec,
                                       int64_t ns, double radius_scale, int same_array,
                                       int32_t *counts, int64_t *slice_off, int32_t *indices,
-                                      int64_t cap, int64_t *totals, int32_t *status)
+                                      int64_t cap, int64_t *totals, int32_t *status,
+                                      const uint8_t *s_near, int near_reach)
 {
     const NnpsDevOut dev = {nd_cells_dev, totals, status};
     return nnps_build(ctx, grid, 1, nullptr, d_rec, d_x, d_y, d_z, d_perm, d_cell_start, d_cells,
                       max_cells, nd_total, nd_real, s_cell_start, nullptr, s_rec, ns,
                       radius_scale, same_array, counts, slice_off, indices, cap, nullptr,
-                      nullptr, &dev);
+                      nullptr, &dev, s_near, near_reach);
+}
+
+// ---------------------------------------------------------------------------
+// cells within reach of a source array
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(NN_THREADS)
+    k_fill_u8(uint8_t *__restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * NN_THREADS)
+        p[i] = 0;
+}
+
+// one lane per non-empty source cell: the cell's coordinates from its first
+// particle (the double coordinates the binning used), then near = 1 for
+// every cell of the box of half-width `reach` around it
+__global__ void __launch_bounds__(NN_THREADS)
+    k_cell_near(GridDev g, const int32_t *__restrict__ cells, const int32_t *__restrict__ ncells_dev,
+                int64_t ncells_max, const int32_t *__restrict__ cell_start,
+                const int32_t *__restrict__ perm, const double *__restrict__ x,
+                const double *__restrict__ y, const double *__restrict__ z, int reach,
+                uint8_t *__restrict__ near)
+{
+    int64_t nc = ncells_max;
+    if (ncells_dev && (int64_t)*ncells_dev < nc) nc = *ncells_dev;
+    const int64_t w = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
+    if (w >= nc) return;
+    const int32_t i = perm[cell_start[cells[w]]];
+    const int cx = min(max(cell_coord(x[i], g.cs) - g.cmin0, 0), g.d0 - 1);
+    const int cy = min(max(cell_coord(y[i], g.cs) - g.cmin1, 0), g.d1 - 1);
+    const int cz = min(max(cell_coord(z[i], g.cs) - g.cmin2, 0), g.d2 - 1);
+    for (int ix = max(cx - reach, 0); ix <= min(cx + reach, g.d0 - 1); ++ix)
+        for (int iy = max(cy - reach, 0); iy <= min(cy + reach, g.d1 - 1); ++iy)
+            for (int iz = max(cz - reach, 0); iz <= min(cz + reach, g.d2 - 1); ++iz)
+                near[cell_key(g, ix, iy, iz)] = 1;
+}
+
+extern "C" int sph_cell_near(sph_ctx *ctx, const sph_grid *grid, const int32_t *s_cell_start,
+                             const int32_t *s_cells, const int32_t *ns_cells_dev,
+                             int64_t ns_cells, const int32_t *s_perm, const double *s_x,
+                             const double *s_y, const double *s_z, int reach, uint8_t *near)
+{
+    if (!ctx) return SPH_EINVAL;
+    SPH_REQUIRE(ctx, grid && near && reach >= 0 && reach <= 8 && ns_cells >= 0,
+                "sph_cell_near: bad argument");
+    SPH_REQUIRE(ctx, ns_cells == 0 || (s_cell_start && s_cells && s_perm && s_x && s_y && s_z),
+                "sph_cell_near: null source arrays");
+    int rc = sph_upload_key_tables(ctx, grid);
+    if (rc) return rc;
+    const GridDev g = make_griddev(ctx, grid);
+    const int64_t nk = grid->ncells;
+    int64_t fb = (nk + NN_THREADS - 1) / NN_THREADS;
+    if (fb > 4096) fb = 4096;
+    hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)(fb > 0 ? fb : 1)), dim3(NN_THREADS), 0,
+                       ctx->stream, near, nk);
+    SPH_LAUNCH_CHECK(ctx, "k_fill_u8");
+    if (ns_cells == 0) return SPH_OK;
+    hipLaunchKernelGGL(k_cell_near, dim3(sph_blocks(ns_cells, NN_THREADS)), dim3(NN_THREADS), 0,
+                       ctx->stream, g, s_cells, ns_cells_dev, ns_cells, s_cell_start, s_perm,
+                       s_x, s_y, s_z, reach, near);
+    SPH_LAUNCH_CHECK(ctx, "k_cell_near");
+    return SPH_OK;
 }
 
 int sph_nnps_build_batch(sph_ctx *ctx, const GridDev *grids, const int32_t *sim_of,
@@ -2218,9 +2341,10 @@ __global__ void __launch_bounds__(NN_THREADS)
 
 // Count pass per destination cell (k_varh_cells; the destinations of flagged
 // cells by k_varh_rows), slice offsets and totals, the host check of the
-// capacity, then the fill.  Scratch: [own masks | appended masks | cell flags
-// | candidate sources | cell of destination | own-query entries | candidate
-// original indices | slice sums | scan partials].
+// capacity, then the fill and the sort of long appended parts.  Scratch:
+// [own masks | appended masks | cell flags | candidate sources | cell of
+// destination | own-query entries | candidate original indices | long-row
+// count and list | slice sums | scan partials].
 template <typename T>
 static int nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const T *d_rec, const double *d_x,
                            const double *d_y, const double *d_z, const int32_t *d_perm,
@@ -2267,8 +2391,9 @@ static int nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const T *d_rec, c
     const size_t dcell_bytes = al((size_t)nd_total * sizeof(int32_t));
     const size_t fc_bytes = al((size_t)nd_total * sizeof(int32_t));
     const size_t sums_bytes = al((size_t)(nslices + 1) * sizeof(int64_t) + 256);
+    const size_t long_bytes = al((size_t)(nd_total + 1) * sizeof(int32_t));
     const size_t sums_off =
-        2 * masks_bytes + ovf_bytes + 2 * cand_bytes + dcell_bytes + fc_bytes;
+        2 * masks_bytes + ovf_bytes + 2 * cand_bytes + dcell_bytes + fc_bytes + long_bytes;
     rc = sph_ensure_scratch(ctx, sums_off + sums_bytes +
                                      sph_scan_scratch_bytes(nslices, sizeof(int64_t)));
     if (rc) return rc;
@@ -2306,6 +2431,9 @@ static int nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const T *d_rec, c
     c.dcell_of = (int32_t *)(sc + 2 * masks_bytes + ovf_bytes + cand_bytes);
     c.fcount = (int32_t *)(sc + 2 * masks_bytes + ovf_bytes + cand_bytes + dcell_bytes);
     c.candj = (int32_t *)(sc + 2 * masks_bytes + ovf_bytes + cand_bytes + dcell_bytes + fc_bytes);
+    c.nlong = (int32_t *)(sc + 2 * masks_bytes + ovf_bytes + 2 * cand_bytes + dcell_bytes +
+                          fc_bytes);
+    c.long_rows = c.nlong + 1;
     int64_t *slice_sum = (int64_t *)(sc + sums_off);
     int64_t *nnz_dev = slice_sum + nslices;
     // the rows of flagged cells: the lane-per-destination kernel
@@ -2378,9 +2506,14 @@ static int nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const T *d_rec, c
                       (long long)slots);
         return SPH_ENEEDCAP;
     }
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, c.nlong, 1, 0);
+    SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
     hipLaunchKernelGGL(k_nnps_fill_slices<true>, dim3(dgrid), dim3(NN_THREADS), 0, ctx->stream,
                        c, nd_total, counts, (const int64_t *)slice_off, indices, s_rank);
     SPH_LAUNCH_CHECK(ctx, "k_nnps_fill_slices<varh>");
+    hipLaunchKernelGGL(k_varh_long_rows, dim3(4096), dim3(CW), 0, ctx->stream, c,
+                       (const int32_t *)counts, (const int64_t *)slice_off, indices, s_rank);
+    SPH_LAUNCH_CHECK(ctx, "k_varh_long_rows");
     auto fill_rows = k_varh_rows<true, T>;
     hipLaunchKernelGGL(fill_rows, dim3(dgrid), dim3(NN_THREADS), 0, ctx->stream, a, counts,
                        (const int64_t *)slice_off, indices, (const int32_t *)c.dcell_of,

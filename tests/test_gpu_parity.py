@@ -402,6 +402,65 @@ def test_device_side_rebuild_matches_and_reports_capacity(gpu):
         _ = loc.nnz
 
 
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_cross_array_lists_with_near_cells_table(gpu, precision):
+    """Fluid <- wall lists of the 3D dam break (dx = 0.04) over builds that
+    find the wall binning unchanged: from the third build on, the count pass
+    skips a fluid cell with no wall cell in reach by one flag per cell
+    (sph_cell_near).  The lists of every such build equal the ones of the
+    same build scanning every stencil, and in FP64 the oracle's."""
+    base, _, _ = _api()
+    from pysph_amd.solver.generators import dam_break_3d
+    fl, wl, prm = dam_break_3d(dx=0.04)
+    fa = base.get_particle_array(name="fluid", cl_precision=precision, **fl)
+    wa = base.get_particle_array(name="wall", cl_precision=precision, **wl)
+    particles = base.Particles([fa, wa])
+    loc = particles.nnps_manager.get_neighbor_particle_locator(wa, fa, 2.0)
+    rng = np.random.default_rng(5)
+    x0 = {k: fa.get(k) for k in ("x", "y", "z")}
+    used = 0
+    for it in range(5):
+        off, idx = (t.cpu().numpy() for t in loc.get_csr())
+        with_table = loc.near_used
+        used += with_table
+        # the same build without the table
+        near_table = loc._near_table
+        loc._near_table = lambda cm, bs: (None, 0)
+        loc.is_dirty = True
+        off1, idx1 = (t.cpu().numpy() for t in loc.get_csr())
+        del loc._near_table
+        assert loc._near_table == near_table
+        assert np.array_equal(off, off1) and np.array_equal(idx, idx1), it
+        if precision == "double":
+            d = {k: fa.get(k) for k in ("x", "y", "z", "h")}
+            s = {k: wa.get(k) for k in ("x", "y", "z")}
+            roff, ridx = cpu_ref.nnps(d, s, 2.0, particles.cell_manager.cell_size, False,
+                                      nthreads=8)
+            assert np.array_equal(off, roff) and np.array_equal(idx, ridx), it
+        empty = int(np.sum(np.diff(off) == 0))
+        print(f"build {it}: near table {'used' if with_table else 'not used'}, {len(idx)} "
+              f"entries, {empty} of {len(off) - 1} fluid rows empty")
+        # move the fluid by up to a third of a spacing (the walls fix the grid)
+        fa.set(**{k: x0[k] + rng.uniform(-1.0, 1.0, x0[k].size) * prm["dx"] / 3
+                  for k in ("x", "y", "z")})
+    assert used >= 2
+    # a larger fluid h written after the grid was built: stencils of reach 2,
+    # so the table (made for reach 1) must not be handed over
+    particles.update()
+    loc.get_csr()
+    fa.set(h=fa.get("h") * 1.5)
+    loc.is_dirty = True
+    off, idx = (t.cpu().numpy() for t in loc.get_csr())
+    assert not loc.near_used
+    if precision == "double":
+        d = {k: fa.get(k) for k in ("x", "y", "z", "h")}
+        s = {k: wa.get(k) for k in ("x", "y", "z")}
+        roff, ridx = cpu_ref.nnps(d, s, 2.0, particles.cell_manager.cell_size, False,
+                                  nthreads=8)
+        assert np.array_equal(off, roff) and np.array_equal(idx, ridx)
+    print(f"fluid h x 1.5 after the grid build: {len(idx)} entries, no near table")
+
+
 def test_lists_exact_at_the_support_radius(gpu):
     """Pairs exactly at, and within a few FP64 ulps of, the support radius,
     far from the origin: the float distance test of the list kernel leaves

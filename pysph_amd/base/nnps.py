@@ -40,6 +40,26 @@ PRHO2_FIELD = 7  # P/rho^2, written by the library (SPH_R_PRHO2)
 PM_FIELDS = ("x", "y", "z", "m")
 #: stamps of binnings: a _Bins' stamp changes whenever it is re-binned
 _BIN_STAMPS = itertools.count(1)
+
+
+class _PinnedSlots:
+    """A few page-locked host buffers for the sizes and status words the
+    device-side builds send back, allocated once: steady-state builds make no
+    host allocation.  take() returns the next buffer and the event of its
+    copy; a buffer whose last copy has not finished is waited for first."""
+
+    def __init__(self, n, numel, dtype):
+        self.bufs = torch.empty((n, numel), dtype=dtype, pin_memory=True)
+        self.events = [None] * n
+        self.i = 0
+
+    def take(self):
+        k = self.i
+        self.i = (k + 1) % len(self.events)
+        if self.events[k] is not None:
+            self.events[k].synchronize()
+        self.events[k] = torch.cuda.Event()
+        return self.bufs[k], self.events[k]
 SHADOW_PROPS = tuple(REC_FIELDS)
 
 
@@ -75,6 +95,7 @@ class _Bins:
         self.pm = None
         self.pm_ver = {}
         self.stamp = 0
+        self.host_slots = None
 
     def poll(self):
         """Take the device cell count of the last binning if it has arrived,
@@ -144,6 +165,7 @@ class CellManager:
         self._grid_key = None
         self._grid_flags = None
         self._grid_pending = None
+        self._grid_host = None
         self.grid_builds = 0
         if initialize:
             self.initialize()
@@ -241,7 +263,10 @@ class CellManager:
                 (ctypes.c_void_p * m)(*[zs[k] for k in moved]),
                 (ctypes.c_int64 * m)(*[ns[k] for k in moved]), ctypes.byref(self.grid),
                 self.GRID_MARGIN, _lib.ptr(self._grid_flags)))
-            host = torch.empty(1, dtype=torch.int32, pin_memory=True)
+            # the last check was read above (_poll_grid_check): its buffer is free
+            if self._grid_host is None:
+                self._grid_host = torch.empty(1, dtype=torch.int32, pin_memory=True)
+            host = self._grid_host
             host.copy_(self._grid_flags, non_blocking=True)
             ev = torch.cuda.Event()
             ev.record()
@@ -287,9 +312,10 @@ class CellManager:
             else:
                 b.poll()
                 ctx.check(lib.sph_bin_array_dev(*args, _lib.ptr(b.ncells_dev)))
-                host = torch.empty(1, dtype=torch.int32, pin_memory=True)
+                if b.host_slots is None:
+                    b.host_slots = _PinnedSlots(4, 1, torch.int32)
+                host, ev = b.host_slots.take()
                 host.copy_(b.ncells_dev, non_blocking=True)
-                ev = torch.cuda.Event()
                 ev.record()
                 b.pending = (ev, host)
             b.stamp = next(_BIN_STAMPS)
@@ -468,6 +494,7 @@ class FixedDestNbrParticleLocator:
         self._near_seen = None
         self._near = None
         self.near_used = False
+        self._host_slots = None
 
     def update_status(self):
         if not self.is_dirty:
@@ -501,8 +528,8 @@ class FixedDestNbrParticleLocator:
         p = _lib.ptr
         ctx.check(ctx.lib.sph_cell_near(
             ctx.handle, ctypes.byref(g), p(bs.cell_start), p(bs.cells), p(bs.ncells_dev),
-            bs.cells_bound, p(bs.perm), p(src.get_carray("x")), p(src.get_carray("y")),
-            p(src.get_carray("z")), reach, p(table)))
+            bs.cells_bound if bs.n > 0 else 0, p(bs.perm), p(src.get_carray("x")),
+            p(src.get_carray("y")), p(src.get_carray("z")), reach, p(table)))
         self._near = (key, table, reach)
         return table, reach
 
@@ -610,9 +637,12 @@ class FixedDestNbrParticleLocator:
                     p(self.counts), p(self.slice_off), p(self.indices), self.indices.numel(),
                     p(totals), p(self._status), p(near), reach)
             ctx.check(rc)
-            host = torch.empty(3, dtype=torch.int64, pin_memory=True)
+            if self._host_slots is None:
+                self._host_slots = _PinnedSlots(4, 3, torch.int64)
+            if len(self._pending) >= 3:
+                self._poll(block=True)  # no buffer is reused before it was read
+            host, ev = self._host_slots.take()
             host.copy_(totals, non_blocking=True)   # {slots, nnz, status}
-            ev = torch.cuda.Event()
             ev.record()
             self._totals = totals
             self._pending.append((ev, host))

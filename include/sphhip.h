@@ -414,6 +414,25 @@ int sph_summation_density_multi_f32(sph_ctx *ctx, int dim, int64_t nd_total, int
                                     const int32_t *d_perm, const sph_sources *src,
                                     const float *d_rec, int uniform_h, int accumulate,
                                     double *rho, float *out_s, int64_t out_s_stride);
+/* sph_summation_density_multi followed by sph_tait on the same destination
+ * array, in one launch: the density calc and the equation-of-state calc of
+ * one evaluation (SPHCalc.sph of SPHRho then of TaitEquation,
+ * sph/sph_calc.pyx:405-440, funcs/density_funcs.pyx:37-73,
+ * funcs/eos_funcs.pyx:118-129).  Results equal the two calls in turn bit for
+ * bit: Tait takes the density just written to the record (out_s, required),
+ * and its record fields p_s, cs_s, prho2_s share the record stride. */
+int sph_summation_density_tait(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                               const int32_t *d_perm, const sph_sources *src,
+                               const double *d_rec, int uniform_h, int accumulate,
+                               double *rho, double *out_s, int64_t out_s_stride, double ro,
+                               double co, double gamma, int eos_accumulate, double *p,
+                               double *cs, double *p_s, double *cs_s, double *prho2_s);
+int sph_summation_density_tait_f32(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                                   const int32_t *d_perm, const sph_sources *src,
+                                   const float *d_rec, int uniform_h, int accumulate,
+                                   double *rho, float *out_s, int64_t out_s_stride, double ro,
+                                   double co, double gamma, int eos_accumulate, double *p,
+                                   double *cs, float *p_s, float *cs_s, float *prho2_s);
 int sph_momentum_av_multi(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
                           const int32_t *d_perm, const sph_sources *src, const double *d_rec,
                           int uniform_h, double alpha, double beta, double eta, int accumulate,

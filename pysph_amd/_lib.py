@@ -162,6 +162,13 @@ _SIGNATURES = {
     "sph_summation_density_multi_f32": (_i32, [_vp, _i32, _i64, _i64, _vp,
                                                ctypes.POINTER(Sources), _vp, _i32, _i32, _vp,
                                                _vp, _i64]),
+    "sph_summation_density_tait": (_i32, [_vp, _i32, _i64, _i64, _vp, ctypes.POINTER(Sources),
+                                          _vp, _i32, _i32, _vp, _vp, _i64, _dbl, _dbl, _dbl,
+                                          _i32, _vp, _vp, _vp, _vp, _vp]),
+    "sph_summation_density_tait_f32": (_i32, [_vp, _i32, _i64, _i64, _vp,
+                                              ctypes.POINTER(Sources), _vp, _i32, _i32, _vp,
+                                              _vp, _i64, _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp,
+                                              _vp, _vp]),
     "sph_momentum_av_multi": (_i32, [_vp, _i32, _i64, _i64, _vp, ctypes.POINTER(Sources), _vp,
                                      _i32, _dbl, _dbl, _dbl, _i32, _vp, _vp, _vp, _vp]),
     "sph_momentum_av_multi_f32": (_i32, [_vp, _i32, _i64, _i64, _vp, ctypes.POINTER(Sources),

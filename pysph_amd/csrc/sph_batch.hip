@@ -1015,12 +1015,13 @@ int batch_lists(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w)
     hipLaunchKernelGGL(k_batch_key_base, dim3(1), dim3(1024), 0, ctx->stream, d->nsims, w.ncell,
                        d->key_cap, w.grids, d->status);
     SPH_LAUNCH_CHECK(ctx, "k_batch_key_base");
+    // cell counts into w.slot (sph_bin_sort turns them into w.cell_start)
     hipLaunchKernelGGL(k_fill_i32, dim3(sph_blocks(d->key_cap + 1, BT_THREADS)),
-                       dim3(BT_THREADS), 0, ctx->stream, w.cell_start, d->key_cap + 1, 0);
+                       dim3(BT_THREADS), 0, ctx->stream, w.slot, d->key_cap + 1, 0);
     SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
     hipLaunchKernelGGL(k_batch_keys, dim3(sph_blocks(n, BT_THREADS)), dim3(BT_THREADS), 0,
                        ctx->stream, w.grids, d->sim_of, d->x, d->y, d->z, n, d->key_cap,
-                       w.cell_start, w.cell_of, w.rank, d->status);
+                       w.slot, w.cell_of, w.rank, d->status);
     SPH_LAUNCH_CHECK(ctx, "k_batch_keys");
     const int32_t trash = (int32_t)(d->key_cap - 1);
     rc = sph_bin_sort(ctx, n, d->key_cap, w.cell_of, w.rank, w.cell_start, w.perm, w.cells,

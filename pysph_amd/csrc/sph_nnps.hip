@@ -415,39 +415,70 @@ __global__ void __launch_bounds__(NN_THREADS)
     tmp[start[cell_of[i]] + slot[i]] = (int32_t)i;
 }
 
-// Cells with at most RANK_DIRECT members are ordered by counting (each
-// member scans its cell); larger ones by k_bin_rank_large, a merge sort of
-// the cell's members, so a cell holding most of the particles (a large h
-// sets the cell size, or a clustered input) costs O(m log^2 m), not O(m^2).
+// Cells with at most 64 members are sorted in a wavefront's registers, up to
+// RANK_DIRECT members ordered by counting (each member scans its cell); larger
+// ones by k_bin_rank_large, a merge sort of the cell's members, so a cell
+// holding most of the particles (a large h sets the cell size, or a clustered
+// input) costs O(m log^2 m), not O(m^2).
 #define RANK_DIRECT 1024
 #define RANK_CHUNK 2048        // members sorted in LDS per pass of k_bin_rank_large
 #define RANK_LARGE_GRID 256    // blocks of k_bin_rank_large (one large cell at a time each)
 
 // Ascending particle index inside every cell (the atomics above hand out
-// slots in arbitrary order): particle i's position is the number of members
-// of its cell with a smaller index.  Writes perm and its inverse rank.  The
-// members of a cell with more than RANK_DIRECT of them are left to
-// k_bin_rank_large: the member holding slot 0 lists the cell there.
+// slots in arbitrary order), one wavefront per non-empty cell (the compact
+// cell list and its count from the binning scan): a cell of at most 64
+// members is sorted across the lanes (bitonic, in registers), a larger one
+// ranked by counting (a member's position is the number of members with a
+// smaller index), lane by lane; cells of more than RANK_DIRECT members are
+// listed for k_bin_rank_large.  Writes perm and its inverse rank.  Members
+// of the trash cell keep their atomic slots.
+#define RANK_CELL_GRID 16384  // blocks of the per-cell ranking (4 wavefronts each)
 __global__ void __launch_bounds__(NN_THREADS)
-    k_bin_rank(const int32_t *__restrict__ cell_of, int64_t n, const int32_t *__restrict__ start,
-               const int32_t *__restrict__ tmp, int32_t trash, int32_t *__restrict__ perm,
-               int32_t *__restrict__ rank, int32_t *__restrict__ large, int32_t *__restrict__ nlarge)
+    k_bin_rank_cells(const int32_t *__restrict__ cells, const int32_t *__restrict__ ncells_dev,
+                     const int32_t *__restrict__ start, const int32_t *__restrict__ tmp,
+                     int32_t trash, int32_t *__restrict__ perm, int32_t *__restrict__ rank,
+                     int32_t *__restrict__ large, int32_t *__restrict__ nlarge)
 {
-    int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
-    if (i >= n) return;
-    const int32_t key = cell_of[i];
-    const int32_t s = start[key], e = start[key + 1];
-    int32_t r = 0;
-    if (key == trash) {
-        r = rank[i];  // unordered: the atomic slot (no lists are built for this cell)
-    } else if (e - s > RANK_DIRECT) {
-        if (rank[i] == 0) large[atomicAdd(nlarge, 1)] = key;  // slot 0: once per cell
-        return;
-    } else {
-        for (int32_t a = s; a < e; ++a) r += tmp[a] < (int32_t)i;
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = ((int64_t)gridDim.x * NN_THREADS) >> 6;
+    const int32_t nce = *ncells_dev;
+    for (int64_t w = ((int64_t)blockIdx.x * NN_THREADS + threadIdx.x) >> 6; w < nce; w += nw) {
+        const int32_t key = cells[w];
+        const int32_t s = start[key], m = start[key + 1] - s;
+        if (key == trash) {  // unordered: the atomic slots (no lists are built for this cell)
+            for (int32_t k = lane; k < m; k += 64) {
+                const int32_t v = tmp[s + k];
+                perm[s + k] = v;
+                rank[v] = s + k;
+            }
+        } else if (m <= 64) {
+            int32_t v = lane < m ? tmp[s + lane] : INT32_MAX;
+#pragma unroll
+            for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    const int32_t o = __shfl_xor(v, j, 64);
+                    // ascending runs of length k; the lower lane of a pair keeps
+                    // the smaller value in an ascending run, the larger otherwise
+                    v = (((lane & j) == 0) == ((lane & k) == 0)) ? min(v, o) : max(v, o);
+                }
+            }
+            if (lane < m) {
+                perm[s + lane] = v;
+                rank[v] = s + lane;
+            }
+        } else if (m > RANK_DIRECT) {
+            if (lane == 0) large[atomicAdd(nlarge, 1)] = key;
+        } else {
+            for (int32_t k = lane; k < m; k += 64) {
+                const int32_t v = tmp[s + k];
+                int32_t r = 0;
+                for (int32_t a = 0; a < m; ++a) r += tmp[s + a] < v;
+                perm[s + r] = v;
+                rank[v] = s + r;
+            }
+        }
     }
-    perm[s + r] = (int32_t)i;
-    rank[i] = s + r;
 }
 
 // number of the n ascending values of a that are smaller than v
@@ -531,6 +562,123 @@ __global__ void __launch_bounds__(NN_THREADS)
     }
 }
 
+// Cell starts and the list of non-empty cells from the cell counts, in two
+// launches: per-tile totals (members, non-empty cells), then each tile adds
+// the totals of the tiles before it to its own scan and writes the starts,
+// its non-empty cells in key order and, in the last tile, the totals.
+// Out of place (counts are not overwritten); up to BSCAN_MAX_TILES tiles,
+// beyond that the multi-level scans run instead.
+#define BSCAN_THREADS 256
+#define BSCAN_ITEMS 16
+#define BSCAN_TILE (BSCAN_THREADS * BSCAN_ITEMS)
+#define BSCAN_MAX_TILES 2048
+
+__global__ void __launch_bounds__(NN_THREADS)
+    k_copy_i32(const int32_t *__restrict__ src, int64_t n, int32_t *__restrict__ dst)
+{
+    const int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// (members, non-empty cells) of a block's BSCAN_TILE cells, summed over the block
+__device__ __forceinline__ int2 bscan_block_sum(int32_t a, int32_t b, int2 *s_part)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        a += __shfl_xor(a, d, 64);
+        b += __shfl_xor(b, d, 64);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) s_part[wave] = make_int2(a, b);
+    __syncthreads();
+    int2 t = make_int2(0, 0);
+    for (int q = 0; q < BSCAN_THREADS / 64; ++q) {
+        t.x += s_part[q].x;
+        t.y += s_part[q].y;
+    }
+    __syncthreads();
+    return t;
+}
+
+__global__ void __launch_bounds__(BSCAN_THREADS)
+    k_bin_tile_sums(const int32_t *__restrict__ counts, int64_t nc, int2 *__restrict__ tile_sum)
+{
+    __shared__ int2 s_part[BSCAN_THREADS / 64];
+    const int64_t base = (int64_t)blockIdx.x * BSCAN_TILE + (int64_t)threadIdx.x * BSCAN_ITEMS;
+    int32_t a = 0, b = 0;
+#pragma unroll
+    for (int k = 0; k < BSCAN_ITEMS; ++k) {
+        const int32_t c = base + k < nc ? counts[base + k] : 0;
+        a += c;
+        b += c > 0;
+    }
+    const int2 t = bscan_block_sum(a, b, s_part);
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(BSCAN_THREADS)
+    k_bin_tile_scan(const int32_t *__restrict__ counts, int64_t nc,
+                    const int2 *__restrict__ tile_sum, int32_t *__restrict__ start,
+                    int32_t *__restrict__ cells, int32_t *__restrict__ ncells_out)
+{
+    __shared__ int2 s_part[BSCAN_THREADS / 64];
+    __shared__ int2 s_wave[BSCAN_THREADS / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // totals of the tiles before this one
+    int32_t pa = 0, pb = 0;
+    for (int64_t q = tid; q < (int64_t)blockIdx.x; q += BSCAN_THREADS) {
+        const int2 v = tile_sum[q];
+        pa += v.x;
+        pb += v.y;
+    }
+    const int2 pre = bscan_block_sum(pa, pb, s_part);
+    const int64_t base = (int64_t)blockIdx.x * BSCAN_TILE + (int64_t)tid * BSCAN_ITEMS;
+    int32_t c[BSCAN_ITEMS];
+    int32_t a = 0, b = 0;
+#pragma unroll
+    for (int k = 0; k < BSCAN_ITEMS; ++k) {
+        c[k] = base + k < nc ? counts[base + k] : 0;
+        a += c[k];
+        b += c[k] > 0;
+    }
+    // exclusive prefix of this thread inside the tile
+    int32_t ia = a, ib = b;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int32_t ua = __shfl_up(ia, d, 64), ub = __shfl_up(ib, d, 64);
+        if (lane >= d) {
+            ia += ua;
+            ib += ub;
+        }
+    }
+    if (lane == 63) s_wave[wave] = make_int2(ia, ib);
+    __syncthreads();
+    int32_t ea = pre.x + ia - a, eb = pre.y + ib - b;
+    int32_t ta = 0, tb = 0;  // the tile's totals
+    for (int q = 0; q < BSCAN_THREADS / 64; ++q) {
+        const int2 v = s_wave[q];
+        if (q < wave) {
+            ea += v.x;
+            eb += v.y;
+        }
+        ta += v.x;
+        tb += v.y;
+    }
+#pragma unroll
+    for (int k = 0; k < BSCAN_ITEMS; ++k) {
+        const int64_t key = base + k;
+        if (key < nc) {
+            start[key] = ea;
+            if (c[k] > 0) cells[eb++] = (int32_t)key;
+        }
+        ea += c[k];
+    }
+    if (blockIdx.x == gridDim.x - 1 && tid == 0) {
+        start[nc] = pre.x + ta;
+        *ncells_out = pre.y + tb;
+    }
+}
+
 __global__ void __launch_bounds__(NN_THREADS)
     k_cell_flags(const int32_t *__restrict__ start, int64_t ncells, int32_t *__restrict__ nonempty)
 {
@@ -561,8 +709,10 @@ static int read_i64(sph_ctx *ctx, const void *dev, int64_t *out, bool is32)
 
 static size_t bin_large_off(int64_t n, int64_t nc, size_t scratch_off)
 {
-    return (scratch_off + sph_scan_scratch_bytes(nc > n ? nc : n, sizeof(int32_t)) + 255) &
-           ~(size_t)255;
+    size_t part = sph_scan_scratch_bytes(nc > n ? nc : n, sizeof(int32_t));
+    const size_t tiles = sizeof(int2) * (size_t)(nc / BSCAN_TILE + 1) + 256;
+    if (tiles > part) part = tiles;
+    return (scratch_off + part + 255) & ~(size_t)255;
 }
 
 size_t sph_bin_sort_scratch_end(int64_t n, int64_t nc, size_t scratch_off)
@@ -571,6 +721,9 @@ size_t sph_bin_sort_scratch_end(int64_t n, int64_t nc, size_t scratch_off)
            sizeof(int32_t) * (size_t)(n / (RANK_DIRECT + 1) + 2);
 }
 
+// slot[0..nc) holds the cell counts on entry (k_bin_keys); cell_start[0..nc]
+// receives the starts, cells the non-empty cell keys ascending and slot[nc]
+// their number
 int sph_bin_sort(sph_ctx *ctx, int64_t n, int64_t nc, const int32_t *cell_of, int32_t *rank,
                  int32_t *cell_start, int32_t *perm, int32_t *cells, int32_t *slot, int32_t *tmp,
                  int32_t trash, size_t scratch_off)
@@ -583,29 +736,49 @@ int sph_bin_sort(sph_ctx *ctx, int64_t n, int64_t nc, const int32_t *cell_of, in
     if (rc) return rc;
     int32_t *nlarge = (int32_t *)((char *)ctx->scratch + large_off);
     int32_t *large = nlarge + 1;
-    rc = sph_scan_exclusive_i32_at(ctx, cell_start, nc, scratch_off);
-    if (rc) return rc;
+    const int64_t ntiles = (nc + BSCAN_TILE - 1) / BSCAN_TILE;
+    if (ntiles <= BSCAN_MAX_TILES) {
+        // starts, the compact list of non-empty cell keys and its count
+        int2 *tile_sum = (int2 *)((char *)ctx->scratch + ((scratch_off + 255) & ~(size_t)255));
+        hipLaunchKernelGGL(k_bin_tile_sums, dim3((unsigned)ntiles), dim3(BSCAN_THREADS), 0,
+                           ctx->stream, slot, nc, tile_sum);
+        SPH_LAUNCH_CHECK(ctx, "k_bin_tile_sums");
+        hipLaunchKernelGGL(k_bin_tile_scan, dim3((unsigned)ntiles), dim3(BSCAN_THREADS), 0,
+                           ctx->stream, slot, nc, (const int2 *)tile_sum, cell_start, cells,
+                           slot + nc);
+        SPH_LAUNCH_CHECK(ctx, "k_bin_tile_scan");
+    } else {
+        hipLaunchKernelGGL(k_copy_i32, dim3(sph_blocks(nc, NN_THREADS)), dim3(NN_THREADS), 0,
+                           ctx->stream, slot, nc, cell_start);
+        SPH_LAUNCH_CHECK(ctx, "k_copy_i32");
+        rc = sph_scan_exclusive_i32_at(ctx, cell_start, nc, scratch_off);
+        if (rc) return rc;
+        // non-empty flags -> slots (scan) -> compact list of cell keys
+        hipLaunchKernelGGL(k_cell_flags, dim3(sph_blocks(nc, NN_THREADS)), dim3(NN_THREADS), 0,
+                           ctx->stream, cell_start, nc, slot);
+        SPH_LAUNCH_CHECK(ctx, "k_cell_flags");
+        rc = sph_scan_exclusive_i32_at(ctx, slot, nc, scratch_off);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_compact_cells, dim3(sph_blocks(nc, NN_THREADS)), dim3(NN_THREADS),
+                           0, ctx->stream, cell_start, slot, nc, cells);
+        SPH_LAUNCH_CHECK(ctx, "k_compact_cells");
+    }
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, nlarge, 1, 0);
     SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
     hipLaunchKernelGGL(k_bin_scatter, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
                        ctx->stream, cell_of, rank, n, cell_start, tmp);
     SPH_LAUNCH_CHECK(ctx, "k_bin_scatter");
-    hipLaunchKernelGGL(k_bin_rank, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
-                       ctx->stream, cell_of, n, cell_start, tmp, trash, perm, rank, large,
+    const int64_t cap_cells = nc < n ? nc : n;  // non-empty cells at most
+    int64_t rgrid = (cap_cells * 64 + NN_THREADS - 1) / NN_THREADS;
+    if (rgrid > RANK_CELL_GRID) rgrid = RANK_CELL_GRID;
+    hipLaunchKernelGGL(k_bin_rank_cells, dim3((unsigned)(rgrid > 0 ? rgrid : 1)),
+                       dim3(NN_THREADS), 0, ctx->stream, (const int32_t *)cells,
+                       (const int32_t *)(slot + nc), cell_start, tmp, trash, perm, rank, large,
                        nlarge);
-    SPH_LAUNCH_CHECK(ctx, "k_bin_rank");
+    SPH_LAUNCH_CHECK(ctx, "k_bin_rank_cells");
     hipLaunchKernelGGL(k_bin_rank_large, dim3(RANK_LARGE_GRID), dim3(NN_THREADS), 0,
                        ctx->stream, cell_start, tmp, large, nlarge, perm, rank);
     SPH_LAUNCH_CHECK(ctx, "k_bin_rank_large");
-    // non-empty flags -> slots (scan) -> compact list of cell keys
-    hipLaunchKernelGGL(k_cell_flags, dim3(sph_blocks(nc, NN_THREADS)), dim3(NN_THREADS), 0,
-                       ctx->stream, cell_start, nc, slot);
-    SPH_LAUNCH_CHECK(ctx, "k_cell_flags");
-    rc = sph_scan_exclusive_i32_at(ctx, slot, nc, scratch_off);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_compact_cells, dim3(sph_blocks(nc, NN_THREADS)), dim3(NN_THREADS), 0,
-                       ctx->stream, cell_start, slot, nc, cells);
-    SPH_LAUNCH_CHECK(ctx, "k_compact_cells");
     return SPH_OK;
 }
 
@@ -627,10 +800,10 @@ static int bin_array(sph_ctx *ctx, const sph_grid *grid, const double *x, const 
     GridDev g = make_griddev(ctx, grid);
     const int64_t nc = grid->ncells;
     if (ncells_out) *ncells_out = 0;
-    hipLaunchKernelGGL(k_fill_i32, dim3(sph_blocks(nc + 1, NN_THREADS)), dim3(NN_THREADS), 0,
-                       ctx->stream, cell_start, nc + 1, 0);
-    SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
     if (n == 0) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(sph_blocks(nc + 1, NN_THREADS)), dim3(NN_THREADS),
+                           0, ctx->stream, cell_start, nc + 1, 0);
+        SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
         if (ncells_dev) {
             hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, ncells_dev, 1, 0);
             SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
@@ -646,10 +819,14 @@ static int bin_array(sph_ctx *ctx, const sph_grid *grid, const double *x, const 
     if (rc) return rc;
     int32_t *slot = (int32_t *)ctx->scratch;
     int32_t *tmp = (int32_t *)((char *)ctx->scratch + slot_bytes);
-    // rank first holds the slot inside the cell handed out by the atomics;
-    // sph_bin_sort overwrites it with the sorted position
+    // cell counts into slot (zeroed first); rank first holds the slot inside
+    // the cell handed out by the atomics, sph_bin_sort overwrites it with the
+    // sorted position
+    hipLaunchKernelGGL(k_fill_i32, dim3(sph_blocks(nc + 1, NN_THREADS)), dim3(NN_THREADS), 0,
+                       ctx->stream, slot, nc + 1, 0);
+    SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
     hipLaunchKernelGGL(k_bin_keys, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
-                       ctx->stream, g, x, y, z, n, cell_start, cell_of, rank);
+                       ctx->stream, g, x, y, z, n, slot, cell_of, rank);
     SPH_LAUNCH_CHECK(ctx, "k_bin_keys");
     rc = sph_bin_sort(ctx, n, nc, cell_of, rank, cell_start, perm, cells, slot, tmp, -1,
                       slot_bytes + tmp_bytes);
@@ -1489,10 +1666,13 @@ __global__ void __launch_bounds__(CW)
     }
 }
 
-// slice s of 64 destinations: storage 64 * max(count) and its entry total
+// slice s of 64 destinations: storage 64 * max(count) and its entry total;
+// the storage also in slice_w2[s] (read by k_slice_offsets while slice_w,
+// which becomes the offsets, is rewritten)
 __global__ void __launch_bounds__(NN_THREADS)
     k_slices(const int32_t *__restrict__ counts, int64_t nd_total, int64_t nslices,
-             int64_t *__restrict__ slice_w, int64_t *__restrict__ slice_sum)
+             int64_t *__restrict__ slice_w, int64_t *__restrict__ slice_sum,
+             int64_t *__restrict__ slice_w2)
 {
     const int64_t t = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
     const int c = t < nd_total ? counts[t] : 0;
@@ -1505,6 +1685,7 @@ __global__ void __launch_bounds__(NN_THREADS)
     const int64_t s = t >> 6;
     if ((threadIdx.x & 63) == 0 && s < nslices) {
         slice_w[s] = 64 * (int64_t)m;
+        slice_w2[s] = 64 * (int64_t)m;
         slice_sum[s] = sum;
     }
 }
@@ -1536,6 +1717,159 @@ __global__ void __launch_bounds__(1024)
     }
 }
 
+// The slice widths (in slice_off[0..nslices), a copy in width) turned into
+// offsets and the entry total in two launches when there are few tiles of
+// slices: per-tile sums of the widths and entry counts, then every tile adds
+// the sums of the tiles before it to its own scan and writes slice_off (the
+// last tile also slice_off[nslices] and *nnz).  Every load of a thread is
+// issued before the first is used.  (The general path, k_sum_i64 and a
+// multi-level scan, takes four launches.)
+#define SLICE_THREADS 256
+#define SLICE_ITEMS 16
+#define SLICE_TILE (SLICE_THREADS * SLICE_ITEMS)
+#define SLICE_MAX_TILES 256
+
+__device__ __forceinline__ void slice_block_sum2(int64_t &a, int64_t &b, longlong2 *s_part)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        a += __shfl_xor(a, d, 64);
+        b += __shfl_xor(b, d, 64);
+    }
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = make_longlong2(a, b);
+    __syncthreads();
+    a = b = 0;
+    for (int q = 0; q < SLICE_THREADS / 64; ++q) {
+        a += s_part[q].x;
+        b += s_part[q].y;
+    }
+    __syncthreads();
+}
+
+// item k of thread t of tile b: slice b*TILE + t + k*THREADS (coalesced)
+__global__ void __launch_bounds__(SLICE_THREADS)
+    k_slice_tile_sums(const int64_t *__restrict__ width, const int64_t *__restrict__ slice_sum,
+                      int64_t nslices, longlong2 *__restrict__ tile_sum)
+{
+    __shared__ longlong2 s_part[SLICE_THREADS / 64];
+    const int64_t base = (int64_t)blockIdx.x * SLICE_TILE + threadIdx.x;
+    const int64_t last = nslices - 1;
+    int64_t w[SLICE_ITEMS], z[SLICE_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SLICE_ITEMS; ++k) {
+        const int64_t q = base + (int64_t)k * SLICE_THREADS;
+        w[k] = width[q < last ? q : last];
+        z[k] = slice_sum[q < last ? q : last];
+    }
+    int64_t a = 0, b = 0;
+#pragma unroll
+    for (int k = 0; k < SLICE_ITEMS; ++k) {
+        const bool in = base + (int64_t)k * SLICE_THREADS < nslices;
+        a += in ? w[k] : 0;
+        b += in ? z[k] : 0;
+    }
+    slice_block_sum2(a, b, s_part);
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = make_longlong2(a, b);
+}
+
+__global__ void __launch_bounds__(SLICE_THREADS)
+    k_slice_tile_scan(const int64_t *__restrict__ width, int64_t nslices,
+                      const longlong2 *__restrict__ tile_sum, int64_t *__restrict__ slice_off,
+                      int64_t *__restrict__ nnz)
+{
+    __shared__ longlong2 s_part[SLICE_THREADS / 64];
+    __shared__ int64_t s_round[SLICE_ITEMS][SLICE_THREADS / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t base = (int64_t)blockIdx.x * SLICE_TILE + tid;
+    const int64_t last = nslices - 1;
+    int64_t w[SLICE_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SLICE_ITEMS; ++k) {
+        const int64_t q = base + (int64_t)k * SLICE_THREADS;
+        w[k] = q < nslices ? width[q < last ? q : last] : 0;
+    }
+    // sums of the tiles before this one (and of all tiles, for the totals)
+    int64_t pa = 0, pz = 0, ta = 0, tz = 0;
+    for (int64_t q = tid; q < (int64_t)gridDim.x; q += SLICE_THREADS) {
+        const longlong2 v = tile_sum[q];
+        if (q < (int64_t)blockIdx.x) {
+            pa += v.x;
+        }
+        ta += v.x;
+        tz += v.y;
+    }
+    slice_block_sum2(pa, pz, s_part);
+    slice_block_sum2(ta, tz, s_part);
+    // round k: a scan over the block of the items k (THREADS consecutive slices)
+    int64_t inc[SLICE_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SLICE_ITEMS; ++k) {
+        int64_t x = w[k];
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int64_t up = __shfl_up(x, d, 64);
+            if (lane >= d) x += up;
+        }
+        inc[k] = x;
+        if (lane == 63) s_round[k][wave] = x;
+    }
+    __syncthreads();
+    int64_t carry = pa;
+#pragma unroll
+    for (int k = 0; k < SLICE_ITEMS; ++k) {
+        int64_t before = 0, total = 0;
+#pragma unroll
+        for (int q = 0; q < SLICE_THREADS / 64; ++q) {
+            const int64_t v = s_round[k][q];
+            before += q < wave ? v : 0;
+            total += v;
+        }
+        const int64_t idx = base + (int64_t)k * SLICE_THREADS;
+        if (idx < nslices) slice_off[idx] = carry + before + inc[k] - w[k];
+        carry += total;
+    }
+    if (blockIdx.x == gridDim.x - 1 && tid == 0) {
+        slice_off[nslices] = ta;
+        *nnz = tz;
+    }
+}
+
+// slice offsets and entry total from the widths k_slices wrote into
+// slice_off (and slice_w2) and its per-slice sums
+// The sums region of a build: per-slice entry totals (nslices), the entry
+// total, a copy of the widths (nslices), then the tile sums (16 B each)
+static size_t slice_sums_bytes(int64_t nslices)
+{
+    const int64_t ntiles = (nslices + SLICE_TILE - 1) / SLICE_TILE;
+    return (((size_t)(2 * nslices + 2 + 2 * ntiles) * sizeof(int64_t) + 256) + 255) &
+           ~(size_t)255;
+}
+
+static int slices_to_offsets(sph_ctx *ctx, int64_t *slice_off, const int64_t *slice_w2,
+                             const int64_t *slice_sum, int64_t nslices, int64_t *nnz_dev,
+                             size_t scan_scratch_off)
+{
+    const int64_t ntiles = (nslices + SLICE_TILE - 1) / SLICE_TILE;
+    if (nslices > 0 && ntiles <= SLICE_MAX_TILES) {
+        // after the copy of the widths (16-byte aligned: slice_sum is)
+        longlong2 *tile_sum = (longlong2 *)(const_cast<int64_t *>(slice_sum) + 2 * nslices + 2);
+        hipLaunchKernelGGL(k_slice_tile_sums, dim3((unsigned)ntiles), dim3(SLICE_THREADS), 0,
+                           ctx->stream, slice_w2, slice_sum, nslices, tile_sum);
+        SPH_LAUNCH_CHECK(ctx, "k_slice_tile_sums");
+        hipLaunchKernelGGL(k_slice_tile_scan, dim3((unsigned)ntiles), dim3(SLICE_THREADS), 0,
+                           ctx->stream, slice_w2, nslices, (const longlong2 *)tile_sum,
+                           slice_off, nnz_dev);
+        SPH_LAUNCH_CHECK(ctx, "k_slice_tile_scan");
+        return SPH_OK;
+    }
+    // deterministic one-block total (atomics on one word from every slice
+    // serialise: 190 us for 15,625 slices)
+    hipLaunchKernelGGL(k_sum_i64, dim3(1), dim3(1024), 0, ctx->stream, slice_sum, nslices,
+                       nnz_dev);
+    SPH_LAUNCH_CHECK(ctx, "k_sum_i64");
+    return sph_scan_exclusive_i64_at(ctx, slice_off, nslices, scan_scratch_off);
+}
+
 // status / totals of a device-side build: bit 0 of *status when the lists
 // need more than cap slots; *nnz_acc += nnz (pair count kept on the device)
 __global__ void k_nnps_dev_status(const int64_t *__restrict__ slots, const int64_t *__restrict__ nnz,
@@ -1560,7 +1894,7 @@ static int nnps_count(sph_ctx *ctx, CellArgs &a, int64_t ncell_launch, int64_t n
     const size_t cand_bytes = al((size_t)ncell_launch * a.maxch * CW * sizeof(int32_t));
     const size_t dcell_bytes = al((size_t)nd_total * sizeof(int32_t));
     const size_t list_bytes = al((size_t)(ncell_launch + 1) * sizeof(int32_t));
-    const size_t sums_bytes = al((size_t)(nslices + 1) * sizeof(int64_t) + 256);
+    const size_t sums_bytes = slice_sums_bytes(nslices);
     const size_t sums_off = masks_bytes + ovf_bytes + cand_bytes + dcell_bytes + list_bytes;
     int rc = sph_ensure_scratch(ctx, sums_off + sums_bytes +
                                          sph_scan_scratch_bytes(nslices, sizeof(int64_t)));
@@ -1588,14 +1922,11 @@ static int nnps_count(sph_ctx *ctx, CellArgs &a, int64_t ncell_launch, int64_t n
     SPH_LAUNCH_CHECK(ctx, "k_nnps_cells<count>");
     // slice widths -> offsets (scan); slice totals -> entry total
     hipLaunchKernelGGL(k_slices, dim3(sph_blocks(nslices * 64, NN_THREADS)), dim3(NN_THREADS), 0,
-                       ctx->stream, counts, nd_total, nslices, slice_off, slice_sum);
+                       ctx->stream, counts, nd_total, nslices, slice_off, slice_sum,
+                       slice_sum + nslices + 1);
     SPH_LAUNCH_CHECK(ctx, "k_slices");
-    // deterministic one-block total (atomics on one word from every slice
-    // serialise: 190 us for 15,625 slices)
-    hipLaunchKernelGGL(k_sum_i64, dim3(1), dim3(1024), 0, ctx->stream, slice_sum, nslices,
-                       *nnz_dev);
-    SPH_LAUNCH_CHECK(ctx, "k_sum_i64");
-    return sph_scan_exclusive_i64_at(ctx, slice_off, nslices, sums_off + sums_bytes);
+    return slices_to_offsets(ctx, slice_off, slice_sum + nslices + 1, slice_sum, nslices,
+                             *nnz_dev, sums_off + sums_bytes);
 }
 
 static int nnps_fill(sph_ctx *ctx, const CellArgs &a, int64_t ncell_launch, int64_t nd_total,
@@ -2390,7 +2721,7 @@ static int nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const T *d_rec, c
     const size_t cand_bytes = al((size_t)nd_cells * maxch * CW * sizeof(int32_t));
     const size_t dcell_bytes = al((size_t)nd_total * sizeof(int32_t));
     const size_t fc_bytes = al((size_t)nd_total * sizeof(int32_t));
-    const size_t sums_bytes = al((size_t)(nslices + 1) * sizeof(int64_t) + 256);
+    const size_t sums_bytes = slice_sums_bytes(nslices);
     const size_t long_bytes = al((size_t)(nd_total + 1) * sizeof(int32_t));
     const size_t sums_off =
         2 * masks_bytes + ovf_bytes + 2 * cand_bytes + dcell_bytes + fc_bytes + long_bytes;
@@ -2474,12 +2805,11 @@ static int nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const T *d_rec, c
                        (const int32_t *)c.dcell_of, (const int32_t *)c.cell_ovf);
     SPH_LAUNCH_CHECK(ctx, "k_varh_rows<count>");
     hipLaunchKernelGGL(k_slices, dim3(sph_blocks(nslices * 64, NN_THREADS)), dim3(NN_THREADS), 0,
-                       ctx->stream, counts, nd_total, nslices, slice_off, slice_sum);
+                       ctx->stream, counts, nd_total, nslices, slice_off, slice_sum,
+                       slice_sum + nslices + 1);
     SPH_LAUNCH_CHECK(ctx, "k_slices");
-    hipLaunchKernelGGL(k_sum_i64, dim3(1), dim3(1024), 0, ctx->stream, slice_sum, nslices,
-                       nnz_dev);
-    SPH_LAUNCH_CHECK(ctx, "k_sum_i64");
-    rc = sph_scan_exclusive_i64_at(ctx, slice_off, nslices, sums_off + sums_bytes);
+    rc = slices_to_offsets(ctx, slice_off, slice_sum + nslices + 1, slice_sum, nslices, nnz_dev,
+                           sums_off + sums_bytes);
     if (rc) return rc;
     rc = sph_ensure_pinned(ctx, 3 * sizeof(int64_t));
     if (rc) return rc;

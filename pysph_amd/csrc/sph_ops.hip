@@ -26,6 +26,49 @@
 #endif
 
 // ---------------------------------------------------------------------------
+// Tait equation of state of one particle (eos_funcs.pyx:118-129)
+// ---------------------------------------------------------------------------
+template <typename T> struct TaitArgs {
+    T ro, co, gamma, B;  // B = co^2 ro / gamma (eos_funcs.pyx:90)
+    int accumulate;
+    double *p, *cs;      // original order
+    T *p_s, *cs_s, *prho2_s;  // sorted record fields (or NULL), stride st
+    int64_t st;
+};
+
+// p, cs of sorted particle t (original index i) from its density rhoa (the
+// record's value); a non-LocalReal particle gets 0 (sph_func.pyx:226-227).
+// P/rho^2 of the record, k_prho2's two operations, when prho2_s is given.
+template <typename T>
+__device__ __forceinline__ void tait_point(const TaitArgs<T> &E, int64_t t, int32_t i,
+                                           bool real, T rhoa)
+{
+    T r0 = T(0.0), r1 = T(0.0);
+    if (real) {
+        const T ratio = rhoa / E.ro;
+        const T gamma2 = T(0.5) * (E.gamma - T(1.0));
+        const T tmp = sph_pow(ratio, E.gamma);
+        r0 = (tmp - T(1.0)) * E.B;
+        r1 = sph_pow(ratio, gamma2) * E.co;
+        if (E.accumulate) {
+            r0 = (E.p_s ? E.p_s[t * E.st] : T(E.p[i])) + r0;
+            r1 = (E.cs_s ? E.cs_s[t * E.st] : T(E.cs[i])) + r1;
+        } else {
+            r0 = T(0.0) + r0;
+            r1 = T(0.0) + r1;
+        }
+    }
+    E.p[i] = r0;
+    E.cs[i] = r1;
+    if (E.p_s) E.p_s[t * E.st] = r0;
+    if (E.cs_s) E.cs_s[t * E.st] = r1;
+    if (E.prho2_s) {
+        const T rhob2 = T(1.0) / (rhoa * rhoa);
+        E.prho2_s[t * E.st] = r0 * rhob2;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // SPHRho (rho_term in sph_terms.h)
 // ---------------------------------------------------------------------------
 // one SELL-64 row of SPHRho, summed from 0 in list order
@@ -67,12 +110,15 @@ __device__ __forceinline__ T rho_row(int dim, T xa, T ya, T za, T ha, T facA,
     return nr;
 }
 
-template <bool UH, typename T>
+// EOS: TaitEquation of the same particle follows, from the density just
+// written to its record (the k_tait arithmetic; the caller passes out_s)
+template <bool UH, bool EOS, typename T>
 __global__ void __launch_bounds__(OP_THREADS)
     k_summation_density(int dim, int64_t nd_total, int64_t nd_real,
                         const int32_t *__restrict__ dperm, SrcLists<T> L,
                         const T *__restrict__ drec, int accumulate,
-                        double *__restrict__ rho, T *__restrict__ out_s, int64_t out_stride)
+                        double *__restrict__ rho, T *__restrict__ out_s, int64_t out_stride,
+                        TaitArgs<T> E)
 {
     const int64_t t =
         (int64_t)sph_xcd_tile(blockIdx.x, gridDim.x) * OP_THREADS + threadIdx.x;
@@ -81,6 +127,9 @@ __global__ void __launch_bounds__(OP_THREADS)
     if (i >= nd_real) {  // non-LocalReal destinations (sph_func.pyx:226-227)
         rho[i] = 0;
         if (out_s) out_s[t * out_stride] = 0;
+        // the record's density as k_tait reads it (a runtime value: a
+        // constant 0 would let the compiler fold P/rho^2's 0 * inf)
+        if (EOS) tait_point(E, t, i, false, *(volatile T *)&out_s[t * out_stride]);
         return;
     }
     int32_t e[SPH_MAX_SRC];
@@ -106,13 +155,14 @@ __global__ void __launch_bounds__(OP_THREADS)
     }
     rho[i] = val;
     if (out_s) out_s[t * out_stride] = val;
+    if (EOS) tait_point(E, t, i, true, val);
 }
 
 template <typename T>
 static int summation_density(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
                              const int32_t *d_perm, const sph_sources *src, const T *d_rec,
                              int uniform_h, int accumulate, double *rho, T *out_s,
-                             int64_t out_s_stride)
+                             int64_t out_s_stride, const TaitArgs<T> *eos = nullptr)
 {
     if (!ctx) return SPH_EINVAL;
     SPH_REQUIRE(ctx, dim >= 1 && dim <= 3, "sph_summation_density: bad dim %d", dim);
@@ -123,10 +173,16 @@ static int summation_density(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd
     SrcLists<T> L;
     int rc = src_lists<T>(ctx, "sph_summation_density", src, &L);
     if (rc) return rc;
-    auto kern = uniform_h ? k_summation_density<true, T> : k_summation_density<false, T>;
+    TaitArgs<T> E;
+    memset(&E, 0, sizeof(E));
+    if (eos) E = *eos;
+    auto kern = eos ? (uniform_h ? k_summation_density<true, true, T>
+                                 : k_summation_density<false, true, T>)
+                    : (uniform_h ? k_summation_density<true, false, T>
+                                 : k_summation_density<false, false, T>);
     hipLaunchKernelGGL(kern, dim3(sph_blocks(nd_total, OP_THREADS)),
                        dim3(OP_THREADS), 0, ctx->stream, dim, nd_total, nd_real, d_perm, L,
-                       d_rec, accumulate, rho, out_s, out_s_stride);
+                       d_rec, accumulate, rho, out_s, out_s_stride, E);
     SPH_LAUNCH_CHECK(ctx, "k_summation_density");
     return SPH_OK;
 }
@@ -193,40 +249,33 @@ extern "C" int sph_summation_density_multi_f32(sph_ctx *ctx, int dim, int64_t nd
 // Tait equation of state
 // ---------------------------------------------------------------------------
 template <typename T>
+static TaitArgs<T> tait_args(double ro, double co, double gamma, int accumulate, double *p,
+                             double *cs, T *p_s, T *cs_s, T *prho2_s, int64_t st)
+{
+    TaitArgs<T> E;
+    E.ro = T(ro);
+    E.co = T(co);
+    E.gamma = T(gamma);
+    E.B = T(co) * T(co) * T(ro) / T(gamma);  // eos_funcs.pyx:90
+    E.accumulate = accumulate;
+    E.p = p;
+    E.cs = cs;
+    E.p_s = p_s;
+    E.cs_s = cs_s;
+    E.prho2_s = prho2_s;
+    E.st = st;
+    return E;
+}
+
+template <typename T>
 __global__ void __launch_bounds__(OP_THREADS)
     k_tait(int64_t n_total, int64_t n_real, const int32_t *__restrict__ perm,
-           const T *__restrict__ rho_s, int64_t st, T ro, T co, T gamma, T B, int accumulate,
-           double *__restrict__ p, double *__restrict__ cs, T *__restrict__ p_s,
-           T *__restrict__ cs_s, T *__restrict__ prho2_s)
+           const T *__restrict__ rho_s, int64_t st, TaitArgs<T> E)
 {
     int64_t t = (int64_t)blockIdx.x * OP_THREADS + threadIdx.x;
     if (t >= n_total) return;
     const int32_t i = perm[t];
-    T r0 = T(0.0), r1 = T(0.0);
-    // read once, before the stores (a load behind a store waits for it)
-    const T rhoa = rho_s[t * st];
-    if (i < n_real) {
-        const T ratio = rhoa / ro;
-        const T gamma2 = T(0.5) * (gamma - T(1.0));
-        const T tmp = sph_pow(ratio, gamma);
-        r0 = (tmp - T(1.0)) * B;
-        r1 = sph_pow(ratio, gamma2) * co;
-        if (accumulate) {
-            r0 = (p_s ? p_s[t * st] : T(p[i])) + r0;
-            r1 = (cs_s ? cs_s[t * st] : T(cs[i])) + r1;
-        } else {
-            r0 = T(0.0) + r0;
-            r1 = T(0.0) + r1;
-        }
-    }
-    p[i] = r0;
-    cs[i] = r1;
-    if (p_s) p_s[t * st] = r0;
-    if (cs_s) cs_s[t * st] = r1;
-    if (prho2_s) {  // k_prho2's two operations on the record's p and rho
-        const T rhob2 = T(1.0) / (rhoa * rhoa);
-        prho2_s[t * st] = r0 * rhob2;
-    }
+    tait_point(E, t, i, i < n_real, rho_s[t * st]);
 }
 
 template <typename T>
@@ -238,10 +287,10 @@ static int tait(sph_ctx *ctx, int64_t n_total, int64_t n_real, const int32_t *pe
     SPH_REQUIRE(ctx, n_real >= 0 && n_total >= n_real, "sph_tait: bad n");
     if (n_total == 0) return SPH_OK;
     SPH_REQUIRE(ctx, perm && rho_s && p && cs && s_stride >= 1, "sph_tait: null argument");
-    const T B = T(co) * T(co) * T(ro) / T(gamma);  // eos_funcs.pyx:90
+    const TaitArgs<T> E = tait_args<T>(ro, co, gamma, accumulate, p, cs, p_s, cs_s, prho2_s,
+                                       s_stride);
     hipLaunchKernelGGL(k_tait<T>, dim3(sph_blocks(n_total, OP_THREADS)), dim3(OP_THREADS), 0,
-                       ctx->stream, n_total, n_real, perm, rho_s, s_stride, T(ro), T(co),
-                       T(gamma), B, accumulate, p, cs, p_s, cs_s, prho2_s);
+                       ctx->stream, n_total, n_real, perm, rho_s, s_stride, E);
     SPH_LAUNCH_CHECK(ctx, "k_tait");
     return SPH_OK;
 }
@@ -262,6 +311,55 @@ extern "C" int sph_tait_f32(sph_ctx *ctx, int64_t n_total, int64_t n_real, const
 {
     return tait<float>(ctx, n_total, n_real, perm, rho_s, s_stride, ro, co, gamma, accumulate,
                        p, cs, p_s, cs_s, prho2_s);
+}
+
+// SPHRho followed by TaitEquation on the same destination array in one launch
+// (the density sums of sph_summation_density_multi, then k_tait's arithmetic
+// on the density just written to the record).  The Tait record fields share
+// the density record's stride; out_s (the record's rho) is required.
+template <typename T>
+static int summation_density_tait(sph_ctx *ctx, int dim, int64_t nd_total, int64_t nd_real,
+                                  const int32_t *d_perm, const sph_sources *src, const T *d_rec,
+                                  int uniform_h, int accumulate, double *rho, T *out_s,
+                                  int64_t out_s_stride, double ro, double co, double gamma,
+                                  int eos_accumulate, double *p, double *cs, T *p_s, T *cs_s,
+                                  T *prho2_s)
+{
+    if (!ctx) return SPH_EINVAL;
+    if (nd_total == 0) return SPH_OK;
+    SPH_REQUIRE(ctx, out_s && p && cs, "sph_summation_density_tait: null argument");
+    const TaitArgs<T> E = tait_args<T>(ro, co, gamma, eos_accumulate, p, cs, p_s, cs_s,
+                                       prho2_s, out_s_stride);
+    return summation_density<T>(ctx, dim, nd_total, nd_real, d_perm, src, d_rec, uniform_h,
+                                accumulate, rho, out_s, out_s_stride, &E);
+}
+
+extern "C" int sph_summation_density_tait(sph_ctx *ctx, int dim, int64_t nd_total,
+                                          int64_t nd_real, const int32_t *d_perm,
+                                          const sph_sources *src, const double *d_rec,
+                                          int uniform_h, int accumulate, double *rho,
+                                          double *out_s, int64_t out_s_stride, double ro,
+                                          double co, double gamma, int eos_accumulate,
+                                          double *p, double *cs, double *p_s, double *cs_s,
+                                          double *prho2_s)
+{
+    return summation_density_tait<double>(ctx, dim, nd_total, nd_real, d_perm, src, d_rec,
+                                          uniform_h, accumulate, rho, out_s, out_s_stride, ro,
+                                          co, gamma, eos_accumulate, p, cs, p_s, cs_s, prho2_s);
+}
+
+extern "C" int sph_summation_density_tait_f32(sph_ctx *ctx, int dim, int64_t nd_total,
+                                              int64_t nd_real, const int32_t *d_perm,
+                                              const sph_sources *src, const float *d_rec,
+                                              int uniform_h, int accumulate, double *rho,
+                                              float *out_s, int64_t out_s_stride, double ro,
+                                              double co, double gamma, int eos_accumulate,
+                                              double *p, double *cs, float *p_s, float *cs_s,
+                                              float *prho2_s)
+{
+    return summation_density_tait<float>(ctx, dim, nd_total, nd_real, d_perm, src, d_rec,
+                                         uniform_h, accumulate, rho, out_s, out_s_stride, ro,
+                                         co, gamma, eos_accumulate, p, cs, p_s, cs_s, prho2_s);
 }
 
 // Tait over sorted records with every particle real and no accumulation

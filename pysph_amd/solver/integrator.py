@@ -102,16 +102,35 @@ class Integrator:
                 array.touch(*[dst for _, dst in pairs])
 
     def eval(self):
+        """Evaluate the calcs in order (solver/integrator.py:287-334).  A
+        density calc followed by the equation-of-state calc of the same
+        array runs as one launch (SPHCalc.fuses_eos) with the results of
+        the two sph() calls."""
         k_num = self.cstep
-        for calc in self.calcs:
+        calcs = self.calcs
+        c = 0
+        while c < len(calcs):
+            calc = calcs[c]
             calc.dest.set_time(self.local_time)
             if calc.integrates:
                 calc.sph(*calc.dst_writes[k_num])
+                c += 1
+                continue
+            nxt = calcs[c + 1] if c + 1 < len(calcs) else None
+            if nxt is not None and calc.fuses_eos(nxt):
+                # SPHRho then TaitEquation on the same array: one launch with
+                # the results of the two calls in turn
+                nxt.dest.set_time(self.local_time)
+                calc.sph_with_eos(nxt)
+                done = (calc, nxt)
             else:
                 calc.sph(*calc.updates)
+                done = (calc,)
+            for d in done:
                 self.particles.barrier()
-                self.rupdate_list[calc.dnum] = [calc.updates]
+                self.rupdate_list[d.dnum] = [d.updates]
                 self.particles.update_remote_particle_properties(self.rupdate_list)
+            c += len(done)
         self.particles.barrier()
 
     def _props_struct(self, array, triples):

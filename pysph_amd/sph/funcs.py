@@ -101,9 +101,13 @@ class SPHRho(CSPHFunctionParticle):
         return all(type(f) is SPHRho and f.dest is funcs[0].dest for f in funcs)
 
     @staticmethod
-    def eval_multi(funcs, kernel, output1, output2, output3, accumulate=True, names=None):
+    def eval_multi(funcs, kernel, output1, output2, output3, accumulate=True, names=None,
+                   eos=None):
         """All functions of one calc (one per source array) in one launch:
-        the same sums, in the same order, as their eval calls in turn."""
+        the same sums, in the same order, as their eval calls in turn.  With
+        eos = (TaitEquation function, its outputs, their names) the launch
+        also evaluates that function on the densities it wrote (accumulate
+        False for both)."""
         f0 = funcs[0]
         ctx = f0._ctx()
         cm = f0.nbr_locator.cell_manager
@@ -135,7 +139,27 @@ class SPHRho(CSPHFunctionParticle):
             if name:
                 cm.written(f0.dest, name, out_s is not None)
             return
-        fn = (ctx.lib.sph_summation_density_multi_f32 if cm.precision == "single"
+        single = cm.precision == "single"
+        if eos is not None:
+            tf, touts, tnames = eos
+            n0, n1 = tnames[0], tnames[1]
+            p_s, cs_s = cm.shadow_out(f0.dest, n0), cm.shadow_out(f0.dest, n1)
+            prho2_s = drec.data_ptr() + drec.element_size() * PRHO2_FIELD if n0 == 'p' else None
+            fn = (ctx.lib.sph_summation_density_tait_f32 if single
+                  else ctx.lib.sph_summation_density_tait)
+            with profiling.region("summation_density"):
+                ctx.check(fn(ctx.handle, kernel.dim, f0.dest.get_number_of_particles(),
+                             f0.dest.num_real_particles, p(cm.bins[f0.dest.name].perm),
+                             ctypes.byref(src), p(drec), int(cm.uniform_h()), 0, p(output1),
+                             out_s, REC, tf.ro, tf.co, tf.gamma, 0, p(touts[0]), p(touts[1]),
+                             p_s, cs_s, prho2_s))
+            cm.written(f0.dest, name, True)
+            cm.written(f0.dest, n0, p_s is not None)
+            cm.written(f0.dest, n1, cs_s is not None)
+            if prho2_s is not None:
+                cm.prho2_written(f0.dest)
+            return
+        fn = (ctx.lib.sph_summation_density_multi_f32 if single
               else ctx.lib.sph_summation_density_multi)
         with profiling.region("summation_density"):
             ctx.check(fn(ctx.handle, kernel.dim, f0.dest.get_number_of_particles(),

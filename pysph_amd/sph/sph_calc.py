@@ -72,13 +72,71 @@ class SPHCalc:
             resolved.append(name)
         return outs, tuple(resolved)
 
+    def _zero(self, ctx, name, arr):
+        """Zero one output of the destination, unless it still holds the zeros
+        of an earlier call (same tensor, not written since): the temporary
+        outputs a function leaves unwritten are zeroed at every evaluation."""
+        dest = self.dest
+        zeroed = dest.__dict__.setdefault("_zeroed", {})
+        mark = (dest.version(name), arr.data_ptr(), arr.numel())
+        if zeroed.get(name) == mark:
+            return
+        ctx.check(ctx.lib.sph_fill(ctx.handle, arr.numel(), 0.0, _lib.ptr(arr)))
+        dest.touch(name)
+        zeroed[name] = (dest.version(name), arr.data_ptr(), arr.numel())
+
+    def fuses_eos(self, nxt):
+        """Whether this calc (SPHRho over one or more sources) and `nxt`
+        (TaitEquation on the same array, from the density just computed) can
+        run as one launch (SPHRho.eval_multi with eos) with the results of
+        their sph() calls in turn."""
+        from .funcs import SPHRho, TaitEquation, _staged
+        if self.integrates or nxt.integrates or not (self.reset_arrays and nxt.reset_arrays):
+            return False
+        if self.to_reset or nxt.to_reset or len(nxt.funcs) != 1:
+            return False
+        funcs = self.funcs
+        if not funcs or len(funcs) > _lib.MAX_SRC or not SPHRho.fusable(funcs):
+            return False
+        tf = nxt.funcs[0]
+        if type(tf) is not TaitEquation or nxt.dest is not self.dest or tf.dest is not self.dest:
+            return False
+        cm = getattr(funcs[0].nbr_locator, "cell_manager", None)
+        if cm is None or self.dest.name not in cm.array_indices or _staged(cm, funcs):
+            return False
+        _, names = self._outputs(self.updates)
+        _, nnames = nxt._outputs(nxt.updates)
+        if names[0] != "rho" or tuple(nnames[:2]) != ("p", "cs"):
+            return False
+        # the zeroed outputs of the two calls must not be what either writes
+        zeroed = {names[k] for k in range(3) if not funcs[0].writes[k]}
+        zeroed |= {nnames[k] for k in range(3) if not tf.writes[k]}
+        return not zeroed & {"rho", "p", "cs"}
+
+    def sph_with_eos(self, nxt):
+        """self.sph(*self.updates) then nxt.sph(*nxt.updates), fused (see
+        fuses_eos)."""
+        from .funcs import SPHRho
+        outs, names = self._outputs(self.updates)
+        nouts, nnames = nxt._outputs(nxt.updates)
+        ctx = _lib.context(self.dest.device)
+        for k in range(3):
+            if not self.funcs[0].writes[k]:
+                self._zero(ctx, names[k], outs[k])
+        for k in range(3):
+            if not nxt.funcs[0].writes[k]:
+                nxt._zero(ctx, nnames[k], nouts[k])
+        SPHRho.eval_multi(self.funcs, self.kernel, outs[0], outs[1], outs[2], accumulate=False,
+                          names=names, eos=(nxt.funcs[0], nouts, nnames))
+        if self.dest.is_dirty:
+            self.particles.update()
+
     def sph(self, output_array1=None, output_array2=None, output_array3=None,
             exclude_self=False):
         if exclude_self:
             raise NotImplementedError("exclude_self is not part of this path")
         outs, names = self._outputs((output_array1, output_array2, output_array3))
         ctx = _lib.context(self.dest.device)
-        n = self.dest.get_number_of_particles()
         accumulate = True
         if not self.integrates and self.reset_arrays:
             # reset_output_arrays (sph_calc.pyx:431-440): the first function
@@ -86,12 +144,10 @@ class SPHCalc:
             first = self.funcs[0]
             for k in range(3):
                 if not first.writes[k]:
-                    ctx.check(ctx.lib.sph_fill(ctx.handle, n, 0.0, _lib.ptr(outs[k])))
-                    self.dest.touch(names[k])
+                    self._zero(ctx, names[k], outs[k])
             accumulate = False
         for prop in self.to_reset:
-            ctx.check(ctx.lib.sph_fill(ctx.handle, n, 0.0, _lib.ptr(self.dest.get_carray(prop))))
-            self.dest.touch(prop)
+            self._zero(ctx, prop, self.dest.get_carray(prop))
         funcs = self.funcs
         multi = getattr(type(funcs[0]), "eval_multi", None) if funcs else None
         if multi is not None and 1 <= len(funcs) <= _lib.MAX_SRC and type(funcs[0]).fusable(funcs):

@@ -415,70 +415,39 @@ __global__ void __launch_bounds__(NN_THREADS)
     tmp[start[cell_of[i]] + slot[i]] = (int32_t)i;
 }
 
-// Cells with at most 64 members are sorted in a wavefront's registers, up to
-// RANK_DIRECT members ordered by counting (each member scans its cell); larger
-// ones by k_bin_rank_large, a merge sort of the cell's members, so a cell
-// holding most of the particles (a large h sets the cell size, or a clustered
-// input) costs O(m log^2 m), not O(m^2).
+// Cells with at most RANK_DIRECT members are ordered by counting (each
+// member scans its cell); larger ones by k_bin_rank_large, a merge sort of
+// the cell's members, so a cell holding most of the particles (a large h
+// sets the cell size, or a clustered input) costs O(m log^2 m), not O(m^2).
 #define RANK_DIRECT 1024
 #define RANK_CHUNK 2048        // members sorted in LDS per pass of k_bin_rank_large
 #define RANK_LARGE_GRID 256    // blocks of k_bin_rank_large (one large cell at a time each)
 
 // Ascending particle index inside every cell (the atomics above hand out
-// slots in arbitrary order), one wavefront per non-empty cell (the compact
-// cell list and its count from the binning scan): a cell of at most 64
-// members is sorted across the lanes (bitonic, in registers), a larger one
-// ranked by counting (a member's position is the number of members with a
-// smaller index), lane by lane; cells of more than RANK_DIRECT members are
-// listed for k_bin_rank_large.  Writes perm and its inverse rank.  Members
-// of the trash cell keep their atomic slots.
-#define RANK_CELL_GRID 16384  // blocks of the per-cell ranking (4 wavefronts each)
+// slots in arbitrary order): particle i's position is the number of members
+// of its cell with a smaller index.  Writes perm and its inverse rank.  The
+// members of a cell with more than RANK_DIRECT of them are left to
+// k_bin_rank_large: the member holding slot 0 lists the cell there.
 __global__ void __launch_bounds__(NN_THREADS)
-    k_bin_rank_cells(const int32_t *__restrict__ cells, const int32_t *__restrict__ ncells_dev,
-                     const int32_t *__restrict__ start, const int32_t *__restrict__ tmp,
-                     int32_t trash, int32_t *__restrict__ perm, int32_t *__restrict__ rank,
-                     int32_t *__restrict__ large, int32_t *__restrict__ nlarge)
+    k_bin_rank(const int32_t *__restrict__ cell_of, int64_t n, const int32_t *__restrict__ start,
+               const int32_t *__restrict__ tmp, int32_t trash, int32_t *__restrict__ perm,
+               int32_t *__restrict__ rank, int32_t *__restrict__ large, int32_t *__restrict__ nlarge)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t nw = ((int64_t)gridDim.x * NN_THREADS) >> 6;
-    const int32_t nce = *ncells_dev;
-    for (int64_t w = ((int64_t)blockIdx.x * NN_THREADS + threadIdx.x) >> 6; w < nce; w += nw) {
-        const int32_t key = cells[w];
-        const int32_t s = start[key], m = start[key + 1] - s;
-        if (key == trash) {  // unordered: the atomic slots (no lists are built for this cell)
-            for (int32_t k = lane; k < m; k += 64) {
-                const int32_t v = tmp[s + k];
-                perm[s + k] = v;
-                rank[v] = s + k;
-            }
-        } else if (m <= 64) {
-            int32_t v = lane < m ? tmp[s + lane] : INT32_MAX;
-#pragma unroll
-            for (int k = 2; k <= 64; k <<= 1) {
-#pragma unroll
-                for (int j = k >> 1; j > 0; j >>= 1) {
-                    const int32_t o = __shfl_xor(v, j, 64);
-                    // ascending runs of length k; the lower lane of a pair keeps
-                    // the smaller value in an ascending run, the larger otherwise
-                    v = (((lane & j) == 0) == ((lane & k) == 0)) ? min(v, o) : max(v, o);
-                }
-            }
-            if (lane < m) {
-                perm[s + lane] = v;
-                rank[v] = s + lane;
-            }
-        } else if (m > RANK_DIRECT) {
-            if (lane == 0) large[atomicAdd(nlarge, 1)] = key;
-        } else {
-            for (int32_t k = lane; k < m; k += 64) {
-                const int32_t v = tmp[s + k];
-                int32_t r = 0;
-                for (int32_t a = 0; a < m; ++a) r += tmp[s + a] < v;
-                perm[s + r] = v;
-                rank[v] = s + r;
-            }
-        }
+    int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
+    if (i >= n) return;
+    const int32_t key = cell_of[i];
+    const int32_t s = start[key], e = start[key + 1];
+    int32_t r = 0;
+    if (key == trash) {
+        r = rank[i];  // unordered: the atomic slot (no lists are built for this cell)
+    } else if (e - s > RANK_DIRECT) {
+        if (rank[i] == 0) large[atomicAdd(nlarge, 1)] = key;  // slot 0: once per cell
+        return;
+    } else {
+        for (int32_t a = s; a < e; ++a) r += tmp[a] < (int32_t)i;
     }
+    perm[s + r] = (int32_t)i;
+    rank[i] = s + r;
 }
 
 // number of the n ascending values of a that are smaller than v
@@ -624,12 +593,20 @@ __global__ void __launch_bounds__(BSCAN_THREADS)
     __shared__ int2 s_part[BSCAN_THREADS / 64];
     __shared__ int2 s_wave[BSCAN_THREADS / 64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    // totals of the tiles before this one
+    // totals of the tiles before this one, eight loads in flight per thread
     int32_t pa = 0, pb = 0;
-    for (int64_t q = tid; q < (int64_t)blockIdx.x; q += BSCAN_THREADS) {
-        const int2 v = tile_sum[q];
-        pa += v.x;
-        pb += v.y;
+    for (int64_t q0 = 0; q0 < (int64_t)blockIdx.x; q0 += 8 * BSCAN_THREADS) {
+        int2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int64_t q = q0 + (int64_t)u * BSCAN_THREADS + tid;
+            v[u] = q < (int64_t)blockIdx.x ? tile_sum[q] : make_int2(0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            pa += v[u].x;
+            pb += v[u].y;
+        }
     }
     const int2 pre = bscan_block_sum(pa, pb, s_part);
     const int64_t base = (int64_t)blockIdx.x * BSCAN_TILE + (int64_t)tid * BSCAN_ITEMS;
@@ -768,14 +745,10 @@ int sph_bin_sort(sph_ctx *ctx, int64_t n, int64_t nc, const int32_t *cell_of, in
     hipLaunchKernelGGL(k_bin_scatter, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
                        ctx->stream, cell_of, rank, n, cell_start, tmp);
     SPH_LAUNCH_CHECK(ctx, "k_bin_scatter");
-    const int64_t cap_cells = nc < n ? nc : n;  // non-empty cells at most
-    int64_t rgrid = (cap_cells * 64 + NN_THREADS - 1) / NN_THREADS;
-    if (rgrid > RANK_CELL_GRID) rgrid = RANK_CELL_GRID;
-    hipLaunchKernelGGL(k_bin_rank_cells, dim3((unsigned)(rgrid > 0 ? rgrid : 1)),
-                       dim3(NN_THREADS), 0, ctx->stream, (const int32_t *)cells,
-                       (const int32_t *)(slot + nc), cell_start, tmp, trash, perm, rank, large,
+    hipLaunchKernelGGL(k_bin_rank, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
+                       ctx->stream, cell_of, n, cell_start, tmp, trash, perm, rank, large,
                        nlarge);
-    SPH_LAUNCH_CHECK(ctx, "k_bin_rank_cells");
+    SPH_LAUNCH_CHECK(ctx, "k_bin_rank");
     hipLaunchKernelGGL(k_bin_rank_large, dim3(RANK_LARGE_GRID), dim3(NN_THREADS), 0,
                        ctx->stream, cell_start, tmp, large, nlarge, perm, rank);
     SPH_LAUNCH_CHECK(ctx, "k_bin_rank_large");

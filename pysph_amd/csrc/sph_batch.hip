@@ -176,13 +176,15 @@ __global__ void __launch_bounds__(BT_THREADS)
                  const double *__restrict__ x, const double *__restrict__ y,
                  const double *__restrict__ z, int64_t n, int64_t key_cap,
                  int32_t *__restrict__ count, int32_t *__restrict__ cell_of,
-                 int32_t *__restrict__ slot, int32_t *__restrict__ status)
+                 int32_t *__restrict__ slot, int32_t *__restrict__ key_sim,
+                 int32_t *__restrict__ status)
 {
     const int64_t i = (int64_t)blockIdx.x * BT_THREADS + threadIdx.x;
     const bool act = i < n;  // the active lanes are a prefix of the wavefront
-    int32_t key32 = -1;
+    int32_t key32 = -1, sim = 0;
     if (act) {
-        const GridDev g = grids[sim_of[i]];
+        sim = sim_of[i];
+        const GridDev g = grids[sim];
         int64_t key = key_cap - 1;  // trash cell of void simulations
         if (g.d0 > 0) {
             const int cx = min(max(cell_coord(x[i], g.cs) - g.cmin0, 0), g.d0 - 1);
@@ -209,7 +211,12 @@ __global__ void __launch_bounds__(BT_THREADS)
     const uint64_t above = heads & ~((2ull << start) - 1ull);
     const int next = above ? __ffsll((long long)above) - 1 : __popcll(acts);
     int32_t base = 0;
-    if (head) base = atomicAdd(&count[key32], next - start);
+    if (head) {
+        base = atomicAdd(&count[key32], next - start);
+        // every particle of a key is of one simulation (the keys of a
+        // simulation are its own range; the trash key's is never read)
+        key_sim[key32] = sim;
+    }
     base = __shfl(base, start, 64);
     if (act) slot[i] = base + (lane - start);
 }
@@ -921,6 +928,10 @@ struct BatchWork {
     GridDev *grids;
     int64_t *ncell;
     int32_t *cell_start, *cell_of, *rank, *perm, *cells, *slot, *tmp;
+    // simulation of each cell key (written by the binning for the keys it
+    // fills): the list count pass finds a cell's grid without going through
+    // its first particle
+    int32_t *key_sim;
     double *rec;
     int32_t *counts;
     int64_t *slice_off;
@@ -965,6 +976,7 @@ BatchWork carve(const sph_batch_desc *d, char *base)
     w.rhobar = (double *)take(sizeof(double) * (size_t)n);
     w.alphabar = (double *)take(sizeof(double) * (size_t)n);
     w.arrived = (int32_t *)take(sizeof(int32_t) * 64);
+    w.key_sim = (int32_t *)take(sizeof(int32_t) * (size_t)kc);
     w.bytes = o;
     return w;
 }
@@ -1021,7 +1033,7 @@ int batch_lists(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w)
     SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
     hipLaunchKernelGGL(k_batch_keys, dim3(sph_blocks(n, BT_THREADS)), dim3(BT_THREADS), 0,
                        ctx->stream, w.grids, d->sim_of, d->x, d->y, d->z, n, d->key_cap,
-                       w.slot, w.cell_of, w.rank, d->status);
+                       w.slot, w.cell_of, w.rank, w.key_sim, d->status);
     SPH_LAUNCH_CHECK(ctx, "k_batch_keys");
     const int32_t trash = (int32_t)(d->key_cap - 1);
     rc = sph_bin_sort(ctx, n, d->key_cap, w.cell_of, w.rank, w.cell_start, w.perm, w.cells,
@@ -1062,7 +1074,7 @@ int batch_lists(sph_ctx *ctx, const sph_batch_desc *d, const BatchWork &w)
     }
     if (rc) return rc;
     // -- neighbour lists
-    rc = sph_nnps_build_batch(ctx, w.grids, d->sim_of, rec32 ? nullptr : w.rec, rec32, d->x,
+    rc = sph_nnps_build_batch(ctx, w.grids, d->sim_of, w.key_sim, rec32 ? nullptr : w.rec, rec32, d->x,
                               d->y, d->z, w.perm, w.cell_start, w.cells, ncell_dev,
                               d->max_cells, n, d->radius_scale, d->maxch, w.counts, w.slice_off,
                               w.indices, d->list_cap, d->status, d->pairs, trash,

@@ -124,6 +124,7 @@ size_t sph_bin_sort_scratch_end(int64_t n, int64_t nc, size_t scratch_off);
 // empty rows.  With rec32 != NULL the records are float (rec unused) and
 // ox, oy, oz are the double coordinates in original order.
 int sph_nnps_build_batch(sph_ctx *ctx, const GridDev *grids, const int32_t *sim_of,
+                         const int32_t *key_sim,
                          const double *rec, const float *rec32, const double *ox,
                          const double *oy, const double *oz, const int32_t *perm,
                          const int32_t *cell_start,

@@ -1022,6 +1022,7 @@ struct CellArgs {
     // not fit set bit 0 of *status and are not written)
     const GridDev *grids;
     const int32_t *sim_of;
+    const int32_t *key_sim;  // or the simulation of each cell key (NULL: via sim_of)
     const int32_t *ncell_dev;
     int64_t nslices;
     int64_t cap;
@@ -1171,7 +1172,8 @@ __device__ void nnps_cell(const CellArgs &a, int64_t w, int32_t *__restrict__ co
     // start at 0) -- most fluid cells against a wall array
     if (!FILL && a.s_near && !a.s_near[key]) return;
     const int32_t ds = a.dstart[key], de = a.dstart[key + 1];
-    const GridDev g = a.grids ? a.grids[a.sim_of[a.dperm[ds]]] : a.g;
+    const GridDev g =
+        a.grids ? a.grids[a.key_sim ? a.key_sim[key] : a.sim_of[a.dperm[ds]]] : a.g;
     // the cell's coordinates, recomputed exactly as the binning did (from
     // the double coordinates it used)
     double c0x, c0y, c0z;
@@ -2195,6 +2197,7 @@ extern "C" int sph_cell_near(sph_ctx *ctx, const sph_grid *grid, const int32_t *
 }
 
 int sph_nnps_build_batch(sph_ctx *ctx, const GridDev *grids, const int32_t *sim_of,
+                         const int32_t *key_sim,
                          const double *rec, const float *rec32, const double *ox,
                          const double *oy, const double *oz, const int32_t *perm,
                          const int32_t *cell_start,
@@ -2210,6 +2213,7 @@ int sph_nnps_build_batch(sph_ctx *ctx, const GridDev *grids, const int32_t *sim_
     CellArgs a = cell_args_base();
     a.grids = grids;
     a.sim_of = sim_of;
+    a.key_sim = key_sim;
     a.ncell_dev = ncell_dev;
     a.f32 = rec32 != nullptr;
     a.drec = rec;

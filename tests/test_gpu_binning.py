@@ -69,3 +69,46 @@ def test_crowded_cells_sorted_by_index(gpu, sizes):
     full = {k: pa.get(k) for k in ("x", "y", "z", "h")}
     roff, ridx = cpu_ref.nnps(full, full, 2.0, cm.cell_size, True, nthreads=16)
     assert np.array_equal(off, roff) and np.array_equal(idx, ridx)
+
+
+@pytest.mark.parametrize("order", ["linear", "morton"])
+def test_large_sparse_grid(gpu, order):
+    """30,000 particles spread over a grid of 198^3 cells: the linear keys
+    (7.8M cells, 1,895 tiles of the two-launch scan of the cell counts) and
+    the Morton keys (2^24 cells, past that scan's tile limit: the multi-level
+    scans) give the same cell starts, non-empty cell list and order inside
+    the cells as a host sort of the cell keys, and lists equal to the
+    oracle's."""
+    import torch
+    from pysph_amd.base import api as base
+    rng = np.random.default_rng(17)
+    n = 30000
+    d = {k: rng.uniform(0.0, 190.0, n) for k in ("x", "y", "z")}
+    d.update(h=np.full(n, 0.5), m=np.ones(n), rho=np.ones(n))
+    pa = base.get_particle_array(name="fluid", **d)
+    particles = base.Particles([pa], cell_order=order)
+    cm = particles.cell_manager
+    cm.update()
+    pa.set_dirty(True)   # a second binning (device-side cell count)
+    cm.update()
+    torch.cuda.synchronize()
+    b = cm.bins["fluid"]
+    nc = int(cm.grid.ncells)
+    assert nc > (7_000_000 if order == "linear" else 16_000_000), nc
+    cell_of = b.cell_of.cpu().numpy()[:n]
+    want_perm = np.lexsort((np.arange(n), cell_of)).astype(np.int32)
+    assert np.array_equal(b.perm.cpu().numpy()[:n], want_perm)
+    keys = np.unique(cell_of)
+    start = b.cell_start.cpu().numpy()
+    assert start.shape[0] == nc + 1
+    want_start = np.searchsorted(np.sort(cell_of), np.arange(nc + 1), side="left")
+    assert np.array_equal(start, want_start.astype(start.dtype))
+    ncells = int(b.ncells_dev.item())
+    assert ncells == len(keys)
+    assert np.array_equal(b.cells.cpu().numpy()[:ncells], keys.astype(np.int32))
+    loc = particles.nnps_manager.get_neighbor_particle_locator(pa, pa, 2.0)
+    off, idx = (t.cpu().numpy() for t in loc.get_csr())
+    full = {k: pa.get(k) for k in ("x", "y", "z", "h")}
+    roff, ridx = cpu_ref.nnps(full, full, 2.0, cm.cell_size, True, nthreads=16)
+    assert np.array_equal(off, roff) and np.array_equal(idx, ridx)
+    print(f"{order}: {nc} cells, {ncells} non-empty, {len(idx)} list entries")

@@ -402,6 +402,37 @@ def test_device_side_rebuild_matches_and_reports_capacity(gpu):
         _ = loc.nnz
 
 
+def test_many_device_side_builds_in_flight(gpu):
+    """Twelve device-side builds queued back to back, their totals coming back
+    through a ring of four page-locked buffers: no build reports an error,
+    and the entry count read after the last is the oracle's for the last
+    positions, as are the exported lists' sizes."""
+    base, _, _ = _api()
+    from pysph_amd.solver.generators import jittered_lattice_3d
+    d = jittered_lattice_3d(n=20, seed=4, hdx=1.3)
+    pa = base.get_particle_array(name="fluid", **d)
+    particles = base.Particles([pa])
+    loc = particles.nnps_manager.get_neighbor_particle_locator(pa, pa, 2.0)
+    loc.get_csr()                                   # first build: synchronous
+    rng = np.random.default_rng(8)
+    x0 = pa.get("x")
+    want = []
+    for k in range(12):
+        x = x0 + rng.uniform(-0.3, 0.3, x0.size) * (d["h"][0] / 1.3)
+        pa.set(x=x)
+        particles.update()
+        loc.update()                                # device-side: totals pending
+        full = {kk: pa.get(kk) for kk in ("x", "y", "z", "h")}
+        roff, _ = cpu_ref.nnps(full, full, 2.0, particles.cell_manager.cell_size, True,
+                               nthreads=8)
+        want.append(int(roff[-1]))
+    assert len(set(want)) > 1                       # the builds differ
+    assert loc.nbuilds == 13
+    assert loc.nnz == want[-1]
+    off, idx = (t.cpu().numpy() for t in loc.get_csr())
+    assert len(idx) == want[-1] and off[-1] == want[-1]
+
+
 @pytest.mark.parametrize("precision", ["double", "single"])
 def test_cross_array_lists_with_near_cells_table(gpu, precision):
     """Fluid <- wall lists of the 3D dam break (dx = 0.04) over builds that

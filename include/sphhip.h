@@ -265,6 +265,39 @@ int sph_nnps_build_f32_dev(sph_ctx *ctx, const sph_grid *grid, const float *d_re
                            int64_t *slice_off, int32_t *indices, int64_t cap, int64_t *totals,
                            int32_t *status, const uint8_t *s_near, int near_reach);
 
+/* The arguments of one sph_nnps_build_dev (f32 = 0: d_rec and s_rec are
+ * double records, d_x/d_y/d_z unused) or sph_nnps_build_f32_dev (f32 = 1),
+ * with the meaning they have there. */
+typedef struct sph_nnps_dev_args {
+    int32_t f32;
+    int32_t same_array;
+    const void *d_rec;
+    const double *d_x, *d_y, *d_z;
+    const int32_t *d_perm, *d_cell_start, *d_cells, *nd_cells_dev;
+    int64_t max_cells, nd_total, nd_real;
+    const int32_t *s_cell_start;
+    const void *s_rec;
+    int64_t ns;
+    double radius_scale;
+    int32_t *counts;
+    int64_t *slice_off;
+    int32_t *indices;
+    int64_t cap;
+    int64_t *totals;
+    int32_t *status;
+    const uint8_t *s_near;
+    int32_t near_reach;
+} sph_nnps_dev_args;
+
+/* Two device-side builds over one grid (the lists of one destination from
+ * two source arrays, say) with the results of two sph_nnps_build(_f32)_dev
+ * calls, their count passes and their fills each in one launch: the smaller
+ * build's blocks run in the tail of the larger one's.  Both builds have one
+ * precision and no output buffer in common.  The reference builds each
+ * locator's cache on its own (base/nnps.pyx:566-660). */
+int sph_nnps_build_dev_pair(sph_ctx *ctx, const sph_grid *grid, const sph_nnps_dev_args *b0,
+                            const sph_nnps_dev_args *b1);
+
 /* Symmetric variable-h lists of VarHNbrParticleLocator (base/nnps.pyx:
  * 691-786, chosen by NNPSManager(variable_h=True), :895-910), in the
  * SELL-64 layout of sph_nnps_build.  Row of sorted destination t (LocalReal

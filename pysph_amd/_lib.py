@@ -71,6 +71,18 @@ class Sources(ctypes.Structure):
                 ("same", ctypes.c_int32 * MAX_SRC)]
 
 
+class NnpsDevArgs(ctypes.Structure):
+    """sph_nnps_dev_args (include/sphhip.h): one device-side list build."""
+    _fields_ = [("f32", ctypes.c_int32), ("same_array", ctypes.c_int32), ("d_rec", _vp),
+                ("d_x", _vp), ("d_y", _vp), ("d_z", _vp), ("d_perm", _vp),
+                ("d_cell_start", _vp), ("d_cells", _vp), ("nd_cells_dev", _vp),
+                ("max_cells", _i64), ("nd_total", _i64), ("nd_real", _i64),
+                ("s_cell_start", _vp), ("s_rec", _vp), ("ns", _i64),
+                ("radius_scale", ctypes.c_double), ("counts", _vp), ("slice_off", _vp),
+                ("indices", _vp), ("cap", _i64), ("totals", _vp), ("status", _vp),
+                ("s_near", _vp), ("near_reach", ctypes.c_int32)]
+
+
 PAIR_PRESSURE_GRADIENT = 1
 PAIR_ARTIFICIAL_VISCOSITY = 2
 PAIR_MOMENTUM = 3
@@ -145,6 +157,9 @@ _SIGNATURES = {
     "sph_nnps_build_dev": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _vp, _vp, _i64,
                                   _i64, _i64, _vp, _vp, _i64, _dbl, _i32, _vp, _vp, _vp, _i64,
                                   _vp, _vp, _vp, _i32]),
+    "sph_nnps_build_dev_pair": (_i32, [_vp, ctypes.POINTER(SphGrid),
+                                       ctypes.POINTER(NnpsDevArgs),
+                                       ctypes.POINTER(NnpsDevArgs)]),
     "sph_cell_near": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _i64, _vp, _vp, _vp,
                              _vp, _i32, _vp]),
     "sph_nnps_build_f32_dev": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _vp, _vp,

@@ -584,11 +584,77 @@ class FixedDestNbrParticleLocator:
             return 0
         with profiling.region("nnps_build"):
             self._build()
-        self.built_version = cm.version
+        self._built()
+        return 0
+
+    def _device_side(self, bd):
+        """Builds after the first run without a host synchronisation."""
+        return self.nbuilds > 0 and bd.ncells_dev is not None
+
+    def _buffers(self):
+        dst = self.dest
+        nd_total = dst.get_number_of_particles()
+        dev = dst.device
+        nslices = (nd_total + 63) // 64
+        if self.counts is None or self.counts.numel() != max(nd_total, 1):
+            self.counts = torch.empty(max(nd_total, 1), dtype=torch.int32, device=dev)
+            self.slice_off = torch.empty(nslices + 1, dtype=torch.int64, device=dev)
+        if self.indices is None:
+            self.indices = torch.empty(max(1, 96 * nd_total), dtype=torch.int32, device=dev)
+
+    def _dev_args(self):
+        """The arguments of a device-side build (sph_nnps_dev_args) and its
+        totals buffer.  The list buffer is grown ahead of need from the last
+        known size (the lists in it are replaced by the build anyway)."""
+        cm = self.cell_manager
+        dst, src = self.dest, self.source
+        bd, bs = cm.bins[dst.name], cm.bins[src.name]
+        dev = dst.device
+        self._buffers()
+        if self.slots > (3 * self.indices.numel()) // 4:
+            self.indices = torch.empty(int(self.slots * 1.5) + 4096, dtype=torch.int32,
+                                       device=dev)
+        if self._status is None:
+            self._status = torch.zeros(1, dtype=torch.int32, device=dev)
+        totals = torch.empty(3, dtype=torch.int64, device=dev)
+        near, reach = self._near_table(cm, bs)
+        self.near_used = near is not None
+        p = _lib.ptr
+        a = _lib.NnpsDevArgs()
+        a.f32 = int(cm.precision == "single")
+        a.same_array = int(src is dst)
+        a.d_rec = p(cm.shadows(dst, ("x", "y", "z", "h")))
+        if a.f32:
+            a.d_x, a.d_y, a.d_z = (p(dst.get_carray(c)) for c in ("x", "y", "z"))
+        a.d_perm, a.d_cell_start, a.d_cells = p(bd.perm), p(bd.cell_start), p(bd.cells)
+        a.nd_cells_dev, a.max_cells = p(bd.ncells_dev), bd.cells_bound
+        a.nd_total, a.nd_real = dst.get_number_of_particles(), dst.num_real_particles
+        a.s_cell_start, a.s_rec, a.ns = p(bs.cell_start), p(cm.shadows(src, ("x", "y", "z"))), bs.n
+        a.radius_scale = self.radius_scale
+        a.counts, a.slice_off, a.indices = p(self.counts), p(self.slice_off), p(self.indices)
+        a.cap = self.indices.numel()
+        a.totals, a.status = p(totals), p(self._status)
+        a.s_near, a.near_reach = p(near), reach
+        return a, totals
+
+    def _dev_done(self, totals):
+        """Queue the copy of a device-side build's totals to the host."""
+        if self._host_slots is None:
+            self._host_slots = _PinnedSlots(4, 3, torch.int64)
+        if len(self._pending) >= 3:
+            self._poll(block=True)  # no buffer is reused before it was read
+        host, ev = self._host_slots.take()
+        host.copy_(totals, non_blocking=True)   # {slots, nnz, status}
+        ev.record()
+        self._totals = totals
+        self._pending.append((ev, host))
+        self.nbuilds += 1
+
+    def _built(self):
+        self.built_version = self.cell_manager.version
         self.is_dirty = False
         self._export = None
         self._host = None
-        return 0
 
     def _build(self):
         cm = self.cell_manager
@@ -601,52 +667,26 @@ class FixedDestNbrParticleLocator:
         drec = cm.shadows(dst, ("x", "y", "z", "h"))
         srec = cm.shadows(src, ("x", "y", "z"))
         dev = dst.device
-        nslices = (nd_total + 63) // 64
-        if self.counts is None or self.counts.numel() != max(nd_total, 1):
-            self.counts = torch.empty(max(nd_total, 1), dtype=torch.int32, device=dev)
-            self.slice_off = torch.empty(nslices + 1, dtype=torch.int64, device=dev)
-        if self.indices is None:
-            self.indices = torch.empty(max(1, 96 * nd_total), dtype=torch.int32, device=dev)
+        self._buffers()
         p = _lib.ptr
         self._poll()
-        if self.nbuilds > 0 and bd.ncells_dev is not None:
-            # device-side build: no synchronisation.  The buffer is grown
-            # ahead of need from the last known size (the lists in it are
-            # replaced by the build below anyway).
-            if self.slots > (3 * self.indices.numel()) // 4:
-                self.indices = torch.empty(int(self.slots * 1.5) + 4096, dtype=torch.int32,
-                                           device=dev)
-            if self._status is None:
-                self._status = torch.zeros(1, dtype=torch.int32, device=dev)
-            totals = torch.empty(3, dtype=torch.int64, device=dev)
-            near, reach = self._near_table(cm, bs)
-            self.near_used = near is not None
-            if cm.precision == "single":
+        if self._device_side(bd):
+            a, totals = self._dev_args()
+            if a.f32:
                 rc = lib.sph_nnps_build_f32_dev(
-                    ctx.handle, ctypes.byref(cm.grid), p(drec), p(dst.get_carray("x")),
-                    p(dst.get_carray("y")), p(dst.get_carray("z")), p(bd.perm),
-                    p(bd.cell_start), p(bd.cells), p(bd.ncells_dev), bd.cells_bound, nd_total,
-                    nd_real, p(bs.cell_start), p(srec), bs.n, self.radius_scale,
-                    int(src is dst), p(self.counts), p(self.slice_off), p(self.indices),
-                    self.indices.numel(), p(totals), p(self._status), p(near), reach)
+                    ctx.handle, ctypes.byref(cm.grid), a.d_rec, a.d_x, a.d_y, a.d_z, a.d_perm,
+                    a.d_cell_start, a.d_cells, a.nd_cells_dev, a.max_cells, a.nd_total,
+                    a.nd_real, a.s_cell_start, a.s_rec, a.ns, a.radius_scale, a.same_array,
+                    a.counts, a.slice_off, a.indices, a.cap, a.totals, a.status, a.s_near,
+                    a.near_reach)
             else:
                 rc = lib.sph_nnps_build_dev(
-                    ctx.handle, ctypes.byref(cm.grid), p(drec), p(bd.perm), p(bd.cell_start),
-                    p(bd.cells), p(bd.ncells_dev), bd.cells_bound, nd_total, nd_real,
-                    p(bs.cell_start), p(srec), bs.n, self.radius_scale, int(src is dst),
-                    p(self.counts), p(self.slice_off), p(self.indices), self.indices.numel(),
-                    p(totals), p(self._status), p(near), reach)
+                    ctx.handle, ctypes.byref(cm.grid), a.d_rec, a.d_perm, a.d_cell_start,
+                    a.d_cells, a.nd_cells_dev, a.max_cells, a.nd_total, a.nd_real,
+                    a.s_cell_start, a.s_rec, a.ns, a.radius_scale, a.same_array, a.counts,
+                    a.slice_off, a.indices, a.cap, a.totals, a.status, a.s_near, a.near_reach)
             ctx.check(rc)
-            if self._host_slots is None:
-                self._host_slots = _PinnedSlots(4, 3, torch.int64)
-            if len(self._pending) >= 3:
-                self._poll(block=True)  # no buffer is reused before it was read
-            host, ev = self._host_slots.take()
-            host.copy_(totals, non_blocking=True)   # {slots, nnz, status}
-            ev.record()
-            self._totals = totals
-            self._pending.append((ev, host))
-            self.nbuilds += 1
+            self._dev_done(totals)
             return
         slots = ctypes.c_int64(0)
         nnz = ctypes.c_int64(0)
@@ -675,6 +715,9 @@ class FixedDestNbrParticleLocator:
         self.nnz = int(nnz.value)
         self.slots = int(slots.value)
         self.nbuilds += 1
+
+    def _stale(self):
+        return self.built_version != self.cell_manager.version or self.is_dirty
 
     def get_sorted_lists(self):
         """SELL-64 lists in sorted space: (counts int32[nd], slice_off
@@ -722,6 +765,41 @@ class FixedDestNbrParticleLocator:
 
     def py_update_status(self):
         return self.update_status()
+
+
+def update_together(locators):
+    """Bring the lists of several locators up to date, as their update()
+    calls in turn would.  Two stale device-side builds of fixed-h lists over
+    one cell manager and one destination (a fluid's lists from itself and
+    from a wall, say) go out as one paired build (sph_nnps_build_dev_pair):
+    the smaller build runs in the tail of the larger one's launches."""
+    pair = []
+    for loc in locators:
+        if type(loc) is not FixedDestNbrParticleLocator or loc in pair:
+            continue
+        cm = loc.cell_manager
+        cm.update()
+        if not loc._stale() or not loc._device_side(cm.bins[loc.dest.name]):
+            continue
+        if pair and (cm is not pair[0].cell_manager or loc.dest is not pair[0].dest):
+            continue
+        pair.append(loc)
+        if len(pair) == 2:
+            break
+    if len(pair) == 2:
+        cm = pair[0].cell_manager
+        ctx = cm._ctx()
+        with profiling.region("nnps_build"):
+            for loc in pair:
+                loc._poll()
+            (a0, t0), (a1, t1) = (loc._dev_args() for loc in pair)
+            ctx.check(ctx.lib.sph_nnps_build_dev_pair(ctx.handle, ctypes.byref(cm.grid),
+                                                      ctypes.byref(a0), ctypes.byref(a1)))
+            for loc, t in zip(pair, (t0, t1)):
+                loc._dev_done(t)
+                loc._built()
+    for loc in locators:
+        loc.update()
 
 
 class VarHNbrParticleLocator(FixedDestNbrParticleLocator):

@@ -1357,6 +1357,37 @@ __device__ void nnps_cell(const CellArgs &a, int64_t w, int32_t *__restrict__ co
 // FILL = false: every non-empty destination cell (XCD x walks the x-th
 // eighth of the cells with its blocks).  FILL = true: the cells the count
 // pass flagged, listed in a.ovf_list.
+// The count pass of one build over block bid of G: every non-empty destination
+// cell (XCD x walks the x-th eighth of the cells with its blocks; bid is
+// the block's XCD-consistent index, bid % 8 being its XCD).
+template <typename T>
+__device__ __forceinline__ void count_cells(const CellArgs &a, int32_t *__restrict__ counts,
+                                            unsigned bid, unsigned G, int32_t *seg_start,
+                                            int32_t *seg_end, typename Quad<T>::type *dsts)
+{
+    // cells read on the device are bounded by the launch size the scratch
+    // was carved for (more cells than that is a status error of the caller,
+    // whose lists are then incomplete, never written past)
+    int64_t ncl = a.ncell_list;
+    if (a.ncell_dev && (int64_t)*a.ncell_dev < ncl) ncl = *a.ncell_dev;
+    // blocks go round-robin to the XCDs (block b to XCD b % 8): XCD x walks
+    // cells [x*ncl/8, (x+1)*ncl/8) with its blocks, so every XCD has an
+    // equal, contiguous share whatever the grid size (the grid is sized for a
+    // bound when the cell count is read on the device)
+    if (G < SPH_NXCD) {  // too few blocks to give every XCD one
+        for (int64_t w = bid; w < ncl; w += G)
+            nnps_cell<false, T>(a, w, counts, nullptr, nullptr, seg_start, seg_end, dsts);
+        return;
+    }
+    const unsigned x = bid % SPH_NXCD, k = bid / SPH_NXCD;
+    const unsigned per = (G - x + SPH_NXCD - 1) / SPH_NXCD;  // blocks on XCD x (>= 1)
+    const int64_t lo = ncl * x / SPH_NXCD, hi = ncl * (x + 1) / SPH_NXCD;
+    for (int64_t w = lo + k; w < hi; w += per)  // uniform: one wavefront per block
+        nnps_cell<false, T>(a, w, counts, nullptr, nullptr, seg_start, seg_end, dsts);
+}
+
+// FILL = false: the count pass (count_cells).  FILL = true: the cells the
+// count pass flagged, listed in a.ovf_list.
 template <bool FILL, typename T>
 __global__ void __launch_bounds__(CW)
     k_nnps_cells(CellArgs a, int32_t *__restrict__ counts, const int64_t *__restrict__ slice_off,
@@ -1372,27 +1403,25 @@ __global__ void __launch_bounds__(CW)
             nnps_cell<true, T>(a, a.ovf_list[k], counts, slice_off, indices, seg_start, seg_end,
                                dsts);
     } else {
-        // cells read on the device are bounded by the launch size the
-        // scratch was carved for (more cells than that is a status error of
-        // the caller, whose lists are then incomplete, never written past)
-        int64_t ncl = a.ncell_list;
-        if (a.ncell_dev && (int64_t)*a.ncell_dev < ncl) ncl = *a.ncell_dev;
-        // blocks go round-robin to the XCDs (block b to XCD b % 8): XCD x
-        // walks cells [x*ncl/8, (x+1)*ncl/8) with its blocks, so every XCD
-        // has an equal, contiguous share whatever the grid size (the grid is
-        // sized for a bound when the cell count is read on the device)
-        const unsigned G = gridDim.x;
-        if (G < SPH_NXCD) {  // too few blocks to give every XCD one
-            for (int64_t w = blockIdx.x; w < ncl; w += G)
-                nnps_cell<false, T>(a, w, counts, slice_off, indices, seg_start, seg_end, dsts);
-            return;
-        }
-        const unsigned x = blockIdx.x % SPH_NXCD, k = blockIdx.x / SPH_NXCD;
-        const unsigned per = (G - x + SPH_NXCD - 1) / SPH_NXCD;  // blocks on XCD x (>= 1)
-        const int64_t lo = ncl * x / SPH_NXCD, hi = ncl * (x + 1) / SPH_NXCD;
-        for (int64_t w = lo + k; w < hi; w += per)  // uniform: one wavefront per block
-            nnps_cell<false, T>(a, w, counts, slice_off, indices, seg_start, seg_end, dsts);
+        count_cells<T>(a, counts, blockIdx.x, gridDim.x, seg_start, seg_end, dsts);
     }
+}
+
+// The count passes of two builds in one launch: blocks [0, g0) the first
+// (g0 a multiple of the XCD count, so block indices keep their XCD), the
+// rest the second -- the smaller build fills the larger one's tail.
+template <typename T>
+__global__ void __launch_bounds__(CW)
+    k_nnps_cells_pair(CellArgs a0, int32_t *__restrict__ counts0, CellArgs a1,
+                      int32_t *__restrict__ counts1, unsigned g0)
+{
+    __shared__ int32_t seg_start[MAXSEG];
+    __shared__ int32_t seg_end[MAXSEG];
+    __shared__ typename Quad<T>::type dsts[CW];
+    if (blockIdx.x < g0)
+        count_cells<T>(a0, counts0, blockIdx.x, g0, seg_start, seg_end, dsts);
+    else
+        count_cells<T>(a1, counts1, blockIdx.x - g0, gridDim.x - g0, seg_start, seg_end, dsts);
 }
 
 // SELL-64 rows from the count pass's hit masks: one lane per destination,
@@ -1408,13 +1437,14 @@ __global__ void __launch_bounds__(CW)
 // skipped.
 #define VARH_COL (2 * NN_MAXCH)
 template <bool VARH>
-__global__ void __launch_bounds__(NN_THREADS)
-    k_nnps_fill_slices(CellArgs a, int64_t nd_total, int32_t *__restrict__ counts,
-                       const int64_t *__restrict__ slice_off, int32_t *__restrict__ indices,
-                       const int32_t *__restrict__ srank)
+__device__ __forceinline__ void fill_rows(const CellArgs &a, int64_t nd_total,
+                                          int32_t *__restrict__ counts,
+                                          const int64_t *__restrict__ slice_off,
+                                          int32_t *__restrict__ indices,
+                                          const int32_t *__restrict__ srank, unsigned bid,
+                                          unsigned G, uint64_t (*s_mask)[NN_THREADS])
 {
-    __shared__ uint64_t s_mask[NN_MAXCH][NN_THREADS];
-    if (a.totals && blockIdx.x == 0 && threadIdx.x == 0) {
+    if (a.totals && bid == 0 && threadIdx.x == 0) {
         // totals and capacity checks of a device-side build: status bit 0
         // when the lists need more than cap slots (nothing is written), bit
         // 4 when more cells are non-empty than the launch bound
@@ -1427,7 +1457,7 @@ __global__ void __launch_bounds__(NN_THREADS)
         a.totals[1] = *a.nnz_dev;
         a.totals[2] = st;
     }
-    const int64_t t = (int64_t)sph_xcd_tile(blockIdx.x, gridDim.x) * NN_THREADS + threadIdx.x;
+    const int64_t t = (int64_t)sph_xcd_tile(bid, G) * NN_THREADS + threadIdx.x;
     if (t >= nd_total) return;
     if (a.status && slice_off[a.nslices] > a.cap) {
         counts[t] = 0;  // no room (status is set): empty rows, so no reader touches the slots
@@ -1583,6 +1613,34 @@ __global__ void __launch_bounds__(NN_THREADS)
         }
     }
     a.long_rows[atomicAdd(a.nlong, 1)] = (int32_t)t;
+}
+
+template <bool VARH>
+__global__ void __launch_bounds__(NN_THREADS)
+    k_nnps_fill_slices(CellArgs a, int64_t nd_total, int32_t *__restrict__ counts,
+                       const int64_t *__restrict__ slice_off, int32_t *__restrict__ indices,
+                       const int32_t *__restrict__ srank)
+{
+    __shared__ uint64_t s_mask[NN_MAXCH][NN_THREADS];
+    fill_rows<VARH>(a, nd_total, counts, slice_off, indices, srank, blockIdx.x, gridDim.x,
+                    s_mask);
+}
+
+// The fills of two fixed-h builds in one launch (blocks [0, g0) the first, g0
+// a multiple of the XCD count)
+__global__ void __launch_bounds__(NN_THREADS)
+    k_nnps_fill_pair(CellArgs a0, int64_t nd0, int32_t *__restrict__ counts0,
+                     const int64_t *__restrict__ slice_off0, int32_t *__restrict__ indices0,
+                     CellArgs a1, int64_t nd1, int32_t *__restrict__ counts1,
+                     const int64_t *__restrict__ slice_off1, int32_t *__restrict__ indices1,
+                     unsigned g0)
+{
+    __shared__ uint64_t s_mask[NN_MAXCH][NN_THREADS];
+    if (blockIdx.x < g0)
+        fill_rows<false>(a0, nd0, counts0, slice_off0, indices0, nullptr, blockIdx.x, g0, s_mask);
+    else
+        fill_rows<false>(a1, nd1, counts1, slice_off1, indices1, nullptr, blockIdx.x - g0,
+                         gridDim.x - g0, s_mask);
 }
 
 // The appended parts the fill left unsorted (longer than its LDS column): one
@@ -1859,49 +1917,94 @@ __global__ void k_nnps_dev_status(const int64_t *__restrict__ slots, const int64
 // Count pass, slice offsets and totals, then (after the caller's check of
 // the capacity) the fill.  Scratch: [hit masks | cell flags | candidate
 // sources | cell of destination | slice sums | scan partials].
-static int nnps_count(sph_ctx *ctx, CellArgs &a, int64_t ncell_launch, int64_t nd_total,
-                      int32_t *counts, int64_t *slice_off, int64_t **nnz_dev)
+// Scratch of one build's count pass, from base (bytes, 256-aligned
+// regions): hit masks | flagged cells | candidate tables | cell of each
+// destination | flagged-cell list | slice sums | scan partials
+struct CountScratch {
+    size_t masks, ovf, cand, dcell, list, sums, scan, total;
+};
+
+static CountScratch count_scratch(const CellArgs &a, int64_t ncell_launch, int64_t nd_total)
 {
-    const int64_t nslices = (nd_total + 63) / 64;
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t masks_bytes = al((size_t)a.maxch * nd_total * sizeof(uint64_t));
-    const size_t ovf_bytes = al((size_t)ncell_launch * sizeof(int32_t));
-    const size_t cand_bytes = al((size_t)ncell_launch * a.maxch * CW * sizeof(int32_t));
-    const size_t dcell_bytes = al((size_t)nd_total * sizeof(int32_t));
-    const size_t list_bytes = al((size_t)(ncell_launch + 1) * sizeof(int32_t));
-    const size_t sums_bytes = slice_sums_bytes(nslices);
-    const size_t sums_off = masks_bytes + ovf_bytes + cand_bytes + dcell_bytes + list_bytes;
-    int rc = sph_ensure_scratch(ctx, sums_off + sums_bytes +
-                                         sph_scan_scratch_bytes(nslices, sizeof(int64_t)));
-    if (rc) return rc;
-    char *sc = (char *)ctx->scratch;
+    const int64_t nslices = (nd_total + 63) / 64;
+    CountScratch c;
+    c.masks = al((size_t)a.maxch * nd_total * sizeof(uint64_t));
+    c.ovf = al((size_t)ncell_launch * sizeof(int32_t));
+    c.cand = al((size_t)ncell_launch * a.maxch * CW * sizeof(int32_t));
+    c.dcell = al((size_t)nd_total * sizeof(int32_t));
+    c.list = al((size_t)(ncell_launch + 1) * sizeof(int32_t));
+    c.sums = slice_sums_bytes(nslices);
+    c.scan = al(sph_scan_scratch_bytes(nslices, sizeof(int64_t)));
+    c.total = c.masks + c.ovf + c.cand + c.dcell + c.list + c.sums + c.scan;
+    return c;
+}
+
+// the count pass's pointers in the scratch at offset base; the slice sums
+// and the scan partials' offset for the slice offsets
+static void count_carve(sph_ctx *ctx, CellArgs &a, const CountScratch &c, size_t base,
+                        int64_t nd_total, int64_t **slice_sum, size_t *scan_off)
+{
+    char *sc = (char *)ctx->scratch + base;
     a.masks = (uint64_t *)sc;
     a.mstride = nd_total;
-    a.cell_ovf = (int32_t *)(sc + masks_bytes);
-    a.cand = (int32_t *)(sc + masks_bytes + ovf_bytes);
-    a.dcell_of = (int32_t *)(sc + masks_bytes + ovf_bytes + cand_bytes);
-    a.novf = (int32_t *)(sc + masks_bytes + ovf_bytes + cand_bytes + dcell_bytes);
+    a.cell_ovf = (int32_t *)(sc + c.masks);
+    a.cand = (int32_t *)(sc + c.masks + c.ovf);
+    a.dcell_of = (int32_t *)(sc + c.masks + c.ovf + c.cand);
+    a.novf = (int32_t *)(sc + c.masks + c.ovf + c.cand + c.dcell);
     a.ovf_list = a.novf + 1;
-    a.nslices = nslices;
-    int64_t *slice_sum = (int64_t *)(sc + sums_off);
-    *nnz_dev = slice_sum + nslices;
+    a.nslices = (nd_total + 63) / 64;
+    const size_t sums_off = c.masks + c.ovf + c.cand + c.dcell + c.list;
+    *slice_sum = (int64_t *)(sc + sums_off);
+    *scan_off = base + sums_off + c.sums;
+}
+
+static int count_reset(sph_ctx *ctx, const CellArgs &a, int32_t *counts, int64_t nd_total)
+{
     hipLaunchKernelGGL(k_nnps_reset, dim3(sph_blocks(nd_total, NN_THREADS)), dim3(NN_THREADS), 0,
                        ctx->stream, counts, nd_total, a.novf);
     SPH_LAUNCH_CHECK(ctx, "k_nnps_reset");
-    auto count_kern = a.f32 ? k_nnps_cells<false, float> : k_nnps_cells<false, double>;
-    const int cpb = a.cells_per_block > 0 ? a.cells_per_block : 1;
-    const int64_t cgrid = (ncell_launch + cpb - 1) / cpb;
-    const int64_t count_grid = cgrid < NN_CELL_GRID ? cgrid : NN_CELL_GRID;
-    hipLaunchKernelGGL(count_kern, dim3((unsigned)count_grid), dim3(CW), 0, ctx->stream, a,
-                       counts, (const int64_t *)nullptr, (int32_t *)nullptr);
-    SPH_LAUNCH_CHECK(ctx, "k_nnps_cells<count>");
-    // slice widths -> offsets (scan); slice totals -> entry total
+    return SPH_OK;
+}
+
+// slice widths -> offsets (scan); slice totals -> entry total
+static int count_slices(sph_ctx *ctx, const CellArgs &a, const int32_t *counts,
+                        int64_t *slice_off, int64_t *slice_sum, size_t scan_off,
+                        int64_t nd_total)
+{
+    const int64_t nslices = a.nslices;
     hipLaunchKernelGGL(k_slices, dim3(sph_blocks(nslices * 64, NN_THREADS)), dim3(NN_THREADS), 0,
                        ctx->stream, counts, nd_total, nslices, slice_off, slice_sum,
                        slice_sum + nslices + 1);
     SPH_LAUNCH_CHECK(ctx, "k_slices");
     return slices_to_offsets(ctx, slice_off, slice_sum + nslices + 1, slice_sum, nslices,
-                             *nnz_dev, sums_off + sums_bytes);
+                             slice_sum + nslices, scan_off);
+}
+
+static int64_t count_grid(const CellArgs &a, int64_t ncell_launch)
+{
+    const int cpb = a.cells_per_block > 0 ? a.cells_per_block : 1;
+    const int64_t cgrid = (ncell_launch + cpb - 1) / cpb;
+    return cgrid < NN_CELL_GRID ? cgrid : NN_CELL_GRID;
+}
+
+static int nnps_count(sph_ctx *ctx, CellArgs &a, int64_t ncell_launch, int64_t nd_total,
+                      int32_t *counts, int64_t *slice_off, int64_t **nnz_dev)
+{
+    const CountScratch c = count_scratch(a, ncell_launch, nd_total);
+    int rc = sph_ensure_scratch(ctx, c.total);
+    if (rc) return rc;
+    int64_t *slice_sum;
+    size_t scan_off;
+    count_carve(ctx, a, c, 0, nd_total, &slice_sum, &scan_off);
+    *nnz_dev = slice_sum + a.nslices;
+    rc = count_reset(ctx, a, counts, nd_total);
+    if (rc) return rc;
+    auto count_kern = a.f32 ? k_nnps_cells<false, float> : k_nnps_cells<false, double>;
+    hipLaunchKernelGGL(count_kern, dim3((unsigned)count_grid(a, ncell_launch)), dim3(CW), 0,
+                       ctx->stream, a, counts, (const int64_t *)nullptr, (int32_t *)nullptr);
+    SPH_LAUNCH_CHECK(ctx, "k_nnps_cells<count>");
+    return count_slices(ctx, a, counts, slice_off, slice_sum, scan_off, nd_total);
 }
 
 static int nnps_fill(sph_ctx *ctx, const CellArgs &a, int64_t ncell_launch, int64_t nd_total,
@@ -1974,15 +2077,26 @@ struct NnpsDevOut {
     int32_t *status;
 };
 
-static int nnps_build(sph_ctx *ctx, const sph_grid *grid, int f32, const double *d_rec,
-                      const float *d_rec32, const double *ox, const double *oy,
-                      const double *oz, const int32_t *d_perm, const int32_t *d_cell_start,
-                      const int32_t *d_cells, int64_t nd_cells, int64_t nd_total,
-                      int64_t nd_real, const int32_t *s_cell_start, const double *s_rec,
-                      const float *s_rec32, int64_t ns, double radius_scale, int same_array,
-                      int32_t *counts, int64_t *slice_off, int32_t *indices, int64_t cap,
-                      int64_t *slots_out, int64_t *nnz_out, const NnpsDevOut *dev,
-                      const uint8_t *s_near = nullptr, int near_reach = 0)
+// One build's validated arguments (nnps_prepare).  empty: every row was
+// cleared (and a device-side build's totals written), nothing more to launch.
+struct NnpsJob {
+    CellArgs a;
+    int64_t nd_cells, nd_total;
+    int32_t *counts;
+    int64_t *slice_off;
+    int32_t *indices;
+    bool empty;
+};
+
+static int nnps_prepare(sph_ctx *ctx, const sph_grid *grid, int f32, const double *d_rec,
+                        const float *d_rec32, const double *ox, const double *oy,
+                        const double *oz, const int32_t *d_perm, const int32_t *d_cell_start,
+                        const int32_t *d_cells, int64_t nd_cells, int64_t nd_total,
+                        int64_t nd_real, const int32_t *s_cell_start, const double *s_rec,
+                        const float *s_rec32, int64_t ns, double radius_scale, int same_array,
+                        int32_t *counts, int64_t *slice_off, int32_t *indices, int64_t cap,
+                        int64_t *slots_out, int64_t *nnz_out, const NnpsDevOut *dev,
+                        const uint8_t *s_near, int near_reach, NnpsJob *job)
 {
     if (!ctx) return SPH_EINVAL;
     SPH_REQUIRE(ctx, !s_near || (near_reach >= 1 && near_reach <= 8),
@@ -2005,8 +2119,13 @@ static int nnps_build(sph_ctx *ctx, const sph_grid *grid, int f32, const double 
         *slots_out = 0;
         *nnz_out = 0;
     }
-    const int64_t nslices = (nd_total + 63) / 64;
-    if (nd_total == 0 || ns == 0 || nd_real == 0 || nd_cells == 0) {
+    job->nd_cells = nd_cells;
+    job->nd_total = nd_total;
+    job->counts = counts;
+    job->slice_off = slice_off;
+    job->indices = indices;
+    job->empty = nd_total == 0 || ns == 0 || nd_real == 0 || nd_cells == 0;
+    if (job->empty) {
         rc = nnps_clear(ctx, nd_total, counts, slice_off);  // every row empty
         if (rc) return rc;
         if (dev) {
@@ -2018,7 +2137,8 @@ static int nnps_build(sph_ctx *ctx, const sph_grid *grid, int f32, const double 
         }
         return SPH_OK;
     }
-    CellArgs a = cell_args_base();
+    CellArgs &a = job->a;
+    a = cell_args_base();
     a.g = make_griddev(ctx, grid);
     a.f32 = f32;
     a.drec = d_rec;
@@ -2041,14 +2161,35 @@ static int nnps_build(sph_ctx *ctx, const sph_grid *grid, int f32, const double 
         a.ncell_dev = dev->ncells_dev;
         a.cap = cap;
         a.status = dev->status;
+        a.totals = dev->totals;  // the fill's first thread writes the totals and the status
+        a.cell_bound = nd_cells;
     }
+    return SPH_OK;
+}
+
+static int nnps_build(sph_ctx *ctx, const sph_grid *grid, int f32, const double *d_rec,
+                      const float *d_rec32, const double *ox, const double *oy,
+                      const double *oz, const int32_t *d_perm, const int32_t *d_cell_start,
+                      const int32_t *d_cells, int64_t nd_cells, int64_t nd_total,
+                      int64_t nd_real, const int32_t *s_cell_start, const double *s_rec,
+                      const float *s_rec32, int64_t ns, double radius_scale, int same_array,
+                      int32_t *counts, int64_t *slice_off, int32_t *indices, int64_t cap,
+                      int64_t *slots_out, int64_t *nnz_out, const NnpsDevOut *dev,
+                      const uint8_t *s_near = nullptr, int near_reach = 0)
+{
+    NnpsJob job;
+    int rc = nnps_prepare(ctx, grid, f32, d_rec, d_rec32, ox, oy, oz, d_perm, d_cell_start,
+                          d_cells, nd_cells, nd_total, nd_real, s_cell_start, s_rec, s_rec32, ns,
+                          radius_scale, same_array, counts, slice_off, indices, cap, slots_out,
+                          nnz_out, dev, s_near, near_reach, &job);
+    if (rc || job.empty) return rc;
+    CellArgs &a = job.a;
+    const int64_t nslices = (nd_total + 63) / 64;
     int64_t *nnz_dev = nullptr;
     rc = nnps_count(ctx, a, nd_cells, nd_total, counts, slice_off, &nnz_dev);
     if (rc) return rc;
-    if (dev) {  // the fill's first thread writes the totals and the status
-        a.totals = dev->totals;
+    if (dev) {
         a.nnz_dev = nnz_dev;
-        a.cell_bound = nd_cells;
         return nnps_fill(ctx, a, nd_cells, nd_total, counts, slice_off, indices);
     }
     rc = sph_ensure_pinned(ctx, 2 * sizeof(int64_t));
@@ -2067,6 +2208,97 @@ static int nnps_build(sph_ctx *ctx, const sph_grid *grid, int f32, const double 
         return SPH_ENEEDCAP;
     }
     return nnps_fill(ctx, a, nd_cells, nd_total, counts, slice_off, indices);
+}
+
+static int nnps_prepare_dev(sph_ctx *ctx, const sph_grid *grid, const sph_nnps_dev_args *b,
+                            const NnpsDevOut *dev, NnpsJob *job)
+{
+    SPH_REQUIRE(ctx, b, "sph_nnps_build_dev_pair: null build arguments");
+    const int f32 = b->f32 != 0;
+    return nnps_prepare(ctx, grid, f32, f32 ? nullptr : (const double *)b->d_rec,
+                        f32 ? (const float *)b->d_rec : nullptr, b->d_x, b->d_y, b->d_z,
+                        b->d_perm, b->d_cell_start, b->d_cells, b->max_cells, b->nd_total,
+                        b->nd_real, b->s_cell_start, f32 ? nullptr : (const double *)b->s_rec,
+                        f32 ? (const float *)b->s_rec : nullptr, b->ns, b->radius_scale,
+                        b->same_array, b->counts, b->slice_off, b->indices, b->cap, nullptr,
+                        nullptr, dev, b->s_near, b->near_reach, job);
+}
+
+// blocks of the first build of a paired launch: a multiple of the XCD count,
+// so the second build's block b - g0 runs on the XCD block b does
+static unsigned pair_split(int64_t g)
+{
+    return (unsigned)((g + SPH_NXCD - 1) / SPH_NXCD * SPH_NXCD);
+}
+
+extern "C" int sph_nnps_build_dev_pair(sph_ctx *ctx, const sph_grid *grid,
+                                       const sph_nnps_dev_args *b0,
+                                       const sph_nnps_dev_args *b1)
+{
+    if (!ctx) return SPH_EINVAL;
+    SPH_REQUIRE(ctx, b0 && b1, "sph_nnps_build_dev_pair: null build arguments");
+    SPH_REQUIRE(ctx, (b0->f32 != 0) == (b1->f32 != 0),
+                "sph_nnps_build_dev_pair: both builds must have one precision");
+    SPH_REQUIRE(ctx, b0->counts != b1->counts && b0->slice_off != b1->slice_off &&
+                         b0->indices != b1->indices && b0->totals != b1->totals,
+                "sph_nnps_build_dev_pair: the two builds share an output buffer");
+    const NnpsDevOut dev0 = {b0->nd_cells_dev, b0->totals, b0->status};
+    const NnpsDevOut dev1 = {b1->nd_cells_dev, b1->totals, b1->status};
+    NnpsJob j0, j1;
+    int rc = nnps_prepare_dev(ctx, grid, b0, &dev0, &j0);
+    if (rc) return rc;
+    rc = nnps_prepare_dev(ctx, grid, b1, &dev1, &j1);
+    if (rc) return rc;
+    if (j0.empty || j1.empty) {  // one build left (or none): its own launches
+        NnpsJob *j = j0.empty ? &j1 : &j0;
+        if (j->empty) return SPH_OK;
+        int64_t *nnz_dev = nullptr;
+        rc = nnps_count(ctx, j->a, j->nd_cells, j->nd_total, j->counts, j->slice_off, &nnz_dev);
+        if (rc) return rc;
+        j->a.nnz_dev = nnz_dev;
+        return nnps_fill(ctx, j->a, j->nd_cells, j->nd_total, j->counts, j->slice_off,
+                         j->indices);
+    }
+    // one scratch allocation for both, so neither carve moves
+    const CountScratch c0 = count_scratch(j0.a, j0.nd_cells, j0.nd_total);
+    const CountScratch c1 = count_scratch(j1.a, j1.nd_cells, j1.nd_total);
+    rc = sph_ensure_scratch(ctx, c0.total + c1.total);
+    if (rc) return rc;
+    int64_t *sum0, *sum1;
+    size_t scan0, scan1;
+    count_carve(ctx, j0.a, c0, 0, j0.nd_total, &sum0, &scan0);
+    count_carve(ctx, j1.a, c1, c0.total, j1.nd_total, &sum1, &scan1);
+    j0.a.nnz_dev = sum0 + j0.a.nslices;
+    j1.a.nnz_dev = sum1 + j1.a.nslices;
+    rc = count_reset(ctx, j0.a, j0.counts, j0.nd_total);
+    if (rc) return rc;
+    rc = count_reset(ctx, j1.a, j1.counts, j1.nd_total);
+    if (rc) return rc;
+    const unsigned g0 = pair_split(count_grid(j0.a, j0.nd_cells));
+    const int64_t g1 = count_grid(j1.a, j1.nd_cells);
+    auto count_kern = j0.a.f32 ? k_nnps_cells_pair<float> : k_nnps_cells_pair<double>;
+    hipLaunchKernelGGL(count_kern, dim3((unsigned)(g0 + g1)), dim3(CW), 0, ctx->stream, j0.a,
+                       j0.counts, j1.a, j1.counts, g0);
+    SPH_LAUNCH_CHECK(ctx, "k_nnps_cells_pair<count>");
+    rc = count_slices(ctx, j0.a, j0.counts, j0.slice_off, sum0, scan0, j0.nd_total);
+    if (rc) return rc;
+    rc = count_slices(ctx, j1.a, j1.counts, j1.slice_off, sum1, scan1, j1.nd_total);
+    if (rc) return rc;
+    const unsigned f0 = pair_split(sph_blocks(j0.nd_total, NN_THREADS));
+    const int64_t f1 = sph_blocks(j1.nd_total, NN_THREADS);
+    hipLaunchKernelGGL(k_nnps_fill_pair, dim3((unsigned)(f0 + f1)), dim3(NN_THREADS), 0,
+                       ctx->stream, j0.a, j0.nd_total, j0.counts, (const int64_t *)j0.slice_off,
+                       j0.indices, j1.a, j1.nd_total, j1.counts, (const int64_t *)j1.slice_off,
+                       j1.indices, f0);
+    SPH_LAUNCH_CHECK(ctx, "k_nnps_fill_pair");
+    for (NnpsJob *j : {&j0, &j1}) {  // the cells the count pass flagged
+        auto fill_kern = j->a.f32 ? k_nnps_cells<true, float> : k_nnps_cells<true, double>;
+        const int64_t fill_grid = j->nd_cells < NN_OVF_GRID ? j->nd_cells : NN_OVF_GRID;
+        hipLaunchKernelGGL(fill_kern, dim3((unsigned)fill_grid), dim3(CW), 0, ctx->stream, j->a,
+                           j->counts, (const int64_t *)j->slice_off, j->indices);
+        SPH_LAUNCH_CHECK(ctx, "k_nnps_cells<fill>");
+    }
+    return SPH_OK;
 }
 
 extern "C" int sph_nnps_build(sph_ctx *ctx, const sph_grid *grid, const double *d_rec,
@@ -2821,8 +3053,8 @@ static int nnps_build_varh(sph_ctx *ctx, const sph_grid *grid, const T *d_rec, c
     hipLaunchKernelGGL(k_varh_long_rows, dim3(4096), dim3(CW), 0, ctx->stream, c,
                        (const int32_t *)counts, (const int64_t *)slice_off, indices, s_rank);
     SPH_LAUNCH_CHECK(ctx, "k_varh_long_rows");
-    auto fill_rows = k_varh_rows<true, T>;
-    hipLaunchKernelGGL(fill_rows, dim3(dgrid), dim3(NN_THREADS), 0, ctx->stream, a, counts,
+    auto varh_fill_kern = k_varh_rows<true, T>;
+    hipLaunchKernelGGL(varh_fill_kern, dim3(dgrid), dim3(NN_THREADS), 0, ctx->stream, a, counts,
                        (const int64_t *)slice_off, indices, (const int32_t *)c.dcell_of,
                        (const int32_t *)c.cell_ovf);
     SPH_LAUNCH_CHECK(ctx, "k_varh_rows<fill>");

@@ -21,7 +21,7 @@ import os
 import torch
 
 from .. import _lib, profiling
-from ..base.nnps import PRHO2_FIELD, REC, REC_FIELDS
+from ..base.nnps import PRHO2_FIELD, REC, REC_FIELDS, update_together
 from .sph_func import CSPHFunctionParticle, SPHFunction, SPHFunctionParticle
 
 _identity = {}
@@ -114,6 +114,7 @@ class SPHRho(CSPHFunctionParticle):
         p = _lib.ptr
         src = _lib.Sources()
         src.nsrc = len(funcs)
+        update_together([f.nbr_locator for f in funcs])
         for k, f in enumerate(funcs):
             loc = f.nbr_locator
             cnt, soff, idx = loc.get_sorted_lists()
@@ -329,6 +330,7 @@ class MomentumEquation(SPHFunctionParticle):
         src = _lib.Sources()
         src.nsrc = len(funcs)
         fresh = []
+        update_together([f.nbr_locator for f in funcs])
         for k, f in enumerate(funcs):
             loc = f.nbr_locator
             cnt, soff, idx = loc.get_sorted_lists()
@@ -384,6 +386,7 @@ def _pair_launch(funcs, kernel, output1, output2, output3, accumulate, names):
     if prm.op == _lib.PAIR_BOUNDARY_FORCE:
         frames = _lib.WallFrames()
     fresh = []
+    update_together([f.nbr_locator for f in funcs])
     for k, f in enumerate(funcs):
         loc = f.nbr_locator
         cnt, soff, idx = loc.get_sorted_lists()

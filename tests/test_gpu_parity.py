@@ -492,6 +492,77 @@ def test_cross_array_lists_with_near_cells_table(gpu, precision):
     print(f"fluid h x 1.5 after the grid build: {len(idx)} entries, no near table")
 
 
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_paired_device_side_builds(gpu, precision):
+    """Fluid <- fluid and fluid <- wall lists of the 3D dam break (dx = 0.04)
+    built together (nnps.update_together: one count launch and one fill
+    launch for both, sph_nnps_build_dev_pair) equal the same builds made one
+    at a time, over moves of the fluid, with and without the wall's
+    near-cells table; in FP64 also the oracle's.  An empty wall leaves the
+    fluid's own build to its single launches."""
+    base, _, _ = _api()
+    from pysph_amd.base.nnps import update_together
+    from pysph_amd.solver.generators import dam_break_3d
+    fl, wl, prm = dam_break_3d(dx=0.04)
+    fa = base.get_particle_array(name="fluid", cl_precision=precision, **fl)
+    wa = base.get_particle_array(name="wall", cl_precision=precision, **wl)
+    particles = base.Particles([fa, wa])
+    mgr = particles.nnps_manager
+    locs = [mgr.get_neighbor_particle_locator(fa, fa, 2.0),
+            mgr.get_neighbor_particle_locator(wa, fa, 2.0)]
+    ctx = particles.cell_manager._ctx()
+    paired = []
+    real = ctx.lib.sph_nnps_build_dev_pair
+
+    def counted(*args):
+        paired.append(1)
+        return real(*args)
+
+    rng = np.random.default_rng(11)
+    x0 = {k: fa.get(k) for k in ("x", "y", "z")}
+    ctx.lib.sph_nnps_build_dev_pair = counted
+    try:
+        for it in range(5):
+            before = len(paired)
+            update_together(locs)
+            got = [tuple(t.cpu().numpy() for t in loc.get_csr()) for loc in locs]
+            if it > 0:  # the first builds run with a host synchronisation
+                assert len(paired) == before + 1, it
+            for loc in locs:  # the same builds one at a time
+                loc.is_dirty = True
+                loc.update()
+            assert len(paired) == before + (it > 0)
+            for loc, (off, idx) in zip(locs, got):
+                off1, idx1 = (t.cpu().numpy() for t in loc.get_csr())
+                assert np.array_equal(off, off1) and np.array_equal(idx, idx1), it
+            if precision == "double":
+                d = {k: fa.get(k) for k in ("x", "y", "z", "h")}
+                for src, (off, idx) in zip((fa, wa), got):
+                    s = {k: src.get(k) for k in ("x", "y", "z")}
+                    roff, ridx = cpu_ref.nnps(d, s, 2.0, particles.cell_manager.cell_size,
+                                              src is fa, nthreads=8)
+                    assert np.array_equal(off, roff) and np.array_equal(idx, ridx), it
+            print(f"build {it}: {len(got[0][1])} + {len(got[1][1])} entries, near table "
+                  f"{'used' if locs[1].near_used else 'not used'}")
+            fa.set(**{k: x0[k] + rng.uniform(-1.0, 1.0, x0[k].size) * prm["dx"] / 3
+                      for k in ("x", "y", "z")})
+        assert locs[1].near_used
+        # an empty wall: the fluid's own build alone
+        wa.remove_particles(np.arange(wa.get_number_of_particles()))
+        particles.update()
+        n0 = len(paired)
+        update_together(locs)
+        off, idx = (t.cpu().numpy() for t in locs[1].get_csr())
+        assert len(idx) == 0 and np.all(off == 0)
+        off, idx = (t.cpu().numpy() for t in locs[0].get_csr())
+        locs[0].is_dirty = True
+        off1, idx1 = (t.cpu().numpy() for t in locs[0].get_csr())
+        assert np.array_equal(off, off1) and np.array_equal(idx, idx1)
+        print(f"empty wall: paired entry called {len(paired) - n0} time(s)")
+    finally:
+        ctx.lib.sph_nnps_build_dev_pair = real
+
+
 def test_lists_exact_at_the_support_radius(gpu):
     """Pairs exactly at, and within a few FP64 ulps of, the support radius,
     far from the origin: the float distance test of the list kernel leaves

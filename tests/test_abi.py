@@ -76,8 +76,8 @@ def test_product_path_has_no_oracle_dependency():
 
 def test_struct_layouts_match_header(tmp_path):
     """The ctypes mirrors of sph_grid, sph_prop_list(_f32), sph_step_props,
-    sph_batch_desc, sph_sources, sph_pair_params and sph_wall_frames have the
-    C compiler's field offsets and sizes."""
+    sph_batch_desc, sph_sources, sph_pair_params, sph_wall_frames and
+    sph_nnps_dev_args have the C compiler's field offsets and sizes."""
     import ctypes
     import shutil
     import subprocess
@@ -89,7 +89,8 @@ def test_struct_layouts_match_header(tmp_path):
                (("sph_grid", "SphGrid"), ("sph_prop_list", "PropList"),
                 ("sph_prop_list_f32", "PropListF32"), ("sph_step_props", "StepProps"),
                 ("sph_batch_desc", "BatchDesc"), ("sph_sources", "Sources"),
-                ("sph_pair_params", "PairParams"), ("sph_wall_frames", "WallFrames"))
+                ("sph_pair_params", "PairParams"), ("sph_wall_frames", "WallFrames"),
+                ("sph_nnps_dev_args", "NnpsDevArgs"))
                if hasattr(_lib, cls)}
     assert "sph_batch_desc" in structs
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "sphhip.h"', 'int main(void){']

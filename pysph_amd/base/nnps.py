@@ -42,24 +42,53 @@ PM_FIELDS = ("x", "y", "z", "m")
 _BIN_STAMPS = itertools.count(1)
 
 
+class _SharedEvent:
+    """The event that tells the host a device-to-host copy has landed,
+    shared by every copy queued on the stream before it is recorded: the
+    grid check, the binning and the list builds of one evaluation end in one
+    event (one system-scope fence) rather than one each.  Recorded by
+    CellManager.fence() after the list builds, or by the first query() or
+    synchronize() if nobody has recorded it yet (it then covers everything
+    queued so far, which still includes its copies)."""
+
+    __slots__ = ("ev",)
+
+    def __init__(self):
+        self.ev = None
+
+    def record(self):
+        if self.ev is None:
+            self.ev = torch.cuda.Event()
+            self.ev.record()
+
+    def query(self):
+        self.record()
+        return self.ev.query()
+
+    def synchronize(self):
+        self.record()
+        self.ev.synchronize()
+
+
 class _PinnedSlots:
     """A few page-locked host buffers for the sizes and status words the
     device-side builds send back, allocated once: steady-state builds make no
-    host allocation.  take() returns the next buffer and the event of its
-    copy; a buffer whose last copy has not finished is waited for first."""
+    host allocation.  take(ev) returns the next buffer, whose copy the
+    caller queues and whose arrival ev (a _SharedEvent) tells; a buffer
+    whose last copy has not arrived is waited for first."""
 
     def __init__(self, n, numel, dtype):
         self.bufs = torch.empty((n, numel), dtype=dtype, pin_memory=True)
         self.events = [None] * n
         self.i = 0
 
-    def take(self):
+    def take(self, ev):
         k = self.i
         self.i = (k + 1) % len(self.events)
         if self.events[k] is not None:
             self.events[k].synchronize()
-        self.events[k] = torch.cuda.Event()
-        return self.bufs[k], self.events[k]
+        self.events[k] = ev
+        return self.bufs[k], ev
 SHADOW_PROPS = tuple(REC_FIELDS)
 
 
@@ -165,6 +194,7 @@ class CellManager:
         self._grid_key = None
         self._grid_flags = None
         self._grid_pending = None
+        self._open_event = None
         self._grid_host = None
         self.grid_builds = 0
         if initialize:
@@ -211,6 +241,19 @@ class CellManager:
         """What the kept grid depends on besides positions: the arrays, their
         sizes and their h."""
         return tuple((a.name, a.get_number_of_particles(), a.version("h")) for a in arrays)
+
+    def _copy_event(self):
+        """The shared event of the device-to-host copies queued now."""
+        if self._open_event is None or self._open_event.ev is not None:
+            self._open_event = _SharedEvent()
+        return self._open_event
+
+    def fence(self):
+        """Record the event of the device-to-host copies queued since the
+        last one (after an evaluation's list builds)."""
+        if self._open_event is not None:
+            self._open_event.record()
+            self._open_event = None
 
     def _poll_grid_check(self):
         """Flags of the last device grid check (waits for it: it was queued
@@ -268,9 +311,7 @@ class CellManager:
                 self._grid_host = torch.empty(1, dtype=torch.int32, pin_memory=True)
             host = self._grid_host
             host.copy_(self._grid_flags, non_blocking=True)
-            ev = torch.cuda.Event()
-            ev.record()
-            self._grid_pending = (ev, host)
+            self._grid_pending = (self._copy_event(), host)
             grid = self.grid
             same_grid = True
         else:
@@ -314,9 +355,8 @@ class CellManager:
                 ctx.check(lib.sph_bin_array_dev(*args, _lib.ptr(b.ncells_dev)))
                 if b.host_slots is None:
                     b.host_slots = _PinnedSlots(4, 1, torch.int32)
-                host, ev = b.host_slots.take()
+                host, ev = b.host_slots.take(self._copy_event())
                 host.copy_(b.ncells_dev, non_blocking=True)
-                ev.record()
                 b.pending = (ev, host)
             b.stamp = next(_BIN_STAMPS)
             b.invalidate()
@@ -643,9 +683,8 @@ class FixedDestNbrParticleLocator:
             self._host_slots = _PinnedSlots(4, 3, torch.int64)
         if len(self._pending) >= 3:
             self._poll(block=True)  # no buffer is reused before it was read
-        host, ev = self._host_slots.take()
+        host, ev = self._host_slots.take(self.cell_manager._copy_event())
         host.copy_(totals, non_blocking=True)   # {slots, nnz, status}
-        ev.record()
         self._totals = totals
         self._pending.append((ev, host))
         self.nbuilds += 1
@@ -687,6 +726,7 @@ class FixedDestNbrParticleLocator:
                     a.slice_off, a.indices, a.cap, a.totals, a.status, a.s_near, a.near_reach)
             ctx.check(rc)
             self._dev_done(totals)
+            cm.fence()
             return
         slots = ctypes.c_int64(0)
         nnz = ctypes.c_int64(0)
@@ -798,6 +838,7 @@ def update_together(locators):
             for loc, t in zip(pair, (t0, t1)):
                 loc._dev_done(t)
                 loc._built()
+            cm.fence()
     for loc in locators:
         loc.update()
 

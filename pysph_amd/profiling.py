@@ -5,6 +5,8 @@ so events recorded on that stream bracket exactly the kernels of one C-ABI
 call.  Disabled by default; ``enable()`` turns recording on.
 """
 import contextlib
+import ctypes
+import os
 from collections import defaultdict
 
 import torch
@@ -31,8 +33,59 @@ def enable(on=True, regions=None, drain=False):
     _drain = bool(drain)
 
 
+_HIP_EVENT_DISABLE_SYSTEM_FENCE = 0x20000000
+_hip = None
+
+
+def _hip_runtime():
+    """The HIP runtime torch loaded (the one every launch here goes through)."""
+    global _hip
+    if _hip is None:
+        path = os.path.join(os.path.dirname(torch.__file__), "lib", "libamdhip64.so")
+        hip = ctypes.CDLL(path if os.path.exists(path) else "libamdhip64.so")
+        vp = ctypes.c_void_p
+        hip.hipEventCreateWithFlags.argtypes = [ctypes.POINTER(vp), ctypes.c_uint]
+        hip.hipEventRecord.argtypes = [vp, vp]
+        hip.hipEventSynchronize.argtypes = [vp]
+        hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
+        _hip = hip
+    return _hip
+
+
+class _TimingEvent:
+    """A HIP timing event recorded without the system-scope fence of a
+    default event (hipEventDisableSystemFence): it only timestamps the
+    stream, so the writeback and invalidation of the caches that a default
+    event's fence makes -- about 6 us of idle GPU per record on gfx950 --
+    stay out of the timed work.  Nothing reads memory through it."""
+
+    def __init__(self):
+        self._hip = _hip_runtime()
+        self.h = ctypes.c_void_p()
+        rc = self._hip.hipEventCreateWithFlags(ctypes.byref(self.h),
+                                               _HIP_EVENT_DISABLE_SYSTEM_FENCE)
+        if rc != 0:
+            raise RuntimeError(f"hipEventCreateWithFlags failed ({rc})")
+
+    def record(self):
+        stream = torch.cuda.current_stream().cuda_stream
+        rc = self._hip.hipEventRecord(self.h, ctypes.c_void_p(stream))
+        if rc != 0:
+            raise RuntimeError(f"hipEventRecord failed ({rc})")
+
+    def elapsed_time(self, end):
+        """Milliseconds from this event to `end` (waits for `end`)."""
+        rc = self._hip.hipEventSynchronize(end.h)
+        ms = ctypes.c_float(0.0)
+        if rc == 0:
+            rc = self._hip.hipEventElapsedTime(ctypes.byref(ms), self.h, end.h)
+        if rc != 0:
+            raise RuntimeError(f"hipEventElapsedTime failed ({rc})")
+        return float(ms.value)
+
+
 def _event():
-    return _pool.pop() if _pool else torch.cuda.Event(enable_timing=True)
+    return _pool.pop() if _pool else _TimingEvent()
 
 
 def reset():

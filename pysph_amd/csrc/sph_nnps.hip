@@ -1386,8 +1386,24 @@ __device__ __forceinline__ void count_cells(const CellArgs &a, int32_t *__restri
         nnps_cell<false, T>(a, w, counts, nullptr, nullptr, seg_start, seg_end, dsts);
 }
 
+// the rows of the cells the count pass flagged (listed in a.ovf_list), by
+// blocks bid of G
+template <typename T>
+__device__ __forceinline__ void fill_flagged(const CellArgs &a, int32_t *__restrict__ counts,
+                                             const int64_t *__restrict__ slice_off,
+                                             int32_t *__restrict__ indices, unsigned bid,
+                                             unsigned G, int32_t *seg_start, int32_t *seg_end,
+                                             typename Quad<T>::type *dsts)
+{
+    if (a.status && slice_off[a.nslices] > a.cap) return;  // no room: nothing is written
+    const int32_t nov = *a.novf;
+    for (int32_t k = bid; k < nov; k += G)
+        nnps_cell<true, T>(a, a.ovf_list[k], counts, slice_off, indices, seg_start, seg_end,
+                           dsts);
+}
+
 // FILL = false: the count pass (count_cells).  FILL = true: the cells the
-// count pass flagged, listed in a.ovf_list.
+// count pass flagged (fill_flagged).
 template <bool FILL, typename T>
 __global__ void __launch_bounds__(CW)
     k_nnps_cells(CellArgs a, int32_t *__restrict__ counts, const int64_t *__restrict__ slice_off,
@@ -1397,11 +1413,8 @@ __global__ void __launch_bounds__(CW)
     __shared__ int32_t seg_end[MAXSEG];  // inclusive prefix of the segment sizes
     __shared__ typename Quad<T>::type dsts[CW];  // (x, y, z, radius^2) of the group's destinations
     if (FILL) {
-        if (a.status && slice_off[a.nslices] > a.cap) return;  // no room: nothing is written
-        const int32_t nov = *a.novf;
-        for (int32_t k = blockIdx.x; k < nov; k += gridDim.x)
-            nnps_cell<true, T>(a, a.ovf_list[k], counts, slice_off, indices, seg_start, seg_end,
-                               dsts);
+        fill_flagged<T>(a, counts, slice_off, indices, blockIdx.x, gridDim.x, seg_start, seg_end,
+                        dsts);
     } else {
         count_cells<T>(a, counts, blockIdx.x, gridDim.x, seg_start, seg_end, dsts);
     }
@@ -1422,6 +1435,25 @@ __global__ void __launch_bounds__(CW)
         count_cells<T>(a0, counts0, blockIdx.x, g0, seg_start, seg_end, dsts);
     else
         count_cells<T>(a1, counts1, blockIdx.x - g0, gridDim.x - g0, seg_start, seg_end, dsts);
+}
+
+// the flagged cells of two builds in one launch (blockIdx.y picks the build)
+template <typename T>
+__global__ void __launch_bounds__(CW)
+    k_nnps_flagged_pair(CellArgs a0, int32_t *__restrict__ counts0,
+                        const int64_t *__restrict__ slice_off0, int32_t *__restrict__ indices0,
+                        CellArgs a1, int32_t *__restrict__ counts1,
+                        const int64_t *__restrict__ slice_off1, int32_t *__restrict__ indices1)
+{
+    __shared__ int32_t seg_start[MAXSEG];
+    __shared__ int32_t seg_end[MAXSEG];
+    __shared__ typename Quad<T>::type dsts[CW];
+    if (blockIdx.y)
+        fill_flagged<T>(a1, counts1, slice_off1, indices1, blockIdx.x, gridDim.x, seg_start,
+                        seg_end, dsts);
+    else
+        fill_flagged<T>(a0, counts0, slice_off0, indices0, blockIdx.x, gridDim.x, seg_start,
+                        seg_end, dsts);
 }
 
 // SELL-64 rows from the count pass's hit masks: one lane per destination,
@@ -1702,13 +1734,16 @@ __global__ void __launch_bounds__(CW)
 // slice s of 64 destinations: storage 64 * max(count) and its entry total;
 // the storage also in slice_w2[s] (read by k_slice_offsets while slice_w,
 // which becomes the offsets, is rewritten)
-__global__ void __launch_bounds__(NN_THREADS)
-    k_slices(const int32_t *__restrict__ counts, int64_t nd_total, int64_t nslices,
-             int64_t *__restrict__ slice_w, int64_t *__restrict__ slice_sum,
-             int64_t *__restrict__ slice_w2)
+struct SliceArgs {
+    const int32_t *counts;
+    int64_t nd_total, nslices;
+    int64_t *slice_w, *slice_sum, *slice_w2;
+};
+
+__device__ __forceinline__ void slices_block(const SliceArgs &a, int64_t bid)
 {
-    const int64_t t = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
-    const int c = t < nd_total ? counts[t] : 0;
+    const int64_t t = bid * NN_THREADS + threadIdx.x;
+    const int c = t < a.nd_total ? a.counts[t] : 0;
     int m = c, sum = c;
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
@@ -1716,11 +1751,29 @@ __global__ void __launch_bounds__(NN_THREADS)
         sum += __shfl_xor(sum, d, 64);
     }
     const int64_t s = t >> 6;
-    if ((threadIdx.x & 63) == 0 && s < nslices) {
-        slice_w[s] = 64 * (int64_t)m;
-        slice_w2[s] = 64 * (int64_t)m;
-        slice_sum[s] = sum;
+    if ((threadIdx.x & 63) == 0 && s < a.nslices) {
+        a.slice_w[s] = 64 * (int64_t)m;
+        a.slice_w2[s] = 64 * (int64_t)m;
+        a.slice_sum[s] = sum;
     }
+}
+
+__global__ void __launch_bounds__(NN_THREADS)
+    k_slices(const int32_t *__restrict__ counts, int64_t nd_total, int64_t nslices,
+             int64_t *__restrict__ slice_w, int64_t *__restrict__ slice_sum,
+             int64_t *__restrict__ slice_w2)
+{
+    const SliceArgs a = {counts, nd_total, nslices, slice_w, slice_sum, slice_w2};
+    slices_block(a, blockIdx.x);
+}
+
+// two builds' slices in one launch: blockIdx.y picks the build, blocks past
+// its own destinations have nothing to do
+__global__ void __launch_bounds__(NN_THREADS) k_slices_pair(SliceArgs a0, SliceArgs a1)
+{
+    const SliceArgs &a = blockIdx.y ? a1 : a0;
+    if ((int64_t)blockIdx.x * NN_THREADS >= a.nslices * 64) return;
+    slices_block(a, blockIdx.x);
 }
 
 // counts of every row and the flagged-cell counter start at 0
@@ -1729,6 +1782,19 @@ __global__ void __launch_bounds__(NN_THREADS)
 {
     const int64_t t = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
     if (t < nd_total) counts[t] = 0;
+    if (t == 0) *novf = 0;
+}
+
+// the same for two builds (blockIdx.y picks the build)
+__global__ void __launch_bounds__(NN_THREADS)
+    k_nnps_reset_pair(int32_t *__restrict__ counts0, int64_t nd0, int32_t *__restrict__ novf0,
+                      int32_t *__restrict__ counts1, int64_t nd1, int32_t *__restrict__ novf1)
+{
+    int32_t *counts = blockIdx.y ? counts1 : counts0;
+    int32_t *novf = blockIdx.y ? novf1 : novf0;
+    const int64_t nd = blockIdx.y ? nd1 : nd0;
+    const int64_t t = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
+    if (t < nd) counts[t] = 0;
     if (t == 0) *novf = 0;
 }
 
@@ -1780,12 +1846,13 @@ __device__ __forceinline__ void slice_block_sum2(int64_t &a, int64_t &b, longlon
 }
 
 // item k of thread t of tile b: slice b*TILE + t + k*THREADS (coalesced)
-__global__ void __launch_bounds__(SLICE_THREADS)
-    k_slice_tile_sums(const int64_t *__restrict__ width, const int64_t *__restrict__ slice_sum,
-                      int64_t nslices, longlong2 *__restrict__ tile_sum)
+__device__ __forceinline__ void slice_tile_sums(const int64_t *__restrict__ width,
+                                                const int64_t *__restrict__ slice_sum,
+                                                int64_t nslices,
+                                                longlong2 *__restrict__ tile_sum, int64_t bid,
+                                                longlong2 *s_part)
 {
-    __shared__ longlong2 s_part[SLICE_THREADS / 64];
-    const int64_t base = (int64_t)blockIdx.x * SLICE_TILE + threadIdx.x;
+    const int64_t base = bid * SLICE_TILE + threadIdx.x;
     const int64_t last = nslices - 1;
     int64_t w[SLICE_ITEMS], z[SLICE_ITEMS];
 #pragma unroll
@@ -1802,18 +1869,28 @@ __global__ void __launch_bounds__(SLICE_THREADS)
         b += in ? z[k] : 0;
     }
     slice_block_sum2(a, b, s_part);
-    if (threadIdx.x == 0) tile_sum[blockIdx.x] = make_longlong2(a, b);
+    if (threadIdx.x == 0) tile_sum[bid] = make_longlong2(a, b);
 }
 
 __global__ void __launch_bounds__(SLICE_THREADS)
-    k_slice_tile_scan(const int64_t *__restrict__ width, int64_t nslices,
-                      const longlong2 *__restrict__ tile_sum, int64_t *__restrict__ slice_off,
-                      int64_t *__restrict__ nnz)
+    k_slice_tile_sums(const int64_t *__restrict__ width, const int64_t *__restrict__ slice_sum,
+                      int64_t nslices, longlong2 *__restrict__ tile_sum)
 {
     __shared__ longlong2 s_part[SLICE_THREADS / 64];
-    __shared__ int64_t s_round[SLICE_ITEMS][SLICE_THREADS / 64];
+    slice_tile_sums(width, slice_sum, nslices, tile_sum, blockIdx.x, s_part);
+}
+
+// tile bid of ntiles: its slice offsets, and (last tile) the totals
+__device__ __forceinline__ void slice_tile_scan(const int64_t *__restrict__ width,
+                                                int64_t nslices,
+                                                const longlong2 *__restrict__ tile_sum,
+                                                int64_t *__restrict__ slice_off,
+                                                int64_t *__restrict__ nnz, int64_t bid,
+                                                int64_t ntiles, longlong2 *s_part,
+                                                int64_t (*s_round)[SLICE_THREADS / 64])
+{
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t base = (int64_t)blockIdx.x * SLICE_TILE + tid;
+    const int64_t base = bid * SLICE_TILE + tid;
     const int64_t last = nslices - 1;
     int64_t w[SLICE_ITEMS];
 #pragma unroll
@@ -1823,9 +1900,9 @@ __global__ void __launch_bounds__(SLICE_THREADS)
     }
     // sums of the tiles before this one (and of all tiles, for the totals)
     int64_t pa = 0, pz = 0, ta = 0, tz = 0;
-    for (int64_t q = tid; q < (int64_t)gridDim.x; q += SLICE_THREADS) {
+    for (int64_t q = tid; q < ntiles; q += SLICE_THREADS) {
         const longlong2 v = tile_sum[q];
-        if (q < (int64_t)blockIdx.x) {
+        if (q < bid) {
             pa += v.x;
         }
         ta += v.x;
@@ -1861,10 +1938,48 @@ __global__ void __launch_bounds__(SLICE_THREADS)
         if (idx < nslices) slice_off[idx] = carry + before + inc[k] - w[k];
         carry += total;
     }
-    if (blockIdx.x == gridDim.x - 1 && tid == 0) {
+    if (bid == ntiles - 1 && tid == 0) {
         slice_off[nslices] = ta;
         *nnz = tz;
     }
+}
+
+__global__ void __launch_bounds__(SLICE_THREADS)
+    k_slice_tile_scan(const int64_t *__restrict__ width, int64_t nslices,
+                      const longlong2 *__restrict__ tile_sum, int64_t *__restrict__ slice_off,
+                      int64_t *__restrict__ nnz)
+{
+    __shared__ longlong2 s_part[SLICE_THREADS / 64];
+    __shared__ int64_t s_round[SLICE_ITEMS][SLICE_THREADS / 64];
+    slice_tile_scan(width, nslices, tile_sum, slice_off, nnz, blockIdx.x, gridDim.x, s_part,
+                    s_round);
+}
+
+// the two launches for two builds (blockIdx.y picks the build; a build's
+// tiles are its first blocks, the rest have nothing to do)
+struct SliceScan {
+    const int64_t *width, *slice_sum;
+    int64_t nslices, ntiles;
+    longlong2 *tile_sum;
+    int64_t *slice_off, *nnz;
+};
+
+__global__ void __launch_bounds__(SLICE_THREADS) k_slice_tile_sums_pair(SliceScan a0, SliceScan a1)
+{
+    __shared__ longlong2 s_part[SLICE_THREADS / 64];
+    const SliceScan &a = blockIdx.y ? a1 : a0;
+    if ((int64_t)blockIdx.x >= a.ntiles) return;
+    slice_tile_sums(a.width, a.slice_sum, a.nslices, a.tile_sum, blockIdx.x, s_part);
+}
+
+__global__ void __launch_bounds__(SLICE_THREADS) k_slice_tile_scan_pair(SliceScan a0, SliceScan a1)
+{
+    __shared__ longlong2 s_part[SLICE_THREADS / 64];
+    __shared__ int64_t s_round[SLICE_ITEMS][SLICE_THREADS / 64];
+    const SliceScan &a = blockIdx.y ? a1 : a0;
+    if ((int64_t)blockIdx.x >= a.ntiles) return;
+    slice_tile_scan(a.width, a.nslices, a.tile_sum, a.slice_off, a.nnz, blockIdx.x, a.ntiles,
+                    s_part, s_round);
 }
 
 // slice offsets and entry total from the widths k_slices wrote into
@@ -2224,6 +2339,39 @@ static int nnps_prepare_dev(sph_ctx *ctx, const sph_grid *grid, const sph_nnps_d
                         nullptr, dev, b->s_near, b->near_reach, job);
 }
 
+// count_slices of two builds, each of its launches for both
+static int pair_slices(sph_ctx *ctx, NnpsJob &j0, int64_t *sum0, size_t scan0, NnpsJob &j1,
+                       int64_t *sum1, size_t scan1)
+{
+    const int64_t ns0 = j0.a.nslices, ns1 = j1.a.nslices;
+    const int64_t nt0 = (ns0 + SLICE_TILE - 1) / SLICE_TILE;
+    const int64_t nt1 = (ns1 + SLICE_TILE - 1) / SLICE_TILE;
+    if (ns0 == 0 || ns1 == 0 || nt0 > SLICE_MAX_TILES || nt1 > SLICE_MAX_TILES) {
+        int rc = count_slices(ctx, j0.a, j0.counts, j0.slice_off, sum0, scan0, j0.nd_total);
+        if (rc) return rc;
+        return count_slices(ctx, j1.a, j1.counts, j1.slice_off, sum1, scan1, j1.nd_total);
+    }
+    const SliceArgs s0 = {j0.counts, j0.nd_total, ns0, j0.slice_off, sum0, sum0 + ns0 + 1};
+    const SliceArgs s1 = {j1.counts, j1.nd_total, ns1, j1.slice_off, sum1, sum1 + ns1 + 1};
+    const int64_t nsl = ns0 > ns1 ? ns0 : ns1;
+    hipLaunchKernelGGL(k_slices_pair, dim3(sph_blocks(nsl * 64, NN_THREADS), 2),
+                       dim3(NN_THREADS), 0, ctx->stream, s0, s1);
+    SPH_LAUNCH_CHECK(ctx, "k_slices_pair");
+    // the tile sums after the copy of the widths (16-byte aligned: the sums are)
+    const SliceScan t0 = {sum0 + ns0 + 1, sum0, ns0, nt0, (longlong2 *)(sum0 + 2 * ns0 + 2),
+                          j0.slice_off, sum0 + ns0};
+    const SliceScan t1 = {sum1 + ns1 + 1, sum1, ns1, nt1, (longlong2 *)(sum1 + 2 * ns1 + 2),
+                          j1.slice_off, sum1 + ns1};
+    const unsigned ntl = (unsigned)(nt0 > nt1 ? nt0 : nt1);
+    hipLaunchKernelGGL(k_slice_tile_sums_pair, dim3(ntl, 2), dim3(SLICE_THREADS), 0, ctx->stream,
+                       t0, t1);
+    SPH_LAUNCH_CHECK(ctx, "k_slice_tile_sums_pair");
+    hipLaunchKernelGGL(k_slice_tile_scan_pair, dim3(ntl, 2), dim3(SLICE_THREADS), 0, ctx->stream,
+                       t0, t1);
+    SPH_LAUNCH_CHECK(ctx, "k_slice_tile_scan_pair");
+    return SPH_OK;
+}
+
 // blocks of the first build of a paired launch: a multiple of the XCD count,
 // so the second build's block b - g0 runs on the XCD block b does
 static unsigned pair_split(int64_t g)
@@ -2270,19 +2418,18 @@ extern "C" int sph_nnps_build_dev_pair(sph_ctx *ctx, const sph_grid *grid,
     count_carve(ctx, j1.a, c1, c0.total, j1.nd_total, &sum1, &scan1);
     j0.a.nnz_dev = sum0 + j0.a.nslices;
     j1.a.nnz_dev = sum1 + j1.a.nslices;
-    rc = count_reset(ctx, j0.a, j0.counts, j0.nd_total);
-    if (rc) return rc;
-    rc = count_reset(ctx, j1.a, j1.counts, j1.nd_total);
-    if (rc) return rc;
+    const int64_t nd_max = j0.nd_total > j1.nd_total ? j0.nd_total : j1.nd_total;
+    hipLaunchKernelGGL(k_nnps_reset_pair, dim3(sph_blocks(nd_max, NN_THREADS), 2),
+                       dim3(NN_THREADS), 0, ctx->stream, j0.counts, j0.nd_total, j0.a.novf,
+                       j1.counts, j1.nd_total, j1.a.novf);
+    SPH_LAUNCH_CHECK(ctx, "k_nnps_reset_pair");
     const unsigned g0 = pair_split(count_grid(j0.a, j0.nd_cells));
     const int64_t g1 = count_grid(j1.a, j1.nd_cells);
     auto count_kern = j0.a.f32 ? k_nnps_cells_pair<float> : k_nnps_cells_pair<double>;
     hipLaunchKernelGGL(count_kern, dim3((unsigned)(g0 + g1)), dim3(CW), 0, ctx->stream, j0.a,
                        j0.counts, j1.a, j1.counts, g0);
     SPH_LAUNCH_CHECK(ctx, "k_nnps_cells_pair<count>");
-    rc = count_slices(ctx, j0.a, j0.counts, j0.slice_off, sum0, scan0, j0.nd_total);
-    if (rc) return rc;
-    rc = count_slices(ctx, j1.a, j1.counts, j1.slice_off, sum1, scan1, j1.nd_total);
+    rc = pair_slices(ctx, j0, sum0, scan0, j1, sum1, scan1);
     if (rc) return rc;
     const unsigned f0 = pair_split(sph_blocks(j0.nd_total, NN_THREADS));
     const int64_t f1 = sph_blocks(j1.nd_total, NN_THREADS);
@@ -2291,13 +2438,14 @@ extern "C" int sph_nnps_build_dev_pair(sph_ctx *ctx, const sph_grid *grid,
                        j0.indices, j1.a, j1.nd_total, j1.counts, (const int64_t *)j1.slice_off,
                        j1.indices, f0);
     SPH_LAUNCH_CHECK(ctx, "k_nnps_fill_pair");
-    for (NnpsJob *j : {&j0, &j1}) {  // the cells the count pass flagged
-        auto fill_kern = j->a.f32 ? k_nnps_cells<true, float> : k_nnps_cells<true, double>;
-        const int64_t fill_grid = j->nd_cells < NN_OVF_GRID ? j->nd_cells : NN_OVF_GRID;
-        hipLaunchKernelGGL(fill_kern, dim3((unsigned)fill_grid), dim3(CW), 0, ctx->stream, j->a,
-                           j->counts, (const int64_t *)j->slice_off, j->indices);
-        SPH_LAUNCH_CHECK(ctx, "k_nnps_cells<fill>");
-    }
+    // the cells the count pass flagged
+    const int64_t ncl = j0.nd_cells > j1.nd_cells ? j0.nd_cells : j1.nd_cells;
+    const int64_t flag_grid = ncl < NN_OVF_GRID ? ncl : NN_OVF_GRID;
+    auto flag_kern = j0.a.f32 ? k_nnps_flagged_pair<float> : k_nnps_flagged_pair<double>;
+    hipLaunchKernelGGL(flag_kern, dim3((unsigned)flag_grid, 2), dim3(CW), 0, ctx->stream, j0.a,
+                       j0.counts, (const int64_t *)j0.slice_off, j0.indices, j1.a, j1.counts,
+                       (const int64_t *)j1.slice_off, j1.indices);
+    SPH_LAUNCH_CHECK(ctx, "k_nnps_flagged_pair");
     return SPH_OK;
 }
 

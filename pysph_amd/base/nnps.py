@@ -196,6 +196,7 @@ class CellManager:
         self._grid_pending = None
         self._open_event = None
         self._grid_host = None
+        self._made = {}  # array -> record fields kernels write themselves
         self.grid_builds = 0
         if initialize:
             self.initialize()
@@ -382,7 +383,13 @@ class CellManager:
             if p not in wanted:
                 wanted.append(p)
         if any(b.shadow_ver.get(p) != array.version(p) for p in props):
-            stale = [p for p in wanted if b.shadow_ver.get(p) != array.version(p)]
+            # ... except the fields a kernel writes into the records itself
+            # (written(shadow_written=True): density and the equation of
+            # state), unless asked for now: after a binning their copies
+            # would be overwritten before anything reads them
+            made = self._made.get(array.name, ())
+            stale = [p for p in wanted if b.shadow_ver.get(p) != array.version(p)
+                     and (p in props or p not in made)]
             # in record order, so that the two fields of a record pair are
             # adjacent in the list and go out as one 16-byte store
             stale.sort(key=lambda q: REC_FIELDS[q])
@@ -455,6 +462,7 @@ class CellManager:
         b = self.bins.get(array.name)
         if b is not None and shadow_written and prop in REC_FIELDS:
             b.shadow_ver[prop] = array.version(prop)
+            self._made.setdefault(array.name, set()).add(prop)
 
     # P/rho^2 of the records (the per-source term of MomentumEquation): fresh
     # while the record's p and rho are the ones it was computed from

@@ -171,6 +171,12 @@ int sph_gather_sorted_f32(sph_ctx *ctx, int64_t n, const int32_t *perm,
  * faster direction when many fields are refreshed).  Same results as the
  * gathers. */
 int sph_scatter_sorted(sph_ctx *ctx, int64_t n, const int32_t *rank, const sph_prop_list *props);
+/* Two sph_scatter_sorted calls over one rank in one pass (each list with its
+ * own dst_stride: the records and another sorted layout of some of the same
+ * fields); same results as the two calls.  No counterpart in the reference,
+ * which keeps no sorted copies (base/cell.pyx:1179-1306 bins indices only). */
+int sph_scatter_sorted2(sph_ctx *ctx, int64_t n, const int32_t *rank, const sph_prop_list *props,
+                        const sph_prop_list *props2);
 int sph_scatter_sorted_f32(sph_ctx *ctx, int64_t n, const int32_t *rank,
                            const sph_prop_list_f32 *props);
 

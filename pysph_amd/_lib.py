@@ -139,6 +139,8 @@ _SIGNATURES = {
     "sph_scatter_sorted": (_i32, [_vp, _i64, _vp, ctypes.POINTER(PropList)]),
     "sph_copy_many": (_i32, [_vp, _i64, ctypes.POINTER(PropList)]),
     "sph_fill_many": (_i32, [_vp, _i64, _dbl, ctypes.POINTER(PropList)]),
+    "sph_scatter_sorted2": (_i32, [_vp, _i64, _vp, ctypes.POINTER(PropList),
+                                   ctypes.POINTER(PropList)]),
     "sph_scatter_sorted_f32": (_i32, [_vp, _i64, _vp, ctypes.POINTER(PropListF32)]),
     "sph_nnps_build": (_i32, [_vp, ctypes.POINTER(SphGrid), _vp, _vp, _vp, _vp, _i64, _i64, _i64,
                               _vp, _vp, _i64, _dbl, _i32, _vp, _vp, _vp, _i64,

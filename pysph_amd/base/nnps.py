@@ -411,12 +411,41 @@ class CellManager:
                 pl.dst[k] = b.field_ptr(p)
             # original order -> sorted rows by rank: coalesced reads of every
             # field (the gather by perm reads one cache line per lane and field)
-            scatter = ctx.lib.sph_scatter_sorted_f32 if single else ctx.lib.sph_scatter_sorted
+            pm = self._pm_props(array, b) if not single else None
             with profiling.region("gather_sorted"):
-                ctx.check(scatter(ctx.handle, b.n, _lib.ptr(b.rank), ctypes.byref(pl)))
+                if pm is not None:  # the compact rows in the same pass
+                    ctx.check(ctx.lib.sph_scatter_sorted2(ctx.handle, b.n, _lib.ptr(b.rank),
+                                                          ctypes.byref(pl), ctypes.byref(pm[0])))
+                else:
+                    scatter = (ctx.lib.sph_scatter_sorted_f32 if single
+                               else ctx.lib.sph_scatter_sorted)
+                    ctx.check(scatter(ctx.handle, b.n, _lib.ptr(b.rank), ctypes.byref(pl)))
             for p in stale:
                 b.shadow_ver[p] = array.version(p)
+            if pm is not None:
+                for q in pm[1]:
+                    b.pm_ver[q] = array.version(q)
         return b.rec
+
+    def _pm_props(self, array, b):
+        """(prop list, fields) of the stale fields of the compact rows of
+        `array`, once posmass() has made them, else None."""
+        if b.pm is None or b.pm.shape[0] != max(b.n, 1):
+            return None
+        stale = [q for q in PM_FIELDS if b.pm_ver.get(q) != array.version(q)]
+        if not stale:
+            return None
+        pl = _lib.PropList()
+        pl.nprops = len(stale)
+        pl.dst_stride = 4
+        base = b.pm.data_ptr()
+        for k, q in enumerate(stale):
+            src = array.get_carray(q)
+            if src is None or src.dtype != torch.float64:
+                raise TypeError(f"property {q} of {array.name} must be float64")
+            pl.src[k] = _lib.ptr(src)
+            pl.dst[k] = base + 8 * PM_FIELDS.index(q)
+        return pl, stale
 
     def posmass(self, array):
         """The compact sorted (x, y, z, m) rows of `array` (4 doubles per
@@ -429,19 +458,10 @@ class CellManager:
         if b.pm is None or b.pm.shape[0] != max(b.n, 1):
             b.pm = torch.empty((max(b.n, 1), 4), dtype=torch.float64, device=b.device)
             b.pm_ver = {}
-        stale = [q for q in PM_FIELDS if b.pm_ver.get(q) != array.version(q)]
-        if stale:
+        pm = self._pm_props(array, b)
+        if pm is not None:
             ctx = self._ctx()
-            pl = _lib.PropList()
-            pl.nprops = len(stale)
-            pl.dst_stride = 4
-            base = b.pm.data_ptr()
-            for k, q in enumerate(stale):
-                src = array.get_carray(q)
-                if src is None or src.dtype != torch.float64:
-                    raise TypeError(f"property {q} of {array.name} must be float64")
-                pl.src[k] = _lib.ptr(src)
-                pl.dst[k] = base + 8 * PM_FIELDS.index(q)
+            pl, stale = pm
             with profiling.region("gather_sorted"):
                 ctx.check(ctx.lib.sph_scatter_sorted(ctx.handle, b.n, _lib.ptr(b.rank),
                                                      ctypes.byref(pl)))

@@ -444,7 +444,17 @@ __global__ void __launch_bounds__(NN_THREADS)
         if (rank[i] == 0) large[atomicAdd(nlarge, 1)] = key;  // slot 0: once per cell
         return;
     } else {
-        for (int32_t a = s; a < e; ++a) r += tmp[a] < (int32_t)i;
+        // eight members' loads in flight before the first compare (a loop of
+        // one load at a time waits out the load latency per member)
+        int32_t a = s;
+        for (; a + 8 <= e; a += 8) {
+            int32_t v[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = tmp[a + k];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) r += v[k] < (int32_t)i;
+        }
+        for (; a < e; ++a) r += tmp[a] < (int32_t)i;
     }
     perm[s + r] = (int32_t)i;
     rank[i] = s + r;
@@ -868,17 +878,9 @@ __global__ void __launch_bounds__(NN_THREADS)
 // of particle i (coalesced) and writes it into row rank[i].  A row is 96
 // contiguous bytes, so the scattered stores of one particle fill its row in
 // L2, while the gather above reads each field with one cache line per lane.
-__global__ void __launch_bounds__(NN_THREADS)
-    k_scatter_sorted(int64_t n, const int32_t *__restrict__ rank, sph_prop_list pl)
+// one particle's fields v into row o of every destination of pl
+__device__ __forceinline__ void scatter_row(const sph_prop_list &pl, const double *v, int64_t o)
 {
-    int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
-    if (i >= n) return;
-    const int64_t o = (int64_t)rank[i] * pl.dst_stride;
-    // every field is loaded before the first store: memory operations retire
-    // in order, so a load issued after a store would wait for that store
-    double v[SPH_MAX_PROPS];
-#pragma unroll
-    for (int k = 0; k < SPH_MAX_PROPS; ++k) v[k] = k < pl.nprops ? pl.src[k][i] : 0.0;
     bool taken = false;  // field k went out with field k - 1
 #pragma unroll
     for (int k = 0; k < SPH_MAX_PROPS; ++k) {
@@ -898,6 +900,40 @@ __global__ void __launch_bounds__(NN_THREADS)
             *d = v[k];
         }
     }
+}
+
+__global__ void __launch_bounds__(NN_THREADS)
+    k_scatter_sorted(int64_t n, const int32_t *__restrict__ rank, sph_prop_list pl)
+{
+    int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
+    if (i >= n) return;
+    const int64_t o = (int64_t)rank[i] * pl.dst_stride;
+    // every field is loaded before the first store: memory operations retire
+    // in order, so a load issued after a store would wait for that store
+    double v[SPH_MAX_PROPS];
+#pragma unroll
+    for (int k = 0; k < SPH_MAX_PROPS; ++k) v[k] = k < pl.nprops ? pl.src[k][i] : 0.0;
+    scatter_row(pl, v, o);
+}
+
+// two prop lists of their own row strides in one pass (the sorted records
+// and the compact rows from the same original-order fields): every load of
+// both lists before the first store; a field both lists read is loaded twice
+// (the second from cache)
+__global__ void __launch_bounds__(NN_THREADS)
+    k_scatter_sorted2(int64_t n, const int32_t *__restrict__ rank, sph_prop_list pa,
+                      sph_prop_list pb)
+{
+    int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
+    if (i >= n) return;
+    const int64_t r = rank[i];
+    double va[SPH_MAX_PROPS], vb[SPH_MAX_PROPS];
+#pragma unroll
+    for (int k = 0; k < SPH_MAX_PROPS; ++k) va[k] = k < pa.nprops ? pa.src[k][i] : 0.0;
+#pragma unroll
+    for (int k = 0; k < SPH_MAX_PROPS; ++k) vb[k] = k < pb.nprops ? pb.src[k][i] : 0.0;
+    scatter_row(pa, va, r * pa.dst_stride);
+    scatter_row(pb, vb, r * pb.dst_stride);
 }
 
 __global__ void __launch_bounds__(NN_THREADS)
@@ -936,6 +972,31 @@ extern "C" int sph_scatter_sorted(sph_ctx *ctx, int64_t n, const int32_t *rank,
                                   const sph_prop_list *props)
 {
     return scatter_sorted(ctx, "sph_scatter_sorted", k_scatter_sorted, n, rank, props);
+}
+
+extern "C" int sph_scatter_sorted2(sph_ctx *ctx, int64_t n, const int32_t *rank,
+                                   const sph_prop_list *props, const sph_prop_list *props2)
+{
+    if (!ctx) return SPH_EINVAL;
+    SPH_REQUIRE(ctx, props2 && props2->nprops >= 0 && props2->nprops <= SPH_MAX_PROPS &&
+                         props2->dst_stride >= 1,
+                "sph_scatter_sorted2: bad second props");
+    if (props2->nprops == 0)
+        return scatter_sorted(ctx, "sph_scatter_sorted2", k_scatter_sorted, n, rank, props);
+    SPH_REQUIRE(ctx, props && props->nprops >= 0 && props->nprops <= SPH_MAX_PROPS &&
+                         props->dst_stride >= 1,
+                "sph_scatter_sorted2: bad props");
+    if (n <= 0) return SPH_OK;
+    SPH_REQUIRE(ctx, rank, "sph_scatter_sorted2: null rank");
+    for (int k = 0; k < props->nprops; ++k)
+        SPH_REQUIRE(ctx, props->src[k] && props->dst[k], "sph_scatter_sorted2: null prop %d", k);
+    for (int k = 0; k < props2->nprops; ++k)
+        SPH_REQUIRE(ctx, props2->src[k] && props2->dst[k],
+                    "sph_scatter_sorted2: null second prop %d", k);
+    hipLaunchKernelGGL(k_scatter_sorted2, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
+                       ctx->stream, n, rank, *props, *props2);
+    SPH_LAUNCH_CHECK(ctx, "sph_scatter_sorted2");
+    return SPH_OK;
 }
 
 extern "C" int sph_scatter_sorted_f32(sph_ctx *ctx, int64_t n, const int32_t *rank,

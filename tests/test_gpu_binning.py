@@ -112,3 +112,54 @@ def test_large_sparse_grid(gpu, order):
     roff, ridx = cpu_ref.nnps(full, full, 2.0, cm.cell_size, True, nthreads=16)
     assert np.array_equal(off, roff) and np.array_equal(idx, ridx)
     print(f"{order}: {nc} cells, {ncells} non-empty, {len(idx)} list entries")
+
+
+def test_two_list_scatter_equals_two_scatters(gpu):
+    """sph_scatter_sorted2 (the records and the compact (x, y, z, m) rows in
+    one pass) writes what two sph_scatter_sorted calls write, and both equal
+    the numpy scatter dst[rank[i]] = src[i]; an empty second list is the
+    plain scatter."""
+    import ctypes
+
+    import torch
+
+    from pysph_amd import _lib
+    ctx = _lib.context(torch.device("cuda", 0))
+    rng = np.random.default_rng(17)
+    n = 100_003
+    rank = torch.from_numpy(rng.permutation(n).astype(np.int32)).cuda()
+    src = [torch.from_numpy(rng.standard_normal(n)).cuda() for _ in range(11)]
+    p = _lib.ptr
+
+    def lists(rec, pm, nb=4):
+        a = _lib.PropList()
+        a.nprops, a.dst_stride = 11, 12
+        for k in range(11):  # fields 0..10 of the 12-double rows (pairs adjacent)
+            a.src[k], a.dst[k] = p(src[k]), rec.data_ptr() + 8 * k
+        b = _lib.PropList()
+        b.nprops, b.dst_stride = nb, 4
+        for k, f in enumerate((0, 1, 2, 3)[:nb]):
+            b.src[k], b.dst[k] = p(src[f]), pm.data_ptr() + 8 * k
+        return a, b
+
+    rec1 = torch.full((n, 12), -1.0, dtype=torch.float64, device="cuda")
+    pm1 = torch.full((n, 4), -1.0, dtype=torch.float64, device="cuda")
+    rec2, pm2 = rec1.clone(), pm1.clone()
+    a, b = lists(rec1, pm1)
+    ctx.check(ctx.lib.sph_scatter_sorted2(ctx.handle, n, p(rank), ctypes.byref(a), ctypes.byref(b)))
+    a, b = lists(rec2, pm2)
+    ctx.check(ctx.lib.sph_scatter_sorted(ctx.handle, n, p(rank), ctypes.byref(a)))
+    ctx.check(ctx.lib.sph_scatter_sorted(ctx.handle, n, p(rank), ctypes.byref(b)))
+    torch.cuda.synchronize()
+    assert torch.equal(rec1, rec2) and torch.equal(pm1, pm2)
+    r = rank.cpu().numpy()
+    want = np.full((n, 12), -1.0)
+    for k in range(11):
+        want[r, k] = src[k].cpu().numpy()
+    assert np.array_equal(rec1.cpu().numpy(), want)
+    assert np.array_equal(pm1.cpu().numpy(), want[:, :4])
+    rec3 = torch.full((n, 12), -1.0, dtype=torch.float64, device="cuda")
+    a, b = lists(rec3, pm1, nb=0)
+    ctx.check(ctx.lib.sph_scatter_sorted2(ctx.handle, n, p(rank), ctypes.byref(a), ctypes.byref(b)))
+    torch.cuda.synchronize()
+    assert np.array_equal(rec3.cpu().numpy(), want)

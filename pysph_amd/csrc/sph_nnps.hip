@@ -408,9 +408,11 @@ __global__ void __launch_bounds__(NN_THREADS)
 
 __global__ void __launch_bounds__(NN_THREADS)
     k_bin_scatter(const int32_t *__restrict__ cell_of, const int32_t *__restrict__ slot,
-                  int64_t n, const int32_t *__restrict__ start, int32_t *__restrict__ tmp)
+                  int64_t n, const int32_t *__restrict__ start, int32_t *__restrict__ tmp,
+                  int32_t *__restrict__ nlarge)
 {
     int64_t i = (int64_t)blockIdx.x * NN_THREADS + threadIdx.x;
+    if (i == 0) *nlarge = 0;  // the large-cell counter of k_bin_rank, which runs next
     if (i >= n) return;
     tmp[start[cell_of[i]] + slot[i]] = (int32_t)i;
 }
@@ -750,10 +752,9 @@ int sph_bin_sort(sph_ctx *ctx, int64_t n, int64_t nc, const int32_t *cell_of, in
                            0, ctx->stream, cell_start, slot, nc, cells);
         SPH_LAUNCH_CHECK(ctx, "k_compact_cells");
     }
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, ctx->stream, nlarge, 1, 0);
-    SPH_LAUNCH_CHECK(ctx, "k_fill_i32");
-    hipLaunchKernelGGL(k_bin_scatter, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
-                       ctx->stream, cell_of, rank, n, cell_start, tmp);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(sph_blocks(n > 0 ? n : 1, NN_THREADS)),
+                       dim3(NN_THREADS), 0, ctx->stream, cell_of, rank, n, cell_start, tmp,
+                       nlarge);
     SPH_LAUNCH_CHECK(ctx, "k_bin_scatter");
     hipLaunchKernelGGL(k_bin_rank, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
                        ctx->stream, cell_of, n, cell_start, tmp, trash, perm, rank, large,

@@ -13,8 +13,12 @@
 #define OP_THREADS 256
 // neighbour records whose loads are in flight before their terms are added
 // (measured on the 100^3 lattice with 75 neighbours per particle: density
-// 0.34 ms at 4, 0.31 ms at 8; momentum 0.66 ms at 2, 0.60 ms at 4)
+// 0.34 ms at 4, 0.31 ms at 8; momentum 0.66 ms at 2, 0.60 ms at 4; on the
+// 1M dam break, density with the Tait equation fused: 0.34 ms at 8 (112
+// registers, 4 waves per SIMD), 0.35 ms at 12 (3 waves), 0.41 ms at 16)
+#ifndef DENS_NT
 #define DENS_NT 8
+#endif
 #ifndef MOM_NT
 #define MOM_NT 4
 #endif

@@ -107,10 +107,11 @@ __global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v);
 // cell), slot[0..nc] (exclusive scan of the non-empty flags; slot[nc] = the
 // number of non-empty cells) and cells[] (their keys).  tmp holds n ints.
 // The members of cell `trash` (-1: none) are left unordered.  Scan partials
-// go to the context scratch from scratch_off on.
+// go to the context scratch from scratch_off on.  ncells_copy (device, may be
+// NULL) receives the number of non-empty cells too.
 int sph_bin_sort(sph_ctx *ctx, int64_t n, int64_t nc, const int32_t *cell_of, int32_t *rank,
                  int32_t *cell_start, int32_t *perm, int32_t *cells, int32_t *slot, int32_t *tmp,
-                 int32_t trash, size_t scratch_off);
+                 int32_t trash, size_t scratch_off, int32_t *ncells_copy = nullptr);
 // end of the context scratch sph_bin_sort uses from scratch_off on (scan
 // partials and the list of cells with many members); a caller that keeps
 // pointers into the scratch below scratch_off ensures this size first

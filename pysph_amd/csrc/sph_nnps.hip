@@ -600,7 +600,8 @@ __global__ void __launch_bounds__(BSCAN_THREADS)
 __global__ void __launch_bounds__(BSCAN_THREADS)
     k_bin_tile_scan(const int32_t *__restrict__ counts, int64_t nc,
                     const int2 *__restrict__ tile_sum, int32_t *__restrict__ start,
-                    int32_t *__restrict__ cells, int32_t *__restrict__ ncells_out)
+                    int32_t *__restrict__ cells, int32_t *__restrict__ ncells_out,
+                    int32_t *__restrict__ ncells_copy)
 {
     __shared__ int2 s_part[BSCAN_THREADS / 64];
     __shared__ int2 s_wave[BSCAN_THREADS / 64];
@@ -665,6 +666,7 @@ __global__ void __launch_bounds__(BSCAN_THREADS)
     if (blockIdx.x == gridDim.x - 1 && tid == 0) {
         start[nc] = pre.x + ta;
         *ncells_out = pre.y + tb;
+        if (ncells_copy) *ncells_copy = pre.y + tb;
     }
 }
 
@@ -715,7 +717,7 @@ size_t sph_bin_sort_scratch_end(int64_t n, int64_t nc, size_t scratch_off)
 // their number
 int sph_bin_sort(sph_ctx *ctx, int64_t n, int64_t nc, const int32_t *cell_of, int32_t *rank,
                  int32_t *cell_start, int32_t *perm, int32_t *cells, int32_t *slot, int32_t *tmp,
-                 int32_t trash, size_t scratch_off)
+                 int32_t trash, size_t scratch_off, int32_t *ncells_copy)
 {
     // the large-cell list (a counter and at most n / (RANK_DIRECT + 1) keys)
     // after the scan partials; the callers size the scratch for it first
@@ -734,7 +736,7 @@ int sph_bin_sort(sph_ctx *ctx, int64_t n, int64_t nc, const int32_t *cell_of, in
         SPH_LAUNCH_CHECK(ctx, "k_bin_tile_sums");
         hipLaunchKernelGGL(k_bin_tile_scan, dim3((unsigned)ntiles), dim3(BSCAN_THREADS), 0,
                            ctx->stream, slot, nc, (const int2 *)tile_sum, cell_start, cells,
-                           slot + nc);
+                           slot + nc, ncells_copy);
         SPH_LAUNCH_CHECK(ctx, "k_bin_tile_scan");
     } else {
         hipLaunchKernelGGL(k_copy_i32, dim3(sph_blocks(nc, NN_THREADS)), dim3(NN_THREADS), 0,
@@ -751,6 +753,9 @@ int sph_bin_sort(sph_ctx *ctx, int64_t n, int64_t nc, const int32_t *cell_of, in
         hipLaunchKernelGGL(k_compact_cells, dim3(sph_blocks(nc, NN_THREADS)), dim3(NN_THREADS),
                            0, ctx->stream, cell_start, slot, nc, cells);
         SPH_LAUNCH_CHECK(ctx, "k_compact_cells");
+        if (ncells_copy)
+            SPH_HIP_CHECK(ctx, hipMemcpyAsync(ncells_copy, slot + nc, sizeof(int32_t),
+                                              hipMemcpyDeviceToDevice, ctx->stream));
     }
     hipLaunchKernelGGL(k_bin_scatter, dim3(sph_blocks(n > 0 ? n : 1, NN_THREADS)),
                        dim3(NN_THREADS), 0, ctx->stream, cell_of, rank, n, cell_start, tmp,
@@ -812,14 +817,11 @@ static int bin_array(sph_ctx *ctx, const sph_grid *grid, const double *x, const 
     hipLaunchKernelGGL(k_bin_keys, dim3(sph_blocks(n, NN_THREADS)), dim3(NN_THREADS), 0,
                        ctx->stream, g, x, y, z, n, slot, cell_of, rank);
     SPH_LAUNCH_CHECK(ctx, "k_bin_keys");
+    // a device-side binning's count also goes to ncells_dev
     rc = sph_bin_sort(ctx, n, nc, cell_of, rank, cell_start, perm, cells, slot, tmp, -1,
-                      slot_bytes + tmp_bytes);
+                      slot_bytes + tmp_bytes, ncells_dev);
     if (rc) return rc;
-    if (ncells_dev) {
-        SPH_HIP_CHECK(ctx, hipMemcpyAsync(ncells_dev, slot + nc, sizeof(int32_t),
-                                          hipMemcpyDeviceToDevice, ctx->stream));
-        return SPH_OK;
-    }
+    if (ncells_dev) return SPH_OK;
     return read_i64(ctx, slot + nc, ncells_out, true);
 }
 

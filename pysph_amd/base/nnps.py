@@ -563,6 +563,7 @@ class FixedDestNbrParticleLocator:
         self._near = None
         self.near_used = False
         self._host_slots = None
+        self._pair_slots = {}  # partner locator -> pinned ring of paired builds
 
     def update_status(self):
         if not self.is_dirty:
@@ -670,10 +671,11 @@ class FixedDestNbrParticleLocator:
         if self.indices is None:
             self.indices = torch.empty(max(1, 96 * nd_total), dtype=torch.int32, device=dev)
 
-    def _dev_args(self):
+    def _dev_args(self, totals=None):
         """The arguments of a device-side build (sph_nnps_dev_args) and its
-        totals buffer.  The list buffer is grown ahead of need from the last
-        known size (the lists in it are replaced by the build anyway)."""
+        totals buffer (`totals`, or a new one).  The list buffer is grown
+        ahead of need from the last known size (the lists in it are replaced
+        by the build anyway)."""
         cm = self.cell_manager
         dst, src = self.dest, self.source
         bd, bs = cm.bins[dst.name], cm.bins[src.name]
@@ -684,7 +686,8 @@ class FixedDestNbrParticleLocator:
                                        device=dev)
         if self._status is None:
             self._status = torch.zeros(1, dtype=torch.int32, device=dev)
-        totals = torch.empty(3, dtype=torch.int64, device=dev)
+        if totals is None:
+            totals = torch.empty(3, dtype=torch.int64, device=dev)
         near, reach = self._near_table(cm, bs)
         self.near_used = near is not None
         p = _lib.ptr
@@ -709,10 +712,18 @@ class FixedDestNbrParticleLocator:
         """Queue the copy of a device-side build's totals to the host."""
         if self._host_slots is None:
             self._host_slots = _PinnedSlots(4, 3, torch.int64)
-        if len(self._pending) >= 3:
-            self._poll(block=True)  # no buffer is reused before it was read
+        self._make_room()
         host, ev = self._host_slots.take(self.cell_manager._copy_event())
         host.copy_(totals, non_blocking=True)   # {slots, nnz, status}
+        self._queue(totals, host, ev)
+
+    def _make_room(self):
+        """At most two totals pending: a pinned buffer (a ring of four) is
+        never reused before the host has read it."""
+        if len(self._pending) >= 3:
+            self._poll(block=True)
+
+    def _queue(self, totals, host, ev):
         self._totals = totals
         self._pending.append((ev, host))
         self.nbuilds += 1
@@ -860,11 +871,20 @@ def update_together(locators):
         with profiling.region("nnps_build"):
             for loc in pair:
                 loc._poll()
-            (a0, t0), (a1, t1) = (loc._dev_args() for loc in pair)
+            tot = torch.empty((2, 3), dtype=torch.int64, device=pair[0].dest.device)
+            (a0, _), (a1, _) = (loc._dev_args(tot[k]) for k, loc in enumerate(pair))
             ctx.check(ctx.lib.sph_nnps_build_dev_pair(ctx.handle, ctypes.byref(cm.grid),
                                                       ctypes.byref(a0), ctypes.byref(a1)))
-            for loc, t in zip(pair, (t0, t1)):
-                loc._dev_done(t)
+            # both builds' totals in one copy, from a ring of this pair's own
+            slots = pair[0]._pair_slots.get(pair[1])
+            if slots is None:
+                slots = pair[0]._pair_slots[pair[1]] = _PinnedSlots(4, 6, torch.int64)
+            for loc in pair:
+                loc._make_room()
+            host, ev = slots.take(cm._copy_event())
+            host.copy_(tot.view(-1), non_blocking=True)
+            for k, loc in enumerate(pair):
+                loc._queue(tot[k], host[3 * k:3 * k + 3], ev)
                 loc._built()
             cm.fence()
     for loc in locators:
